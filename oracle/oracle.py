@@ -1,0 +1,323 @@
+"""ctypes access to the CPU checker (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module.  The product package never does.
+
+Two libraries are wrapped:
+
+* ``liboracle_fmm2d.so`` -- our C restatement of the reference FMM path
+  (fmm2d_oracle.c), always built by ``make -C oracle``;
+* ``_ref/libref_fmm2d.so`` -- the reference's own ``TreeCell``/``fmm2D``
+  sources compiled with ``ref_harness.cpp`` (``make -C oracle ref``, only where
+  the reference tree exists; the built file travels with the repository).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle_fmm2d.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_fmm2d.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_int64)
+_up = C.POINTER(C.c_uint8)
+
+
+class OrcOpts(C.Structure):
+    _fields_ = [
+        ("p", C.c_int),
+        ("max_par", C.c_int),
+        ("max_src", C.c_int),
+        ("exp_tree", C.c_double),
+        ("min_level", C.c_int),
+        ("max_level", C.c_int),
+        ("binom_mode", C.c_int),
+        ("list4_mode", C.c_int),
+    ]
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference build when its sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if ref and os.path.isdir("/root/reference/src/FMM"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        L = C.CDLL(ORACLE_SO)
+        L.orc_default_opts.argtypes = [C.POINTER(OrcOpts)]
+        L.orc_tree_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _dp, _up,
+                                     C.POINTER(OrcOpts)]
+        L.orc_tree_free.argtypes = [C.c_void_p]
+        L.orc_tree_counts.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip]
+        L.orc_tree_geometry.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_tree_cells.argtypes = [C.c_void_p, _lp, _ip, _ip, _ip, _ip, _ip, _up, _up]
+        L.orc_tree_leaves.argtypes = [C.c_void_p, _lp, _ip, _ip]
+        L.orc_int_list.argtypes = [C.c_void_p, C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
+        L.orc_ext_list.argtypes = [C.c_void_p, C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
+        L.orc_fmm_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.POINTER(OrcOpts), _dp, _dp]
+        L.orc_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _up, C.POINTER(OrcOpts),
+                                   C.c_double, C.c_double, _dp, _dp]
+        L.orc_direct_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.c_int, _ip, _dp, _dp]
+        L.orc_direct_field.restype = None
+        L.orc_binom_ref.argtypes = [C.c_int, C.c_int]
+        L.orc_binom_exact.argtypes = [C.c_int, C.c_int]
+        L.orc_binom_exact.restype = C.c_double
+        for name in ("orc_p2m",):
+            getattr(L, name).argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+                                         C.c_int, _dp]
+        L.orc_m2m.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_m2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+        L.orc_l2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_m2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_l2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        for name in ("orc_p2m", "orc_m2m", "orc_m2l", "orc_l2l", "orc_m2p", "orc_l2p",
+                     "orc_tree_free", "orc_tree_geometry", "orc_tree_cells",
+                     "orc_tree_leaves", "orc_default_opts"):
+            getattr(L, name).restype = None
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference build (raises if it was never compiled)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise FileNotFoundError(REF_SO)
+        L = C.CDLL(REF_SO)
+        L.ref_set_params.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
+        L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_set_threads.restype = None
+        L.ref_binom.argtypes = [C.c_int, C.c_int]
+        L.ref_build_tree.argtypes = [C.c_int, _dp, _dp, _up]
+        L.ref_tree_counts.argtypes = [_ip, _ip, _ip, _ip]
+        L.ref_tree_geometry.argtypes = [_dp, _dp]
+        L.ref_tree_cells.argtypes = [_lp, _ip, _ip, _ip, _ip, _ip, _up, _up]
+        L.ref_tree_leaves.argtypes = [_lp, _ip, _ip]
+        L.ref_int_list.argtypes = [C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
+        L.ref_ext_list.argtypes = [C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
+        L.ref_fmm_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, _dp, _dp, _dp]
+        _ref = L
+    return _ref
+
+
+# --------------------------------------------------------------------------
+# options
+# --------------------------------------------------------------------------
+def opts(p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
+         binom_mode=1, list4_mode=1) -> OrcOpts:
+    return OrcOpts(p, max_par, max_src, exp_tree, min_level, max_level, binom_mode,
+                   list4_mode)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(st, what):
+    if st != 0:
+        raise OracleError(f"{what} failed with status {st}")
+
+
+# --------------------------------------------------------------------------
+# oracle: tree, lists, field, direct sum
+# --------------------------------------------------------------------------
+class Tree:
+    """A tree built by the oracle (reference semantics)."""
+
+    def __init__(self, x, y, active, o: OrcOpts | None = None):
+        L = lib()
+        self._x, self._y, self._a = _d(x), _d(y), _u8(active)
+        self.o = o or opts()
+        h = C.c_void_p()
+        _check(L.orc_tree_build(C.byref(h), len(self._x), _ptr(self._x, _dp),
+                                _ptr(self._y, _dp), _ptr(self._a, _up), C.byref(self.o)),
+               "orc_tree_build")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tree_free(self.h)
+            self.h = None
+
+    def counts(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib().orc_tree_counts(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def geometry(self):
+        r = C.c_double()
+        piv = np.zeros(2)
+        lib().orc_tree_geometry(self.h, C.byref(r), _ptr(piv, _dp))
+        return r.value, piv
+
+    def cells(self):
+        return _cells(lib().orc_tree_cells, self.counts()[0], self.h)
+
+    def leaves(self):
+        nc, nl, _, _ = self.counts()
+        lid = np.zeros(nl, np.int64)
+        poff = np.zeros(nl + 1, np.int32)
+        pidx = np.zeros(len(self._x), np.int32)
+        lib().orc_tree_leaves(self.h, _ptr(lid, _lp), _ptr(poff, _ip), _ptr(pidx, _ip))
+        return lid, poff, pidx
+
+    def int_list(self, cid):
+        return _lists(lib().orc_int_list, self.h, cid)
+
+    def ext_list(self, cid):
+        return _lists(lib().orc_ext_list, self.h, cid)
+
+
+def _cells(fn, n, *pre):
+    out = dict(id=np.zeros(n, np.int64), level=np.zeros(n, np.int32),
+               ix=np.zeros(n, np.int32), iy=np.zeros(n, np.int32),
+               par_num=np.zeros(n, np.int32), src_num=np.zeros(n, np.int32),
+               is_leaf=np.zeros(n, np.uint8), is_active=np.zeros(n, np.uint8))
+    fn(*pre, _ptr(out["id"], _lp), _ptr(out["level"], _ip), _ptr(out["ix"], _ip),
+       _ptr(out["iy"], _ip), _ptr(out["par_num"], _ip), _ptr(out["src_num"], _ip),
+       _ptr(out["is_leaf"], _up), _ptr(out["is_active"], _up))
+    return out
+
+
+def _lists(fn, *pre_and_id, cap=4096):
+    a = np.zeros(cap, np.int64)
+    b = np.zeros(cap, np.int64)
+    na, nb = C.c_int(), C.c_int()
+    st = fn(*pre_and_id, _ptr(a, _lp), C.byref(na), _ptr(b, _lp), C.byref(nb), cap)
+    _check(st, fn.__name__)
+    if na.value > cap or nb.value > cap:
+        return _lists(fn, *pre_and_id, cap=max(na.value, nb.value))
+    return a[: na.value].copy(), b[: nb.value].copy()
+
+
+def fmm_field(x, y, q, active, o: OrcOpts | None = None):
+    L = lib()
+    x, y, q, a = _d(x), _d(y), _d(q), _u8(active)
+    n = len(x)
+    ex, ey = np.zeros(n), np.zeros(n)
+    o = o or opts()
+    _check(L.orc_fmm_field(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
+                           C.byref(o), _ptr(ex, _dp), _ptr(ey, _dp)), "orc_fmm_field")
+    return ex, ey
+
+
+def velocity(x, y, s, omega, active, o: OrcOpts | None = None, u_inf=1.0, v_inf=0.0):
+    L = lib()
+    x, y, s, w, a = _d(x), _d(y), _d(s), _d(omega), _u8(active)
+    n = len(x)
+    u, v = np.zeros(n), np.zeros(n)
+    o = o or opts()
+    _check(L.orc_velocity(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(w, _dp),
+                          _ptr(a, _up), C.byref(o), u_inf, v_inf, _ptr(u, _dp),
+                          _ptr(v, _dp)), "orc_velocity")
+    return u, v
+
+
+def direct_field(x, y, q, targets=None, src_mask=None):
+    """Singular FP64 direct sum (j == i skipped) at `targets` (indices) or all."""
+    L = lib()
+    x, y, q = _d(x), _d(y), _d(q)
+    n = len(x)
+    if targets is None:
+        nt, tp = n, None
+    else:
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        nt, tp = len(t), _ptr(t, _ip)
+    ex, ey = np.zeros(nt), np.zeros(nt)
+    m = None if src_mask is None else _ptr(_u8(src_mask), _up)
+    L.orc_direct_field(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), m, nt, tp,
+                       _ptr(ex, _dp), _ptr(ey, _dp))
+    return ex, ey
+
+
+# --------------------------------------------------------------------------
+# the reference build
+# --------------------------------------------------------------------------
+class RefTree:
+    """A tree built by the reference's own TreeCell (oracle/_ref)."""
+
+    def __init__(self, x, y, active, p=10, max_par=75, max_src=50, exp_tree=1.0):
+        R = ref()
+        R.ref_set_params(p, max_par, max_src, exp_tree)
+        self._x, self._y, self._a = _d(x), _d(y), _u8(active)
+        _check(R.ref_build_tree(len(self._x), _ptr(self._x, _dp), _ptr(self._y, _dp),
+                                _ptr(self._a, _up)), "ref_build_tree")
+
+    def counts(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ref().ref_tree_counts(C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def geometry(self):
+        r = C.c_double()
+        piv = np.zeros(2)
+        ref().ref_tree_geometry(C.byref(r), _ptr(piv, _dp))
+        return r.value, piv
+
+    def cells(self):
+        return _cells(ref().ref_tree_cells, self.counts()[0])
+
+    def leaves(self):
+        nc, nl, _, _ = self.counts()
+        lid = np.zeros(nl, np.int64)
+        poff = np.zeros(nl + 1, np.int32)
+        pidx = np.zeros(len(self._x), np.int32)
+        ref().ref_tree_leaves(_ptr(lid, _lp), _ptr(poff, _ip), _ptr(pidx, _ip))
+        return lid, poff, pidx
+
+    def int_list(self, cid):
+        return _lists(ref().ref_int_list, cid)
+
+    def ext_list(self, cid):
+        return _lists(ref().ref_ext_list, cid)
+
+
+def ref_fmm_field(x, y, q, active, p=10, max_par=75, max_src=50, exp_tree=1.0,
+                  threads=None):
+    """Field from the reference's own FMM; also returns (tree_s, fmm_s)."""
+    R = ref()
+    R.ref_set_params(p, max_par, max_src, exp_tree)
+    if threads:
+        R.ref_set_threads(int(threads))
+    x, y, q, a = _d(x), _d(y), _d(q), _u8(active)
+    n = len(x)
+    ex, ey, sec = np.zeros(n), np.zeros(n), np.zeros(2)
+    _check(R.ref_fmm_field(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
+                           _ptr(ex, _dp), _ptr(ey, _dp), _ptr(sec, _dp)), "ref_fmm_field")
+    return ex, ey, (float(sec[0]), float(sec[1]))
+
+
+def ref_binom(n, k) -> int:
+    return int(ref().ref_binom(n, k))
