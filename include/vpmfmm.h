@@ -1,0 +1,160 @@
+/*
+ * vpmfmm.h -- C ABI of the MI355X (gfx950) 2-D FMM Biot-Savart velocity
+ * library (lsmps_vpm_v2_amd/libvpmfmm.so).
+ *
+ * Drop-in boundary for the velocity path of anggarapangestu/LSMPS_VPM_v2:
+ *   VelocityCalc::get_velocity(Particle&, const GridNode&, int)
+ *       src/velocity_calculation/velocity_calc.hpp:42, velocity_calc.cpp:149-251
+ *   VelocityCalc::velocity_fmm_2d(Particle&, int)     velocity_calc.cpp:455-578
+ *   TreeCell::initializeTree / updateTree              src/FMM/treeCell.hpp:152-153
+ *   fmm2D::calcField + get_Field_x / get_Field_y       src/FMM/fmm2D.hpp:59-67
+ * Plain pointers and sizes only; every function returns a vpmfmm_status and
+ * never throws.  All work of one call runs on the handle's HIP stream and is
+ * complete when the call returns.  One handle per host thread.
+ */
+#ifndef VPMFMM_H
+#define VPMFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPMFMM_ABI_VERSION 1
+
+typedef enum vpmfmm_status {
+    VPMFMM_OK = 0,
+    VPMFMM_BAD_ARG = 1,              /* invalid option, size or pointer          */
+    VPMFMM_NONFINITE_INPUT = 2,      /* NaN/inf particle coordinate              */
+    VPMFMM_DEGENERATE_DOMAIN = 3,    /* all particles at one point (root size 0) */
+    VPMFMM_TREE_DEPTH_EXCEEDED = 4,  /* a cell still over threshold at level 16  */
+    VPMFMM_OUT_OF_MEMORY = 5,
+    VPMFMM_INTERNAL = 6,
+    VPMFMM_DUPLICATE_POINTS = 7,     /* two particles of one list-1 pair coincide */
+    VPMFMM_HIP_ERROR = 8
+} vpmfmm_status;
+
+typedef enum vpmfmm_precision {
+    VPMFMM_FP64 = 0,   /* plain FP64 path, reference semantics                     */
+    VPMFMM_MIXED = 1   /* FP32 P2P/P2M/L2P, FP32 MFMA M2L, FP64 geometry           */
+} vpmfmm_precision;
+
+/* Options.  Defaults (vpmfmm_default_opts) are the reference's values:
+ * P_max 10, par_count_max 75, src_count_max 50, expTree 1.0 (global.cpp:303-313),
+ * u_inf 1, v_inf 0 (global.cpp:164-165). */
+typedef struct vpmfmm_opts {
+    int32_t p;           /* expansion order, 1..40 (Pars::P_max)                       */
+    int32_t max_par;     /* leaf rule, particles per cell, 1..4096 (Pars::par_count_max) */
+    int32_t max_src;     /* leaf rule, active particles per cell, 0..4096 (src_count_max) */
+    double exp_tree;     /* root expansion in percent, > 0 (Pars::expTree)              */
+    int32_t min_level;   /* cells above this level always split (reference: 1)          */
+    int32_t max_level;   /* cells at this level always leaves, 0 = none (max 16)        */
+    int32_t precision;   /* vpmfmm_precision                                            */
+    int32_t binom_mode;  /* M2L binomials: 1 = the reference's int32 helper
+                            (fmm2D.cpp:28-58, parity), 0 = exact                         */
+    int32_t list4_mode;  /* 1 = M2L from the list-4 leaf multipole (M2L_P2_METHOD_TYPE 1,
+                            reference default), 2 = P2L from its particles              */
+    double u_inf;        /* free stream added by the velocity entry points              */
+    double v_inf;
+    int32_t device;      /* HIP device ordinal                                          */
+    int32_t timing;      /* 1 = record per-stage times into vpmfmm_stats                */
+    int32_t stage_mask;  /* evaluation parts (diagnostics): 1 P2P, 2 M2P (list 3),
+                            4 L2P (far field); 0 = all                                 */
+    int32_t reserved[7];
+} vpmfmm_opts;
+
+typedef struct vpmfmm_stats {
+    int32_t n, n_active, n_cells, n_leaves;
+    int32_t min_leaf_level, max_leaf_level;
+    int64_t n_list1, n_list2, n_list3, n_list4;  /* list entries summed over cells   */
+    int64_t p2p_pairs;                             /* ordered (target, source) pairs  */
+    double ms_total;                               /* stage times (timing = 1)        */
+    double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_eval;
+} vpmfmm_stats;
+
+typedef struct vpmfmm_handle_s vpmfmm_handle;
+
+/* Fills *o with the reference defaults. */
+void vpmfmm_default_opts(vpmfmm_opts *o);
+
+/* Handle lifetime.  Replaces the solver objects held by VelocityCalc
+ * (TreeCell treeData member, velocity_calc.hpp:19; fmm2D local,
+ * velocity_calc.cpp:560).  Device buffers persist and grow on demand. */
+int vpmfmm_create(const vpmfmm_opts *o, vpmfmm_handle **out);
+int vpmfmm_destroy(vpmfmm_handle *h);
+int vpmfmm_set_opts(vpmfmm_handle *h, const vpmfmm_opts *o);
+int vpmfmm_get_opts(const vpmfmm_handle *h, vpmfmm_opts *o);
+/* Use the caller's HIP stream (hipStream_t passed as void*); NULL restores
+ * the handle's own stream. */
+int vpmfmm_set_stream(vpmfmm_handle *h, void *hip_stream);
+
+/* Field of the singular 2-D kernel, host pointers:
+ *   ex_i = sum_j q_j (x_i - x_j)/r_ij^2,  ey_i = sum_j q_j (y_i - y_j)/r_ij^2
+ * with the reference's FMM semantics (inactive particles are P2P sources only).
+ * Replaces TreeCell::initializeTree(pos, active) (treeCell.cpp:1043) followed by
+ * fmm2D::calcField(tree, pos, active, src) and get_Field_x/y (fmm2D.cpp:907,
+ * 1146-1157).  active: one byte per particle, nonzero = active. */
+int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                 const double *q, const uint8_t *active, double *ex, double *ey);
+/* Same with device pointers (inputs resident in HBM). */
+int vpmfmm_field_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                        const double *q, const uint8_t *active, double *ex, double *ey);
+
+/* Velocity of VelocityCalc::velocity_fmm_2d + get_velocity's free stream
+ * (velocity_calc.cpp:465-481, 572-575, 226-230):
+ *   q = -omega s^2 / (2 pi),  u = Ey + u_inf,  v = -Ex + v_inf. */
+int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                    const double *s, const double *omega, const uint8_t *active, double *u,
+                    double *v);
+int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                           const double *s, const double *omega, const uint8_t *active,
+                           double *u, double *v);
+
+/* FP64 O(N^2) direct sum of the same kernel on the GPU, device pointers: the
+ * field at targets tidx[0..nt) (indices into x/y; j == i skipped), or at all
+ * n particles when tidx is NULL.  src_mask (may be NULL) selects sources.
+ * An accuracy reference, not part of the FMM path (the reference keeps a
+ * regularised variant in biotsavart_direct_2d.cpp). */
+int vpmfmm_direct_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                         const double *q, const uint8_t *src_mask, int32_t nt,
+                         const int32_t *tidx, double *ex, double *ey);
+
+/* Statistics of the last successful call. */
+int vpmfmm_last_stats(const vpmfmm_handle *h, vpmfmm_stats *st);
+/* Message of the last failed call on this handle ("" if none). */
+const char *vpmfmm_last_error(const vpmfmm_handle *h);
+
+/* ---- inspection of the last tree (parity tests; data copied to the host) ----
+ * Cells are identified by the reference's cell ID
+ *   ID = (4^L - 1)/3 + ix + iy 2^L   (treeCell.cpp:114-148, 223-250).       */
+
+/* Builds the tree and lists only (TreeCell::initializeTree, treeCell.cpp:1043). */
+int vpmfmm_build_tree(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                      const uint8_t *active);
+int vpmfmm_tree_counts(vpmfmm_handle *h, int32_t *n_cells, int32_t *n_leaves,
+                       int32_t *min_leaf_level, int32_t *max_leaf_level);
+int vpmfmm_tree_geometry(vpmfmm_handle *h, double *root_len, double *pivot2);
+/* All cells sorted by ID (arrays of n_cells). */
+int vpmfmm_tree_cells(vpmfmm_handle *h, int64_t *id, int32_t *level, int32_t *ix, int32_t *iy,
+                      int32_t *par_num, int32_t *src_num, uint8_t *is_leaf, uint8_t *is_active);
+/* Leaves sorted by ID with their particles (caller indices, ascending):
+ * leaf_id[n_leaves], poff[n_leaves + 1], pidx[n]. */
+int vpmfmm_tree_leaves(vpmfmm_handle *h, int64_t *leaf_id, int32_t *poff, int32_t *pidx);
+/* Lists of one cell (TreeCell::intList / extList, treeCell.cpp:636, 818):
+ * lists 1 and 3 for leaves, lists 2 and 4 for cells of level >= 2.  Counts
+ * are always written; entries only up to cap. */
+int vpmfmm_tree_lists(vpmfmm_handle *h, int64_t id, int64_t *l1, int32_t *n1, int64_t *l2,
+                      int32_t *n2, int64_t *l3, int32_t *n3, int64_t *l4, int32_t *n4,
+                      int32_t cap);
+
+/* Multipole (ak) and local (bk) coefficients of every cell after the last
+ * FP64 field/velocity call, cells sorted by ID: ids[n_cells],
+ * ak/bk[n_cells * 2(p+1)] as (re, im) pairs.  b_0 is not formed (it never
+ * reaches the field).  Stage-level parity hook (P2M, M2M, M2L + L2L). */
+int vpmfmm_tree_expansions(vpmfmm_handle *h, int64_t *ids, double *ak, double *bk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""ctypes binding of the C ABI in include/vpmfmm.h (libvpmfmm.so, in-tree).
+
+There is no fallback: if the HIP library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvpmfmm.so")
+
+# Symbols declared in include/vpmfmm.h (checked by the CPU tests).
+EXPORTS = (
+    "vpmfmm_default_opts", "vpmfmm_create", "vpmfmm_destroy", "vpmfmm_set_opts",
+    "vpmfmm_get_opts", "vpmfmm_set_stream", "vpmfmm_field", "vpmfmm_field_device",
+    "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_direct_device",
+    "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
+    "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
+    "vpmfmm_tree_expansions",
+)
+
+STATUS = {
+    0: "OK", 1: "BAD_ARG", 2: "NONFINITE_INPUT", 3: "DEGENERATE_DOMAIN",
+    4: "TREE_DEPTH_EXCEEDED", 5: "OUT_OF_MEMORY", 6: "INTERNAL", 7: "DUPLICATE_POINTS",
+    8: "HIP_ERROR",
+}
+
+FP64, MIXED = 0, 1
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("p", C.c_int32), ("max_par", C.c_int32), ("max_src", C.c_int32),
+        ("exp_tree", C.c_double), ("min_level", C.c_int32), ("max_level", C.c_int32),
+        ("precision", C.c_int32), ("binom_mode", C.c_int32), ("list4_mode", C.c_int32),
+        ("u_inf", C.c_double), ("v_inf", C.c_double), ("device", C.c_int32),
+        ("timing", C.c_int32), ("stage_mask", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("n_active", C.c_int32), ("n_cells", C.c_int32),
+        ("n_leaves", C.c_int32), ("min_leaf_level", C.c_int32), ("max_leaf_level", C.c_int32),
+        ("n_list1", C.c_int64), ("n_list2", C.c_int64), ("n_list3", C.c_int64),
+        ("n_list4", C.c_int64), ("p2p_pairs", C.c_int64), ("ms_total", C.c_double),
+        ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_upward", C.c_double),
+        ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_eval", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VpmFmmError(RuntimeError):
+    """A non-OK status from the C ABI (``code``, ``status`` name, message)."""
+
+    def __init__(self, code: int, where: str, msg: str):
+        self.code = code
+        self.status = STATUS.get(code, str(code))
+        super().__init__(f"{where}: {self.status}: {msg}")
+
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_up = C.POINTER(C.c_uint8)
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libvpmfmm.so (raises FileNotFoundError if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -m lsmps_vpm_v2_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.vpmfmm_default_opts.argtypes = [C.POINTER(Opts)]
+    L.vpmfmm_default_opts.restype = None
+    L.vpmfmm_create.argtypes = [C.POINTER(Opts), C.POINTER(_vp)]
+    L.vpmfmm_destroy.argtypes = [_vp]
+    L.vpmfmm_set_opts.argtypes = [_vp, C.POINTER(Opts)]
+    L.vpmfmm_get_opts.argtypes = [_vp, C.POINTER(Opts)]
+    L.vpmfmm_set_stream.argtypes = [_vp, _vp]
+    for name in ("vpmfmm_field", "vpmfmm_field_device"):
+        getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]
+    for name in ("vpmfmm_velocity", "vpmfmm_velocity_device"):
+        getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.vpmfmm_direct_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp,
+                                       _vp, _vp]
+    L.vpmfmm_last_stats.argtypes = [_vp, C.POINTER(Stats)]
+    L.vpmfmm_last_error.argtypes = [_vp]
+    L.vpmfmm_last_error.restype = C.c_char_p
+    L.vpmfmm_build_tree.argtypes = [_vp, C.c_int32, _vp, _vp, _vp]
+    L.vpmfmm_tree_counts.argtypes = [_vp, _ip, _ip, _ip, _ip]
+    L.vpmfmm_tree_geometry.argtypes = [_vp, _dp, _dp]
+    L.vpmfmm_tree_cells.argtypes = [_vp, _lp, _ip, _ip, _ip, _ip, _ip, _up, _up]
+    L.vpmfmm_tree_leaves.argtypes = [_vp, _lp, _ip, _ip]
+    L.vpmfmm_tree_lists.argtypes = [_vp, C.c_int64, _lp, _ip, _lp, _ip, _lp, _ip, _lp, _ip,
+                                    C.c_int32]
+    L.vpmfmm_tree_expansions.argtypes = [_vp, _lp, _dp, _dp]
+    _lib = L
+    return L
+
+
+def default_opts() -> Opts:
+    o = Opts()
+    load().vpmfmm_default_opts(C.byref(o))
+    return o

@@ -1,0 +1,623 @@
+// api.cpp -- C ABI (include/vpmfmm.h): handle, option checks, stage
+// orchestration, host staging and tree inspection.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vpmfmm.h"
+#include "ctx.hpp"
+
+namespace vpm {
+
+void *DBuf::get(size_t bytes) {
+    if (bytes <= cap && p) return p;
+    release();
+    size_t nb = std::max(bytes, cap + cap / 2);
+    nb = (nb + 255) & ~(size_t)255;
+    if (hipMalloc(&p, nb) != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        throw Error(kNoMem, "device allocation of " + std::to_string(nb) + " bytes failed");
+    }
+    cap = nb;
+    return p;
+}
+
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// fmm2D::binComb (reference src/FMM/fmm2D.cpp:28-58): product n..max+1, then
+// division by min..2, in 32-bit int.  The product wraps for the M2L arguments
+// at P >= 10 (and for M2M/L2L from P = 18); g++ on x86-64 wraps two's
+// complement, emulated here with unsigned arithmetic.
+int ref_binom_int32(int n, int k) {
+    if (k > n) return 0;
+    const int inv = n - k;
+    const int mx = inv > k ? inv : k, mn = inv > k ? k : inv;
+    uint32_t c = 1u;
+    for (int i = n; i > mx; --i) c *= (uint32_t)i;
+    int32_t cs = (int32_t)c;
+    for (int i = mn; i > 1; --i) cs /= i;
+    return cs;
+}
+
+double exact_binom(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    if (k > n - k) k = n - k;
+    double c = 1.0;
+    for (int i = 1; i <= k; ++i) c = c * (double)(n - k + i) / (double)i;
+    return std::nearbyint(c);
+}
+
+// Row-major (P+1) x (P+1) tables:
+//   m2l[p][0] = -1/p, m2l[p][k] = -C(p+k-1, k-1) (-1)^k / k   (fmm2D.cpp:465-481)
+//   m2m[p][k] = C(p-1, k-1) p / k                             (fmm2D.cpp:371-374)
+//   l2l[p][k] = C(k, p)                                       (fmm2D.cpp:639-641)
+void fill_tables(int P, int binom_mode, std::vector<double> &m2l, std::vector<double> &m2m,
+                 std::vector<double> &l2l) {
+    const int P1 = P + 1;
+    auto B = [&](int n, int k) {
+        return binom_mode ? (double)ref_binom_int32(n, k) : exact_binom(n, k);
+    };
+    m2l.assign((size_t)P1 * P1, 0.0);
+    m2m.assign((size_t)P1 * P1, 0.0);
+    l2l.assign((size_t)P1 * P1, 0.0);
+    for (int p = 1; p <= P; ++p) {
+        m2l[p * P1 + 0] = -1.0 / (double)p;
+        for (int k = 1; k <= P; ++k) {
+            const double sg = (k & 1) ? -1.0 : 1.0;
+            m2l[p * P1 + k] = -(B(p + k - 1, k - 1) * sg / (double)k);
+        }
+        for (int k = 1; k <= p; ++k) m2m[p * P1 + k] = B(p - 1, k - 1) * ((double)p / (double)k);
+    }
+    for (int p = 0; p <= P; ++p)
+        for (int k = p; k <= P; ++k) l2l[p * P1 + k] = B(k, p);
+}
+
+void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st);
+
+}  // namespace vpm
+
+using namespace vpm;
+
+struct vpmfmm_handle_s {
+    vpmfmm_opts o{};
+    Ctx ctx;
+    hipStream_t own_stream = nullptr;
+    std::string err;
+    vpmfmm_stats st{};
+    int tab_p = -1, tab_binom = -1;
+    int exp_p = -1;  // order of the expansions held on the device (-1: none)
+    hipEvent_t ev[8] = {};
+    // inspection cache of the last tree (host copies)
+    bool insp = false;
+    std::vector<uint8_t> h_level, h_flags;
+    std::vector<uint32_t> h_key, h_begin, h_end, h_src, h_perm, h_leaf_idx, h_off1, h_off3;
+    std::vector<int> h_l1, h_l3, h_l2, h_l4;
+    std::vector<uint8_t> h_n2, h_n4;
+    std::vector<int64_t> h_id;
+};
+
+namespace {
+
+constexpr int kMaxP = 40;
+constexpr int kMaxThreshold = 4096;
+
+int opts_check(const vpmfmm_opts &o, std::string &msg) {
+    if (o.p < 1 || o.p > kMaxP) msg = "p must be in [1, 40]";
+    else if (o.max_par < 1 || o.max_par > kMaxThreshold) msg = "max_par must be in [1, 4096]";
+    else if (o.max_src < 0 || o.max_src > kMaxThreshold) msg = "max_src must be in [0, 4096]";
+    else if (!(o.exp_tree > 0.0) || !std::isfinite(o.exp_tree)) msg = "exp_tree must be > 0";
+    else if (o.min_level < 1 || o.min_level > kMaxDepth) msg = "min_level must be in [1, 16]";
+    else if (o.max_level < 0 || o.max_level > kMaxDepth) msg = "max_level must be in [0, 16]";
+    else if (o.max_level && o.max_level < o.min_level) msg = "max_level < min_level";
+    else if (o.precision != VPMFMM_FP64 && o.precision != VPMFMM_MIXED) msg = "bad precision";
+    else if (o.binom_mode != 0 && o.binom_mode != 1) msg = "binom_mode must be 0 or 1";
+    else if (o.list4_mode != 1 && o.list4_mode != 2) msg = "list4_mode must be 1 or 2";
+    else if (o.stage_mask < 0 || o.stage_mask > 7) msg = "stage_mask must be in [0, 7]";
+    else return VPMFMM_OK;
+    return VPMFMM_BAD_ARG;
+}
+
+template <class F>
+int guarded(vpmfmm_handle *h, F &&f) {
+    if (!h) return VPMFMM_BAD_ARG;
+    try {
+        h->err.clear();
+        f();
+        return VPMFMM_OK;
+    } catch (const Error &e) {
+        h->err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return VPMFMM_INTERNAL;
+    }
+}
+
+void ensure_tables(vpmfmm_handle *h) {
+    const int P = h->o.p;
+    if (h->tab_p == P && h->tab_binom == h->o.binom_mode) return;
+    std::vector<double> m2l, m2m, l2l;
+    fill_tables(P, h->o.binom_mode, m2l, m2m, l2l);
+    Ctx &c = h->ctx;
+    const size_t nb = m2l.size() * sizeof(double);
+    VPM_HIP(hipMemcpyAsync(c.tab_m2l.get(nb), m2l.data(), nb, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipMemcpyAsync(c.tab_m2m.get(nb), m2m.data(), nb, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipMemcpyAsync(c.tab_l2l.get(nb), l2l.data(), nb, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipStreamSynchronize(c.stream));
+    h->tab_p = P;
+    h->tab_binom = h->o.binom_mode;
+}
+
+void mark(vpmfmm_handle *h, int i) {
+    if (h->o.timing) VPM_HIP(hipEventRecord(h->ev[i], h->ctx.stream));
+}
+
+float span(vpmfmm_handle *h, int a, int b) {
+    float ms = 0.f;
+    VPM_HIP(hipEventElapsedTime(&ms, h->ev[a], h->ev[b]));
+    return ms;
+}
+
+void build(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+           const uint8_t *a) {
+    std::string msg;
+    if (opts_check(h->o, msg)) throw Error(kBadArg, msg);
+    if (n < 1) throw Error(kBadArg, "n must be >= 1");
+    if (!x || !y || !a) throw Error(kBadArg, "null pointer");
+    VPM_HIP(hipSetDevice(h->o.device));
+    h->insp = false;
+    h->exp_p = -1;
+    TreeParams tp{h->o.max_par, h->o.max_src, h->o.exp_tree, h->o.min_level, h->o.max_level};
+    mark(h, 0);
+    tree_build(h->ctx, n, x, y, q, a, tp);
+    mark(h, 1);
+    tree_lists(h->ctx);
+    mark(h, 2);
+}
+
+void finish_stats(vpmfmm_handle *h, bool fmm) {
+    Ctx &c = h->ctx;
+    VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
+    VPM_HIP(hipStreamSynchronize(c.stream));
+    const TreeInfo &ti = c.info;
+    vpmfmm_stats &s = h->st;
+    s = vpmfmm_stats{};
+    s.n = ti.n;
+    s.n_active = ti.n_active;
+    s.n_cells = ti.n_cells;
+    s.n_leaves = ti.n_leaves;
+    s.min_leaf_level = ti.min_leaf_level;
+    s.max_leaf_level = ti.max_level;
+    s.n_list1 = ti.tot1;
+    s.n_list3 = ti.tot3;
+    s.n_list2 = c.hdr_h->sum2;
+    s.n_list4 = c.hdr_h->sum4;
+    s.p2p_pairs = (int64_t)c.hdr_h->pairs;
+    if (h->o.timing) {
+        s.ms_tree = span(h, 0, 1);
+        s.ms_lists = span(h, 1, 2);
+        if (fmm) {
+            s.ms_upward = span(h, 2, 3);
+            s.ms_m2l = span(h, 3, 4);
+            s.ms_downward = span(h, 4, 5);
+            s.ms_eval = span(h, 5, 6);
+            s.ms_total = span(h, 0, 6);
+        } else {
+            s.ms_total = span(h, 0, 2);
+        }
+    }
+    if (c.hdr_h->flags & kFlagDuplicate)
+        throw Error(kDuplicate, "two particles of a list-1 pair share a position (r = 0)");
+}
+
+void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+         const uint8_t *a, double *oa, double *ob, bool velocity) {
+    if (!q || !oa || !ob) throw Error(kBadArg, "null pointer");
+    build(h, n, x, y, q, a);
+    ensure_tables(h);
+    Ctx &c = h->ctx;
+    FmmMarks mk{h->o.timing ? h->ev : nullptr};
+    fmm64_run(c, h->o.p, h->o.list4_mode, h->o.stage_mask ? h->o.stage_mask : 7, oa, ob,
+              velocity, h->o.u_inf, h->o.v_inf, mk);
+    h->exp_p = h->o.p;
+    finish_stats(h, true);
+}
+
+// Host-pointer staging.
+struct Staged {
+    const double *x, *y, *q;
+    const uint8_t *a;
+};
+
+Staged stage_in(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+                const uint8_t *a) {
+    if (n < 1) throw Error(kBadArg, "n must be >= 1");
+    if (!x || !y || !q || !a) throw Error(kBadArg, "null pointer");
+    VPM_HIP(hipSetDevice(h->o.device));
+    Ctx &c = h->ctx;
+    double *dx = c.x_in.as<double>(n), *dy = c.y_in.as<double>(n), *dq = c.q_in.as<double>(n);
+    uint8_t *da = c.a_in.as<uint8_t>(n);
+    VPM_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipMemcpyAsync(dq, q, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    VPM_HIP(hipMemcpyAsync(da, a, n, hipMemcpyHostToDevice, c.stream));
+    return {dx, dy, dq, da};
+}
+
+void stage_out(vpmfmm_handle *h, int n, double *a, double *b) {
+    Ctx &c = h->ctx;
+    VPM_HIP(hipMemcpyAsync(a, c.ex_out.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    VPM_HIP(hipMemcpyAsync(b, c.ey_out.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    VPM_HIP(hipStreamSynchronize(c.stream));
+}
+
+int64_t ref_id(int L, uint32_t ix, uint32_t iy) {
+    int64_t p4 = 1;
+    for (int i = 0; i < L; ++i) p4 *= 4;
+    return (p4 - 1) / 3 + (int64_t)ix + (int64_t)iy * ((int64_t)1 << L);
+}
+
+uint32_t compact(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
+
+template <class T>
+void d2h(std::vector<T> &dst, const void *src, size_t count, hipStream_t s) {
+    dst.resize(count);
+    if (count) VPM_HIP(hipMemcpyAsync(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+void inspect(vpmfmm_handle *h) {
+    if (h->insp) return;
+    Ctx &c = h->ctx;
+    const TreeInfo &ti = c.info;
+    if (ti.n_cells == 0) throw Error(kBadArg, "no tree has been built on this handle");
+    const int nc = ti.n_cells, nl = ti.n_leaves;
+    hipStream_t s = c.stream;
+    d2h(h->h_level, c.c_level.p, nc, s);
+    d2h(h->h_flags, c.c_flags.p, nc, s);
+    d2h(h->h_key, c.c_key.p, nc, s);
+    d2h(h->h_begin, c.c_begin.p, nc, s);
+    d2h(h->h_end, c.c_end.p, nc, s);
+    d2h(h->h_src, c.c_src.p, nc, s);
+    d2h(h->h_perm, c.perm_sorted, ti.n, s);
+    d2h(h->h_leaf_idx, c.leaf_idx.p, nl, s);
+    d2h(h->h_off1, c.off1.p, nl + 1, s);
+    d2h(h->h_off3, c.off3.p, nl + 1, s);
+    d2h(h->h_l1, c.l1.p, ti.tot1, s);
+    d2h(h->h_l3, c.l3.p, ti.tot3, s);
+    d2h(h->h_l2, c.l2.p, 27 * (size_t)nc, s);
+    d2h(h->h_l4, c.l4.p, 8 * (size_t)nc, s);
+    d2h(h->h_n2, c.n2.p, nc, s);
+    d2h(h->h_n4, c.n4.p, nc, s);
+    VPM_HIP(hipStreamSynchronize(s));
+    h->h_id.resize(nc);
+    for (int j = 0; j < nc; ++j) {
+        const uint32_t k = h->h_key[j];
+        h->h_id[j] = ref_id(h->h_level[j], compact(k), compact(k >> 1));
+    }
+    h->insp = true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// exported functions
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void vpmfmm_default_opts(vpmfmm_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->p = 10;            // global.cpp:311
+    o->max_par = 75;      // global.cpp:312
+    o->max_src = 50;      // global.cpp:313
+    o->exp_tree = 1.0;    // global.cpp:303
+    o->min_level = 1;
+    o->max_level = 0;
+    o->precision = VPMFMM_FP64;
+    o->binom_mode = 1;    // reference helper (parity)
+    o->list4_mode = 1;    // M2L_P2_METHOD_TYPE 1 (fmm2D.cpp:7)
+    o->u_inf = 1.0;       // global.cpp:164
+    o->v_inf = 0.0;       // global.cpp:165
+    o->device = 0;
+    o->timing = 0;
+}
+
+int vpmfmm_create(const vpmfmm_opts *o, vpmfmm_handle **out) {
+    if (!out) return VPMFMM_BAD_ARG;
+    *out = nullptr;
+    vpmfmm_opts oo;
+    if (o) oo = *o;
+    else vpmfmm_default_opts(&oo);
+    std::string msg;
+    if (int st = opts_check(oo, msg)) return st;
+    vpmfmm_handle *h = new (std::nothrow) vpmfmm_handle();
+    if (!h) return VPMFMM_OUT_OF_MEMORY;
+    h->o = oo;
+    int st = guarded(h, [&] {
+        VPM_HIP(hipSetDevice(oo.device));
+        VPM_HIP(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+        h->ctx.stream = h->own_stream;
+        VPM_HIP(hipMalloc(&h->ctx.hdr_d, sizeof(DevHdr)));
+        VPM_HIP(hipHostMalloc(&h->ctx.hdr_h, sizeof(DevHdr), hipHostMallocDefault));
+        for (auto &e : h->ev) VPM_HIP(hipEventCreate(&e));
+    });
+    if (st != VPMFMM_OK) {
+        vpmfmm_destroy(h);
+        return st;
+    }
+    *out = h;
+    return VPMFMM_OK;
+}
+
+int vpmfmm_destroy(vpmfmm_handle *h) {
+    if (!h) return VPMFMM_OK;
+    (void)hipSetDevice(h->o.device);
+    if (h->ctx.stream) (void)hipStreamSynchronize(h->ctx.stream);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->ctx.hdr_d) (void)hipFree(h->ctx.hdr_d);
+    if (h->ctx.hdr_h) (void)hipHostFree(h->ctx.hdr_h);
+    h->ctx.hdr_d = nullptr;
+    h->ctx.hdr_h = nullptr;
+    hipStream_t own = h->own_stream;
+    delete h;  // DBuf destructors free device buffers
+    if (own) (void)hipStreamDestroy(own);
+    return VPMFMM_OK;
+}
+
+int vpmfmm_set_opts(vpmfmm_handle *h, const vpmfmm_opts *o) {
+    if (!h || !o) return VPMFMM_BAD_ARG;
+    std::string msg;
+    if (int st = opts_check(*o, msg)) {
+        h->err = msg;
+        return st;
+    }
+    if (o->device != h->o.device) {
+        h->err = "the device of a handle is fixed at creation";
+        return VPMFMM_BAD_ARG;
+    }
+    h->o = *o;
+    return VPMFMM_OK;
+}
+
+int vpmfmm_get_opts(const vpmfmm_handle *h, vpmfmm_opts *o) {
+    if (!h || !o) return VPMFMM_BAD_ARG;
+    *o = h->o;
+    return VPMFMM_OK;
+}
+
+int vpmfmm_set_stream(vpmfmm_handle *h, void *stream) {
+    if (!h) return VPMFMM_BAD_ARG;
+    h->ctx.stream = stream ? (hipStream_t)stream : h->own_stream;
+    return VPMFMM_OK;
+}
+
+int vpmfmm_field_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                        const double *q, const uint8_t *active, double *ex, double *ey) {
+    return guarded(h, [&] { run(h, n, x, y, q, active, ex, ey, false); });
+}
+
+int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y, const double *q,
+                 const uint8_t *active, double *ex, double *ey) {
+    return guarded(h, [&] {
+        if (!ex || !ey) throw Error(kBadArg, "null pointer");
+        Staged s = stage_in(h, n, x, y, q, active);
+        Ctx &c = h->ctx;
+        double *oa = c.ex_out.as<double>(n), *ob = c.ey_out.as<double>(n);
+        run(h, n, s.x, s.y, s.q, s.a, oa, ob, false);
+        stage_out(h, n, ex, ey);
+    });
+}
+
+int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                           const double *s, const double *omega, const uint8_t *active, double *u,
+                           double *v) {
+    return guarded(h, [&] {
+        if (n < 1 || !s || !omega) throw Error(kBadArg, "bad size or null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        double *q = h->ctx.q_in.as<double>(n);
+        sources64(n, s, omega, q, h->ctx.stream);
+        run(h, n, x, y, q, active, u, v, true);
+    });
+}
+
+int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *y, const double *s,
+                    const double *omega, const uint8_t *active, double *u, double *v) {
+    return guarded(h, [&] {
+        if (n < 1 || !x || !y || !s || !omega || !active || !u || !v)
+            throw Error(kBadArg, "bad size or null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        Ctx &c = h->ctx;
+        double *dx = c.x_in.as<double>(n), *dy = c.y_in.as<double>(n);
+        double *ds = c.ex_out.as<double>(n), *dw = c.ey_out.as<double>(n);
+        double *dq = c.q_in.as<double>(n);
+        uint8_t *da = c.a_in.as<uint8_t>(n);
+        VPM_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(ds, s, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(dw, omega, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(da, active, n, hipMemcpyHostToDevice, c.stream));
+        sources64(n, ds, dw, dq, c.stream);
+        run(h, n, dx, dy, dq, da, ds, dw, true);  // outputs reuse the s/omega staging
+        VPM_HIP(hipMemcpyAsync(u, ds, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipMemcpyAsync(v, dw, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int vpmfmm_direct_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                         const double *q, const uint8_t *src_mask, int32_t nt, const int32_t *tidx,
+                         double *ex, double *ey) {
+    return guarded(h, [&] {
+        if (n < 1 || !x || !y || !q || !ex || !ey) throw Error(kBadArg, "bad size or null pointer");
+        if (!tidx) nt = n;
+        if (nt < 1) throw Error(kBadArg, "nt must be >= 1");
+        VPM_HIP(hipSetDevice(h->o.device));
+        direct64(n, x, y, q, src_mask, nt, tidx, ex, ey, h->ctx.stream);
+        VPM_HIP(hipStreamSynchronize(h->ctx.stream));
+    });
+}
+
+int vpmfmm_last_stats(const vpmfmm_handle *h, vpmfmm_stats *st) {
+    if (!h || !st) return VPMFMM_BAD_ARG;
+    *st = h->st;
+    return VPMFMM_OK;
+}
+
+const char *vpmfmm_last_error(const vpmfmm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int vpmfmm_build_tree(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                      const uint8_t *active) {
+    return guarded(h, [&] {
+        std::vector<double> zeros((size_t)std::max(n, 1), 0.0);
+        Staged s = stage_in(h, n, x, y, zeros.data(), active);
+        build(h, n, s.x, s.y, s.q, s.a);
+        finish_stats(h, false);
+    });
+}
+
+int vpmfmm_tree_counts(vpmfmm_handle *h, int32_t *n_cells, int32_t *n_leaves,
+                       int32_t *min_leaf_level, int32_t *max_leaf_level) {
+    return guarded(h, [&] {
+        const TreeInfo &ti = h->ctx.info;
+        if (ti.n_cells == 0) throw Error(kBadArg, "no tree has been built on this handle");
+        if (n_cells) *n_cells = ti.n_cells;
+        if (n_leaves) *n_leaves = ti.n_leaves;
+        if (min_leaf_level) *min_leaf_level = ti.min_leaf_level;
+        if (max_leaf_level) *max_leaf_level = ti.max_level;
+    });
+}
+
+int vpmfmm_tree_geometry(vpmfmm_handle *h, double *root_len, double *pivot2) {
+    return guarded(h, [&] {
+        const TreeInfo &ti = h->ctx.info;
+        if (ti.n_cells == 0) throw Error(kBadArg, "no tree has been built on this handle");
+        if (root_len) *root_len = ti.root_len;
+        if (pivot2) {
+            pivot2[0] = ti.pivot[0];
+            pivot2[1] = ti.pivot[1];
+        }
+    });
+}
+
+int vpmfmm_tree_cells(vpmfmm_handle *h, int64_t *id, int32_t *level, int32_t *ix, int32_t *iy,
+                      int32_t *par_num, int32_t *src_num, uint8_t *is_leaf, uint8_t *is_active) {
+    return guarded(h, [&] {
+        inspect(h);
+        const int nc = h->ctx.info.n_cells;
+        std::vector<int> ord(nc);
+        for (int j = 0; j < nc; ++j) ord[j] = j;
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return h->h_id[a] < h->h_id[b]; });
+        for (int k = 0; k < nc; ++k) {
+            const int j = ord[k];
+            const uint32_t key = h->h_key[j];
+            if (id) id[k] = h->h_id[j];
+            if (level) level[k] = h->h_level[j];
+            if (ix) ix[k] = (int32_t)compact(key);
+            if (iy) iy[k] = (int32_t)compact(key >> 1);
+            if (par_num) par_num[k] = (int32_t)(h->h_end[j] - h->h_begin[j]);
+            if (src_num) src_num[k] = (int32_t)h->h_src[j];
+            if (is_leaf) is_leaf[k] = (h->h_flags[j] & 1u) ? 1 : 0;
+            if (is_active) is_active[k] = (h->h_flags[j] & 2u) ? 1 : 0;
+        }
+    });
+}
+
+int vpmfmm_tree_leaves(vpmfmm_handle *h, int64_t *leaf_id, int32_t *poff, int32_t *pidx) {
+    return guarded(h, [&] {
+        inspect(h);
+        const int nl = h->ctx.info.n_leaves;
+        std::vector<int> ord(h->h_leaf_idx.begin(), h->h_leaf_idx.end());
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return h->h_id[a] < h->h_id[b]; });
+        int off = 0;
+        for (int k = 0; k < nl; ++k) {
+            const int j = ord[k];
+            if (leaf_id) leaf_id[k] = h->h_id[j];
+            if (poff) poff[k] = off;
+            const int b = off;
+            for (uint32_t i = h->h_begin[j]; i < h->h_end[j]; ++i, ++off)
+                if (pidx) pidx[off] = (int32_t)h->h_perm[i];
+            // inside a leaf the device keeps Morton order; report index order
+            // (the order of the reference's parIDList, treeCell.cpp:1215-1232)
+            if (pidx) std::sort(pidx + b, pidx + off);
+        }
+        if (poff) poff[nl] = off;
+    });
+}
+
+int vpmfmm_tree_lists(vpmfmm_handle *h, int64_t id, int64_t *l1, int32_t *n1, int64_t *l2,
+                      int32_t *n2, int64_t *l3, int32_t *n3, int64_t *l4, int32_t *n4,
+                      int32_t cap) {
+    return guarded(h, [&] {
+        inspect(h);
+        const int nc = h->ctx.info.n_cells;
+        int j = -1;
+        for (int k = 0; k < nc; ++k)
+            if (h->h_id[k] == id) {
+                j = k;
+                break;
+            }
+        if (j < 0) throw Error(kBadArg, "no cell with this ID");
+        auto put = [&](const int *src, int cnt, int64_t *dst, int32_t *ncnt) {
+            if (ncnt) *ncnt = cnt;
+            if (dst)
+                for (int u = 0; u < cnt && u < cap; ++u) dst[u] = h->h_id[src[u]];
+        };
+        int c1 = 0, c3 = 0;
+        const int *p1 = nullptr, *p3 = nullptr;
+        if (h->h_flags[j] & 1u) {
+            const auto it = std::find(h->h_leaf_idx.begin(), h->h_leaf_idx.end(), (uint32_t)j);
+            const size_t t = (size_t)(it - h->h_leaf_idx.begin());
+            c1 = (int)(h->h_off1[t + 1] - h->h_off1[t]);
+            c3 = (int)(h->h_off3[t + 1] - h->h_off3[t]);
+            p1 = h->h_l1.data() + h->h_off1[t];
+            p3 = h->h_l3.data() + h->h_off3[t];
+        }
+        put(p1, c1, l1, n1);
+        put(p3, c3, l3, n3);
+        const bool m2l = h->h_level[j] >= 2;
+        put(h->h_l2.data() + 27 * (size_t)j, m2l ? h->h_n2[j] : 0, l2, n2);
+        put(h->h_l4.data() + 8 * (size_t)j, m2l ? h->h_n4[j] : 0, l4, n4);
+    });
+}
+
+int vpmfmm_tree_expansions(vpmfmm_handle *h, int64_t *ids, double *ak, double *bk) {
+    return guarded(h, [&] {
+        inspect(h);
+        if (h->exp_p < 0) throw Error(kBadArg, "no FP64 expansions from the last call");
+        const int nc = h->ctx.info.n_cells, P1 = h->exp_p + 1;
+        std::vector<double> a((size_t)nc * 2 * P1), b((size_t)nc * 2 * P1);
+        Ctx &c = h->ctx;
+        VPM_HIP(hipMemcpyAsync(a.data(), c.ak.p, a.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               c.stream));
+        VPM_HIP(hipMemcpyAsync(b.data(), c.bk.p, b.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        std::vector<int> ord(nc);
+        for (int j = 0; j < nc; ++j) ord[j] = j;
+        std::sort(ord.begin(), ord.end(), [&](int x, int y) { return h->h_id[x] < h->h_id[y]; });
+        for (int k = 0; k < nc; ++k) {
+            const int j = ord[k];
+            if (ids) ids[k] = h->h_id[j];
+            if (ak) std::memcpy(ak + (size_t)k * 2 * P1, a.data() + (size_t)j * 2 * P1, 16 * P1);
+            if (bk) std::memcpy(bk + (size_t)k * 2 * P1, b.data() + (size_t)j * 2 * P1, 16 * P1);
+        }
+    });
+}
+
+}  // extern "C"

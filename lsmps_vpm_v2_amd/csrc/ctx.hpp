@@ -1,0 +1,131 @@
+// ctx.hpp -- solver context: device buffers and the stage entry points.
+//
+// Layout in HBM (all arrays are structure-of-arrays):
+//   particles, Morton-sorted:  xs, ys, qs (f64), acts (u8), perm (u32: sorted
+//                              slot -> caller index), key (u32 Morton key)
+//   cells, level-major and Morton-ordered inside a level (root at index 0):
+//                              level (u8), flags (u8: leaf, active), key (u32
+//                              level prefix), begin/end (u32 particle range),
+//                              src (u32 active count), cx, cy (f64 centre),
+//                              parent (i32), child[4] (i32), nbr[9] (i32)
+//   lists:  M2L sources per cell, fixed stride (list 2: 27, list 4: 8);
+//           per leaf CSR of list-1 cells and list-3 cells
+//   expansions: ak, bk as (P+1) complex per cell, cell-major
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace vpm {
+
+// Header written by device kernels and read back by the host.
+struct DevHdr {
+    double root_len;
+    double pivot[2];
+    double mn[2], mx[2];
+    unsigned long long pairs;    // P2P (target, source) pairs evaluated
+    uint32_t sum2, sum4;         // list-2 / list-4 entries over all cells
+    unsigned flags;
+    uint32_t n_active;
+    uint32_t n_cells_pre;        // cells excluding the root
+    uint32_t n_leaves;
+    uint32_t lev_off[kMaxDepth + 2];
+    uint32_t tot1, tot3;
+    uint32_t leaf_lvl_min, leaf_lvl_max;
+    uint32_t pad[4];
+};
+
+// Grow-only device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes);
+    template <class T>
+    T *as(size_t count) {
+        return static_cast<T *>(get(count * sizeof(T) + 16));
+    }
+    void release();
+    ~DBuf() { release(); }
+};
+
+// Tree-build parameters (reference knobs plus the level extensions).
+struct TreeParams {
+    int max_par, max_src;
+    double exp_tree;
+    int min_level, max_level;
+};
+
+// Host copy of what the tree build produced.
+struct TreeInfo {
+    int n = 0;
+    int n_active = 0;
+    int n_cells = 0;         // including the root
+    int n_leaves = 0;
+    int max_level = 0;       // deepest leaf level
+    int min_leaf_level = 0;  // shallowest leaf level
+    uint32_t lev_off[kMaxDepth + 2] = {};
+    int tot1 = 0, tot3 = 0;
+    double root_len = 0, pivot[2] = {0, 0};
+};
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    DevHdr *hdr_d = nullptr;  // device header
+    DevHdr *hdr_h = nullptr;  // pinned host mirror
+
+    // particles
+    DBuf x_in, y_in, q_in, a_in, ex_out, ey_out;  // staging for host-pointer calls
+    DBuf key_a, key_b, val_a, val_b;
+    DBuf xs, ys, qs, acts;
+    uint32_t *key_sorted = nullptr;   // points into key_a or key_b after the sort
+    uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
+    DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
+    DBuf tmp;  // scan / sort scratch
+    DBuf bbox_part;
+
+    // cells
+    DBuf pre_level, pre_slot, lev_level, lev2pre;
+    DBuf c_level, c_flags, c_key, c_begin, c_end, c_src, c_cx, c_cy, c_parent, c_child, c_nbr;
+    DBuf leaf_flag_scan, leaf_idx;
+    DBuf l2, n2, l4, n4;
+    DBuf cnt1, cnt3, off1, off3, l1, l3;
+
+    // expansions
+    DBuf ak, bk, tab_m2l, tab_m2m, tab_l2l;
+
+    TreeInfo info;
+    std::vector<double> stage_ms;
+};
+
+// Precomputed coefficient tables (host-side builders).
+void fill_tables(int p, int binom_mode, std::vector<double> &m2l, std::vector<double> &m2m,
+                 std::vector<double> &l2l);
+int ref_binom_int32(int n, int k);
+double exact_binom(int n, int k);
+
+// Stage entry points (tree.hip).  Inputs are device pointers.
+void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q,
+                const uint8_t *active, const TreeParams &tp);
+void tree_lists(Ctx &c);
+
+// Optional stage markers: events 3..6 are recorded after the upward pass,
+// M2L, the downward pass and the evaluation.
+struct FmmMarks {
+    hipEvent_t *ev = nullptr;
+    void mark(int i, hipStream_t s) const {
+        if (ev) VPM_HIP(hipEventRecord(ev[i], s));
+    }
+};
+
+// FP64 FMM (fmm64.hip): expansions and evaluation.  Output either the field
+// (ex, ey) or the velocity (u = ey + u_inf, v = -ex + v_inf), scattered to the
+// caller's particle order.
+void fmm64_run(Ctx &c, int p, int list4_mode, int stage_mask, double *out_a, double *out_b,
+               bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
+
+// FP64 direct sum on the GPU (accuracy reference, not part of the FMM).
+void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,
+              int nt, const int *tidx, double *ex, double *ey, hipStream_t s);
+
+}  // namespace vpm

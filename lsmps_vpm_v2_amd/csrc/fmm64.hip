@@ -1,0 +1,363 @@
+// fmm64.hip -- plain FP64 FMM stages on gfx950.
+//
+// Expansion conventions follow fmm2D (reference src/FMM/fmm2D.cpp):
+//   multipole a_p = sum q (z - zc)^p                 (P2M, :237-284)
+//   M2M  a_p += a'_0 d^p + sum_k C(p-1,k-1)(p/k) a'_k d^(p-k),  d = zc' - zc   (:361-376)
+//   M2L  b_p -= a_0/p d^-p + sum_k C(p+k-1,k-1)(-1)^k/k a_k d^-(p+k), d = zs - zt (:465-481)
+//   L2L  b'_p += sum_{k>=p} C(k,p) b_k d^(k-p),  d = zc' - zc      (:637-643)
+//   M2P  F = sum a_p (z - zc)^-(p+1)                  (list 3, :1052-1062)
+//   L2P  F = sum_{p>=1} p b_p (z - zc)^(p-1)          (:1083-1093)
+//   Ex += Re F, Ey -= Im F; P2P Ex += q dx/r^2, Ey += q dy/r^2 (:164-173).
+// b_0 is never formed: L2L feeds b_p only from b_k with k >= p and L2P starts
+// at p = 1, so b_0 cannot reach the field.
+//
+// The per-order coefficient tables (M2L including the reference's int32
+// binomials in parity mode) are built on the host (api.cpp) and read from
+// global memory.  Thread mappings: one thread per (cell, order) for the
+// translations (Horner in the translation distance, no per-thread arrays),
+// one wavefront per leaf for the evaluation.
+#include "ctx.hpp"
+
+namespace vpm {
+namespace {
+
+constexpr unsigned kLeaf = 1u, kActive = 2u;
+
+struct cd {
+    double x, y;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd cscale(cd a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cd cinv(cd a) {
+    const double r = 1.0 / (a.x * a.x + a.y * a.y);
+    return {a.x * r, -a.y * r};
+}
+__device__ __forceinline__ cd cpowi(cd z, int p) {
+    cd r = {1.0, 0.0};
+    while (p > 0) {
+        if (p & 1) r = cmul(r, z);
+        z = cmul(z, z);
+        p >>= 1;
+    }
+    return r;
+}
+__device__ __forceinline__ cd ld(const double2 *a, size_t i) {
+    double2 v = a[i];
+    return {v.x, v.y};
+}
+__device__ __forceinline__ void st(double2 *a, size_t i, cd v) { a[i] = make_double2(v.x, v.y); }
+
+// P2M: thread per (leaf, order p); active particles of active leaves only.
+__global__ void __launch_bounds__(256)
+    k_p2m64(const uint32_t *leaf_idx, int nl, int P, const uint8_t *c_flags, const uint32_t *c_begin,
+            const uint32_t *c_end, const double *c_cx, const double *c_cy, const double *xs,
+            const double *ys, const double *qs, const uint8_t *as, double2 *ak) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P1 = P + 1;
+    if (idx >= (long long)nl * P1) return;
+    const int t = (int)(idx / P1), p = (int)(idx % P1);
+    const int j = (int)leaf_idx[t];
+    if (!(c_flags[j] & kActive)) return;
+    const double cx = c_cx[j], cy = c_cy[j];
+    cd acc = {0.0, 0.0};
+    for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
+        if (!as[i]) continue;
+        const cd zp = cpowi(cd{xs[i] - cx, ys[i] - cy}, p);
+        acc = cadd(acc, cscale(zp, qs[i]));
+    }
+    st(ak, (size_t)j * P1 + p, acc);
+}
+
+// M2M for the cells of one level: non-leaf active parents gather children.
+__global__ void __launch_bounds__(256)
+    k_m2m64(int j0, int j1, int P, const double *tab, const uint8_t *c_flags, const int *c_child,
+            const double *c_cx, const double *c_cy, double2 *ak) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P1 = P + 1;
+    if (idx >= (long long)(j1 - j0) * P1) return;
+    const int j = j0 + (int)(idx / P1), p = (int)(idx % P1);
+    const unsigned f = c_flags[j];
+    if ((f & kLeaf) || !(f & kActive)) return;
+    const double cx = c_cx[j], cy = c_cy[j];
+    cd acc = {0.0, 0.0};
+    for (int q = 0; q < 4; ++q) {
+        const int ch = c_child[4 * j + q];
+        if (ch < 0) continue;
+        const cd d = {c_cx[ch] - cx, c_cy[ch] - cy};
+        const size_t b = (size_t)ch * P1;
+        // sum_{k=0..p} c_k a'_k d^(p-k), c_0 = 1: Horner from k = 0 (power d^p)
+        cd s = ld(ak, b);
+        for (int k = 1; k <= p; ++k) s = cadd(cmul(s, d), cscale(ld(ak, b + k), tab[p * P1 + k]));
+        acc = cadd(acc, s);
+    }
+    st(ak, (size_t)j * P1 + p, acc);
+}
+
+// M2L for every cell of level >= 2 from lists 2 and 4.  tab[p][k] holds the
+// coefficient of a_k w^(p+k), w = 1/(zs - zt).
+__global__ void __launch_bounds__(256)
+    k_m2l64(int j0, int nc, int P, int list4_mode, const double *tab, const int *l2,
+            const uint8_t *n2, const int *l4, const uint8_t *n4, const double *c_cx,
+            const double *c_cy, const uint32_t *c_begin, const uint32_t *c_end, const double *xs,
+            const double *ys, const double *qs, const uint8_t *as, const double2 *ak, double2 *bk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P1 = P + 1;
+    if (idx >= (long long)(nc - j0) * P) return;
+    const int j = j0 + (int)(idx / P), p = 1 + (int)(idx % P);
+    const double cx = c_cx[j], cy = c_cy[j];
+    const double *row = tab + (size_t)p * P1;
+    cd acc = {0.0, 0.0};
+    const int m2 = n2[j], m4 = n4[j];
+    for (int u = 0; u < m2 + (list4_mode == 1 ? m4 : 0); ++u) {
+        const int sidx = u < m2 ? l2[27 * j + u] : l4[8 * j + (u - m2)];
+        const cd w = cinv(cd{c_cx[sidx] - cx, c_cy[sidx] - cy});
+        const size_t b = (size_t)sidx * P1;
+        cd s = cscale(ld(ak, b + P), row[P]);
+        for (int k = P - 1; k >= 0; --k) s = cadd(cmul(s, w), cscale(ld(ak, b + k), row[k]));
+        acc = cadd(acc, cmul(s, cpowi(w, p)));
+    }
+    if (list4_mode == 2) {
+        // P2L from the active particles of each list-4 leaf (fmm2D.cpp:526-553)
+        const double inv_p = 1.0 / (double)p;
+        for (int u = 0; u < m4; ++u) {
+            const int sidx = l4[8 * j + u];
+            for (uint32_t i = c_begin[sidx]; i < c_end[sidx]; ++i) {
+                if (!as[i]) continue;
+                const cd w = cinv(cd{xs[i] - cx, ys[i] - cy});
+                acc = cadd(acc, cscale(cpowi(w, p), -qs[i] * inv_p));
+            }
+        }
+    }
+    st(bk, (size_t)j * P1 + p, acc);
+}
+
+// L2L for the cells of one level (>= 3): each child adds its parent's shifted
+// local expansion.  tab[p][k] = C(k, p).
+__global__ void __launch_bounds__(256)
+    k_l2l64(int j0, int j1, int P, const double *tab, const int *c_parent, const double *c_cx,
+            const double *c_cy, double2 *bk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int P1 = P + 1;
+    if (idx >= (long long)(j1 - j0) * P) return;
+    const int j = j0 + (int)(idx / P), p = 1 + (int)(idx % P);
+    const int par = c_parent[j];
+    const cd d = {c_cx[j] - c_cx[par], c_cy[j] - c_cy[par]};
+    const size_t b = (size_t)par * P1;
+    const double *row = tab + (size_t)p * P1;
+    cd s = cscale(ld(bk, b + P), row[P]);
+    for (int k = P - 1; k >= p; --k) s = cadd(cmul(s, d), cscale(ld(bk, b + k), row[k]));
+    const size_t o = (size_t)j * P1 + p;
+    st(bk, o, cadd(ld(bk, o), s));
+}
+
+// Evaluation: one wavefront per leaf.  Lanes are split into G groups of
+// 64/G lanes; a group owns targets, the G groups split the sources and are
+// reduced with cross-lane shuffles at the end.
+constexpr int kEvalWaves = 4;
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+    return __shfl_xor(v, m, kWave);
+}
+
+__global__ void __launch_bounds__(kWave *kEvalWaves)
+    k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
+             const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
+             const uint32_t *off3, const int *l3, const double *xs, const double *ys,
+             const double *qs, const uint32_t *perm, const double2 *ak, const double2 *bk,
+             int velocity, double u_inf, double v_inf, int mask, double *out_a, double *out_b,
+             unsigned *flags, unsigned long long *pairs) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kEvalWaves + (threadIdx.x / kWave);
+    if (t >= nl) return;
+    const int j = (int)leaf_idx[t];
+    const uint32_t b = c_begin[j], e = c_end[j];
+    const int nt = (int)(e - b);
+    if (lane == 0) {
+        unsigned long long ns = 0;
+        bool self = false;
+        for (uint32_t u = off1[t]; u < off1[t + 1]; ++u) {
+            ns += c_end[l1[u]] - c_begin[l1[u]];
+            self |= l1[u] == j;
+        }
+        atomicAdd(pairs, ns * (unsigned long long)nt - (self ? (unsigned long long)nt : 0ull));
+    }
+    const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
+    const int LG = kWave / G;
+    const int g = lane / LG, tl = lane % LG;
+    const int P1 = P + 1;
+    const double lcx = c_cx[j], lcy = c_cy[j];
+    const uint32_t a1 = off1[t], z1 = off1[t + 1], a3 = off3[t], z3 = off3[t + 1];
+    bool dup = false;
+    for (uint32_t tb = b; tb < e; tb += LG) {
+        const uint32_t i = tb + tl;
+        const bool valid = i < e;
+        const double xi = valid ? xs[i] : 0.0, yi = valid ? ys[i] : 0.0;
+        double ex = 0.0, ey = 0.0;
+        // P2P over every particle of the list-1 cells, j != i (fmm2D.cpp:135-179)
+        for (uint32_t u = a1; u < ((mask & 1) ? z1 : a1); ++u) {
+            const int sc = l1[u];
+            const uint32_t sb = c_begin[sc], se = c_end[sc];
+            for (uint32_t src = sb + g; src < se; src += G) {
+                const double dx = xi - xs[src], dy = yi - ys[src];
+                const double r2 = dx * dx + dy * dy;
+                if (valid && src != i) {
+                    dup |= r2 == 0.0;
+                    const double qr = qs[src] / r2;
+                    ex += qr * dx;
+                    ey += qr * dy;
+                }
+            }
+        }
+        // M2P over list 3, split across groups
+        for (uint32_t u = a3 + g; u < ((mask & 2) ? z3 : 0); u += G) {
+            const int sc = l3[u];
+            const cd w = cinv(cd{xi - c_cx[sc], yi - c_cy[sc]});
+            const size_t sbk = (size_t)sc * P1;
+            cd f = ld(ak, sbk + P);
+            for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, w), ld(ak, sbk + p));
+            f = cmul(f, w);
+            if (valid) {
+                ex += f.x;
+                ey -= f.y;
+            }
+        }
+        for (int m = LG; m < kWave; m <<= 1) {
+            ex += shfl_xor_d(ex, m);
+            ey += shfl_xor_d(ey, m);
+        }
+        if (valid && g == 0) {
+            // L2P
+            const cd z = {xi - lcx, yi - lcy};
+            const size_t lb = (size_t)j * P1;
+            cd f = cscale(ld(bk, lb + P), (double)P);
+            for (int p = P - 1; p >= 1; --p) f = cadd(cmul(f, z), cscale(ld(bk, lb + p), (double)p));
+            if (mask & 4) {
+                ex += f.x;
+                ey -= f.y;
+            }
+            const uint32_t o = perm[i];
+            if (velocity) {
+                out_a[o] = ey + u_inf;
+                out_b[o] = -ex + v_inf;
+            } else {
+                out_a[o] = ex;
+                out_b[o] = ey;
+            }
+        }
+    }
+    if (dup) atomicOr(flags, kFlagDuplicate);
+}
+
+// FP64 direct sum with LDS source tiles (accuracy reference only).
+__global__ void __launch_bounds__(256)
+    k_direct64(int n, const double *x, const double *y, const double *q, const uint8_t *mask,
+               int nt, const int *tidx, double *ex, double *ey) {
+    __shared__ double sx[256], sy[256], sq[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = t < nt ? (tidx ? tidx[t] : t) : -1;
+    if (i >= n) i = -1;
+    const bool valid = i >= 0;
+    const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+    double ax = 0.0, ay = 0.0;
+    for (int base = 0; base < n; base += 256) {
+        const int jj = base + threadIdx.x;
+        if (jj < n) {
+            sx[threadIdx.x] = x[jj];
+            sy[threadIdx.x] = y[jj];
+            sq[threadIdx.x] = (mask && !mask[jj]) ? 0.0 : q[jj];
+        }
+        __syncthreads();
+        const int m = min(256, n - base);
+        for (int k = 0; k < m; ++k) {
+            const int src = base + k;
+            if (src == i) continue;
+            const double dx = xi - sx[k], dy = yi - sy[k];
+            const double qr = sq[k] / (dx * dx + dy * dy);
+            ax += qr * dx;
+            ay += qr * dy;
+        }
+        __syncthreads();
+    }
+    if (t < nt) {  // an out-of-range target index yields NaN
+        ex[t] = valid ? ax : __builtin_nan("");
+        ey[t] = valid ? ay : __builtin_nan("");
+    }
+}
+
+}  // namespace
+
+void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, double *out_b,
+               bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
+    hipStream_t s = c.stream;
+    const TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves, P1 = P + 1;
+    double2 *ak = c.ak.as<double2>((size_t)nc * P1), *bk = c.bk.as<double2>((size_t)nc * P1);
+    VPM_HIP(hipMemsetAsync(ak, 0, sizeof(double2) * (size_t)nc * P1, s));
+    VPM_HIP(hipMemsetAsync(bk, 0, sizeof(double2) * (size_t)nc * P1, s));
+    const uint8_t *c_flags = (const uint8_t *)c.c_flags.p;
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
+    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
+    const int *c_child = (const int *)c.c_child.p, *c_parent = (const int *)c.c_parent.p;
+    const double *xs = (const double *)c.xs.p, *ys = (const double *)c.ys.p,
+                 *qs = (const double *)c.qs.p;
+    const uint8_t *as = (const uint8_t *)c.acts.p;
+    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
+    const double *t_m2l = (const double *)c.tab_m2l.p, *t_m2m = (const double *)c.tab_m2m.p,
+                 *t_l2l = (const double *)c.tab_l2l.p;
+
+    k_p2m64<<<div_up((long long)nl * P1, 256), 256, 0, s>>>(leaf_idx, nl, P, c_flags, c_begin, c_end,
+                                                            c_cx, c_cy, xs, ys, qs, as, ak);
+    // M2M: levels maxL-1 .. 1 (fmm2D.cpp:309)
+    for (int L = ti.max_level - 1; L >= 1; --L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        if (j1 <= j0) continue;
+        k_m2m64<<<div_up((long long)(j1 - j0) * P1, 256), 256, 0, s>>>(j0, j1, P, t_m2m, c_flags,
+                                                                        c_child, c_cx, c_cy, ak);
+    }
+    mk.mark(3, s);
+    // M2L: every cell of level >= 2
+    const int j2 = (int)ti.lev_off[2];
+    if (nc > j2)
+        k_m2l64<<<div_up((long long)(nc - j2) * P, 256), 256, 0, s>>>(
+            j2, nc, P, list4_mode, t_m2l, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
+            (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
+            as, ak, bk);
+    mk.mark(4, s);
+    // L2L: children at levels 3 .. maxL (parents 2 .. maxL-1, fmm2D.cpp:587)
+    for (int L = 3; L <= ti.max_level; ++L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        if (j1 <= j0) continue;
+        k_l2l64<<<div_up((long long)(j1 - j0) * P, 256), 256, 0, s>>>(j0, j1, P, t_l2l, c_parent,
+                                                                      c_cx, c_cy, bk);
+    }
+    mk.mark(5, s);
+    k_eval64<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
+        leaf_idx, nl, P, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
+        (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs,
+        c.perm_sorted, ak, bk,
+        velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b, &c.hdr_d->flags,
+        &c.hdr_d->pairs);
+    mk.mark(6, s);
+    VPM_LAUNCH_CHECK();
+}
+
+// q = -omega s^2 / (2 pi) (velocity_calc.cpp:477).
+__global__ void __launch_bounds__(256)
+    k_sources64(int n, const double *s, const double *omega, double *q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = -omega[i] * (s[i] * s[i]) / (2 * M_PI);
+}
+
+void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st) {
+    k_sources64<<<div_up(n, 256), 256, 0, st>>>(n, s, omega, q);
+    VPM_LAUNCH_CHECK();
+}
+
+void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,
+              int nt, const int *tidx, double *ex, double *ey, hipStream_t s) {
+    k_direct64<<<div_up(nt, 256), 256, 0, s>>>(n, x, y, q, src_mask, nt, tidx, ex, ey);
+    VPM_LAUNCH_CHECK();
+}
+
+}  // namespace vpm

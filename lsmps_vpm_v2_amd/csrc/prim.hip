@@ -1,0 +1,234 @@
+// prim.hip -- exclusive scans and a stable LSD radix sort for gfx950.
+//
+// Scans: reduce -> spine -> down-sweep over 2048-element tiles (256 threads x
+// 8 items), wave64 shuffles inside a tile.  Radix sort: 8-bit digits over
+// 4096-element tiles; per-block digit histograms, a scan of the digit-major
+// histogram table, then a stable scatter whose in-tile ranks come from
+// 64-lane ballot "peer masks" (lanes of a wavefront holding the same digit).
+#include "prim.hpp"
+
+namespace vpm {
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+constexpr int kRsBlock = 256;   // = number of digit values
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsBlock * kRsItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan across a block of BLOCK threads.  `sm` holds BLOCK/64 words.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, uint32_t &total) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == kWave - 1) sm[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / kWave; ++i) {
+        uint32_t s = sm[i];
+        before += (i < w) ? s : 0u;
+        all += s;
+    }
+    __syncthreads();
+    total = all;
+    return before + inc - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const T *in, int n, uint32_t *bsum) {
+    __shared__ uint32_t sm[kScanBlock / kWave];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        if (i < n) s += (uint32_t)in[i];
+    }
+    uint32_t tot;
+    block_excl_scan<kScanBlock>(s, sm, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_spine(uint32_t *bsum, int nb, uint32_t *total) {
+    __shared__ uint32_t sm[1024 / kWave];
+    uint32_t carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        int i = base + threadIdx.x;
+        uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan<1024>(v, sm, tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kScanBlock)
+    k_scan_down(const T *in, uint32_t *out, int n, const uint32_t *bsum, const uint32_t *total,
+                int write_end) {
+    __shared__ uint32_t sm[kScanBlock / kWave];
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        v[j] = i < n ? (uint32_t)in[i] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t run = bsum[blockIdx.x] + block_excl_scan<kScanBlock>(s, sm, tot);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (write_end && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = *total;
+}
+
+__global__ void k_write_zero_total(uint32_t *out, int n, uint32_t *total, int write_end) {
+    if (total) *total = 0;
+    if (write_end) out[n] = 0;
+}
+
+template <typename T>
+void scan_impl(const T *in, uint32_t *out, int n, uint32_t *total, bool write_end, uint32_t *tmp,
+               hipStream_t s) {
+    if (n <= 0) {
+        k_write_zero_total<<<1, 1, 0, s>>>(out, 0, total, write_end ? 1 : 0);
+        VPM_LAUNCH_CHECK();
+        return;
+    }
+    const int nb = div_up(n, kScanTile);
+    uint32_t *bsum = tmp;
+    uint32_t *tot = total ? total : tmp + nb;
+    k_scan_reduce<T><<<nb, kScanBlock, 0, s>>>(in, n, bsum);
+    k_scan_spine<<<1, 1024, 0, s>>>(bsum, nb, tot);
+    k_scan_down<T><<<nb, kScanBlock, 0, s>>>(in, out, n, bsum, tot, write_end ? 1 : 0);
+    VPM_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(kRsBlock)
+    k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *table, int nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kRsTile;
+#pragma unroll 4
+    for (int j = 0; j < kRsItems; ++j) {
+        long long i = base + (long long)j * kRsBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    table[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: element order inside a tile is (item j, wave w, lane l),
+// which is the index order; ranks are built in that order.
+__global__ void __launch_bounds__(kRsBlock)
+    k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int n,
+                 int shift, const uint32_t *table, int nb) {
+    constexpr int kW = kRsBlock / kWave;
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t wcnt[kW][256];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    goff[t] = table[(size_t)t * nb + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < kW; ++ww) wcnt[ww][t] = 0;
+    __syncthreads();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    const long long base = (long long)blockIdx.x * kRsTile;
+    for (int j = 0; j < kRsItems; ++j) {
+        const long long i = base + (long long)j * kRsBlock + t;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u;
+        const uint32_t v = valid ? vin[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint32_t bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank_w = (uint32_t)__popcll(peers & below);
+        if (valid && (peers & below) == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t c[kW], sum = 0;
+#pragma unroll
+        for (int ww = 0; ww < kW; ++ww) {
+            c[ww] = wcnt[ww][t];
+            wcnt[ww][t] = sum;
+            sum += c[ww];
+        }
+        __syncthreads();
+        uint32_t dst = 0;
+        if (valid) dst = goff[d] + wcnt[w][d] + rank_w;
+        __syncthreads();
+        goff[t] += sum;
+#pragma unroll
+        for (int ww = 0; ww < kW; ++ww) wcnt[ww][t] = 0;
+        __syncthreads();
+        if (valid) {
+            kout[dst] = k;
+            vout[dst] = v;
+        }
+    }
+}
+
+}  // namespace
+
+size_t scan_tmp_words(long long n) { return (size_t)div_up(n > 0 ? n : 1, kScanTile) + 2; }
+
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
+                        bool write_total_at_end, uint32_t *tmp, hipStream_t s) {
+    scan_impl<uint32_t>(in, out, n, total, write_total_at_end, tmp, s);
+}
+
+void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
+                       bool write_total_at_end, uint32_t *tmp, hipStream_t s) {
+    scan_impl<uint8_t>(in, out, n, total, write_total_at_end, tmp, s);
+}
+
+size_t radix_tmp_words(long long n) {
+    const long long nb = div_up(n > 0 ? n : 1, kRsTile);
+    return (size_t)(256 * nb) + scan_tmp_words(256 * nb);
+}
+
+bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s) {
+    if (n <= 1) return false;
+    const int nb = div_up(n, kRsTile);
+    uint32_t *table = tmp;
+    uint32_t *scan_tmp = tmp + (size_t)256 * nb;
+    bool alt = false;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        const uint32_t *ki = alt ? keys_alt : keys;
+        const uint32_t *vi = alt ? vals_alt : vals;
+        uint32_t *ko = alt ? keys : keys_alt;
+        uint32_t *vo = alt ? vals : vals_alt;
+        k_rs_hist<<<nb, kRsBlock, 0, s>>>(ki, n, shift, table, nb);
+        scan_impl<uint32_t>(table, table, 256 * nb, nullptr, false, scan_tmp, s);
+        k_rs_scatter<<<nb, kRsBlock, 0, s>>>(ki, vi, ko, vo, n, shift, table, nb);
+        VPM_LAUNCH_CHECK();
+        alt = !alt;
+    }
+    return alt;
+}
+
+}  // namespace vpm

@@ -1,0 +1,28 @@
+// prim.hpp -- device-wide primitives: exclusive scans and an LSD radix sort.
+#pragma once
+
+#include "common.hpp"
+
+namespace vpm {
+
+// Scratch words needed by the scans for n elements.
+size_t scan_tmp_words(long long n);
+
+// out[i] = sum_{j<i} in[j]; if total != nullptr, *total (device) = sum of all.
+// out may have room for n+1 entries: when write_total_at_end, out[n] = total.
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *total,
+                        bool write_total_at_end, uint32_t *tmp, hipStream_t s);
+void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
+                       bool write_total_at_end, uint32_t *tmp, hipStream_t s);
+
+// Scratch words needed by the radix sort for n elements.
+size_t radix_tmp_words(long long n);
+
+// Stable LSD radix sort of (key, val) pairs on key bits [begin_bit, end_bit),
+// 8 bits per pass.  Buffers ping-pong between (keys, vals) and
+// (keys_alt, vals_alt); the return value tells where the result lies
+// (false: keys/vals, true: keys_alt/vals_alt).
+bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s);
+
+}  // namespace vpm

@@ -1,0 +1,327 @@
+"""Host-side interface of the MI355X FMM velocity solver.
+
+``FmmSolver`` wraps one C-ABI handle (include/vpmfmm.h).  The classes at the
+bottom mirror the reference's interface for this path so code written against
+it reads the same:
+
+* ``Particle``      -- the fields of ``struct Particle`` the path touches
+                       (reference Utils.hpp:77-159);
+* ``VelocityCalc``  -- ``get_velocity(particle, base_grid, step)``
+                       (velocity_calc.hpp:42, velocity_calc.cpp:149-251);
+* ``TreeCell``      -- ``initializeTree`` / ``updateTree`` / ``intList`` /
+                       ``extList`` and the tree summary (treeCell.hpp:83-195);
+* ``fmm2D``         -- ``calcField`` and ``get_Field_x/y`` (fmm2D.hpp:15-68).
+
+Host arrays are numpy; device entry points take torch tensors on the GPU
+(torch only provides device memory and streams here).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP64, MIXED, Opts, Stats, VpmFmmError
+
+_PRECISION = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u8(a):
+    return np.ascontiguousarray(np.asarray(a) != 0, dtype=np.uint8)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _dptr(t):
+    return C.c_void_p(int(t.data_ptr()))
+
+
+class FmmSolver:
+    """One solver handle: persistent device buffers, one HIP stream."""
+
+    def __init__(self, p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
+                 precision="fp64", binom_mode=1, list4_mode=1, u_inf=1.0, v_inf=0.0,
+                 device=0, timing=False, stage_mask=0):
+        self._L = _lib.load()
+        o = _lib.default_opts()
+        o.p, o.max_par, o.max_src, o.exp_tree = p, max_par, max_src, exp_tree
+        o.min_level, o.max_level = min_level, max_level
+        o.precision = _PRECISION[precision]
+        o.binom_mode, o.list4_mode = binom_mode, list4_mode
+        o.u_inf, o.v_inf, o.device, o.timing = u_inf, v_inf, device, 1 if timing else 0
+        o.stage_mask = stage_mask
+        self._o = o
+        h = C.c_void_p()
+        st = self._L.vpmfmm_create(C.byref(o), C.byref(h))
+        if st != 0:
+            raise VpmFmmError(st, "vpmfmm_create", "invalid options or device")
+        self._h = h
+
+    # ------------------------------------------------------------------ utils
+    def _check(self, st, where):
+        if st != 0:
+            msg = self._L.vpmfmm_last_error(self._h).decode(errors="replace")
+            raise VpmFmmError(st, where, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vpmfmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def opts(self) -> Opts:
+        return self._o
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            if k == "precision":
+                v = _PRECISION[v]
+            if k == "timing":
+                v = 1 if v else 0
+            setattr(self._o, k, v)
+        self._check(self._L.vpmfmm_set_opts(self._h, C.byref(self._o)), "vpmfmm_set_opts")
+
+    def set_stream(self, stream):
+        """Run on a torch.cuda.Stream (None: the handle's own stream)."""
+        ptr = None if stream is None else C.c_void_p(int(stream.cuda_stream))
+        self._check(self._L.vpmfmm_set_stream(self._h, ptr), "vpmfmm_set_stream")
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self._L.vpmfmm_last_stats(self._h, C.byref(s)), "vpmfmm_last_stats")
+        return s.as_dict()
+
+    # ------------------------------------------------------------- host paths
+    def field(self, x, y, q, active):
+        """(ex, ey) of fmm2D::calcField for host arrays."""
+        x, y, q, a = _f64(x), _f64(y), _f64(q), _u8(active)
+        n = len(x)
+        ex, ey = np.empty(n), np.empty(n)
+        self._check(self._L.vpmfmm_field(self._h, n, _ptr(x), _ptr(y), _ptr(q), _ptr(a),
+                                         _ptr(ex), _ptr(ey)), "vpmfmm_field")
+        return ex, ey
+
+    def velocity(self, x, y, s, omega, active):
+        """(u, v) of VelocityCalc::get_velocity (FMM branch) for host arrays."""
+        x, y, s, w, a = _f64(x), _f64(y), _f64(s), _f64(omega), _u8(active)
+        n = len(x)
+        u, v = np.empty(n), np.empty(n)
+        self._check(self._L.vpmfmm_velocity(self._h, n, _ptr(x), _ptr(y), _ptr(s), _ptr(w),
+                                            _ptr(a), _ptr(u), _ptr(v)), "vpmfmm_velocity")
+        return u, v
+
+    # ----------------------------------------------------------- device paths
+    def field_device(self, x, y, q, active, ex=None, ey=None):
+        """Device tensors in, device tensors out (float64 / uint8, contiguous)."""
+        n = x.numel()
+        if ex is None:
+            ex = x.new_empty(n)
+        if ey is None:
+            ey = x.new_empty(n)
+        self._check(self._L.vpmfmm_field_device(self._h, n, _dptr(x), _dptr(y), _dptr(q),
+                                                _dptr(active), _dptr(ex), _dptr(ey)),
+                    "vpmfmm_field_device")
+        return ex, ey
+
+    def velocity_device(self, x, y, s, omega, active, u=None, v=None):
+        n = x.numel()
+        if u is None:
+            u = x.new_empty(n)
+        if v is None:
+            v = x.new_empty(n)
+        self._check(self._L.vpmfmm_velocity_device(self._h, n, _dptr(x), _dptr(y), _dptr(s),
+                                                   _dptr(omega), _dptr(active), _dptr(u),
+                                                   _dptr(v)), "vpmfmm_velocity_device")
+        return u, v
+
+    def direct_device(self, x, y, q, src_mask=None, targets=None):
+        """FP64 direct sum on the GPU at `targets` (int32 device tensor) or all."""
+        n = x.numel()
+        nt = n if targets is None else targets.numel()
+        ex, ey = x.new_empty(nt), x.new_empty(nt)
+        self._check(self._L.vpmfmm_direct_device(
+            self._h, n, _dptr(x), _dptr(y), _dptr(q),
+            None if src_mask is None else _dptr(src_mask), nt,
+            None if targets is None else _dptr(targets), _dptr(ex), _dptr(ey)),
+            "vpmfmm_direct_device")
+        return ex, ey
+
+    # --------------------------------------------------------- tree inspection
+    def build_tree(self, x, y, active):
+        x, y, a = _f64(x), _f64(y), _u8(active)
+        self._check(self._L.vpmfmm_build_tree(self._h, len(x), _ptr(x), _ptr(y), _ptr(a)),
+                    "vpmfmm_build_tree")
+
+    def tree_counts(self):
+        v = [C.c_int32() for _ in range(4)]
+        self._check(self._L.vpmfmm_tree_counts(self._h, *[C.byref(t) for t in v]),
+                    "vpmfmm_tree_counts")
+        return tuple(t.value for t in v)
+
+    def tree_geometry(self):
+        r = C.c_double()
+        piv = np.zeros(2)
+        self._check(self._L.vpmfmm_tree_geometry(self._h, C.byref(r),
+                                                 piv.ctypes.data_as(C.POINTER(C.c_double))),
+                    "vpmfmm_tree_geometry")
+        return r.value, piv
+
+    def tree_cells(self) -> dict:
+        nc = self.tree_counts()[0]
+        out = dict(id=np.zeros(nc, np.int64), level=np.zeros(nc, np.int32),
+                   ix=np.zeros(nc, np.int32), iy=np.zeros(nc, np.int32),
+                   par_num=np.zeros(nc, np.int32), src_num=np.zeros(nc, np.int32),
+                   is_leaf=np.zeros(nc, np.uint8), is_active=np.zeros(nc, np.uint8))
+        self._check(self._L.vpmfmm_tree_cells(
+            self._h, *[out[k].ctypes.data_as(t) for k, t in (
+                ("id", _lib._lp), ("level", _lib._ip), ("ix", _lib._ip), ("iy", _lib._ip),
+                ("par_num", _lib._ip), ("src_num", _lib._ip), ("is_leaf", _lib._up),
+                ("is_active", _lib._up))]), "vpmfmm_tree_cells")
+        return out
+
+    def tree_leaves(self, n):
+        nl = self.tree_counts()[1]
+        lid = np.zeros(nl, np.int64)
+        poff = np.zeros(nl + 1, np.int32)
+        pidx = np.zeros(n, np.int32)
+        self._check(self._L.vpmfmm_tree_leaves(self._h, lid.ctypes.data_as(_lib._lp),
+                                               poff.ctypes.data_as(_lib._ip),
+                                               pidx.ctypes.data_as(_lib._ip)),
+                    "vpmfmm_tree_leaves")
+        return lid, poff, pidx
+
+    def tree_expansions(self):
+        """(ids, ak, bk) of the last FP64 call; ak/bk (n_cells, p+1) complex."""
+        nc = self.tree_counts()[0]
+        P1 = self._o.p + 1
+        ids = np.zeros(nc, np.int64)
+        ak = np.zeros((nc, P1), np.complex128)
+        bk = np.zeros((nc, P1), np.complex128)
+        self._check(self._L.vpmfmm_tree_expansions(
+            self._h, ids.ctypes.data_as(_lib._lp), ak.ctypes.data_as(_lib._dp),
+            bk.ctypes.data_as(_lib._dp)), "vpmfmm_tree_expansions")
+        return ids, ak, bk
+
+    def tree_lists(self, cell_id, cap=4096):
+        bufs = [np.zeros(cap, np.int64) for _ in range(4)]
+        cnt = [C.c_int32() for _ in range(4)]
+        self._check(self._L.vpmfmm_tree_lists(
+            self._h, int(cell_id),
+            bufs[0].ctypes.data_as(_lib._lp), C.byref(cnt[0]),
+            bufs[1].ctypes.data_as(_lib._lp), C.byref(cnt[1]),
+            bufs[2].ctypes.data_as(_lib._lp), C.byref(cnt[2]),
+            bufs[3].ctypes.data_as(_lib._lp), C.byref(cnt[3]), cap), "vpmfmm_tree_lists")
+        if max(c.value for c in cnt) > cap:
+            return self.tree_lists(cell_id, cap=max(c.value for c in cnt))
+        return tuple(b[: c.value].copy() for b, c in zip(bufs, cnt))
+
+
+# ---------------------------------------------------------------------------
+# mirrors of the reference interface
+# ---------------------------------------------------------------------------
+@dataclass
+class Particle:
+    """Fields of the reference ``Particle`` used by the velocity path."""
+    x: np.ndarray
+    y: np.ndarray
+    s: np.ndarray
+    vorticity: np.ndarray
+    isActive: np.ndarray
+    u: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    v: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+    @property
+    def num(self) -> int:
+        return len(self.x)
+
+
+class VelocityCalc:
+    """``VelocityCalc`` with the FMM branch (VELOCITY_2D_TYPE 2) on the GPU."""
+
+    def __init__(self, **solver_opts):
+        self._solver = FmmSolver(**solver_opts)
+
+    @property
+    def solver(self) -> FmmSolver:
+        return self._solver
+
+    def get_velocity(self, particle: Particle, base_grid=None, step: int = 0) -> None:
+        """velocity_calc.cpp:149-251: p.u, p.v = rotational part + free stream.
+
+        ``base_grid`` is accepted and unused, as on the reference's FMM branch;
+        ``step`` is accepted for signature parity (the tree is rebuilt every
+        call, as TreeCell::updateTree does, treeCell.cpp:1245-1249)."""
+        particle.u, particle.v = self._solver.velocity(
+            particle.x, particle.y, particle.s, particle.vorticity, particle.isActive)
+
+
+class TreeCell:
+    """Tree summary and lists of the last build (treeCell.hpp:83-195)."""
+
+    def __init__(self, solver: FmmSolver | None = None, **solver_opts):
+        self._solver = solver or FmmSolver(**solver_opts)
+        self.min_level = 2
+        self.max_level = 2
+        self.cellCount = 0
+        self.leafList = np.zeros(0, np.int64)
+        self.rootLength = 0.0
+        self.pivotCoord = np.zeros(2)
+
+    def initializeTree(self, parPos, activeFlag):
+        pos = np.asarray(parPos, dtype=np.float64)
+        self._n = len(pos)
+        self._solver.build_tree(pos[:, 0], pos[:, 1], activeFlag)
+        nc, nl, lo, hi = self._solver.tree_counts()
+        self.cellCount, self.min_level, self.max_level = nc, lo, hi
+        self.rootLength, self.pivotCoord = self._solver.tree_geometry()
+        self.leafList, self._poff, self._pidx = self._solver.tree_leaves(self._n)
+
+    def updateTree(self, parPos, activeFlag):
+        self.initializeTree(parPos, activeFlag)
+
+    def cells(self) -> dict:
+        return self._solver.tree_cells()
+
+    def parIDList(self, leaf_pos: int) -> np.ndarray:
+        return self._pidx[self._poff[leaf_pos]:self._poff[leaf_pos + 1]]
+
+    def intList(self, cell_id):
+        l1, _, l3, _ = self._solver.tree_lists(cell_id)
+        return l1, l3
+
+    def extList(self, cell_id):
+        _, l2, _, l4 = self._solver.tree_lists(cell_id)
+        return l2, l4
+
+
+class fmm2D:  # noqa: N801 -- reference class name
+    """``fmm2D::calcField`` / ``get_Field_x`` / ``get_Field_y``."""
+
+    def __init__(self, solver: FmmSolver | None = None, **solver_opts):
+        self._solver = solver or FmmSolver(**solver_opts)
+        self._ex = np.zeros(0)
+        self._ey = np.zeros(0)
+
+    def calcField(self, cellTree: TreeCell | None, parPos, activeFlag, srcVal):
+        pos = np.asarray(parPos, dtype=np.float64)
+        self._ex, self._ey = self._solver.field(pos[:, 0], pos[:, 1], srcVal, activeFlag)
+
+    def get_Field_x(self):
+        return self._ex.copy()
+
+    def get_Field_y(self):
+        return self._ey.copy()

@@ -1,0 +1,118 @@
+"""Seeded synthetic particle fields for the benchmark configurations.
+
+Every generator returns a dict of float64 arrays ``x, y, s, omega`` and uint8
+``active``, plus ``q = -omega s^2 / (2 pi)`` (the reference's source packing,
+velocity_calc.cpp:477).  Random numbers come from numpy's PCG64 with a fixed
+seed, so the fields are identical on every machine.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _pack(x, y, s, omega, active=None, **extra):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    s = np.ascontiguousarray(s, np.float64)
+    omega = np.ascontiguousarray(omega, np.float64)
+    active = np.ones(len(x), np.uint8) if active is None else np.ascontiguousarray(active, np.uint8)
+    q = -omega * (s * s) / (2 * np.pi)
+    return dict(x=x, y=y, s=s, omega=omega, active=active, q=q, n=len(x), **extra)
+
+
+def lamb_oseen(x, y, cx, cy, gamma, core):
+    """Vorticity of a Lamb-Oseen vortex: gamma / (pi a^2) exp(-r^2 / a^2)."""
+    r2 = (x - cx) ** 2 + (y - cy) ** 2
+    return gamma / (np.pi * core * core) * np.exp(-r2 / (core * core))
+
+
+def uniform_vortices(seed: int = 1, h: float = 0.0632, half: float = 10.0, n_vort: int = 3,
+                     core: float = 0.5, noise: float = 1e-3) -> dict:
+    """Uniform cell-centred grid on [-half, half]^2 with spacing h (316^2 =
+    99,856 particles at the defaults), vorticity = sum of n_vort Lamb-Oseen
+    vortices (core 0.5, circulation U[-2, 2], centres U[-5, 5]^2) + N(0, noise)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = int(np.floor(2 * half / h))
+    c = -half + (np.arange(m) + 0.5) * h
+    X, Y = np.meshgrid(c, c, indexing="xy")
+    x, y = X.ravel(), Y.ravel()
+    gam = rng.uniform(-2.0, 2.0, n_vort)
+    cen = rng.uniform(-5.0, 5.0, (n_vort, 2))
+    w = np.zeros_like(x)
+    for k in range(n_vort):
+        w += lamb_oseen(x, y, cen[k, 0], cen[k, 1], gam[k], core)
+    w += rng.normal(0.0, noise, len(x))
+    return _pack(x, y, np.full(len(x), h), w, name="uniform_vortices_100k")
+
+
+def boundary_layer(seed: int = 2, lx: float = 10.2, h0: float = 0.0025, band: float = 0.2,
+                   levels: int = 4, half: float = 10.0, delta: float = 0.02, U: float = 1.0,
+                   perturb: float = 0.05, drop: float = 1e-6) -> dict:
+    """Impulsively started flat-plate boundary layer (plate length 1, thickness
+    0.02, at the origin): rows of spacing h0 for |y| < band, then spacing doubling
+    per band of doubled width for `levels` bands, out to |y| = half; columns span
+    x in [-lx/2, lx/2] at the row spacing.  omega = sign(y) (U/delta) exp(-|y|/delta)
+    (1 + perturb U[-1, 1]); particles with |omega| < drop are removed.  The
+    defaults give about 780,000 particles in the strip |y| < 0.36."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs, ys, hs = [], [], []
+    lo, hi, h = 0.0, band, h0
+    for lev in range(levels + 1):
+        if lev == levels:
+            hi = half
+        nrow = int(round((hi - lo) / h))
+        yrow = lo + (np.arange(nrow) + 0.5) * h
+        ncol = int(round(lx / h))
+        xcol = -lx / 2 + (np.arange(ncol) + 0.5) * h
+        X, Y = np.meshgrid(xcol, yrow, indexing="xy")
+        for sgn in (1.0, -1.0):
+            xs.append(X.ravel())
+            ys.append(sgn * Y.ravel())
+            hs.append(np.full(X.size, h))
+        lo, hi, h = hi, 2 * hi, 2 * h
+        if lo >= half:
+            break
+    x, y, s = np.concatenate(xs), np.concatenate(ys), np.concatenate(hs)
+    w = np.sign(y) * (U / delta) * np.exp(-np.abs(y) / delta)
+    keep = np.abs(w) >= drop
+    x, y, s, w = x[keep], y[keep], s[keep], w[keep]
+    w = w * (1.0 + perturb * rng.uniform(-1.0, 1.0, len(w)))
+    return _pack(x, y, s, w, name="flat_plate_boundary_layer_780k")
+
+
+def perlman(h: float = 0.02, half: float = 1.2) -> dict:
+    """Perlman vortex omega = (1 - r^2)^7 (r <= 1) on a uniform grid
+    (reference src/initialization/initVor.cpp:10-53) with its closed-form
+    velocity u = -y/(16 r^2)(1 - (1 - r^2)^8), v = x/(16 r^2)(...) inside r <= 1
+    and without the bracket outside (initVor.cpp:62-184)."""
+    m = int(round(2 * half / h))
+    c = -half + (np.arange(m) + 0.5) * h
+    X, Y = np.meshgrid(c, c, indexing="xy")
+    x, y = X.ravel(), Y.ravel()
+    r2 = x * x + y * y
+    w = np.where(r2 <= 1.0, np.clip(1.0 - r2, 0.0, None) ** 7, 0.0)
+    brk = np.where(r2 <= 1.0, 1.0 - np.clip(1.0 - r2, 0.0, None) ** 8, 1.0)
+    u = -y / (16 * r2) * brk
+    v = x / (16 * r2) * brk
+    return _pack(x, y, np.full(len(x), h), w, u_exact=u, v_exact=v, name="perlman")
+
+
+def random_clusters(n: int, seed: int = 0, active_frac: float = 1.0) -> dict:
+    """Half clustered Gaussian blob, half uniform background (test input)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = n // 2
+    x = np.concatenate([rng.normal(0.0, 0.3, m), rng.uniform(-3, 3, n - m)])
+    y = np.concatenate([rng.normal(0.5, 0.2, m), rng.uniform(-2, 2, n - m)])
+    s = np.full(n, 0.01)
+    w = rng.normal(0.0, 1.0, n) * 100.0
+    a = (rng.uniform(0, 1, n) < active_frac).astype(np.uint8)
+    return _pack(x, y, s, w, active=a, name=f"clusters_{n}")
+
+
+def subsample(field: dict, n_targets: int, seed: int = 7) -> np.ndarray:
+    """Seeded sorted sample of target indices for accuracy checks."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = field["n"]
+    if n_targets >= n:
+        return np.arange(n, dtype=np.int32)
+    return np.sort(rng.choice(n, n_targets, replace=False)).astype(np.int32)

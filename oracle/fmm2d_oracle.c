@@ -49,6 +49,7 @@ void orc_default_opts(orc_opts *o) {
     o->max_level = 0;
     o->binom_mode = 1;   /* reference helper */
     o->list4_mode = 1;   /* M2L_P2_METHOD_TYPE 1, fmm2D.cpp:7 */
+    o->stage_mask = 0;
 }
 
 static int opts_ok(const orc_opts *o) {
@@ -61,6 +62,7 @@ static int opts_ok(const orc_opts *o) {
     if (o->max_level && o->max_level < o->min_level) return 0;
     if (o->binom_mode != 0 && o->binom_mode != 1) return 0;
     if (o->list4_mode != 1 && o->list4_mode != 2) return 0;
+    if (o->stage_mask < 0 || o->stage_mask > 7) return 0;
     return 1;
 }
 
@@ -555,8 +557,13 @@ void orc_p2m(int np, const double *x, const double *y, const double *q,
     }
 }
 
-/* fmm2D.cpp:361-376 with d = z_child - z_parent. */
-void orc_m2m(const double *akc, double dx, double dy, int p, double *akp) {
+static double binom_sel(int binom_mode, int n, int k) {
+    return binom_mode ? (double)orc_binom_ref(n, k) : orc_binom_exact(n, k);
+}
+
+/* fmm2D.cpp:361-376 with d = z_child - z_parent.  binComb(p-1, k-1) also
+ * wraps in int32 once P >= 19, hence binom_mode here too. */
+void orc_m2m(const double *akc, double dx, double dy, int p, int binom_mode, double *akp) {
     cx_t d = cx(dx, dy);
     cx_t *dp = (cx_t *)malloc(sizeof(cx_t) * (p + 1));
     dp[0] = cx(1.0, 0.0);
@@ -566,7 +573,7 @@ void orc_m2m(const double *akc, double dx, double dy, int p, double *akp) {
     for (int m = 1; m <= p; ++m) {
         cx_t s = cmul(a0, dp[m]);
         for (int k = 1; k <= m; ++k) {
-            double c = orc_binom_exact(m - 1, k - 1) * ((double)m / (double)k);
+            double c = binom_sel(binom_mode, m - 1, k - 1) * ((double)m / (double)k);
             s = cadd(s, cscale(cmul(cx(akc[2 * k], akc[2 * k + 1]), dp[m - k]), c));
         }
         akp[2 * m] += s.re; akp[2 * m + 1] += s.im;
@@ -585,8 +592,7 @@ void orc_m2l(const double *aks, double dx, double dy, int p, int binom_mode, dou
     for (int m = 1; m <= p; ++m) {
         cx_t s = cscale(cmul(a0, wp[m]), 1.0 / (double)m);
         for (int k = 1; k <= p; ++k) {
-            double c = binom_mode ? (double)orc_binom_ref(m + k - 1, k - 1)
-                                  : orc_binom_exact(m + k - 1, k - 1);
+            double c = binom_sel(binom_mode, m + k - 1, k - 1);
             double sg = (k & 1) ? -1.0 : 1.0;
             s = cadd(s, cscale(cmul(cx(aks[2 * k], aks[2 * k + 1]), wp[m + k]), c * sg / (double)k));
         }
@@ -595,8 +601,8 @@ void orc_m2l(const double *aks, double dx, double dy, int p, int binom_mode, dou
     free(wp);
 }
 
-/* fmm2D.cpp:637-643 with d = z_child - z_parent. */
-void orc_l2l(const double *bkp, double dx, double dy, int p, double *bkc) {
+/* fmm2D.cpp:637-643 with d = z_child - z_parent (binComb(k, p) wraps from P = 18). */
+void orc_l2l(const double *bkp, double dx, double dy, int p, int binom_mode, double *bkc) {
     cx_t d = cx(dx, dy);
     cx_t *dp = (cx_t *)malloc(sizeof(cx_t) * (p + 1));
     dp[0] = cx(1.0, 0.0);
@@ -605,7 +611,7 @@ void orc_l2l(const double *bkp, double dx, double dy, int p, double *bkc) {
         cx_t s = cx(0.0, 0.0);
         for (int k = m; k <= p; ++k)
             s = cadd(s, cscale(cmul(cx(bkp[2 * k], bkp[2 * k + 1]), dp[k - m]),
-                               orc_binom_exact(k, m)));
+                               binom_sel(binom_mode, k, m)));
         bkc[2 * m] += s.re; bkc[2 * m + 1] += s.im;
     }
     free(dp);
@@ -639,8 +645,11 @@ static int cmp_i64(const void *a, const void *b) {
     return x < y ? -1 : x > y;
 }
 
-int orc_fmm_field(int n, const double *x, const double *y, const double *q,
-                  const uint8_t *active, const orc_opts *o, double *ex, double *ey) {
+/* Core of fmm2D::setupFMM + calcField.  With ids/ak_out/bk_out non-NULL the
+ * per-cell expansions are also returned, cells sorted by ID. */
+static int fmm_run(int n, const double *x, const double *y, const double *q,
+                   const uint8_t *active, const orc_opts *o, double *ex, double *ey,
+                   int64_t *ids, double *ak_out, double *bk_out) {
     orc_tree *t = NULL;
     int st = orc_tree_build(&t, n, x, y, active, o);
     if (st) return st;
@@ -686,7 +695,7 @@ int orc_fmm_field(int n, const double *x, const double *y, const double *q,
                 int cc = h_find(t, ch[i]);
                 if (cc < 0) continue;
                 orc_cell *d = &t->cells[cc];
-                orc_m2m(&ak[(size_t)cc * W], d->cx - c->cx, d->cy - c->cy, P,
+                orc_m2m(&ak[(size_t)cc * W], d->cx - c->cx, d->cy - c->cy, P, o->binom_mode,
                         &ak[(size_t)(c - t->cells) * W]);
             }
         }
@@ -739,13 +748,15 @@ int orc_fmm_field(int n, const double *x, const double *y, const double *q,
                 int cc = h_find(t, ch[i]);
                 if (cc < 0) continue;
                 orc_cell *d = &t->cells[cc];
-                orc_l2l(&bk[(size_t)(c - t->cells) * W], d->cx - c->cx, d->cy - c->cy, P,
+                orc_l2l(&bk[(size_t)(c - t->cells) * W], d->cx - c->cx, d->cy - c->cy, P, o->binom_mode,
                         &bk[(size_t)cc * W]);
             }
         }
     }
 
-    /* evaluation per leaf (fmm2D.cpp:940-1106) */
+    /* evaluation per leaf (fmm2D.cpp:940-1106); stage_mask selects the parts */
+    const int mask = o->stage_mask ? o->stage_mask : 7;
+    const int do_p2p = mask & 1, do_m2p = mask & 2, do_l2p = mask & 4;
     for (int i = 0; i < n; ++i) { ex[i] = 0.0; ey[i] = 0.0; }
     idvec l1 = {0}, l3 = {0};
     for (int li = 0; li < t->n_leaves; ++li) {
@@ -753,7 +764,7 @@ int orc_fmm_field(int n, const double *x, const double *y, const double *q,
         l1.n = 0; l3.n = 0;
         int_list(t, c, &l1, &l3);
         /* P2P over every particle of the list-1 cells, skipping j == i (:135-179) */
-        for (int a = 0; a < l1.n; ++a) {
+        for (int a = 0; a < (do_p2p ? l1.n : 0); ++a) {
             orc_cell *s = &t->cells[h_find(t, l1.v[a])];
             for (int jj = 0; jj < s->pcnt; ++jj) {
                 int j = t->plist[s->poff + jj];
@@ -769,21 +780,44 @@ int orc_fmm_field(int n, const double *x, const double *y, const double *q,
         }
         for (int ii = 0; ii < c->pcnt; ++ii) {
             int i = t->plist[c->poff + ii];
-            for (int a = 0; a < l3.n; ++a) {       /* M2P over list 3 */
+            for (int a = 0; a < (do_m2p ? l3.n : 0); ++a) {       /* M2P over list 3 */
                 int sidx = h_find(t, l3.v[a]);
                 orc_cell *s = &t->cells[sidx];
                 orc_m2p(&ak[(size_t)sidx * W], x[i] - s->cx, y[i] - s->cy, P, &ex[i], &ey[i]);
             }
-            orc_l2p(&bk[(size_t)(c - t->cells) * W], x[i] - c->cx, y[i] - c->cy, P,
-                    &ex[i], &ey[i]);
+            if (do_l2p)
+                orc_l2p(&bk[(size_t)(c - t->cells) * W], x[i] - c->cx, y[i] - c->cy, P,
+                        &ex[i], &ey[i]);
         }
     }
     free(l1.v); free(l3.v);
+    if (ids)
+        for (int k = 0; k < t->n_cells; ++k) {
+            ids[k] = t->cells[ord[k]].id;
+            memcpy(&ak_out[(size_t)k * W], &ak[(size_t)ord[k] * W], sizeof(double) * W);
+            memcpy(&bk_out[(size_t)k * W], &bk[(size_t)ord[k] * W], sizeof(double) * W);
+        }
     free(sx); free(sy); free(sq);
     free(ord); free(lstart); free(ak); free(bk);
     orc_tree_free(t);
     (void)cmp_i64;
     return ORC_OK;
+}
+
+int orc_fmm_field(int n, const double *x, const double *y, const double *q,
+                  const uint8_t *active, const orc_opts *o, double *ex, double *ey) {
+    return fmm_run(n, x, y, q, active, o, ex, ey, NULL, NULL, NULL);
+}
+
+int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
+                       const uint8_t *active, const orc_opts *o, int64_t *ids, double *ak,
+                       double *bk) {
+    double *ex = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *ey = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (!ex || !ey) { free(ex); free(ey); return ORC_NOMEM; }
+    int st = fmm_run(n, x, y, q, active, o, ex, ey, ids, ak, bk);
+    free(ex); free(ey);
+    return st;
 }
 
 /* velocity_calc.cpp:465-481 and 572-575, plus get_velocity's free stream

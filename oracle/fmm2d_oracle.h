@@ -45,9 +45,11 @@ typedef struct {
     double exp_tree;   /* root expansion in percent (Pars::expTree = 1.0)           */
     int    min_level;  /* cells above this level are always split (reference: 1)    */
     int    max_level;  /* cells at this level are always leaves (0: no cap)         */
-    int    binom_mode; /* 0: exact binomials in M2L, 1: the reference int32 helper  */
+    int    binom_mode; /* 0: exact binomials, 1: the reference int32 helper (M2M,
+                          M2L and L2L all call it)                                  */
     int    list4_mode; /* 1: M2L from the list-4 leaf multipole (reference),
                           2: P2L from the list-4 leaf's active particles            */
+    int    stage_mask; /* evaluation parts: 1 P2P, 2 M2P (list 3), 4 L2P; 0 = all  */
 } orc_opts;
 
 /* Status codes (shared numbering with the product C-ABI). */
@@ -95,6 +97,13 @@ int  orc_fmm_field(int n, const double *x, const double *y, const double *q,
                    const uint8_t *active, const orc_opts *o,
                    double *ex, double *ey);
 
+/* The same run, returning every cell's multipole (ak) and local (bk)
+ * coefficients, cells sorted by ID: ids[n_cells], ak/bk[n_cells * 2(p+1)]
+ * (re, im interleaved).  b_0 is not formed (it never reaches the field). */
+int  orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
+                        const uint8_t *active, const orc_opts *o, int64_t *ids,
+                        double *ak, double *bk);
+
 /* Velocity as VelocityCalc::get_velocity with the FMM branch:
  * u = Ey + u_inf, v = -Ex + v_inf, q = -omega s^2/(2 pi). */
 int  orc_velocity(int n, const double *x, const double *y, const double *s,
@@ -116,10 +125,12 @@ double orc_binom_exact(int n, int k);
 /* Stage-level entry points for the series-identity tests. */
 void orc_p2m(int np, const double *x, const double *y, const double *q,
              double cx, double cy, int p, double *ak /* 2(p+1) re,im */);
-void orc_m2m(const double *ak_child, double dx, double dy, int p, double *ak_parent_acc);
+void orc_m2m(const double *ak_child, double dx, double dy, int p, int binom_mode,
+             double *ak_parent_acc);
 void orc_m2l(const double *ak_src, double dx, double dy, int p, int binom_mode,
              double *bk_acc /* d = z_src - z_tgt */);
-void orc_l2l(const double *bk_parent, double dx, double dy, int p, double *bk_child_acc);
+void orc_l2l(const double *bk_parent, double dx, double dy, int p, int binom_mode,
+             double *bk_child_acc);
 void orc_m2p(const double *ak, double dx, double dy, int p, double *ex, double *ey);
 void orc_l2p(const double *bk, double dx, double dy, int p, double *ex, double *ey);
 

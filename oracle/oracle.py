@@ -39,6 +39,7 @@ class OrcOpts(C.Structure):
         ("max_level", C.c_int),
         ("binom_mode", C.c_int),
         ("list4_mode", C.c_int),
+        ("stage_mask", C.c_int),
     ]
 
 
@@ -84,6 +85,8 @@ def lib():
         L.orc_fmm_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.POINTER(OrcOpts), _dp, _dp]
         L.orc_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _up, C.POINTER(OrcOpts),
                                    C.c_double, C.c_double, _dp, _dp]
+        L.orc_fmm_expansions.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.POINTER(OrcOpts),
+                                         _lp, _dp, _dp]
         L.orc_direct_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.c_int, _ip, _dp, _dp]
         L.orc_direct_field.restype = None
         L.orc_binom_ref.argtypes = [C.c_int, C.c_int]
@@ -92,9 +95,9 @@ def lib():
         for name in ("orc_p2m",):
             getattr(L, name).argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
                                          C.c_int, _dp]
-        L.orc_m2m.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_m2m.argtypes = [_dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         L.orc_m2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
-        L.orc_l2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_l2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         L.orc_m2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_l2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
         for name in ("orc_p2m", "orc_m2m", "orc_m2l", "orc_l2l", "orc_m2p", "orc_l2p",
@@ -136,9 +139,9 @@ def ref():
 # options
 # --------------------------------------------------------------------------
 def opts(p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
-         binom_mode=1, list4_mode=1) -> OrcOpts:
+         binom_mode=1, list4_mode=1, stage_mask=0) -> OrcOpts:
     return OrcOpts(p, max_par, max_src, exp_tree, min_level, max_level, binom_mode,
-                   list4_mode)
+                   list4_mode, stage_mask)
 
 
 class OracleError(RuntimeError):
@@ -231,6 +234,22 @@ def fmm_field(x, y, q, active, o: OrcOpts | None = None):
     _check(L.orc_fmm_field(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
                            C.byref(o), _ptr(ex, _dp), _ptr(ey, _dp)), "orc_fmm_field")
     return ex, ey
+
+
+def fmm_expansions(x, y, q, active, o: OrcOpts | None = None):
+    """(ids sorted, ak, bk) with ak/bk of shape (n_cells, p+1) complex."""
+    L = lib()
+    o = o or opts()
+    x, y, q, a = _d(x), _d(y), _d(q), _u8(active)
+    nc = Tree(x, y, a, o).counts()[0]
+    ids = np.zeros(nc, np.int64)
+    ak = np.zeros((nc, o.p + 1), np.complex128)
+    bk = np.zeros((nc, o.p + 1), np.complex128)
+    _check(L.orc_fmm_expansions(len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp),
+                                _ptr(a, _up), C.byref(o), _ptr(ids, _lp),
+                                ak.ctypes.data_as(_dp), bk.ctypes.data_as(_dp)),
+           "orc_fmm_expansions")
+    return ids, ak, bk
 
 
 def velocity(x, y, s, omega, active, o: OrcOpts | None = None, u_inf=1.0, v_inf=0.0):
