@@ -69,6 +69,7 @@ typedef struct vpmfmm_stats {
     int32_t min_leaf_level, max_leaf_level;
     int64_t n_list1, n_list2, n_list3, n_list4;  /* list entries summed over cells   */
     int64_t p2p_pairs;                             /* ordered (target, source) pairs  */
+    int64_t m2p_evals;                             /* (particle, list-3 cell) pairs   */
     double ms_total;                               /* stage times (timing = 1)        */
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_eval;
 } vpmfmm_stats;

@@ -44,7 +44,8 @@ class Stats(C.Structure):
         ("n", C.c_int32), ("n_active", C.c_int32), ("n_cells", C.c_int32),
         ("n_leaves", C.c_int32), ("min_leaf_level", C.c_int32), ("max_leaf_level", C.c_int32),
         ("n_list1", C.c_int64), ("n_list2", C.c_int64), ("n_list3", C.c_int64),
-        ("n_list4", C.c_int64), ("p2p_pairs", C.c_int64), ("ms_total", C.c_double),
+        ("n_list4", C.c_int64), ("p2p_pairs", C.c_int64), ("m2p_evals", C.c_int64),
+        ("ms_total", C.c_double),
         ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_upward", C.c_double),
         ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_eval", C.c_double),
     ]

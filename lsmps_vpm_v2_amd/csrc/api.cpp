@@ -201,6 +201,7 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
     s.n_list2 = c.hdr_h->sum2;
     s.n_list4 = c.hdr_h->sum4;
     s.p2p_pairs = (int64_t)c.hdr_h->pairs;
+    s.m2p_evals = (int64_t)c.hdr_h->m2p;
     if (h->o.timing) {
         s.ms_tree = span(h, 0, 1);
         s.ms_lists = span(h, 1, 2);

@@ -25,6 +25,7 @@ struct DevHdr {
     double pivot[2];
     double mn[2], mx[2];
     unsigned long long pairs;    // P2P (target, source) pairs evaluated
+    unsigned long long m2p;      // (target, list-3 cell) evaluations
     uint32_t sum2, sum4;         // list-2 / list-4 entries over all cells
     unsigned flags;
     uint32_t n_active;

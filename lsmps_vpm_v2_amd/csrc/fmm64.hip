@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
              const uint32_t *off3, const int *l3, const double *xs, const double *ys,
              const double *qs, const uint32_t *perm, const double2 *ak, const double2 *bk,
              int velocity, double u_inf, double v_inf, int mask, double *out_a, double *out_b,
-             unsigned *flags, unsigned long long *pairs) {
+             unsigned *flags, unsigned long long *pairs, unsigned long long *m2p) {
     const int lane = threadIdx.x & (kWave - 1);
     const int t = blockIdx.x * kEvalWaves + (threadIdx.x / kWave);
     if (t >= nl) return;
@@ -181,6 +181,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             self |= l1[u] == j;
         }
         atomicAdd(pairs, ns * (unsigned long long)nt - (self ? (unsigned long long)nt : 0ull));
+        atomicAdd(m2p, (unsigned long long)(off3[t + 1] - off3[t]) * (unsigned long long)nt);
     }
     const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
     const int LG = kWave / G;
@@ -337,7 +338,7 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
         (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs,
         c.perm_sorted, ak, bk,
         velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b, &c.hdr_d->flags,
-        &c.hdr_d->pairs);
+        &c.hdr_d->pairs, &c.hdr_d->m2p);
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }
