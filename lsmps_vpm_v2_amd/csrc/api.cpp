@@ -47,12 +47,14 @@ int ref_binom_int32(int n, int k) {
     return cs;
 }
 
+// Exact in 128-bit integers (C(n-k+i, i) is an integer after every step),
+// rounded once to double.
 double exact_binom(int n, int k) {
     if (k < 0 || k > n) return 0.0;
     if (k > n - k) k = n - k;
-    double c = 1.0;
-    for (int i = 1; i <= k; ++i) c = c * (double)(n - k + i) / (double)i;
-    return std::nearbyint(c);
+    unsigned __int128 c = 1;
+    for (int i = 1; i <= k; ++i) c = c * (unsigned)(n - k + i) / (unsigned)i;
+    return (double)c;
 }
 
 // Row-major (P+1) x (P+1) tables:

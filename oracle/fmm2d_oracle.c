@@ -84,12 +84,14 @@ int orc_binom_ref(int n, int k) {
     return cs;
 }
 
+/* Exact in 128-bit integers (C(n-k+i, i) is an integer at every step),
+ * rounded once to double; exact for every n <= 127. */
 double orc_binom_exact(int n, int k) {
     if (k < 0 || k > n) return 0.0;
     if (k > n - k) k = n - k;
-    double c = 1.0;
-    for (int i = 1; i <= k; ++i) c = c * (double)(n - k + i) / (double)i;
-    return nearbyint(c);
+    unsigned __int128 c = 1;
+    for (int i = 1; i <= k; ++i) c = c * (unsigned)(n - k + i) / (unsigned)i;
+    return (double)c;
 }
 
 /* ------------------------------------------------------------------------ */

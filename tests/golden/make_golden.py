@@ -1,0 +1,106 @@
+"""Write the golden fixtures of tests/golden from the reference build.
+
+Run in a container that has the reference sources (for oracle/_ref) :
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Every output is data: inputs and the reference FMM's outputs (fields, tree,
+lists, binomial helper values), computed by the reference's own TreeCell and
+fmm2D sources compiled into oracle/_ref/libref_fmm2d.so.  The resume-state
+fixture is the reference's input file input/resume_data/particle_state_00500.csv
+converted to npz (same columns, same values).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from lsmps_vpm_v2_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+RESUME_CSV = "/root/reference/input/resume_data/particle_state_00500.csv"
+
+
+def tree_dump(f, p=10, max_par=75, max_src=50):
+    t = O.RefTree(f["x"], f["y"], f["active"], p=p, max_par=max_par, max_src=max_src)
+    nc, nl, lo, hi = t.counts()
+    cells = t.cells()
+    lid, poff, pidx = t.leaves()
+    R, piv = t.geometry()
+    l_off = {k: [0] for k in ("l1", "l2", "l3", "l4")}
+    l_val = {k: [] for k in l_off}
+    for cid, lev, leaf in zip(cells["id"], cells["level"], cells["is_leaf"]):
+        a1, a3 = t.int_list(cid) if leaf else (np.zeros(0, np.int64), np.zeros(0, np.int64))
+        a2, a4 = t.ext_list(cid) if lev >= 2 else (np.zeros(0, np.int64), np.zeros(0, np.int64))
+        for k, a in (("l1", a1), ("l2", a2), ("l3", a3), ("l4", a4)):
+            l_val[k].extend(a.tolist())
+            l_off[k].append(len(l_val[k]))
+    out = dict(counts=np.array([nc, nl, lo, hi], np.int64), root_len=np.array(R),
+               pivot=piv, leaf_id=lid, leaf_poff=poff, leaf_pidx=pidx)
+    out.update({"cell_" + k: v for k, v in cells.items()})
+    for k in l_off:
+        out[k + "_off"] = np.array(l_off[k], np.int64)
+        out[k] = np.array(l_val[k], np.int64)
+    return out
+
+
+def fields(f, ps):
+    out = {}
+    for p in ps:
+        ex, ey, _ = O.ref_fmm_field(f["x"], f["y"], f["q"], f["active"], p=p)
+        out[f"ex_p{p}"] = ex
+        out[f"ey_p{p}"] = ey
+    return out
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libref_fmm2d.so is missing: run `make -C oracle ref` first")
+
+    # the reference's binomial helper (fmm2D.cpp:28-58), int32 semantics
+    nk = [(n, k) for n in range(0, 64) for k in range(0, n + 1)]
+    save("binom_ref.npz", n=np.array([a for a, _ in nk], np.int32),
+         k=np.array([b for _, b in nk], np.int32),
+         value=np.array([O.ref_binom(a, b) for a, b in nk], np.int64))
+
+    cases = {
+        "uniform_1k": dict(x=np.random.Generator(np.random.PCG64(101)).uniform(-1, 1, 1000),
+                           y=np.random.Generator(np.random.PCG64(102)).uniform(-1, 1, 1000),
+                           q=np.random.Generator(np.random.PCG64(103)).normal(0, 1, 1000),
+                           active=np.ones(1000, np.uint8)),
+        "clusters_5k": W.random_clusters(5000, seed=104, active_frac=0.8),
+    }
+    for name, f in cases.items():
+        save(f"ref_{name}.npz", x=f["x"], y=f["y"], q=f["q"], active=f["active"],
+             **tree_dump(f), **fields(f, (8, 10, 12, 16, 20)))
+
+    # the reference's resume state (a real multiresolution field with velocities)
+    if os.path.exists(RESUME_CSV):
+        d = np.genfromtxt(RESUME_CSV, delimiter=",", names=True)
+        f = dict(x=d["xp"], y=d["yp"], s=d["sp"], omega=d["vor"],
+                 active=d["active"].astype(np.uint8))
+        f["q"] = -f["omega"] * f["s"] ** 2 / (2 * np.pi)
+        save("resume_state_00500.npz", **{c: d[c] for c in d.dtype.names})
+        save("ref_resume_00500.npz", **fields(f, (10, 12)))
+
+    # config 1 (99,856 particles): reference field at p = 10 on every 97th particle
+    f = W.uniform_vortices()
+    ex, ey, _ = O.ref_fmm_field(f["x"], f["y"], f["q"], f["active"], p=10)
+    idx = np.arange(0, f["n"], 97)
+    save("ref_uniform_vortices_100k.npz", idx=idx, ex=ex[idx], ey=ey[idx],
+         ex_norm=np.array(np.linalg.norm(ex)), ey_norm=np.array(np.linalg.norm(ey)),
+         x_sum=np.array(f["x"].sum()), q_sum=np.array(f["q"].sum()))
+
+
+if __name__ == "__main__":
+    main()
