@@ -149,6 +149,13 @@ int vpmfmm_tree_lists(vpmfmm_handle *h, int64_t id, int64_t *l1, int32_t *n1, in
                       int32_t *n2, int64_t *l3, int32_t *n3, int64_t *l4, int32_t *n4,
                       int32_t cap);
 
+/* Expansion order per tree level for the mixed-precision path: p[L] for
+ * levels 0..nlev-1 (0 or a missing entry: opts.p), each in [1, 16]; nlev = 0
+ * clears the table.  Translations between levels of different order truncate
+ * or zero-pad.  No counterpart in the reference, which has one global P_max
+ * (global.cpp:311); used by the learned order selection. */
+int vpmfmm_set_level_orders(vpmfmm_handle *h, const int32_t *p, int32_t nlev);
+
 /* Multipole (ak) and local (bk) coefficients of every cell after the last
  * FP64 field/velocity call, cells sorted by ID: ids[n_cells],
  * ak/bk[n_cells * 2(p+1)] as (re, im) pairs.  b_0 is not formed (it never

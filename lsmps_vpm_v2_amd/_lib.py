@@ -17,7 +17,7 @@ EXPORTS = (
     "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_direct_device",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
-    "vpmfmm_tree_expansions",
+    "vpmfmm_tree_expansions", "vpmfmm_set_level_orders",
 )
 
 STATUS = {
@@ -104,6 +104,7 @@ def load() -> C.CDLL:
     L.vpmfmm_tree_lists.argtypes = [_vp, C.c_int64, _lp, _ip, _lp, _ip, _lp, _ip, _lp, _ip,
                                     C.c_int32]
     L.vpmfmm_tree_expansions.argtypes = [_vp, _lp, _dp, _dp]
+    L.vpmfmm_set_level_orders.argtypes = [_vp, _ip, C.c_int32]
     _lib = L
     return L
 

@@ -8,6 +8,7 @@
 
 #include "../../include/vpmfmm.h"
 #include "ctx.hpp"
+#include "fmm32.hpp"
 
 namespace vpm {
 
@@ -96,6 +97,8 @@ struct vpmfmm_handle_s {
     vpmfmm_stats st{};
     int tab_p = -1, tab_binom = -1;
     int exp_p = -1;  // order of the expansions held on the device (-1: none)
+    std::vector<int> level_p;  // per-level orders of the mixed path (empty: opts.p everywhere)
+    int mx_pmax = -1;          // order the mixed tables were built for
     hipEvent_t ev[8] = {};
     // inspection cache of the last tree (host copies)
     bool insp = false;
@@ -221,16 +224,100 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
         throw Error(kDuplicate, "two particles of a list-1 pair share a position (r = 0)");
 }
 
+// Mixed path set-up: per-level orders and sides, level-independent tables
+// (rebuilt only when the largest order changes) and the 16-cell M2L tiles.
+Mixed32 mixed_setup(vpmfmm_handle *h) {
+    Ctx &c = h->ctx;
+    const TreeInfo &ti = c.info;
+    LevelTab lt{};
+    int pmax = 0;
+    for (int L = 0; L < kMaxDepth + 2; ++L) {
+        int p = h->o.p;
+        if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
+        if (p > 16) throw Error(kBadArg, "the mixed path supports expansion orders up to 16");
+        lt.p[L] = p;
+        const double s = ti.root_len / std::ldexp(1.0, L);
+        lt.s[L] = (float)s;
+        lt.inv_s[L] = (float)(1.0 / s);
+        if (L <= ti.max_level) pmax = std::max(pmax, p);
+    }
+    Mixed32 mx;
+    if (h->mx_pmax != pmax) {
+        std::vector<float> m2m, l2l, tm2l;
+        std::vector<std::vector<float>> rl;
+        mixed_tables(pmax, m2m, l2l, tm2l, rl);
+        auto up = [&](DBuf &b, const std::vector<float> &v) {
+            VPM_HIP(hipMemcpyAsync(b.get(v.size() * 4 + 16), v.data(), v.size() * 4,
+                                   hipMemcpyHostToDevice, c.stream));
+        };
+        up(c.mx_m2m, m2m);
+        up(c.mx_l2l, l2l);
+        up(c.mx_tm2l, tm2l);
+        size_t tot = 0;
+        for (auto &r : rl) tot += r.size();
+        float *rd = c.mx_rdata.as<float>(tot + 1);
+        std::vector<const float *> ptrs(rl.size(), nullptr);
+        size_t off = 0;
+        for (size_t p = 0; p < rl.size(); ++p) {
+            ptrs[p] = rd + off;
+            if (!rl[p].empty())
+                VPM_HIP(hipMemcpyAsync(rd + off, rl[p].data(), rl[p].size() * 4,
+                                       hipMemcpyHostToDevice, c.stream));
+            off += rl[p].size();
+        }
+        VPM_HIP(hipMemcpyAsync(c.mx_rptr.get(ptrs.size() * sizeof(void *)), ptrs.data(),
+                               ptrs.size() * sizeof(void *), hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        h->mx_pmax = pmax;
+    }
+    // tiles of 16 consecutive cells per level (levels >= 2), grouped by the
+    // number of 16-row MFMA blocks the level's order needs
+    std::vector<int2> t1, t2;
+    for (int L = 2; L <= ti.max_level; ++L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        std::vector<int2> &dst = (2 * lt.p[L] <= 16) ? t1 : t2;
+        for (int j = j0; j < j1; j += 16) dst.push_back(make_int2(j, std::min(16, j1 - j)));
+    }
+    int2 *td = c.mx_tiles.as<int2>(t1.size() + t2.size() + 1);
+    if (!t1.empty())
+        VPM_HIP(hipMemcpyAsync(td, t1.data(), t1.size() * sizeof(int2), hipMemcpyHostToDevice,
+                               c.stream));
+    if (!t2.empty())
+        VPM_HIP(hipMemcpyAsync(td + t1.size(), t2.data(), t2.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, c.stream));
+    LevelTab *ltd = c.mx_lt.as<LevelTab>(1);
+    VPM_HIP(hipMemcpyAsync(ltd, &lt, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
+    // the host vectors above must outlive the copies
+    VPM_HIP(hipStreamSynchronize(c.stream));
+    mx.pmax = pmax;
+    mx.lt = ltd;
+    mx.m2m = (const float *)c.mx_m2m.p;
+    mx.l2l = (const float *)c.mx_l2l.p;
+    mx.tm2l = (const float *)c.mx_tm2l.p;
+    mx.rtab = (const float *const *)c.mx_rptr.p;
+    mx.tiles1 = td;
+    mx.ntiles1 = (int)t1.size();
+    mx.tiles2 = td + t1.size();
+    mx.ntiles2 = (int)t2.size();
+    return mx;
+}
+
 void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
          const uint8_t *a, double *oa, double *ob, bool velocity) {
     if (!q || !oa || !ob) throw Error(kBadArg, "null pointer");
     build(h, n, x, y, q, a);
-    ensure_tables(h);
     Ctx &c = h->ctx;
     FmmMarks mk{h->o.timing ? h->ev : nullptr};
-    fmm64_run(c, h->o.p, h->o.list4_mode, h->o.stage_mask ? h->o.stage_mask : 7, oa, ob,
-              velocity, h->o.u_inf, h->o.v_inf, mk);
-    h->exp_p = h->o.p;
+    const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
+    if (h->o.precision == VPMFMM_MIXED) {
+        Mixed32 mx = mixed_setup(h);
+        fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
+    } else {
+        ensure_tables(h);
+        fmm64_run(c, h->o.p, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf,
+                  mk);
+        h->exp_p = h->o.p;
+    }
     finish_stats(h, true);
 }
 
@@ -596,6 +683,17 @@ int vpmfmm_tree_lists(vpmfmm_handle *h, int64_t id, int64_t *l1, int32_t *n1, in
         const bool m2l = h->h_level[j] >= 2;
         put(h->h_l2.data() + 27 * (size_t)j, m2l ? h->h_n2[j] : 0, l2, n2);
         put(h->h_l4.data() + 8 * (size_t)j, m2l ? h->h_n4[j] : 0, l4, n4);
+    });
+}
+
+int vpmfmm_set_level_orders(vpmfmm_handle *h, const int32_t *p, int32_t nlev) {
+    return guarded(h, [&] {
+        if (nlev < 0 || nlev > kMaxDepth + 1 || (nlev > 0 && !p))
+            throw Error(kBadArg, "nlev must be in [0, 17]");
+        std::vector<int> v(p ? p : nullptr, p ? p + nlev : nullptr);
+        for (int x : v)
+            if (x < 0 || x > 16) throw Error(kBadArg, "per-level orders must be in [0, 16]");
+        h->level_p = v;
     });
 }
 

@@ -95,6 +95,10 @@ struct Ctx {
     // expansions
     DBuf ak, bk, tab_m2l, tab_m2m, tab_l2l;
 
+    // mixed path: FP32 particle offsets and sources, tables, tiles
+    DBuf oxy, qf;
+    DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_rptr, mx_tiles;
+
     TreeInfo info;
     std::vector<double> stage_ms;
 };

@@ -1,0 +1,555 @@
+// fmm32.hip -- mixed-precision FMM stages on gfx950.
+//
+// Precision split: cell geometry, centre differences and particle-minus-centre
+// displacements are formed in FP64 (tree.hip / k_prep32) and rounded once;
+// everything else is FP32 with FP32 accumulation.
+//
+// Scaled coefficients.  With s_L the side of a level-L cell,
+//   a^_k = a_k / s^k,   b^_m = b_m s^m,
+// every translation becomes level independent:
+//   M2M  a^par_m = sum_k C(m,k) e^(m-k) 2^-m a^ch_k     (e = (zch - zpar)/s_ch)
+//   M2L  b^_m    = sum_k T_mk delta^-(m+k) a^_k         (delta = integer offset)
+//   L2L  b^ch_m  = sum_k C(k,m) e^(k-m) 2^-k b^par_k
+// with T_m0 = -1/m, T_mk = -C(m+k-1,k-1)(-1)^k/k (the reference's M2L,
+// fmm2D.cpp:465-481, with exact binomials).  The scaled magnitudes stay O(1)
+// (|(z - zc)/s| <= 1/sqrt(2), |delta| >= 2), so FP32 neither under- nor
+// overflows at any level.
+//
+// M2L over list 2 (same-level cells at integer offsets in [-3,3]^2 \ [-1,1]^2,
+// 40 offsets, at most 27 per cell) runs on MFMA: one wavefront owns 16 target
+// cells of one level; for each offset it multiplies the realified 2p x 2(p+1)
+// matrix of that offset with the 16 sources' scaled coefficient vectors
+// (v_mfma_f32_16x16x4f32, K = 4 per instruction, FP32 accumulate).  Offsets a
+// target does not use contribute zero columns.  List 4 (coarser leaves) has
+// irregular offsets and is done on the vector ALUs.
+#include "ctx.hpp"
+#include "fmm32.hpp"
+
+namespace vpm {
+namespace {
+
+constexpr unsigned kLeaf = 1u, kActive = 2u;
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct cf {
+    float x, y;
+};
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf cscale(cf a, float s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cf cinv(cf a) {
+    const float r = __builtin_amdgcn_rcpf(a.x * a.x + a.y * a.y);
+    return {a.x * r, -a.y * r};
+}
+__device__ __forceinline__ cf cpowi(cf z, int p) {
+    cf r = {1.f, 0.f};
+    while (p > 0) {
+        if (p & 1) r = cmul(r, z);
+        z = cmul(z, z);
+        p >>= 1;
+    }
+    return r;
+}
+__device__ __forceinline__ cf ldc(const float *a, size_t i) {
+    const float2 v = reinterpret_cast<const float2 *>(a)[i];
+    return {v.x, v.y};
+}
+__device__ __forceinline__ void stc(float *a, size_t i, cf v) {
+    reinterpret_cast<float2 *>(a)[i] = make_float2(v.x, v.y);
+}
+__device__ __forceinline__ uint32_t compact1by1(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// particle preparation: FP32 offsets from the leaf centre (FP64 difference)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_prep32(const uint32_t *leaf_idx, int nl, const uint32_t *c_begin, const uint32_t *c_end,
+             const double *c_cx, const double *c_cy, const double *xs, const double *ys,
+             const double *qs, float2 *oxy, float *qf) {
+    const int t = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= nl) return;
+    const int j = (int)leaf_idx[t];
+    const double cx = c_cx[j], cy = c_cy[j];
+    for (uint32_t i = c_begin[j] + lane; i < c_end[j]; i += kWave) {
+        oxy[i] = make_float2((float)(xs[i] - cx), (float)(ys[i] - cy));
+        qf[i] = (float)qs[i];
+    }
+}
+
+// P2M: thread per (leaf, k); a^_k = sum q ((z - zc)/s)^k over active particles.
+__global__ void __launch_bounds__(256)
+    k_p2m32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
+            const uint32_t *c_begin, const uint32_t *c_end, const float2 *oxy, const float *qf,
+            const uint8_t *as, const LevelTab *lt, int kst, float *ahat) {
+    const int PM1 = kst / 2;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nl * PM1) return;
+    const int t = (int)(idx / PM1), k = (int)(idx % PM1);
+    const int j = (int)leaf_idx[t];
+    const int L = c_level[j];
+    if (k > lt->p[L] || !(c_flags[j] & kActive)) return;
+    const float inv_s = lt->inv_s[L];
+    cf acc = {0.f, 0.f};
+    for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
+        if (!as[i]) continue;
+        const float2 o = oxy[i];
+        acc = cadd(acc, cscale(cpowi(cf{o.x * inv_s, o.y * inv_s}, k), qf[i]));
+    }
+    stc(ahat, (size_t)j * PM1 + k, acc);
+}
+
+// M2M for one level: thread per (parent, m).  m2m[q][m][k] complex, q = child
+// quadrant, stride PM1 x PM1.
+__global__ void __launch_bounds__(256)
+    k_m2m32(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
+            const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    const int PM1 = kst / 2;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)(j1 - j0) * PM1) return;
+    const int j = j0 + (int)(idx / PM1), m = (int)(idx % PM1);
+    const unsigned f = c_flags[j];
+    const int L = c_level[j];
+    if ((f & kLeaf) || !(f & kActive) || m > lt->p[L]) return;
+    const int pc = lt->p[L + 1];
+    cf acc = {0.f, 0.f};
+    for (int q = 0; q < 4; ++q) {
+        const int ch = c_child[4 * j + q];
+        if (ch < 0) continue;
+        const float *row = m2m + ((size_t)q * PM1 + m) * PM1 * 2;
+        const int kmax = min(m, pc);
+        for (int k = 0; k <= kmax; ++k)
+            acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(ahat, (size_t)ch * PM1 + k)));
+    }
+    stc(ahat, (size_t)j * PM1 + m, acc);
+}
+
+// M2L over list 2 on MFMA.  Tile = 16 consecutive cells of one level.
+// Lane l: column n = l % 16 (target), k-quarter kq = l / 16.  A operand
+// (translation matrix) is read from a table laid out in lane order.
+template <int MT>
+__global__ void __launch_bounds__(256)
+    k_m2l_mfma(const int2 *tiles, int ntiles, const uint8_t *c_level, const uint32_t *c_key,
+               const int *l2, const uint8_t *n2, const LevelTab *lt, const float *const *rtab,
+               const float *ahat, int kst, float *bhat) {
+    __shared__ int smap[4][16][49];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int tile = blockIdx.x * 4 + w;
+    const bool live = tile < ntiles;
+    const int2 td = live ? tiles[tile] : make_int2(0, 0);
+    const int j0 = td.x, cnt = td.y;
+    const int n = lane % 16, kq = lane / 16;
+    const int j = j0 + n;
+    const bool tvalid = live && n < cnt;
+    // source map of this tile: slot (dx+3) + 7 (dy+3) -> source cell or -1
+    for (int s = kq; s < 49; s += 4) smap[w][n][s] = -1;
+    __syncthreads();
+    if (tvalid) {
+        const uint32_t kj = c_key[j];
+        const int jx = (int)compact1by1(kj), jy = (int)compact1by1(kj >> 1);
+        const int m2 = n2[j];
+        for (int u = kq; u < m2; u += 4) {
+            const int sidx = l2[27 * j + u];
+            const uint32_t ks = c_key[sidx];
+            const int dx = (int)compact1by1(ks) - jx, dy = (int)compact1by1(ks >> 1) - jy;
+            smap[w][n][(dx + 3) + 7 * (dy + 3)] = sidx;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    const int L = c_level[j0];
+    const int p = lt->p[L];
+    const int K = 2 * (p + 1);
+    const int KS = (K + 3) / 4;
+    const float *R = rtab[p];
+    v4f acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+    int o = 0;
+    for (int slot = 0; slot < 49; ++slot) {
+        const int dx = slot % 7 - 3, dy = slot / 7 - 3;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
+        const int sidx = smap[w][n][slot];
+        const float *src = ahat + (size_t)(sidx < 0 ? 0 : sidx) * kst;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = ks * 4 + kq;
+            const float b = (sidx >= 0 && k < K) ? src[k] : 0.f;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const float a = R[(((size_t)o * KS + ks) * MT + mt) * kWave + lane];
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[mt], 0, 0, 0);
+            }
+        }
+        ++o;
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = mt * 16 + 4 * kq + v;  // row of (Re b^_1, Im b^_1, Re b^_2, ...)
+            if (n < cnt && row < 2 * p) bhat[(size_t)j * kst + 2 + row] = acc[mt][v];
+        }
+    }
+}
+
+// List 4: P2L (mode 2) from the active particles of coarser leaves, or M2L
+// (mode 1) from their scaled multipoles; thread per (target cell, m).
+__global__ void __launch_bounds__(256)
+    k_l4_32(int j0, int nc, int mode, const uint8_t *c_level, const int *l4, const uint8_t *n4,
+            const double *c_cx, const double *c_cy, const uint32_t *c_begin,
+            const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
+            const LevelTab *lt, const float *tm2l, const float *ahat, int kst, float *bhat) {
+    const int PM1 = kst / 2;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)(nc - j0) * (PM1 - 1)) return;
+    const int j = j0 + (int)(idx / (PM1 - 1)), m = 1 + (int)(idx % (PM1 - 1));
+    const int m4 = n4[j];
+    const int L = c_level[j];
+    if (m4 == 0 || m > lt->p[L]) return;
+    const double cx = c_cx[j], cy = c_cy[j];
+    const float st = lt->s[L];
+    cf acc = {0.f, 0.f};
+    for (int u = 0; u < m4; ++u) {
+        const int s = l4[8 * j + u];
+        if (mode == 2) {
+            // b^_m += -q/m (s_t / (z_j - z_t))^m; z_j - z_t = (c_s - c_t) + o_j
+            const float dcx = (float)(c_cx[s] - cx), dcy = (float)(c_cy[s] - cy);
+            for (uint32_t i = c_begin[s]; i < c_end[s]; ++i) {
+                if (!as[i]) continue;
+                const float2 o = oxy[i];
+                const cf w = cinv(cf{(dcx + o.x) / st, (dcy + o.y) / st});
+                acc = cadd(acc, cscale(cpowi(w, m), -qf[i] / (float)m));
+            }
+        } else {
+            // b^_m += sum_k T_mk a^_k (s_s w)^k (s_t w)^m, w = 1/(z_s - z_t)
+            const int Ls = c_level[s];
+            const int ps = lt->p[Ls];
+            const cf dz = {(float)((c_cx[s] - cx) / st), (float)((c_cy[s] - cy) / st)};
+            const cf wt = cinv(dz);                                  // s_t w
+            const cf ws = cscale(wt, lt->s[Ls] / st);                // s_s w
+            const float *row = tm2l + (size_t)m * PM1;
+            cf sacc = cscale(ldc(ahat, (size_t)s * PM1 + ps), row[ps]);
+            for (int k = ps - 1; k >= 0; --k)
+                sacc = cadd(cmul(sacc, ws), cscale(ldc(ahat, (size_t)s * PM1 + k), row[k]));
+            acc = cadd(acc, cmul(sacc, cpowi(wt, m)));
+        }
+    }
+    const size_t o = (size_t)j * PM1 + m;
+    stc(bhat, o, cadd(ldc(bhat, o), acc));
+}
+
+// L2L for one level (>= 3): thread per (child, m).
+__global__ void __launch_bounds__(256)
+    k_l2l32(int j0, int j1, const uint8_t *c_level, const uint32_t *c_key, const int *c_parent,
+            const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    const int PM1 = kst / 2;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)(j1 - j0) * (PM1 - 1)) return;
+    const int j = j0 + (int)(idx / (PM1 - 1)), m = 1 + (int)(idx % (PM1 - 1));
+    const int L = c_level[j];
+    if (m > lt->p[L]) return;
+    const int par = c_parent[j];
+    const int pp = lt->p[L - 1];
+    const uint32_t key = c_key[j];
+    const int q = (int)(compact1by1(key) & 1u) + 2 * (int)(compact1by1(key >> 1) & 1u);
+    const float *row = l2l + ((size_t)q * PM1 + m) * PM1 * 2;
+    cf acc = {0.f, 0.f};
+    for (int k = m; k <= pp; ++k)
+        acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(bhat, (size_t)par * PM1 + k)));
+    const size_t o = (size_t)j * PM1 + m;
+    stc(bhat, o, cadd(ldc(bhat, o), acc));
+}
+
+// Evaluation: one wavefront per leaf; list-1 sources are staged in LDS as
+// (x - c_t, y - c_t, q, index) with the centre shift done in FP64.
+constexpr int kEvalWaves = 4;
+constexpr int kStage = 256;  // sources staged per wavefront per round
+
+__global__ void __launch_bounds__(kWave *kEvalWaves)
+    k_eval32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint32_t *c_begin,
+             const uint32_t *c_end, const double *c_cx, const double *c_cy,
+             const uint32_t *off1, const int *l1, const uint32_t *off3, const int *l3,
+             const float2 *oxy, const float *qf, const uint32_t *perm, const LevelTab *lt,
+             const float *ahat, const float *bhat, int kst, int mask, int velocity, double u_inf,
+             double v_inf, double *out_a, double *out_b, unsigned long long *pairs,
+             unsigned long long *m2p) {
+    __shared__ float4 stage[kEvalWaves][kStage];
+    const int PM1 = kst / 2;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kEvalWaves + w;
+    if (t >= nl) return;
+    const int j = (int)leaf_idx[t];
+    const uint32_t b = c_begin[j], e = c_end[j];
+    const int nt = (int)(e - b);
+    const double tcx = c_cx[j], tcy = c_cy[j];
+    const uint32_t a1 = off1[t], z1 = off1[t + 1], a3 = off3[t], z3 = off3[t + 1];
+    if (lane == 0) {
+        unsigned long long ns = 0;
+        bool self = false;
+        for (uint32_t u = a1; u < z1; ++u) {
+            ns += c_end[l1[u]] - c_begin[l1[u]];
+            self |= l1[u] == j;
+        }
+        atomicAdd(pairs, ns * (unsigned long long)nt - (self ? (unsigned long long)nt : 0ull));
+        atomicAdd(m2p, (unsigned long long)(z3 - a3) * (unsigned long long)nt);
+    }
+    const int L = c_level[j];
+    const int p = lt->p[L];
+    const float inv_s = lt->inv_s[L];
+    for (uint32_t tb = b; tb < e; tb += kWave) {
+        const uint32_t i = tb + lane;
+        const bool valid = i < e;
+        const float2 ti = valid ? oxy[i] : make_float2(0.f, 0.f);
+        float ex = 0.f, ey = 0.f;
+        // P2P, staged in rounds of kStage sources
+        uint32_t u = a1, pos = a1 < z1 ? c_begin[l1[a1]] : 0u;
+        while ((mask & 1) && u < z1) {
+            int fill = 0;
+            while (u < z1 && fill < kStage) {
+                const int sc = l1[u];
+                const uint32_t se = c_end[sc];
+                const float dcx = (float)(c_cx[sc] - tcx), dcy = (float)(c_cy[sc] - tcy);
+                const int take = (int)min<uint32_t>(se - pos, (uint32_t)(kStage - fill));
+                for (int r = lane; r < take; r += kWave) {
+                    const uint32_t src = pos + r;
+                    const float2 o = oxy[src];
+                    stage[w][fill + r] =
+                        make_float4(o.x + dcx, o.y + dcy, qf[src], __uint_as_float(src));
+                }
+                fill += take;
+                pos += take;
+                if (pos >= se) {
+                    ++u;
+                    if (u < z1) pos = c_begin[l1[u]];
+                }
+            }
+            // lanes read what other lanes of the same wavefront wrote
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int r = 0; r < fill; ++r) {
+                const float4 sv = stage[w][r];
+                const float dx = ti.x - sv.x, dy = ti.y - sv.y;
+                const float r2 = dx * dx + dy * dy;
+                const float inv = (__float_as_uint(sv.w) == i) ? 0.f : __builtin_amdgcn_rcpf(r2);
+                const float qr = sv.z * inv;
+                ex += qr * dx;
+                ey += qr * dy;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        // M2P over list 3: F = (1/s) sum a^_k xi^-(k+1), xi = (z - zc)/s
+        for (uint32_t v = a3; (mask & 2) && v < z3; ++v) {
+            const int sc = l3[v];
+            const int Ls = c_level[sc];
+            const int ps = lt->p[Ls];
+            const float is = lt->inv_s[Ls];
+            const float dcx = (float)(tcx - c_cx[sc]), dcy = (float)(tcy - c_cy[sc]);
+            const cf wv = cinv(cf{(ti.x + dcx) * is, (ti.y + dcy) * is});
+            const size_t base = (size_t)sc * PM1;
+            cf f = ldc(ahat, base + ps);
+            for (int k = ps - 1; k >= 0; --k) f = cadd(cmul(f, wv), ldc(ahat, base + k));
+            f = cscale(cmul(f, wv), is);
+            ex += f.x;
+            ey -= f.y;
+        }
+        if (valid) {
+            if (mask & 4) {
+                // L2P: F = (1/s) sum_m m b^_m zeta^(m-1)
+                const cf z = {ti.x * inv_s, ti.y * inv_s};
+                const size_t base = (size_t)j * PM1;
+                cf f = cscale(ldc(bhat, base + p), (float)p);
+                for (int m = p - 1; m >= 1; --m) f = cadd(cmul(f, z), cscale(ldc(bhat, base + m), (float)m));
+                f = cscale(f, inv_s);
+                ex += f.x;
+                ey -= f.y;
+            }
+            const uint32_t o = perm[i];
+            if (velocity) {
+                out_a[o] = (double)ey + u_inf;
+                out_b[o] = -(double)ex + v_inf;
+            } else {
+                out_a[o] = (double)ex;
+                out_b[o] = (double)ey;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static const int kOffsets[40][2] = {
+#define O(dx, dy) {dx, dy}
+    O(-3, -3), O(-2, -3), O(-1, -3), O(0, -3), O(1, -3), O(2, -3), O(3, -3),
+    O(-3, -2), O(-2, -2), O(-1, -2), O(0, -2), O(1, -2), O(2, -2), O(3, -2),
+    O(-3, -1), O(-2, -1), O(2, -1), O(3, -1),
+    O(-3, 0), O(-2, 0), O(2, 0), O(3, 0),
+    O(-3, 1), O(-2, 1), O(2, 1), O(3, 1),
+    O(-3, 2), O(-2, 2), O(-1, 2), O(0, 2), O(1, 2), O(2, 2), O(3, 2),
+    O(-3, 3), O(-2, 3), O(-1, 3), O(0, 3), O(1, 3), O(2, 3), O(3, 3),
+#undef O
+};
+
+void mixed_tables(int pmax, std::vector<float> &m2m, std::vector<float> &l2l,
+                  std::vector<float> &tm2l, std::vector<std::vector<float>> &rlane) {
+    const int PM1 = pmax + 1;
+    // child centre minus parent centre over the child side: (+-1/2, +-1/2)
+    const double ex[4] = {-0.5, 0.5, -0.5, 0.5}, ey[4] = {-0.5, -0.5, 0.5, 0.5};
+    m2m.assign((size_t)4 * PM1 * PM1 * 2, 0.f);
+    l2l.assign((size_t)4 * PM1 * PM1 * 2, 0.f);
+    for (int q = 0; q < 4; ++q) {
+        for (int m = 0; m <= pmax; ++m) {
+            for (int k = 0; k <= pmax; ++k) {
+                // M2M: C(m,k) e^(m-k) 2^-m, k <= m ; L2L: C(k,m) e^(k-m) 2^-k, k >= m
+                const int d = k <= m ? m - k : k - m;
+                double pr = 1.0, pi = 0.0;
+                for (int t = 0; t < d; ++t) {
+                    const double nr = pr * ex[q] - pi * ey[q], ni = pr * ey[q] + pi * ex[q];
+                    pr = nr;
+                    pi = ni;
+                }
+                const size_t o = (((size_t)q * PM1 + m) * PM1 + k) * 2;
+                if (k <= m) {
+                    const double c = exact_binom(m, k) * std::ldexp(1.0, -m);
+                    m2m[o] = (float)(c * pr);
+                    m2m[o + 1] = (float)(c * pi);
+                }
+                if (k >= m) {
+                    const double c = exact_binom(k, m) * std::ldexp(1.0, -k);
+                    l2l[o] = (float)(c * pr);
+                    l2l[o + 1] = (float)(c * pi);
+                }
+            }
+        }
+    }
+    // T_mk (exact binomials) for the vector list-4 M2L
+    tm2l.assign((size_t)PM1 * PM1, 0.f);
+    for (int m = 1; m <= pmax; ++m) {
+        tm2l[(size_t)m * PM1] = (float)(-1.0 / m);
+        for (int k = 1; k <= pmax; ++k)
+            tm2l[(size_t)m * PM1 + k] =
+                (float)(-(exact_binom(m + k - 1, k - 1) * ((k & 1) ? -1.0 : 1.0) / k));
+    }
+    // realified M2L matrices per offset, in MFMA lane order, one table per p
+    rlane.assign(pmax + 1, {});
+    for (int p = 1; p <= pmax; ++p) {
+        const int K = 2 * (p + 1), KS = (K + 3) / 4, MT = (2 * p + 15) / 16;
+        std::vector<float> &R = rlane[p];
+        R.assign((size_t)40 * KS * MT * kWave, 0.f);
+        for (int o = 0; o < 40; ++o) {
+            const double dx = kOffsets[o][0], dy = kOffsets[o][1];
+            const double r2 = dx * dx + dy * dy;
+            const double wr = dx / r2, wi = -dy / r2;  // 1/delta
+            std::vector<double> pwr(2 * p + 2), pwi(2 * p + 2);
+            pwr[0] = 1.0;
+            pwi[0] = 0.0;
+            for (int t = 1; t <= 2 * p + 1; ++t) {
+                pwr[t] = pwr[t - 1] * wr - pwi[t - 1] * wi;
+                pwi[t] = pwr[t - 1] * wi + pwi[t - 1] * wr;
+            }
+            for (int m = 1; m <= p; ++m) {
+                for (int k = 0; k <= p; ++k) {
+                    const double T = k == 0 ? -1.0 / m
+                                            : -(exact_binom(m + k - 1, k - 1) *
+                                                ((k & 1) ? -1.0 : 1.0) / k);
+                    const double Mr = T * pwr[m + k], Mi = T * pwi[m + k];
+                    // rows 2(m-1) (Re) and 2(m-1)+1 (Im); columns 2k (Re a^), 2k+1 (Im a^)
+                    const int r0 = 2 * (m - 1), r1 = r0 + 1, c0 = 2 * k, c1 = c0 + 1;
+                    const double val[2][2] = {{Mr, -Mi}, {Mi, Mr}};
+                    const int rows[2] = {r0, r1}, cols[2] = {c0, c1};
+                    for (int a = 0; a < 2; ++a)
+                        for (int bq = 0; bq < 2; ++bq) {
+                            const int row = rows[a], col = cols[bq];
+                            const int mt = row / 16, ks = col / 4;
+                            // lane holds A[row % 16][col % 4] with lane = (row%16) + 16 (col%4)
+                            const int lane = (row % 16) + 16 * (col % 4);
+                            R[(((size_t)o * KS + ks) * MT + mt) * kWave + lane] = (float)val[a][bq];
+                        }
+                }
+            }
+        }
+    }
+}
+
+void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
+               double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
+    hipStream_t s = c.stream;
+    const TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves, n = ti.n;
+    const int kst = 2 * (mx.pmax + 1), PM1 = mx.pmax + 1;
+    float *ahat = c.ak.as<float>((size_t)nc * kst), *bhat = c.bk.as<float>((size_t)nc * kst);
+    VPM_HIP(hipMemsetAsync(ahat, 0, sizeof(float) * (size_t)nc * kst, s));
+    VPM_HIP(hipMemsetAsync(bhat, 0, sizeof(float) * (size_t)nc * kst, s));
+    float2 *oxy = c.oxy.as<float2>(n);
+    float *qf = c.qf.as<float>(n);
+    const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
+    const uint32_t *c_key = (const uint32_t *)c.c_key.p;
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
+    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
+    const int *c_child = (const int *)c.c_child.p, *c_parent = (const int *)c.c_parent.p;
+    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
+    const uint8_t *as = (const uint8_t *)c.acts.p;
+
+    k_prep32<<<div_up(nl, 4), 256, 0, s>>>(leaf_idx, nl, c_begin, c_end, c_cx, c_cy,
+                                           (const double *)c.xs.p, (const double *)c.ys.p,
+                                           (const double *)c.qs.p, oxy, qf);
+    k_p2m32<<<div_up((long long)nl * PM1, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_begin,
+                                                             c_end, oxy, qf, as, mx.lt, kst, ahat);
+    for (int L = ti.max_level - 1; L >= 1; --L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        if (j1 <= j0) continue;
+        k_m2m32<<<div_up((long long)(j1 - j0) * PM1, 256), 256, 0, s>>>(
+            j0, j1, c_level, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
+    }
+    mk.mark(3, s);
+    // M2L list 2 on MFMA, one launch per row-tile count
+    for (int mt = 1; mt <= 2; ++mt) {
+        const int ntiles = mt == 1 ? mx.ntiles1 : mx.ntiles2;
+        const int2 *tiles = mt == 1 ? mx.tiles1 : mx.tiles2;
+        if (ntiles <= 0) continue;
+        if (mt == 1)
+            k_m2l_mfma<1><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_level, c_key,
+                                                            (const int *)c.l2.p,
+                                                            (const uint8_t *)c.n2.p, mx.lt,
+                                                            mx.rtab, ahat, kst, bhat);
+        else
+            k_m2l_mfma<2><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_level, c_key,
+                                                            (const int *)c.l2.p,
+                                                            (const uint8_t *)c.n2.p, mx.lt,
+                                                            mx.rtab, ahat, kst, bhat);
+    }
+    const int j2 = (int)ti.lev_off[2];
+    if (nc > j2)
+        k_l4_32<<<div_up((long long)(nc - j2) * (PM1 - 1), 256), 256, 0, s>>>(
+            j2, nc, list4_mode, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy,
+            c_begin, c_end, oxy, qf, as, mx.lt, mx.tm2l, ahat, kst, bhat);
+    mk.mark(4, s);
+    for (int L = 3; L <= ti.max_level; ++L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        if (j1 <= j0) continue;
+        k_l2l32<<<div_up((long long)(j1 - j0) * (PM1 - 1), 256), 256, 0, s>>>(
+            j0, j1, c_level, c_key, c_parent, mx.lt, mx.l2l, kst, bhat);
+    }
+    mk.mark(5, s);
+    k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
+        leaf_idx, nl, c_level, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
+        (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, oxy, qf,
+        c.perm_sorted, mx.lt, ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a,
+        out_b, &c.hdr_d->pairs, &c.hdr_d->m2p);
+    mk.mark(6, s);
+    VPM_LAUNCH_CHECK();
+}
+
+}  // namespace vpm

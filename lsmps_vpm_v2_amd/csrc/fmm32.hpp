@@ -1,0 +1,34 @@
+// fmm32.hpp -- mixed-precision path: per-level table and launch context.
+#pragma once
+
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace vpm {
+
+// Per-level expansion order and cell side (levels 0..16), device resident.
+struct LevelTab {
+    int p[kMaxDepth + 2];
+    float s[kMaxDepth + 2];
+    float inv_s[kMaxDepth + 2];
+};
+
+// Device pointers and sizes for one mixed-precision run.
+struct Mixed32 {
+    int pmax = 0;                      // coefficient storage is 2 (pmax + 1) floats per cell
+    const LevelTab *lt = nullptr;
+    const float *m2m = nullptr, *l2l = nullptr, *tm2l = nullptr;
+    const float *const *rtab = nullptr;  // rtab[p]: MFMA-ordered M2L matrices for order p
+    const int2 *tiles1 = nullptr, *tiles2 = nullptr;  // (first cell, count) of 16-cell tiles
+    int ntiles1 = 0, ntiles2 = 0;      // tiles with one / two 16-row MFMA blocks (2p <= 16 / 32)
+};
+
+// Host builders of the level-independent tables (see fmm32.hip).
+void mixed_tables(int pmax, std::vector<float> &m2m, std::vector<float> &l2l,
+                  std::vector<float> &tm2l, std::vector<std::vector<float>> &rlane);
+
+void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
+               double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
+
+}  // namespace vpm

@@ -95,6 +95,15 @@ class FmmSolver:
             setattr(self._o, k, v)
         self._check(self._L.vpmfmm_set_opts(self._h, C.byref(self._o)), "vpmfmm_set_opts")
 
+    def set_level_orders(self, orders):
+        """Per-level expansion orders of the mixed path (None: opts.p everywhere)."""
+        if orders is None:
+            st = self._L.vpmfmm_set_level_orders(self._h, None, 0)
+        else:
+            a = np.ascontiguousarray(orders, dtype=np.int32)
+            st = self._L.vpmfmm_set_level_orders(self._h, a.ctypes.data_as(_lib._ip), len(a))
+        self._check(st, "vpmfmm_set_level_orders")
+
     def set_stream(self, stream):
         """Run on a torch.cuda.Stream (None: the handle's own stream)."""
         ptr = None if stream is None else C.c_void_p(int(stream.cuda_stream))
