@@ -38,6 +38,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
 # Solver options of each path (reference defaults for the plain FP64 path).
 PATHS = {
     "fp64": dict(precision="fp64", p=10, max_par=75, max_src=50, binom_mode=1, list4_mode=1),
+    # FP32 + MFMA M2L, exact binomials, list 4 by P2L, uniform order 8
+    "mixed": dict(precision="mixed", p=8, max_par=48, max_src=48, binom_mode=0, list4_mode=2),
 }
 
 

@@ -97,6 +97,7 @@ struct Ctx {
 
     // mixed path: FP32 particle offsets and sources, tables, tiles
     DBuf oxy, qf;
+    DBuf part;  // per-block partial results of the tree and list kernels
     DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_rptr, mx_tiles;
 
     TreeInfo info;

@@ -201,13 +201,74 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// List 4: P2L (mode 2) from the active particles of coarser leaves, or M2L
-// (mode 1) from their scaled multipoles; thread per (target cell, m).
+// List 4, mode 2: P2L from the active particles of coarser leaves,
+//   b^_m += -q/m (s_t / (z_j - z_t))^m.
+// One wavefront per target cell; lanes stride over the source particles and
+// keep the running powers for every order (<= 16) in registers; the orders
+// are reduced across lanes at the end.
+constexpr int kP2lWaves = 4;
+
+__global__ void __launch_bounds__(kWave *kP2lWaves)
+    k_p2l_32(int j0, int nc, const uint8_t *c_level, const int *l4, const uint8_t *n4,
+             const double *c_cx, const double *c_cy, const uint32_t *c_begin,
+             const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
+             const LevelTab *lt, int kst, float *bhat) {
+    const int PM1 = kst / 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int j = j0 + blockIdx.x * kP2lWaves + threadIdx.x / kWave;
+    if (j >= nc) return;
+    const int m4 = n4[j];
+    if (m4 == 0) return;
+    const int L = c_level[j];
+    const int p = lt->p[L];
+    const double cx = c_cx[j], cy = c_cy[j];
+    const float inv_st = lt->inv_s[L];
+    float ar[16], ai[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) ar[m] = ai[m] = 0.f;
+    for (int u = 0; u < m4; ++u) {
+        const int s = l4[8 * j + u];
+        const float dcx = (float)(c_cx[s] - cx), dcy = (float)(c_cy[s] - cy);
+        for (uint32_t i = c_begin[s] + lane; i < c_end[s]; i += kWave) {
+            if (!as[i]) continue;
+            const float2 o = oxy[i];
+            const cf w = cinv(cf{(dcx + o.x) * inv_st, (dcy + o.y) * inv_st});
+            const float q = qf[i];
+            cf pw = w;
+#pragma unroll
+            for (int m = 1; m <= 16; ++m) {
+                if (m <= p) {
+                    const float c = -q / (float)m;
+                    ar[m - 1] += c * pw.x;
+                    ai[m - 1] += c * pw.y;
+                    pw = cmul(pw, w);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m <= 16; ++m) {
+        if (m <= p) {
+            float xr = ar[m - 1], xi = ai[m - 1];
+            for (int o = 1; o < kWave; o <<= 1) {
+                xr += __shfl_xor(xr, o, kWave);
+                xi += __shfl_xor(xi, o, kWave);
+            }
+            if (lane == m) {
+                const size_t idx = (size_t)j * PM1 + m;
+                stc(bhat, idx, cadd(ldc(bhat, idx), cf{xr, xi}));
+            }
+        }
+    }
+}
+
+// List 4, mode 1: M2L from the scaled multipoles of the coarser leaves;
+// thread per (target cell, m):
+//   b^_m += sum_k T_mk a^_k (s_s w)^k (s_t w)^m,  w = 1/(z_s - z_t).
 __global__ void __launch_bounds__(256)
-    k_l4_32(int j0, int nc, int mode, const uint8_t *c_level, const int *l4, const uint8_t *n4,
-            const double *c_cx, const double *c_cy, const uint32_t *c_begin,
-            const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
-            const LevelTab *lt, const float *tm2l, const float *ahat, int kst, float *bhat) {
+    k_m2l4_32(int j0, int nc, const uint8_t *c_level, const int *l4, const uint8_t *n4,
+              const double *c_cx, const double *c_cy, const LevelTab *lt, const float *tm2l,
+              const float *ahat, int kst, float *bhat) {
     const int PM1 = kst / 2;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)(nc - j0) * (PM1 - 1)) return;
@@ -220,28 +281,16 @@ __global__ void __launch_bounds__(256)
     cf acc = {0.f, 0.f};
     for (int u = 0; u < m4; ++u) {
         const int s = l4[8 * j + u];
-        if (mode == 2) {
-            // b^_m += -q/m (s_t / (z_j - z_t))^m; z_j - z_t = (c_s - c_t) + o_j
-            const float dcx = (float)(c_cx[s] - cx), dcy = (float)(c_cy[s] - cy);
-            for (uint32_t i = c_begin[s]; i < c_end[s]; ++i) {
-                if (!as[i]) continue;
-                const float2 o = oxy[i];
-                const cf w = cinv(cf{(dcx + o.x) / st, (dcy + o.y) / st});
-                acc = cadd(acc, cscale(cpowi(w, m), -qf[i] / (float)m));
-            }
-        } else {
-            // b^_m += sum_k T_mk a^_k (s_s w)^k (s_t w)^m, w = 1/(z_s - z_t)
-            const int Ls = c_level[s];
-            const int ps = lt->p[Ls];
-            const cf dz = {(float)((c_cx[s] - cx) / st), (float)((c_cy[s] - cy) / st)};
-            const cf wt = cinv(dz);                                  // s_t w
-            const cf ws = cscale(wt, lt->s[Ls] / st);                // s_s w
-            const float *row = tm2l + (size_t)m * PM1;
-            cf sacc = cscale(ldc(ahat, (size_t)s * PM1 + ps), row[ps]);
-            for (int k = ps - 1; k >= 0; --k)
-                sacc = cadd(cmul(sacc, ws), cscale(ldc(ahat, (size_t)s * PM1 + k), row[k]));
-            acc = cadd(acc, cmul(sacc, cpowi(wt, m)));
-        }
+        const int Ls = c_level[s];
+        const int ps = lt->p[Ls];
+        const cf dz = {(float)((c_cx[s] - cx) / st), (float)((c_cy[s] - cy) / st)};
+        const cf wt = cinv(dz);                       // s_t w
+        const cf ws = cscale(wt, lt->s[Ls] / st);     // s_s w
+        const float *row = tm2l + (size_t)m * PM1;
+        cf sacc = cscale(ldc(ahat, (size_t)s * PM1 + ps), row[ps]);
+        for (int k = ps - 1; k >= 0; --k)
+            sacc = cadd(cmul(sacc, ws), cscale(ldc(ahat, (size_t)s * PM1 + k), row[k]));
+        acc = cadd(acc, cmul(sacc, cpowi(wt, m)));
     }
     const size_t o = (size_t)j * PM1 + m;
     stc(bhat, o, cadd(ldc(bhat, o), acc));
@@ -270,7 +319,9 @@ __global__ void __launch_bounds__(256)
 }
 
 // Evaluation: one wavefront per leaf; list-1 sources are staged in LDS as
-// (x - c_t, y - c_t, q, index) with the centre shift done in FP64.
+// (x - c_t, y - c_t, q, index) with the centre shift done in FP64.  Lanes are
+// split into G groups of 64/G: a group owns the targets, groups split the
+// sources and list-3 cells and are reduced with cross-lane shuffles.
 constexpr int kEvalWaves = 4;
 constexpr int kStage = 256;  // sources staged per wavefront per round
 
@@ -280,8 +331,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
              const uint32_t *off1, const int *l1, const uint32_t *off3, const int *l3,
              const float2 *oxy, const float *qf, const uint32_t *perm, const LevelTab *lt,
              const float *ahat, const float *bhat, int kst, int mask, int velocity, double u_inf,
-             double v_inf, double *out_a, double *out_b, unsigned long long *pairs,
-             unsigned long long *m2p) {
+             double v_inf, double *out_a, double *out_b) {
     __shared__ float4 stage[kEvalWaves][kStage];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -290,23 +340,16 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
     const int j = (int)leaf_idx[t];
     const uint32_t b = c_begin[j], e = c_end[j];
     const int nt = (int)(e - b);
+    const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
+    const int LG = kWave / G;
+    const int g = lane / LG, tl = lane % LG;
     const double tcx = c_cx[j], tcy = c_cy[j];
     const uint32_t a1 = off1[t], z1 = off1[t + 1], a3 = off3[t], z3 = off3[t + 1];
-    if (lane == 0) {
-        unsigned long long ns = 0;
-        bool self = false;
-        for (uint32_t u = a1; u < z1; ++u) {
-            ns += c_end[l1[u]] - c_begin[l1[u]];
-            self |= l1[u] == j;
-        }
-        atomicAdd(pairs, ns * (unsigned long long)nt - (self ? (unsigned long long)nt : 0ull));
-        atomicAdd(m2p, (unsigned long long)(z3 - a3) * (unsigned long long)nt);
-    }
     const int L = c_level[j];
     const int p = lt->p[L];
     const float inv_s = lt->inv_s[L];
-    for (uint32_t tb = b; tb < e; tb += kWave) {
-        const uint32_t i = tb + lane;
+    for (uint32_t tb = b; tb < e; tb += LG) {
+        const uint32_t i = tb + tl;
         const bool valid = i < e;
         const float2 ti = valid ? oxy[i] : make_float2(0.f, 0.f);
         float ex = 0.f, ey = 0.f;
@@ -327,15 +370,12 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 }
                 fill += take;
                 pos += take;
-                if (pos >= se) {
-                    ++u;
-                    if (u < z1) pos = c_begin[l1[u]];
-                }
+                if (pos >= se && ++u < z1) pos = c_begin[l1[u]];
             }
             // lanes read what other lanes of the same wavefront wrote
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (int r = 0; r < fill; ++r) {
+            for (int r = g; r < fill; r += G) {
                 const float4 sv = stage[w][r];
                 const float dx = ti.x - sv.x, dy = ti.y - sv.y;
                 const float r2 = dx * dx + dy * dy;
@@ -348,7 +388,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             __builtin_amdgcn_wave_barrier();
         }
         // M2P over list 3: F = (1/s) sum a^_k xi^-(k+1), xi = (z - zc)/s
-        for (uint32_t v = a3; (mask & 2) && v < z3; ++v) {
+        for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
             const int sc = l3[v];
             const int Ls = c_level[sc];
             const int ps = lt->p[Ls];
@@ -359,10 +399,16 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             cf f = ldc(ahat, base + ps);
             for (int k = ps - 1; k >= 0; --k) f = cadd(cmul(f, wv), ldc(ahat, base + k));
             f = cscale(cmul(f, wv), is);
-            ex += f.x;
-            ey -= f.y;
+            if (valid) {
+                ex += f.x;
+                ey -= f.y;
+            }
         }
-        if (valid) {
+        for (int m = LG; m < kWave; m <<= 1) {
+            ex += __shfl_xor(ex, m, kWave);
+            ey += __shfl_xor(ey, m, kWave);
+        }
+        if (valid && g == 0) {
             if (mask & 4) {
                 // L2P: F = (1/s) sum_m m b^_m zeta^(m-1)
                 const cf z = {ti.x * inv_s, ti.y * inv_s};
@@ -531,10 +577,16 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
                                                             mx.rtab, ahat, kst, bhat);
     }
     const int j2 = (int)ti.lev_off[2];
-    if (nc > j2)
-        k_l4_32<<<div_up((long long)(nc - j2) * (PM1 - 1), 256), 256, 0, s>>>(
-            j2, nc, list4_mode, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy,
-            c_begin, c_end, oxy, qf, as, mx.lt, mx.tm2l, ahat, kst, bhat);
+    if (nc > j2) {
+        if (list4_mode == 2)
+            k_p2l_32<<<div_up(nc - j2, kP2lWaves), kWave * kP2lWaves, 0, s>>>(
+                j2, nc, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy,
+                c_begin, c_end, oxy, qf, as, mx.lt, kst, bhat);
+        else
+            k_m2l4_32<<<div_up((long long)(nc - j2) * (PM1 - 1), 256), 256, 0, s>>>(
+                j2, nc, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, mx.lt,
+                mx.tm2l, ahat, kst, bhat);
+    }
     mk.mark(4, s);
     for (int L = 3; L <= ti.max_level; ++L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
@@ -547,7 +599,7 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
         leaf_idx, nl, c_level, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
         (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, oxy, qf,
         c.perm_sorted, mx.lt, ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a,
-        out_b, &c.hdr_d->pairs, &c.hdr_d->m2p);
+        out_b);
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }

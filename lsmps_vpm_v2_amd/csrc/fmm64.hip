@@ -48,37 +48,44 @@ __device__ __forceinline__ cd ld(const double2 *a, size_t i) {
 }
 __device__ __forceinline__ void st(double2 *a, size_t i, cd v) { a[i] = make_double2(v.x, v.y); }
 
-// P2M: thread per (leaf, order p); active particles of active leaves only.
-__global__ void __launch_bounds__(256)
-    k_p2m64(const uint32_t *leaf_idx, int nl, int P, const uint8_t *c_flags, const uint32_t *c_begin,
-            const uint32_t *c_end, const double *c_cx, const double *c_cy, const double *xs,
-            const double *ys, const double *qs, const uint8_t *as, double2 *ak) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// P2M: one wavefront per leaf, lane p forms a_p = sum q (z - zc)^p over the
+// leaf's active particles (fmm2D.cpp:237-284, with the sign and 1/p folded
+// into the translation tables).
+constexpr int kUpWaves = 4;
+
+__global__ void __launch_bounds__(kWave *kUpWaves)
+    k_p2m64(const uint32_t *leaf_idx, int nl, int P, const uint8_t *c_flags,
+            const uint32_t *c_begin, const uint32_t *c_end, const double *c_cx,
+            const double *c_cy, const double *xs, const double *ys, const double *qs,
+            const uint8_t *as, double2 *ak) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kUpWaves + threadIdx.x / kWave;
+    if (t >= nl) return;
     const int P1 = P + 1;
-    if (idx >= (long long)nl * P1) return;
-    const int t = (int)(idx / P1), p = (int)(idx % P1);
     const int j = (int)leaf_idx[t];
-    if (!(c_flags[j] & kActive)) return;
-    const double cx = c_cx[j], cy = c_cy[j];
+    if (lane >= P1) return;
     cd acc = {0.0, 0.0};
-    for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
-        if (!as[i]) continue;
-        const cd zp = cpowi(cd{xs[i] - cx, ys[i] - cy}, p);
-        acc = cadd(acc, cscale(zp, qs[i]));
+    if (c_flags[j] & kActive) {
+        const double cx = c_cx[j], cy = c_cy[j];
+        for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
+            if (!as[i]) continue;
+            acc = cadd(acc, cscale(cpowi(cd{xs[i] - cx, ys[i] - cy}, lane), qs[i]));
+        }
     }
-    st(ak, (size_t)j * P1 + p, acc);
+    st(ak, (size_t)j * P1 + lane, acc);
 }
 
-// M2M for the cells of one level: non-leaf active parents gather children.
+// M2M for the non-leaf cells of one level: thread per (parent, p),
+//   a_p(parent) = sum_children sum_{k=0..p} c_pk a_k(child) d^(p-k),
+// d = z_child - z_parent, c_p0 = 1 (fmm2D.cpp:309-379).
 __global__ void __launch_bounds__(256)
-    k_m2m64(int j0, int j1, int P, const double *tab, const uint8_t *c_flags, const int *c_child,
-            const double *c_cx, const double *c_cy, double2 *ak) {
+    k_m2m64(int j0, int j1, int P, const double *tab, const uint8_t *c_flags, const double *c_cx,
+            const double *c_cy, const int *c_child, double2 *ak) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int P1 = P + 1;
     if (idx >= (long long)(j1 - j0) * P1) return;
     const int j = j0 + (int)(idx / P1), p = (int)(idx % P1);
-    const unsigned f = c_flags[j];
-    if ((f & kLeaf) || !(f & kActive)) return;
+    if (c_flags[j] & kLeaf) return;
     const double cx = c_cx[j], cy = c_cy[j];
     cd acc = {0.0, 0.0};
     for (int q = 0; q < 4; ++q) {
@@ -86,10 +93,10 @@ __global__ void __launch_bounds__(256)
         if (ch < 0) continue;
         const cd d = {c_cx[ch] - cx, c_cy[ch] - cy};
         const size_t b = (size_t)ch * P1;
-        // sum_{k=0..p} c_k a'_k d^(p-k), c_0 = 1: Horner from k = 0 (power d^p)
-        cd s = ld(ak, b);
-        for (int k = 1; k <= p; ++k) s = cadd(cmul(s, d), cscale(ld(ak, b + k), tab[p * P1 + k]));
-        acc = cadd(acc, s);
+        // Horner from k = 0, which carries the power d^p
+        cd sacc = ld(ak, b);
+        for (int k = 1; k <= p; ++k) sacc = cadd(cmul(sacc, d), cscale(ld(ak, b + k), tab[p * P1 + k]));
+        acc = cadd(acc, sacc);
     }
     st(ak, (size_t)j * P1 + p, acc);
 }
@@ -152,12 +159,23 @@ __global__ void __launch_bounds__(256)
 }
 
 // Evaluation: one wavefront per leaf.  Lanes are split into G groups of
-// 64/G lanes; a group owns targets, the G groups split the sources and are
-// reduced with cross-lane shuffles at the end.
+// 64/G lanes; a group owns the targets, the G groups split the sources and are
+// reduced with cross-lane shuffles.  List-1 sources are staged in LDS (x, y, q,
+// index) by the whole wavefront, then read by every lane.
 constexpr int kEvalWaves = 4;
+constexpr int kStage64 = 256;
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) {
     return __shfl_xor(v, m, kWave);
+}
+
+// 1/x to full double precision: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
 }
 
 __global__ void __launch_bounds__(kWave *kEvalWaves)
@@ -166,23 +184,14 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
              const uint32_t *off3, const int *l3, const double *xs, const double *ys,
              const double *qs, const uint32_t *perm, const double2 *ak, const double2 *bk,
              int velocity, double u_inf, double v_inf, int mask, double *out_a, double *out_b,
-             unsigned *flags, unsigned long long *pairs, unsigned long long *m2p) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int t = blockIdx.x * kEvalWaves + (threadIdx.x / kWave);
+             unsigned *flags) {
+    __shared__ double4 stage[kEvalWaves][kStage64];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kEvalWaves + w;
     if (t >= nl) return;
     const int j = (int)leaf_idx[t];
     const uint32_t b = c_begin[j], e = c_end[j];
     const int nt = (int)(e - b);
-    if (lane == 0) {
-        unsigned long long ns = 0;
-        bool self = false;
-        for (uint32_t u = off1[t]; u < off1[t + 1]; ++u) {
-            ns += c_end[l1[u]] - c_begin[l1[u]];
-            self |= l1[u] == j;
-        }
-        atomicAdd(pairs, ns * (unsigned long long)nt - (self ? (unsigned long long)nt : 0ull));
-        atomicAdd(m2p, (unsigned long long)(off3[t + 1] - off3[t]) * (unsigned long long)nt);
-    }
     const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
     const int LG = kWave / G;
     const int g = lane / LG, tl = lane % LG;
@@ -196,28 +205,44 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
         const double xi = valid ? xs[i] : 0.0, yi = valid ? ys[i] : 0.0;
         double ex = 0.0, ey = 0.0;
         // P2P over every particle of the list-1 cells, j != i (fmm2D.cpp:135-179)
-        for (uint32_t u = a1; u < ((mask & 1) ? z1 : a1); ++u) {
-            const int sc = l1[u];
-            const uint32_t sb = c_begin[sc], se = c_end[sc];
-            for (uint32_t src = sb + g; src < se; src += G) {
-                const double dx = xi - xs[src], dy = yi - ys[src];
+        uint32_t u = a1, pos = a1 < z1 ? c_begin[l1[a1]] : 0u;
+        while ((mask & 1) && u < z1) {
+            int fill = 0;
+            while (u < z1 && fill < kStage64) {
+                const uint32_t se = c_end[l1[u]];
+                const int take = (int)min<uint32_t>(se - pos, (uint32_t)(kStage64 - fill));
+                for (int r = lane; r < take; r += kWave) {
+                    const uint32_t src = pos + r;
+                    stage[w][fill + r] = make_double4(xs[src], ys[src], qs[src], (double)src);
+                }
+                fill += take;
+                pos += take;
+                if (pos >= se && ++u < z1) pos = c_begin[l1[u]];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int r = g; r < fill; r += G) {
+                const double4 sv = stage[w][r];
+                const double dx = xi - sv.x, dy = yi - sv.y;
                 const double r2 = dx * dx + dy * dy;
-                if (valid && src != i) {
+                if (valid && (uint32_t)sv.w != i) {
                     dup |= r2 == 0.0;
-                    const double qr = qs[src] / r2;
+                    const double qr = sv.z * rcp64(r2);
                     ex += qr * dx;
                     ey += qr * dy;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
         // M2P over list 3, split across groups
-        for (uint32_t u = a3 + g; u < ((mask & 2) ? z3 : 0); u += G) {
-            const int sc = l3[u];
-            const cd w = cinv(cd{xi - c_cx[sc], yi - c_cy[sc]});
+        for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
+            const int sc = l3[v];
+            const cd wv = cinv(cd{xi - c_cx[sc], yi - c_cy[sc]});
             const size_t sbk = (size_t)sc * P1;
             cd f = ld(ak, sbk + P);
-            for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, w), ld(ak, sbk + p));
-            f = cmul(f, w);
+            for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, wv), ld(ak, sbk + p));
+            f = cmul(f, wv);
             if (valid) {
                 ex += f.x;
                 ey -= f.y;
@@ -228,12 +253,12 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             ey += shfl_xor_d(ey, m);
         }
         if (valid && g == 0) {
-            // L2P
-            const cd z = {xi - lcx, yi - lcy};
-            const size_t lb = (size_t)j * P1;
-            cd f = cscale(ld(bk, lb + P), (double)P);
-            for (int p = P - 1; p >= 1; --p) f = cadd(cmul(f, z), cscale(ld(bk, lb + p), (double)p));
             if (mask & 4) {
+                // L2P
+                const cd z = {xi - lcx, yi - lcy};
+                const size_t lb = (size_t)j * P1;
+                cd f = cscale(ld(bk, lb + P), (double)P);
+                for (int p = P - 1; p >= 1; --p) f = cadd(cmul(f, z), cscale(ld(bk, lb + p), (double)p));
                 ex += f.x;
                 ey -= f.y;
             }
@@ -307,14 +332,13 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     const double *t_m2l = (const double *)c.tab_m2l.p, *t_m2m = (const double *)c.tab_m2m.p,
                  *t_l2l = (const double *)c.tab_l2l.p;
 
-    k_p2m64<<<div_up((long long)nl * P1, 256), 256, 0, s>>>(leaf_idx, nl, P, c_flags, c_begin, c_end,
-                                                            c_cx, c_cy, xs, ys, qs, as, ak);
-    // M2M: levels maxL-1 .. 1 (fmm2D.cpp:309)
+    k_p2m64<<<div_up(nl, kUpWaves), kWave * kUpWaves, 0, s>>>(
+        leaf_idx, nl, P, c_flags, c_begin, c_end, c_cx, c_cy, xs, ys, qs, as, ak);
     for (int L = ti.max_level - 1; L >= 1; --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
         k_m2m64<<<div_up((long long)(j1 - j0) * P1, 256), 256, 0, s>>>(j0, j1, P, t_m2m, c_flags,
-                                                                        c_child, c_cx, c_cy, ak);
+                                                                       c_cx, c_cy, c_child, ak);
     }
     mk.mark(3, s);
     // M2L: every cell of level >= 2
@@ -337,8 +361,7 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
         leaf_idx, nl, P, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
         (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs,
         c.perm_sorted, ak, bk,
-        velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b, &c.hdr_d->flags,
-        &c.hdr_d->pairs, &c.hdr_d->m2p);
+        velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b, &c.hdr_d->flags);
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }

@@ -106,18 +106,39 @@ __global__ void __launch_bounds__(256)
 }
 
 // treeCell.cpp:1079-1092: R = max side * (1 + expTree/100), pivot = (max + min - R)/2.
-__global__ void k_bbox_final(const double *part, int nb, double expansion, DevHdr *hdr) {
+// One block reduces the per-block partials; thread 0 forms the root geometry.
+__global__ void __launch_bounds__(256)
+    k_bbox_final(const double *part, int nb, double expansion, DevHdr *hdr) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double mn[2] = {part[0], part[2]}, mx[2] = {part[1], part[3]};
-    bool bad = part[4] != 0.0;
-    for (int b = 1; b < nb; ++b) {
-        mn[0] = fmin(mn[0], part[5 * b + 0]);
-        mx[0] = fmax(mx[0], part[5 * b + 1]);
-        mn[1] = fmin(mn[1], part[5 * b + 2]);
-        mx[1] = fmax(mx[1], part[5 * b + 3]);
+    __shared__ double s[4][256];
+    __shared__ int sbad;
+    if (threadIdx.x == 0) sbad = 0;
+    double mn0 = part[0], mx0 = part[1], mn1 = part[2], mx1 = part[3];
+    int bad = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        mn0 = fmin(mn0, part[5 * b + 0]);
+        mx0 = fmax(mx0, part[5 * b + 1]);
+        mn1 = fmin(mn1, part[5 * b + 2]);
+        mx1 = fmax(mx1, part[5 * b + 3]);
         bad |= part[5 * b + 4] != 0.0;
     }
+    s[0][threadIdx.x] = mn0;
+    s[1][threadIdx.x] = mx0;
+    s[2][threadIdx.x] = mn1;
+    s[3][threadIdx.x] = mx1;
+    __syncthreads();
+    if (bad) atomicOr(&sbad, 1);
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + o]);
+            s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + o]);
+            s[2][threadIdx.x] = fmin(s[2][threadIdx.x], s[2][threadIdx.x + o]);
+            s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double mn[2] = {s[0][0], s[2][0]}, mx[2] = {s[1][0], s[3][0]};
     double R = 0.0;
     for (int d = 0; d < 2; ++d) {
         double side = fabs(mx[d] - mn[d]);
@@ -131,7 +152,7 @@ __global__ void k_bbox_final(const double *part, int nb, double expansion, DevHd
         hdr->pivot[d] = (mx[d] + mn[d] - R) / 2.0;
     }
     unsigned f = 0;
-    if (bad) f |= kFlagNonFinite;
+    if (sbad) f |= kFlagNonFinite;
     if (!(R > 0.0)) f |= kFlagRange;
     hdr->flags |= f;
     hdr->leaf_lvl_min = kMaxDepth + 1;
@@ -197,7 +218,117 @@ __global__ void __launch_bounds__(256)
     v[m] = (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]]);
 }
 
+// Per-block partial results (two 64-bit words per block) and their one-block
+// reduction.  Global atomics on one header word from thousands of blocks
+// serialize across the chiplets; partials plus one small reduction do not.
+enum PartOp { kPartSplit = 0, kPartM2lLists = 1, kPartLeafLists = 2 };
+
+// Block-wide (sum or min/max) of two 64-bit values; thread 0 writes part[blk].
+template <bool MINMAX>
+__device__ __forceinline__ void block_part(unsigned long long a, unsigned long long b,
+                                           ulonglong2 *part) {
+    __shared__ unsigned long long sa[16], sb[16];
+    for (int m = 1; m < kWave; m <<= 1) {
+        const unsigned long long oa = __shfl_xor(a, m, kWave), ob = __shfl_xor(b, m, kWave);
+        if (MINMAX) {
+            a = min(a, oa);
+            b = max(b, ob);
+        } else {
+            a += oa;
+            b += ob;
+        }
+    }
+    const int w = threadIdx.x / kWave, nw = (blockDim.x + kWave - 1) / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int u = 1; u < nw; ++u) {
+            if (MINMAX) {
+                a = min(a, sa[u]);
+                b = max(b, sb[u]);
+            } else {
+                a += sa[u];
+                b += sb[u];
+            }
+        }
+        part[blockIdx.x] = make_ulonglong2(a, b);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_reduce_part(const ulonglong2 *part, int nblk, int op, DevHdr *hdr) {
+    const bool mm = op == kPartSplit;
+    unsigned long long a = mm ? ~0ull : 0ull, b = 0ull;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        const ulonglong2 v = part[i];
+        if (mm) {
+            a = min(a, v.x);
+            b = max(b, v.y);
+        } else {
+            a += v.x;
+            b += v.y;
+        }
+    }
+    __shared__ unsigned long long sa[4], sb[4];
+    for (int m = 1; m < kWave; m <<= 1) {
+        const unsigned long long oa = __shfl_xor(a, m, kWave), ob = __shfl_xor(b, m, kWave);
+        a = mm ? min(a, oa) : a + oa;
+        b = mm ? max(b, ob) : b + ob;
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int u = 1; u < 4; ++u) {
+        a = mm ? min(a, sa[u]) : a + sa[u];
+        b = mm ? max(b, sb[u]) : b + sb[u];
+    }
+    if (op == kPartSplit) {
+        // a = min leaf level; b = max leaf level | depth-overflow bit 32
+        hdr->leaf_lvl_min = (uint32_t)a;
+        hdr->leaf_lvl_max = (uint32_t)(b & 0xffffffffull);
+        if (b >> 32) hdr->flags |= kFlagDepth;
+    } else if (op == kPartM2lLists) {
+        hdr->sum2 = (uint32_t)a;
+        hdr->sum4 = (uint32_t)b;
+    } else {
+        hdr->pairs = a;
+        hdr->m2p = b;
+    }
+}
+
 constexpr int kSplitBlock = 256;
+
+// Sliding-window maxima by the van Herk / Gil-Werman method: with blocks of
+// length B aligned at 0, the max over [t, t + B - 1] is max(h[t], g[t + B - 1])
+// where g / h are block-prefix / block-suffix maxima.  Entries outside the
+// valid range hold 0, which never raises a maximum of valid lcp values.
+__device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *g, uint8_t *h,
+                                                    int len, int B) {
+    const int nb = (len + B - 1) / B;
+    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) {
+        const int blk = b < nb ? b : b - nb;
+        const int lo = blk * B, hi = min(lo + B, len);
+        uint8_t m = 0;
+        if (b < nb) {
+            for (int x = lo; x < hi; ++x) {
+                m = max(m, a[x]);
+                g[x] = m;
+            }
+        } else {
+            for (int x = hi - 1; x >= lo; --x) {
+                m = max(m, a[x]);
+                h[x] = m;
+            }
+        }
+    }
+}
 
 // Per particle: the deepest level whose cell is over the particle (W) or
 // source (S) threshold, hence the leaf level; and the number of cells whose
@@ -205,83 +336,68 @@ constexpr int kSplitBlock = 256;
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, DevHdr *hdr_w) {
+            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, ulonglong2 *part) {
     extern __shared__ uint8_t lds[];
-    __shared__ int smin, smax, sdepth;
     const int i0 = blockIdx.x * kSplitBlock;
     const int i = i0 + threadIdx.x;
     const int ilast = min(i0 + kSplitBlock, n) - 1;
-    const long long na = hdr->n_active;
-    if (threadIdx.x == 0) {
-        smin = kMaxDepth + 1;
-        smax = 0;
-        sdepth = 0;
-    }
-    // window over w: j in [i0 - W, i0 + 255] intersected with [0, n-1-W]
-    uint8_t *sw = lds;
+    const int na = (int)hdr->n_active;
+    // runs of W+1 particles: start j in [i - W, i]; LDS index j - (i0 - W)
     const int wlen = W + kSplitBlock;
-    const long long wbase = (long long)i0 - W;
-    const long long wlast = (long long)n - 1 - W;  // last valid window start
+    const int wbase = i0 - W;
+    const int wlast = n - 1 - W;  // last valid run start (< 0: no run of W+1 particles)
+    uint8_t *sw = lds, *gw = sw + wlen, *hw = gw + wlen;
     for (int t = threadIdx.x; t < wlen; t += kSplitBlock) {
-        long long j = wbase + t;
+        const int j = wbase + t;
         sw[t] = (j >= 0 && j <= wlast) ? w[j] : 0;
     }
-    // window over v: m in [r_lo - S - 1, r_hi]
-    uint8_t *sv = lds + wlen;
-    const long long r_lo = A[i0], r_hi = A[ilast];
-    const long long vbase = r_lo - S - 1;
-    const long long vlast = na - 1 - S;  // last valid active window start
-    const int vlen = (int)(r_hi - vbase + 1);
+    // runs of S+1 active particles: start m in [r - S, r - 1]; LDS index m - vbase
+    const int r_lo = (int)A[i0], r_hi = (int)A[ilast];
+    const int vbase = r_lo - S - 1;
+    const int vlast = na - 1 - S;
+    const int vlen = r_hi - vbase + 1;
+    uint8_t *sv = hw + wlen, *gv = sv + vlen, *hv = gv + vlen;
     for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
-        long long m = vbase + t;
+        const int m = vbase + t;
         sv[t] = (m >= 0 && m <= vlast) ? v[m] : 0;
     }
     __syncthreads();
+    block_prefix_suffix(sw, gw, hw, wlen, W + 1);
+    if (S > 0) block_prefix_suffix(sv, gv, hv, vlen, S);
+    __syncthreads();
+    unsigned long long my_min = kMaxDepth + 1, my_max = 0;
     if (i < n) {
         const uint32_t ki = key[i];
-        int par_split = -1, src_split = -1;
-        // runs of W+1 particles that contain i: starts j in [i-W, i]
-        {
-            long long jlo = (long long)i - W, jhi = i;
-            if (jlo < 0) jlo = 0;
-            if (jhi > wlast) jhi = wlast;
-            for (long long j = jlo; j <= jhi; ++j) par_split = max(par_split, (int)sw[j - wbase]);
+        const int t = threadIdx.x;
+        int par_split = wlast >= 0 ? (int)max(hw[t], gw[t + W]) : -1;
+        int src_split = -1;
+        const int r = (int)A[i];
+        if (S > 0) {
+            const int a = r - S - vbase;  // window [r - S, r - 1] in LDS coordinates
+            src_split = (int)max(hv[a], gv[a + S - 1]);
         }
-        // runs of S+1 active particles around rank r
-        {
-            const long long r = A[i];
-            long long mlo = r - S, mhi = r - 1;
-            if (mlo < 0) mlo = 0;
-            if (mhi > vlast) mhi = vlast;
-            for (long long m = mlo; m <= mhi; ++m) src_split = max(src_split, (int)sv[m - vbase]);
-            if (r <= vlast) {  // run starting at the first active at or after i
-                int e = min((int)sv[r - vbase], lcp_levels(ki, key[apos[r]]));
-                src_split = max(src_split, e);
-            }
-            const long long m0 = r - S - 1;
-            if (m0 >= 0 && m0 <= vlast) {  // run ending at the last active before i
-                int e = min((int)sv[m0 - vbase], lcp_levels(key[apos[r - 1]], ki));
-                src_split = max(src_split, e);
-            }
+        if (r <= vlast) {  // run starting at the first active at or after i
+            const int e = min((int)sv[r - vbase], lcp_levels(ki, key[apos[r]]));
+            src_split = max(src_split, e);
+        }
+        const int m0 = r - S - 1;
+        if (m0 >= 0 && m0 <= vlast) {  // run ending at the last active before i
+            const int e = min((int)sv[m0 - vbase], lcp_levels(key[apos[r - 1]], ki));
+            src_split = max(src_split, e);
         }
         int lvl = max(max(par_split, src_split) + 1, max(min_level, 1));
         if (max_level > 0) lvl = min(lvl, max_level);
         if (lvl > kMaxDepth) {
-            atomicOr(&sdepth, 1);
+            my_max |= 1ull << 32;  // depth overflow
             lvl = kMaxDepth;
         }
         leaf_lvl[i] = (uint8_t)lvl;
         const int di = d[i];
         ccount[i] = lvl > di ? (uint32_t)(lvl - di) : 0u;
-        atomicMin(&smin, lvl);
-        atomicMax(&smax, lvl);
+        my_min = (unsigned long long)lvl;
+        my_max |= (unsigned long long)lvl;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicMin(&hdr_w->leaf_lvl_min, (uint32_t)smin);
-        atomicMax(&hdr_w->leaf_lvl_max, (uint32_t)smax);
-        if (sdepth) atomicOr(&hdr_w->flags, kFlagDepth);
-    }
+    block_part<true>(my_min, my_max, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,39 +533,88 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // 6. neighbour table and lists
 // ---------------------------------------------------------------------------
+// Index of the cell with key `pre` in the sorted range [lo, hi), or -1;
+// galloping search from `start` (neighbours lie close in Morton order).
+__device__ __forceinline__ int find_near(const uint32_t *key, int lo, int hi, int start,
+                                         uint32_t pre) {
+    const uint32_t ks = key[start];
+    if (ks == pre) return start;
+    int a, b;
+    if (ks < pre) {
+        int prev = start, step = 1;
+        for (;;) {
+            const int nx = start + step;
+            if (nx >= hi) {
+                a = prev + 1;
+                b = hi;
+                break;
+            }
+            if (key[nx] >= pre) {
+                a = prev + 1;
+                b = nx;
+                break;
+            }
+            prev = nx;
+            step <<= 1;
+        }
+    } else {
+        int prev = start, step = 1;
+        for (;;) {
+            const int nx = start - step;
+            if (nx < lo) {
+                a = lo;
+                b = prev;
+                break;
+            }
+            if (key[nx] < pre) {
+                a = nx + 1;
+                b = prev;
+                break;
+            }
+            prev = nx;
+            step <<= 1;
+        }
+    }
+    while (a < b) {  // lower bound in [a, b]; key[b] >= pre or b == hi
+        const int m = (a + b) >> 1;
+        if (key[m] < pre) a = m + 1;
+        else b = m;
+    }
+    return (a < hi && key[a] == pre) ? a : -1;
+}
+
 // nbr[9j + k], k = (dx+1) + 3 (dy+1) (treeCell.cpp:593-622 order): the
 // deepest existing cell at level <= L containing the level-L cell at
-// (ix+dx, iy+dy), or -1 outside [0, 2^L)^2.
+// (ix+dx, iy+dy), or -1 outside [0, 2^L)^2.  One thread per (cell, k).
 __global__ void __launch_bounds__(256)
     k_nbr(const uint8_t *c_level, const uint32_t *c_key, int nc, const DevHdr *hdr, int *nbr) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nc) return;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 9LL * nc) return;
+    const int j = (int)(idx / 9), k = (int)(idx % 9);
     const int L = c_level[j];
     if (L == 0) {
-        for (int k = 0; k < 9; ++k) nbr[9 * j + k] = k == 4 ? 0 : -1;
+        nbr[9 * j + k] = k == 4 ? 0 : -1;
+        return;
+    }
+    if (k == 4) {
+        nbr[9 * j + k] = j;
         return;
     }
     const uint32_t pre = c_key[j];
     const int ix = (int)compact1by1(pre), iy = (int)compact1by1(pre >> 1);
     const int cnt = 1 << L;
-    for (int k = 0; k < 9; ++k) {
-        const int X = ix + (k % 3) - 1, Y = iy + (k / 3) - 1;
-        int res = -1;
-        if (k == 4) {
-            res = j;
-        } else if (X >= 0 && X < cnt && Y >= 0 && Y < cnt) {
-            for (int l = L; l >= 0; --l) {
-                const int sh = L - l;
-                const uint32_t p = part1by1((uint32_t)X >> sh) | (part1by1((uint32_t)Y >> sh) << 1);
-                const int f = l == 0 ? 0 : find_cell(c_key, hdr->lev_off, l, p);
-                if (f >= 0) {
-                    res = f;
-                    break;
-                }
-            }
+    const int X = ix + (k % 3) - 1, Y = iy + (k / 3) - 1;
+    int res = -1;
+    if (X >= 0 && X < cnt && Y >= 0 && Y < cnt) {
+        const uint32_t p = part1by1((uint32_t)X) | (part1by1((uint32_t)Y) << 1);
+        res = find_near(c_key, (int)hdr->lev_off[L], (int)hdr->lev_off[L + 1], j, p);
+        for (int l = L - 1; res < 0 && l >= 0; --l) {
+            const int sh = L - l;
+            const uint32_t q = part1by1((uint32_t)X >> sh) | (part1by1((uint32_t)Y >> sh) << 1);
+            res = l == 0 ? 0 : find_cell(c_key, hdr->lev_off, l, q);
         }
-        nbr[9 * j + k] = res;
     }
+    nbr[9 * j + k] = res;
 }
 
 // Lists 2 and 4 of every cell of level >= 2 (treeCell.cpp:818-1032).
@@ -457,15 +622,15 @@ __global__ void __launch_bounds__(256)
     k_m2l_lists(const uint8_t *c_level, const uint8_t *c_flags, const double *c_cx,
                 const double *c_cy, const int *c_parent, const int *c_child, const int *nbr,
                 const DevHdr *hdr, int j0, int nc, int *l2, uint8_t *n2, int *l4, uint8_t *n4,
-                DevHdr *hdr_w) {
+                DevHdr *hdr_w, ulonglong2 *part) {
 #pragma clang fp contract(off)
     const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nc) return;
+    int m2 = 0, m4c = 0, m4 = 0;
+    if (j < nc) {
     const int L = c_level[j];
     const double R = hdr->root_len, s = cell_len(R, L);
     const double cx = c_cx[j], cy = c_cy[j];
     const int P = c_parent[j];
-    int m2 = 0, m4c = 0, m4 = 0;
     int cand[8];
     const double md2 = s + kListEps;
     for (int k = 0; k < 9; ++k) {
@@ -498,85 +663,98 @@ __global__ void __launch_bounds__(256)
     }
     n2[j] = (uint8_t)m2;
     n4[j] = (uint8_t)m4;
-    if (m2) atomicAdd(&hdr_w->sum2, (uint32_t)m2);
-    if (m4) atomicAdd(&hdr_w->sum4, (uint32_t)m4);
+    }
+    block_part<false>((unsigned long long)m2, (unsigned long long)m4, part);
 }
 
 // Lists 1 and 3 of every leaf (treeCell.cpp:636-804), count or fill pass.
+// The fill pass also counts the P2P pairs and M2P evaluations the lists imply
+// (statistics), reduced per block into per-block partials.
 template <bool FILL>
 __global__ void __launch_bounds__(256)
     k_leaf_lists(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
                  const double *c_cx, const double *c_cy, const int *c_child, const int *nbr,
                  const DevHdr *hdr, uint32_t *cnt1, uint32_t *cnt3, const uint32_t *off1,
-                 const uint32_t *off3, int *l1, int *l3, DevHdr *hdr_w) {
+                 const uint32_t *off3, int *l1, int *l3, const uint32_t *c_begin,
+                 const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nl) return;
-    const int j = (int)leaf_idx[t];
-    const int L = c_level[j];
-    const double R = hdr->root_len, s = cell_len(R, L);
-    const double cx = c_cx[j], cy = c_cy[j];
-    uint32_t n1 = 0, n3 = 0;
-    const uint32_t o1 = FILL ? off1[t] : 0, o3 = FILL ? off3[t] : 0;
-    int coarse[9];
-    int nco = 0;
-    int stack[64];
-    int sp = 0;
+    unsigned long long my_pairs = 0, my_m2p = 0;
     bool overflow = false;
-    for (int k = 0; k < 9; ++k) {
-        const int A = nbr[9 * j + k];
-        if (A < 0) continue;
-        const unsigned fa = c_flags[A];
-        if (c_level[A] < L) {  // reached through the parent walk (treeCell.cpp:706-720, 738)
-            if ((fa & kLeaf) && (fa & kActive)) {
-                bool dup = false;
-                for (int u = 0; u < nco; ++u) dup |= coarse[u] == A;
-                if (!dup) {
-                    coarse[nco++] = A;
-                    if (FILL) l1[o1 + n1] = A;
-                    ++n1;
-                }
+    if (t < nl) {
+        const int j = (int)leaf_idx[t];
+        const int L = c_level[j];
+        const double R = hdr->root_len, s = cell_len(R, L);
+        const double cx = c_cx[j], cy = c_cy[j];
+        uint32_t n1 = 0, n3 = 0;
+        unsigned long long src_total = 0;
+        bool self = false;
+        const uint32_t o1 = FILL ? off1[t] : 0, o3 = FILL ? off3[t] : 0;
+        int coarse[9];
+        int nco = 0;
+        int stack[64];
+        int sp = 0;
+        auto emit1 = [&](int c) {
+            if (FILL) {
+                l1[o1 + n1] = c;
+                src_total += c_end[c] - c_begin[c];
+                self |= c == j;
             }
-            continue;
-        }
-        if (!(fa & kActive)) continue;
-        if (fa & kLeaf) {
-            if (FILL) l1[o1 + n1] = A;
             ++n1;
-            continue;
-        }
-        if (sp < 64) stack[sp++] = A;
-        else overflow = true;
-    }
-    while (sp > 0) {
-        const int B = stack[--sp];
-        const double adj = ((s + (cell_len(R, c_level[B]) / 2.0)) / 2.0) + kListEps;
-        for (int q = 0; q < 4; ++q) {
-            const int ch = c_child[4 * B + q];
-            if (ch < 0) continue;
-            const unsigned fc = c_flags[ch];
-            if (!(fc & kActive)) continue;
-            const bool is_adj = fabs(c_cx[ch] - cx) <= adj && fabs(c_cy[ch] - cy) <= adj;
-            if (is_adj) {
-                if (fc & kLeaf) {
-                    if (FILL) l1[o1 + n1] = ch;
-                    ++n1;
-                } else if (sp < 64) {
-                    stack[sp++] = ch;
-                } else {
-                    overflow = true;
+        };
+        for (int k = 0; k < 9; ++k) {
+            const int A = nbr[9 * j + k];
+            if (A < 0) continue;
+            const unsigned fa = c_flags[A];
+            if (c_level[A] < L) {  // reached through the parent walk (treeCell.cpp:706-720, 738)
+                if ((fa & kLeaf) && (fa & kActive)) {
+                    bool dup = false;
+                    for (int u = 0; u < nco; ++u) dup |= coarse[u] == A;
+                    if (!dup) {
+                        coarse[nco++] = A;
+                        emit1(A);
+                    }
                 }
-            } else {
-                if (FILL) l3[o3 + n3] = ch;
-                ++n3;
+                continue;
+            }
+            if (!(fa & kActive)) continue;
+            if (fa & kLeaf) {
+                emit1(A);
+                continue;
+            }
+            if (sp < 64) stack[sp++] = A;
+            else overflow = true;
+        }
+        while (sp > 0) {
+            const int B = stack[--sp];
+            const double adj = ((s + (cell_len(R, c_level[B]) / 2.0)) / 2.0) + kListEps;
+            for (int q = 0; q < 4; ++q) {
+                const int ch = c_child[4 * B + q];
+                if (ch < 0) continue;
+                const unsigned fc = c_flags[ch];
+                if (!(fc & kActive)) continue;
+                const bool is_adj = fabs(c_cx[ch] - cx) <= adj && fabs(c_cy[ch] - cy) <= adj;
+                if (is_adj) {
+                    if (fc & kLeaf) emit1(ch);
+                    else if (sp < 64) stack[sp++] = ch;
+                    else overflow = true;
+                } else {
+                    if (FILL) l3[o3 + n3] = ch;
+                    ++n3;
+                }
             }
         }
-    }
-    if (!FILL) {
-        cnt1[t] = n1;
-        cnt3[t] = n3;
+        if (!FILL) {
+            cnt1[t] = n1;
+            cnt3[t] = n3;
+        } else {
+            const unsigned long long nt = c_end[j] - c_begin[j];
+            my_pairs = src_total * nt - (self ? nt : 0ull);
+            my_m2p = (unsigned long long)n3 * nt;
+        }
     }
     if (overflow) atomicOr(&hdr_w->flags, kFlagListOverflow);
+    if (FILL) block_part<false>(my_pairs, my_m2p, part);
 }
 
 }  // namespace
@@ -608,7 +786,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     double *part = c.bbox_part.as<double>(5 * (size_t)nbb);
     k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part);
     const double expansion = 1.0 + (tp.exp_tree / 100.0);
-    k_bbox_final<<<1, 1, 0, s>>>(part, nbb, expansion, c.hdr_d);
+    k_bbox_final<<<1, 256, 0, s>>>(part, nbb, expansion, c.hdr_d);
     VPM_LAUNCH_CHECK();
 
     // 2-3. keys, sort, gather
@@ -638,10 +816,12 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     k_lcp_v<<<div_up(n, 256), 256, 0, s>>>(key, apos, c.hdr_d, S, n, v);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
-    const size_t lds = (size_t)(W + kSplitBlock) + (size_t)(S + kSplitBlock + 2) + 16;
-    k_split<<<div_up(n, kSplitBlock), kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W,
-                                                              S, tp.min_level, tp.max_level,
-                                                              leaf_lvl, cnt, c.hdr_d);
+    const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
+    const int nbs = div_up(n, kSplitBlock);
+    ulonglong2 *spart = c.part.as<ulonglong2>(nbs);
+    k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
+                                          tp.max_level, leaf_lvl, cnt, spart);
+    k_reduce_part<<<1, 256, 0, s>>>(spart, nbs, kPartSplit, c.hdr_d);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
     scan_exclusive_u32(cnt, base, n, &c.hdr_d->n_cells_pre, true, tmp, s);
     VPM_LAUNCH_CHECK();
@@ -697,7 +877,7 @@ void tree_lists(Ctx &c) {
     int *c_parent = (int *)c.c_parent.p, *c_child = (int *)c.c_child.p;
     int *nbr = c.c_nbr.as<int>(9 * (size_t)nc);
     uint32_t *tmp = (uint32_t *)c.tmp.p;
-    k_nbr<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, nbr);
+    k_nbr<<<div_up(9LL * nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, nbr);
 
     // level offsets and leaf count are needed on the host for the launches below
     sync_hdr(c);
@@ -712,17 +892,20 @@ void tree_lists(Ctx &c) {
     const int j0 = (int)ti.lev_off[2];
     VPM_HIP(hipMemsetAsync(n2, 0, nc, s));
     VPM_HIP(hipMemsetAsync(n4, 0, nc, s));
-    if (nc > j0)
-        k_m2l_lists<<<div_up(nc - j0, 256), 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent,
-                                                          c_child, nbr, c.hdr_d, j0, nc, l2, n2,
-                                                          l4, n4, c.hdr_d);
+    const int nb2 = nc > j0 ? div_up(nc - j0, 256) : 0;
+    ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, div_up(nl, 256)), 1));
+    if (nb2 > 0)
+        k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
+                                        c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d, part);
+    k_reduce_part<<<1, 256, 0, s>>>(part, nb2, kPartM2lLists, c.hdr_d);
     const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
     uint32_t *cnt1 = c.cnt1.as<uint32_t>(nl + 1), *cnt3 = c.cnt3.as<uint32_t>(nl + 1);
     uint32_t *off1 = c.off1.as<uint32_t>(nl + 1), *off3 = c.off3.as<uint32_t>(nl + 1);
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
     k_leaf_lists<false><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
                                                          c_cy, c_child, nbr, c.hdr_d, cnt1, cnt3,
                                                          nullptr, nullptr, nullptr, nullptr,
-                                                         c.hdr_d);
+                                                         c_begin, c_end, c.hdr_d, nullptr);
     scan_exclusive_u32(cnt1, off1, nl, &c.hdr_d->tot1, true, tmp, s);
     scan_exclusive_u32(cnt3, off3, nl, &c.hdr_d->tot3, true, tmp, s);
     VPM_LAUNCH_CHECK();
@@ -733,7 +916,9 @@ void tree_lists(Ctx &c) {
     int *l1 = c.l1.as<int>((size_t)ti.tot1 + 1), *l3 = c.l3.as<int>((size_t)ti.tot3 + 1);
     k_leaf_lists<true><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
                                                         c_cy, c_child, nbr, c.hdr_d, nullptr,
-                                                        nullptr, off1, off3, l1, l3, c.hdr_d);
+                                                        nullptr, off1, off3, l1, l3, c_begin, c_end,
+                                                        c.hdr_d, part);
+    k_reduce_part<<<1, 256, 0, s>>>(part, div_up(nl, 256), kPartLeafLists, c.hdr_d);
     VPM_LAUNCH_CHECK();
 }
 

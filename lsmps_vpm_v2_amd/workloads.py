@@ -80,6 +80,91 @@ def boundary_layer(seed: int = 2, lx: float = 10.2, h0: float = 0.0025, band: fl
     return _pack(x, y, s, w, name="flat_plate_boundary_layer_780k")
 
 
+def _square_patch(cx, cy, half_width, n_side):
+    c = (np.arange(n_side) + 0.5) * (2 * half_width / n_side) - half_width
+    X, Y = np.meshgrid(cx + c, cy + c, indexing="xy")
+    return X.ravel(), Y.ravel(), 2 * half_width / n_side
+
+
+def shed_vortices(seed: int = 4, n_total: int = 2_000_000, n_vort: int = 8) -> dict:
+    """The flat-plate boundary layer (``boundary_layer()``, about 780k
+    particles) plus ``n_vort`` shed Lamb-Oseen vortices near the plate: core
+    radius U[0.05, 0.3], |circulation| U[10, 100] with a random sign, centres at
+    x U[-4.5, 4.5], |y| U[0.5, 2.0] (outside the |y| < 0.36 band, patches not
+    overlapping).  Each vortex is a square patch of side 6 core radii on its own
+    grid, sized so that the whole field holds about ``n_total`` particles, and
+    carries that vortex's vorticity; the boundary-layer particles keep theirs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bl = boundary_layer()
+    per = (n_total - bl["n"]) // n_vort
+    side = int(np.sqrt(per))
+    xs, ys, ss, ws = [bl["x"]], [bl["y"]], [bl["s"]], [bl["omega"]]
+    placed = []
+    while len(placed) < n_vort:
+        a = rng.uniform(0.05, 0.3)
+        cx, cy = rng.uniform(-4.5, 4.5), rng.choice([-1.0, 1.0]) * rng.uniform(0.5, 2.0)
+        if abs(cy) - 3 * a < 0.4:
+            continue
+        if any(max(abs(cx - px), abs(cy - py)) < 3 * (a + pa) for px, py, pa in placed):
+            continue
+        placed.append((cx, cy, a))
+    for k, (cx, cy, a) in enumerate(placed):
+        gam = rng.choice([-1.0, 1.0]) * rng.uniform(10.0, 100.0)
+        x, y, h = _square_patch(cx, cy, 3 * a, side)
+        xs.append(x)
+        ys.append(y)
+        ss.append(np.full(len(x), h))
+        ws.append(lamb_oseen(x, y, cx, cy, gam, a))
+    x, y, s, w = (np.concatenate(v) for v in (xs, ys, ss, ws))
+    return _pack(x, y, s, w, name=f"boundary_layer_shed_vortices_{len(x) // 1000}k",
+                 vortices=np.array(placed))
+
+
+def von_karman(seed: int = 5, n_total: int = 8_000_000, half: float = 20.0, n_vort: int = 40,
+               core: float = 0.3, gamma: float = 1.5, radius: float = 0.5, h_body: float = 0.002,
+               layer: float = 0.1) -> dict:
+    """von Karman-street-like field on [-half, half]^2: ``n_vort`` Lamb-Oseen
+    vortices (core 0.3, |circulation| 1.5, alternating sign) in two staggered
+    rows y = +-0.6 behind a cylinder of radius 0.5 at the origin (spacing
+    half / (n_vort / 2) in x from x = 1.5), on a uniform background grid outside
+    r = radius + layer, plus a body-surface layer of rings at spacing h_body for
+    radius <= r < radius + layer with omega = sign(y) 10 exp(-(r - radius) / 0.02).
+    Every particle is kept (no vorticity cut), the background spacing is chosen
+    so that the total is about ``n_total``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    # body layer: rings of equal arc spacing
+    rr = radius + (np.arange(int(round(layer / h_body))) + 0.5) * h_body
+    bx, by = [], []
+    for r in rr:
+        m = int(round(2 * np.pi * r / h_body))
+        th = (np.arange(m) + 0.5) * (2 * np.pi / m) + rng.uniform(0, 2 * np.pi / m)
+        bx.append(r * np.cos(th))
+        by.append(r * np.sin(th))
+    bx, by = np.concatenate(bx), np.concatenate(by)
+    rb = np.hypot(bx, by)
+    wb = np.sign(by) * 10.0 * np.exp(-(rb - radius) / 0.02)
+    # background grid
+    n_bg = n_total - len(bx)
+    m = int(round(np.sqrt(n_bg / (1.0 - np.pi * (radius + layer) ** 2 / (2 * half) ** 2))))
+    h = 2 * half / m
+    c = -half + (np.arange(m) + 0.5) * h
+    X, Y = np.meshgrid(c, c, indexing="xy")
+    gx, gy = X.ravel(), Y.ravel()
+    keep = gx * gx + gy * gy >= (radius + layer) ** 2
+    gx, gy = gx[keep], gy[keep]
+    dxv = half / (n_vort / 2)
+    centres = [(1.5 + (k // 2) * dxv + (0.5 * dxv if k % 2 else 0.0), 0.6 if k % 2 == 0 else -0.6)
+               for k in range(n_vort)]
+    wg = np.zeros_like(gx)
+    for k, (cx, cy) in enumerate(centres):
+        sgn = 1.0 if k % 2 == 0 else -1.0
+        wg += lamb_oseen(gx, gy, cx, cy, -sgn * gamma, core)
+    x, y = np.concatenate([bx, gx]), np.concatenate([by, gy])
+    s = np.concatenate([np.full(len(bx), h_body), np.full(len(gx), h)])
+    w = np.concatenate([wb, wg])
+    return _pack(x, y, s, w, name=f"von_karman_street_{len(x) // 1000}k")
+
+
 def perlman(h: float = 0.02, half: float = 1.2) -> dict:
     """Perlman vortex omega = (1 - r^2)^7 (r <= 1) on a uniform grid
     (reference src/initialization/initVor.cpp:10-53) with its closed-form
