@@ -4,16 +4,19 @@
 Workload: the flat-plate boundary-layer field (779,280 particles, seeded,
 lsmps_vpm_v2_amd.workloads.boundary_layer).  A step is one full velocity
 solve of all particles with inputs already resident in HBM: binning, tree,
-lists, every FMM stage and the write-back of (u, v) in caller order.
+lists, order selection (learned path), every FMM stage and the write-back of
+(u, v) in caller order.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fp64|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path learned|mixed|fp64]
 
-Prints one JSON line.  ``value`` = particles / (seconds per step).  The
-relative L2 error of the rotational velocity against an FP64 direct sum on
-the GPU (seeded target sample, all sources) is reported next to it, as is
-the plain FP64 path when the headline path is another one.  The CPU baseline
-is the reference's own FMM (oracle/_ref, built from the reference sources)
-timed on this host's cores on the same field.
+Prints one JSON line.  ``value`` = particles / (seconds per step) of the
+headline path (default: the learned-order FP32/MFMA path).  The plain FP64
+path (reference options: p = 10, 75/50 leaf rule) is timed beside it and the
+ratio reported.  Errors: relative L2 of the rotational velocity against the
+FP64 direct sum on the GPU (20,000 seeded targets, all sources) and against
+the reference's own CPU FMM (all particles).  The CPU baseline is that
+reference FMM (oracle/_ref, built from the reference sources) on this host's
+cores.
 """
 from __future__ import annotations
 
@@ -32,58 +35,68 @@ sys.path.insert(0, ROOT)
 METRIC = "FMM particle-velocities/sec at 780k particles, rel-L2 ≤1e-4 vs FP64 direct"
 UNIT = "particle-velocities/s"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector; matrix is the same figure)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector; matrix is the same figure)
 
-# Solver options of each path (reference defaults for the plain FP64 path).
+# Solver options of each path (the plain FP64 path uses the reference defaults).
 PATHS = {
     "fp64": dict(precision="fp64", p=10, max_par=75, max_src=50, binom_mode=1, list4_mode=1),
     # FP32 + MFMA M2L, exact binomials, list 4 by P2L, uniform order 8
     "mixed": dict(precision="mixed", p=8, max_par=48, max_src=48, binom_mode=0, list4_mode=2),
+    # the packaged MLP picks p per level and the leaf threshold; p = 12 fallback
+    "learned": dict(precision="mixed", p=12, max_par=48, max_src=48, binom_mode=0, list4_mode=2,
+                    order="learned"),
 }
+KERNEL = {"fp64": "k_eval64", "mixed": "k_eval32", "learned": "k_eval32"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def eval_flops(st, p):
+def eval_flops(st):
     """Algorithmic flops of the evaluation kernel for one step: 11 per P2P
-    pair, 8(p+1) per M2P (particle, list-3 cell), 8p per L2P particle."""
-    return 11.0 * st["p2p_pairs"] + 8.0 * (p + 1) * st["m2p_evals"] + 8.0 * p * st["n"]
+    (target, source) pair, 8 per M2P term (order + 1 per (particle, list-3
+    cell)), 8 per L2P term (the leaf's order per particle)."""
+    return 11.0 * st["p2p_pairs"] + 8.0 * st["m2p_terms"] + 8.0 * st["l2p_terms"]
 
 
 def time_path(torch, solver, dev, steps, warmup):
-    for _ in range(warmup):
+    """Mean seconds per step and the per-step statistics; the selection of
+    the first call (which chooses the leaf threshold on the learned path) is
+    kept on the solver as ``first_selection``."""
+    for k in range(warmup):
         solver.velocity_device(*dev)
+        if k == 0:
+            solver.first_selection = solver.last_selection()
     torch.cuda.synchronize()
-    per_stage = []
+    per_step = []
     t0 = time.perf_counter()
     for _ in range(steps):
         solver.velocity_device(*dev)
-        per_stage.append(solver.stats())
+        per_step.append(solver.stats())
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
-    return dt, per_stage
+    return dt, per_step
 
 
-def accuracy(torch, solver, field, dev_in, u, v, n_targets=20000):
+def rel_l2(u, v, ur, vr):
+    return float(np.sqrt(np.sum((u - ur) ** 2 + (v - vr) ** 2) / np.sum(ur ** 2 + vr ** 2)))
+
+
+def direct_reference(torch, solver, field, dev, n_targets=20000):
     from lsmps_vpm_v2_amd import workloads
     tidx = workloads.subsample(field, n_targets)
     td = torch.from_numpy(tidx).to("cuda")
     q = torch.from_numpy(field["q"]).to("cuda")
-    ex, ey = solver.direct_device(dev_in[0], dev_in[1], q, targets=td)
-    u_ref, v_ref = ey.cpu().numpy(), -ex.cpu().numpy()
-    uu = u.cpu().numpy()[tidx] - solver.opts.u_inf
-    vv = v.cpu().numpy()[tidx] - solver.opts.v_inf
-    num = np.sum((uu - u_ref) ** 2 + (vv - v_ref) ** 2)
-    return float(np.sqrt(num / np.sum(u_ref ** 2 + v_ref ** 2))), tidx, (u_ref, v_ref)
+    ex, ey = solver.direct_device(dev[0], dev[1], q, targets=td)
+    return tidx, ey.cpu().numpy(), -ex.cpu().numpy()
 
 
 def cpu_baseline(field, reps=3):
-    """Reference FMM (oracle/_ref) on all host threads; falls back to the C
-    restatement (oracle/liboracle_fmm2d.so, single thread) if the reference
-    build is absent."""
+    """Reference FMM (oracle/_ref) on the host threads; falls back to the C
+    restatement (single thread) if the reference build is absent.  Returns
+    (baseline dict, reference field (ex, ey) or None)."""
     from oracle import oracle as O
     n = field["n"]
     cores = os.cpu_count() or 1
@@ -91,25 +104,38 @@ def cpu_baseline(field, reps=3):
         cores = int(os.environ.get("OMP_NUM_THREADS", cores))
     except ValueError:
         pass
-    times = []
+    times, ref = [], None
     if O.ref_available():
         for _ in range(reps):
             t0 = time.perf_counter()
-            O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"], threads=cores)
+            ref = O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"],
+                                  threads=cores)
             times.append(time.perf_counter() - t0)
         kind, used = "reference", cores
-        sample = (f"reference TreeCell+fmm2D built from the reference sources, "
-                  f"{n} particles (the full field), p=10, 75/50 leaf rule, median of {reps} solves")
+        sample = (f"reference TreeCell::initializeTree + fmm2D::calcField built from the "
+                  f"reference sources (g++ -O2 -fopenmp), the full {n}-particle field, p=10, "
+                  f"75/50 leaf rule, median of {reps} solves")
     else:
-        for _ in range(1):
-            t0 = time.perf_counter()
-            O.fmm_field(field["x"], field["y"], field["q"], field["active"], O.opts())
-            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        O.fmm_field(field["x"], field["y"], field["q"], field["active"], O.opts())
+        times.append(time.perf_counter() - t0)
         kind, used = "port", 1
         sample = f"C restatement, {n} particles, single thread, 1 solve"
     t = statistics.median(times)
-    return {"value": n / t, "unit": UNIT, "cores": used, "kind": kind, "sample": sample,
-            "seconds_per_solve": t}
+    out = {"value": n / t, "unit": UNIT, "cores": used, "kind": kind, "sample": sample,
+           "seconds_per_solve": t}
+    return out, (None if ref is None else (ref[0], ref[1]))
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(path, ROOT)
 
 
 def main():
@@ -117,9 +143,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", default="fp64", choices=sorted(PATHS))
+    ap.add_argument("--path", default="learned", choices=sorted(PATHS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -127,6 +157,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        a = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "copies": 5}))
+        return
     from lsmps_vpm_v2_amd import FmmSolver, workloads
 
     field = workloads.boundary_layer()
@@ -139,44 +179,94 @@ def main():
     solver = FmmSolver(timing=True, **opts)
     dt, stats = time_path(torch, solver, dev, args.steps, args.warmup)
     st = stats[-1]
+    sel = solver.last_selection()
     log(f"[bench] {args.path}: {dt * 1e3:.3f} ms/step, stages(ms) tree {st['ms_tree']:.3f} "
         f"lists {st['ms_lists']:.3f} up {st['ms_upward']:.3f} m2l {st['ms_m2l']:.3f} "
         f"down {st['ms_downward']:.3f} eval {st['ms_eval']:.3f}")
-
+    if args.path == "learned":
+        log(f"[bench] selection: source {sel['source']} threshold {sel['threshold']} "
+            f"p {sel['p'][:st['max_leaf_level'] + 1]} predicted err {sel['pred_err']:.2e} "
+            f"ms {sel['pred_ms']:.3f}")
     u, v = solver.velocity_device(*dev)
+    u_h, v_h = u.cpu().numpy() - solver.opts.u_inf, v.cpu().numpy() - solver.opts.v_inf
+
+    out_fp64 = None
+    if args.path != "fp64" and not args.no_fp64:
+        s64 = FmmSolver(timing=True, **PATHS["fp64"])
+        dt64, st64 = time_path(torch, s64, dev, args.steps, args.warmup)
+        u64, v64 = s64.velocity_device(*dev)
+        out_fp64 = {"value": n / dt64, "ms_per_step": dt64 * 1e3,
+                    "stage_ms": {k: float(np.mean([s[k] for s in st64])) for k in
+                                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward",
+                                  "ms_eval")},
+                    "config": PATHS["fp64"],
+                    "_uv": (u64.cpu().numpy() - 1.0, v64.cpu().numpy())}
+        log(f"[bench] fp64: {dt64 * 1e3:.3f} ms/step; speedup of {args.path} "
+            f"{dt64 / dt:.2f}x")
+
     err = None
     if not args.no_accuracy:
-        err, _, _ = accuracy(torch, solver, field, dev, u, v)
+        tidx, ur, vr = direct_reference(torch, solver, field, dev)
+        err = rel_l2(u_h[tidx], v_h[tidx], ur, vr)
         log(f"[bench] rel-L2 vs FP64 direct (20k targets): {err:.3e}")
+        if out_fp64:
+            uu, vv = out_fp64["_uv"]
+            out_fp64["rel_l2_vs_fp64_direct"] = rel_l2(uu[tidx], vv[tidx], ur, vr)
 
-    # dominant kernel: the evaluation kernel (P2P + M2P + L2P), vector-ALU bound
     stage_ms = {k: float(np.mean([s[k] for s in stats])) for k in
                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_eval")}
+    # dominant kernel: the evaluation kernel (P2P + M2P + L2P), vector-ALU bound
     eval_ms = stage_ms["ms_eval"]
-    flops = eval_flops(st, opts["p"])
-    peak = FP64_PEAK_TFLOPS if opts["precision"] == "fp64" else FP32_PEAK_TFLOPS
+    flops = eval_flops(st)
+    fp64 = opts["precision"] == "fp64"
+    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
     achieved = flops / (eval_ms * 1e-3) / 1e12
+    traffic, traffic_src = load_traffic(KERNEL[args.path], field["name"])
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": "k_eval64",
-                "flops_per_launch": flops, "ms_per_launch": eval_ms}
+                "frac": achieved / peak, "traffic": traffic, "kernel": KERNEL[args.path],
+                "compute_unit": "vector ALU (" + ("FP64" if fp64 else "FP32") + ")",
+                "flops_per_launch": flops, "ms_per_launch": eval_ms,
+                "timing": "HIP events on the solver stream around the kernel, mean over the "
+                          "timed steps",
+                "traffic_source": traffic_src}
 
+    cfg = {k: v for k, v in opts.items()}
     out = {
         "metric": METRIC, "value": n / dt, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if opts["precision"] == "fp64" else "f32",
-        "data": "synthetic",
-        "config": {"workload": field["name"], "n": n, "path": args.path, **opts,
+        "dtype": "f64" if fp64 else "f32",
+        "data": "synthetic (seeded flat-plate boundary-layer field)",
+        "config": {"workload": field["name"], "n": n, "path": args.path, **cfg,
                    "n_cells": st["n_cells"], "n_leaves": st["n_leaves"],
                    "leaf_levels": [st["min_leaf_level"], st["max_leaf_level"]]},
         "rel_l2_vs_fp64_direct": err,
         "stage_ms": stage_ms,
         "roofline": roofline,
     }
+    if args.path != "fp64":
+        out["selection"] = {k: sel[k] for k in ("source", "threshold", "pred_err", "pred_ms")}
+        out["selection"]["p_per_level"] = sel["p"][:st["max_leaf_level"] + 1]
+        first = getattr(solver, "first_selection", None)
+        if first and first["reselected"]:
+            # once per run (and when n changes by > 10 %): not part of a step
+            out["selection"]["threshold_choice_ms"] = first["ms_reselect"]
+    if out_fp64:
+        out["fp64_path"] = {k: val for k, val in out_fp64.items() if not k.startswith("_")}
+        out["speedup_vs_fp64"] = out["value"] / out_fp64["value"]
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(field)
-        log(f"[bench] cpu baseline {out['cpu_baseline']['seconds_per_solve']:.2f} s/solve")
+        base, ref = cpu_baseline(field)
+        out["cpu_baseline"] = base
+        log(f"[bench] cpu baseline {base['seconds_per_solve']:.3f} s/solve on {base['cores']} "
+            f"threads")
+        if ref is not None:
+            ur_all, vr_all = ref[1], -ref[0]
+            out["rel_l2_vs_reference_fmm"] = rel_l2(u_h, v_h, ur_all, vr_all)
+            if out_fp64:
+                uu, vv = out_fp64["_uv"]
+                out["fp64_path"]["rel_l2_vs_reference_fmm"] = rel_l2(uu, vv, ur_all, vr_all)
+            log(f"[bench] rel-L2 vs reference FMM: {out['rel_l2_vs_reference_fmm']:.3e}")
     print(json.dumps(out), flush=True)
 
 

@@ -61,8 +61,53 @@ typedef struct vpmfmm_opts {
     int32_t timing;      /* 1 = record per-stage times into vpmfmm_stats                */
     int32_t stage_mask;  /* evaluation parts (diagnostics): 1 P2P, 2 M2P (list 3),
                             4 L2P (far field); 0 = all                                 */
-    int32_t reserved[7];
+    int32_t order_select;   /* mixed path: vpmfmm_order_select (fixed / learned)        */
+    int32_t reselect_every; /* learned: re-choose the leaf threshold every k calls
+                               (0 = on the first call and when n changes by > 10%)      */
+    int32_t reserved[5];
 } vpmfmm_opts;
+
+/* Expansion-order source of the mixed path. */
+typedef enum vpmfmm_order_select {
+    VPMFMM_ORDER_FIXED = 0,   /* opts.p, or the table of vpmfmm_set_level_orders          */
+    VPMFMM_ORDER_LEARNED = 1  /* the MLP of vpmfmm_set_selector picks p per level and
+                                 the leaf threshold; fixed order opts.p as the fallback  */
+} vpmfmm_order_select;
+
+#define VPMFMM_SEL_FEATURES 9
+#define VPMFMM_SEL_HIDDEN 64
+#define VPMFMM_SEL_PARAMS 4930  /* 9*64 + 64 + 64*64 + 64 + 64*2 + 2 */
+
+/* Learned selection model: 9 features per tree level -> 64 tanh -> 64 tanh
+ * -> (e, t) = ln(level's share of the squared relative L2 error),
+ * ln(level's share of the solve time in ms).  Features of level L (raw, before
+ * x_norm = (x - mu) / sd): L, log2 cells, log2(particles / cell), log2 side,
+ * log10(sum|q| of the level / sum|q|), log10(max|q| N / sum|q|),
+ * leaves / cells, log2 leaf threshold, p (0 for levels 0 and 1).
+ * Parameters: W1[9][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]. */
+typedef struct vpmfmm_selector {
+    int32_t n_features;      /* VPMFMM_SEL_FEATURES                                     */
+    int32_t hidden;          /* VPMFMM_SEL_HIDDEN                                       */
+    int32_t n_params;        /* VPMFMM_SEL_PARAMS                                       */
+    int32_t p_min, p_max;    /* candidate orders: 1 <= p_min <= p_max <= 16, < 16 values */
+    int32_t n_thresholds;    /* candidate leaf thresholds (max_par = max_src), 1..8     */
+    int32_t thresholds[8];
+    float target_err;        /* relative L2 velocity error to meet, e.g. 1e-4           */
+    float margin;            /* feature excursion allowed beyond [lo, hi], in hi - lo   */
+    float mu[9], sd[9], lo[9], hi[9];
+} vpmfmm_selector;
+
+/* What the last mixed-path call used. */
+typedef struct vpmfmm_selection {
+    int32_t source;          /* 0 fixed, 1 learned, 2 learned model fell back to opts.p */
+    int32_t threshold;       /* leaf threshold of the tree (max_par; max_src)           */
+    int32_t p[17];           /* order per level 0..16                                  */
+    int32_t reselected;      /* 1 if this call re-chose the leaf threshold              */
+    float pred_err;          /* predicted relative L2 error (source 1), else -1          */
+    float pred_ms;           /* predicted solve time (source 1), else -1                 */
+    float ms_reselect;       /* time spent re-choosing the threshold in this call       */
+    int32_t pad;
+} vpmfmm_selection;
 
 typedef struct vpmfmm_stats {
     int32_t n, n_active, n_cells, n_leaves;
@@ -72,6 +117,8 @@ typedef struct vpmfmm_stats {
     int64_t m2p_evals;                             /* (particle, list-3 cell) pairs   */
     double ms_total;                               /* stage times (timing = 1)        */
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_eval;
+    int64_t m2p_terms;   /* sum over (particle, list-3 cell) of (order of the cell + 1) */
+    int64_t l2p_terms;   /* sum over particles of the order of their leaf             */
 } vpmfmm_stats;
 
 typedef struct vpmfmm_handle_s vpmfmm_handle;
@@ -161,6 +208,41 @@ int vpmfmm_set_level_orders(vpmfmm_handle *h, const int32_t *p, int32_t nlev);
  * ak/bk[n_cells * 2(p+1)] as (re, im) pairs.  b_0 is not formed (it never
  * reaches the field).  Stage-level parity hook (P2M, M2M, M2L + L2L). */
 int vpmfmm_tree_expansions(vpmfmm_handle *h, int64_t *ids, double *ak, double *bk);
+
+/* ---- learned order / tree-level selection (no counterpart in the reference,
+ * whose order is one global P_max, global.cpp:311, and whose tree depth
+ * follows the fixed 75/50 leaf rule, global.cpp:303-313) ---- */
+
+/* Installs the selection model (host pointers, copied; params has
+ * sel->n_params floats).  sel = NULL removes it: learned calls then use the
+ * fixed order. */
+int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const float *params);
+
+/* The orders, threshold and predictions used by the last field/velocity call. */
+int vpmfmm_last_selection(vpmfmm_handle *h, vpmfmm_selection *out);
+
+/* Feature rows of the tree of the last call (order column 0), levels 0..16:
+ * feats[17 * 9]; raw[17 * 5] = cells, particles, sum|q|, max|q|, leaves per
+ * level; *n_levels = deepest level + 1.  Builds the training data. */
+int vpmfmm_level_features(vpmfmm_handle *h, float *feats, double *raw, int32_t *n_levels);
+
+/* MLP kernels on the handle's stream; every pointer is a device pointer.
+ * x[rows * 9] are normalised feature rows, params[VPMFMM_SEL_PARAMS].
+ * forward: out[rows * 2] = (e, t).
+ * loss_grad: solves s = 0..nseg-1 own rows seg[s] .. seg[s+1]-1 (non-empty),
+ *   y[2s] = ln(err^2), y[2s+1] = ln(ms); loss = mean_s (LSE_s(e) - y[2s])^2
+ *   + w_time (LSE_s(t) - y[2s+1])^2 into *loss, its gradient into grad.
+ * train: `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, cosine learning rate
+ *   from lr to lr_min, bias correction counted from step0 + 1) updating params
+ *   and the moments m, v in place; loss_hist[steps] receives each step's loss. */
+int vpmfmm_mlp_forward(vpmfmm_handle *h, const float *params, const float *x, int32_t rows,
+                       float *out);
+int vpmfmm_mlp_loss_grad(vpmfmm_handle *h, const float *params, const float *x, int32_t rows,
+                         const int32_t *seg, int32_t nseg, const float *y, float w_time,
+                         float *grad, float *loss);
+int vpmfmm_mlp_train(vpmfmm_handle *h, float *params, float *m, float *v, const float *x,
+                     int32_t rows, const int32_t *seg, int32_t nseg, const float *y, float w_time,
+                     int32_t steps, int32_t step0, float lr, float lr_min, float *loss_hist);
 
 #ifdef __cplusplus
 }

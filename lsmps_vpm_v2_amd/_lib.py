@@ -17,7 +17,9 @@ EXPORTS = (
     "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_direct_device",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
-    "vpmfmm_tree_expansions", "vpmfmm_set_level_orders",
+    "vpmfmm_tree_expansions", "vpmfmm_set_level_orders", "vpmfmm_set_selector",
+    "vpmfmm_last_selection", "vpmfmm_level_features", "vpmfmm_mlp_forward",
+    "vpmfmm_mlp_loss_grad", "vpmfmm_mlp_train",
 )
 
 STATUS = {
@@ -27,6 +29,8 @@ STATUS = {
 }
 
 FP64, MIXED = 0, 1
+ORDER_FIXED, ORDER_LEARNED = 0, 1
+SEL_FEATURES, SEL_HIDDEN, SEL_PARAMS = 9, 64, 4930
 
 
 class Opts(C.Structure):
@@ -35,8 +39,32 @@ class Opts(C.Structure):
         ("exp_tree", C.c_double), ("min_level", C.c_int32), ("max_level", C.c_int32),
         ("precision", C.c_int32), ("binom_mode", C.c_int32), ("list4_mode", C.c_int32),
         ("u_inf", C.c_double), ("v_inf", C.c_double), ("device", C.c_int32),
-        ("timing", C.c_int32), ("stage_mask", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("timing", C.c_int32), ("stage_mask", C.c_int32), ("order_select", C.c_int32),
+        ("reselect_every", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
+
+
+class Selector(C.Structure):
+    _fields_ = [
+        ("n_features", C.c_int32), ("hidden", C.c_int32), ("n_params", C.c_int32),
+        ("p_min", C.c_int32), ("p_max", C.c_int32), ("n_thresholds", C.c_int32),
+        ("thresholds", C.c_int32 * 8), ("target_err", C.c_float), ("margin", C.c_float),
+        ("mu", C.c_float * 9), ("sd", C.c_float * 9), ("lo", C.c_float * 9),
+        ("hi", C.c_float * 9),
+    ]
+
+
+class Selection(C.Structure):
+    _fields_ = [
+        ("source", C.c_int32), ("threshold", C.c_int32), ("p", C.c_int32 * 17),
+        ("reselected", C.c_int32), ("pred_err", C.c_float), ("pred_ms", C.c_float),
+        ("ms_reselect", C.c_float), ("pad", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"source": self.source, "threshold": self.threshold, "p": list(self.p),
+                "reselected": self.reselected, "pred_err": self.pred_err,
+                "pred_ms": self.pred_ms, "ms_reselect": self.ms_reselect}
 
 
 class Stats(C.Structure):
@@ -48,6 +76,7 @@ class Stats(C.Structure):
         ("ms_total", C.c_double),
         ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_upward", C.c_double),
         ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_eval", C.c_double),
+        ("m2p_terms", C.c_int64), ("l2p_terms", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -105,6 +134,14 @@ def load() -> C.CDLL:
                                     C.c_int32]
     L.vpmfmm_tree_expansions.argtypes = [_vp, _lp, _dp, _dp]
     L.vpmfmm_set_level_orders.argtypes = [_vp, _ip, C.c_int32]
+    L.vpmfmm_set_selector.argtypes = [_vp, C.POINTER(Selector), _vp]
+    L.vpmfmm_last_selection.argtypes = [_vp, C.POINTER(Selection)]
+    L.vpmfmm_level_features.argtypes = [_vp, _vp, _vp, _ip]
+    L.vpmfmm_mlp_forward.argtypes = [_vp, _vp, _vp, C.c_int32, _vp]
+    L.vpmfmm_mlp_loss_grad.argtypes = [_vp, _vp, _vp, C.c_int32, _vp, C.c_int32, _vp, C.c_float,
+                                       _vp, _vp]
+    L.vpmfmm_mlp_train.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32, _vp,
+                                   C.c_float, C.c_int32, C.c_int32, C.c_float, C.c_float, _vp]
     _lib = L
     return L
 

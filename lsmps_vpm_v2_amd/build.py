@@ -39,6 +39,7 @@ SOURCES = {
     "tree.hip": ["-ffp-contract=off"],
     "fmm64.hip": [],
     "fmm32.hip": [],
+    "select.hip": [],
     "api.cpp": [],
 }
 

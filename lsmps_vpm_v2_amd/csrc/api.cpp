@@ -1,6 +1,7 @@
 // api.cpp -- C ABI (include/vpmfmm.h): handle, option checks, stage
 // orchestration, host staging and tree inspection.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -9,6 +10,7 @@
 #include "../../include/vpmfmm.h"
 #include "ctx.hpp"
 #include "fmm32.hpp"
+#include "select.hpp"
 
 namespace vpm {
 
@@ -99,7 +101,19 @@ struct vpmfmm_handle_s {
     int exp_p = -1;  // order of the expansions held on the device (-1: none)
     std::vector<int> level_p;  // per-level orders of the mixed path (empty: opts.p everywhere)
     int mx_pmax = -1;          // order the mixed tables were built for
+    const float *mx_rptr_host[17] = {};  // device addresses of the M2L tables per order
     hipEvent_t ev[8] = {};
+    // learned selection
+    bool has_sel = false;
+    vpmfmm_selector sel{};
+    DBuf sel_model_d, sel_params_d;
+    int sel_threshold = 0;   // leaf threshold chosen by the last re-selection (0: none)
+    int sel_n = 0;           // particle count at the last re-selection
+    int sel_calls = 0;       // calls since the last re-selection
+    vpmfmm_selection last_sel{};
+    uint8_t h_orders[kMaxDepth + 2] = {};  // order table copied into the device header
+    MlpWork mlp;
+    DBuf mlp_grad;
     // inspection cache of the last tree (host copies)
     bool insp = false;
     std::vector<uint8_t> h_level, h_flags;
@@ -126,6 +140,9 @@ int opts_check(const vpmfmm_opts &o, std::string &msg) {
     else if (o.binom_mode != 0 && o.binom_mode != 1) msg = "binom_mode must be 0 or 1";
     else if (o.list4_mode != 1 && o.list4_mode != 2) msg = "list4_mode must be 1 or 2";
     else if (o.stage_mask < 0 || o.stage_mask > 7) msg = "stage_mask must be in [0, 7]";
+    else if (o.order_select != VPMFMM_ORDER_FIXED && o.order_select != VPMFMM_ORDER_LEARNED)
+        msg = "order_select must be 0 (fixed) or 1 (learned)";
+    else if (o.reselect_every < 0) msg = "reselect_every must be >= 0";
     else return VPMFMM_OK;
     return VPMFMM_BAD_ARG;
 }
@@ -171,6 +188,51 @@ float span(vpmfmm_handle *h, int a, int b) {
     return ms;
 }
 
+bool learned(const vpmfmm_handle *h) {
+    return h->o.precision == VPMFMM_MIXED && h->o.order_select == VPMFMM_ORDER_LEARNED &&
+           h->has_sel;
+}
+
+TreeParams tree_params(const vpmfmm_handle *h, int threshold) {
+    TreeParams tp{h->o.max_par, h->o.max_src, h->o.exp_tree, h->o.min_level, h->o.max_level};
+    if (threshold > 0) tp.max_par = tp.max_src = threshold;
+    return tp;
+}
+
+// Features of the tree just built and the selected orders, into the header
+// (read back at the next header synchronisation).
+void select_on_tree(vpmfmm_handle *h, int threshold) {
+    Ctx &c = h->ctx;
+    float *feats = c.sel_feats.as<float>((kMaxDepth + 1) * kSelFeatures);
+    float *raw = c.sel_raw.as<float>((kMaxDepth + 1) * kSelRaw);
+    level_features(c, threshold, feats, raw);
+    select_orders(c, (const SelModel *)h->sel_model_d.p, (const float *)h->sel_params_d.p, feats,
+                  h->o.p);
+}
+
+// Learned path: choose the leaf threshold among the model's candidates by the
+// predicted solve time of each candidate tree (orders selected per tree).
+void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *y,
+                        const double *q, const uint8_t *a) {
+    Ctx &c = h->ctx;
+    float best = INFINITY;
+    int best_t = 0;
+    for (int k = 0; k < h->sel.n_thresholds; ++k) {
+        const int t = h->sel.thresholds[k];
+        tree_build(c, n, x, y, q, a, tree_params(h, t));
+        select_on_tree(h, t);
+        VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        if (c.hdr_h->sel_source == 1 && c.hdr_h->sel_pred_ms < best) {
+            best = c.hdr_h->sel_pred_ms;
+            best_t = t;
+        }
+    }
+    h->sel_threshold = best_t;  // 0: no candidate met the target (fixed options)
+    h->sel_n = n;
+    h->sel_calls = 0;
+}
+
 void build(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
            const uint8_t *a) {
     std::string msg;
@@ -180,12 +242,42 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     VPM_HIP(hipSetDevice(h->o.device));
     h->insp = false;
     h->exp_p = -1;
-    TreeParams tp{h->o.max_par, h->o.max_src, h->o.exp_tree, h->o.min_level, h->o.max_level};
+    const bool lrn = learned(h);
+    h->last_sel = vpmfmm_selection{};
+    if (lrn) {
+        const bool due = h->sel_n == 0 || std::abs(n - h->sel_n) * 10 > h->sel_n ||
+                         (h->o.reselect_every > 0 && h->sel_calls >= h->o.reselect_every);
+        if (due) {
+            const auto t0 = std::chrono::steady_clock::now();
+            reselect_threshold(h, n, x, y, q, a);
+            h->last_sel.reselected = 1;
+            h->last_sel.ms_reselect = (float)std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - t0)
+                                          .count();
+        }
+        ++h->sel_calls;
+    }
+    const int thr = lrn ? h->sel_threshold : 0;
+    const TreeParams tp = tree_params(h, thr);
     mark(h, 0);
     tree_build(h->ctx, n, x, y, q, a, tp);
+    if (lrn) {
+        select_on_tree(h, tp.max_par);
+    } else {
+        // orders per level for the evaluation statistics of the list build
+        for (int L = 0; L < kMaxDepth + 2; ++L) {
+            int p = h->o.p;
+            if (h->o.precision == VPMFMM_MIXED && L < (int)h->level_p.size() && h->level_p[L] > 0)
+                p = h->level_p[L];
+            h->h_orders[L] = (uint8_t)std::min(p, 255);
+        }
+        VPM_HIP(hipMemcpyAsync(h->ctx.hdr_d->sel_p, h->h_orders, sizeof(h->h_orders),
+                               hipMemcpyHostToDevice, h->ctx.stream));
+    }
     mark(h, 1);
     tree_lists(h->ctx);
     mark(h, 2);
+    h->last_sel.threshold = tp.max_par;
 }
 
 void finish_stats(vpmfmm_handle *h, bool fmm) {
@@ -207,6 +299,8 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
     s.n_list4 = c.hdr_h->sum4;
     s.p2p_pairs = (int64_t)c.hdr_h->pairs;
     s.m2p_evals = (int64_t)c.hdr_h->m2p;
+    s.m2p_terms = (int64_t)c.hdr_h->m2p_terms;
+    s.l2p_terms = (int64_t)c.hdr_h->l2p_terms;
     if (h->o.timing) {
         s.ms_tree = span(h, 0, 1);
         s.ms_lists = span(h, 1, 2);
@@ -231,9 +325,11 @@ Mixed32 mixed_setup(vpmfmm_handle *h) {
     const TreeInfo &ti = c.info;
     LevelTab lt{};
     int pmax = 0;
+    const bool lrn = learned(h);
     for (int L = 0; L < kMaxDepth + 2; ++L) {
         int p = h->o.p;
-        if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
+        if (lrn) p = c.hdr_h->sel_p[L];
+        else if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
         if (p > 16) throw Error(kBadArg, "the mixed path supports expansion orders up to 16");
         lt.p[L] = p;
         const double s = ti.root_len / std::ldexp(1.0, L);
@@ -242,7 +338,7 @@ Mixed32 mixed_setup(vpmfmm_handle *h) {
         if (L <= ti.max_level) pmax = std::max(pmax, p);
     }
     Mixed32 mx;
-    if (h->mx_pmax != pmax) {
+    if (h->mx_pmax < pmax) {
         std::vector<float> m2m, l2l, tm2l;
         std::vector<std::vector<float>> rl;
         mixed_tables(pmax, m2m, l2l, tm2l, rl);
@@ -265,40 +361,36 @@ Mixed32 mixed_setup(vpmfmm_handle *h) {
                                        hipMemcpyHostToDevice, c.stream));
             off += rl[p].size();
         }
-        VPM_HIP(hipMemcpyAsync(c.mx_rptr.get(ptrs.size() * sizeof(void *)), ptrs.data(),
-                               ptrs.size() * sizeof(void *), hipMemcpyHostToDevice, c.stream));
+        for (size_t p = 0; p < 17; ++p) h->mx_rptr_host[p] = p < ptrs.size() ? ptrs[p] : nullptr;
         VPM_HIP(hipStreamSynchronize(c.stream));
         h->mx_pmax = pmax;
     }
-    // tiles of 16 consecutive cells per level (levels >= 2), grouped by the
-    // number of 16-row MFMA blocks the level's order needs
-    std::vector<int2> t1, t2;
-    for (int L = 2; L <= ti.max_level; ++L) {
-        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
-        std::vector<int2> &dst = (2 * lt.p[L] <= 16) ? t1 : t2;
-        for (int j = j0; j < j1; j += 16) dst.push_back(make_int2(j, std::min(16, j1 - j)));
+    // tiles of 16 consecutive cells per level (levels >= 2), grouped by order
+    std::vector<int2> tl;
+    for (int p = 1; p <= 16; ++p) {
+        mx.tile_off[p] = (int)tl.size();
+        for (int L = 2; L <= ti.max_level; ++L) {
+            if (lt.p[L] != p) continue;
+            const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+            for (int j = j0; j < j1; j += 16) tl.push_back(make_int2(j, std::min(16, j1 - j)));
+        }
     }
-    int2 *td = c.mx_tiles.as<int2>(t1.size() + t2.size() + 1);
-    if (!t1.empty())
-        VPM_HIP(hipMemcpyAsync(td, t1.data(), t1.size() * sizeof(int2), hipMemcpyHostToDevice,
+    mx.tile_off[17] = (int)tl.size();
+    int2 *td = c.mx_tiles.as<int2>(tl.size() + 1);
+    if (!tl.empty())
+        VPM_HIP(hipMemcpyAsync(td, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice,
                                c.stream));
-    if (!t2.empty())
-        VPM_HIP(hipMemcpyAsync(td + t1.size(), t2.data(), t2.size() * sizeof(int2),
-                               hipMemcpyHostToDevice, c.stream));
     LevelTab *ltd = c.mx_lt.as<LevelTab>(1);
     VPM_HIP(hipMemcpyAsync(ltd, &lt, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
     // the host vectors above must outlive the copies
     VPM_HIP(hipStreamSynchronize(c.stream));
-    mx.pmax = pmax;
+    mx.pmax = h->mx_pmax;  // tables (and coefficient stride) of the largest order built so far
     mx.lt = ltd;
     mx.m2m = (const float *)c.mx_m2m.p;
     mx.l2l = (const float *)c.mx_l2l.p;
     mx.tm2l = (const float *)c.mx_tm2l.p;
-    mx.rtab = (const float *const *)c.mx_rptr.p;
-    mx.tiles1 = td;
-    mx.ntiles1 = (int)t1.size();
-    mx.tiles2 = td + t1.size();
-    mx.ntiles2 = (int)t2.size();
+    for (int p = 0; p <= 16; ++p) mx.rptr_host[p] = h->mx_rptr_host[p];
+    mx.tiles = td;
     return mx;
 }
 
@@ -309,11 +401,27 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double
     Ctx &c = h->ctx;
     FmmMarks mk{h->o.timing ? h->ev : nullptr};
     const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
+    vpmfmm_selection &ls = h->last_sel;
     if (h->o.precision == VPMFMM_MIXED) {
         Mixed32 mx = mixed_setup(h);
+        if (learned(h)) {
+            ls.source = (int)c.hdr_h->sel_source;
+            ls.pred_err = c.hdr_h->sel_pred_err;
+            ls.pred_ms = c.hdr_h->sel_pred_ms;
+        } else {
+            ls.pred_err = ls.pred_ms = -1.f;
+        }
+        for (int L = 0; L <= kMaxDepth; ++L) {
+            int p = h->o.p;
+            if (learned(h)) p = c.hdr_h->sel_p[L];
+            else if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
+            ls.p[L] = p;
+        }
         fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
     } else {
         ensure_tables(h);
+        ls.pred_err = ls.pred_ms = -1.f;
+        for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = h->o.p;
         fmm64_run(c, h->o.p, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf,
                   mk);
         h->exp_p = h->o.p;
@@ -694,6 +802,139 @@ int vpmfmm_set_level_orders(vpmfmm_handle *h, const int32_t *p, int32_t nlev) {
         for (int x : v)
             if (x < 0 || x > 16) throw Error(kBadArg, "per-level orders must be in [0, 16]");
         h->level_p = v;
+    });
+}
+
+int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const float *params) {
+    return guarded(h, [&] {
+        if (!sel) {
+            h->has_sel = false;
+            h->sel_threshold = h->sel_n = h->sel_calls = 0;
+            return;
+        }
+        const vpmfmm_selector &m = *sel;
+        if (m.n_features != kSelFeatures || m.hidden != kSelHidden || m.n_params != kSelParams)
+            throw Error(kBadArg, "selector shape must be 9 features, 64 hidden, 4930 parameters");
+        if (m.p_min < 1 || m.p_max > 16 || m.p_min > m.p_max || m.p_max - m.p_min >= 16)
+            throw Error(kBadArg, "selector orders must satisfy 1 <= p_min <= p_max <= 16");
+        if (m.n_thresholds < 1 || m.n_thresholds > 8)
+            throw Error(kBadArg, "selector needs 1..8 leaf thresholds");
+        for (int k = 0; k < m.n_thresholds; ++k)
+            if (m.thresholds[k] < 1 || m.thresholds[k] > kMaxThreshold)
+                throw Error(kBadArg, "selector thresholds must be in [1, 4096]");
+        if (!(m.target_err > 0.f) || !std::isfinite(m.target_err) || !(m.margin >= 0.f))
+            throw Error(kBadArg, "selector target_err must be > 0 and margin >= 0");
+        if (!params) throw Error(kBadArg, "null pointer");
+        for (int i = 0; i < kSelFeatures; ++i)
+            if (!(m.sd[i] > 0.f) || !std::isfinite(m.mu[i]) || !std::isfinite(m.lo[i]) ||
+                !std::isfinite(m.hi[i]) || m.lo[i] > m.hi[i])
+                throw Error(kBadArg, "selector normalisation / range is invalid");
+        for (int k = 0; k < kSelParams; ++k)
+            if (!std::isfinite(params[k])) throw Error(kNonFinite, "non-finite selector parameter");
+        SelModel sm{};
+        for (int i = 0; i < kSelFeatures; ++i) {
+            sm.mu[i] = m.mu[i];
+            sm.sd[i] = m.sd[i];
+            sm.lo[i] = m.lo[i];
+            sm.hi[i] = m.hi[i];
+        }
+        sm.margin = m.margin;
+        sm.target_err = m.target_err;
+        sm.p_min = m.p_min;
+        sm.p_max = m.p_max;
+        VPM_HIP(hipSetDevice(h->o.device));
+        Ctx &c = h->ctx;
+        VPM_HIP(hipMemcpyAsync(h->sel_model_d.get(sizeof(SelModel)), &sm, sizeof(SelModel),
+                               hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(h->sel_params_d.get(sizeof(float) * kSelParams), params,
+                               sizeof(float) * kSelParams, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        h->sel = m;
+        h->has_sel = true;
+        h->sel_threshold = h->sel_n = h->sel_calls = 0;
+    });
+}
+
+int vpmfmm_last_selection(vpmfmm_handle *h, vpmfmm_selection *out) {
+    return guarded(h, [&] {
+        if (!out) throw Error(kBadArg, "null pointer");
+        *out = h->last_sel;
+    });
+}
+
+int vpmfmm_level_features(vpmfmm_handle *h, float *feats, double *raw, int32_t *n_levels) {
+    return guarded(h, [&] {
+        Ctx &c = h->ctx;
+        if (c.info.n_cells == 0) throw Error(kBadArg, "no tree has been built on this handle");
+        VPM_HIP(hipSetDevice(h->o.device));
+        const int thr = h->last_sel.threshold > 0 ? h->last_sel.threshold : h->o.max_par;
+        float *fd = c.sel_feats.as<float>((kMaxDepth + 1) * kSelFeatures);
+        float *rd = c.sel_raw.as<float>((kMaxDepth + 1) * kSelRaw);
+        level_features(c, thr, fd, rd);
+        std::vector<float> f((kMaxDepth + 1) * kSelFeatures), r((kMaxDepth + 1) * kSelRaw);
+        VPM_HIP(hipMemcpyAsync(f.data(), fd, f.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipMemcpyAsync(r.data(), rd, r.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        if (feats) std::memcpy(feats, f.data(), f.size() * 4);
+        if (raw)
+            for (size_t k = 0; k < r.size(); ++k) raw[k] = r[k];
+        if (n_levels) *n_levels = c.info.max_level + 1;
+    });
+}
+
+namespace {
+void check_segments(vpmfmm_handle *h, const int32_t *seg, int32_t nseg, int32_t rows) {
+    if (nseg < 1 || rows < 1 || !seg) throw Error(kBadArg, "need nseg >= 1 and rows >= 1");
+    std::vector<int32_t> hs(nseg + 1);
+    VPM_HIP(hipMemcpyAsync(hs.data(), seg, hs.size() * 4, hipMemcpyDeviceToHost, h->ctx.stream));
+    VPM_HIP(hipStreamSynchronize(h->ctx.stream));
+    if (hs[0] != 0 || hs[nseg] != rows) throw Error(kBadArg, "seg must run from 0 to rows");
+    for (int s = 0; s < nseg; ++s)
+        if (hs[s + 1] <= hs[s]) throw Error(kBadArg, "every solve needs at least one row");
+}
+}  // namespace
+
+int vpmfmm_mlp_forward(vpmfmm_handle *h, const float *params, const float *x, int32_t rows,
+                       float *out) {
+    return guarded(h, [&] {
+        if (!params || !x || !out || rows < 0) throw Error(kBadArg, "null pointer or rows < 0");
+        VPM_HIP(hipSetDevice(h->o.device));
+        mlp_forward(h->ctx.stream, params, x, rows, out, nullptr, nullptr);
+        VPM_HIP(hipStreamSynchronize(h->ctx.stream));
+    });
+}
+
+int vpmfmm_mlp_loss_grad(vpmfmm_handle *h, const float *params, const float *x, int32_t rows,
+                         const int32_t *seg, int32_t nseg, const float *y, float w_time,
+                         float *grad, float *loss) {
+    return guarded(h, [&] {
+        if (!params || !x || !y || !grad || !loss) throw Error(kBadArg, "null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        check_segments(h, seg, nseg, rows);
+        mlp_loss_grad(h->ctx.stream, params, x, rows, seg, nseg, y, w_time, h->mlp, grad, loss);
+        VPM_HIP(hipStreamSynchronize(h->ctx.stream));
+    });
+}
+
+int vpmfmm_mlp_train(vpmfmm_handle *h, float *params, float *m, float *v, const float *x,
+                     int32_t rows, const int32_t *seg, int32_t nseg, const float *y, float w_time,
+                     int32_t steps, int32_t step0, float lr, float lr_min, float *loss_hist) {
+    return guarded(h, [&] {
+        if (!params || !m || !v || !x || !y || !loss_hist) throw Error(kBadArg, "null pointer");
+        if (steps < 1 || step0 < 0 || !(lr > 0.f) || !(lr_min >= 0.f))
+            throw Error(kBadArg, "need steps >= 1, step0 >= 0, lr > 0, lr_min >= 0");
+        VPM_HIP(hipSetDevice(h->o.device));
+        check_segments(h, seg, nseg, rows);
+        hipStream_t s = h->ctx.stream;
+        float *grad = h->mlp_grad.as<float>(kSelParams);
+        const double pi = 3.14159265358979323846;
+        for (int k = 0; k < steps; ++k) {
+            const float lr_k = lr_min + 0.5f * (lr - lr_min) *
+                                            (float)(1.0 + std::cos(pi * (double)k / (double)steps));
+            mlp_loss_grad(s, params, x, rows, seg, nseg, y, w_time, h->mlp, grad, loss_hist + k);
+            mlp_adam(s, params, grad, m, v, kSelParams, step0 + k + 1, lr_k, 0.9f, 0.999f, 1e-8f);
+        }
+        VPM_HIP(hipStreamSynchronize(s));
     });
 }
 

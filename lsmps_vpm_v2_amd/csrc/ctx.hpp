@@ -34,7 +34,13 @@ struct DevHdr {
     uint32_t lev_off[kMaxDepth + 2];
     uint32_t tot1, tot3;
     uint32_t leaf_lvl_min, leaf_lvl_max;
-    uint32_t pad[4];
+    uint8_t sel_p[kMaxDepth + 2];  // selected order per level
+    uint8_t sel_pad[2];
+    uint32_t sel_source;           // 0 fixed, 1 learned, 2 fallback to the fixed order
+    float sel_pred_err, sel_pred_ms;
+    unsigned long long m2p_terms;  // sum over (particle, list-3 cell) of (order of the cell + 1)
+    unsigned long long l2p_terms;  // sum over particles of the order of their leaf
+    uint32_t pad[2];
 };
 
 // Grow-only device buffer.
@@ -97,8 +103,9 @@ struct Ctx {
 
     // mixed path: FP32 particle offsets and sources, tables, tiles
     DBuf oxy, qf;
+    DBuf sel_part, sel_feats, sel_raw;  // level statistics of the selection
     DBuf part;  // per-block partial results of the tree and list kernels
-    DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_rptr, mx_tiles;
+    DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_tiles;
 
     TreeInfo info;
     std::vector<double> stage_ms;

@@ -133,14 +133,17 @@ __global__ void __launch_bounds__(256)
     stc(ahat, (size_t)j * PM1 + m, acc);
 }
 
-// M2L over list 2 on MFMA.  Tile = 16 consecutive cells of one level.
-// Lane l: column n = l % 16 (target), k-quarter kq = l / 16.  A operand
-// (translation matrix) is read from a table laid out in lane order.
-template <int MT>
+// M2L over list 2 on MFMA.  Tile = 16 consecutive cells of one level (all of
+// order P).  Lane l: column n = l % 16 (target), k-quarter kq = l / 16.  The
+// realified translation D[2(m-1)+{0,1}][2k+{0,1}] of each of the 40 offsets
+// (|dx|,|dy| <= 3, not both <= 1) is the MFMA A operand, read from a table in
+// lane order; the B operand is the source cell's scaled multipole (a^_k re/im
+// along K = 2(P+1)).  An offset no target of the tile uses is skipped.
+template <int P>
 __global__ void __launch_bounds__(256)
-    k_m2l_mfma(const int2 *tiles, int ntiles, const uint8_t *c_level, const uint32_t *c_key,
-               const int *l2, const uint8_t *n2, const LevelTab *lt, const float *const *rtab,
-               const float *ahat, int kst, float *bhat) {
+    k_m2l_mfma(const int2 *tiles, int ntiles, const uint32_t *c_key, const int *l2,
+               const uint8_t *n2, const float *R, const float *ahat, int kst, float *bhat) {
+    constexpr int K = 2 * (P + 1), KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
     __shared__ int smap[4][16][49];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int tile = blockIdx.x * 4 + w;
@@ -166,11 +169,6 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if (!live) return;
-    const int L = c_level[j0];
-    const int p = lt->p[L];
-    const int K = 2 * (p + 1);
-    const int KS = (K + 3) / 4;
-    const float *R = rtab[p];
     v4f acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -179,27 +177,48 @@ __global__ void __launch_bounds__(256)
         const int dx = slot % 7 - 3, dy = slot / 7 - 3;
         if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
         const int sidx = smap[w][n][slot];
+        const int oc = o++;
+        if (__ballot(sidx >= 0) == 0) continue;  // no target of the tile uses this offset
         const float *src = ahat + (size_t)(sidx < 0 ? 0 : sidx) * kst;
+        float b[KS];
+#pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int k = ks * 4 + kq;
-            const float b = (sidx >= 0 && k < K) ? src[k] : 0.f;
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const float a = R[(((size_t)o * KS + ks) * MT + mt) * kWave + lane];
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[mt], 0, 0, 0);
-            }
+            b[ks] = (sidx >= 0 && k < K) ? src[k] : 0.f;
         }
-        ++o;
+        const float *Ro = R + (size_t)oc * KS * MT * kWave + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ro[(ks * MT + mt) * kWave], b[ks],
+                                                               acc[mt], 0, 0, 0);
     }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = mt * 16 + 4 * kq + v;  // row of (Re b^_1, Im b^_1, Re b^_2, ...)
-            if (n < cnt && row < 2 * p) bhat[(size_t)j * kst + 2 + row] = acc[mt][v];
+            if (n < cnt && row < 2 * P) bhat[(size_t)j * kst + 2 + row] = acc[mt][v];
         }
     }
 }
+
+template <int P>
+void launch_m2l(const int2 *tiles, int ntiles, const uint32_t *c_key, const int *l2,
+                const uint8_t *n2, const float *R, const float *ahat, int kst, float *bhat,
+                hipStream_t s) {
+    k_m2l_mfma<P><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_key, l2, n2, R, ahat, kst,
+                                                     bhat);
+}
+
+typedef void (*M2lLaunch)(const int2 *, int, const uint32_t *, const int *, const uint8_t *,
+                          const float *, const float *, int, float *, hipStream_t);
+const M2lLaunch kM2l[17] = {nullptr,        launch_m2l<1>,  launch_m2l<2>,  launch_m2l<3>,
+                            launch_m2l<4>,  launch_m2l<5>,  launch_m2l<6>,  launch_m2l<7>,
+                            launch_m2l<8>,  launch_m2l<9>,  launch_m2l<10>, launch_m2l<11>,
+                            launch_m2l<12>, launch_m2l<13>, launch_m2l<14>, launch_m2l<15>,
+                            launch_m2l<16>};
 
 // List 4, mode 2: P2L from the active particles of coarser leaves,
 //   b^_m += -q/m (s_t / (z_j - z_t))^m.
@@ -560,21 +579,12 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
             j0, j1, c_level, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
     }
     mk.mark(3, s);
-    // M2L list 2 on MFMA, one launch per row-tile count
-    for (int mt = 1; mt <= 2; ++mt) {
-        const int ntiles = mt == 1 ? mx.ntiles1 : mx.ntiles2;
-        const int2 *tiles = mt == 1 ? mx.tiles1 : mx.tiles2;
-        if (ntiles <= 0) continue;
-        if (mt == 1)
-            k_m2l_mfma<1><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_level, c_key,
-                                                            (const int *)c.l2.p,
-                                                            (const uint8_t *)c.n2.p, mx.lt,
-                                                            mx.rtab, ahat, kst, bhat);
-        else
-            k_m2l_mfma<2><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_level, c_key,
-                                                            (const int *)c.l2.p,
-                                                            (const uint8_t *)c.n2.p, mx.lt,
-                                                            mx.rtab, ahat, kst, bhat);
+    // M2L list 2 on MFMA, one launch per order present
+    for (int p = 1; p <= 16; ++p) {
+        const int t0 = mx.tile_off[p], t1 = mx.tile_off[p + 1];
+        if (t1 > t0)
+            kM2l[p](mx.tiles + t0, t1 - t0, c_key, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
+                    mx.rptr_host[p], ahat, kst, bhat, s);
     }
     const int j2 = (int)ti.lev_off[2];
     if (nc > j2) {

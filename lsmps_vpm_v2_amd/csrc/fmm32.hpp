@@ -19,9 +19,9 @@ struct Mixed32 {
     int pmax = 0;                      // coefficient storage is 2 (pmax + 1) floats per cell
     const LevelTab *lt = nullptr;
     const float *m2m = nullptr, *l2l = nullptr, *tm2l = nullptr;
-    const float *const *rtab = nullptr;  // rtab[p]: MFMA-ordered M2L matrices for order p
-    const int2 *tiles1 = nullptr, *tiles2 = nullptr;  // (first cell, count) of 16-cell tiles
-    int ntiles1 = 0, ntiles2 = 0;      // tiles with one / two 16-row MFMA blocks (2p <= 16 / 32)
+    const float *rptr_host[17] = {};   // device pointer of the MFMA-ordered M2L matrices per p
+    const int2 *tiles = nullptr;       // (first cell, count) of 16-cell tiles, grouped by order
+    int tile_off[18] = {};             // tiles of order p: tile_off[p] .. tile_off[p + 1] - 1
 };
 
 // Host builders of the level-independent tables (see fmm32.hip).

@@ -1,8 +1,9 @@
 // prim.hip -- exclusive scans and a stable LSD radix sort for gfx950.
 //
-// Scans: reduce -> spine -> down-sweep over 2048-element tiles (256 threads x
-// 8 items), wave64 shuffles inside a tile.  Radix sort: 8-bit digits over
-// 4096-element tiles; per-block digit histograms, a scan of the digit-major
+// Scans: tile reduce -> down-sweep over 2048-element tiles (256 threads x
+// 8 items), wave64 shuffles inside a tile; the down-sweep block sums the tile
+// totals before it itself (up to kFusedSpine tiles), larger scans add a spine
+// launch.  Radix sort: 8-bit digits over 1024-element tiles; per-block digit histograms, a scan of the digit-major
 // histogram table, then a stable scatter whose in-tile ranks come from
 // 64-lane ballot "peer masks" (lanes of a wavefront holding the same digit).
 #include "prim.hpp"
@@ -15,7 +16,8 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 constexpr int kRsBlock = 256;   // = number of digit values
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 4;
+constexpr int kFusedSpine = 2048;  // tiles up to which the down-sweep forms its own prefix
 constexpr int kRsTile = kRsBlock * kRsItems;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -101,6 +103,39 @@ __global__ void __launch_bounds__(kScanBlock)
     if (write_end && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = *total;
 }
 
+// Down-sweep that forms its tile prefix itself: block b sums bsum[0..b).
+template <typename T>
+__global__ void __launch_bounds__(kScanBlock)
+    k_scan_down_fused(const T *in, uint32_t *out, int n, const uint32_t *bsum, uint32_t *total,
+                      int write_end) {
+    __shared__ uint32_t sm[kScanBlock / kWave];
+    uint32_t pre = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanBlock) pre += bsum[b];
+    uint32_t pre_all;
+    block_excl_scan<kScanBlock>(pre, sm, pre_all);
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        v[j] = i < n ? (uint32_t)in[i] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t run = pre_all + block_excl_scan<kScanBlock>(s, sm, tot);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        if (total) *total = pre_all + tot;
+        if (write_end) out[n] = pre_all + tot;
+    }
+}
+
 __global__ void k_write_zero_total(uint32_t *out, int n, uint32_t *total, int write_end) {
     if (total) *total = 0;
     if (write_end) out[n] = 0;
@@ -116,8 +151,13 @@ void scan_impl(const T *in, uint32_t *out, int n, uint32_t *total, bool write_en
     }
     const int nb = div_up(n, kScanTile);
     uint32_t *bsum = tmp;
-    uint32_t *tot = total ? total : tmp + nb;
     k_scan_reduce<T><<<nb, kScanBlock, 0, s>>>(in, n, bsum);
+    if (nb <= kFusedSpine) {
+        k_scan_down_fused<T><<<nb, kScanBlock, 0, s>>>(in, out, n, bsum, total, write_end ? 1 : 0);
+        VPM_LAUNCH_CHECK();
+        return;
+    }
+    uint32_t *tot = total ? total : tmp + nb;
     k_scan_spine<<<1, 1024, 0, s>>>(bsum, nb, tot);
     k_scan_down<T><<<nb, kScanBlock, 0, s>>>(in, out, n, bsum, tot, write_end ? 1 : 0);
     VPM_LAUNCH_CHECK();

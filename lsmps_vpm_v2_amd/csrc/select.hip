@@ -1,0 +1,519 @@
+// select.hip -- learned choice of the expansion order per level and of the
+// leaf threshold (tree level) for the mixed-precision path.
+//
+// Per-level statistics of a built tree (particle count, cell count, box size,
+// sum |q|, max |q|, leaf fraction) become a feature row per level; a small MLP
+// (9 -> 64 tanh -> 64 tanh -> 2) maps a row plus a candidate order p to two
+// numbers: e = ln of the level's contribution to the squared relative L2
+// velocity error and t = ln of its contribution to the solve time (ms).  A
+// solve's predicted error^2 / time are the sums of exp(e) / exp(t) over its
+// levels.  The orders are chosen by minimising predicted time subject to
+// predicted error <= target (Lagrange multiplier, bisection), in one block on
+// the device, right after the tree build.  A fixed order (12 by default) is
+// the fallback whenever the model is absent, a feature lies outside the
+// training range, or no order choice meets the target.
+//
+// The same kernels train the network: forward (thread per row), per-solve
+// loss (thread per solve), backward (thread per row), weight gradients (the
+// 64 x 64 layer on FP32 MFMA 16x16x4, block per row chunk, partials reduced
+// per parameter) and Adam.
+#include "select.hpp"
+
+namespace vpm {
+namespace {
+
+constexpr int F = kSelFeatures, H = kSelHidden;
+// parameter offsets: W1[F][H], b1[H], W2[H][H], b2[H], W3[H][2], b3[2]
+constexpr int oW1 = 0, ob1 = oW1 + F * H, oW2 = ob1 + H, ob2 = oW2 + H * H, oW3 = ob2 + H,
+              ob3 = oW3 + 2 * H;
+static_assert(ob3 + 2 == kSelParams, "parameter layout");
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// level statistics of the built tree
+// ---------------------------------------------------------------------------
+// Per block: LDS accumulators per level of (particles, sum |q|, max |q|) over
+// the particles of its leaves, added to every level 0..leaf level.
+constexpr int kStatBlock = 256;
+constexpr int kLv = kMaxDepth + 1;
+constexpr unsigned kLeafBit = 1u;  // cell flag: leaf
+constexpr int kFeatBlock = 16 * (kLv + 1);  // 16 threads per level
+
+__global__ void __launch_bounds__(kStatBlock)
+    k_level_stats(const uint32_t *leaf_idx, int nl, const uint8_t *c_level,
+                  const uint32_t *c_begin, const uint32_t *c_end, const double *qs,
+                  float *part) {
+    __shared__ float s_n[kLv], s_sum[kLv], s_max[kLv];
+    if (threadIdx.x < kLv) {
+        s_n[threadIdx.x] = 0.f;
+        s_sum[threadIdx.x] = 0.f;
+        s_max[threadIdx.x] = 0.f;
+    }
+    __syncthreads();
+    const int t = blockIdx.x * kStatBlock + threadIdx.x;
+    if (t < nl) {
+        const int j = (int)leaf_idx[t];
+        const int L = c_level[j];
+        float sum = 0.f, mx = 0.f;
+        for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
+            const float a = (float)fabs(qs[i]);
+            sum += a;
+            mx = fmaxf(mx, a);
+        }
+        const float cnt = (float)(c_end[j] - c_begin[j]);
+        for (int l = 0; l <= L; ++l) {
+            atomicAdd(&s_n[l], cnt);
+            atomicAdd(&s_sum[l], sum);
+            // non-negative floats order like their bit patterns
+            atomicMax(reinterpret_cast<unsigned *>(&s_max[l]), __float_as_uint(mx));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kLv) {
+        float *o = part + (size_t)blockIdx.x * 3 * kLv;
+        o[threadIdx.x] = s_n[threadIdx.x];
+        o[kLv + threadIdx.x] = s_sum[threadIdx.x];
+        o[2 * kLv + threadIdx.x] = s_max[threadIdx.x];
+    }
+}
+
+// Reduce the block partials and form the raw feature rows (p column = 0):
+//   f0 L, f1 log2 cells, f2 log2(particles / cells), f3 log2(side),
+//   f4 log10(sum|q|_L / sum|q|_0), f5 log10(max|q|_L N / sum|q|_0),
+//   f6 leaves / cells, f7 log2(leaf threshold), f8 p.
+// Levels with no cell get a row of zeros and nrows = deepest level + 1.
+__global__ void __launch_bounds__(kFeatBlock)
+    k_level_feats(const float *part, int nblk, const DevHdr *hdr, const uint8_t *c_level,
+                  const uint8_t *c_flags, int threshold, float *raw, float *feats) {
+    const int L = threadIdx.x / 16, sub = threadIdx.x % 16;  // 16 threads per level
+    float n = 0.f, sum = 0.f, mx = 0.f, leaves = 0.f;
+    if (L < kLv) {
+        for (int b = sub; b < nblk; b += 16) {
+            const float *o = part + (size_t)b * 3 * kLv;
+            n += o[L];
+            sum += o[kLv + L];
+            mx = fmaxf(mx, o[2 * kLv + L]);
+        }
+        const int j0 = (int)hdr->lev_off[L], j1 = (int)hdr->lev_off[L + 1];
+        for (int j = j0 + sub; j < j1; j += 16) leaves += (c_flags[j] & kLeafBit) ? 1.f : 0.f;
+    }
+    for (int m = 1; m < 16; m <<= 1) {
+        n += __shfl_xor(n, m, kWave);
+        sum += __shfl_xor(sum, m, kWave);
+        mx = fmaxf(mx, __shfl_xor(mx, m, kWave));
+        leaves += __shfl_xor(leaves, m, kWave);
+    }
+    __shared__ float sn[kLv], ss[kLv], sm[kLv], sl[kLv];
+    if (L < kLv && sub == 0) {
+        sn[L] = n;
+        ss[L] = sum;
+        sm[L] = mx;
+        sl[L] = leaves;
+    }
+    __syncthreads();
+    if (threadIdx.x >= kLv) return;
+    const int l = threadIdx.x;
+    const float cells = (float)(hdr->lev_off[l + 1] - hdr->lev_off[l]);
+    float *r = raw + (size_t)l * kSelRaw;
+    r[0] = cells;
+    r[1] = sn[l];
+    r[2] = ss[l];
+    r[3] = sm[l];
+    r[4] = sl[l];
+    float *f = feats + (size_t)l * F;
+    if (cells <= 0.f) {
+        for (int k = 0; k < F; ++k) f[k] = 0.f;
+        return;
+    }
+    const float sum0 = ss[0] > 0.f ? ss[0] : 1.f;
+    const float ntot = sn[0] > 0.f ? sn[0] : 1.f;
+    f[0] = (float)l;
+    f[1] = log2f(cells);
+    f[2] = log2f(fmaxf(sn[l], 1.f) / cells);
+    f[3] = (float)(log2(hdr->root_len) - (double)l);
+    f[4] = log10f(fmaxf(ss[l] / sum0, 1e-12f));
+    f[5] = log10f(fmaxf(sm[l] * ntot / sum0, 1e-12f));
+    f[6] = sl[l] / cells;
+    f[7] = log2f((float)threshold);
+    f[8] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// MLP
+// ---------------------------------------------------------------------------
+// One row through the network.  The first layer's activations stay in
+// registers (fully unrolled); the second layer is a runtime loop over its
+// units that feeds the output layer directly, so no register array is indexed
+// at run time.  h1o / h2o (global, may be null) receive the activations.
+__device__ __forceinline__ void mlp_eval(const float *P, const float *x, float *h1o, float *h2o,
+                                         float &o0, float &o1) {
+    float h1[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        float a = P[ob1 + j];
+#pragma unroll
+        for (int i = 0; i < F; ++i) a += x[i] * P[oW1 + i * H + j];
+        h1[j] = tanhf(a);
+    }
+    if (h1o) {
+#pragma unroll
+        for (int j = 0; j < H; ++j) h1o[j] = h1[j];
+    }
+    o0 = P[ob3];
+    o1 = P[ob3 + 1];
+    for (int j = 0; j < H; ++j) {
+        float a = P[ob2 + j];
+#pragma unroll
+        for (int i = 0; i < H; ++i) a += h1[i] * P[oW2 + i * H + j];
+        const float h = tanhf(a);
+        if (h2o) h2o[j] = h;
+        o0 += h * P[oW3 + 2 * j];
+        o1 += h * P[oW3 + 2 * j + 1];
+    }
+}
+
+__device__ __forceinline__ void load_params(const float *params, float *sp) {
+    for (int k = threadIdx.x; k < kSelParams; k += blockDim.x) sp[k] = params[k];
+    __syncthreads();
+}
+
+// Forward: thread per row; x normalised, out[r] = (e, t); optionally keeps the
+// hidden activations for the backward pass.
+__global__ void __launch_bounds__(256)
+    k_mlp_forward(const float *params, const float *x, int rows, float *out, float *h1g,
+                  float *h2g) {
+    __shared__ float sp[kSelParams];
+    load_params(params, sp);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float xr[F], o0, o1;
+#pragma unroll
+    for (int i = 0; i < F; ++i) xr[i] = x[(size_t)r * F + i];
+    mlp_eval(sp, xr, h1g ? h1g + (size_t)r * H : nullptr, h2g ? h2g + (size_t)r * H : nullptr,
+             o0, o1);
+    out[2 * (size_t)r] = o0;
+    out[2 * (size_t)r + 1] = o1;
+}
+
+// Per-solve loss (thread per solve s, rows seg[s] .. seg[s+1]):
+//   L_s = (LSE(e) - y_err)^2 + w (LSE(t) - y_time)^2, loss = mean_s L_s,
+//   dout[r] = (2 (LSE(e) - y_err) softmax(e)_r, 2 w (LSE(t) - y_time) softmax(t)_r) / S.
+__global__ void __launch_bounds__(256)
+    k_mlp_loss(const float *out, const int *seg, int nseg, const float *y, float w_time,
+               float *dout, float *part) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    float ls = 0.f;
+    if (s < nseg) {
+        const int r0 = seg[s], r1 = seg[s + 1];
+        float me = -INFINITY, mt = -INFINITY;
+        for (int r = r0; r < r1; ++r) {
+            me = fmaxf(me, out[2 * r]);
+            mt = fmaxf(mt, out[2 * r + 1]);
+        }
+        float se = 0.f, st = 0.f;
+        for (int r = r0; r < r1; ++r) {
+            se += expf(out[2 * r] - me);
+            st += expf(out[2 * r + 1] - mt);
+        }
+        const float lse_e = me + logf(se), lse_t = mt + logf(st);
+        const float de = lse_e - y[2 * s], dt = lse_t - y[2 * s + 1];
+        ls = de * de + w_time * dt * dt;
+        const float ge = 2.f * de / (float)nseg, gt = 2.f * w_time * dt / (float)nseg;
+        for (int r = r0; r < r1; ++r) {
+            dout[2 * r] = ge * expf(out[2 * r] - lse_e);
+            dout[2 * r + 1] = gt * expf(out[2 * r + 1] - lse_t);
+        }
+    }
+    // block sum of the per-solve losses
+    for (int m = 1; m < kWave; m <<= 1) ls += __shfl_xor(ls, m, kWave);
+    __shared__ float sw[4];
+    if ((threadIdx.x & (kWave - 1)) == 0) sw[threadIdx.x / kWave] = ls;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+}
+
+// Backward through the activations: thread per row.
+//   dh2 = (W3 dout) (1 - h2^2),  dh1 = (W2 dh2) (1 - h1^2)
+__global__ void __launch_bounds__(256)
+    k_mlp_backward(const float *params, const float *dout, const float *h1g, const float *h2g,
+                   int rows, float *dh1g, float *dh2g) {
+    __shared__ float sp[kSelParams];
+    load_params(params, sp);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float d0 = dout[2 * (size_t)r], d1 = dout[2 * (size_t)r + 1];
+    float dh2[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const float h = h2g[(size_t)r * H + i];
+        dh2[i] = (sp[oW3 + 2 * i] * d0 + sp[oW3 + 2 * i + 1] * d1) * (1.f - h * h);
+        dh2g[(size_t)r * H + i] = dh2[i];
+    }
+    for (int i = 0; i < H; ++i) {
+        float a = 0.f;
+#pragma unroll
+        for (int j = 0; j < H; ++j) a += sp[oW2 + i * H + j] * dh2[j];
+        const float h = h1g[(size_t)r * H + i];
+        dh1g[(size_t)r * H + i] = a * (1.f - h * h);
+    }
+}
+
+// Weight gradients of one chunk of kChunk rows, written as a partial row of
+// kSelParams floats.  dW2 = H1^T dH2 on MFMA: wave w owns output tiles
+// (ti, tj) = (w, 0..3) of the 64 x 64 result, K = rows of the chunk.
+// v_mfma_f32_16x16x4f32 lane layout: A[l % 16][l / 16], B[l / 16][l % 16],
+// D register v at lane l is D[4 (l / 16) + v][l % 16].
+constexpr int kChunk = 256;
+
+__global__ void __launch_bounds__(256)
+    k_mlp_wgrad(const float *x, const float *dout, const float *h1g, const float *h2g,
+                const float *dh1g, const float *dh2g, int rows, float *part) {
+    const int r0 = blockIdx.x * kChunk;
+    const int nr = min(kChunk, rows - r0);
+    float *o = part + (size_t)blockIdx.x * kSelParams;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    // dW2 tiles
+    {
+        const int col = lane % 16, kk = lane / 16;
+        v4f acc[4];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[tj] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < nr; k0 += 4) {
+            const int r = r0 + k0 + kk;
+            const bool ok = k0 + kk < nr;
+            const float a = ok ? h1g[(size_t)r * H + 16 * w + col] : 0.f;
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const float b = ok ? dh2g[(size_t)r * H + 16 * tj + col] : 0.f;
+                acc[tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[tj], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * w + 4 * kk + v, j = 16 * tj + col;
+                o[oW2 + i * H + j] = acc[tj][v];
+            }
+    }
+    // the small blocks: thread per parameter, loop over the chunk
+    for (int k = threadIdx.x; k < kSelParams - H * H; k += blockDim.x) {
+        int pidx;
+        float a = 0.f;
+        if (k < F * H) {  // W1[i][j]
+            const int i = k / H, j = k % H;
+            pidx = oW1 + k;
+            for (int r = r0; r < r0 + nr; ++r) a += x[(size_t)r * F + i] * dh1g[(size_t)r * H + j];
+        } else if (k < F * H + H) {  // b1
+            const int j = k - F * H;
+            pidx = ob1 + j;
+            for (int r = r0; r < r0 + nr; ++r) a += dh1g[(size_t)r * H + j];
+        } else if (k < F * H + 2 * H) {  // b2
+            const int j = k - F * H - H;
+            pidx = ob2 + j;
+            for (int r = r0; r < r0 + nr; ++r) a += dh2g[(size_t)r * H + j];
+        } else if (k < F * H + 4 * H) {  // W3[i][c]
+            const int q = k - F * H - 2 * H, i = q / 2, c = q % 2;
+            pidx = oW3 + q;
+            for (int r = r0; r < r0 + nr; ++r) a += h2g[(size_t)r * H + i] * dout[2 * (size_t)r + c];
+        } else {  // b3
+            const int c = k - F * H - 4 * H;
+            pidx = ob3 + c;
+            for (int r = r0; r < r0 + nr; ++r) a += dout[2 * (size_t)r + c];
+        }
+        o[pidx] = a;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_sum_parts(const float *part, int nparts, int width, float *outv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    float a = 0.f;
+    for (int b = 0; b < nparts; ++b) a += part[(size_t)b * width + k];
+    outv[k] = a;
+}
+
+__global__ void __launch_bounds__(256)
+    k_adam(float *params, const float *grad, float *m, float *v, int n, float lr, float b1,
+           float b2, float eps, float c1, float c2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float g = grad[k];
+    const float mk = b1 * m[k] + (1.f - b1) * g;
+    const float vk = b2 * v[k] + (1.f - b2) * g * g;
+    m[k] = mk;
+    v[k] = vk;
+    params[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// selection (one block)
+// ---------------------------------------------------------------------------
+constexpr int kSelBlock = 256;
+constexpr int kMaxCand = 16;  // orders p_min .. p_min + 15
+
+__global__ void __launch_bounds__(kSelBlock)
+    k_select(const SelModel *model, const float *params, const float *feats, const DevHdr *hdr,
+             int fixed_p, DevHdr *hdr_w) {
+    __shared__ float sp[kSelParams];
+    __shared__ float se[kLv][kMaxCand], st[kLv][kMaxCand];
+    __shared__ int s_bad;
+    load_params(params, sp);
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    const SelModel &M = *model;
+    const int maxL = (int)hdr->leaf_lvl_max;
+    const int nrow = maxL + 1;
+    const int np = M.p_max - M.p_min + 1;
+    // range check of every row (the order column excluded)
+    for (int k = threadIdx.x; k < nrow * (F - 1); k += blockDim.x) {
+        const int L = k / (F - 1), i = k % (F - 1);
+        const float v = feats[L * F + i];
+        const float span = M.hi[i] - M.lo[i];
+        if (v < M.lo[i] - M.margin * span || v > M.hi[i] + M.margin * span) s_bad = 1;
+    }
+    // predictions for every (level, order); levels 0 and 1 use order 0
+    for (int k = threadIdx.x; k < nrow * np; k += blockDim.x) {
+        const int L = k / np, pi = k % np;
+        const int p = L < 2 ? 0 : M.p_min + pi;
+        if (L < 2 && pi > 0) continue;
+        float xr[F], o0, o1;
+#pragma unroll
+        for (int i = 0; i < F; ++i) {
+            const float raw = i == F - 1 ? (float)p : feats[L * F + i];
+            xr[i] = (raw - M.mu[i]) / M.sd[i];
+        }
+        mlp_eval(sp, xr, nullptr, nullptr, o0, o1);
+        se[L][pi] = o0;
+        st[L][pi] = o1;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const float tgt = 2.f * logf(M.target_err);
+    // constant part: levels 0 and 1
+    float c_err = 0.f, c_time = 0.f;
+    for (int L = 0; L < min(2, nrow); ++L) {
+        c_err += expf(se[L][0]);
+        c_time += expf(st[L][0]);
+    }
+    int choice[kLv];
+    auto pick = [&](float lam, float &err2, float &time) {
+        err2 = c_err;
+        time = c_time;
+        for (int L = 2; L < nrow; ++L) {
+            int best = 0;
+            float bv = INFINITY;
+            for (int pi = 0; pi < np; ++pi) {
+                const float v = expf(st[L][pi]) + expf(lam + se[L][pi]);
+                if (v < bv) {
+                    bv = v;
+                    best = pi;
+                }
+            }
+            choice[L] = best;
+            err2 += expf(se[L][best]);
+            time += expf(st[L][best]);
+        }
+    };
+    float err2, time;
+    int source = 1;
+    // feasibility at the largest multiplier
+    float hi = 80.f, lo = -10.f;
+    pick(hi, err2, time);
+    if (s_bad || !(logf(err2) <= tgt)) {
+        source = 2;
+    } else {
+        for (int it = 0; it < 60; ++it) {
+            const float mid = 0.5f * (lo + hi);
+            pick(mid, err2, time);
+            if (logf(err2) <= tgt) hi = mid;
+            else lo = mid;
+        }
+        pick(hi, err2, time);
+    }
+    for (int L = 0; L < kMaxDepth + 2; ++L) {
+        int p = fixed_p;
+        if (source == 1 && L >= 2 && L < nrow) p = M.p_min + choice[L];
+        else if (source == 1 && L < 2 && nrow > 2) p = M.p_min + choice[2];
+        hdr_w->sel_p[L] = (uint8_t)p;
+    }
+    hdr_w->sel_source = (uint32_t)source;
+    hdr_w->sel_pred_err = source == 1 ? sqrtf(err2) : -1.f;
+    hdr_w->sel_pred_ms = source == 1 ? time : -1.f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void level_features(Ctx &c, int threshold, float *feats_dev, float *raw_dev) {
+    hipStream_t s = c.stream;
+    const TreeInfo &ti = c.info;
+    const int nl = ti.n_leaves;
+    const int nb = div_up(nl, kStatBlock);
+    float *part = c.sel_part.as<float>((size_t)std::max(nb, 1) * 3 * kLv);
+    if (nb > 0)
+        k_level_stats<<<nb, kStatBlock, 0, s>>>((const uint32_t *)c.leaf_idx.p, nl,
+                                                (const uint8_t *)c.c_level.p,
+                                                (const uint32_t *)c.c_begin.p,
+                                                (const uint32_t *)c.c_end.p,
+                                                (const double *)c.qs.p, part);
+    k_level_feats<<<1, kFeatBlock, 0, s>>>(part, nb, c.hdr_d, (const uint8_t *)c.c_level.p,
+                                         (const uint8_t *)c.c_flags.p, threshold, raw_dev,
+                                         feats_dev);
+    VPM_LAUNCH_CHECK();
+}
+
+void select_orders(Ctx &c, const SelModel *model_dev, const float *params_dev,
+                   const float *feats_dev, int fixed_p) {
+    k_select<<<1, kSelBlock, 0, c.stream>>>(model_dev, params_dev, feats_dev, c.hdr_d, fixed_p,
+                                            c.hdr_d);
+    VPM_LAUNCH_CHECK();
+}
+
+void mlp_forward(hipStream_t s, const float *params, const float *x, int rows, float *out,
+                 float *h1, float *h2) {
+    if (rows <= 0) return;
+    k_mlp_forward<<<div_up(rows, 256), 256, 0, s>>>(params, x, rows, out, h1, h2);
+    VPM_LAUNCH_CHECK();
+}
+
+void mlp_loss_grad(hipStream_t s, const float *params, const float *x, int rows, const int *seg,
+                   int nseg, const float *y, float w_time, MlpWork &wk, float *grad,
+                   float *loss) {
+    wk.reserve(rows, nseg);
+    mlp_forward(s, params, x, rows, wk.out, wk.h1, wk.h2);
+    const int nbl = div_up(nseg, 256);
+    k_mlp_loss<<<nbl, 256, 0, s>>>(wk.out, seg, nseg, y, w_time, wk.dout, wk.lpart);
+    k_mlp_backward<<<div_up(rows, 256), 256, 0, s>>>(params, wk.dout, wk.h1, wk.h2, rows, wk.dh1,
+                                                     wk.dh2);
+    const int nch = div_up(rows, kChunk);
+    k_mlp_wgrad<<<nch, 256, 0, s>>>(x, wk.dout, wk.h1, wk.h2, wk.dh1, wk.dh2, rows, wk.gpart);
+    k_sum_parts<<<div_up(kSelParams, 256), 256, 0, s>>>(wk.gpart, nch, kSelParams, grad);
+    k_sum_parts<<<1, 256, 0, s>>>(wk.lpart, nbl, 1, loss);
+    VPM_LAUNCH_CHECK();
+}
+
+void mlp_adam(hipStream_t s, float *params, const float *grad, float *m, float *v, int n,
+              int step, float lr, float b1, float b2, float eps) {
+    const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+    k_adam<<<div_up(n, 256), 256, 0, s>>>(params, grad, m, v, n, lr, b1, b2, eps, c1, c2);
+    VPM_LAUNCH_CHECK();
+}
+
+void MlpWork::reserve(int rows, int nseg) {
+    const size_t r = (size_t)std::max(rows, 1);
+    out = b_out.as<float>(2 * r);
+    dout = b_dout.as<float>(2 * r);
+    h1 = b_h1.as<float>(r * kSelHidden);
+    h2 = b_h2.as<float>(r * kSelHidden);
+    dh1 = b_dh1.as<float>(r * kSelHidden);
+    dh2 = b_dh2.as<float>(r * kSelHidden);
+    gpart = b_gpart.as<float>((size_t)div_up((int)r, kChunk) * kSelParams);
+    lpart = b_lpart.as<float>((size_t)div_up(std::max(nseg, 1), 256));
+}
+
+}  // namespace vpm

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256)
 // Per-block partial results (two 64-bit words per block) and their one-block
 // reduction.  Global atomics on one header word from thousands of blocks
 // serialize across the chiplets; partials plus one small reduction do not.
-enum PartOp { kPartSplit = 0, kPartM2lLists = 1, kPartLeafLists = 2 };
+enum PartOp { kPartSplit = 0, kPartM2lLists = 1, kPartLeafLists = 2, kPartLeafTerms = 3 };
 
 // Block-wide (sum or min/max) of two 64-bit values; thread 0 writes part[blk].
 template <bool MINMAX>
@@ -297,9 +297,12 @@ __global__ void __launch_bounds__(256)
     } else if (op == kPartM2lLists) {
         hdr->sum2 = (uint32_t)a;
         hdr->sum4 = (uint32_t)b;
-    } else {
+    } else if (op == kPartLeafLists) {
         hdr->pairs = a;
         hdr->m2p = b;
+    } else {
+        hdr->m2p_terms = a;
+        hdr->l2p_terms = b;
     }
 }
 
@@ -676,10 +679,11 @@ __global__ void __launch_bounds__(256)
                  const double *c_cx, const double *c_cy, const int *c_child, const int *nbr,
                  const DevHdr *hdr, uint32_t *cnt1, uint32_t *cnt3, const uint32_t *off1,
                  const uint32_t *off3, int *l1, int *l3, const uint32_t *c_begin,
-                 const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part) {
+                 const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part, ulonglong2 *part_terms) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long my_pairs = 0, my_m2p = 0;
+    unsigned long long my_pairs = 0, my_m2p = 0, my_m2p_terms = 0, my_l2p_terms = 0;
+    unsigned long long m2p_w = 0;  // sum over list-3 cells of (order of the cell's level + 1)
     bool overflow = false;
     if (t < nl) {
         const int j = (int)leaf_idx[t];
@@ -739,7 +743,10 @@ __global__ void __launch_bounds__(256)
                     else if (sp < 64) stack[sp++] = ch;
                     else overflow = true;
                 } else {
-                    if (FILL) l3[o3 + n3] = ch;
+                    if (FILL) {
+                        l3[o3 + n3] = ch;
+                        m2p_w += (unsigned long long)hdr->sel_p[c_level[ch]] + 1ull;
+                    }
                     ++n3;
                 }
             }
@@ -751,10 +758,16 @@ __global__ void __launch_bounds__(256)
             const unsigned long long nt = c_end[j] - c_begin[j];
             my_pairs = src_total * nt - (self ? nt : 0ull);
             my_m2p = (unsigned long long)n3 * nt;
+            my_m2p_terms = m2p_w * nt;
+            my_l2p_terms = (unsigned long long)hdr->sel_p[L] * nt;
         }
     }
     if (overflow) atomicOr(&hdr_w->flags, kFlagListOverflow);
-    if (FILL) block_part<false>(my_pairs, my_m2p, part);
+    if (FILL) {
+        block_part<false>(my_pairs, my_m2p, part);
+        __syncthreads();  // block_part's LDS is reused
+        block_part<false>(my_m2p_terms, my_l2p_terms, part_terms);
+    }
 }
 
 }  // namespace
@@ -893,7 +906,8 @@ void tree_lists(Ctx &c) {
     VPM_HIP(hipMemsetAsync(n2, 0, nc, s));
     VPM_HIP(hipMemsetAsync(n4, 0, nc, s));
     const int nb2 = nc > j0 ? div_up(nc - j0, 256) : 0;
-    ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, div_up(nl, 256)), 1));
+    const int nbl = div_up(nl, 256);
+    ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, 2 * nbl), 1));
     if (nb2 > 0)
         k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
                                         c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d, part);
@@ -905,7 +919,7 @@ void tree_lists(Ctx &c) {
     k_leaf_lists<false><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
                                                          c_cy, c_child, nbr, c.hdr_d, cnt1, cnt3,
                                                          nullptr, nullptr, nullptr, nullptr,
-                                                         c_begin, c_end, c.hdr_d, nullptr);
+                                                         c_begin, c_end, c.hdr_d, nullptr, nullptr);
     scan_exclusive_u32(cnt1, off1, nl, &c.hdr_d->tot1, true, tmp, s);
     scan_exclusive_u32(cnt3, off3, nl, &c.hdr_d->tot3, true, tmp, s);
     VPM_LAUNCH_CHECK();
@@ -917,8 +931,9 @@ void tree_lists(Ctx &c) {
     k_leaf_lists<true><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
                                                         c_cy, c_child, nbr, c.hdr_d, nullptr,
                                                         nullptr, off1, off3, l1, l3, c_begin, c_end,
-                                                        c.hdr_d, part);
-    k_reduce_part<<<1, 256, 0, s>>>(part, div_up(nl, 256), kPartLeafLists, c.hdr_d);
+                                                        c.hdr_d, part, part + nbl);
+    k_reduce_part<<<1, 256, 0, s>>>(part, nbl, kPartLeafLists, c.hdr_d);
+    k_reduce_part<<<1, 256, 0, s>>>(part + nbl, nbl, kPartLeafTerms, c.hdr_d);
     VPM_LAUNCH_CHECK();
 }
 

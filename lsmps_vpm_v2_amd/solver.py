@@ -23,9 +23,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import FP64, MIXED, Opts, Stats, VpmFmmError
+from ._lib import FP64, MIXED, ORDER_FIXED, ORDER_LEARNED, Opts, Stats, VpmFmmError
 
 _PRECISION = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+_ORDER = {"fixed": ORDER_FIXED, "learned": ORDER_LEARNED, ORDER_FIXED: ORDER_FIXED,
+          ORDER_LEARNED: ORDER_LEARNED}
 
 
 def _f64(a):
@@ -49,7 +51,12 @@ class FmmSolver:
 
     def __init__(self, p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
                  precision="fp64", binom_mode=1, list4_mode=1, u_inf=1.0, v_inf=0.0,
-                 device=0, timing=False, stage_mask=0):
+                 device=0, timing=False, stage_mask=0, order="fixed", reselect_every=0,
+                 selector=None):
+        """order = "learned" (mixed path only) installs ``selector`` (a
+        ``selector.Model`` or a model file path; None: the packaged model) and
+        lets it pick p per level and the leaf threshold, with ``p`` as the
+        fixed-order fallback."""
         self._L = _lib.load()
         o = _lib.default_opts()
         o.p, o.max_par, o.max_src, o.exp_tree = p, max_par, max_src, exp_tree
@@ -58,12 +65,18 @@ class FmmSolver:
         o.binom_mode, o.list4_mode = binom_mode, list4_mode
         o.u_inf, o.v_inf, o.device, o.timing = u_inf, v_inf, device, 1 if timing else 0
         o.stage_mask = stage_mask
+        o.order_select = _ORDER[order]
+        o.reselect_every = reselect_every
         self._o = o
         h = C.c_void_p()
         st = self._L.vpmfmm_create(C.byref(o), C.byref(h))
         if st != 0:
             raise VpmFmmError(st, "vpmfmm_create", "invalid options or device")
         self._h = h
+        self._stream = None  # torch stream the handle runs on (created at the first device call)
+        if o.order_select == ORDER_LEARNED:
+            from . import selector as _sel
+            self.set_selector(_sel.load_model() if selector is None else selector)
 
     # ------------------------------------------------------------------ utils
     def _check(self, st, where):
@@ -104,10 +117,82 @@ class FmmSolver:
             st = self._L.vpmfmm_set_level_orders(self._h, a.ctypes.data_as(_lib._ip), len(a))
         self._check(st, "vpmfmm_set_level_orders")
 
+    def set_selector(self, model):
+        """Install a learned selection model (``selector.Model`` or file path;
+        None removes it)."""
+        if model is None:
+            self._check(self._L.vpmfmm_set_selector(self._h, None, None), "vpmfmm_set_selector")
+            return
+        from . import selector as _sel
+        if isinstance(model, str):
+            model = _sel.Model.load(model)
+        sel = model.to_struct()
+        params = np.ascontiguousarray(model.params, dtype=np.float32)
+        self._check(self._L.vpmfmm_set_selector(self._h, C.byref(sel), _ptr(params)),
+                    "vpmfmm_set_selector")
+
+    def last_selection(self) -> dict:
+        """Orders, leaf threshold and predictions of the last mixed-path call."""
+        s = _lib.Selection()
+        self._check(self._L.vpmfmm_last_selection(self._h, C.byref(s)), "vpmfmm_last_selection")
+        return s.as_dict()
+
+    def level_features(self):
+        """(feats[n_levels, 9] float32 with order column 0, raw[n_levels, 5])
+        of the last tree."""
+        f = np.zeros((17, _lib.SEL_FEATURES), np.float32)
+        r = np.zeros((17, 5), np.float64)
+        n = C.c_int32()
+        self._check(self._L.vpmfmm_level_features(self._h, _ptr(f), _ptr(r), C.byref(n)),
+                    "vpmfmm_level_features")
+        return f[: n.value].copy(), r[: n.value].copy()
+
+    # MLP kernels (device tensors, float32 / int32, contiguous)
+    def mlp_forward(self, params, x):
+        import torch
+        out = torch.empty((x.shape[0], 2), dtype=torch.float32, device=x.device)
+        self._enter(x)
+        self._check(self._L.vpmfmm_mlp_forward(self._h, _dptr(params), _dptr(x), x.shape[0],
+                                               _dptr(out)), "vpmfmm_mlp_forward")
+        return out
+
+    def mlp_loss_grad(self, params, x, seg, y, w_time):
+        import torch
+        grad = torch.empty_like(params)
+        loss = torch.empty(1, dtype=torch.float32, device=params.device)
+        self._enter(params)
+        self._check(self._L.vpmfmm_mlp_loss_grad(self._h, _dptr(params), _dptr(x), x.shape[0],
+                                                 _dptr(seg), seg.shape[0] - 1, _dptr(y),
+                                                 float(w_time), _dptr(grad), _dptr(loss)),
+                    "vpmfmm_mlp_loss_grad")
+        return float(loss.item()), grad
+
+    def mlp_train(self, params, m, v, x, seg, y, w_time, steps, step0=0, lr=3e-3, lr_min=1e-4):
+        """Adam steps on the device, in place; returns the loss of every step."""
+        import torch
+        hist = torch.empty(steps, dtype=torch.float32, device=params.device)
+        self._enter(params)
+        self._check(self._L.vpmfmm_mlp_train(self._h, _dptr(params), _dptr(m), _dptr(v),
+                                             _dptr(x), x.shape[0], _dptr(seg), seg.shape[0] - 1,
+                                             _dptr(y), float(w_time), int(steps), int(step0),
+                                             float(lr), float(lr_min), _dptr(hist)),
+                    "vpmfmm_mlp_train")
+        return hist.cpu().numpy()
+
     def set_stream(self, stream):
         """Run on a torch.cuda.Stream (None: the handle's own stream)."""
         ptr = None if stream is None else C.c_void_p(int(stream.cuda_stream))
         self._check(self._L.vpmfmm_set_stream(self._h, ptr), "vpmfmm_set_stream")
+        self._stream = stream
+
+    def _enter(self, t):
+        """Order the handle's stream after the work torch has queued for the
+        device tensors (the calls themselves return with their results
+        complete)."""
+        import torch
+        if self._stream is None:
+            self.set_stream(torch.cuda.Stream(device=t.device))
+        self._stream.wait_stream(torch.cuda.current_stream(t.device))
 
     def stats(self) -> dict:
         s = Stats()
@@ -141,6 +226,7 @@ class FmmSolver:
             ex = x.new_empty(n)
         if ey is None:
             ey = x.new_empty(n)
+        self._enter(x)
         self._check(self._L.vpmfmm_field_device(self._h, n, _dptr(x), _dptr(y), _dptr(q),
                                                 _dptr(active), _dptr(ex), _dptr(ey)),
                     "vpmfmm_field_device")
@@ -152,6 +238,7 @@ class FmmSolver:
             u = x.new_empty(n)
         if v is None:
             v = x.new_empty(n)
+        self._enter(x)
         self._check(self._L.vpmfmm_velocity_device(self._h, n, _dptr(x), _dptr(y), _dptr(s),
                                                    _dptr(omega), _dptr(active), _dptr(u),
                                                    _dptr(v)), "vpmfmm_velocity_device")
@@ -162,6 +249,7 @@ class FmmSolver:
         n = x.numel()
         nt = n if targets is None else targets.numel()
         ex, ey = x.new_empty(nt), x.new_empty(nt)
+        self._enter(x)
         self._check(self._L.vpmfmm_direct_device(
             self._h, n, _dptr(x), _dptr(y), _dptr(q),
             None if src_mask is None else _dptr(src_mask), nt,

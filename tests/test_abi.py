@@ -53,19 +53,21 @@ def test_default_options_are_the_reference_values():
     assert (o.u_inf, o.v_inf) == (1.0, 0.0)
     assert (o.min_level, o.max_level, o.binom_mode, o.list4_mode) == (1, 0, 1, 1)
     assert o.precision == _lib.FP64 and o.stage_mask == 0
+    assert o.order_select == _lib.ORDER_FIXED and o.reselect_every == 0
 
 
 def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof from the C header (compiled with gcc) equal the ctypes view."""
     _lib = _ensure_built()
     probe = tmp_path / "probe.c"
-    fields_o = [f for f, _ in _lib.Opts._fields_]
-    fields_s = [f for f, _ in _lib.Stats._fields_]
+    structs = [("vpmfmm_opts", _lib.Opts), ("vpmfmm_stats", _lib.Stats),
+               ("vpmfmm_selector", _lib.Selector), ("vpmfmm_selection", _lib.Selection)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vpmfmm.h"',
-             "int main(void) {",
-             'printf("%zu %zu\\n", sizeof(vpmfmm_opts), sizeof(vpmfmm_stats));']
-    lines += [f'printf("%zu\\n", offsetof(vpmfmm_opts, {f}));' for f in fields_o]
-    lines += [f'printf("%zu\\n", offsetof(vpmfmm_stats, {f}));' for f in fields_s]
+             "int main(void) {"]
+    for cname, _ in structs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+    for cname, ct in structs:
+        lines += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f, _ in ct._fields_]
     lines += ["return 0; }"]
     probe.write_text("\n".join(lines) + "\n")
     exe = tmp_path / "probe"
@@ -73,10 +75,10 @@ def test_struct_layouts_match_header(tmp_path):
                     str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     vals = [int(v) for v in out]
-    assert vals[0] == C.sizeof(_lib.Opts) and vals[1] == C.sizeof(_lib.Stats)
-    offs = vals[2:]
-    assert offs[:len(fields_o)] == [getattr(_lib.Opts, f).offset for f in fields_o]
-    assert offs[len(fields_o):] == [getattr(_lib.Stats, f).offset for f in fields_s]
+    assert vals[:len(structs)] == [C.sizeof(ct) for _, ct in structs]
+    offs = vals[len(structs):]
+    want = [getattr(ct, f).offset for _, ct in structs for f, _ in ct._fields_]
+    assert offs == want
 
 
 def test_null_handle_is_rejected():
@@ -85,6 +87,8 @@ def test_null_handle_is_rejected():
     assert L.vpmfmm_destroy(None) == 0
     assert L.vpmfmm_field(None, 10, None, None, None, None, None, None) == 1
     assert L.vpmfmm_last_stats(None, None) == 1
+    assert L.vpmfmm_set_selector(None, None, None) == 1
+    assert L.vpmfmm_mlp_forward(None, None, None, 0, None) == 1
 
 
 def test_invalid_options_are_rejected_before_any_device_call():
@@ -92,7 +96,8 @@ def test_invalid_options_are_rejected_before_any_device_call():
     L = _lib.load()
     for field, value in (("p", 0), ("p", 41), ("max_par", 0), ("max_par", 5000),
                          ("exp_tree", 0.0), ("min_level", 0), ("max_level", 17),
-                         ("binom_mode", 2), ("list4_mode", 3), ("stage_mask", 8)):
+                         ("binom_mode", 2), ("list4_mode", 3), ("stage_mask", 8),
+                         ("order_select", 2), ("reselect_every", -1)):
         o = _lib.default_opts()
         setattr(o, field, value)
         h = C.c_void_p()
@@ -116,3 +121,22 @@ def test_product_never_imports_the_oracle():
                 src = open(os.path.join(dirpath, fn)).read()
                 assert "import oracle" not in src and "from oracle" not in src, fn
                 assert "fmm2d_oracle" not in src and "libref_fmm2d" not in src, fn
+
+
+def build_drop_in(tmp_path):
+    """Compile tests/cpp/drop_in_velocity.cpp against the C++ drop-in header."""
+    _lib = _ensure_built()
+    exe = tmp_path / "drop_in_velocity"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I",
+                    os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "drop_in_velocity.cpp"), f"-L{libdir}",
+                    "-lvpmfmm", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_drop_in_header_compiles(tmp_path):
+    """The reference's call site get_velocity(particle, grid, step) compiles
+    against include/vpmfmm_velocity_calc.hpp with the reference's member types."""
+    exe = build_drop_in(tmp_path)
+    assert exe.exists()
