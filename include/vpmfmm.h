@@ -64,7 +64,11 @@ typedef struct vpmfmm_opts {
     int32_t order_select;   /* mixed path: vpmfmm_order_select (fixed / learned)        */
     int32_t reselect_every; /* learned: re-choose the leaf threshold every k calls
                                (0 = on the first call and when n changes by > 10%)      */
-    int32_t reserved[5];
+    int32_t judge_every;    /* learned: every k-th call also solves at the fixed order p
+                               on the same tree and compares (0 = never); a difference
+                               above the model's target returns the fixed-order result
+                               and keeps the fixed order until the next re-selection    */
+    int32_t reserved[4];
 } vpmfmm_opts;
 
 /* Expansion-order source of the mixed path. */
@@ -99,14 +103,15 @@ typedef struct vpmfmm_selector {
 
 /* What the last mixed-path call used. */
 typedef struct vpmfmm_selection {
-    int32_t source;          /* 0 fixed, 1 learned, 2 learned model fell back to opts.p */
+    int32_t source;          /* 0 fixed, 1 learned, 2 learned model fell back to opts.p,
+                                3 the fixed-order judge rejected the learned orders      */
     int32_t threshold;       /* leaf threshold of the tree (max_par; max_src)           */
     int32_t p[17];           /* order per level 0..16                                  */
     int32_t reselected;      /* 1 if this call re-chose the leaf threshold              */
     float pred_err;          /* predicted relative L2 error (source 1), else -1          */
     float pred_ms;           /* predicted solve time (source 1), else -1                 */
     float ms_reselect;       /* time spent re-choosing the threshold in this call       */
-    int32_t pad;
+    float judge_err;         /* rel-L2 difference learned vs fixed order (judged calls), else -1 */
 } vpmfmm_selection;
 
 typedef struct vpmfmm_stats {

@@ -40,7 +40,8 @@ class Opts(C.Structure):
         ("precision", C.c_int32), ("binom_mode", C.c_int32), ("list4_mode", C.c_int32),
         ("u_inf", C.c_double), ("v_inf", C.c_double), ("device", C.c_int32),
         ("timing", C.c_int32), ("stage_mask", C.c_int32), ("order_select", C.c_int32),
-        ("reselect_every", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("reselect_every", C.c_int32), ("judge_every", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -58,13 +59,14 @@ class Selection(C.Structure):
     _fields_ = [
         ("source", C.c_int32), ("threshold", C.c_int32), ("p", C.c_int32 * 17),
         ("reselected", C.c_int32), ("pred_err", C.c_float), ("pred_ms", C.c_float),
-        ("ms_reselect", C.c_float), ("pad", C.c_int32),
+        ("ms_reselect", C.c_float), ("judge_err", C.c_float),
     ]
 
     def as_dict(self) -> dict:
         return {"source": self.source, "threshold": self.threshold, "p": list(self.p),
                 "reselected": self.reselected, "pred_err": self.pred_err,
-                "pred_ms": self.pred_ms, "ms_reselect": self.ms_reselect}
+                "pred_ms": self.pred_ms, "ms_reselect": self.ms_reselect,
+                "judge_err": self.judge_err}
 
 
 class Stats(C.Structure):

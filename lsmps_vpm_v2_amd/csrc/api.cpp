@@ -86,6 +86,8 @@ void fill_tables(int P, int binom_mode, std::vector<double> &m2l, std::vector<do
 }
 
 void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st);
+double rel_diff64(int n, const double *a, const double *b, const double *ja, const double *jb,
+                  double a0, double b0, double *part_dev, hipStream_t s);
 
 }  // namespace vpm
 
@@ -102,6 +104,10 @@ struct vpmfmm_handle_s {
     std::vector<int> level_p;  // per-level orders of the mixed path (empty: opts.p everywhere)
     int mx_pmax = -1;          // order the mixed tables were built for
     const float *mx_rptr_host[17] = {};  // device addresses of the M2L tables per order
+    // pinned staging of the per-call mixed-path uploads (tiles, level table);
+    // rewritten only after the next tree build has synchronised the stream
+    void *pin = nullptr;
+    size_t pin_cap = 0;
     hipEvent_t ev[8] = {};
     // learned selection
     bool has_sel = false;
@@ -110,6 +116,9 @@ struct vpmfmm_handle_s {
     int sel_threshold = 0;   // leaf threshold chosen by the last re-selection (0: none)
     int sel_n = 0;           // particle count at the last re-selection
     int sel_calls = 0;       // calls since the last re-selection
+    int judge_calls = 0;     // learned calls since the last re-selection (judge cadence)
+    bool judge_reject = false;  // the judge rejected the learned orders: fixed order until re-selection
+    DBuf judge_a, judge_b, judge_part;
     vpmfmm_selection last_sel{};
     uint8_t h_orders[kMaxDepth + 2] = {};  // order table copied into the device header
     MlpWork mlp;
@@ -143,6 +152,7 @@ int opts_check(const vpmfmm_opts &o, std::string &msg) {
     else if (o.order_select != VPMFMM_ORDER_FIXED && o.order_select != VPMFMM_ORDER_LEARNED)
         msg = "order_select must be 0 (fixed) or 1 (learned)";
     else if (o.reselect_every < 0) msg = "reselect_every must be >= 0";
+    else if (o.judge_every < 0) msg = "judge_every must be >= 0";
     else return VPMFMM_OK;
     return VPMFMM_BAD_ARG;
 }
@@ -231,6 +241,8 @@ void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *
     h->sel_threshold = best_t;  // 0: no candidate met the target (fixed options)
     h->sel_n = n;
     h->sel_calls = 0;
+    h->judge_calls = 0;
+    h->judge_reject = false;
 }
 
 void build(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
@@ -261,13 +273,13 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     const TreeParams tp = tree_params(h, thr);
     mark(h, 0);
     tree_build(h->ctx, n, x, y, q, a, tp);
-    if (lrn) {
-        select_on_tree(h, tp.max_par);
-    } else {
+    if (lrn) select_on_tree(h, tp.max_par);
+    if (!lrn || h->judge_reject) {
         // orders per level for the evaluation statistics of the list build
         for (int L = 0; L < kMaxDepth + 2; ++L) {
             int p = h->o.p;
-            if (h->o.precision == VPMFMM_MIXED && L < (int)h->level_p.size() && h->level_p[L] > 0)
+            if (h->o.precision == VPMFMM_MIXED && !lrn && L < (int)h->level_p.size() &&
+                h->level_p[L] > 0)
                 p = h->level_p[L];
             h->h_orders[L] = (uint8_t)std::min(p, 255);
         }
@@ -320,16 +332,16 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
 
 // Mixed path set-up: per-level orders and sides, level-independent tables
 // (rebuilt only when the largest order changes) and the 16-cell M2L tiles.
-Mixed32 mixed_setup(vpmfmm_handle *h) {
+Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     Ctx &c = h->ctx;
     const TreeInfo &ti = c.info;
     LevelTab lt{};
     int pmax = 0;
-    const bool lrn = learned(h);
+    const bool lrn = learned(h) && !h->judge_reject && !force_fixed;
     for (int L = 0; L < kMaxDepth + 2; ++L) {
         int p = h->o.p;
         if (lrn) p = c.hdr_h->sel_p[L];
-        else if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
+        else if (!learned(h) && L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
         if (p > 16) throw Error(kBadArg, "the mixed path supports expansion orders up to 16");
         lt.p[L] = p;
         const double s = ti.root_len / std::ldexp(1.0, L);
@@ -377,13 +389,24 @@ Mixed32 mixed_setup(vpmfmm_handle *h) {
     }
     mx.tile_off[17] = (int)tl.size();
     int2 *td = c.mx_tiles.as<int2>(tl.size() + 1);
-    if (!tl.empty())
-        VPM_HIP(hipMemcpyAsync(td, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice,
+    const size_t need = sizeof(LevelTab) + tl.size() * sizeof(int2);
+    if (h->pin_cap < need) {
+        if (h->pin) VPM_HIP(hipHostFree(h->pin));
+        h->pin = nullptr;
+        h->pin_cap = 0;
+        VPM_HIP(hipHostMalloc(&h->pin, need + need / 2, hipHostMallocDefault));
+        h->pin_cap = need + need / 2;
+    }
+    LevelTab *lt_pin = (LevelTab *)h->pin;
+    int2 *tl_pin = (int2 *)((char *)h->pin + sizeof(LevelTab));
+    *lt_pin = lt;
+    if (!tl.empty()) {
+        std::memcpy(tl_pin, tl.data(), tl.size() * sizeof(int2));
+        VPM_HIP(hipMemcpyAsync(td, tl_pin, tl.size() * sizeof(int2), hipMemcpyHostToDevice,
                                c.stream));
+    }
     LevelTab *ltd = c.mx_lt.as<LevelTab>(1);
-    VPM_HIP(hipMemcpyAsync(ltd, &lt, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
-    // the host vectors above must outlive the copies
-    VPM_HIP(hipStreamSynchronize(c.stream));
+    VPM_HIP(hipMemcpyAsync(ltd, lt_pin, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
     mx.pmax = h->mx_pmax;  // tables (and coefficient stride) of the largest order built so far
     mx.lt = ltd;
     mx.m2m = (const float *)c.mx_m2m.p;
@@ -402,22 +425,48 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double
     FmmMarks mk{h->o.timing ? h->ev : nullptr};
     const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
     vpmfmm_selection &ls = h->last_sel;
+    ls.judge_err = -1.f;
     if (h->o.precision == VPMFMM_MIXED) {
+        const bool lrn = learned(h) && !h->judge_reject;
         Mixed32 mx = mixed_setup(h);
         if (learned(h)) {
-            ls.source = (int)c.hdr_h->sel_source;
-            ls.pred_err = c.hdr_h->sel_pred_err;
-            ls.pred_ms = c.hdr_h->sel_pred_ms;
+            ls.source = h->judge_reject ? 3 : (int)c.hdr_h->sel_source;
+            ls.pred_err = lrn ? c.hdr_h->sel_pred_err : -1.f;
+            ls.pred_ms = lrn ? c.hdr_h->sel_pred_ms : -1.f;
         } else {
             ls.pred_err = ls.pred_ms = -1.f;
         }
         for (int L = 0; L <= kMaxDepth; ++L) {
             int p = h->o.p;
-            if (learned(h)) p = c.hdr_h->sel_p[L];
-            else if (L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
+            if (lrn) p = c.hdr_h->sel_p[L];
+            else if (!learned(h) && L < (int)h->level_p.size() && h->level_p[L] > 0)
+                p = h->level_p[L];
             ls.p[L] = p;
         }
         fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
+        // the fixed-order judge (same tree): every judge_every-th learned call
+        if (lrn && h->o.judge_every > 0 && (h->judge_calls++ % h->o.judge_every) == 0 &&
+            c.hdr_h->sel_source == 1) {
+            const int nn = c.info.n;
+            // buffers may grow below: let the learned solve finish first
+            VPM_HIP(hipStreamSynchronize(c.stream));
+            double *ja = h->judge_a.as<double>(nn), *jb = h->judge_b.as<double>(nn);
+            Mixed32 mf = mixed_setup(h, true);
+            fmm32_run(c, mf, h->o.list4_mode, mask, ja, jb, velocity, h->o.u_inf, h->o.v_inf,
+                      FmmMarks{nullptr});
+            const double a0 = velocity ? h->o.u_inf : 0.0, b0 = velocity ? h->o.v_inf : 0.0;
+            ls.judge_err = (float)rel_diff64(nn, oa, ob, ja, jb, a0, b0,
+                                             h->judge_part.as<double>(1024), c.stream);
+            if (!(ls.judge_err <= h->sel.target_err)) {
+                h->judge_reject = true;
+                VPM_HIP(hipMemcpyAsync(oa, ja, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                       c.stream));
+                VPM_HIP(hipMemcpyAsync(ob, jb, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                       c.stream));
+                ls.source = 3;
+                for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = h->o.p;
+            }
+        }
     } else {
         ensure_tables(h);
         ls.pred_err = ls.pred_ms = -1.f;
@@ -570,6 +619,7 @@ int vpmfmm_destroy(vpmfmm_handle *h) {
         if (e) (void)hipEventDestroy(e);
     if (h->ctx.hdr_d) (void)hipFree(h->ctx.hdr_d);
     if (h->ctx.hdr_h) (void)hipHostFree(h->ctx.hdr_h);
+    if (h->pin) (void)hipHostFree(h->pin);
     h->ctx.hdr_d = nullptr;
     h->ctx.hdr_h = nullptr;
     hipStream_t own = h->own_stream;
@@ -809,7 +859,8 @@ int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const floa
     return guarded(h, [&] {
         if (!sel) {
             h->has_sel = false;
-            h->sel_threshold = h->sel_n = h->sel_calls = 0;
+            h->sel_threshold = h->sel_n = h->sel_calls = h->judge_calls = 0;
+            h->judge_reject = false;
             return;
         }
         const vpmfmm_selector &m = *sel;
@@ -851,7 +902,8 @@ int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const floa
         VPM_HIP(hipStreamSynchronize(c.stream));
         h->sel = m;
         h->has_sel = true;
-        h->sel_threshold = h->sel_n = h->sel_calls = 0;
+        h->sel_threshold = h->sel_n = h->sel_calls = h->judge_calls = 0;
+        h->judge_reject = false;
     });
 }
 

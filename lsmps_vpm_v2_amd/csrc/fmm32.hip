@@ -97,10 +97,11 @@ __global__ void __launch_bounds__(256)
     const int t = (int)(idx / PM1), k = (int)(idx % PM1);
     const int j = (int)leaf_idx[t];
     const int L = c_level[j];
-    if (k > lt->p[L] || !(c_flags[j] & kActive)) return;
+    if (k > lt->p[L]) return;
     const float inv_s = lt->inv_s[L];
     cf acc = {0.f, 0.f};
-    for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
+    const uint32_t e = (c_flags[j] & kActive) ? c_end[j] : c_begin[j];  // inactive: zero
+    for (uint32_t i = c_begin[j]; i < e; ++i) {
         if (!as[i]) continue;
         const float2 o = oxy[i];
         acc = cadd(acc, cscale(cpowi(cf{o.x * inv_s, o.y * inv_s}, k), qf[i]));
@@ -108,29 +109,133 @@ __global__ void __launch_bounds__(256)
     stc(ahat, (size_t)j * PM1 + k, acc);
 }
 
-// M2M for one level: thread per (parent, m).  m2m[q][m][k] complex, q = child
-// quadrant, stride PM1 x PM1.
-__global__ void __launch_bounds__(256)
-    k_m2m32(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
-            const LevelTab *lt, const float *m2m, int kst, float *ahat) {
-    const int PM1 = kst / 2;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)(j1 - j0) * PM1) return;
-    const int j = j0 + (int)(idx / PM1), m = (int)(idx % PM1);
+// M2M of one (parent j, order m): m2m[q][m][k] complex, q = child quadrant,
+// stride PM1 x PM1; inactive parents get zero (their parents read them).
+__device__ __forceinline__ void m2m_cell(int j, int m, int PM1, const uint8_t *c_level,
+                                         const uint8_t *c_flags, const int *c_child,
+                                         const LevelTab *lt, const float *m2m, float *ahat) {
     const unsigned f = c_flags[j];
     const int L = c_level[j];
-    if ((f & kLeaf) || !(f & kActive) || m > lt->p[L]) return;
-    const int pc = lt->p[L + 1];
+    if ((f & kLeaf) || m > lt->p[L]) return;
     cf acc = {0.f, 0.f};
-    for (int q = 0; q < 4; ++q) {
-        const int ch = c_child[4 * j + q];
-        if (ch < 0) continue;
-        const float *row = m2m + ((size_t)q * PM1 + m) * PM1 * 2;
-        const int kmax = min(m, pc);
-        for (int k = 0; k <= kmax; ++k)
-            acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(ahat, (size_t)ch * PM1 + k)));
+    if (f & kActive) {
+        const int pc = lt->p[L + 1];
+        for (int q = 0; q < 4; ++q) {
+            const int ch = c_child[4 * j + q];
+            if (ch < 0) continue;
+            const float *row = m2m + ((size_t)q * PM1 + m) * PM1 * 2;
+            const int kmax = min(m, pc);
+            for (int k = 0; k <= kmax; ++k)
+                acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(ahat, (size_t)ch * PM1 + k)));
+        }
     }
     stc(ahat, (size_t)j * PM1 + m, acc);
+}
+
+// L2L into (child j, order m >= 1) from its parent: l2l[q][m][k], k >= m.
+__device__ __forceinline__ void l2l_cell(int j, int m, int PM1, const uint8_t *c_level,
+                                         const uint32_t *c_key, const int *c_parent,
+                                         const LevelTab *lt, const float *l2l, float *bhat) {
+    const int L = c_level[j];
+    if (m > lt->p[L]) return;
+    const int par = c_parent[j];
+    const int pp = lt->p[L - 1];
+    const uint32_t key = c_key[j];
+    const int q = (int)(compact1by1(key) & 1u) + 2 * (int)(compact1by1(key >> 1) & 1u);
+    const float *row = l2l + ((size_t)q * PM1 + m) * PM1 * 2;
+    cf acc = {0.f, 0.f};
+    for (int k = m; k <= pp; ++k)
+        acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(bhat, (size_t)par * PM1 + k)));
+    const size_t o = (size_t)j * PM1 + m;
+    stc(bhat, o, cadd(ldc(bhat, o), acc));
+}
+
+// Subtree passes.  Cells are level-major and Morton-ordered inside a level, so
+// the descendants of a level-K cell with key r form, at each level L > K, the
+// contiguous range of keys [r 4^(L-K), (r+1) 4^(L-K)).  One workgroup owns one
+// level-K cell and walks its subtree level by level with workgroup barriers;
+// the cells above level K are done by one workgroup (k_*_top).
+constexpr int kSubBlock = 256;
+constexpr int kTopBlock = 1024;
+constexpr int kSubRoots = 256;  // subtree roots wanted at the split level
+
+__device__ int lower_key(const uint32_t *c_key, int lo, int hi, unsigned long long v) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((unsigned long long)c_key[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void subtree_ranges(int K, int maxL, const DevHdr *hdr,
+                                               const uint32_t *c_key, int *s_lo, int *s_hi) {
+    const int r = (int)hdr->lev_off[K] + blockIdx.x;
+    const unsigned long long kr = c_key[r];
+    const int L = K + (int)threadIdx.x;
+    if (L <= maxL) {
+        const int sh = 2 * (L - K);
+        const int lo = (int)hdr->lev_off[L], hi = (int)hdr->lev_off[L + 1];
+        s_lo[L] = lower_key(c_key, lo, hi, kr << sh);
+        s_hi[L] = lower_key(c_key, lo, hi, (kr + 1ull) << sh);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kSubBlock)
+    k_m2m_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
+                  const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
+                  const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    const int PM1 = kst / 2;
+    subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
+    for (int L = maxL - 1; L >= K; --L) {
+        const int lo = s_lo[L], cnt = s_hi[L] - lo;
+        for (int idx = threadIdx.x; idx < cnt * PM1; idx += kSubBlock)
+            m2m_cell(lo + idx / PM1, idx % PM1, PM1, c_level, c_flags, c_child, lt, m2m, ahat);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kTopBlock)
+    k_m2m_top(int K, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
+              const int *c_child, const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    const int PM1 = kst / 2;
+    for (int L = K - 1; L >= 1; --L) {
+        const int lo = (int)hdr->lev_off[L], cnt = (int)hdr->lev_off[L + 1] - lo;
+        for (int idx = threadIdx.x; idx < cnt * PM1; idx += kTopBlock)
+            m2m_cell(lo + idx / PM1, idx % PM1, PM1, c_level, c_flags, c_child, lt, m2m, ahat);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kTopBlock)
+    k_l2l_top(int K, const DevHdr *hdr, const uint8_t *c_level, const uint32_t *c_key,
+              const int *c_parent, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    const int PM1 = kst / 2;
+    for (int L = 3; L <= K; ++L) {
+        const int lo = (int)hdr->lev_off[L], cnt = (int)hdr->lev_off[L + 1] - lo;
+        for (int idx = threadIdx.x; idx < cnt * (PM1 - 1); idx += kTopBlock)
+            l2l_cell(lo + idx / (PM1 - 1), 1 + idx % (PM1 - 1), PM1, c_level, c_key, c_parent, lt,
+                     l2l, bhat);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSubBlock)
+    k_l2l_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
+                  const uint8_t *c_level, const int *c_parent, const LevelTab *lt,
+                  const float *l2l, int kst, float *bhat) {
+    __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    const int PM1 = kst / 2;
+    subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
+    for (int L = max(K + 1, 3); L <= maxL; ++L) {
+        const int lo = s_lo[L], cnt = s_hi[L] - lo;
+        for (int idx = threadIdx.x; idx < cnt * (PM1 - 1); idx += kSubBlock)
+            l2l_cell(lo + idx / (PM1 - 1), 1 + idx % (PM1 - 1), PM1, c_level, c_key, c_parent, lt,
+                     l2l, bhat);
+        __syncthreads();
+    }
 }
 
 // M2L over list 2 on MFMA.  Tile = 16 consecutive cells of one level (all of
@@ -315,30 +420,9 @@ __global__ void __launch_bounds__(256)
     stc(bhat, o, cadd(ldc(bhat, o), acc));
 }
 
-// L2L for one level (>= 3): thread per (child, m).
-__global__ void __launch_bounds__(256)
-    k_l2l32(int j0, int j1, const uint8_t *c_level, const uint32_t *c_key, const int *c_parent,
-            const LevelTab *lt, const float *l2l, int kst, float *bhat) {
-    const int PM1 = kst / 2;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)(j1 - j0) * (PM1 - 1)) return;
-    const int j = j0 + (int)(idx / (PM1 - 1)), m = 1 + (int)(idx % (PM1 - 1));
-    const int L = c_level[j];
-    if (m > lt->p[L]) return;
-    const int par = c_parent[j];
-    const int pp = lt->p[L - 1];
-    const uint32_t key = c_key[j];
-    const int q = (int)(compact1by1(key) & 1u) + 2 * (int)(compact1by1(key >> 1) & 1u);
-    const float *row = l2l + ((size_t)q * PM1 + m) * PM1 * 2;
-    cf acc = {0.f, 0.f};
-    for (int k = m; k <= pp; ++k)
-        acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(bhat, (size_t)par * PM1 + k)));
-    const size_t o = (size_t)j * PM1 + m;
-    stc(bhat, o, cadd(ldc(bhat, o), acc));
-}
-
 // Evaluation: one wavefront per leaf; list-1 sources are staged in LDS as
-// (x - c_t, y - c_t, q, index) with the centre shift done in FP64.  Lanes are
+// (x - c_t, y - c_t, q, index) with the centre shift done in FP64 and rounded
+// once.  Lanes are
 // split into G groups of 64/G: a group owns the targets, groups split the
 // sources and list-3 cells and are reduced with cross-lane shuffles.
 constexpr int kEvalWaves = 4;
@@ -352,6 +436,8 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
              const float *ahat, const float *bhat, int kst, int mask, int velocity, double u_inf,
              double v_inf, double *out_a, double *out_b) {
     __shared__ float4 stage[kEvalWaves][kStage];
+    __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
+    __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int t = blockIdx.x * kEvalWaves + w;
@@ -372,39 +458,65 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
         const bool valid = i < e;
         const float2 ti = valid ? oxy[i] : make_float2(0.f, 0.f);
         float ex = 0.f, ey = 0.f;
-        // P2P, staged in rounds of kStage sources
-        uint32_t u = a1, pos = a1 < z1 ? c_begin[l1[a1]] : 0u;
-        while ((mask & 1) && u < z1) {
-            int fill = 0;
-            while (u < z1 && fill < kStage) {
-                const int sc = l1[u];
-                const uint32_t se = c_end[sc];
-                const float dcx = (float)(c_cx[sc] - tcx), dcy = (float)(c_cy[sc] - tcy);
-                const int take = (int)min<uint32_t>(se - pos, (uint32_t)(kStage - fill));
-                for (int r = lane; r < take; r += kWave) {
-                    const uint32_t src = pos + r;
-                    const float2 o = oxy[src];
-                    stage[w][fill + r] =
-                        make_float4(o.x + dcx, o.y + dcy, qf[src], __uint_as_float(src));
-                }
-                fill += take;
-                pos += take;
-                if (pos >= se && ++u < z1) pos = c_begin[l1[u]];
+        // P2P.  Up to 64 list-1 cells at a time: lane k fetches cell k's range and
+        // centre shift (one round of loads), a wave scan gives each cell its
+        // offset in the concatenated source list, and the sources are staged
+        // through LDS in rounds of kStage, each lane locating its source's cell
+        // by binary search over the offsets.
+        for (uint32_t c0 = a1; (mask & 1) && c0 < z1; c0 += kWave) {
+            const int ncell = (int)min<uint32_t>(kWave, z1 - c0);
+            uint32_t cnt = 0, beg = 0;
+            float dcx = 0.f, dcy = 0.f;
+            if (lane < ncell) {
+                const int sc = l1[c0 + lane];
+                beg = c_begin[sc];
+                cnt = c_end[sc] - beg;
+                dcx = (float)(c_cx[sc] - tcx);
+                dcy = (float)(c_cy[sc] - tcy);
             }
+            uint32_t inc = cnt;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, kWave);
+                if (lane >= o) inc += y;
+            }
+            const uint32_t total = __shfl(inc, kWave - 1, kWave);
+            s_off[w][lane] = inc - cnt;
+            s_beg[w][lane] = beg;
+            s_dx[w][lane] = dcx;
+            s_dy[w][lane] = dcy;
             // lanes read what other lanes of the same wavefront wrote
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (int r = g; r < fill; r += G) {
-                const float4 sv = stage[w][r];
-                const float dx = ti.x - sv.x, dy = ti.y - sv.y;
-                const float r2 = dx * dx + dy * dy;
-                const float inv = (__float_as_uint(sv.w) == i) ? 0.f : __builtin_amdgcn_rcpf(r2);
-                const float qr = sv.z * inv;
-                ex += qr * dx;
-                ey += qr * dy;
+            for (uint32_t base = 0; base < total; base += kStage) {
+                const int fill = (int)min<uint32_t>((uint32_t)kStage, total - base);
+                for (int r = lane; r < fill; r += kWave) {
+                    const uint32_t gi = base + (uint32_t)r;
+                    int lo = 0, hi = ncell - 1;  // last cell whose offset is <= gi
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[w][mid] <= gi) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint32_t src = s_beg[w][lo] + (gi - s_off[w][lo]);
+                    const float2 o = oxy[src];
+                    stage[w][r] = make_float4(o.x + s_dx[w][lo], o.y + s_dy[w][lo], qf[src],
+                                              __uint_as_float(src));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int r = g; r < fill; r += G) {
+                    const float4 sv = stage[w][r];
+                    const float dx = ti.x - sv.x, dy = ti.y - sv.y;
+                    const float r2 = dx * dx + dy * dy;
+                    const float inv =
+                        (__float_as_uint(sv.w) == i) ? 0.f : __builtin_amdgcn_rcpf(r2);
+                    const float qr = sv.z * inv;
+                    ex += qr * dx;
+                    ey += qr * dy;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
         }
         // M2P over list 3: F = (1/s) sum a^_k xi^-(k+1), xi = (z - zc)/s
         for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
@@ -555,8 +667,9 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     const int nc = ti.n_cells, nl = ti.n_leaves, n = ti.n;
     const int kst = 2 * (mx.pmax + 1), PM1 = mx.pmax + 1;
     float *ahat = c.ak.as<float>((size_t)nc * kst), *bhat = c.bk.as<float>((size_t)nc * kst);
-    VPM_HIP(hipMemsetAsync(ahat, 0, sizeof(float) * (size_t)nc * kst, s));
-    VPM_HIP(hipMemsetAsync(bhat, 0, sizeof(float) * (size_t)nc * kst, s));
+    // no clearing needed: P2M / M2M write every coefficient a cell's readers use
+    // (zero for inactive cells), the list-2 M2L writes b^_1..p of every cell of
+    // level >= 2 before list 4 and L2L add to them
     float2 *oxy = c.oxy.as<float2>(n);
     float *qf = c.qf.as<float>(n);
     const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
@@ -572,12 +685,21 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
                                            (const double *)c.qs.p, oxy, qf);
     k_p2m32<<<div_up((long long)nl * PM1, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_begin,
                                                              c_end, oxy, qf, as, mx.lt, kst, ahat);
-    for (int L = ti.max_level - 1; L >= 1; --L) {
-        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
-        if (j1 <= j0) continue;
-        k_m2m32<<<div_up((long long)(j1 - j0) * PM1, 256), 256, 0, s>>>(
-            j0, j1, c_level, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
-    }
+    // split level: the first level with at least kSubRoots cells (subtree
+    // workgroups below it, one workgroup for the levels above)
+    const int maxL = ti.max_level;
+    int K = maxL;
+    for (int L = 2; L <= maxL; ++L)
+        if ((int)(ti.lev_off[L + 1] - ti.lev_off[L]) >= kSubRoots) {
+            K = L;
+            break;
+        }
+    const int nK = (int)(ti.lev_off[K + 1] - ti.lev_off[K]);
+    if (K < maxL)
+        k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_flags, c_child,
+                                               mx.lt, mx.m2m, kst, ahat);
+    k_m2m_top<<<1, kTopBlock, 0, s>>>(K, c.hdr_d, c_level, c_flags, c_child, mx.lt, mx.m2m, kst,
+                                      ahat);
     mk.mark(3, s);
     // M2L list 2 on MFMA, one launch per order present
     for (int p = 1; p <= 16; ++p) {
@@ -598,11 +720,12 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
                 mx.tm2l, ahat, kst, bhat);
     }
     mk.mark(4, s);
-    for (int L = 3; L <= ti.max_level; ++L) {
-        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
-        if (j1 <= j0) continue;
-        k_l2l32<<<div_up((long long)(j1 - j0) * (PM1 - 1), 256), 256, 0, s>>>(
-            j0, j1, c_level, c_key, c_parent, mx.lt, mx.l2l, kst, bhat);
+    if (maxL >= 3) {
+        k_l2l_top<<<1, kTopBlock, 0, s>>>(K, c.hdr_d, c_level, c_key, c_parent, mx.lt, mx.l2l, kst,
+                                          bhat);
+        if (K < maxL)
+            k_l2l_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_parent,
+                                                   mx.lt, mx.l2l, kst, bhat);
     }
     mk.mark(5, s);
     k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(

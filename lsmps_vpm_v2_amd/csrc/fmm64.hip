@@ -16,6 +16,9 @@
 // global memory.  Thread mappings: one thread per (cell, order) for the
 // translations (Horner in the translation distance, no per-thread arrays),
 // one wavefront per leaf for the evaluation.
+#include <cmath>
+#include <vector>
+
 #include "ctx.hpp"
 
 namespace vpm {
@@ -371,6 +374,54 @@ __global__ void __launch_bounds__(256)
     k_sources64(int n, const double *s, const double *omega, double *q) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) q[i] = -omega[i] * (s[i] * s[i]) / (2 * M_PI);
+}
+
+// Partial sums of |(a, b) - (ja, jb)|^2 and |(ja - a0, jb - b0)|^2 per block.
+__global__ void __launch_bounds__(256)
+    k_diff_norms(int n, const double *a, const double *b, const double *ja, const double *jb,
+                 double a0, double b0, double *part) {
+    double num = 0.0, den = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double da = a[i] - ja[i], db = b[i] - jb[i];
+        const double ra = ja[i] - a0, rb = jb[i] - b0;
+        num += da * da + db * db;
+        den += ra * ra + rb * rb;
+    }
+    for (int m = 1; m < kWave; m <<= 1) {
+        num += __shfl_xor(num, m, kWave);
+        den += __shfl_xor(den, m, kWave);
+    }
+    __shared__ double sn[4], sd[4];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sn[w] = num;
+        sd[w] = den;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sn[0] + sn[1] + sn[2] + sn[3];
+        part[2 * blockIdx.x + 1] = sd[0] + sd[1] + sd[2] + sd[3];
+    }
+}
+
+// Relative L2 difference of (a, b) against the judge (ja, jb), whose
+// reference level (a0, b0) is removed from the norm (the free stream of a
+// velocity): sqrt(sum |d|^2 / sum |j - (a0, b0)|^2).  Synchronises the stream.
+double rel_diff64(int n, const double *a, const double *b, const double *ja, const double *jb,
+                  double a0, double b0, double *part_dev, hipStream_t s) {
+    constexpr int kBlocks = 512;
+    k_diff_norms<<<kBlocks, 256, 0, s>>>(n, a, b, ja, jb, a0, b0, part_dev);
+    VPM_LAUNCH_CHECK();
+    std::vector<double> h(2 * kBlocks);
+    VPM_HIP(hipMemcpyAsync(h.data(), part_dev, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+    VPM_HIP(hipStreamSynchronize(s));
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < kBlocks; ++k) {
+        num += h[2 * k];
+        den += h[2 * k + 1];
+    }
+    return den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
 }
 
 void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st) {

@@ -242,9 +242,40 @@ def test_trained_selector_gains_over_fixed_order(gpu):
         t_lrn, u, v = timed(learned)
         e_lrn = err_of(u, v)
         sel = learned.last_selection()
+        learned.set_opts(judge_every=1)  # one judged call: learned vs fixed p = 12
+        learned.velocity_device(*dev)
+        judged = learned.last_selection()
         print(f"{f['name']}: fixed p=12 {t_fix * 1e3:.3f} ms err {e_fix:.2e}; learned "
-              f"{t_lrn * 1e3:.3f} ms err {e_lrn:.2e} (pred {sel['pred_err']:.2e}), "
-              f"threshold {sel['threshold']}, p {sel['p'][:12]}, source {sel['source']}")
-        assert sel["source"] == 1
-        assert e_lrn <= 1e-4
+              f"{t_lrn * 1e3:.3f} ms err {e_lrn:.2e} (pred {sel['pred_err']:.2e}, judge "
+              f"{judged['judge_err']:.2e}), threshold {sel['threshold']}, p {sel['p'][:12]}, "
+              f"source {sel['source']}")
+        assert sel["source"] == 1 and judged["source"] == 1
+        assert e_lrn <= 1e-4 and judged["judge_err"] <= 1e-4
         assert t_lrn <= 0.9 * t_fix
+
+
+def test_fixed_order_judge_rejects_a_bad_model(gpu):
+    """A model that predicts negligible error everywhere picks the lowest
+    order; the fixed-order judge (p = 12 on the same tree) sees the real
+    difference, returns its own result and keeps the fixed order."""
+    import torch
+    from lsmps_vpm_v2_amd import FmmSolver, selector as S, workloads as W
+    f = W.random_clusters(30000, seed=12)
+    dev = [torch.from_numpy(f[k]).cuda() for k in ("x", "y", "s", "omega", "active")]
+    kw = dict(p=12, max_par=48, max_src=48, precision="mixed", binom_mode=0, list4_mode=2)
+    u0, v0 = FmmSolver(**kw).velocity_device(*dev)
+    params = np.zeros(S.NP, np.float32)
+    params[S.OFF["b3"][0]] = -60.0  # ln err^2 = -60 at every level and order
+    big = np.full(9, 1e6, np.float32)
+    model = S.Model(params=params, mu=np.zeros(9, np.float32), sd=np.ones(9, np.float32),
+                    lo=-big, hi=big, p_min=4, p_max=16, thresholds=[48], target_err=1e-4)
+    s = FmmSolver(order="learned", selector=model, **kw)
+    s.set_opts(judge_every=1)
+    u1, v1 = s.velocity_device(*dev)
+    sel = s.last_selection()
+    print(f"judge: source {sel['source']} judge_err {sel['judge_err']:.2e}")
+    assert sel["source"] == 3 and sel["judge_err"] > 1e-4
+    assert torch.equal(u0, u1) and torch.equal(v0, v1)
+    u2, v2 = s.velocity_device(*dev)  # stays on the fixed order
+    assert s.last_selection()["source"] == 3
+    assert torch.equal(u0, u2) and torch.equal(v0, v2)
