@@ -457,7 +457,9 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
         const uint32_t i = tb + tl;
         const bool valid = i < e;
         const float2 ti = valid ? oxy[i] : make_float2(0.f, 0.f);
-        float ex = 0.f, ey = 0.f;
+        // each staging round sums in FP32; the round totals add in FP64, so
+        // leaves with very many sources do not lose the FP32 accumulation
+        double exd = 0.0, eyd = 0.0;
         // P2P.  Up to 64 list-1 cells at a time: lane k fetches cell k's range and
         // centre shift (one round of loads), a wave scan gives each cell its
         // offset in the concatenated source list, and the sources are staged
@@ -504,6 +506,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                float ex = 0.f, ey = 0.f;
                 for (int r = g; r < fill; r += G) {
                     const float4 sv = stage[w][r];
                     const float dx = ti.x - sv.x, dy = ti.y - sv.y;
@@ -514,6 +517,8 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                     ex += qr * dx;
                     ey += qr * dy;
                 }
+                exd += (double)ex;
+                eyd += (double)ey;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
@@ -531,13 +536,13 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             for (int k = ps - 1; k >= 0; --k) f = cadd(cmul(f, wv), ldc(ahat, base + k));
             f = cscale(cmul(f, wv), is);
             if (valid) {
-                ex += f.x;
-                ey -= f.y;
+                exd += (double)f.x;
+                eyd -= (double)f.y;
             }
         }
         for (int m = LG; m < kWave; m <<= 1) {
-            ex += __shfl_xor(ex, m, kWave);
-            ey += __shfl_xor(ey, m, kWave);
+            exd += __shfl_xor(exd, m, kWave);
+            eyd += __shfl_xor(eyd, m, kWave);
         }
         if (valid && g == 0) {
             if (mask & 4) {
@@ -547,16 +552,16 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 cf f = cscale(ldc(bhat, base + p), (float)p);
                 for (int m = p - 1; m >= 1; --m) f = cadd(cmul(f, z), cscale(ldc(bhat, base + m), (float)m));
                 f = cscale(f, inv_s);
-                ex += f.x;
-                ey -= f.y;
+                exd += (double)f.x;
+                eyd -= (double)f.y;
             }
             const uint32_t o = perm[i];
             if (velocity) {
-                out_a[o] = (double)ey + u_inf;
-                out_b[o] = -(double)ex + v_inf;
+                out_a[o] = eyd + u_inf;
+                out_b[o] = -exd + v_inf;
             } else {
-                out_a[o] = (double)ex;
-                out_b[o] = (double)ey;
+                out_a[o] = exd;
+                out_b[o] = eyd;
             }
         }
     }

@@ -33,70 +33,85 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // level statistics of the built tree
 // ---------------------------------------------------------------------------
-// Per block: LDS accumulators per level of (particles, sum |q|, max |q|) over
-// the particles of its leaves, added to every level 0..leaf level.
+// Per block: LDS accumulators per EXACT leaf level of (particles, sum |q|,
+// max |q|, leaves).  Thread t reads the 16 consecutive sorted particles
+// 16t .. 16t+15 with their leaf level (from the leaf split) and flushes a
+// running sum whenever the level changes (runs are whole leaves, so few
+// flushes); leaves are counted over the cells, thread per cell.
 constexpr int kStatBlock = 256;
+constexpr int kStatRun = 16;
 constexpr int kLv = kMaxDepth + 1;
-constexpr unsigned kLeafBit = 1u;  // cell flag: leaf
-constexpr int kFeatBlock = 16 * (kLv + 1);  // 16 threads per level
+constexpr unsigned kLeafBit = 1u;             // cell flag: leaf
+constexpr int kFeatBlock = 16 * (kLv + 1);   // 16 threads per level
 
 __global__ void __launch_bounds__(kStatBlock)
-    k_level_stats(const uint32_t *leaf_idx, int nl, const uint8_t *c_level,
-                  const uint32_t *c_begin, const uint32_t *c_end, const double *qs,
-                  float *part) {
-    __shared__ float s_n[kLv], s_sum[kLv], s_max[kLv];
+    k_level_stats(const uint8_t *leaf_lvl, const double *qs, int n, const uint8_t *c_level,
+                  const uint8_t *c_flags, int nc, float *part) {
+    __shared__ float s_n[kLv], s_sum[kLv], s_max[kLv], s_leaf[kLv];
     if (threadIdx.x < kLv) {
         s_n[threadIdx.x] = 0.f;
         s_sum[threadIdx.x] = 0.f;
         s_max[threadIdx.x] = 0.f;
+        s_leaf[threadIdx.x] = 0.f;
     }
     __syncthreads();
-    const int t = blockIdx.x * kStatBlock + threadIdx.x;
-    if (t < nl) {
-        const int j = (int)leaf_idx[t];
-        const int L = c_level[j];
-        float sum = 0.f, mx = 0.f;
-        for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
-            const float a = (float)fabs(qs[i]);
-            sum += a;
-            mx = fmaxf(mx, a);
+    const long long i0 = ((long long)blockIdx.x * kStatBlock + threadIdx.x) * kStatRun;
+    int cur = -1;
+    float cn = 0.f, cs = 0.f, cm = 0.f;
+    auto flush = [&]() {
+        if (cur < 0) return;
+        atomicAdd(&s_n[cur], cn);
+        atomicAdd(&s_sum[cur], cs);
+        // non-negative floats order like their bit patterns
+        atomicMax(reinterpret_cast<unsigned *>(&s_max[cur]), __float_as_uint(cm));
+    };
+    for (int k = 0; k < kStatRun; ++k) {
+        const long long i = i0 + k;
+        if (i >= n) break;
+        const int lv = leaf_lvl[i];
+        const float a = (float)fabs(qs[i]);
+        if (lv != cur) {
+            flush();
+            cur = lv;
+            cn = cs = cm = 0.f;
         }
-        const float cnt = (float)(c_end[j] - c_begin[j]);
-        for (int l = 0; l <= L; ++l) {
-            atomicAdd(&s_n[l], cnt);
-            atomicAdd(&s_sum[l], sum);
-            // non-negative floats order like their bit patterns
-            atomicMax(reinterpret_cast<unsigned *>(&s_max[l]), __float_as_uint(mx));
-        }
+        cn += 1.f;
+        cs += a;
+        cm = fmaxf(cm, a);
     }
+    flush();
+    for (int j = blockIdx.x * kStatBlock + threadIdx.x; j < nc; j += gridDim.x * kStatBlock)
+        if (c_flags[j] & kLeafBit) atomicAdd(&s_leaf[c_level[j]], 1.f);
     __syncthreads();
     if (threadIdx.x < kLv) {
-        float *o = part + (size_t)blockIdx.x * 3 * kLv;
+        float *o = part + (size_t)blockIdx.x * 4 * kLv;
         o[threadIdx.x] = s_n[threadIdx.x];
         o[kLv + threadIdx.x] = s_sum[threadIdx.x];
         o[2 * kLv + threadIdx.x] = s_max[threadIdx.x];
+        o[3 * kLv + threadIdx.x] = s_leaf[threadIdx.x];
     }
 }
 
-// Reduce the block partials and form the raw feature rows (p column = 0):
+// Reduce the block partials per exact leaf level, turn them into "under level
+// L" quantities (particles whose leaf level is >= L, and so on) and form the
+// raw feature rows (p column = 0):
 //   f0 L, f1 log2 cells, f2 log2(particles / cells), f3 log2(side),
 //   f4 log10(sum|q|_L / sum|q|_0), f5 log10(max|q|_L N / sum|q|_0),
 //   f6 leaves / cells, f7 log2(leaf threshold), f8 p.
-// Levels with no cell get a row of zeros and nrows = deepest level + 1.
+// Levels with no cell get a row of zeros.
 __global__ void __launch_bounds__(kFeatBlock)
-    k_level_feats(const float *part, int nblk, const DevHdr *hdr, const uint8_t *c_level,
-                  const uint8_t *c_flags, int threshold, float *raw, float *feats) {
+    k_level_feats(const float *part, int nblk, const DevHdr *hdr, int threshold, float *raw,
+                  float *feats) {
     const int L = threadIdx.x / 16, sub = threadIdx.x % 16;  // 16 threads per level
     float n = 0.f, sum = 0.f, mx = 0.f, leaves = 0.f;
     if (L < kLv) {
         for (int b = sub; b < nblk; b += 16) {
-            const float *o = part + (size_t)b * 3 * kLv;
+            const float *o = part + (size_t)b * 4 * kLv;
             n += o[L];
             sum += o[kLv + L];
             mx = fmaxf(mx, o[2 * kLv + L]);
+            leaves += o[3 * kLv + L];
         }
-        const int j0 = (int)hdr->lev_off[L], j1 = (int)hdr->lev_off[L + 1];
-        for (int j = j0 + sub; j < j1; j += 16) leaves += (c_flags[j] & kLeafBit) ? 1.f : 0.f;
     }
     for (int m = 1; m < 16; m <<= 1) {
         n += __shfl_xor(n, m, kWave);
@@ -111,6 +126,13 @@ __global__ void __launch_bounds__(kFeatBlock)
         sm[L] = mx;
         sl[L] = leaves;
     }
+    __syncthreads();
+    if (threadIdx.x == 0)  // suffix over deeper leaf levels
+        for (int l = kLv - 2; l >= 0; --l) {
+            sn[l] += sn[l + 1];
+            ss[l] += ss[l + 1];
+            sm[l] = fmaxf(sm[l], sm[l + 1]);
+        }
     __syncthreads();
     if (threadIdx.x >= kLv) return;
     const int l = threadIdx.x;
@@ -218,7 +240,7 @@ __global__ void __launch_bounds__(256)
         }
         const float lse_e = me + logf(se), lse_t = mt + logf(st);
         const float de = lse_e - y[2 * s], dt = lse_t - y[2 * s + 1];
-        ls = de * de + w_time * dt * dt;
+        ls = (de * de + w_time * dt * dt) / (float)nseg;  // the mean over solves
         const float ge = 2.f * de / (float)nseg, gt = 2.f * w_time * dt / (float)nseg;
         for (int r = r0; r < r1; ++r) {
             dout[2 * r] = ge * expf(out[2 * r] - lse_e);
@@ -351,7 +373,8 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // selection (one block)
 // ---------------------------------------------------------------------------
-constexpr int kSelBlock = 256;
+constexpr int kSelBlock = 1024;
+constexpr int kSelRows = 64;  // feature rows per pass through the layers
 constexpr int kMaxCand = 16;  // orders p_min .. p_min + 15
 
 __global__ void __launch_bounds__(kSelBlock)
@@ -359,6 +382,7 @@ __global__ void __launch_bounds__(kSelBlock)
              int fixed_p, DevHdr *hdr_w) {
     __shared__ float sp[kSelParams];
     __shared__ float se[kLv][kMaxCand], st[kLv][kMaxCand];
+    __shared__ float sx[kSelRows][F], sh1[kSelRows][H + 1], sh2[kSelRows][H + 1];
     __shared__ int s_bad;
     load_params(params, sp);
     if (threadIdx.x == 0) s_bad = 0;
@@ -374,74 +398,112 @@ __global__ void __launch_bounds__(kSelBlock)
         const float span = M.hi[i] - M.lo[i];
         if (v < M.lo[i] - M.margin * span || v > M.hi[i] + M.margin * span) s_bad = 1;
     }
-    // predictions for every (level, order); levels 0 and 1 use order 0
+    // predictions for every (level, order), 64 rows at a time through the
+    // layers with one thread per (row, unit); row r is level r / np, order
+    // p_min + r % np (levels 0 and 1 carry order 0, only column 0 is used)
+    const int R = nrow * np;
+    for (int r0 = 0; r0 < R; r0 += kSelRows) {
+        const int rc = min(kSelRows, R - r0);
+        for (int k = threadIdx.x; k < rc * F; k += blockDim.x) {
+            const int rr = k / F, i = k % F, r = r0 + rr;
+            const int L = r / np, pi = r % np;
+            const float raw = i == F - 1 ? (L < 2 ? 0.f : (float)(M.p_min + pi)) : feats[L * F + i];
+            sx[rr][i] = (raw - M.mu[i]) / M.sd[i];
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < rc * H; k += blockDim.x) {
+            const int rr = k / H, j = k % H;
+            float a = sp[ob1 + j];
+#pragma unroll
+            for (int i = 0; i < F; ++i) a += sx[rr][i] * sp[oW1 + i * H + j];
+            sh1[rr][j] = tanhf(a);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < rc * H; k += blockDim.x) {
+            const int rr = k / H, j = k % H;
+            float a = sp[ob2 + j];
+#pragma unroll 16
+            for (int i = 0; i < H; ++i) a += sh1[rr][i] * sp[oW2 + i * H + j];
+            sh2[rr][j] = tanhf(a);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < rc * 2; k += blockDim.x) {
+            const int rr = k / 2, o = k % 2, r = r0 + rr;
+            float a = sp[ob3 + o];
+#pragma unroll 16
+            for (int i = 0; i < H; ++i) a += sh2[rr][i] * sp[oW3 + 2 * i + o];
+            const int L = r / np, pi = r % np;
+            if (o == 0) se[L][pi] = a;
+            else st[L][pi] = a;
+        }
+        __syncthreads();
+    }
+    // exponentials once, in parallel
     for (int k = threadIdx.x; k < nrow * np; k += blockDim.x) {
         const int L = k / np, pi = k % np;
-        const int p = L < 2 ? 0 : M.p_min + pi;
         if (L < 2 && pi > 0) continue;
-        float xr[F], o0, o1;
-#pragma unroll
-        for (int i = 0; i < F; ++i) {
-            const float raw = i == F - 1 ? (float)p : feats[L * F + i];
-            xr[i] = (raw - M.mu[i]) / M.sd[i];
-        }
-        mlp_eval(sp, xr, nullptr, nullptr, o0, o1);
-        se[L][pi] = o0;
-        st[L][pi] = o1;
+        se[L][pi] = expf(se[L][pi]);
+        st[L][pi] = expf(st[L][pi]);
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    const float tgt = 2.f * logf(M.target_err);
-    // constant part: levels 0 and 1
-    float c_err = 0.f, c_time = 0.f;
-    for (int L = 0; L < min(2, nrow); ++L) {
-        c_err += expf(se[L][0]);
-        c_time += expf(st[L][0]);
-    }
-    int choice[kLv];
+    // Lagrangian bisection on wavefront 0: lane L owns level L
+    if (threadIdx.x >= kWave) return;
+    const int lane = threadIdx.x;
+    const bool free_lvl = lane >= 2 && lane < nrow;
+    const float c_e = (lane < 2 && lane < nrow) ? se[lane][0] : 0.f;
+    const float c_t = (lane < 2 && lane < nrow) ? st[lane][0] : 0.f;
+    auto wave_sum = [](float v) {
+        for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+        return v;
+    };
+    int choice = 0;
+    // sum over levels of the chosen (error^2, time) for multiplier exp(lam)
     auto pick = [&](float lam, float &err2, float &time) {
-        err2 = c_err;
-        time = c_time;
-        for (int L = 2; L < nrow; ++L) {
-            int best = 0;
+        float e = c_e, t = c_t;
+        if (free_lvl) {
+            const float scale = expf(lam);
             float bv = INFINITY;
             for (int pi = 0; pi < np; ++pi) {
-                const float v = expf(st[L][pi]) + expf(lam + se[L][pi]);
+                const float v = st[lane][pi] + scale * se[lane][pi];
                 if (v < bv) {
                     bv = v;
-                    best = pi;
+                    choice = pi;
                 }
             }
-            choice[L] = best;
-            err2 += expf(se[L][best]);
-            time += expf(st[L][best]);
+            e = se[lane][choice];
+            t = st[lane][choice];
         }
+        err2 = wave_sum(e);
+        time = wave_sum(t);
     };
+    const float tgt = M.target_err * M.target_err;
     float err2, time;
     int source = 1;
-    // feasibility at the largest multiplier
     float hi = 80.f, lo = -10.f;
-    pick(hi, err2, time);
-    if (s_bad || !(logf(err2) <= tgt)) {
+    pick(hi, err2, time);  // feasibility at the largest multiplier
+    if (s_bad || !(err2 <= tgt)) {
         source = 2;
     } else {
         for (int it = 0; it < 60; ++it) {
             const float mid = 0.5f * (lo + hi);
             pick(mid, err2, time);
-            if (logf(err2) <= tgt) hi = mid;
+            if (err2 <= tgt) hi = mid;
             else lo = mid;
         }
         pick(hi, err2, time);
     }
-    for (int L = 0; L < kMaxDepth + 2; ++L) {
+    const int p2 = __shfl(M.p_min + choice, 2, kWave);  // level 2's order (levels 0, 1)
+    for (int L = lane; L < kMaxDepth + 2; L += kWave) {
         int p = fixed_p;
-        if (source == 1 && L >= 2 && L < nrow) p = M.p_min + choice[L];
-        else if (source == 1 && L < 2 && nrow > 2) p = M.p_min + choice[2];
+        if (source == 1 && L >= 2 && L < nrow) p = M.p_min + choice;
+        else if (source == 1 && L < 2 && nrow > 2) p = p2;
         hdr_w->sel_p[L] = (uint8_t)p;
     }
-    hdr_w->sel_source = (uint32_t)source;
-    hdr_w->sel_pred_err = source == 1 ? sqrtf(err2) : -1.f;
-    hdr_w->sel_pred_ms = source == 1 ? time : -1.f;
+    if (lane == 0) {
+        hdr_w->sel_source = (uint32_t)source;
+        hdr_w->sel_pred_err = source == 1 ? sqrtf(err2) : -1.f;
+        hdr_w->sel_pred_ms = source == 1 ? time : -1.f;
+    }
 }
 
 }  // namespace
@@ -452,18 +514,12 @@ __global__ void __launch_bounds__(kSelBlock)
 void level_features(Ctx &c, int threshold, float *feats_dev, float *raw_dev) {
     hipStream_t s = c.stream;
     const TreeInfo &ti = c.info;
-    const int nl = ti.n_leaves;
-    const int nb = div_up(nl, kStatBlock);
-    float *part = c.sel_part.as<float>((size_t)std::max(nb, 1) * 3 * kLv);
-    if (nb > 0)
-        k_level_stats<<<nb, kStatBlock, 0, s>>>((const uint32_t *)c.leaf_idx.p, nl,
-                                                (const uint8_t *)c.c_level.p,
-                                                (const uint32_t *)c.c_begin.p,
-                                                (const uint32_t *)c.c_end.p,
-                                                (const double *)c.qs.p, part);
-    k_level_feats<<<1, kFeatBlock, 0, s>>>(part, nb, c.hdr_d, (const uint8_t *)c.c_level.p,
-                                         (const uint8_t *)c.c_flags.p, threshold, raw_dev,
-                                         feats_dev);
+    const int nb = std::max(div_up(ti.n, kStatBlock * kStatRun), 1);
+    float *part = c.sel_part.as<float>((size_t)nb * 4 * kLv);
+    k_level_stats<<<nb, kStatBlock, 0, s>>>((const uint8_t *)c.leaf_lvl.p, (const double *)c.qs.p,
+                                            ti.n, (const uint8_t *)c.c_level.p,
+                                            (const uint8_t *)c.c_flags.p, ti.n_cells, part);
+    k_level_feats<<<1, kFeatBlock, 0, s>>>(part, nb, c.hdr_d, threshold, raw_dev, feats_dev);
     VPM_LAUNCH_CHECK();
 }
 

@@ -279,3 +279,38 @@ def test_fixed_order_judge_rejects_a_bad_model(gpu):
     u2, v2 = s.velocity_device(*dev)  # stays on the fixed order
     assert s.last_selection()["source"] == 3
     assert torch.equal(u0, u2) and torch.equal(v0, v2)
+
+
+def test_first_call_chooses_the_threshold_the_oracle_predicts(gpu):
+    """A fresh learned solver re-selects the leaf threshold on its first call
+    (every candidate tree built and scored on the device) and picks the one
+    with the smallest predicted time under the oracle's evaluation of the same
+    model on the same trees."""
+    import torch
+    from lsmps_vpm_v2_amd import FmmSolver, selector as S, workloads as W
+    if not os.path.exists(S.DEFAULT_MODEL):
+        pytest.skip("no trained selector model in the package yet")
+    model = S.load_model()
+    md = dict(params=model.params.astype(np.float64), mu=model.mu.astype(np.float64),
+              sd=model.sd.astype(np.float64), lo=model.lo.astype(np.float64),
+              hi=model.hi.astype(np.float64), margin=model.margin, p_min=model.p_min,
+              p_max=model.p_max, target_err=model.target_err)
+    f = W.boundary_layer()
+    dev = [torch.from_numpy(f[k]).cuda() for k in ("x", "y", "s", "omega", "active")]
+    pred = {}
+    for t in model.thresholds:
+        s = FmmSolver(p=12, max_par=t, max_src=t, precision="mixed", binom_mode=0, list4_mode=2)
+        s.velocity_device(*dev)
+        feats, _ = s.level_features()
+        _, src, _, pms = M.select(md, feats.astype(np.float64), 12)
+        if src == 1:
+            pred[t] = pms
+    assert pred, "the model should accept some candidate on this field"
+    learned = FmmSolver(p=12, precision="mixed", binom_mode=0, list4_mode=2, order="learned")
+    learned.velocity_device(*dev)
+    sel = learned.last_selection()
+    best = min(pred, key=pred.get)
+    print(f"oracle predictions {pred}; device chose {sel['threshold']} (source {sel['source']})")
+    assert sel["reselected"] == 1 and sel["source"] == 1
+    assert sel["threshold"] in pred
+    assert pred[sel["threshold"]] <= pred[best] * 1.01
