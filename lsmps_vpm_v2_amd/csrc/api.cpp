@@ -118,6 +118,10 @@ struct vpmfmm_handle_s {
     int sel_calls = 0;       // calls since the last re-selection
     int judge_calls = 0;     // learned calls since the last re-selection (judge cadence)
     bool judge_reject = false;  // the judge rejected the learned orders: fixed order until re-selection
+    // orders chosen by the last selection, reused while the tree depth is unchanged
+    bool sel_cached = false;
+    int sel_cached_maxl = -1;
+    DevHdr sel_cache{};  // sel_p, sel_source, sel_pred_* of that selection
     DBuf judge_a, judge_b, judge_part;
     vpmfmm_selection last_sel{};
     uint8_t h_orders[kMaxDepth + 2] = {};  // order table copied into the device header
@@ -243,6 +247,7 @@ void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *
     h->sel_calls = 0;
     h->judge_calls = 0;
     h->judge_reject = false;
+    h->sel_cached = false;
 }
 
 void build(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
@@ -273,7 +278,15 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     const TreeParams tp = tree_params(h, thr);
     mark(h, 0);
     tree_build(h->ctx, n, x, y, q, a, tp);
-    if (lrn) select_on_tree(h, tp.max_par);
+    // orders: a fresh selection after a threshold re-selection or when the
+    // tree depth changed, else the orders of the last selection
+    const bool reuse = lrn && h->sel_cached && h->sel_cached_maxl == h->ctx.info.max_level;
+    if (lrn && !reuse) select_on_tree(h, tp.max_par);
+    if (reuse && !h->judge_reject) {
+        std::memcpy(h->h_orders, h->sel_cache.sel_p, sizeof(h->h_orders));
+        VPM_HIP(hipMemcpyAsync(h->ctx.hdr_d->sel_p, h->h_orders, sizeof(h->h_orders),
+                               hipMemcpyHostToDevice, h->ctx.stream));
+    }
     if (!lrn || h->judge_reject) {
         // orders per level for the evaluation statistics of the list build
         for (int L = 0; L < kMaxDepth + 2; ++L) {
@@ -290,6 +303,20 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     tree_lists(h->ctx);
     mark(h, 2);
     h->last_sel.threshold = tp.max_par;
+    DevHdr &hh = *h->ctx.hdr_h;  // synchronised by the list build
+    if (lrn && !reuse) {
+        std::memcpy(h->sel_cache.sel_p, hh.sel_p, sizeof(hh.sel_p));
+        h->sel_cache.sel_source = hh.sel_source;
+        h->sel_cache.sel_pred_err = hh.sel_pred_err;
+        h->sel_cache.sel_pred_ms = hh.sel_pred_ms;
+        h->sel_cached = true;
+        h->sel_cached_maxl = h->ctx.info.max_level;
+    } else if (reuse) {  // report the orders in use
+        std::memcpy(hh.sel_p, h->sel_cache.sel_p, sizeof(hh.sel_p));
+        hh.sel_source = h->sel_cache.sel_source;
+        hh.sel_pred_err = h->sel_cache.sel_pred_err;
+        hh.sel_pred_ms = h->sel_cache.sel_pred_ms;
+    }
 }
 
 void finish_stats(vpmfmm_handle *h, bool fmm) {
@@ -861,6 +888,7 @@ int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const floa
             h->has_sel = false;
             h->sel_threshold = h->sel_n = h->sel_calls = h->judge_calls = 0;
             h->judge_reject = false;
+            h->sel_cached = false;
             return;
         }
         const vpmfmm_selector &m = *sel;
@@ -904,6 +932,7 @@ int vpmfmm_set_selector(vpmfmm_handle *h, const vpmfmm_selector *sel, const floa
         h->has_sel = true;
         h->sel_threshold = h->sel_n = h->sel_calls = h->judge_calls = 0;
         h->judge_reject = false;
+        h->sel_cached = false;
     });
 }
 

@@ -86,78 +86,146 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// P2M: thread per (leaf, k); a^_k = sum q ((z - zc)/s)^k over active particles.
-__global__ void __launch_bounds__(256)
+// P2M: one wavefront per leaf, lanes over its particles; lane-private running
+// powers accumulate a^_k = sum q ((z - zc)/s)^k (k <= p, active particles of
+// active leaves; inactive leaves get zero), reduced across lanes per order.
+constexpr int kP2mWaves = 4;
+
+__global__ void __launch_bounds__(kWave *kP2mWaves)
     k_p2m32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
             const uint32_t *c_begin, const uint32_t *c_end, const float2 *oxy, const float *qf,
             const uint8_t *as, const LevelTab *lt, int kst, float *ahat) {
     const int PM1 = kst / 2;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)nl * PM1) return;
-    const int t = (int)(idx / PM1), k = (int)(idx % PM1);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kP2mWaves + threadIdx.x / kWave;
+    if (t >= nl) return;
     const int j = (int)leaf_idx[t];
     const int L = c_level[j];
-    if (k > lt->p[L]) return;
+    const int p = lt->p[L];
     const float inv_s = lt->inv_s[L];
-    cf acc = {0.f, 0.f};
+    float ar[17], ai[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) ar[k] = ai[k] = 0.f;
     const uint32_t e = (c_flags[j] & kActive) ? c_end[j] : c_begin[j];  // inactive: zero
-    for (uint32_t i = c_begin[j]; i < e; ++i) {
+    for (uint32_t i = c_begin[j] + lane; i < e; i += kWave) {
         if (!as[i]) continue;
         const float2 o = oxy[i];
-        acc = cadd(acc, cscale(cpowi(cf{o.x * inv_s, o.y * inv_s}, k), qf[i]));
-    }
-    stc(ahat, (size_t)j * PM1 + k, acc);
-}
-
-// M2M of one (parent j, order m): m2m[q][m][k] complex, q = child quadrant,
-// stride PM1 x PM1; inactive parents get zero (their parents read them).
-__device__ __forceinline__ void m2m_cell(int j, int m, int PM1, const uint8_t *c_level,
-                                         const uint8_t *c_flags, const int *c_child,
-                                         const LevelTab *lt, const float *m2m, float *ahat) {
-    const unsigned f = c_flags[j];
-    const int L = c_level[j];
-    if ((f & kLeaf) || m > lt->p[L]) return;
-    cf acc = {0.f, 0.f};
-    if (f & kActive) {
-        const int pc = lt->p[L + 1];
-        for (int q = 0; q < 4; ++q) {
-            const int ch = c_child[4 * j + q];
-            if (ch < 0) continue;
-            const float *row = m2m + ((size_t)q * PM1 + m) * PM1 * 2;
-            const int kmax = min(m, pc);
-            for (int k = 0; k <= kmax; ++k)
-                acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(ahat, (size_t)ch * PM1 + k)));
+        const cf z = {o.x * inv_s, o.y * inv_s};
+        cf w = {qf[i], 0.f};
+#pragma unroll
+        for (int k = 0; k < 17; ++k) {
+            if (k <= p) {
+                ar[k] += w.x;
+                ai[k] += w.y;
+                w = cmul(w, z);
+            }
         }
     }
-    stc(ahat, (size_t)j * PM1 + m, acc);
+#pragma unroll
+    for (int k = 0; k < 17; ++k) {
+        if (k <= p) {
+            float xr = ar[k], xi = ai[k];
+            for (int m = 1; m < kWave; m <<= 1) {
+                xr += __shfl_xor(xr, m, kWave);
+                xi += __shfl_xor(xi, m, kWave);
+            }
+            if (lane == k) stc(ahat, (size_t)j * PM1 + k, cf{xr, xi});
+        }
+    }
 }
 
-// L2L into (child j, order m >= 1) from its parent: l2l[q][m][k], k >= m.
-__device__ __forceinline__ void l2l_cell(int j, int m, int PM1, const uint8_t *c_level,
-                                         const uint32_t *c_key, const int *c_parent,
-                                         const LevelTab *lt, const float *l2l, float *bhat) {
+// M2M and L2L by one wavefront per parent cell.  The children's (M2M) or the
+// parent's (L2L) coefficients are first loaded by all lanes at once into LDS,
+// then each lane forms one output order from LDS and the level-independent
+// table m2m[q][m][k] / l2l[q][m][k] (complex, stride PM1 x PM1, also in LDS).
+// q is the child quadrant, c_child's order: x bit + 2 * y bit.
+constexpr int kMaxPM1 = 17;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// M2M into parent j (inactive parents get zero: their own parents read them).
+__device__ __forceinline__ void m2m_wave(int j, int lane, int PM1, const uint8_t *c_level,
+                                         const uint8_t *c_flags, const int *c_child,
+                                         const LevelTab *lt, const float2 *tab, float2 *sa,
+                                         float *ahat) {
+    const unsigned f = c_flags[j];
+    if (f & kLeaf) return;
     const int L = c_level[j];
-    if (m > lt->p[L]) return;
-    const int par = c_parent[j];
-    const int pp = lt->p[L - 1];
-    const uint32_t key = c_key[j];
-    const int q = (int)(compact1by1(key) & 1u) + 2 * (int)(compact1by1(key >> 1) & 1u);
-    const float *row = l2l + ((size_t)q * PM1 + m) * PM1 * 2;
-    cf acc = {0.f, 0.f};
-    for (int k = m; k <= pp; ++k)
-        acc = cadd(acc, cmul(cf{row[2 * k], row[2 * k + 1]}, ldc(bhat, (size_t)par * PM1 + k)));
-    const size_t o = (size_t)j * PM1 + m;
-    stc(bhat, o, cadd(ldc(bhat, o), acc));
+    const int p = lt->p[L], pc = lt->p[L + 1];
+    const bool act = (f & kActive) != 0;
+    for (int idx = lane; idx < 4 * PM1; idx += kWave) {
+        const int q = idx / PM1, k = idx % PM1;
+        const int ch = act ? c_child[4 * j + q] : -1;
+        sa[idx] = (ch >= 0 && k <= pc)
+                      ? reinterpret_cast<const float2 *>(ahat)[(size_t)ch * PM1 + k]
+                      : make_float2(0.f, 0.f);
+    }
+    wave_sync();
+    if (lane <= p) {
+        const int m = lane, kmax = min(m, pc);
+        cf acc = {0.f, 0.f};
+        for (int q = 0; q < 4; ++q) {
+            const float2 *row = tab + (q * PM1 + m) * PM1;
+            for (int k = 0; k <= kmax; ++k) {
+                const float2 tv = row[k], av = sa[q * PM1 + k];
+                acc = cadd(acc, cmul(cf{tv.x, tv.y}, cf{av.x, av.y}));
+            }
+        }
+        stc(ahat, (size_t)j * PM1 + m, acc);
+    }
+    wave_sync();
 }
 
-// Subtree passes.  Cells are level-major and Morton-ordered inside a level, so
-// the descendants of a level-K cell with key r form, at each level L > K, the
-// contiguous range of keys [r 4^(L-K), (r+1) 4^(L-K)).  One workgroup owns one
-// level-K cell and walks its subtree level by level with workgroup barriers;
-// the cells above level K are done by one workgroup (k_*_top).
+// L2L from parent j into its children: lane = (q, m), q = lane / 16,
+// m = lane % 16 + 1, adds sum_{k >= m} l2l[q][m][k] b^_k(parent).
+__device__ __forceinline__ void l2l_wave(int j, int lane, int PM1, const uint8_t *c_level,
+                                         const uint8_t *c_flags, const int *c_child,
+                                         const LevelTab *lt, const float2 *tab, float2 *sb,
+                                         float *bhat) {
+    if (c_flags[j] & kLeaf) return;
+    const int L = c_level[j];
+    const int pp = lt->p[L], pc = lt->p[L + 1];
+    for (int k = lane; k < PM1; k += kWave)
+        sb[k] = k <= pp ? reinterpret_cast<const float2 *>(bhat)[(size_t)j * PM1 + k]
+                        : make_float2(0.f, 0.f);
+    wave_sync();
+    const int q = lane >> 4, m = (lane & 15) + 1;
+    const int ch = c_child[4 * j + q];
+    if (ch >= 0 && m <= pc) {
+        const float2 *row = tab + (q * PM1 + m) * PM1;
+        cf acc = {0.f, 0.f};
+        for (int k = m; k <= pp; ++k) {
+            const float2 tv = row[k], bv = sb[k];
+            acc = cadd(acc, cmul(cf{tv.x, tv.y}, cf{bv.x, bv.y}));
+        }
+        const size_t o = (size_t)ch * PM1 + m;
+        stc(bhat, o, cadd(ldc(bhat, o), acc));
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab) {
+    const float2 *src = reinterpret_cast<const float2 *>(g);
+    for (int k = threadIdx.x; k < 4 * PM1 * PM1; k += blockDim.x) tab[k] = src[k];
+    __syncthreads();
+}
+
+// Upward / downward passes in three tiers.  Cells are level-major and
+// Morton-ordered inside a level, so the descendants of a level-K cell with key
+// r form, at each level L > K, the contiguous range of keys
+// [r 4^(L-K), (r+1) 4^(L-K)).  Below the split level K (the first level with
+// kSubRoots cells) one workgroup owns one level-K cell and walks its subtree
+// level by level with workgroup barriers (a wavefront per parent cell); the
+// levels between the first level with kTopCells cells and K get one launch
+// each (a wavefront per cell over the whole GPU); the few cells above go to
+// one workgroup (k_*_top).
 constexpr int kSubBlock = 256;
 constexpr int kTopBlock = 1024;
-constexpr int kSubRoots = 256;  // subtree roots wanted at the split level
+constexpr int kSubRoots = 1024;  // subtree roots wanted at the split level K
+constexpr int kTopCells = 64;    // levels with fewer cells go to the one-workgroup kernel
 
 __device__ int lower_key(const uint32_t *c_key, int lo, int hi, unsigned long long v) {
     while (lo < hi) {
@@ -187,53 +255,87 @@ __global__ void __launch_bounds__(kSubBlock)
                   const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
                   const LevelTab *lt, const float *m2m, int kst, float *ahat) {
     __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(m2m, PM1, tab);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
     for (int L = maxL - 1; L >= K; --L) {
-        const int lo = s_lo[L], cnt = s_hi[L] - lo;
-        for (int idx = threadIdx.x; idx < cnt * PM1; idx += kSubBlock)
-            m2m_cell(lo + idx / PM1, idx % PM1, PM1, c_level, c_flags, c_child, lt, m2m, ahat);
+        for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
+            m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
         __syncthreads();
     }
 }
 
+// M2M for the parents of levels Lhi .. 1 (top levels, few cells): one workgroup.
 __global__ void __launch_bounds__(kTopBlock)
-    k_m2m_top(int K, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
+    k_m2m_top(int Lhi, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
               const int *c_child, const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kTopBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
-    for (int L = K - 1; L >= 1; --L) {
-        const int lo = (int)hdr->lev_off[L], cnt = (int)hdr->lev_off[L + 1] - lo;
-        for (int idx = threadIdx.x; idx < cnt * PM1; idx += kTopBlock)
-            m2m_cell(lo + idx / PM1, idx % PM1, PM1, c_level, c_flags, c_child, lt, m2m, ahat);
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(m2m, PM1, tab);
+    for (int L = Lhi; L >= 1; --L) {
+        for (int j = (int)hdr->lev_off[L] + w; j < (int)hdr->lev_off[L + 1];
+             j += kTopBlock / kWave)
+            m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(kTopBlock)
-    k_l2l_top(int K, const DevHdr *hdr, const uint8_t *c_level, const uint32_t *c_key,
-              const int *c_parent, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+// M2M / L2L for the parent cells [j0, j1) of one level: wavefront per cell.
+__global__ void __launch_bounds__(kSubBlock)
+    k_m2m_level(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags,
+                const int *c_child, const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
-    for (int L = 3; L <= K; ++L) {
-        const int lo = (int)hdr->lev_off[L], cnt = (int)hdr->lev_off[L + 1] - lo;
-        for (int idx = threadIdx.x; idx < cnt * (PM1 - 1); idx += kTopBlock)
-            l2l_cell(lo + idx / (PM1 - 1), 1 + idx % (PM1 - 1), PM1, c_level, c_key, c_parent, lt,
-                     l2l, bhat);
-        __syncthreads();
-    }
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(m2m, PM1, tab);
+    const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
+    if (j < j1) m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
 }
 
 __global__ void __launch_bounds__(kSubBlock)
-    k_l2l_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
-                  const uint8_t *c_level, const int *c_parent, const LevelTab *lt,
-                  const float *l2l, int kst, float *bhat) {
-    __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    k_l2l_level(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags,
+                const int *c_child, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(l2l, PM1, tab);
+    const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
+    if (j < j1) l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
+}
+
+// L2L parents at levels 2 .. Lhi (children 3 .. Lhi + 1): one workgroup.
+__global__ void __launch_bounds__(kTopBlock)
+    k_l2l_top(int Lhi, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
+              const int *c_child, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kTopBlock / kWave][kMaxPM1];
+    const int PM1 = kst / 2;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(l2l, PM1, tab);
+    for (int L = 2; L <= Lhi; ++L) {
+        for (int j = (int)hdr->lev_off[L] + w; j < (int)hdr->lev_off[L + 1];
+             j += kTopBlock / kWave)
+            l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
+        __syncthreads();
+    }
+}
+
+// L2L parents at levels max(K, 2) .. maxL-1 of one subtree (children K+1 ..)
+__global__ void __launch_bounds__(kSubBlock)
+    k_l2l_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
+                  const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
+                  const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
+    const int PM1 = kst / 2;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    load_tab(l2l, PM1, tab);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
-    for (int L = max(K + 1, 3); L <= maxL; ++L) {
-        const int lo = s_lo[L], cnt = s_hi[L] - lo;
-        for (int idx = threadIdx.x; idx < cnt * (PM1 - 1); idx += kSubBlock)
-            l2l_cell(lo + idx / (PM1 - 1), 1 + idx % (PM1 - 1), PM1, c_level, c_key, c_parent, lt,
-                     l2l, bhat);
+    for (int L = max(K, 2); L <= maxL - 1; ++L) {
+        for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
+            l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
         __syncthreads();
     }
 }
@@ -688,23 +790,30 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     k_prep32<<<div_up(nl, 4), 256, 0, s>>>(leaf_idx, nl, c_begin, c_end, c_cx, c_cy,
                                            (const double *)c.xs.p, (const double *)c.ys.p,
                                            (const double *)c.qs.p, oxy, qf);
-    k_p2m32<<<div_up((long long)nl * PM1, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_begin,
-                                                             c_end, oxy, qf, as, mx.lt, kst, ahat);
-    // split level: the first level with at least kSubRoots cells (subtree
-    // workgroups below it, one workgroup for the levels above)
+    k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
+        leaf_idx, nl, c_level, c_flags, c_begin, c_end, oxy, qf, as, mx.lt, kst, ahat);
+    // split levels: K = first level with >= kSubRoots cells (subtrees below),
+    // T = first level with >= kTopCells cells (one workgroup above T)
     const int maxL = ti.max_level;
-    int K = maxL;
-    for (int L = 2; L <= maxL; ++L)
-        if ((int)(ti.lev_off[L + 1] - ti.lev_off[L]) >= kSubRoots) {
-            K = L;
-            break;
-        }
+    int K = maxL, T = maxL;
+    for (int L = maxL; L >= 1; --L) {
+        const int cnt = (int)(ti.lev_off[L + 1] - ti.lev_off[L]);
+        if (cnt >= kSubRoots) K = L;
+        if (cnt >= kTopCells) T = L;
+    }
+    T = std::min(T, K);
     const int nK = (int)(ti.lev_off[K + 1] - ti.lev_off[K]);
     if (K < maxL)
         k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_flags, c_child,
                                                mx.lt, mx.m2m, kst, ahat);
-    k_m2m_top<<<1, kTopBlock, 0, s>>>(K, c.hdr_d, c_level, c_flags, c_child, mx.lt, mx.m2m, kst,
-                                      ahat);
+    for (int L = K - 1; L >= std::max(T, 1); --L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        k_m2m_level<<<div_up(j1 - j0, kSubBlock / kWave), kSubBlock, 0, s>>>(
+            j0, j1, c_level, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
+    }
+    if (std::min(T, K) - 1 >= 1)
+        k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_level, c_flags, c_child,
+                                          mx.lt, mx.m2m, kst, ahat);
     mk.mark(3, s);
     // M2L list 2 on MFMA, one launch per order present
     for (int p = 1; p <= 16; ++p) {
@@ -726,11 +835,18 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     }
     mk.mark(4, s);
     if (maxL >= 3) {
-        k_l2l_top<<<1, kTopBlock, 0, s>>>(K, c.hdr_d, c_level, c_key, c_parent, mx.lt, mx.l2l, kst,
-                                          bhat);
+        // parents 2 .. T-1 (one workgroup), T .. K-1 (a launch per level), K .. (subtrees)
+        if (T - 1 >= 2)
+            k_l2l_top<<<1, kTopBlock, 0, s>>>(T - 1, c.hdr_d, c_level, c_flags, c_child, mx.lt,
+                                              mx.l2l, kst, bhat);
+        for (int L = std::max(T, 2); L <= K - 1; ++L) {
+            const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+            k_l2l_level<<<div_up(j1 - j0, kSubBlock / kWave), kSubBlock, 0, s>>>(
+                j0, j1, c_level, c_flags, c_child, mx.lt, mx.l2l, kst, bhat);
+        }
         if (K < maxL)
-            k_l2l_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_parent,
-                                                   mx.lt, mx.l2l, kst, bhat);
+            k_l2l_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_flags,
+                                                   c_child, mx.lt, mx.l2l, kst, bhat);
     }
     mk.mark(5, s);
     k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(

@@ -8,7 +8,7 @@
 // velocity error and t = ln of its contribution to the solve time (ms).  A
 // solve's predicted error^2 / time are the sums of exp(e) / exp(t) over its
 // levels.  The orders are chosen by minimising predicted time subject to
-// predicted error <= target (Lagrange multiplier, bisection), in one block on
+// predicted error <= target (Lagrange multiplier, 30 bisection steps), in one block on
 // the device, right after the tree build.  A fixed order (12 by default) is
 // the fallback whenever the model is absent, a feature lies outside the
 // training range, or no order choice meets the target.
@@ -373,8 +373,9 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // selection (one block)
 // ---------------------------------------------------------------------------
-constexpr int kSelBlock = 1024;
+constexpr int kSelBlock = 1024;  // 16 wavefronts = the 4 x 4 tiles of a 64-row layer pass
 constexpr int kSelRows = 64;  // feature rows per pass through the layers
+constexpr int kBisect = 30;   // halvings of the log-multiplier range [-10, 80]
 constexpr int kMaxCand = 16;  // orders p_min .. p_min + 15
 
 __global__ void __launch_bounds__(kSelBlock)
@@ -419,12 +420,28 @@ __global__ void __launch_bounds__(kSelBlock)
             sh1[rr][j] = tanhf(a);
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < rc * H; k += blockDim.x) {
-            const int rr = k / H, j = k % H;
-            float a = sp[ob2 + j];
-#pragma unroll 16
-            for (int i = 0; i < H; ++i) a += sh1[rr][i] * sp[oW2 + i * H + j];
-            sh2[rr][j] = tanhf(a);
+        {
+            // layer 2 on MFMA: wavefront w owns output tile (rows 16 (w / 4), units
+            // 16 (w % 4)); A = H1 rows, B = W2, K = 64 in steps of 4
+            // (lane layout: A[l % 16][l / 16], B[l / 16][l % 16],
+            //  D register v at lane l is D[4 (l / 16) + v][l % 16])
+            const int w = threadIdx.x / kWave, l = threadIdx.x & (kWave - 1);
+            const int tr = 16 * (w / 4), tc = 16 * (w % 4);
+            const int col = l % 16, kk = l / 16;
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            if (tr < rc) {
+#pragma unroll 4
+                for (int k0 = 0; k0 < H; k0 += 4) {
+                    const float av = sh1[tr + col][k0 + kk];
+                    const float bv = sp[oW2 + (k0 + kk) * H + tc + col];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = tr + 4 * kk + v, j = tc + col;
+                    sh2[rr][j] = tanhf(acc[v] + sp[ob2 + j]);
+                }
+            }
         }
         __syncthreads();
         for (int k = threadIdx.x; k < rc * 2; k += blockDim.x) {
@@ -484,7 +501,7 @@ __global__ void __launch_bounds__(kSelBlock)
     if (s_bad || !(err2 <= tgt)) {
         source = 2;
     } else {
-        for (int it = 0; it < 60; ++it) {
+        for (int it = 0; it < kBisect; ++it) {
             const float mid = 0.5f * (lo + hi);
             pick(mid, err2, time);
             if (err2 <= tgt) hi = mid;

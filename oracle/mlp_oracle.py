@@ -179,7 +179,7 @@ def select(model, feats, fixed_p):
 
 def select_tables(e, t, p_min, target_err, fixed_p, ok=True):
     """p_L(lam) = argmin_p exp(t[L, p]) + exp(lam + e[L, p]); the smallest lam
-    in [-10, 80] (60 halvings) with sum_L exp(e) <= target^2; rows 0 and 1
+    in [-10, 80] (30 halvings) with sum_L exp(e) <= target^2; rows 0 and 1
     contribute column 0 as constants."""
     nrow = len(e)
     tgt = 2 * math.log(target_err)
@@ -200,7 +200,7 @@ def select_tables(e, t, p_min, target_err, fixed_p, ok=True):
     ch, err2, tim = pick(hi)
     if not ok or not (math.log(err2) <= tgt):
         return [fixed_p] * 18, 2, -1.0, -1.0
-    for _ in range(60):
+    for _ in range(30):
         mid = 0.5 * (lo + hi)
         _, e2, _ = pick(mid)
         if math.log(e2) <= tgt:
