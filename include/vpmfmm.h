@@ -64,10 +64,11 @@ typedef struct vpmfmm_opts {
     int32_t order_select;   /* mixed path: vpmfmm_order_select (fixed / learned)        */
     int32_t reselect_every; /* learned: re-choose the leaf threshold every k calls
                                (0 = on the first call and when n changes by > 10%)      */
-    int32_t judge_every;    /* learned: every k-th call also solves at the fixed order p
-                               on the same tree and compares (0 = never); a difference
-                               above the model's target returns the fixed-order result
-                               and keeps the fixed order until the next re-selection    */
+    int32_t judge_every;    /* learned: the call that makes a fresh order selection (and,
+                               for k > 0, every k-th call) also solves at the fixed order
+                               p on the same tree and compares; a difference above the
+                               model's target returns the fixed-order result and keeps
+                               the fixed order until the next re-selection; -1 = never  */
     int32_t reserved[4];
 } vpmfmm_opts;
 

@@ -120,6 +120,7 @@ struct vpmfmm_handle_s {
     bool judge_reject = false;  // the judge rejected the learned orders: fixed order until re-selection
     // orders chosen by the last selection, reused while the tree depth is unchanged
     bool sel_cached = false;
+    bool sel_fresh = false;  // this call selected its orders (judged by default)
     int sel_cached_maxl = -1;
     DevHdr sel_cache{};  // sel_p, sel_source, sel_pred_* of that selection
     DBuf judge_a, judge_b, judge_part;
@@ -156,7 +157,7 @@ int opts_check(const vpmfmm_opts &o, std::string &msg) {
     else if (o.order_select != VPMFMM_ORDER_FIXED && o.order_select != VPMFMM_ORDER_LEARNED)
         msg = "order_select must be 0 (fixed) or 1 (learned)";
     else if (o.reselect_every < 0) msg = "reselect_every must be >= 0";
-    else if (o.judge_every < 0) msg = "judge_every must be >= 0";
+    else if (o.judge_every < -1) msg = "judge_every must be >= -1";
     else return VPMFMM_OK;
     return VPMFMM_BAD_ARG;
 }
@@ -282,6 +283,7 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     // tree depth changed, else the orders of the last selection
     const bool reuse = lrn && h->sel_cached && h->sel_cached_maxl == h->ctx.info.max_level;
     if (lrn && !reuse) select_on_tree(h, tp.max_par);
+    h->sel_fresh = lrn && !reuse;
     if (reuse && !h->judge_reject) {
         std::memcpy(h->h_orders, h->sel_cache.sel_p, sizeof(h->h_orders));
         VPM_HIP(hipMemcpyAsync(h->ctx.hdr_d->sel_p, h->h_orders, sizeof(h->h_orders),
@@ -471,9 +473,13 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double
             ls.p[L] = p;
         }
         fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
-        // the fixed-order judge (same tree): every judge_every-th learned call
-        if (lrn && h->o.judge_every > 0 && (h->judge_calls++ % h->o.judge_every) == 0 &&
-            c.hdr_h->sel_source == 1) {
+        // the fixed-order judge (same tree): every fresh selection, and every
+        // judge_every-th learned call
+        const bool judge_due =
+            h->o.judge_every >= 0 &&
+            (h->sel_fresh || (h->o.judge_every > 0 && (h->judge_calls % h->o.judge_every) == 0));
+        ++h->judge_calls;
+        if (lrn && judge_due && c.hdr_h->sel_source == 1) {
             const int nn = c.info.n;
             // buffers may grow below: let the learned solve finish first
             VPM_HIP(hipStreamSynchronize(c.stream));

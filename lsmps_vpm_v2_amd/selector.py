@@ -198,12 +198,15 @@ def sweep(fields, thresholds=(32, 48, 64, 96, 128), n_orders=8, p_range=(4, 16),
                 meta=meta)
 
 
-def training_fields(seed: int = 0, n_sub: int = 8, sub_size: int = 50000):
-    """Fields of the sweep: the benchmark fields at full size and seeded
-    ``sub_size``-particle random subsamples of them (sparser trees)."""
+def training_fields(seed: int = 0, n_sub: int = 10, sub_size: int = 50000):
+    """Fields of the sweep: the uniform, boundary-layer, shed-vortex, cluster
+    and wake (1M-particle von Karman street) fields, seeded ``sub_size``-
+    particle random subsamples of them (sparser trees), and small fields
+    (5k / 20k clusters, the Perlman vortex) for shallow trees."""
     from . import workloads as W
     full = [W.uniform_vortices(), W.boundary_layer(), W.shed_vortices(seed=11),
-            W.random_clusters(200000, seed=3)]
+            W.random_clusters(200000, seed=3), W.von_karman(seed=6, n_total=1_000_000)]
+    small = [W.random_clusters(5000, seed=9), W.random_clusters(20000, seed=10), W.perlman()]
     rng = np.random.Generator(np.random.PCG64(seed))
     subs = []
     for k in range(n_sub):
@@ -214,7 +217,7 @@ def training_fields(seed: int = 0, n_sub: int = 8, sub_size: int = 50000):
         d["n"] = len(idx)
         d["name"] = f"{src['name']}_sub{k}"
         subs.append(d)
-    return full + subs
+    return full + subs + small
 
 
 # ---------------------------------------------------------------------------

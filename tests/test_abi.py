@@ -97,7 +97,7 @@ def test_invalid_options_are_rejected_before_any_device_call():
     for field, value in (("p", 0), ("p", 41), ("max_par", 0), ("max_par", 5000),
                          ("exp_tree", 0.0), ("min_level", 0), ("max_level", 17),
                          ("binom_mode", 2), ("list4_mode", 3), ("stage_mask", 8),
-                         ("order_select", 2), ("reselect_every", -1)):
+                         ("order_select", 2), ("reselect_every", -1), ("judge_every", -2)):
         o = _lib.default_opts()
         setattr(o, field, value)
         h = C.c_void_p()

@@ -132,7 +132,7 @@ def _model_for(feats, seed, target_scale):
     e, t = M.predictions(m, feats)
     lo2 = sum(np.nanmin(np.exp(e[L])) for L in range(len(e)))
     hi2 = sum(np.nanmax(np.exp(e[L])) for L in range(len(e)))
-    m["target_err"] = math.sqrt(lo2 + target_scale * (hi2 - lo2))
+    m["target_err"] = math.sqrt(lo2 + target_scale * (hi2 - lo2)) / M.SEL_AIM
     return m
 
 
@@ -156,7 +156,7 @@ def test_device_selection_matches_oracle(gpu, scale):
     feats, _ = s.level_features()
     m = _model_for(feats.astype(np.float64), 5, scale)
     _install(s, m, [48])
-    s.set_opts(order_select=1)
+    s.set_opts(order_select=1, judge_every=-1)  # the selection itself, not the judge
     s.velocity_device(*dev)
     sel = s.last_selection()
     orders, src, perr, pms = M.select(m, feats.astype(np.float64), 12)
@@ -175,7 +175,7 @@ def test_device_selection_matches_oracle(gpu, scale):
     if list(sel["p"][2:nrow]) != list(orders[2:nrow]):
         e2, tm = cost(sel["p"])
         _, tref = cost(orders)
-        assert e2 <= m["target_err"] ** 2 * (1 + 1e-3) and tm <= tref * (1 + 1e-3)
+        assert e2 <= (M.SEL_AIM * m["target_err"]) ** 2 * (1 + 1e-3) and tm <= tref * (1 + 1e-3)
     assert abs(sel["pred_err"] - perr) <= 1e-3 * perr
     assert abs(sel["pred_ms"] - pms) <= 1e-3 * pms
 
@@ -277,8 +277,7 @@ def test_fixed_order_judge_rejects_a_bad_model(gpu):
     big = np.full(9, 1e6, np.float32)
     model = S.Model(params=params, mu=np.zeros(9, np.float32), sd=np.ones(9, np.float32),
                     lo=-big, hi=big, p_min=4, p_max=16, thresholds=[48], target_err=1e-4)
-    s = FmmSolver(order="learned", selector=model, **kw)
-    s.set_opts(judge_every=1)
+    s = FmmSolver(order="learned", selector=model, **kw)  # default: judge fresh selections
     u1, v1 = s.velocity_device(*dev)
     sel = s.last_selection()
     print(f"judge: source {sel['source']} judge_err {sel['judge_err']:.2e}")
@@ -315,6 +314,7 @@ def test_first_call_chooses_the_threshold_the_oracle_predicts(gpu):
             pred[t] = pms
     assert pred, "the model should accept some candidate on this field"
     learned = FmmSolver(p=12, precision="mixed", binom_mode=0, list4_mode=2, order="learned")
+    learned.set_opts(judge_every=-1)  # the threshold choice, not the judge
     learned.velocity_device(*dev)
     sel = learned.last_selection()
     best = min(pred, key=pred.get)
