@@ -408,17 +408,20 @@ Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     }
     // tiles of 16 consecutive cells per level (levels >= 2), grouped by order
     std::vector<int2> tl;
+    std::vector<uint8_t> tp;
     for (int p = 1; p <= 16; ++p) {
-        mx.tile_off[p] = (int)tl.size();
         for (int L = 2; L <= ti.max_level; ++L) {
             if (lt.p[L] != p) continue;
             const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
-            for (int j = j0; j < j1; j += 16) tl.push_back(make_int2(j, std::min(16, j1 - j)));
+            for (int j = j0; j < j1; j += 16) {
+                tl.push_back(make_int2(j, std::min(16, j1 - j)));
+                tp.push_back((uint8_t)p);
+            }
         }
     }
-    mx.tile_off[17] = (int)tl.size();
     int2 *td = c.mx_tiles.as<int2>(tl.size() + 1);
-    const size_t need = sizeof(LevelTab) + tl.size() * sizeof(int2);
+    uint8_t *tpd = c.mx_tile_p.as<uint8_t>(tp.size() + 1);
+    const size_t need = sizeof(LevelTab) + tl.size() * sizeof(int2) + tp.size() + 16;
     if (h->pin_cap < need) {
         if (h->pin) VPM_HIP(hipHostFree(h->pin));
         h->pin = nullptr;
@@ -428,11 +431,14 @@ Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     }
     LevelTab *lt_pin = (LevelTab *)h->pin;
     int2 *tl_pin = (int2 *)((char *)h->pin + sizeof(LevelTab));
+    uint8_t *tp_pin = (uint8_t *)(tl_pin + tl.size());
     *lt_pin = lt;
     if (!tl.empty()) {
         std::memcpy(tl_pin, tl.data(), tl.size() * sizeof(int2));
+        std::memcpy(tp_pin, tp.data(), tp.size());
         VPM_HIP(hipMemcpyAsync(td, tl_pin, tl.size() * sizeof(int2), hipMemcpyHostToDevice,
                                c.stream));
+        VPM_HIP(hipMemcpyAsync(tpd, tp_pin, tp.size(), hipMemcpyHostToDevice, c.stream));
     }
     LevelTab *ltd = c.mx_lt.as<LevelTab>(1);
     VPM_HIP(hipMemcpyAsync(ltd, lt_pin, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
@@ -443,6 +449,8 @@ Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     mx.tm2l = (const float *)c.mx_tm2l.p;
     for (int p = 0; p <= 16; ++p) mx.rptr_host[p] = h->mx_rptr_host[p];
     mx.tiles = td;
+    mx.tile_p = tpd;
+    mx.ntiles = (int)tl.size();
     return mx;
 }
 

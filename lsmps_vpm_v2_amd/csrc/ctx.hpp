@@ -105,7 +105,7 @@ struct Ctx {
     DBuf oxy, qf;
     DBuf sel_part, sel_feats, sel_raw;  // level statistics of the selection
     DBuf part;  // per-block partial results of the tree and list kernels
-    DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_tiles;
+    DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_tiles, mx_tile_p;
 
     TreeInfo info;
     std::vector<double> stage_ms;

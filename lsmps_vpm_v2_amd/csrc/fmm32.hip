@@ -346,24 +346,20 @@ __global__ void __launch_bounds__(kSubBlock)
 // (|dx|,|dy| <= 3, not both <= 1) is the MFMA A operand, read from a table in
 // lane order; the B operand is the source cell's scaled multipole (a^_k re/im
 // along K = 2(P+1)).  An offset no target of the tile uses is skipped.
+// One tile (wavefront-local: the source map is private to the wavefront, so
+// the tiles of one workgroup may have different orders).
 template <int P>
-__global__ void __launch_bounds__(256)
-    k_m2l_mfma(const int2 *tiles, int ntiles, const uint32_t *c_key, const int *l2,
-               const uint8_t *n2, const float *R, const float *ahat, int kst, float *bhat) {
+__device__ __forceinline__ void m2l_tile(int j0, int cnt, int lane, int (*smap)[49],
+                                         const uint32_t *c_key, const int *l2, const uint8_t *n2,
+                                         const float *R, const float *ahat, int kst,
+                                         float *bhat) {
     constexpr int K = 2 * (P + 1), KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
-    __shared__ int smap[4][16][49];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int tile = blockIdx.x * 4 + w;
-    const bool live = tile < ntiles;
-    const int2 td = live ? tiles[tile] : make_int2(0, 0);
-    const int j0 = td.x, cnt = td.y;
     const int n = lane % 16, kq = lane / 16;
     const int j = j0 + n;
-    const bool tvalid = live && n < cnt;
     // source map of this tile: slot (dx+3) + 7 (dy+3) -> source cell or -1
-    for (int s = kq; s < 49; s += 4) smap[w][n][s] = -1;
-    __syncthreads();
-    if (tvalid) {
+    for (int s = kq; s < 49; s += 4) smap[n][s] = -1;
+    wave_sync();
+    if (n < cnt) {
         const uint32_t kj = c_key[j];
         const int jx = (int)compact1by1(kj), jy = (int)compact1by1(kj >> 1);
         const int m2 = n2[j];
@@ -371,11 +367,10 @@ __global__ void __launch_bounds__(256)
             const int sidx = l2[27 * j + u];
             const uint32_t ks = c_key[sidx];
             const int dx = (int)compact1by1(ks) - jx, dy = (int)compact1by1(ks >> 1) - jy;
-            smap[w][n][(dx + 3) + 7 * (dy + 3)] = sidx;
+            smap[n][(dx + 3) + 7 * (dy + 3)] = sidx;
         }
     }
-    __syncthreads();
-    if (!live) return;
+    wave_sync();
     v4f acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -383,7 +378,7 @@ __global__ void __launch_bounds__(256)
     for (int slot = 0; slot < 49; ++slot) {
         const int dx = slot % 7 - 3, dy = slot / 7 - 3;
         if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
-        const int sidx = smap[w][n][slot];
+        const int sidx = smap[n][slot];
         const int oc = o++;
         if (__ballot(sidx >= 0) == 0) continue;  // no target of the tile uses this offset
         const float *src = ahat + (size_t)(sidx < 0 ? 0 : sidx) * kst;
@@ -411,21 +406,35 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-template <int P>
-void launch_m2l(const int2 *tiles, int ntiles, const uint32_t *c_key, const int *l2,
-                const uint8_t *n2, const float *R, const float *ahat, int kst, float *bhat,
-                hipStream_t s) {
-    k_m2l_mfma<P><<<div_up(ntiles, 4), 256, 0, s>>>(tiles, ntiles, c_key, l2, n2, R, ahat, kst,
-                                                     bhat);
-}
+// Table addresses per order (kernel argument).
+struct M2lTabs {
+    const float *r[17];
+};
 
-typedef void (*M2lLaunch)(const int2 *, int, const uint32_t *, const int *, const uint8_t *,
-                          const float *, const float *, int, float *, hipStream_t);
-const M2lLaunch kM2l[17] = {nullptr,        launch_m2l<1>,  launch_m2l<2>,  launch_m2l<3>,
-                            launch_m2l<4>,  launch_m2l<5>,  launch_m2l<6>,  launch_m2l<7>,
-                            launch_m2l<8>,  launch_m2l<9>,  launch_m2l<10>, launch_m2l<11>,
-                            launch_m2l<12>, launch_m2l<13>, launch_m2l<14>, launch_m2l<15>,
-                            launch_m2l<16>};
+// All tiles of every order in one launch: a wavefront per tile, dispatched on
+// the tile's order (tiles are grouped by order, so neighbours mostly agree).
+__global__ void __launch_bounds__(256)
+    k_m2l_mfma(const int2 *tiles, const uint8_t *tile_p, int ntiles, const uint32_t *c_key,
+               const int *l2, const uint8_t *n2, M2lTabs tabs, const float *ahat, int kst,
+               float *bhat) {
+    __shared__ int smap[4][16][49];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int tile = blockIdx.x * 4 + w;
+    if (tile >= ntiles) return;
+    const int2 td = tiles[tile];
+    const int p = tile_p[tile];
+#define M2L_CASE(P)                                                                          \
+    case P:                                                                                  \
+        m2l_tile<P>(td.x, td.y, lane, smap[w], c_key, l2, n2, tabs.r[P], ahat, kst, bhat);    \
+        break;
+    switch (p) {
+        M2L_CASE(1) M2L_CASE(2) M2L_CASE(3) M2L_CASE(4) M2L_CASE(5) M2L_CASE(6) M2L_CASE(7)
+        M2L_CASE(8) M2L_CASE(9) M2L_CASE(10) M2L_CASE(11) M2L_CASE(12) M2L_CASE(13)
+        M2L_CASE(14) M2L_CASE(15) M2L_CASE(16)
+        default: break;
+    }
+#undef M2L_CASE
+}
 
 // List 4, mode 2: P2L from the active particles of coarser leaves,
 //   b^_m += -q/m (s_t / (z_j - z_t))^m.
@@ -815,12 +824,14 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
         k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_level, c_flags, c_child,
                                           mx.lt, mx.m2m, kst, ahat);
     mk.mark(3, s);
-    // M2L list 2 on MFMA, one launch per order present
-    for (int p = 1; p <= 16; ++p) {
-        const int t0 = mx.tile_off[p], t1 = mx.tile_off[p + 1];
-        if (t1 > t0)
-            kM2l[p](mx.tiles + t0, t1 - t0, c_key, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
-                    mx.rptr_host[p], ahat, kst, bhat, s);
+    // M2L list 2 on MFMA: every tile in one launch
+    if (mx.ntiles > 0) {
+        M2lTabs tabs;
+        for (int p = 0; p <= 16; ++p) tabs.r[p] = mx.rptr_host[p];
+        k_m2l_mfma<<<div_up(mx.ntiles, 4), 256, 0, s>>>(mx.tiles, mx.tile_p, mx.ntiles, c_key,
+                                                         (const int *)c.l2.p,
+                                                         (const uint8_t *)c.n2.p, tabs, ahat, kst,
+                                                         bhat);
     }
     const int j2 = (int)ti.lev_off[2];
     if (nc > j2) {

@@ -21,7 +21,8 @@ struct Mixed32 {
     const float *m2m = nullptr, *l2l = nullptr, *tm2l = nullptr;
     const float *rptr_host[17] = {};   // device pointer of the MFMA-ordered M2L matrices per p
     const int2 *tiles = nullptr;       // (first cell, count) of 16-cell tiles, grouped by order
-    int tile_off[18] = {};             // tiles of order p: tile_off[p] .. tile_off[p + 1] - 1
+    const uint8_t *tile_p = nullptr;   // order of each tile
+    int ntiles = 0;
 };
 
 // Host builders of the level-independent tables (see fmm32.hip).
