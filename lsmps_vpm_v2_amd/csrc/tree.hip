@@ -311,25 +311,33 @@ constexpr int kSplitBlock = 256;
 // Sliding-window maxima by the van Herk / Gil-Werman method: with blocks of
 // length B aligned at 0, the max over [t, t + B - 1] is max(h[t], g[t + B - 1])
 // where g / h are block-prefix / block-suffix maxima.  Entries outside the
-// valid range hold 0, which never raises a maximum of valid lcp values.
-__device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *g, uint8_t *h,
-                                                    int len, int B) {
-    const int nb = (len + B - 1) / B;
-    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) {
-        const int blk = b < nb ? b : b - nb;
-        const int lo = blk * B, hi = min(lo + B, len);
-        uint8_t m = 0;
-        if (b < nb) {
-            for (int x = lo; x < hi; ++x) {
-                m = max(m, a[x]);
-                g[x] = m;
-            }
-        } else {
-            for (int x = hi - 1; x >= lo; --x) {
-                m = max(m, a[x]);
-                h[x] = m;
-            }
+// valid range hold 0, which never raises a maximum of valid lcp values.  The
+// prefix / suffix maxima are segmented Hillis-Steele scans over the whole
+// workgroup (log2 B steps, ping-pong buffers); g and h return pointing at the
+// buffers that hold the results.
+__device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *&g, uint8_t *&h,
+                                                    uint8_t *g2, uint8_t *h2, int len, int B) {
+    for (int x = threadIdx.x; x < len; x += blockDim.x) {
+        g[x] = a[x];
+        h[x] = a[x];
+    }
+    __syncthreads();
+    for (int dd = 1; dd < B; dd <<= 1) {
+        for (int x = threadIdx.x; x < len; x += blockDim.x) {
+            const int off = x % B;  // position inside its block
+            uint8_t gv = g[x], hv = h[x];
+            if (off >= dd) gv = max(gv, g[x - dd]);
+            if (off + dd < B && x + dd < len) hv = max(hv, h[x + dd]);
+            g2[x] = gv;
+            h2[x] = hv;
         }
+        __syncthreads();
+        uint8_t *t = g;
+        g = g2;
+        g2 = t;
+        t = h;
+        h = h2;
+        h2 = t;
     }
 }
 
@@ -349,7 +357,7 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int wlen = W + kSplitBlock;
     const int wbase = i0 - W;
     const int wlast = n - 1 - W;  // last valid run start (< 0: no run of W+1 particles)
-    uint8_t *sw = lds, *gw = sw + wlen, *hw = gw + wlen;
+    uint8_t *sw = lds, *gw = sw + wlen, *hw = gw + wlen, *gw2 = hw + wlen, *hw2 = gw2 + wlen;
     for (int t = threadIdx.x; t < wlen; t += kSplitBlock) {
         const int j = wbase + t;
         sw[t] = (j >= 0 && j <= wlast) ? w[j] : 0;
@@ -359,14 +367,14 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int vbase = r_lo - S - 1;
     const int vlast = na - 1 - S;
     const int vlen = r_hi - vbase + 1;
-    uint8_t *sv = hw + wlen, *gv = sv + vlen, *hv = gv + vlen;
+    uint8_t *sv = hw2 + wlen, *gv = sv + vlen, *hv = gv + vlen, *gv2 = hv + vlen, *hv2 = gv2 + vlen;
     for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
         const int m = vbase + t;
         sv[t] = (m >= 0 && m <= vlast) ? v[m] : 0;
     }
     __syncthreads();
-    block_prefix_suffix(sw, gw, hw, wlen, W + 1);
-    if (S > 0) block_prefix_suffix(sv, gv, hv, vlen, S);
+    block_prefix_suffix(sw, gw, hw, gw2, hw2, wlen, W + 1);
+    if (S > 0) block_prefix_suffix(sv, gv, hv, gv2, hv2, vlen, S);
     __syncthreads();
     unsigned long long my_min = kMaxDepth + 1, my_max = 0;
     if (i < n) {
@@ -829,7 +837,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     k_lcp_v<<<div_up(n, 256), 256, 0, s>>>(key, apos, c.hdr_d, S, n, v);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
-    const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
+    const size_t lds = 5 * (size_t)(W + kSplitBlock) + 5 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
     ulonglong2 *spart = c.part.as<ulonglong2>(nbs);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
@@ -903,8 +911,8 @@ void tree_lists(Ctx &c) {
     int *l2 = c.l2.as<int>(27 * (size_t)nc), *l4 = c.l4.as<int>(8 * (size_t)nc);
     uint8_t *n2 = c.n2.as<uint8_t>(nc), *n4 = c.n4.as<uint8_t>(nc);
     const int j0 = (int)ti.lev_off[2];
-    VPM_HIP(hipMemsetAsync(n2, 0, nc, s));
-    VPM_HIP(hipMemsetAsync(n4, 0, nc, s));
+    // n2 / n4 of every cell of level >= 2 are written by k_m2l_lists; levels 0
+    // and 1 have no lists 2 / 4 and nothing reads their entries
     const int nb2 = nc > j0 ? div_up(nc - j0, 256) : 0;
     const int nbl = div_up(nl, 256);
     ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, 2 * nbl), 1));
