@@ -85,7 +85,6 @@ void fill_tables(int P, int binom_mode, std::vector<double> &m2l, std::vector<do
         for (int k = p; k <= P; ++k) l2l[p * P1 + k] = B(k, p);
 }
 
-void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st);
 double rel_diff64(int n, const double *a, const double *b, const double *ja, const double *jb,
                   double a0, double b0, double *part_dev, hipStream_t s);
 
@@ -104,10 +103,10 @@ struct vpmfmm_handle_s {
     std::vector<int> level_p;  // per-level orders of the mixed path (empty: opts.p everywhere)
     int mx_pmax = -1;          // order the mixed tables were built for
     const float *mx_rptr_host[17] = {};  // device addresses of the M2L tables per order
-    // pinned staging of the per-call mixed-path uploads (tiles, level table);
-    // rewritten only after the next tree build has synchronised the stream
-    void *pin = nullptr;
-    size_t pin_cap = 0;
+    // fixed orders per level as last uploaded to the device slot (valid once
+    // fix_valid), so they are copied only when they change
+    uint8_t fix_orders[kMaxDepth + 2] = {};
+    bool fix_valid = false;
     hipEvent_t ev[8] = {};
     // learned selection
     bool has_sel = false;
@@ -125,7 +124,7 @@ struct vpmfmm_handle_s {
     DevHdr sel_cache{};  // sel_p, sel_source, sel_pred_* of that selection
     DBuf judge_a, judge_b, judge_part;
     vpmfmm_selection last_sel{};
-    uint8_t h_orders[kMaxDepth + 2] = {};  // order table copied into the device header
+    bool judged_reject_now = false;  // the judge of this call rejected the learned orders
     MlpWork mlp;
     DBuf mlp_grad;
     // inspection cache of the last tree (host copies)
@@ -214,6 +213,31 @@ TreeParams tree_params(const vpmfmm_handle *h, int threshold) {
     return tp;
 }
 
+// Device slots of the per-level orders: 0 the learned selection, 1 the fixed
+// orders of the options.
+uint8_t *learned_slot(vpmfmm_handle *h) { return h->ctx.orders_buf.as<uint8_t>(64); }
+
+// Fixed orders per level (opts.p, or the per-level orders of the mixed path),
+// uploaded to slot 1 when they differ from the last upload.
+const uint8_t *fixed_slot(vpmfmm_handle *h) {
+    uint8_t o[kMaxDepth + 2];
+    for (int L = 0; L < kMaxDepth + 2; ++L) {
+        int p = h->o.p;
+        if (h->o.precision == VPMFMM_MIXED && !learned(h) && L < (int)h->level_p.size() &&
+            h->level_p[L] > 0)
+            p = h->level_p[L];
+        o[L] = (uint8_t)std::min(p, 255);
+    }
+    uint8_t *slot = h->ctx.orders_buf.as<uint8_t>(64) + 32;
+    if (!h->fix_valid || std::memcmp(o, h->fix_orders, sizeof(o)) != 0) {
+        std::memcpy(h->fix_orders, o, sizeof(o));
+        VPM_HIP(hipMemcpyAsync(slot, h->fix_orders, sizeof(o), hipMemcpyHostToDevice,
+                               h->ctx.stream));
+        h->fix_valid = true;
+    }
+    return slot;
+}
+
 // Features of the tree just built and the selected orders, into the header
 // (read back at the next header synchronisation).
 void select_on_tree(vpmfmm_handle *h, int threshold) {
@@ -222,13 +246,13 @@ void select_on_tree(vpmfmm_handle *h, int threshold) {
     float *raw = c.sel_raw.as<float>((kMaxDepth + 1) * kSelRaw);
     level_features(c, threshold, feats, raw);
     select_orders(c, (const SelModel *)h->sel_model_d.p, (const float *)h->sel_params_d.p, feats,
-                  h->o.p);
+                  h->o.p, learned_slot(h));
 }
 
 // Learned path: choose the leaf threshold among the model's candidates by the
 // predicted solve time of each candidate tree (orders selected per tree).
 void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *y,
-                        const double *q, const uint8_t *a) {
+                        const Sources &q, const uint8_t *a) {
     Ctx &c = h->ctx;
     float best = INFINITY;
     int best_t = 0;
@@ -251,7 +275,7 @@ void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *
     h->sel_cached = false;
 }
 
-void build(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+void build(vpmfmm_handle *h, int n, const double *x, const double *y, const Sources &q,
            const uint8_t *a) {
     std::string msg;
     if (opts_check(h->o, msg)) throw Error(kBadArg, msg);
@@ -280,51 +304,39 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const doub
     mark(h, 0);
     tree_build(h->ctx, n, x, y, q, a, tp);
     // orders: a fresh selection after a threshold re-selection or when the
-    // tree depth changed, else the orders of the last selection
+    // tree depth changed, else the orders of the last selection (still in
+    // the learned slot); the fixed orders when not learning or after the
+    // judge rejected the learned ones
     const bool reuse = lrn && h->sel_cached && h->sel_cached_maxl == h->ctx.info.max_level;
     if (lrn && !reuse) select_on_tree(h, tp.max_par);
     h->sel_fresh = lrn && !reuse;
-    if (reuse && !h->judge_reject) {
-        std::memcpy(h->h_orders, h->sel_cache.sel_p, sizeof(h->h_orders));
-        VPM_HIP(hipMemcpyAsync(h->ctx.hdr_d->sel_p, h->h_orders, sizeof(h->h_orders),
-                               hipMemcpyHostToDevice, h->ctx.stream));
-    }
-    if (!lrn || h->judge_reject) {
-        // orders per level for the evaluation statistics of the list build
-        for (int L = 0; L < kMaxDepth + 2; ++L) {
-            int p = h->o.p;
-            if (h->o.precision == VPMFMM_MIXED && !lrn && L < (int)h->level_p.size() &&
-                h->level_p[L] > 0)
-                p = h->level_p[L];
-            h->h_orders[L] = (uint8_t)std::min(p, 255);
-        }
-        VPM_HIP(hipMemcpyAsync(h->ctx.hdr_d->sel_p, h->h_orders, sizeof(h->h_orders),
-                               hipMemcpyHostToDevice, h->ctx.stream));
-    }
+    h->ctx.orders = (lrn && !h->judge_reject) ? learned_slot(h) : fixed_slot(h);
     mark(h, 1);
     tree_lists(h->ctx);
     mark(h, 2);
     h->last_sel.threshold = tp.max_par;
-    DevHdr &hh = *h->ctx.hdr_h;  // synchronised by the list build
-    if (lrn && !reuse) {
+}
+
+void read_hdr(Ctx &c) {
+    VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
+    VPM_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Final synchronisation of a call: list totals and checks, the selection
+// cache of a fresh selection, statistics and stage times.
+void finish_stats(vpmfmm_handle *h, bool fmm) {
+    Ctx &c = h->ctx;
+    read_hdr(c);
+    tree_lists_absorb(c);
+    const DevHdr &hh = *c.hdr_h;
+    if (h->sel_fresh) {
         std::memcpy(h->sel_cache.sel_p, hh.sel_p, sizeof(hh.sel_p));
         h->sel_cache.sel_source = hh.sel_source;
         h->sel_cache.sel_pred_err = hh.sel_pred_err;
         h->sel_cache.sel_pred_ms = hh.sel_pred_ms;
         h->sel_cached = true;
-        h->sel_cached_maxl = h->ctx.info.max_level;
-    } else if (reuse) {  // report the orders in use
-        std::memcpy(hh.sel_p, h->sel_cache.sel_p, sizeof(hh.sel_p));
-        hh.sel_source = h->sel_cache.sel_source;
-        hh.sel_pred_err = h->sel_cache.sel_pred_err;
-        hh.sel_pred_ms = h->sel_cache.sel_pred_ms;
+        h->sel_cached_maxl = c.info.max_level;
     }
-}
-
-void finish_stats(vpmfmm_handle *h, bool fmm) {
-    Ctx &c = h->ctx;
-    VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
-    VPM_HIP(hipStreamSynchronize(c.stream));
     const TreeInfo &ti = c.info;
     vpmfmm_stats &s = h->st;
     s = vpmfmm_stats{};
@@ -359,26 +371,19 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
         throw Error(kDuplicate, "two particles of a list-1 pair share a position (r = 0)");
 }
 
-// Mixed path set-up: per-level orders and sides, level-independent tables
-// (rebuilt only when the largest order changes) and the 16-cell M2L tiles.
+// Mixed path set-up: level-independent tables (rebuilt only when the largest
+// order grows), then the per-level table and the 16-cell M2L tiles built on
+// the device from the device-resident orders (no host round trip).
 Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     Ctx &c = h->ctx;
     const TreeInfo &ti = c.info;
-    LevelTab lt{};
-    int pmax = 0;
     const bool lrn = learned(h) && !h->judge_reject && !force_fixed;
-    for (int L = 0; L < kMaxDepth + 2; ++L) {
-        int p = h->o.p;
-        if (lrn) p = c.hdr_h->sel_p[L];
-        else if (!learned(h) && L < (int)h->level_p.size() && h->level_p[L] > 0) p = h->level_p[L];
-        if (p > 16) throw Error(kBadArg, "the mixed path supports expansion orders up to 16");
-        lt.p[L] = p;
-        const double s = ti.root_len / std::ldexp(1.0, L);
-        lt.s[L] = (float)s;
-        lt.inv_s[L] = (float)(1.0 / s);
-        if (L <= ti.max_level) pmax = std::max(pmax, p);
-    }
-    Mixed32 mx;
+    const uint8_t *fix = fixed_slot(h);
+    const uint8_t *ord = lrn ? learned_slot(h) : fix;
+    // the fixed orders also serve the learned path (fallback and judge)
+    int pmax = learned(h) ? h->sel.p_max : 0;
+    for (int L = 0; L <= ti.max_level; ++L) pmax = std::max(pmax, (int)h->fix_orders[L]);
+    if (pmax > 16) throw Error(kBadArg, "the mixed path supports expansion orders up to 16");
     if (h->mx_pmax < pmax) {
         std::vector<float> m2m, l2l, tm2l;
         std::vector<std::vector<float>> rl;
@@ -406,42 +411,12 @@ Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
         VPM_HIP(hipStreamSynchronize(c.stream));
         h->mx_pmax = pmax;
     }
-    // tiles of 16 consecutive cells per level (levels >= 2), grouped by order
-    std::vector<int2> tl;
-    std::vector<uint8_t> tp;
-    for (int p = 1; p <= 16; ++p) {
-        for (int L = 2; L <= ti.max_level; ++L) {
-            if (lt.p[L] != p) continue;
-            const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
-            for (int j = j0; j < j1; j += 16) {
-                tl.push_back(make_int2(j, std::min(16, j1 - j)));
-                tp.push_back((uint8_t)p);
-            }
-        }
-    }
-    int2 *td = c.mx_tiles.as<int2>(tl.size() + 1);
-    uint8_t *tpd = c.mx_tile_p.as<uint8_t>(tp.size() + 1);
-    const size_t need = sizeof(LevelTab) + tl.size() * sizeof(int2) + tp.size() + 16;
-    if (h->pin_cap < need) {
-        if (h->pin) VPM_HIP(hipHostFree(h->pin));
-        h->pin = nullptr;
-        h->pin_cap = 0;
-        VPM_HIP(hipHostMalloc(&h->pin, need + need / 2, hipHostMallocDefault));
-        h->pin_cap = need + need / 2;
-    }
-    LevelTab *lt_pin = (LevelTab *)h->pin;
-    int2 *tl_pin = (int2 *)((char *)h->pin + sizeof(LevelTab));
-    uint8_t *tp_pin = (uint8_t *)(tl_pin + tl.size());
-    *lt_pin = lt;
-    if (!tl.empty()) {
-        std::memcpy(tl_pin, tl.data(), tl.size() * sizeof(int2));
-        std::memcpy(tp_pin, tp.data(), tp.size());
-        VPM_HIP(hipMemcpyAsync(td, tl_pin, tl.size() * sizeof(int2), hipMemcpyHostToDevice,
-                               c.stream));
-        VPM_HIP(hipMemcpyAsync(tpd, tp_pin, tp.size(), hipMemcpyHostToDevice, c.stream));
-    }
+    const int ntiles = mixed_tile_count(c);
+    int2 *td = c.mx_tiles.as<int2>((size_t)ntiles + 1);
+    uint8_t *tpd = c.mx_tile_p.as<uint8_t>((size_t)ntiles + 1);
     LevelTab *ltd = c.mx_lt.as<LevelTab>(1);
-    VPM_HIP(hipMemcpyAsync(ltd, lt_pin, sizeof(LevelTab), hipMemcpyHostToDevice, c.stream));
+    mixed_level_setup(c, ord, ltd, td, tpd);
+    Mixed32 mx;
     mx.pmax = h->mx_pmax;  // tables (and coefficient stride) of the largest order built so far
     mx.lt = ltd;
     mx.m2m = (const float *)c.mx_m2m.p;
@@ -450,36 +425,44 @@ Mixed32 mixed_setup(vpmfmm_handle *h, bool force_fixed = false) {
     for (int p = 0; p <= 16; ++p) mx.rptr_host[p] = h->mx_rptr_host[p];
     mx.tiles = td;
     mx.tile_p = tpd;
-    mx.ntiles = (int)tl.size();
+    mx.ntiles = ntiles;
     return mx;
 }
 
-void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+// Selection report of the last call (after its final synchronisation).
+void report_selection(vpmfmm_handle *h, bool lrn_run) {
+    vpmfmm_selection &ls = h->last_sel;
+    const DevHdr &sel = h->sel_fresh ? *h->ctx.hdr_h : h->sel_cache;
+    if (h->o.precision != VPMFMM_MIXED) {
+        ls.pred_err = ls.pred_ms = -1.f;
+        for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = h->o.p;
+        return;
+    }
+    if (learned(h)) {
+        ls.source = (!lrn_run || h->judged_reject_now) ? 3 : (int)sel.sel_source;
+        ls.pred_err = lrn_run ? sel.sel_pred_err : -1.f;
+        ls.pred_ms = lrn_run ? sel.sel_pred_ms : -1.f;
+    } else {
+        ls.pred_err = ls.pred_ms = -1.f;
+    }
+    for (int L = 0; L <= kMaxDepth; ++L)
+        ls.p[L] = (lrn_run && !h->judged_reject_now) ? sel.sel_p[L] : h->fix_orders[L];
+}
+
+void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Sources &q,
          const uint8_t *a, double *oa, double *ob, bool velocity) {
-    if (!q || !oa || !ob) throw Error(kBadArg, "null pointer");
+    if (!(q.q || (q.s && q.omega)) || !oa || !ob) throw Error(kBadArg, "null pointer");
     build(h, n, x, y, q, a);
     Ctx &c = h->ctx;
     FmmMarks mk{h->o.timing ? h->ev : nullptr};
     const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
     vpmfmm_selection &ls = h->last_sel;
     ls.judge_err = -1.f;
+    h->judged_reject_now = false;
+    bool lrn = false;
     if (h->o.precision == VPMFMM_MIXED) {
-        const bool lrn = learned(h) && !h->judge_reject;
+        lrn = learned(h) && !h->judge_reject;
         Mixed32 mx = mixed_setup(h);
-        if (learned(h)) {
-            ls.source = h->judge_reject ? 3 : (int)c.hdr_h->sel_source;
-            ls.pred_err = lrn ? c.hdr_h->sel_pred_err : -1.f;
-            ls.pred_ms = lrn ? c.hdr_h->sel_pred_ms : -1.f;
-        } else {
-            ls.pred_err = ls.pred_ms = -1.f;
-        }
-        for (int L = 0; L <= kMaxDepth; ++L) {
-            int p = h->o.p;
-            if (lrn) p = c.hdr_h->sel_p[L];
-            else if (!learned(h) && L < (int)h->level_p.size() && h->level_p[L] > 0)
-                p = h->level_p[L];
-            ls.p[L] = p;
-        }
         fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
         // the fixed-order judge (same tree): every fresh selection, and every
         // judge_every-th learned call
@@ -487,36 +470,38 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const double
             h->o.judge_every >= 0 &&
             (h->sel_fresh || (h->o.judge_every > 0 && (h->judge_calls % h->o.judge_every) == 0));
         ++h->judge_calls;
-        if (lrn && judge_due && c.hdr_h->sel_source == 1) {
-            const int nn = c.info.n;
-            // buffers may grow below: let the learned solve finish first
-            VPM_HIP(hipStreamSynchronize(c.stream));
-            double *ja = h->judge_a.as<double>(nn), *jb = h->judge_b.as<double>(nn);
-            Mixed32 mf = mixed_setup(h, true);
-            fmm32_run(c, mf, h->o.list4_mode, mask, ja, jb, velocity, h->o.u_inf, h->o.v_inf,
-                      FmmMarks{nullptr});
-            const double a0 = velocity ? h->o.u_inf : 0.0, b0 = velocity ? h->o.v_inf : 0.0;
-            ls.judge_err = (float)rel_diff64(nn, oa, ob, ja, jb, a0, b0,
-                                             h->judge_part.as<double>(1024), c.stream);
-            if (!(ls.judge_err <= h->sel.target_err)) {
-                h->judge_reject = true;
-                VPM_HIP(hipMemcpyAsync(oa, ja, sizeof(double) * nn, hipMemcpyDeviceToDevice,
-                                       c.stream));
-                VPM_HIP(hipMemcpyAsync(ob, jb, sizeof(double) * nn, hipMemcpyDeviceToDevice,
-                                       c.stream));
-                ls.source = 3;
-                for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = h->o.p;
+        if (lrn && judge_due) {
+            // the selection's source: read back for a fresh selection (this also
+            // lets the learned solve finish before buffers may grow below)
+            read_hdr(c);
+            const uint32_t src = h->sel_fresh ? c.hdr_h->sel_source : h->sel_cache.sel_source;
+            if (src == 1) {
+                const int nn = c.info.n;
+                double *ja = h->judge_a.as<double>(nn), *jb = h->judge_b.as<double>(nn);
+                Mixed32 mf = mixed_setup(h, true);
+                fmm32_run(c, mf, h->o.list4_mode, mask, ja, jb, velocity, h->o.u_inf,
+                          h->o.v_inf, FmmMarks{nullptr});
+                const double a0 = velocity ? h->o.u_inf : 0.0, b0 = velocity ? h->o.v_inf : 0.0;
+                ls.judge_err = (float)rel_diff64(nn, oa, ob, ja, jb, a0, b0,
+                                                 h->judge_part.as<double>(1024), c.stream);
+                if (!(ls.judge_err <= h->sel.target_err)) {
+                    h->judge_reject = true;
+                    h->judged_reject_now = true;
+                    VPM_HIP(hipMemcpyAsync(oa, ja, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                           c.stream));
+                    VPM_HIP(hipMemcpyAsync(ob, jb, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                           c.stream));
+                }
             }
         }
     } else {
         ensure_tables(h);
-        ls.pred_err = ls.pred_ms = -1.f;
-        for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = h->o.p;
         fmm64_run(c, h->o.p, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf,
                   mk);
         h->exp_p = h->o.p;
     }
     finish_stats(h, true);
+    report_selection(h, lrn);
 }
 
 // Host-pointer staging.
@@ -660,7 +645,6 @@ int vpmfmm_destroy(vpmfmm_handle *h) {
         if (e) (void)hipEventDestroy(e);
     if (h->ctx.hdr_d) (void)hipFree(h->ctx.hdr_d);
     if (h->ctx.hdr_h) (void)hipHostFree(h->ctx.hdr_h);
-    if (h->pin) (void)hipHostFree(h->pin);
     h->ctx.hdr_d = nullptr;
     h->ctx.hdr_h = nullptr;
     hipStream_t own = h->own_stream;
@@ -698,7 +682,7 @@ int vpmfmm_set_stream(vpmfmm_handle *h, void *stream) {
 
 int vpmfmm_field_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
                         const double *q, const uint8_t *active, double *ex, double *ey) {
-    return guarded(h, [&] { run(h, n, x, y, q, active, ex, ey, false); });
+    return guarded(h, [&] { run(h, n, x, y, Sources{q}, active, ex, ey, false); });
 }
 
 int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y, const double *q,
@@ -708,7 +692,7 @@ int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y, 
         Staged s = stage_in(h, n, x, y, q, active);
         Ctx &c = h->ctx;
         double *oa = c.ex_out.as<double>(n), *ob = c.ey_out.as<double>(n);
-        run(h, n, s.x, s.y, s.q, s.a, oa, ob, false);
+        run(h, n, s.x, s.y, Sources{s.q}, s.a, oa, ob, false);
         stage_out(h, n, ex, ey);
     });
 }
@@ -719,9 +703,7 @@ int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const d
     return guarded(h, [&] {
         if (n < 1 || !s || !omega) throw Error(kBadArg, "bad size or null pointer");
         VPM_HIP(hipSetDevice(h->o.device));
-        double *q = h->ctx.q_in.as<double>(n);
-        sources64(n, s, omega, q, h->ctx.stream);
-        run(h, n, x, y, q, active, u, v, true);
+        run(h, n, x, y, Sources{nullptr, s, omega}, active, u, v, true);
     });
 }
 
@@ -734,15 +716,14 @@ int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *
         Ctx &c = h->ctx;
         double *dx = c.x_in.as<double>(n), *dy = c.y_in.as<double>(n);
         double *ds = c.ex_out.as<double>(n), *dw = c.ey_out.as<double>(n);
-        double *dq = c.q_in.as<double>(n);
         uint8_t *da = c.a_in.as<uint8_t>(n);
         VPM_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
         VPM_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
         VPM_HIP(hipMemcpyAsync(ds, s, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
         VPM_HIP(hipMemcpyAsync(dw, omega, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
         VPM_HIP(hipMemcpyAsync(da, active, n, hipMemcpyHostToDevice, c.stream));
-        sources64(n, ds, dw, dq, c.stream);
-        run(h, n, dx, dy, dq, da, ds, dw, true);  // outputs reuse the s/omega staging
+        // the outputs reuse the s / omega staging (read by the tree build first)
+        run(h, n, dx, dy, Sources{nullptr, ds, dw}, da, ds, dw, true);
         VPM_HIP(hipMemcpyAsync(u, ds, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipMemcpyAsync(v, dw, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipStreamSynchronize(c.stream));
@@ -775,7 +756,7 @@ int vpmfmm_build_tree(vpmfmm_handle *h, int32_t n, const double *x, const double
     return guarded(h, [&] {
         std::vector<double> zeros((size_t)std::max(n, 1), 0.0);
         Staged s = stage_in(h, n, x, y, zeros.data(), active);
-        build(h, n, s.x, s.y, s.q, s.a);
+        build(h, n, s.x, s.y, Sources{s.q}, s.a);
         finish_stats(h, false);
     });
 }

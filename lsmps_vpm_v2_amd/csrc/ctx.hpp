@@ -40,8 +40,13 @@ struct DevHdr {
     float sel_pred_err, sel_pred_ms;
     unsigned long long m2p_terms;  // sum over (particle, list-3 cell) of (order of the cell + 1)
     unsigned long long l2p_terms;  // sum over particles of the order of their leaf
-    uint32_t pad[2];
+    uint32_t lev_cnt[kMaxDepth + 2];  // cells per level, counted by the leaf split
+    uint32_t n_leaves_split;          // leaves, counted by the leaf split
+    uint32_t pad[3];
 };
+
+// Columns of the leaf split's per-block partials: cells per level 0..16, leaves.
+constexpr int kLvCols = kMaxDepth + 2;
 
 // Grow-only device buffer.
 struct DBuf {
@@ -73,6 +78,7 @@ struct TreeInfo {
     int min_leaf_level = 0;  // shallowest leaf level
     uint32_t lev_off[kMaxDepth + 2] = {};
     int tot1 = 0, tot3 = 0;
+    size_t cap1 = 0, cap3 = 0;  // capacities of the list-1 / list-3 arrays
     double root_len = 0, pivot[2] = {0, 0};
 };
 
@@ -89,14 +95,14 @@ struct Ctx {
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
     DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
     DBuf tmp;  // scan / sort scratch
-    DBuf bbox_part;
+    DBuf bbox_part, lvpart;
 
     // cells
     DBuf pre_level, pre_slot, lev_level, lev2pre;
     DBuf c_level, c_flags, c_key, c_begin, c_end, c_src, c_cx, c_cy, c_parent, c_child, c_nbr;
     DBuf leaf_flag_scan, leaf_idx;
     DBuf l2, n2, l4, n4;
-    DBuf cnt1, cnt3, off1, off3, l1, l3;
+    DBuf cnt1, cnt3, cntk, off1, off3, l1, l3;
 
     // expansions
     DBuf ak, bk, tab_m2l, tab_m2m, tab_l2l;
@@ -105,6 +111,12 @@ struct Ctx {
     DBuf oxy, qf;
     DBuf sel_part, sel_feats, sel_raw;  // level statistics of the selection
     DBuf part;  // per-block partial results of the tree and list kernels
+    // expansion order per level (levels 0..17), device resident: slot 0 holds
+    // the learned selection (written by the selection kernel), slot 1 the
+    // fixed orders (uploaded when they change); `orders` points at the slot
+    // the list build and the mixed path use
+    DBuf orders_buf;
+    const uint8_t *orders = nullptr;
     DBuf mx_lt, mx_m2m, mx_l2l, mx_tm2l, mx_rdata, mx_tiles, mx_tile_p;
 
     TreeInfo info;
@@ -117,10 +129,23 @@ void fill_tables(int p, int binom_mode, std::vector<double> &m2l, std::vector<do
 int ref_binom_int32(int n, int k);
 double exact_binom(int n, int k);
 
+// Source strengths of a build: q per particle, or (q == nullptr) the particle
+// size s and vorticity omega, from which the build forms q = -omega s^2 / (2 pi).
+struct Sources {
+    const double *q = nullptr;
+    const double *s = nullptr;
+    const double *omega = nullptr;
+};
+
 // Stage entry points (tree.hip).  Inputs are device pointers.
-void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q,
+void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &src,
                 const uint8_t *active, const TreeParams &tp);
+// Lists of the tree in c, with no host synchronisation: sizes come from the
+// counts of tree_build and the list arrays are sized by proven bounds.
 void tree_lists(Ctx &c);
+// Host copy of the list totals and the checks that need the list build's
+// header (call after the header has been read back).
+void tree_lists_absorb(Ctx &c);
 
 // Optional stage markers: events 3..6 are recorded after the upward pass,
 // M2L, the downward pass and the evaluation.

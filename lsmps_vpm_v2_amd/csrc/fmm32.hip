@@ -1,7 +1,7 @@
 // fmm32.hip -- mixed-precision FMM stages on gfx950.
 //
 // Precision split: cell geometry, centre differences and particle-minus-centre
-// displacements are formed in FP64 (tree.hip / k_prep32) and rounded once;
+// displacements are formed in FP64 (tree.hip / k_p2m32) and rounded once;
 // everything else is FP32 with FP32 accumulation.
 //
 // Scaled coefficients.  With s_L the side of a level-L cell,
@@ -68,69 +68,76 @@ __device__ __forceinline__ uint32_t compact1by1(uint32_t v) {
     return v;
 }
 
-// ---------------------------------------------------------------------------
-// particle preparation: FP32 offsets from the leaf centre (FP64 difference)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-    k_prep32(const uint32_t *leaf_idx, int nl, const uint32_t *c_begin, const uint32_t *c_end,
-             const double *c_cx, const double *c_cy, const double *xs, const double *ys,
-             const double *qs, float2 *oxy, float *qf) {
-    const int t = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (t >= nl) return;
-    const int j = (int)leaf_idx[t];
-    const double cx = c_cx[j], cy = c_cy[j];
-    for (uint32_t i = c_begin[j] + lane; i < c_end[j]; i += kWave) {
-        oxy[i] = make_float2((float)(xs[i] - cx), (float)(ys[i] - cy));
-        qf[i] = (float)qs[i];
-    }
+// Lanes of one wavefront read what other lanes of it wrote to LDS.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-// P2M: one wavefront per leaf, lanes over its particles; lane-private running
+// P2M: one wavefront per leaf, lanes over its particles.  Each lane also
+// writes its particles' FP32 offsets from the leaf centre (FP64 difference,
+// rounded once) and FP32 strengths for the later stages; lane-private running
 // powers accumulate a^_k = sum q ((z - zc)/s)^k (k <= p, active particles of
-// active leaves; inactive leaves get zero), reduced across lanes per order.
+// active leaves; inactive leaves get zero).  The lanes' sums are reduced
+// through LDS: lane l writes column l of a (order x lane) table, lane k sums
+// row k (stride 65 keeps the rows on distinct banks).
 constexpr int kP2mWaves = 4;
 
 __global__ void __launch_bounds__(kWave *kP2mWaves)
     k_p2m32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
-            const uint32_t *c_begin, const uint32_t *c_end, const float2 *oxy, const float *qf,
-            const uint8_t *as, const LevelTab *lt, int kst, float *ahat) {
+            const uint32_t *c_begin, const uint32_t *c_end, const double *c_cx,
+            const double *c_cy, const double *xs, const double *ys, const double *qs,
+            const uint8_t *as, const LevelTab *lt, int kst, float2 *oxy, float *qf,
+            float *ahat) {
+    __shared__ float2 red[kP2mWaves][17][kWave + 1];
     const int PM1 = kst / 2;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int t = blockIdx.x * kP2mWaves + threadIdx.x / kWave;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kP2mWaves + w;
     if (t >= nl) return;
     const int j = (int)leaf_idx[t];
     const int L = c_level[j];
     const int p = lt->p[L];
     const float inv_s = lt->inv_s[L];
+    const double cx = c_cx[j], cy = c_cy[j];
+    const bool act = (c_flags[j] & kActive) != 0;  // inactive leaves: zero
     float ar[17], ai[17];
 #pragma unroll
     for (int k = 0; k < 17; ++k) ar[k] = ai[k] = 0.f;
-    const uint32_t e = (c_flags[j] & kActive) ? c_end[j] : c_begin[j];  // inactive: zero
+    const uint32_t e = c_end[j];
     for (uint32_t i = c_begin[j] + lane; i < e; i += kWave) {
-        if (!as[i]) continue;
-        const float2 o = oxy[i];
+        const float2 o = make_float2((float)(xs[i] - cx), (float)(ys[i] - cy));
+        const float qi = (float)qs[i];
+        oxy[i] = o;
+        qf[i] = qi;
+        if (!act || !as[i]) continue;
         const cf z = {o.x * inv_s, o.y * inv_s};
-        cf w = {qf[i], 0.f};
+        cf v = {qi, 0.f};
 #pragma unroll
         for (int k = 0; k < 17; ++k) {
             if (k <= p) {
-                ar[k] += w.x;
-                ai[k] += w.y;
-                w = cmul(w, z);
+                ar[k] += v.x;
+                ai[k] += v.y;
+                v = cmul(v, z);
             }
         }
     }
 #pragma unroll
-    for (int k = 0; k < 17; ++k) {
-        if (k <= p) {
-            float xr = ar[k], xi = ai[k];
-            for (int m = 1; m < kWave; m <<= 1) {
-                xr += __shfl_xor(xr, m, kWave);
-                xi += __shfl_xor(xi, m, kWave);
-            }
-            if (lane == k) stc(ahat, (size_t)j * PM1 + k, cf{xr, xi});
+    for (int k = 0; k < 17; ++k)
+        if (k <= p) red[w][k][lane] = make_float2(ar[k], ai[k]);
+    wave_sync();
+    if (lane <= p) {
+        float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+#pragma unroll 4
+        for (int l = 0; l < kWave; l += 4) {
+            const float2 a0 = red[w][lane][l], a1 = red[w][lane][l + 1];
+            const float2 a2 = red[w][lane][l + 2], a3 = red[w][lane][l + 3];
+            s0.x += a0.x; s0.y += a0.y;
+            s1.x += a1.x; s1.y += a1.y;
+            s2.x += a2.x; s2.y += a2.y;
+            s3.x += a3.x; s3.y += a3.y;
         }
+        stc(ahat, (size_t)j * PM1 + lane,
+            cf{(s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y)});
     }
 }
 
@@ -140,11 +147,6 @@ __global__ void __launch_bounds__(kWave *kP2mWaves)
 // table m2m[q][m][k] / l2l[q][m][k] (complex, stride PM1 x PM1, also in LDS).
 // q is the child quadrant, c_child's order: x bit + 2 * y bit.
 constexpr int kMaxPM1 = 17;
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // M2M into parent j (inactive parents get zero: their own parents read them).
 __device__ __forceinline__ void m2m_wave(int j, int lane, int PM1, const uint8_t *c_level,
@@ -165,16 +167,21 @@ __device__ __forceinline__ void m2m_wave(int j, int lane, int PM1, const uint8_t
     }
     wave_sync();
     if (lane <= p) {
+        // one accumulator per child and a fully unrolled order loop: the LDS
+        // loads issue ahead of four independent multiply-add chains
         const int m = lane, kmax = min(m, pc);
-        cf acc = {0.f, 0.f};
-        for (int q = 0; q < 4; ++q) {
-            const float2 *row = tab + (q * PM1 + m) * PM1;
-            for (int k = 0; k <= kmax; ++k) {
-                const float2 tv = row[k], av = sa[q * PM1 + k];
-                acc = cadd(acc, cmul(cf{tv.x, tv.y}, cf{av.x, av.y}));
+        cf acc[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+#pragma unroll
+        for (int k = 0; k < kMaxPM1; ++k) {
+            if (k <= kmax) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float2 tv = tab[(q * PM1 + m) * PM1 + k], av = sa[q * PM1 + k];
+                    acc[q] = cadd(acc[q], cmul(cf{tv.x, tv.y}, cf{av.x, av.y}));
+                }
             }
         }
-        stc(ahat, (size_t)j * PM1 + m, acc);
+        stc(ahat, (size_t)j * PM1 + m, cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3])));
     }
     wave_sync();
 }
@@ -195,14 +202,18 @@ __device__ __forceinline__ void l2l_wave(int j, int lane, int PM1, const uint8_t
     const int q = lane >> 4, m = (lane & 15) + 1;
     const int ch = c_child[4 * j + q];
     if (ch >= 0 && m <= pc) {
+        // fully unrolled over k with two accumulators (even / odd k)
         const float2 *row = tab + (q * PM1 + m) * PM1;
-        cf acc = {0.f, 0.f};
-        for (int k = m; k <= pp; ++k) {
-            const float2 tv = row[k], bv = sb[k];
-            acc = cadd(acc, cmul(cf{tv.x, tv.y}, cf{bv.x, bv.y}));
+        cf acc[2] = {{0.f, 0.f}, {0.f, 0.f}};
+#pragma unroll
+        for (int k = 1; k < kMaxPM1; ++k) {
+            if (k >= m && k <= pp) {
+                const float2 tv = row[k], bv = sb[k];
+                acc[k & 1] = cadd(acc[k & 1], cmul(cf{tv.x, tv.y}, cf{bv.x, bv.y}));
+            }
         }
         const size_t o = (size_t)ch * PM1 + m;
-        stc(bhat, o, cadd(ldc(bhat, o), acc));
+        stc(bhat, o, cadd(ldc(bhat, o), cadd(acc[0], acc[1])));
     }
     wave_sync();
 }
@@ -436,6 +447,46 @@ __global__ void __launch_bounds__(256)
 #undef M2L_CASE
 }
 
+// Per-level table (order clamped to 1..16, side R / 2^L, its inverse) and the
+// 16-cell M2L tiles of levels 2..maxL, grouped by order (ascending) and then
+// by level, from the device-resident orders: one workgroup.
+__global__ void __launch_bounds__(256)
+    k_mixed_setup(const uint8_t *orders, const DevHdr *hdr, int maxL, LevelTab *lt, int2 *tiles,
+                  uint8_t *tile_p) {
+    __shared__ int s_first[kMaxDepth + 2], s_p[kMaxDepth + 2], s_off[kMaxDepth + 2],
+        s_cnt[kMaxDepth + 2];
+    const int tid = threadIdx.x;
+    if (tid < kMaxDepth + 2) {
+        const int p = min(max((int)orders[tid], 1), 16);
+        s_p[tid] = p;
+        s_off[tid] = (int)hdr->lev_off[tid];
+        s_cnt[tid] = (int)hdr->lev_cnt[tid];
+        const double side = hdr->root_len / ldexp(1.0, tid);
+        lt->p[tid] = p;
+        lt->s[tid] = (float)side;
+        lt->inv_s[tid] = (float)(1.0 / side);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int off = 0;
+        for (int p = 1; p <= 16; ++p)
+            for (int L = 2; L <= maxL; ++L) {
+                if (s_p[L] != p) continue;
+                s_first[L] = off;
+                off += (s_cnt[L] + 15) / 16;
+            }
+    }
+    __syncthreads();
+    for (int L = 2; L <= maxL; ++L) {
+        const int j0 = s_off[L], cnt = s_cnt[L];
+        const int nt = (cnt + 15) / 16;
+        for (int u = tid; u < nt; u += blockDim.x) {
+            tiles[s_first[L] + u] = make_int2(j0 + 16 * u, min(16, cnt - 16 * u));
+            tile_p[s_first[L] + u] = (uint8_t)s_p[L];
+        }
+    }
+}
+
 // List 4, mode 2: P2L from the active particles of coarser leaves,
 //   b^_m += -q/m (s_t / (z_j - z_t))^m.
 // One wavefront per target cell; lanes stride over the source particles and
@@ -536,8 +587,28 @@ __global__ void __launch_bounds__(256)
 // once.  Lanes are
 // split into G groups of 64/G: a group owns the targets, groups split the
 // sources and list-3 cells and are reduced with cross-lane shuffles.
+// Two particles at one position always share a leaf, so only the sources of
+// the target's own leaf need the self test (by index); the other list-1 cells
+// run a loop without it.
 constexpr int kEvalWaves = 4;
 constexpr int kStage = 256;  // sources staged per wavefront per round
+
+// P2P over staged sources r = r0, r0 + G, ... < r1 (r0 = g mod G shifted to
+// the first r >= lo); SELF adds the self test.
+template <bool SELF>
+__device__ __forceinline__ void p2p_run(const float4 *stage, int lo, int hi, int g, int G,
+                                        float2 ti, uint32_t i, float &ex, float &ey) {
+    for (int r = lo + ((g - lo) & (G - 1)); r < hi; r += G) {
+        const float4 sv = stage[r];
+        const float dx = ti.x - sv.x, dy = ti.y - sv.y;
+        const float r2 = dx * dx + dy * dy;
+        float inv = __builtin_amdgcn_rcpf(r2);
+        if (SELF) inv = (__float_as_uint(sv.w) == i) ? 0.f : inv;
+        const float qr = sv.z * inv;
+        ex += qr * dx;
+        ey += qr * dy;
+    }
+}
 
 __global__ void __launch_bounds__(kWave *kEvalWaves)
     k_eval32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint32_t *c_begin,
@@ -580,8 +651,9 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             const int ncell = (int)min<uint32_t>(kWave, z1 - c0);
             uint32_t cnt = 0, beg = 0;
             float dcx = 0.f, dcy = 0.f;
+            int sc = -1;
             if (lane < ncell) {
-                const int sc = l1[c0 + lane];
+                sc = l1[c0 + lane];
                 beg = c_begin[sc];
                 cnt = c_end[sc] - beg;
                 dcx = (float)(c_cx[sc] - tcx);
@@ -593,6 +665,14 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 if (lane >= o) inc += y;
             }
             const uint32_t total = __shfl(inc, kWave - 1, kWave);
+            // the target leaf's own sources in this chunk: [own_lo, own_hi)
+            const unsigned long long own = __ballot(sc == j);
+            uint32_t own_lo = 0, own_hi = 0;
+            if (own) {
+                const int ol = __ffsll(own) - 1;
+                own_hi = __shfl(inc, ol, kWave);
+                own_lo = own_hi - __shfl(cnt, ol, kWave);
+            }
             s_off[w][lane] = inc - cnt;
             s_beg[w][lane] = beg;
             s_dx[w][lane] = dcx;
@@ -618,16 +698,12 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 float ex = 0.f, ey = 0.f;
-                for (int r = g; r < fill; r += G) {
-                    const float4 sv = stage[w][r];
-                    const float dx = ti.x - sv.x, dy = ti.y - sv.y;
-                    const float r2 = dx * dx + dy * dy;
-                    const float inv =
-                        (__float_as_uint(sv.w) == i) ? 0.f : __builtin_amdgcn_rcpf(r2);
-                    const float qr = sv.z * inv;
-                    ex += qr * dx;
-                    ey += qr * dy;
-                }
+                // the own leaf's part of this round, [sa, sb) (empty: sa = sb)
+                const int sa = (int)min(max(own_lo, base), base + (uint32_t)fill) - (int)base;
+                const int sb = (int)min(max(own_hi, base), base + (uint32_t)fill) - (int)base;
+                p2p_run<false>(stage[w], 0, sa, g, G, ti, i, ex, ey);
+                p2p_run<true>(stage[w], sa, sb, g, G, ti, i, ex, ey);
+                p2p_run<false>(stage[w], sb, fill, g, G, ti, i, ex, ey);
                 exd += (double)ex;
                 eyd += (double)ey;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -776,6 +852,23 @@ void mixed_tables(int pmax, std::vector<float> &m2m, std::vector<float> &l2l,
     }
 }
 
+int mixed_tile_count(const Ctx &c) {
+    const TreeInfo &ti = c.info;
+    int ntiles = 0;
+    for (int L = 2; L <= ti.max_level; ++L)
+        ntiles += div_up((long long)(ti.lev_off[L + 1] - ti.lev_off[L]), 16);
+    return ntiles;
+}
+
+int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
+                      uint8_t *tile_p) {
+    const TreeInfo &ti = c.info;
+    const int ntiles = mixed_tile_count(c);
+    k_mixed_setup<<<1, 256, 0, c.stream>>>(orders, c.hdr_d, ti.max_level, lt, tiles, tile_p);
+    VPM_LAUNCH_CHECK();
+    return ntiles;
+}
+
 void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
     hipStream_t s = c.stream;
@@ -796,11 +889,9 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
     const uint8_t *as = (const uint8_t *)c.acts.p;
 
-    k_prep32<<<div_up(nl, 4), 256, 0, s>>>(leaf_idx, nl, c_begin, c_end, c_cx, c_cy,
-                                           (const double *)c.xs.p, (const double *)c.ys.p,
-                                           (const double *)c.qs.p, oxy, qf);
     k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
-        leaf_idx, nl, c_level, c_flags, c_begin, c_end, oxy, qf, as, mx.lt, kst, ahat);
+        leaf_idx, nl, c_level, c_flags, c_begin, c_end, c_cx, c_cy, (const double *)c.xs.p,
+        (const double *)c.ys.p, (const double *)c.qs.p, as, mx.lt, kst, oxy, qf, ahat);
     // split levels: K = first level with >= kSubRoots cells (subtrees below),
     // T = first level with >= kTopCells cells (one workgroup above T)
     const int maxL = ti.max_level;

@@ -29,6 +29,13 @@ struct Mixed32 {
 void mixed_tables(int pmax, std::vector<float> &m2m, std::vector<float> &l2l,
                   std::vector<float> &tm2l, std::vector<std::vector<float>> &rlane);
 
+// Level table and M2L tiles on the device from the device-resident orders
+// (levels 0..17) and the built tree; returns the number of tiles, which
+// mixed_tile_count gives beforehand for sizing tiles and tile_p.
+int mixed_tile_count(const Ctx &c);
+int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
+                      uint8_t *tile_p);
+
 void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 

@@ -369,13 +369,6 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     VPM_LAUNCH_CHECK();
 }
 
-// q = -omega s^2 / (2 pi) (velocity_calc.cpp:477).
-__global__ void __launch_bounds__(256)
-    k_sources64(int n, const double *s, const double *omega, double *q) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) q[i] = -omega[i] * (s[i] * s[i]) / (2 * M_PI);
-}
-
 // Partial sums of |(a, b) - (ja, jb)|^2 and |(ja - a0, jb - b0)|^2 per block.
 __global__ void __launch_bounds__(256)
     k_diff_norms(int n, const double *a, const double *b, const double *ja, const double *jb,
@@ -422,11 +415,6 @@ double rel_diff64(int n, const double *a, const double *b, const double *ja, con
         den += h[2 * k + 1];
     }
     return den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
-}
-
-void sources64(int n, const double *s, const double *omega, double *q, hipStream_t st) {
-    k_sources64<<<div_up(n, 256), 256, 0, st>>>(n, s, omega, q);
-    VPM_LAUNCH_CHECK();
 }
 
 void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,

@@ -381,7 +381,7 @@ constexpr int kMaxCand = 16;  // orders p_min .. p_min + 15
 
 __global__ void __launch_bounds__(kSelBlock)
     k_select(const SelModel *model, const float *params, const float *feats, const DevHdr *hdr,
-             int fixed_p, DevHdr *hdr_w) {
+             int fixed_p, DevHdr *hdr_w, uint8_t *orders) {
     __shared__ float sp[kSelParams];
     __shared__ float se[kLv][kMaxCand], st[kLv][kMaxCand];
     __shared__ float sx[kSelRows][F], sh1[kSelRows][H + 1], sh2[kSelRows][H + 1];
@@ -519,6 +519,7 @@ __global__ void __launch_bounds__(kSelBlock)
         if (source == 1 && L >= 2 && L < nrow) p = M.p_min + choice;
         else if (source == 1 && L < 2 && nrow > 2) p = p2;
         hdr_w->sel_p[L] = (uint8_t)p;
+        orders[L] = (uint8_t)p;
     }
     if (lane == 0) {
         hdr_w->sel_source = (uint32_t)source;
@@ -545,9 +546,9 @@ void level_features(Ctx &c, int threshold, float *feats_dev, float *raw_dev) {
 }
 
 void select_orders(Ctx &c, const SelModel *model_dev, const float *params_dev,
-                   const float *feats_dev, int fixed_p) {
+                   const float *feats_dev, int fixed_p, uint8_t *orders_dev) {
     k_select<<<1, kSelBlock, 0, c.stream>>>(model_dev, params_dev, feats_dev, c.hdr_d, fixed_p,
-                                            c.hdr_d);
+                                            c.hdr_d, orders_dev);
     VPM_LAUNCH_CHECK();
 }
 

@@ -32,9 +32,10 @@ struct MlpWork {
 // Level statistics of the built tree in c: raw[(L) * kSelRaw + k] and feature
 // rows feats[L * kSelFeatures + i] for L = 0 .. kMaxDepth (order column 0).
 void level_features(Ctx &c, int threshold, float *feats_dev, float *raw_dev);
-// Orders per level into c.hdr_d->sel_* (one block, after level_features).
+// Orders per level into c.hdr_d->sel_* and orders_dev[0 .. kMaxDepth + 1]
+// (one block, after level_features).
 void select_orders(Ctx &c, const SelModel *model_dev, const float *params_dev,
-                   const float *feats_dev, int fixed_p);
+                   const float *feats_dev, int fixed_p, uint8_t *orders_dev);
 
 void mlp_forward(hipStream_t s, const float *params, const float *x, int rows, float *out,
                  float *h1, float *h2);

@@ -65,11 +65,16 @@ __device__ __forceinline__ double cell_len(double R, int level) {
 // ---------------------------------------------------------------------------
 // 1. bounding box and root geometry
 // ---------------------------------------------------------------------------
+// Block 0 also clears the device header (every later kernel of the build
+// runs after this one on the stream).
 __global__ void __launch_bounds__(256)
-    k_bbox_partial(const double *x, const double *y, int n, double *part) {
+    k_bbox_partial(const double *x, const double *y, int n, double *part, DevHdr *hdr) {
     __shared__ double s[4][256];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
+    if (blockIdx.x == 0)
+        for (int u = threadIdx.x; u < (int)(sizeof(DevHdr) / 4); u += blockDim.x)
+            reinterpret_cast<uint32_t *>(hdr)[u] = 0u;
     double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
     int nf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -178,15 +183,24 @@ __global__ void __launch_bounds__(256)
     val[i] = (uint32_t)i;
 }
 
+// Particles in Morton order.  The source strength is q[o], or, for the
+// velocity entry points (q == nullptr), -omega s^2 / (2 pi) formed here in the
+// operation order of velocity_calc.cpp:477 (pow(s, 2) is the exact square).
 __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
-             const uint8_t *a, int n, double *xs, double *ys, double *qs, uint8_t *as) {
+             const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
+             double *ys, double *qs, uint8_t *as) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = perm[i];
     xs[i] = x[o];
     ys[i] = y[o];
-    qs[i] = q[o];
+    if (q) {
+        qs[i] = q[o];
+    } else {
+        const double sz = ssz[o];
+        qs[i] = -omega[o] * (sz * sz) / (2 * M_PI);
+    }
     as[i] = a[o] ? 1 : 0;
 }
 
@@ -307,6 +321,68 @@ __global__ void __launch_bounds__(256)
 }
 
 constexpr int kSplitBlock = 256;
+constexpr int kRedBlock = 1024;
+
+// Reduction of the leaf split's per-block results in one workgroup of 1024:
+// min / max leaf level (and the depth-overflow bit), cells per level, leaves;
+// the level offsets of the cell arrays follow (root = cell 0, then levels in
+// order).  lvp is column-major: lvp[c * nblk + block].
+__global__ void __launch_bounds__(kRedBlock)
+    k_reduce_split(const ulonglong2 *part, const uint32_t *lvp, int nblk, DevHdr *hdr) {
+    __shared__ uint32_t sl[kRedBlock / kWave][kLvCols];
+    __shared__ unsigned long long smin[kRedBlock / kWave], smax[kRedBlock / kWave];
+    const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
+    unsigned long long a = ~0ull, b = 0ull;
+    uint32_t acc[kLvCols];
+#pragma unroll
+    for (int c = 0; c < kLvCols; ++c) acc[c] = 0;
+    for (int blk = tid; blk < nblk; blk += kRedBlock) {
+        const ulonglong2 v = part[blk];
+        a = min(a, v.x);
+        b = max(b, v.y);
+#pragma unroll
+        for (int c = 0; c < kLvCols; ++c) acc[c] += lvp[(size_t)c * nblk + blk];
+    }
+    for (int m = 1; m < kWave; m <<= 1) {
+        a = min(a, (unsigned long long)__shfl_xor(a, m, kWave));
+        b = max(b, (unsigned long long)__shfl_xor(b, m, kWave));
+#pragma unroll
+        for (int c = 0; c < kLvCols; ++c) acc[c] += __shfl_xor(acc[c], m, kWave);
+    }
+    if (lane == 0) {
+        smin[w] = a;
+        smax[w] = b;
+#pragma unroll
+        for (int c = 0; c < kLvCols; ++c) sl[w][c] = acc[c];
+    }
+    __syncthreads();
+    __shared__ uint32_t scol[kLvCols];
+    if (tid < kLvCols) {  // column sums over the waves, one thread per column
+        uint32_t v = 0;
+        for (int u = 0; u < kRedBlock / kWave; ++u) v += sl[u][tid];
+        scol[tid] = v;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    for (int u = 1; u < kRedBlock / kWave; ++u) {
+        a = min(a, smin[u]);
+        b = max(b, smax[u]);
+    }
+    // a = min leaf level; b = max leaf level | depth-overflow bit 32
+    hdr->leaf_lvl_min = (uint32_t)a;
+    hdr->leaf_lvl_max = (uint32_t)(b & 0xffffffffull);
+    if (b >> 32) hdr->flags |= kFlagDepth;
+    uint32_t off = 1;  // the root is cell 0
+    hdr->lev_off[0] = 0;
+    hdr->lev_cnt[0] = 1;
+    for (int L = 1; L <= kMaxDepth + 1; ++L) {
+        const uint32_t cnt = L <= kMaxDepth ? scol[L] : 0u;
+        hdr->lev_off[L] = off;
+        hdr->lev_cnt[L] = cnt;
+        off += cnt;
+    }
+    hdr->n_leaves_split = scol[kLvCols - 1];
+}
 
 // Sliding-window maxima by the van Herk / Gil-Werman method: with blocks of
 // length B aligned at 0, the max over [t, t + B - 1] is max(h[t], g[t + B - 1])
@@ -347,8 +423,13 @@ __device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *&
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, ulonglong2 *part) {
+            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, ulonglong2 *part,
+            uint32_t *lvpart) {
     extern __shared__ uint8_t lds[];
+    // cells started per level (a particle whose lcp with its predecessor is d
+    // starts the cells of levels d+1 .. its leaf level, the last a leaf)
+    __shared__ uint32_t lvc[kLvCols];
+    if (threadIdx.x < kLvCols) lvc[threadIdx.x] = 0;
     const int i0 = blockIdx.x * kSplitBlock;
     const int i = i0 + threadIdx.x;
     const int ilast = min(i0 + kSplitBlock, n) - 1;
@@ -405,10 +486,13 @@ __global__ void __launch_bounds__(kSplitBlock)
         leaf_lvl[i] = (uint8_t)lvl;
         const int di = d[i];
         ccount[i] = lvl > di ? (uint32_t)(lvl - di) : 0u;
+        for (int L = di + 1; L <= lvl; ++L) atomicAdd(&lvc[L], 1u);
+        if (lvl > di) atomicAdd(&lvc[kLvCols - 1], 1u);
         my_min = (unsigned long long)lvl;
         my_max |= (unsigned long long)lvl;
     }
-    block_part<true>(my_min, my_max, part);
+    block_part<true>(my_min, my_max, part);  // synchronises the block
+    if (threadIdx.x < kLvCols) lvpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = lvc[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -431,24 +515,13 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void k_lev_off(const uint32_t *lev_level, int nc, DevHdr *hdr) {
-    const int L = threadIdx.x;
-    if (L > kMaxDepth + 1) return;
-    int lo = 0, hi = nc;  // first index with level >= L
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if ((int)lev_level[mid] < L) lo = mid + 1;
-        else hi = mid;
-    }
-    hdr->lev_off[L] = (uint32_t)lo;
-}
-
 // Cell attributes: range, counts, flags, index, centre (treeCell.hpp:68).
 __global__ void __launch_bounds__(256)
     k_cells1(const uint32_t *lev_level, const uint32_t *lev_begin, int nc, const uint32_t *key,
              const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
              uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
-             uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy) {
+             uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
+             uint8_t *leaf01) {
 #pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nc) return;
@@ -496,6 +569,8 @@ __global__ void __launch_bounds__(256)
     c_src[j] = src;
     c_cx[j] = hdr->pivot[0] + ((0.5 + (double)ix) * len);
     c_cy[j] = hdr->pivot[1] + ((0.5 + (double)iy) * len);
+    reinterpret_cast<int4 *>(c_child)[j] = make_int4(-1, -1, -1, -1);  // filled by k_cells2
+    leaf01[j] = (f & kLeaf) ? 1 : 0;
 }
 
 __device__ __forceinline__ int find_cell(const uint32_t *c_key, const uint32_t *lev_off, int level,
@@ -527,12 +602,6 @@ __global__ void __launch_bounds__(256)
     c_parent[j] = par;
     const int qx = (int)(compact1by1(pre) & 1u), qy = (int)(compact1by1(pre >> 1) & 1u);
     if (par >= 0) c_child[4 * par + qx + 2 * qy] = j;
-}
-
-__global__ void __launch_bounds__(256)
-    k_leaf01(const uint8_t *c_flags, int nc, uint8_t *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nc) out[j] = (c_flags[j] & kLeaf) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(256)
@@ -628,147 +697,247 @@ __global__ void __launch_bounds__(256)
     nbr[9 * j + k] = res;
 }
 
-// Lists 2 and 4 of every cell of level >= 2 (treeCell.cpp:818-1032).
+// Lists 2 and 4 of every cell of level >= 2 (treeCell.cpp:818-1032).  Eight
+// lanes per cell; lane u examines the parent's neighbour k = u (u < 4) or
+// u + 1 (the parent itself, k = 4, is skipped as at treeCell.cpp:872):
+//  * a non-leaf neighbour at the parent's level contributes its active
+//    children that are well separated from the cell (list 2);
+//  * an active leaf neighbour is a list-4 candidate, kept at its first
+//    occurrence among the eight lanes and if separated from the cell.
+// The lanes' entries are compacted with a prefix sum over the eight lanes, in
+// lane order and child order within a lane.
+constexpr int kM2lLanes = 8;
+
 __global__ void __launch_bounds__(256)
     k_m2l_lists(const uint8_t *c_level, const uint8_t *c_flags, const double *c_cx,
                 const double *c_cy, const int *c_parent, const int *c_child, const int *nbr,
                 const DevHdr *hdr, int j0, int nc, int *l2, uint8_t *n2, int *l4, uint8_t *n4,
                 DevHdr *hdr_w, ulonglong2 *part) {
 #pragma clang fp contract(off)
-    const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
-    int m2 = 0, m4c = 0, m4 = 0;
-    if (j < nc) {
-    const int L = c_level[j];
-    const double R = hdr->root_len, s = cell_len(R, L);
-    const double cx = c_cx[j], cy = c_cy[j];
-    const int P = c_parent[j];
-    int cand[8];
-    const double md2 = s + kListEps;
-    for (int k = 0; k < 9; ++k) {
-        if (k == 4) continue;  // the parent itself (treeCell.cpp:872)
-        const int A = nbr[9 * P + k];
-        if (A < 0) continue;
-        const unsigned fa = c_flags[A];
-        if (!(fa & kActive)) continue;
-        if (c_level[A] == L - 1 && !(fa & kLeaf)) {
-            for (int q = 0; q < 4; ++q) {
-                const int ch = c_child[4 * A + q];
-                if (ch < 0 || !(c_flags[ch] & kActive)) continue;
-                const bool sep = fabs(c_cx[ch] - cx) > md2 || fabs(c_cy[ch] - cy) > md2;
-                if (sep) {
-                    if (m2 < 27) l2[27 * j + m2++] = ch;
-                    else atomicOr(&hdr_w->flags, kFlagListOverflow);
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + (int)(gid / kM2lLanes);
+    const int u = (int)(gid % kM2lLanes);
+    const int k = u < 4 ? u : u + 1;
+    const bool live = j < nc;
+    int ch[4] = {-1, -1, -1, -1};
+    bool keep[4] = {false, false, false, false};
+    int cand = -1;
+    double cx = 0.0, cy = 0.0, s = 0.0, R = 0.0;
+    if (live) {
+        const int L = c_level[j];
+        R = hdr->root_len;
+        s = cell_len(R, L);
+        cx = c_cx[j];
+        cy = c_cy[j];
+        const int A = nbr[9 * c_parent[j] + k];
+        if (A >= 0) {
+            const unsigned fa = c_flags[A];
+            if (fa & kActive) {
+                if (c_level[A] == L - 1 && !(fa & kLeaf)) {
+                    const double md2 = s + kListEps;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) ch[q] = c_child[4 * A + q];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (ch[q] < 0 || !(c_flags[ch[q]] & kActive)) continue;
+                        keep[q] = fabs(c_cx[ch[q]] - cx) > md2 || fabs(c_cy[ch[q]] - cy) > md2;
+                    }
+                } else if (fa & kLeaf) {
+                    cand = A;
                 }
             }
-        } else if (fa & kLeaf) {
-            bool dup = false;
-            for (int t = 0; t < m4c; ++t) dup |= cand[t] == A;
-            if (!dup) cand[m4c++] = A;
         }
     }
-    for (int t = 0; t < m4c; ++t) {
-        const int A = cand[t];
-        const double md = ((cell_len(R, c_level[A]) + s) / 2.0) + kListEps;
-        const bool sep = fabs(c_cx[A] - cx) > md || fabs(c_cy[A] - cy) > md;
-        if (sep) l4[8 * j + m4++] = A;
+    // first occurrence of a list-4 candidate among the cell's lanes
+    bool dup = false;
+#pragma unroll
+    for (int v = 0; v < kM2lLanes - 1; ++v) {
+        const int o = __shfl(cand, v, kM2lLanes);
+        dup |= v < u && o == cand;
     }
-    n2[j] = (uint8_t)m2;
-    n4[j] = (uint8_t)m4;
+    bool sep4 = false;
+    if (cand >= 0 && !dup) {
+        const double md = ((cell_len(R, c_level[cand]) + s) / 2.0) + kListEps;
+        sep4 = fabs(c_cx[cand] - cx) > md || fabs(c_cy[cand] - cy) > md;
     }
-    block_part<false>((unsigned long long)m2, (unsigned long long)m4, part);
+    const int m2 = (int)keep[0] + (int)keep[1] + (int)keep[2] + (int)keep[3];
+    const int m4 = sep4 ? 1 : 0;
+    int inc2 = m2, inc4 = m4;
+#pragma unroll
+    for (int o = 1; o < kM2lLanes; o <<= 1) {
+        const int y2 = __shfl_up(inc2, o, kM2lLanes), y4 = __shfl_up(inc4, o, kM2lLanes);
+        if (u >= o) {
+            inc2 += y2;
+            inc4 += y4;
+        }
+    }
+    const int tot2 = __shfl(inc2, kM2lLanes - 1, kM2lLanes);
+    const int tot4 = __shfl(inc4, kM2lLanes - 1, kM2lLanes);
+    if (live) {
+        int o2 = inc2 - m2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!keep[q]) continue;
+            if (o2 < 27) l2[27 * (size_t)j + o2] = ch[q];
+            ++o2;
+        }
+        if (sep4) l4[8 * (size_t)j + (inc4 - m4)] = cand;
+        if (u == 0) {
+            if (tot2 > 27) atomicOr(&hdr_w->flags, kFlagListOverflow);
+            n2[j] = (uint8_t)min(tot2, 27);
+            n4[j] = (uint8_t)tot4;
+        }
+    }
+    block_part<false>((unsigned long long)(live ? m2 : 0), (unsigned long long)(live ? m4 : 0),
+                      part);
 }
 
 // Lists 1 and 3 of every leaf (treeCell.cpp:636-804), count or fill pass.
-// The fill pass also counts the P2P pairs and M2P evaluations the lists imply
-// (statistics), reduced per block into per-block partials.
+// Sixteen lanes per leaf; lane k < 9 handles the leaf's neighbour k:
+//  * a coarser neighbour (reached by the parent walk, treeCell.cpp:706-720,
+//    738) joins list 1 if it is an active leaf, at its first occurrence among
+//    the nine lanes;
+//  * an active leaf at the leaf's level joins list 1;
+//  * an active non-leaf is walked depth first: children adjacent to the leaf
+//    join list 1 (leaves) or are walked (non-leaves), the others join list 3.
+// Each lane's walk only visits its own neighbour's subtree.  Along an edge a
+// level adds at most two adjacent children and the walk pops one, so the
+// per-lane stack (in LDS) grows by at most one entry per level.
+// The count pass stores each lane's counts (cntk) and the leaf totals; the
+// fill pass writes at the leaf offset plus the prefix of the lanes' counts and
+// also counts the P2P pairs and M2P / L2P terms the lists imply (statistics),
+// reduced per block into per-block partials.
+constexpr int kLeafLanes = 16;
+constexpr int kWalkDepth = 24;
+
 template <bool FILL>
 __global__ void __launch_bounds__(256)
     k_leaf_lists(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
                  const double *c_cx, const double *c_cy, const int *c_child, const int *nbr,
-                 const DevHdr *hdr, uint32_t *cnt1, uint32_t *cnt3, const uint32_t *off1,
-                 const uint32_t *off3, int *l1, int *l3, const uint32_t *c_begin,
-                 const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part, ulonglong2 *part_terms) {
+                 const DevHdr *hdr, const uint8_t *orders, uint32_t *cnt1, uint32_t *cnt3,
+                 uint2 *cntk, const uint32_t *off1, const uint32_t *off3, int *l1, int *l3,
+                 uint32_t cap1, uint32_t cap3, const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part,
+                 ulonglong2 *part_terms) {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int stk[kWalkDepth][256];
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(gid / kLeafLanes);
+    const int k = (int)(gid % kLeafLanes);
+    const bool leaf_live = t < nl;
+    const bool live = leaf_live && k < 9;
     unsigned long long my_pairs = 0, my_m2p = 0, my_m2p_terms = 0, my_l2p_terms = 0;
-    unsigned long long m2p_w = 0;  // sum over list-3 cells of (order of the cell's level + 1)
     bool overflow = false;
-    if (t < nl) {
-        const int j = (int)leaf_idx[t];
-        const int L = c_level[j];
-        const double R = hdr->root_len, s = cell_len(R, L);
-        const double cx = c_cx[j], cy = c_cy[j];
-        uint32_t n1 = 0, n3 = 0;
-        unsigned long long src_total = 0;
-        bool self = false;
-        const uint32_t o1 = FILL ? off1[t] : 0, o3 = FILL ? off3[t] : 0;
-        int coarse[9];
-        int nco = 0;
-        int stack[64];
-        int sp = 0;
-        auto emit1 = [&](int c) {
-            if (FILL) {
-                l1[o1 + n1] = c;
-                src_total += c_end[c] - c_begin[c];
-                self |= c == j;
+    int j = 0, L = 0;
+    double R = 0.0, s = 0.0, cx = 0.0, cy = 0.0;
+    int A = -1, cand = -1;
+    unsigned fa = 0;
+    if (live) {
+        j = (int)leaf_idx[t];
+        L = c_level[j];
+        R = hdr->root_len;
+        s = cell_len(R, L);
+        cx = c_cx[j];
+        cy = c_cy[j];
+        A = nbr[9 * j + k];
+        if (A >= 0) {
+            fa = c_flags[A];
+            if (c_level[A] < L && (fa & kLeaf) && (fa & kActive)) cand = A;
+        }
+    }
+    bool dup = false;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const int o = __shfl(cand, v, kLeafLanes);
+        dup |= v < k && o == cand;
+    }
+    uint32_t o1 = 0, o3 = 0;
+    if (FILL) {
+        const uint2 mine = live ? cntk[9 * (size_t)t + k] : make_uint2(0u, 0u);
+        uint32_t i1 = mine.x, i3 = mine.y;
+#pragma unroll
+        for (int o = 1; o < kLeafLanes; o <<= 1) {
+            const uint32_t y1 = __shfl_up(i1, o, kLeafLanes), y3 = __shfl_up(i3, o, kLeafLanes);
+            if (k >= o) {
+                i1 += y1;
+                i3 += y3;
             }
-            ++n1;
-        };
-        for (int k = 0; k < 9; ++k) {
-            const int A = nbr[9 * j + k];
-            if (A < 0) continue;
-            const unsigned fa = c_flags[A];
-            if (c_level[A] < L) {  // reached through the parent walk (treeCell.cpp:706-720, 738)
-                if ((fa & kLeaf) && (fa & kActive)) {
-                    bool dup = false;
-                    for (int u = 0; u < nco; ++u) dup |= coarse[u] == A;
-                    if (!dup) {
-                        coarse[nco++] = A;
-                        emit1(A);
-                    }
-                }
-                continue;
-            }
-            if (!(fa & kActive)) continue;
+        }
+        if (leaf_live) {
+            o1 = off1[t] + i1 - mine.x;
+            o3 = off3[t] + i3 - mine.y;
+        }
+    }
+    uint32_t n1 = 0, n3 = 0;
+    unsigned long long src_total = 0, m2p_w = 0;
+    bool self = false;
+    auto emit1 = [&](int c) {
+        if (FILL) {
+            if (o1 + n1 < cap1) l1[o1 + n1] = c;
+            else overflow = true;
+            src_total += c_end[c] - c_begin[c];
+            self |= c == j;
+        }
+        ++n1;
+    };
+    if (live && A >= 0) {
+        if (c_level[A] < L) {
+            if (cand >= 0 && !dup) emit1(A);
+        } else if (fa & kActive) {
             if (fa & kLeaf) {
                 emit1(A);
-                continue;
-            }
-            if (sp < 64) stack[sp++] = A;
-            else overflow = true;
-        }
-        while (sp > 0) {
-            const int B = stack[--sp];
-            const double adj = ((s + (cell_len(R, c_level[B]) / 2.0)) / 2.0) + kListEps;
-            for (int q = 0; q < 4; ++q) {
-                const int ch = c_child[4 * B + q];
-                if (ch < 0) continue;
-                const unsigned fc = c_flags[ch];
-                if (!(fc & kActive)) continue;
-                const bool is_adj = fabs(c_cx[ch] - cx) <= adj && fabs(c_cy[ch] - cy) <= adj;
-                if (is_adj) {
-                    if (fc & kLeaf) emit1(ch);
-                    else if (sp < 64) stack[sp++] = ch;
-                    else overflow = true;
-                } else {
-                    if (FILL) {
-                        l3[o3 + n3] = ch;
-                        m2p_w += (unsigned long long)hdr->sel_p[c_level[ch]] + 1ull;
+            } else {
+                const int me = threadIdx.x;
+                int sp = 0;
+                stk[sp++][me] = A;
+                while (sp > 0) {
+                    const int B = stk[--sp][me];
+                    const double adj = ((s + (cell_len(R, c_level[B]) / 2.0)) / 2.0) + kListEps;
+                    int chq[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) chq[q] = c_child[4 * B + q];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int ch = chq[q];
+                        if (ch < 0) continue;
+                        const unsigned fc = c_flags[ch];
+                        if (!(fc & kActive)) continue;
+                        const bool is_adj =
+                            fabs(c_cx[ch] - cx) <= adj && fabs(c_cy[ch] - cy) <= adj;
+                        if (is_adj) {
+                            if (fc & kLeaf) emit1(ch);
+                            else if (sp < kWalkDepth) stk[sp++][me] = ch;
+                            else overflow = true;
+                        } else {
+                            if (FILL) {
+                                if (o3 + n3 < cap3) l3[o3 + n3] = ch;
+                                else overflow = true;
+                                m2p_w += (unsigned long long)orders[c_level[ch]] + 1ull;
+                            }
+                            ++n3;
+                        }
                     }
-                    ++n3;
                 }
             }
         }
-        if (!FILL) {
-            cnt1[t] = n1;
-            cnt3[t] = n3;
-        } else {
-            const unsigned long long nt = c_end[j] - c_begin[j];
-            my_pairs = src_total * nt - (self ? nt : 0ull);
-            my_m2p = (unsigned long long)n3 * nt;
-            my_m2p_terms = m2p_w * nt;
-            my_l2p_terms = (unsigned long long)hdr->sel_p[L] * nt;
+    }
+    if (!FILL) {
+        uint32_t s1 = n1, s3 = n3;
+#pragma unroll
+        for (int o = 1; o < kLeafLanes; o <<= 1) {
+            s1 += __shfl_xor(s1, o, kLeafLanes);
+            s3 += __shfl_xor(s3, o, kLeafLanes);
         }
+        if (live) cntk[9 * (size_t)t + k] = make_uint2(n1, n3);
+        if (leaf_live && k == 0) {
+            cnt1[t] = s1;
+            cnt3[t] = s3;
+        }
+    } else if (live) {
+        const unsigned long long nt = c_end[j] - c_begin[j];
+        my_pairs = src_total * nt - (self ? nt : 0ull);
+        my_m2p = (unsigned long long)n3 * nt;
+        my_m2p_terms = m2p_w * nt;
+        if (k == 4) my_l2p_terms = (unsigned long long)orders[L] * nt;
     }
     if (overflow) atomicOr(&hdr_w->flags, kFlagListOverflow);
     if (FILL) {
@@ -796,16 +965,15 @@ static void check_flags(const DevHdr &h) {
     if (h.flags & kFlagListOverflow) throw Error(kInternal, "list traversal stack overflow");
 }
 
-void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q,
+void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &src,
                 const uint8_t *active, const TreeParams &tp) {
     hipStream_t s = c.stream;
     const int W = tp.max_par, S = tp.max_src;
-    VPM_HIP(hipMemsetAsync(c.hdr_d, 0, sizeof(DevHdr), s));
 
-    // 1. bounding box
+    // 1. bounding box (the first kernel also clears the header)
     const int nbb = std::min(1024, div_up(n, 256 * 8));
     double *part = c.bbox_part.as<double>(5 * (size_t)nbb);
-    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part);
+    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d);
     const double expansion = 1.0 + (tp.exp_tree / 100.0);
     k_bbox_final<<<1, 256, 0, s>>>(part, nbb, expansion, c.hdr_d);
     VPM_LAUNCH_CHECK();
@@ -824,9 +992,10 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     c.perm_sorted = perm;
     double *xs = c.xs.as<double>(n), *ys = c.ys.as<double>(n), *qs = c.qs.as<double>(n);
     uint8_t *as = c.acts.as<uint8_t>(n);
-    k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, q, active, n, xs, ys, qs, as);
+    k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
+                                            ys, qs, as);
 
-    // 4. leaf levels
+    // 4. leaf levels, cells per level and leaves
     uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
     k_lcp<<<div_up(n + 1, 256), 256, 0, s>>>(key, n, W, d, w);
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
@@ -840,26 +1009,34 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     const size_t lds = 5 * (size_t)(W + kSplitBlock) + 5 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
     ulonglong2 *spart = c.part.as<ulonglong2>(nbs);
+    uint32_t *lvp = c.lvpart.as<uint32_t>((size_t)nbs * kLvCols);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
-                                          tp.max_level, leaf_lvl, cnt, spart);
-    k_reduce_part<<<1, 256, 0, s>>>(spart, nbs, kPartSplit, c.hdr_d);
+                                          tp.max_level, leaf_lvl, cnt, spart, lvp);
+    k_reduce_split<<<1, kRedBlock, 0, s>>>(spart, lvp, nbs, c.hdr_d);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
     scan_exclusive_u32(cnt, base, n, &c.hdr_d->n_cells_pre, true, tmp, s);
     VPM_LAUNCH_CHECK();
+    // the one synchronisation of the build: sizes of every later launch
     sync_hdr(c);
-    check_flags(*c.hdr_h);
+    const DevHdr &hh = *c.hdr_h;
+    check_flags(hh);
 
     // 5. cells
-    const int nc = 1 + (int)c.hdr_h->n_cells_pre;
+    const int nc = 1 + (int)hh.n_cells_pre;
     TreeInfo &ti = c.info;
     ti.n = n;
-    ti.n_active = (int)c.hdr_h->n_active;
+    ti.n_active = (int)hh.n_active;
     ti.n_cells = nc;
-    ti.max_level = (int)c.hdr_h->leaf_lvl_max;
-    ti.min_leaf_level = (int)c.hdr_h->leaf_lvl_min;
-    ti.root_len = c.hdr_h->root_len;
-    ti.pivot[0] = c.hdr_h->pivot[0];
-    ti.pivot[1] = c.hdr_h->pivot[1];
+    ti.max_level = (int)hh.leaf_lvl_max;
+    ti.min_leaf_level = (int)hh.leaf_lvl_min;
+    ti.root_len = hh.root_len;
+    ti.pivot[0] = hh.pivot[0];
+    ti.pivot[1] = hh.pivot[1];
+    for (int l = 0; l < kMaxDepth + 2; ++l) ti.lev_off[l] = hh.lev_off[l];
+    ti.n_leaves = (int)hh.n_leaves_split;
+    ti.tot1 = ti.tot3 = 0;
+    if ((int)hh.lev_off[kMaxDepth + 1] != nc || ti.n_leaves < 1 || ti.n_leaves > nc)
+        throw Error(kInternal, "cell counts of the leaf split disagree with the cell emission");
 
     uint32_t *pl = c.pre_level.as<uint32_t>(nc), *pb = c.pre_slot.as<uint32_t>(nc);
     uint32_t *ll = c.lev_level.as<uint32_t>(nc), *lb = c.lev2pre.as<uint32_t>(nc);
@@ -867,21 +1044,18 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
     const bool alt2 = radix_sort_pairs(pl, pb, ll, lb, nc, 0, 8, tmp, s);
     const uint32_t *lev_level = alt2 ? ll : pl;
     const uint32_t *lev_begin = alt2 ? lb : pb;
-    k_lev_off<<<1, 64, 0, s>>>(lev_level, nc, c.hdr_d);
 
     uint8_t *c_level = c.c_level.as<uint8_t>(nc), *c_flags = c.c_flags.as<uint8_t>(nc);
     uint32_t *c_key = c.c_key.as<uint32_t>(nc), *c_begin = c.c_begin.as<uint32_t>(nc);
     uint32_t *c_end = c.c_end.as<uint32_t>(nc), *c_src = c.c_src.as<uint32_t>(nc);
     double *c_cx = c.c_cx.as<double>(nc), *c_cy = c.c_cy.as<double>(nc);
     int *c_parent = c.c_parent.as<int>(nc), *c_child = c.c_child.as<int>(4 * (size_t)nc);
+    uint8_t *leaf01 = c.lcp_w.as<uint8_t>(std::max(n + 1, nc));
     k_cells1<<<div_up(nc, 256), 256, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
-                                             c_src, c_cx, c_cy);
-    VPM_HIP(hipMemsetAsync(c_child, 0xFF, 4 * sizeof(int) * (size_t)nc, s));
+                                             c_src, c_cx, c_cy, c_child, leaf01);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
 
-    uint8_t *leaf01 = c.lcp_w.as<uint8_t>(std::max(n + 1, nc));
-    k_leaf01<<<div_up(nc, 256), 256, 0, s>>>(c_flags, nc, leaf01);
     uint32_t *lscan = c.leaf_flag_scan.as<uint32_t>(nc + 1);
     scan_exclusive_u8(leaf01, lscan, nc, &c.hdr_d->n_leaves, true, tmp, s);
     uint32_t *leaf_idx = c.leaf_idx.as<uint32_t>(nc);
@@ -891,7 +1065,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const double *q
 
 void tree_lists(Ctx &c) {
     hipStream_t s = c.stream;
-    const int nc = c.info.n_cells, n = c.info.n;
+    TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves;
+    if (!c.orders) throw Error(kInternal, "expansion orders not set before the list build");
     uint8_t *c_level = (uint8_t *)c.c_level.p, *c_flags = (uint8_t *)c.c_flags.p;
     uint32_t *c_key = (uint32_t *)c.c_key.p;
     double *c_cx = (double *)c.c_cx.p, *c_cy = (double *)c.c_cy.p;
@@ -900,21 +1076,13 @@ void tree_lists(Ctx &c) {
     uint32_t *tmp = (uint32_t *)c.tmp.p;
     k_nbr<<<div_up(9LL * nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, nbr);
 
-    // level offsets and leaf count are needed on the host for the launches below
-    sync_hdr(c);
-    TreeInfo &ti = c.info;
-    for (int l = 0; l < kMaxDepth + 2; ++l) ti.lev_off[l] = c.hdr_h->lev_off[l];
-    ti.n_leaves = (int)c.hdr_h->n_leaves;
-    const int nl = ti.n_leaves;
-    (void)n;
-
     int *l2 = c.l2.as<int>(27 * (size_t)nc), *l4 = c.l4.as<int>(8 * (size_t)nc);
     uint8_t *n2 = c.n2.as<uint8_t>(nc), *n4 = c.n4.as<uint8_t>(nc);
     const int j0 = (int)ti.lev_off[2];
     // n2 / n4 of every cell of level >= 2 are written by k_m2l_lists; levels 0
     // and 1 have no lists 2 / 4 and nothing reads their entries
-    const int nb2 = nc > j0 ? div_up(nc - j0, 256) : 0;
-    const int nbl = div_up(nl, 256);
+    const int nb2 = nc > j0 ? (int)div_up((long long)(nc - j0) * kM2lLanes, 256LL) : 0;
+    const int nbl = (int)div_up((long long)nl * kLeafLanes, 256LL);
     ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, 2 * nbl), 1));
     if (nb2 > 0)
         k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
@@ -924,25 +1092,41 @@ void tree_lists(Ctx &c) {
     uint32_t *cnt1 = c.cnt1.as<uint32_t>(nl + 1), *cnt3 = c.cnt3.as<uint32_t>(nl + 1);
     uint32_t *off1 = c.off1.as<uint32_t>(nl + 1), *off3 = c.off3.as<uint32_t>(nl + 1);
     const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
-    k_leaf_lists<false><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
-                                                         c_cy, c_child, nbr, c.hdr_d, cnt1, cnt3,
-                                                         nullptr, nullptr, nullptr, nullptr,
-                                                         c_begin, c_end, c.hdr_d, nullptr, nullptr);
+    uint2 *cntk = c.cntk.as<uint2>(9 * (size_t)nl + 1);
+    k_leaf_lists<false><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
+                                            nbr, c.hdr_d, c.orders, cnt1, cnt3, cntk, nullptr,
+                                            nullptr, nullptr, nullptr, 0, 0, c_begin, c_end,
+                                            c.hdr_d, nullptr, nullptr);
     scan_exclusive_u32(cnt1, off1, nl, &c.hdr_d->tot1, true, tmp, s);
     scan_exclusive_u32(cnt3, off3, nl, &c.hdr_d->tot3, true, tmp, s);
-    VPM_LAUNCH_CHECK();
-    sync_hdr(c);
-    check_flags(*c.hdr_h);
-    ti.tot1 = (int)c.hdr_h->tot1;
-    ti.tot3 = (int)c.hdr_h->tot3;
-    int *l1 = c.l1.as<int>((size_t)ti.tot1 + 1), *l3 = c.l3.as<int>((size_t)ti.tot3 + 1);
-    k_leaf_lists<true><<<div_up(nl, 256), 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx,
-                                                        c_cy, c_child, nbr, c.hdr_d, nullptr,
-                                                        nullptr, off1, off3, l1, l3, c_begin, c_end,
-                                                        c.hdr_d, part, part + nbl);
+    // Capacities without reading the totals back.  A leaf's list 1 holds
+    // itself, at most 8 adjacent leaves at its level or coarser (one per
+    // direction) and the finer adjacent leaves; each finer leaf is adjacent to
+    // at most 8 coarser leaves, so sum |list 1| <= 17 nl.  A cell is in the
+    // list 3 of leaves coarser than it and adjacent to its parent: at most 8,
+    // so sum |list 3| <= 8 nc.  The fill pass checks both anyway.
+    ti.cap1 = 17 * (size_t)nl + 1;
+    ti.cap3 = 8 * (size_t)nc + 1;
+    int *l1 = c.l1.as<int>(ti.cap1), *l3 = c.l3.as<int>(ti.cap3);
+    k_leaf_lists<true><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
+                                           nbr, c.hdr_d, c.orders, nullptr, nullptr, cntk, off1,
+                                           off3, l1, l3, (uint32_t)ti.cap1, (uint32_t)ti.cap3,
+                                           c_begin, c_end, c.hdr_d, part, part + nbl);
     k_reduce_part<<<1, 256, 0, s>>>(part, nbl, kPartLeafLists, c.hdr_d);
     k_reduce_part<<<1, 256, 0, s>>>(part + nbl, nbl, kPartLeafTerms, c.hdr_d);
     VPM_LAUNCH_CHECK();
+}
+
+void tree_lists_absorb(Ctx &c) {
+    const DevHdr &hh = *c.hdr_h;
+    TreeInfo &ti = c.info;
+    check_flags(hh);
+    if ((int)hh.n_leaves != ti.n_leaves)
+        throw Error(kInternal, "leaf count of the cell build disagrees with the leaf split");
+    if ((size_t)hh.tot1 >= ti.cap1 || (size_t)hh.tot3 >= ti.cap3)
+        throw Error(kInternal, "interaction lists exceed their bounds");
+    ti.tot1 = (int)hh.tot1;
+    ti.tot3 = (int)hh.tot3;
 }
 
 }  // namespace vpm
