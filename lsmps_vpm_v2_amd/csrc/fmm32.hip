@@ -148,19 +148,25 @@ __global__ void __launch_bounds__(kWave *kP2mWaves)
 // q is the child quadrant, c_child's order: x bit + 2 * y bit.
 constexpr int kMaxPM1 = 17;
 
-// M2M into parent j (inactive parents get zero: their own parents read them).
-__device__ __forceinline__ void m2m_wave(int j, int lane, int PM1, const uint8_t *c_level,
+// Child q's index from the parent's four child slots.
+__device__ __forceinline__ int child_of(int4 c4, int q) {
+    return q == 0 ? c4.x : q == 1 ? c4.y : q == 2 ? c4.z : c4.w;
+}
+
+// M2M into parent j of order p from children of order pc (inactive parents
+// get zero: their own parents read them).  The caller knows the level, so
+// the parent's flags and child slots are the only loads before the children's
+// coefficients.
+__device__ __forceinline__ void m2m_wave(int j, int p, int pc, int lane, int PM1,
                                          const uint8_t *c_flags, const int *c_child,
-                                         const LevelTab *lt, const float2 *tab, float2 *sa,
-                                         float *ahat) {
+                                         const float2 *tab, float2 *sa, float *ahat) {
+    const int4 c4 = reinterpret_cast<const int4 *>(c_child)[j];
     const unsigned f = c_flags[j];
     if (f & kLeaf) return;
-    const int L = c_level[j];
-    const int p = lt->p[L], pc = lt->p[L + 1];
     const bool act = (f & kActive) != 0;
     for (int idx = lane; idx < 4 * PM1; idx += kWave) {
         const int q = idx / PM1, k = idx % PM1;
-        const int ch = act ? c_child[4 * j + q] : -1;
+        const int ch = act ? child_of(c4, q) : -1;
         sa[idx] = (ch >= 0 && k <= pc)
                       ? reinterpret_cast<const float2 *>(ahat)[(size_t)ch * PM1 + k]
                       : make_float2(0.f, 0.f);
@@ -186,21 +192,19 @@ __device__ __forceinline__ void m2m_wave(int j, int lane, int PM1, const uint8_t
     wave_sync();
 }
 
-// L2L from parent j into its children: lane = (q, m), q = lane / 16,
-// m = lane % 16 + 1, adds sum_{k >= m} l2l[q][m][k] b^_k(parent).
-__device__ __forceinline__ void l2l_wave(int j, int lane, int PM1, const uint8_t *c_level,
+// L2L from parent j (order pp) into its children (order pc): lane = (q, m),
+// q = lane / 16, m = lane % 16 + 1, adds sum_{k >= m} l2l[q][m][k] b^_k(parent).
+__device__ __forceinline__ void l2l_wave(int j, int pp, int pc, int lane, int PM1,
                                          const uint8_t *c_flags, const int *c_child,
-                                         const LevelTab *lt, const float2 *tab, float2 *sb,
-                                         float *bhat) {
+                                         const float2 *tab, float2 *sb, float *bhat) {
+    const int4 c4 = reinterpret_cast<const int4 *>(c_child)[j];
     if (c_flags[j] & kLeaf) return;
-    const int L = c_level[j];
-    const int pp = lt->p[L], pc = lt->p[L + 1];
     for (int k = lane; k < PM1; k += kWave)
         sb[k] = k <= pp ? reinterpret_cast<const float2 *>(bhat)[(size_t)j * PM1 + k]
                         : make_float2(0.f, 0.f);
     wave_sync();
     const int q = lane >> 4, m = (lane & 15) + 1;
-    const int ch = c_child[4 * j + q];
+    const int ch = child_of(c4, q);
     if (ch >= 0 && m <= pc) {
         // fully unrolled over k with two accumulators (even / odd k)
         const float2 *row = tab + (q * PM1 + m) * PM1;
@@ -218,9 +222,12 @@ __device__ __forceinline__ void l2l_wave(int j, int lane, int PM1, const uint8_t
     wave_sync();
 }
 
-__device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab) {
+// Table (4 PM1 x PM1 complex) and the order of every level into LDS.
+__device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab,
+                                         const LevelTab *lt, int *s_lp) {
     const float2 *src = reinterpret_cast<const float2 *>(g);
     for (int k = threadIdx.x; k < 4 * PM1 * PM1; k += blockDim.x) tab[k] = src[k];
+    if (threadIdx.x < kMaxDepth + 2) s_lp[threadIdx.x] = lt->p[threadIdx.x];
     __syncthreads();
 }
 
@@ -263,72 +270,81 @@ __device__ __forceinline__ void subtree_ranges(int K, int maxL, const DevHdr *hd
 
 __global__ void __launch_bounds__(kSubBlock)
     k_m2m_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
-                  const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
+                  const uint8_t *c_flags, const int *c_child,
                   const LevelTab *lt, const float *m2m, int kst, float *ahat) {
     __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    __shared__ int s_lp[kMaxDepth + 2];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab);
+    load_tab(m2m, PM1, tab, lt, s_lp);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
     for (int L = maxL - 1; L >= K; --L) {
         for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
-            m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
+            m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
         __syncthreads();
     }
 }
 
 // M2M for the parents of levels Lhi .. 1 (top levels, few cells): one workgroup.
 __global__ void __launch_bounds__(kTopBlock)
-    k_m2m_top(int Lhi, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
-              const int *c_child, const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    k_m2m_top(int Lhi, const DevHdr *hdr, const uint8_t *c_flags, const int *c_child,
+              const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    __shared__ int s_lp[kMaxDepth + 2], s_off[kMaxDepth + 2];
+    if (threadIdx.x < kMaxDepth + 2) s_off[threadIdx.x] = (int)hdr->lev_off[threadIdx.x];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kTopBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab);
+    load_tab(m2m, PM1, tab, lt, s_lp);
     for (int L = Lhi; L >= 1; --L) {
-        for (int j = (int)hdr->lev_off[L] + w; j < (int)hdr->lev_off[L + 1];
-             j += kTopBlock / kWave)
-            m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
+        for (int j = s_off[L] + w; j < s_off[L + 1]; j += kTopBlock / kWave)
+            m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
         __syncthreads();
     }
 }
 
 // M2M / L2L for the parent cells [j0, j1) of one level: wavefront per cell.
 __global__ void __launch_bounds__(kSubBlock)
-    k_m2m_level(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags,
-                const int *c_child, const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    k_m2m_level(int L, int j0, int j1, const uint8_t *c_flags, const int *c_child,
+                const LevelTab *lt, const float *m2m, int kst, float *ahat) {
+    __shared__ int s_lp[kMaxDepth + 2];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab);
+    load_tab(m2m, PM1, tab, lt, s_lp);
     const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
-    if (j < j1) m2m_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sa[w], ahat);
+    if (j < j1) {
+        m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
+    }
 }
 
 __global__ void __launch_bounds__(kSubBlock)
-    k_l2l_level(int j0, int j1, const uint8_t *c_level, const uint8_t *c_flags,
-                const int *c_child, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    k_l2l_level(int L, int j0, int j1, const uint8_t *c_flags, const int *c_child,
+                const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    __shared__ int s_lp[kMaxDepth + 2];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab);
+    load_tab(l2l, PM1, tab, lt, s_lp);
     const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
-    if (j < j1) l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
+    if (j < j1) {
+        l2l_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sb[w], bhat);
+    }
 }
 
 // L2L parents at levels 2 .. Lhi (children 3 .. Lhi + 1): one workgroup.
 __global__ void __launch_bounds__(kTopBlock)
-    k_l2l_top(int Lhi, const DevHdr *hdr, const uint8_t *c_level, const uint8_t *c_flags,
-              const int *c_child, const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    k_l2l_top(int Lhi, const DevHdr *hdr, const uint8_t *c_flags, const int *c_child,
+              const LevelTab *lt, const float *l2l, int kst, float *bhat) {
+    __shared__ int s_lp[kMaxDepth + 2], s_off[kMaxDepth + 2];
+    if (threadIdx.x < kMaxDepth + 2) s_off[threadIdx.x] = (int)hdr->lev_off[threadIdx.x];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kTopBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab);
+    load_tab(l2l, PM1, tab, lt, s_lp);
     for (int L = 2; L <= Lhi; ++L) {
-        for (int j = (int)hdr->lev_off[L] + w; j < (int)hdr->lev_off[L + 1];
-             j += kTopBlock / kWave)
-            l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
+        for (int j = s_off[L] + w; j < s_off[L + 1]; j += kTopBlock / kWave)
+            l2l_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sb[w], bhat);
         __syncthreads();
     }
 }
@@ -336,17 +352,18 @@ __global__ void __launch_bounds__(kTopBlock)
 // L2L parents at levels max(K, 2) .. maxL-1 of one subtree (children K+1 ..)
 __global__ void __launch_bounds__(kSubBlock)
     k_l2l_subtree(int K, int maxL, const DevHdr *hdr, const uint32_t *c_key,
-                  const uint8_t *c_level, const uint8_t *c_flags, const int *c_child,
+                  const uint8_t *c_flags, const int *c_child,
                   const LevelTab *lt, const float *l2l, int kst, float *bhat) {
     __shared__ int s_lo[kMaxDepth + 2], s_hi[kMaxDepth + 2];
+    __shared__ int s_lp[kMaxDepth + 2];
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab);
+    load_tab(l2l, PM1, tab, lt, s_lp);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
     for (int L = max(K, 2); L <= maxL - 1; ++L) {
         for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
-            l2l_wave(j, lane, PM1, c_level, c_flags, c_child, lt, tab, sb[w], bhat);
+            l2l_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sb[w], bhat);
         __syncthreads();
     }
 }
@@ -467,14 +484,15 @@ __global__ void __launch_bounds__(256)
         lt->inv_s[tid] = (float)(1.0 / side);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid >= 2 && tid <= maxL) {
+        // tiles of the levels ordered before this one: lower order, or the
+        // same order and a lower level
+        const int L = tid, pL = s_p[L];
         int off = 0;
-        for (int p = 1; p <= 16; ++p)
-            for (int L = 2; L <= maxL; ++L) {
-                if (s_p[L] != p) continue;
-                s_first[L] = off;
-                off += (s_cnt[L] + 15) / 16;
-            }
+#pragma unroll
+        for (int u = 2; u < kMaxDepth + 2; ++u)
+            if (u <= maxL && (s_p[u] < pL || (s_p[u] == pL && u < L))) off += (s_cnt[u] + 15) / 16;
+        s_first[L] = off;
     }
     __syncthreads();
     for (int L = 2; L <= maxL; ++L) {
@@ -582,24 +600,31 @@ __global__ void __launch_bounds__(256)
     stc(bhat, o, cadd(ldc(bhat, o), acc));
 }
 
-// Evaluation: one wavefront per leaf; list-1 sources are staged in LDS as
-// (x - c_t, y - c_t, q, index) with the centre shift done in FP64 and rounded
-// once.  Lanes are
-// split into G groups of 64/G: a group owns the targets, groups split the
-// sources and list-3 cells and are reduced with cross-lane shuffles.
-// Two particles at one position always share a leaf, so only the sources of
-// the target's own leaf need the self test (by index); the other list-1 cells
-// run a loop without it.
+// Evaluation: one wavefront per leaf.  Sources are staged in LDS in rounds of
+// kStage as (x - c_t, y - c_t, q, index), the centre shift done in FP64 and
+// rounded once.  Lanes split into G groups of 64/G (G = 8, 4, 2, 1 for leaves
+// of up to 8, 16, 32, more targets): a group owns the targets, the groups
+// split the sources and list-3 cells and are reduced with cross-lane shuffles.
+// Every round is padded to a multiple of G with far-away sources of zero
+// strength, so all lanes run one trip count with a compile-time stride.  Two
+// particles at one position always share a leaf, so only the leaf's own
+// particles (a phase of their own) need the self test, by index.
 constexpr int kEvalWaves = 4;
 constexpr int kStage = 256;  // sources staged per wavefront per round
 
-// P2P over staged sources r = r0, r0 + G, ... < r1 (r0 = g mod G shifted to
-// the first r >= lo); SELF adds the self test.
-template <bool SELF>
-__device__ __forceinline__ void p2p_run(const float4 *stage, int lo, int hi, int g, int G,
-                                        float2 ti, uint32_t i, float &ex, float &ey) {
-    for (int r = lo + ((g - lo) & (G - 1)); r < hi; r += G) {
-        const float4 sv = stage[r];
+// Padding source: r^2 ~ 1e36 stays finite and q = 0 contributes exactly 0.
+__device__ __forceinline__ float4 far_source() {
+    return make_float4(1e18f, 1e18f, 0.f, __uint_as_float(0xffffffffu));
+}
+
+// P2P of one target over the staged sources g, g + G, ... (st = stage + g,
+// nu = padded round length / G).
+template <int G, bool SELF>
+__device__ __forceinline__ void p2p_round(const float4 *st, int nu, float2 ti, uint32_t i,
+                                          float &ex, float &ey) {
+#pragma unroll 4
+    for (int u = 0; u < nu; ++u) {
+        const float4 sv = st[u * G];
         const float dx = ti.x - sv.x, dy = ti.y - sv.y;
         const float r2 = dx * dx + dy * dy;
         float inv = __builtin_amdgcn_rcpf(r2);
@@ -610,148 +635,193 @@ __device__ __forceinline__ void p2p_run(const float4 *stage, int lo, int hi, int
     }
 }
 
-__global__ void __launch_bounds__(kWave *kEvalWaves)
-    k_eval32(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint32_t *c_begin,
-             const uint32_t *c_end, const double *c_cx, const double *c_cy,
-             const uint32_t *off1, const int *l1, const uint32_t *off3, const int *l3,
-             const float2 *oxy, const float *qf, const uint32_t *perm, const LevelTab *lt,
-             const float *ahat, const float *bhat, int kst, int mask, int velocity, double u_inf,
-             double v_inf, double *out_a, double *out_b) {
-    __shared__ float4 stage[kEvalWaves][kStage];
-    __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
-    __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
-    const int PM1 = kst / 2;
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int t = blockIdx.x * kEvalWaves + w;
-    if (t >= nl) return;
-    const int j = (int)leaf_idx[t];
-    const uint32_t b = c_begin[j], e = c_end[j];
-    const int nt = (int)(e - b);
-    const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
-    const int LG = kWave / G;
+struct EvalArgs {
+    const uint32_t *leaf_idx;
+    const uint8_t *c_level, *c_flags;
+    const uint32_t *c_begin, *c_end;
+    const double *c_cx, *c_cy;
+    const uint32_t *off1;
+    const int *l1;
+    const uint32_t *off3;
+    const int *l3;
+    const float2 *oxy;
+    const float *qf;
+    const uint32_t *perm;
+    const LevelTab *lt;
+    const float *ahat, *bhat;
+    int kst, mask, velocity;
+    double u_inf, v_inf;
+    double *out_a, *out_b;
+};
+
+template <int G>
+__device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, float4 *stg,
+                                          uint32_t *s_off, uint32_t *s_beg, float *s_dx,
+                                          float *s_dy) {
+    constexpr int LG = kWave / G;
+    const int PM1 = A.kst / 2;
     const int g = lane / LG, tl = lane % LG;
-    const double tcx = c_cx[j], tcy = c_cy[j];
-    const uint32_t a1 = off1[t], z1 = off1[t + 1], a3 = off3[t], z3 = off3[t + 1];
-    const int L = c_level[j];
-    const int p = lt->p[L];
-    const float inv_s = lt->inv_s[L];
+    const int j = __builtin_amdgcn_readfirstlane((int)A.leaf_idx[t]);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)A.c_begin[j]);
+    const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)A.c_end[j]);
+    const double tcx = A.c_cx[j], tcy = A.c_cy[j];
+    const bool own_src = (A.c_flags[j] & kActive) != 0;  // the leaf is in its own list 1
+    const uint32_t a1 = A.off1[t], z1 = A.off1[t + 1], a3 = A.off3[t], z3 = A.off3[t + 1];
+    const int L = A.c_level[j];
+    const int p = A.lt->p[L];
+    const float inv_s = A.lt->inv_s[L];
     for (uint32_t tb = b; tb < e; tb += LG) {
         const uint32_t i = tb + tl;
         const bool valid = i < e;
-        const float2 ti = valid ? oxy[i] : make_float2(0.f, 0.f);
+        const float2 ti = valid ? A.oxy[i] : make_float2(0.f, 0.f);
         // each staging round sums in FP32; the round totals add in FP64, so
         // leaves with very many sources do not lose the FP32 accumulation
         double exd = 0.0, eyd = 0.0;
-        // P2P.  Up to 64 list-1 cells at a time: lane k fetches cell k's range and
-        // centre shift (one round of loads), a wave scan gives each cell its
-        // offset in the concatenated source list, and the sources are staged
-        // through LDS in rounds of kStage, each lane locating its source's cell
-        // by binary search over the offsets.
-        for (uint32_t c0 = a1; (mask & 1) && c0 < z1; c0 += kWave) {
+        if ((A.mask & 1) && own_src) {  // P2P from the leaf's own particles
+            for (uint32_t base = b; base < e; base += kStage) {
+                const int fill = (int)min<uint32_t>((uint32_t)kStage, e - base);
+                const int fillp = (fill + G - 1) / G * G;
+                for (int r = lane; r < fillp; r += kWave) {
+                    if (r < fill) {
+                        const uint32_t src = base + (uint32_t)r;
+                        const float2 o = A.oxy[src];
+                        stg[r] = make_float4(o.x, o.y, A.qf[src], __uint_as_float(src));
+                    } else {
+                        stg[r] = far_source();
+                    }
+                }
+                wave_sync();
+                float ex = 0.f, ey = 0.f;
+                p2p_round<G, true>(stg + g, fillp / G, ti, i, ex, ey);
+                exd += (double)ex;
+                eyd += (double)ey;
+                wave_sync();
+            }
+        }
+        // P2P from the other list-1 cells.  Up to 64 cells at a time: lane k
+        // fetches cell k's range and centre shift (one round of loads), a wave
+        // scan gives each cell its offset in the concatenated source list, and
+        // the sources are staged in rounds of kStage, each lane locating its
+        // source's cell by binary search over the offsets.
+        for (uint32_t c0 = a1; (A.mask & 1) && c0 < z1; c0 += kWave) {
             const int ncell = (int)min<uint32_t>(kWave, z1 - c0);
             uint32_t cnt = 0, beg = 0;
             float dcx = 0.f, dcy = 0.f;
-            int sc = -1;
             if (lane < ncell) {
-                sc = l1[c0 + lane];
-                beg = c_begin[sc];
-                cnt = c_end[sc] - beg;
-                dcx = (float)(c_cx[sc] - tcx);
-                dcy = (float)(c_cy[sc] - tcy);
+                const int sc = A.l1[c0 + lane];
+                if (sc != j) {  // the own leaf was done above
+                    beg = A.c_begin[sc];
+                    cnt = A.c_end[sc] - beg;
+                    dcx = (float)(A.c_cx[sc] - tcx);
+                    dcy = (float)(A.c_cy[sc] - tcy);
+                }
             }
             uint32_t inc = cnt;
             for (int o = 1; o < kWave; o <<= 1) {
                 const uint32_t y = __shfl_up(inc, o, kWave);
                 if (lane >= o) inc += y;
             }
-            const uint32_t total = __shfl(inc, kWave - 1, kWave);
-            // the target leaf's own sources in this chunk: [own_lo, own_hi)
-            const unsigned long long own = __ballot(sc == j);
-            uint32_t own_lo = 0, own_hi = 0;
-            if (own) {
-                const int ol = __ffsll(own) - 1;
-                own_hi = __shfl(inc, ol, kWave);
-                own_lo = own_hi - __shfl(cnt, ol, kWave);
-            }
-            s_off[w][lane] = inc - cnt;
-            s_beg[w][lane] = beg;
-            s_dx[w][lane] = dcx;
-            s_dy[w][lane] = dcy;
-            // lanes read what other lanes of the same wavefront wrote
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            // wave-uniform: the round loops run on the scalar unit
+            const uint32_t total =
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(inc, kWave - 1, kWave));
+            s_off[lane] = inc - cnt;
+            s_beg[lane] = beg;
+            s_dx[lane] = dcx;
+            s_dy[lane] = dcy;
+            wave_sync();
             for (uint32_t base = 0; base < total; base += kStage) {
                 const int fill = (int)min<uint32_t>((uint32_t)kStage, total - base);
-                for (int r = lane; r < fill; r += kWave) {
+                const int fillp = (fill + G - 1) / G * G;
+                for (int r = lane; r < fillp; r += kWave) {
+                    if (r >= fill) {
+                        stg[r] = far_source();
+                        continue;
+                    }
                     const uint32_t gi = base + (uint32_t)r;
-                    int lo = 0, hi = ncell - 1;  // last cell whose offset is <= gi
+                    // last cell whose offset is <= gi (empty cells share the
+                    // offset of the next one and are never the last)
+                    int lo = 0, hi = ncell - 1;
                     while (lo < hi) {
                         const int mid = (lo + hi + 1) >> 1;
-                        if (s_off[w][mid] <= gi) lo = mid;
+                        if (s_off[mid] <= gi) lo = mid;
                         else hi = mid - 1;
                     }
-                    const uint32_t src = s_beg[w][lo] + (gi - s_off[w][lo]);
-                    const float2 o = oxy[src];
-                    stage[w][r] = make_float4(o.x + s_dx[w][lo], o.y + s_dy[w][lo], qf[src],
-                                              __uint_as_float(src));
+                    const uint32_t src = s_beg[lo] + (gi - s_off[lo]);
+                    const float2 o = A.oxy[src];
+                    stg[r] = make_float4(o.x + s_dx[lo], o.y + s_dy[lo], A.qf[src],
+                                         __uint_as_float(src));
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                __builtin_amdgcn_wave_barrier();
+                wave_sync();
                 float ex = 0.f, ey = 0.f;
-                // the own leaf's part of this round, [sa, sb) (empty: sa = sb)
-                const int sa = (int)min(max(own_lo, base), base + (uint32_t)fill) - (int)base;
-                const int sb = (int)min(max(own_hi, base), base + (uint32_t)fill) - (int)base;
-                p2p_run<false>(stage[w], 0, sa, g, G, ti, i, ex, ey);
-                p2p_run<true>(stage[w], sa, sb, g, G, ti, i, ex, ey);
-                p2p_run<false>(stage[w], sb, fill, g, G, ti, i, ex, ey);
+                p2p_round<G, false>(stg + g, fillp / G, ti, i, ex, ey);
                 exd += (double)ex;
                 eyd += (double)ey;
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                __builtin_amdgcn_wave_barrier();
+                wave_sync();
             }
         }
         // M2P over list 3: F = (1/s) sum a^_k xi^-(k+1), xi = (z - zc)/s
-        for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
-            const int sc = l3[v];
-            const int Ls = c_level[sc];
-            const int ps = lt->p[Ls];
-            const float is = lt->inv_s[Ls];
-            const float dcx = (float)(tcx - c_cx[sc]), dcy = (float)(tcy - c_cy[sc]);
+        for (uint32_t v = a3 + g; (A.mask & 2) && v < z3; v += G) {
+            const int sc = A.l3[v];
+            const int Ls = A.c_level[sc];
+            const int ps = A.lt->p[Ls];
+            const float is = A.lt->inv_s[Ls];
+            const float dcx = (float)(tcx - A.c_cx[sc]), dcy = (float)(tcy - A.c_cy[sc]);
             const cf wv = cinv(cf{(ti.x + dcx) * is, (ti.y + dcy) * is});
-            const size_t base = (size_t)sc * PM1;
-            cf f = ldc(ahat, base + ps);
-            for (int k = ps - 1; k >= 0; --k) f = cadd(cmul(f, wv), ldc(ahat, base + k));
+            const size_t cb = (size_t)sc * PM1;
+            cf f = ldc(A.ahat, cb + ps);
+            for (int k = ps - 1; k >= 0; --k) f = cadd(cmul(f, wv), ldc(A.ahat, cb + k));
             f = cscale(cmul(f, wv), is);
             if (valid) {
                 exd += (double)f.x;
                 eyd -= (double)f.y;
             }
         }
+#pragma unroll
         for (int m = LG; m < kWave; m <<= 1) {
             exd += __shfl_xor(exd, m, kWave);
             eyd += __shfl_xor(eyd, m, kWave);
         }
         if (valid && g == 0) {
-            if (mask & 4) {
+            if (A.mask & 4) {
                 // L2P: F = (1/s) sum_m m b^_m zeta^(m-1)
                 const cf z = {ti.x * inv_s, ti.y * inv_s};
-                const size_t base = (size_t)j * PM1;
-                cf f = cscale(ldc(bhat, base + p), (float)p);
-                for (int m = p - 1; m >= 1; --m) f = cadd(cmul(f, z), cscale(ldc(bhat, base + m), (float)m));
+                const size_t cb = (size_t)j * PM1;
+                cf f = cscale(ldc(A.bhat, cb + p), (float)p);
+                for (int m = p - 1; m >= 1; --m)
+                    f = cadd(cmul(f, z), cscale(ldc(A.bhat, cb + m), (float)m));
                 f = cscale(f, inv_s);
                 exd += (double)f.x;
                 eyd -= (double)f.y;
             }
-            const uint32_t o = perm[i];
-            if (velocity) {
-                out_a[o] = eyd + u_inf;
-                out_b[o] = -exd + v_inf;
+            const uint32_t o = A.perm[i];
+            if (A.velocity) {
+                A.out_a[o] = eyd + A.u_inf;
+                A.out_b[o] = -exd + A.v_inf;
             } else {
-                out_a[o] = exd;
-                out_b[o] = eyd;
+                A.out_a[o] = exd;
+                A.out_b[o] = eyd;
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl) {
+    __shared__ float4 stage[kEvalWaves][kStage];
+    __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
+    __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kEvalWaves + w));
+    if (t >= nl) return;
+    const int j = (int)A.leaf_idx[t];
+    const int nt = (int)(A.c_end[j] - A.c_begin[j]);
+    if (nt <= 8)
+        eval_leaf<8>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+    else if (nt <= 16)
+        eval_leaf<4>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+    else if (nt <= 32)
+        eval_leaf<2>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+    else
+        eval_leaf<1>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
 }
 
 }  // namespace
@@ -904,16 +974,16 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     T = std::min(T, K);
     const int nK = (int)(ti.lev_off[K + 1] - ti.lev_off[K]);
     if (K < maxL)
-        k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_flags, c_child,
-                                               mx.lt, mx.m2m, kst, ahat);
+        k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_flags, c_child, mx.lt,
+                                               mx.m2m, kst, ahat);
     for (int L = K - 1; L >= std::max(T, 1); --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         k_m2m_level<<<div_up(j1 - j0, kSubBlock / kWave), kSubBlock, 0, s>>>(
-            j0, j1, c_level, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
+            L, j0, j1, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
     }
     if (std::min(T, K) - 1 >= 1)
-        k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_level, c_flags, c_child,
-                                          mx.lt, mx.m2m, kst, ahat);
+        k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_flags, c_child, mx.lt,
+                                          mx.m2m, kst, ahat);
     mk.mark(3, s);
     // M2L list 2 on MFMA: every tile in one launch
     if (mx.ntiles > 0) {
@@ -939,23 +1009,23 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     if (maxL >= 3) {
         // parents 2 .. T-1 (one workgroup), T .. K-1 (a launch per level), K .. (subtrees)
         if (T - 1 >= 2)
-            k_l2l_top<<<1, kTopBlock, 0, s>>>(T - 1, c.hdr_d, c_level, c_flags, c_child, mx.lt,
-                                              mx.l2l, kst, bhat);
+            k_l2l_top<<<1, kTopBlock, 0, s>>>(T - 1, c.hdr_d, c_flags, c_child, mx.lt, mx.l2l, kst,
+                                              bhat);
         for (int L = std::max(T, 2); L <= K - 1; ++L) {
             const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
             k_l2l_level<<<div_up(j1 - j0, kSubBlock / kWave), kSubBlock, 0, s>>>(
-                j0, j1, c_level, c_flags, c_child, mx.lt, mx.l2l, kst, bhat);
+                L, j0, j1, c_flags, c_child, mx.lt, mx.l2l, kst, bhat);
         }
         if (K < maxL)
-            k_l2l_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_level, c_flags,
-                                                   c_child, mx.lt, mx.l2l, kst, bhat);
+            k_l2l_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_flags, c_child,
+                                                   mx.lt, mx.l2l, kst, bhat);
     }
     mk.mark(5, s);
-    k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
-        leaf_idx, nl, c_level, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
-        (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, oxy, qf,
-        c.perm_sorted, mx.lt, ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a,
-        out_b);
+    EvalArgs ea{leaf_idx, c_level, c_flags, c_begin, c_end, c_cx, c_cy,
+                (const uint32_t *)c.off1.p, (const int *)c.l1.p, (const uint32_t *)c.off3.p,
+                (const int *)c.l3.p, oxy, qf, c.perm_sorted, mx.lt, ahat, bhat, kst,
+                stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a, out_b};
+    k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(ea, nl);
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }

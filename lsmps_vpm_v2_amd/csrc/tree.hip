@@ -336,12 +336,18 @@ __global__ void __launch_bounds__(kRedBlock)
     uint32_t acc[kLvCols];
 #pragma unroll
     for (int c = 0; c < kLvCols; ++c) acc[c] = 0;
-    for (int blk = tid; blk < nblk; blk += kRedBlock) {
-        const ulonglong2 v = part[blk];
-        a = min(a, v.x);
-        b = max(b, v.y);
+    // four blocks per thread per pass, their loads issued together
+    for (int b0 = tid; b0 < nblk; b0 += 4 * kRedBlock) {
 #pragma unroll
-        for (int c = 0; c < kLvCols; ++c) acc[c] += lvp[(size_t)c * nblk + blk];
+        for (int u = 0; u < 4; ++u) {
+            const int blk = b0 + u * kRedBlock;
+            if (blk >= nblk) break;
+            const ulonglong2 v = part[blk];
+            a = min(a, v.x);
+            b = max(b, v.y);
+#pragma unroll
+            for (int c = 0; c < kLvCols; ++c) acc[c] += lvp[(size_t)c * nblk + blk];
+        }
     }
     for (int m = 1; m < kWave; m <<= 1) {
         a = min(a, (unsigned long long)__shfl_xor(a, m, kWave));
@@ -359,11 +365,13 @@ __global__ void __launch_bounds__(kRedBlock)
     __shared__ uint32_t scol[kLvCols];
     if (tid < kLvCols) {  // column sums over the waves, one thread per column
         uint32_t v = 0;
+#pragma unroll
         for (int u = 0; u < kRedBlock / kWave; ++u) v += sl[u][tid];
         scol[tid] = v;
     }
     __syncthreads();
     if (tid != 0) return;
+#pragma unroll
     for (int u = 1; u < kRedBlock / kWave; ++u) {
         a = min(a, smin[u]);
         b = max(b, smax[u]);
