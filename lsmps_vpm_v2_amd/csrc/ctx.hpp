@@ -45,8 +45,13 @@ struct DevHdr {
     uint32_t pad[3];
 };
 
-// Columns of the leaf split's per-block partials: cells per level 0..16, leaves.
+// Leaf-split statistics, accumulated by atomics into kSlotRows rows of one
+// cache line each (row = block mod kSlotRows): cells per level 0..16, leaves,
+// then 31 - min leaf level, max leaf level and the depth-overflow flag.
 constexpr int kLvCols = kMaxDepth + 2;
+constexpr int kSlotRows = 64;
+constexpr int kSlotStride = 32;
+constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCols + 2;
 
 // Grow-only device buffer.
 struct DBuf {
@@ -95,7 +100,10 @@ struct Ctx {
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
     DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
     DBuf tmp;  // scan / sort scratch
-    DBuf bbox_part, lvpart;
+    DBuf bbox_part, lvslot;
+    // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
+    // resolved): the previous build's choice as the first guess of the next
+    int sort_bits = 32;
 
     // cells
     DBuf pre_level, pre_slot, lev_level, lev2pre;

@@ -65,16 +65,19 @@ __device__ __forceinline__ double cell_len(double R, int level) {
 // ---------------------------------------------------------------------------
 // 1. bounding box and root geometry
 // ---------------------------------------------------------------------------
-// Block 0 also clears the device header (every later kernel of the build
-// runs after this one on the stream).
+// Block 0 also clears the device header and the split's statistic rows
+// (every later kernel of the build runs after this one on the stream).
 __global__ void __launch_bounds__(256)
-    k_bbox_partial(const double *x, const double *y, int n, double *part, DevHdr *hdr) {
+    k_bbox_partial(const double *x, const double *y, int n, double *part, DevHdr *hdr,
+                   uint32_t *lvslot) {
     __shared__ double s[4][256];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
-    if (blockIdx.x == 0)
+    if (blockIdx.x == 0) {
         for (int u = threadIdx.x; u < (int)(sizeof(DevHdr) / 4); u += blockDim.x)
             reinterpret_cast<uint32_t *>(hdr)[u] = 0u;
+        for (int u = threadIdx.x; u < kSlotRows * kSlotStride; u += blockDim.x) lvslot[u] = 0u;
+    }
     double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
     int nf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -235,22 +238,15 @@ __global__ void __launch_bounds__(256)
 // Per-block partial results (two 64-bit words per block) and their one-block
 // reduction.  Global atomics on one header word from thousands of blocks
 // serialize across the chiplets; partials plus one small reduction do not.
-enum PartOp { kPartSplit = 0, kPartM2lLists = 1, kPartLeafLists = 2, kPartLeafTerms = 3 };
+enum PartOp { kPartM2lLists = 1, kPartLeafLists = 2, kPartLeafTerms = 3 };
 
-// Block-wide (sum or min/max) of two 64-bit values; thread 0 writes part[blk].
-template <bool MINMAX>
+// Block-wide sum of two 64-bit values; thread 0 writes part[blk].
 __device__ __forceinline__ void block_part(unsigned long long a, unsigned long long b,
                                            ulonglong2 *part) {
     __shared__ unsigned long long sa[16], sb[16];
     for (int m = 1; m < kWave; m <<= 1) {
-        const unsigned long long oa = __shfl_xor(a, m, kWave), ob = __shfl_xor(b, m, kWave);
-        if (MINMAX) {
-            a = min(a, oa);
-            b = max(b, ob);
-        } else {
-            a += oa;
-            b += ob;
-        }
+        a += __shfl_xor(a, m, kWave);
+        b += __shfl_xor(b, m, kWave);
     }
     const int w = threadIdx.x / kWave, nw = (blockDim.x + kWave - 1) / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -260,13 +256,8 @@ __device__ __forceinline__ void block_part(unsigned long long a, unsigned long l
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int u = 1; u < nw; ++u) {
-            if (MINMAX) {
-                a = min(a, sa[u]);
-                b = max(b, sb[u]);
-            } else {
-                a += sa[u];
-                b += sb[u];
-            }
+            a += sa[u];
+            b += sb[u];
         }
         part[blockIdx.x] = make_ulonglong2(a, b);
     }
@@ -274,23 +265,16 @@ __device__ __forceinline__ void block_part(unsigned long long a, unsigned long l
 
 __global__ void __launch_bounds__(256)
     k_reduce_part(const ulonglong2 *part, int nblk, int op, DevHdr *hdr) {
-    const bool mm = op == kPartSplit;
-    unsigned long long a = mm ? ~0ull : 0ull, b = 0ull;
+    unsigned long long a = 0ull, b = 0ull;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
         const ulonglong2 v = part[i];
-        if (mm) {
-            a = min(a, v.x);
-            b = max(b, v.y);
-        } else {
-            a += v.x;
-            b += v.y;
-        }
+        a += v.x;
+        b += v.y;
     }
     __shared__ unsigned long long sa[4], sb[4];
     for (int m = 1; m < kWave; m <<= 1) {
-        const unsigned long long oa = __shfl_xor(a, m, kWave), ob = __shfl_xor(b, m, kWave);
-        a = mm ? min(a, oa) : a + oa;
-        b = mm ? max(b, ob) : b + ob;
+        a += __shfl_xor(a, m, kWave);
+        b += __shfl_xor(b, m, kWave);
     }
     const int w = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -300,15 +284,10 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     if (threadIdx.x != 0) return;
     for (int u = 1; u < 4; ++u) {
-        a = mm ? min(a, sa[u]) : a + sa[u];
-        b = mm ? max(b, sb[u]) : b + sb[u];
+        a += sa[u];
+        b += sb[u];
     }
-    if (op == kPartSplit) {
-        // a = min leaf level; b = max leaf level | depth-overflow bit 32
-        hdr->leaf_lvl_min = (uint32_t)a;
-        hdr->leaf_lvl_max = (uint32_t)(b & 0xffffffffull);
-        if (b >> 32) hdr->flags |= kFlagDepth;
-    } else if (op == kPartM2lLists) {
+    if (op == kPartM2lLists) {
         hdr->sum2 = (uint32_t)a;
         hdr->sum4 = (uint32_t)b;
     } else if (op == kPartLeafLists) {
@@ -321,75 +300,39 @@ __global__ void __launch_bounds__(256)
 }
 
 constexpr int kSplitBlock = 256;
-constexpr int kRedBlock = 1024;
 
-// Reduction of the leaf split's per-block results in one workgroup of 1024:
-// min / max leaf level (and the depth-overflow bit), cells per level, leaves;
-// the level offsets of the cell arrays follow (root = cell 0, then levels in
-// order).  lvp is column-major: lvp[c * nblk + block].
-__global__ void __launch_bounds__(kRedBlock)
-    k_reduce_split(const ulonglong2 *part, const uint32_t *lvp, int nblk, DevHdr *hdr) {
-    __shared__ uint32_t sl[kRedBlock / kWave][kLvCols];
-    __shared__ unsigned long long smin[kRedBlock / kWave], smax[kRedBlock / kWave];
-    const int tid = threadIdx.x, w = tid / kWave, lane = tid & (kWave - 1);
-    unsigned long long a = ~0ull, b = 0ull;
-    uint32_t acc[kLvCols];
+// The split's statistic rows into the header: min / max leaf level (and the
+// depth-overflow flag), cells per level, leaves, and the level offsets of the
+// cell arrays (root = cell 0, then the levels in order).  One wavefront, lane
+// = row.
+__global__ void __launch_bounds__(kWave) k_reduce_split(const uint32_t *lvslot, DevHdr *hdr) {
+    const int lane = threadIdx.x;
+    const uint32_t *row = lvslot + (size_t)lane * kSlotStride;
+    uint32_t v[kLvCols + 3];
 #pragma unroll
-    for (int c = 0; c < kLvCols; ++c) acc[c] = 0;
-    // four blocks per thread per pass, their loads issued together
-    for (int b0 = tid; b0 < nblk; b0 += 4 * kRedBlock) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int blk = b0 + u * kRedBlock;
-            if (blk >= nblk) break;
-            const ulonglong2 v = part[blk];
-            a = min(a, v.x);
-            b = max(b, v.y);
-#pragma unroll
-            for (int c = 0; c < kLvCols; ++c) acc[c] += lvp[(size_t)c * nblk + blk];
-        }
-    }
+    for (int c = 0; c < kLvCols + 3; ++c) v[c] = row[c];
     for (int m = 1; m < kWave; m <<= 1) {
-        a = min(a, (unsigned long long)__shfl_xor(a, m, kWave));
-        b = max(b, (unsigned long long)__shfl_xor(b, m, kWave));
 #pragma unroll
-        for (int c = 0; c < kLvCols; ++c) acc[c] += __shfl_xor(acc[c], m, kWave);
+        for (int c = 0; c < kLvCols; ++c) v[c] += __shfl_xor(v[c], m, kWave);
+        v[kSlotMinInv] = max(v[kSlotMinInv], (uint32_t)__shfl_xor(v[kSlotMinInv], m, kWave));
+        v[kSlotMax] = max(v[kSlotMax], (uint32_t)__shfl_xor(v[kSlotMax], m, kWave));
+        v[kSlotDepth] |= (uint32_t)__shfl_xor(v[kSlotDepth], m, kWave);
     }
-    if (lane == 0) {
-        smin[w] = a;
-        smax[w] = b;
-#pragma unroll
-        for (int c = 0; c < kLvCols; ++c) sl[w][c] = acc[c];
-    }
-    __syncthreads();
-    __shared__ uint32_t scol[kLvCols];
-    if (tid < kLvCols) {  // column sums over the waves, one thread per column
-        uint32_t v = 0;
-#pragma unroll
-        for (int u = 0; u < kRedBlock / kWave; ++u) v += sl[u][tid];
-        scol[tid] = v;
-    }
-    __syncthreads();
-    if (tid != 0) return;
-#pragma unroll
-    for (int u = 1; u < kRedBlock / kWave; ++u) {
-        a = min(a, smin[u]);
-        b = max(b, smax[u]);
-    }
-    // a = min leaf level; b = max leaf level | depth-overflow bit 32
-    hdr->leaf_lvl_min = (uint32_t)a;
-    hdr->leaf_lvl_max = (uint32_t)(b & 0xffffffffull);
-    if (b >> 32) hdr->flags |= kFlagDepth;
+    if (lane != 0) return;
+    hdr->leaf_lvl_min = 31u - v[kSlotMinInv];
+    hdr->leaf_lvl_max = v[kSlotMax];
+    if (v[kSlotDepth]) hdr->flags |= kFlagDepth;
     uint32_t off = 1;  // the root is cell 0
     hdr->lev_off[0] = 0;
     hdr->lev_cnt[0] = 1;
+#pragma unroll
     for (int L = 1; L <= kMaxDepth + 1; ++L) {
-        const uint32_t cnt = L <= kMaxDepth ? scol[L] : 0u;
+        const uint32_t cnt = L <= kMaxDepth ? v[L] : 0u;
         hdr->lev_off[L] = off;
         hdr->lev_cnt[L] = cnt;
         off += cnt;
     }
-    hdr->n_leaves_split = scol[kLvCols - 1];
+    hdr->n_leaves_split = v[kLvCols - 1];
 }
 
 // Sliding-window maxima by the van Herk / Gil-Werman method: with blocks of
@@ -431,8 +374,7 @@ __device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *&
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, ulonglong2 *part,
-            uint32_t *lvpart) {
+            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, uint32_t *lvslot) {
     extern __shared__ uint8_t lds[];
     // cells started per level (a particle whose lcp with its predecessor is d
     // starts the cells of levels d+1 .. its leaf level, the last a leaf)
@@ -465,7 +407,7 @@ __global__ void __launch_bounds__(kSplitBlock)
     block_prefix_suffix(sw, gw, hw, gw2, hw2, wlen, W + 1);
     if (S > 0) block_prefix_suffix(sv, gv, hv, gv2, hv2, vlen, S);
     __syncthreads();
-    unsigned long long my_min = kMaxDepth + 1, my_max = 0;
+    int my_min = kMaxDepth + 1, my_max = 0, my_depth = 0;
     if (i < n) {
         const uint32_t ki = key[i];
         const int t = threadIdx.x;
@@ -488,7 +430,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         int lvl = max(max(par_split, src_split) + 1, max(min_level, 1));
         if (max_level > 0) lvl = min(lvl, max_level);
         if (lvl > kMaxDepth) {
-            my_max |= 1ull << 32;  // depth overflow
+            my_depth = 1;  // depth overflow
             lvl = kMaxDepth;
         }
         leaf_lvl[i] = (uint8_t)lvl;
@@ -496,11 +438,38 @@ __global__ void __launch_bounds__(kSplitBlock)
         ccount[i] = lvl > di ? (uint32_t)(lvl - di) : 0u;
         for (int L = di + 1; L <= lvl; ++L) atomicAdd(&lvc[L], 1u);
         if (lvl > di) atomicAdd(&lvc[kLvCols - 1], 1u);
-        my_min = (unsigned long long)lvl;
-        my_max |= (unsigned long long)lvl;
+        my_min = lvl;
+        my_max = lvl;
     }
-    block_part<true>(my_min, my_max, part);  // synchronises the block
-    if (threadIdx.x < kLvCols) lvpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = lvc[threadIdx.x];
+    // block min / max / overflow, then one row of atomics per block
+    for (int m = 1; m < kWave; m <<= 1) {
+        my_min = min(my_min, __shfl_xor(my_min, m, kWave));
+        my_max = max(my_max, __shfl_xor(my_max, m, kWave));
+        my_depth |= __shfl_xor(my_depth, m, kWave);
+    }
+    __shared__ int s_mm[kSplitBlock / kWave][3];
+    const int wv = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_mm[wv][0] = my_min;
+        s_mm[wv][1] = my_max;
+        s_mm[wv][2] = my_depth;
+    }
+    __syncthreads();
+    uint32_t *row = lvslot + (size_t)(blockIdx.x % kSlotRows) * kSlotStride;
+    if (threadIdx.x < kLvCols) {
+        const uint32_t c = lvc[threadIdx.x];
+        if (c) atomicAdd(row + threadIdx.x, c);
+    } else if (threadIdx.x == kLvCols) {
+        int mn = s_mm[0][0], mx = s_mm[0][1], dp = s_mm[0][2];
+        for (int u = 1; u < kSplitBlock / kWave; ++u) {
+            mn = min(mn, s_mm[u][0]);
+            mx = max(mx, s_mm[u][1]);
+            dp |= s_mm[u][2];
+        }
+        atomicMax(row + kSlotMinInv, 31u - (uint32_t)mn);
+        atomicMax(row + kSlotMax, (uint32_t)mx);
+        if (dp) atomicOr(row + kSlotDepth, 1u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -796,7 +765,7 @@ __global__ void __launch_bounds__(256)
             n4[j] = (uint8_t)tot4;
         }
     }
-    block_part<false>((unsigned long long)(live ? m2 : 0), (unsigned long long)(live ? m4 : 0),
+    block_part((unsigned long long)(live ? m2 : 0), (unsigned long long)(live ? m4 : 0),
                       part);
 }
 
@@ -949,9 +918,9 @@ __global__ void __launch_bounds__(256)
     }
     if (overflow) atomicOr(&hdr_w->flags, kFlagListOverflow);
     if (FILL) {
-        block_part<false>(my_pairs, my_m2p, part);
+        block_part(my_pairs, my_m2p, part);
         __syncthreads();  // block_part's LDS is reused
-        block_part<false>(my_m2p_terms, my_l2p_terms, part_terms);
+        block_part(my_m2p_terms, my_l2p_terms, part_terms);
     }
 }
 
@@ -973,6 +942,10 @@ static void check_flags(const DevHdr &h) {
     if (h.flags & kFlagListOverflow) throw Error(kInternal, "list traversal stack overflow");
 }
 
+// Key bits sorted for a tree of the given depth: the levels down to the
+// deepest leaf, in whole 8-bit radix passes.
+static int canonical_sort_bits(int depth) { return std::max(8, 8 * div_up(2LL * depth, 8)); }
+
 void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &src,
                 const uint8_t *active, const TreeParams &tp) {
     hipStream_t s = c.stream;
@@ -981,7 +954,8 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     // 1. bounding box (the first kernel also clears the header)
     const int nbb = std::min(1024, div_up(n, 256 * 8));
     double *part = c.bbox_part.as<double>(5 * (size_t)nbb);
-    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d);
+    uint32_t *lvslot = c.lvslot.as<uint32_t>((size_t)kSlotRows * kSlotStride);
+    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d, lvslot);
     const double expansion = 1.0 + (tp.exp_tree / 100.0);
     k_bbox_final<<<1, 256, 0, s>>>(part, nbb, expansion, c.hdr_d);
     VPM_LAUNCH_CHECK();
@@ -993,7 +967,10 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
         std::max(radix_tmp_words(n), scan_tmp_words((long long)n + 1) + 64) + 64;
     uint32_t *tmp = c.tmp.as<uint32_t>(std::max(tmp_words, radix_tmp_words(16LL * n + 16)));
     k_keys<<<div_up(n, 256), 256, 0, s>>>(x, y, n, c.hdr_d, ka, va);
-    const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 0, 32, tmp, s);
+    // the top sort_bits of the key (levels <= sort_bits / 2 resolved), a guess
+    // from the previous build checked after the split
+    const int bits = c.sort_bits;
+    const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 32 - bits, 32, tmp, s);
     uint32_t *key = alt ? kb : ka;
     uint32_t *perm = alt ? vb : va;
     c.key_sorted = key;
@@ -1016,18 +993,37 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
     const size_t lds = 5 * (size_t)(W + kSplitBlock) + 5 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
-    ulonglong2 *spart = c.part.as<ulonglong2>(nbs);
-    uint32_t *lvp = c.lvpart.as<uint32_t>((size_t)nbs * kLvCols);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
-                                          tp.max_level, leaf_lvl, cnt, spart, lvp);
-    k_reduce_split<<<1, kRedBlock, 0, s>>>(spart, lvp, nbs, c.hdr_d);
+                                          tp.max_level, leaf_lvl, cnt, lvslot);
+    k_reduce_split<<<1, kWave, 0, s>>>(lvslot, c.hdr_d);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
     scan_exclusive_u32(cnt, base, n, &c.hdr_d->n_cells_pre, true, tmp, s);
     VPM_LAUNCH_CHECK();
     // the one synchronisation of the build: sizes of every later launch
     sync_hdr(c);
     const DevHdr &hh = *c.hdr_h;
-    check_flags(hh);
+    if (hh.flags & (kFlagNonFinite | kFlagRange)) check_flags(hh);
+    // A split on the sorted top bits is exact while its deepest leaf level is
+    // at most bits / 2: a window's lcp is taken from its two end keys, which
+    // bounds the whole window only where the keys are sorted, so every lcp
+    // below bits / 2 is exact and one at or beyond it (possibly overstated,
+    // up to a false depth overflow) puts a leaf deeper than bits / 2.  Such a
+    // build is repeated with the full key, whose depth is exact.  The sorted
+    // bit count is a function of the depth alone (canonical_sort_bits), so the
+    // particle order inside a leaf, and with it every rounding, depends on the
+    // input only; a wrong guess builds again (at most twice).
+    {
+        const int depth = (int)hh.leaf_lvl_max;
+        const bool partial_too_shallow =
+            bits < 32 && ((hh.flags & kFlagDepth) || 2 * depth > bits);
+        const int want = partial_too_shallow ? 32 : canonical_sort_bits(depth);
+        if (!partial_too_shallow) check_flags(hh);  // a depth overflow of the full key is real
+        if (want != bits) {
+            c.sort_bits = want;
+            tree_build(c, n, x, y, src, active, tp);
+            return;
+        }
+    }
 
     // 5. cells
     const int nc = 1 + (int)hh.n_cells_pre;

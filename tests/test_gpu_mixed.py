@@ -90,3 +90,32 @@ def test_per_level_orders(gpu, oracle):
     s = FmmSolver(p=8, precision="mixed")
     with pytest.raises(VpmFmmError):
         s.set_level_orders([17])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_results_do_not_depend_on_call_history(gpu, precision):
+    """The particle sort covers the key bits the tree's depth needs, guessed
+    from the previous call and checked after the leaf split; the bit count is
+    a function of the depth alone, so the order of particles inside a leaf
+    (and every rounding) depends on the input only.  Fresh solvers, a solver
+    whose previous tree was deeper or shallower, and repeated calls agree bit
+    for bit (uniform field: depth 5, 16 key bits; field with a tight cluster:
+    depth 13, 32 key bits)."""
+    rng = np.random.default_rng(64)
+    n = 20000
+    uni = (rng.random(n), rng.random(n), rng.standard_normal(n), np.ones(n, np.uint8))
+    h = n // 2
+    deep = (np.concatenate([rng.random(h), 0.3 + 1e-3 * rng.standard_normal(h)]),
+            np.concatenate([rng.random(h), 0.6 + 1e-3 * rng.standard_normal(h)]),
+            rng.standard_normal(n), np.ones(n, np.uint8))
+    kw = dict(p=8, precision=precision, binom_mode=0, list4_mode=2)
+    fu, fd = FmmSolver(**kw), FmmSolver(**kw)
+    eu, ed = fu.field(*uni), fd.field(*deep)
+    du, dd = fu.stats()["max_leaf_level"], fd.stats()["max_leaf_level"]
+    print(f"{precision}: depth uniform {du}, clustered {dd}")
+    assert du <= 8 and dd >= 13  # 16 and 32 sorted key bits
+    s = FmmSolver(**kw)
+    runs = [(deep, ed), (uni, eu), (uni, eu), (deep, ed), (deep, ed)]
+    for k, (f, ref) in enumerate(runs):
+        e = s.field(*f)
+        assert np.array_equal(e[0], ref[0]) and np.array_equal(e[1], ref[1]), k
