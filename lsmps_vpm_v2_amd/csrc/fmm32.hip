@@ -369,36 +369,44 @@ __global__ void __launch_bounds__(kSubBlock)
 }
 
 // M2L over list 2 on MFMA.  Tile = 16 consecutive cells of one level (all of
-// order P).  Lane l: column n = l % 16 (target), k-quarter kq = l / 16.  The
-// realified translation D[2(m-1)+{0,1}][2k+{0,1}] of each of the 40 offsets
-// (|dx|,|dy| <= 3, not both <= 1) is the MFMA A operand, read from a table in
-// lane order; the B operand is the source cell's scaled multipole (a^_k re/im
-// along K = 2(P+1)).  An offset no target of the tile uses is skipped.
-// One tile (wavefront-local: the source map is private to the wavefront, so
-// the tiles of one workgroup may have different orders).
+// order P), one workgroup of kM2lWaves wavefronts per tile.  Lane l: column
+// n = l % 16 (target), k-quarter kq = l / 16.  The realified translation
+// D[2(m-1)+{0,1}][2k+{0,1}] of each of the 40 offsets (|dx|,|dy| <= 3, not
+// both <= 1) is the MFMA A operand, read from a table in lane order; the B
+// operand is the source cell's scaled multipole (a^_k re/im along
+// K = 2(P+1)).  Wavefront w takes the offsets oc = w, w + kM2lWaves, ... (an
+// offset no target of the tile uses is skipped), so the chain of source
+// gathers per wavefront is kM2lWaves times shorter; the partial results are
+// summed through LDS.
+constexpr int kM2lWaves = 4;
+
 template <int P>
-__device__ __forceinline__ void m2l_tile(int j0, int cnt, int lane, int (*smap)[49],
-                                         const uint32_t *c_key, const int *l2, const uint8_t *n2,
-                                         const float *R, const float *ahat, int kst,
-                                         float *bhat) {
+__device__ __forceinline__ void m2l_tile(int j0, int cnt, int w, int lane, int (*smap)[49],
+                                         float *red, const uint32_t *c_key, const int *l2,
+                                         const uint8_t *n2, const float *R, const float *ahat,
+                                         int kst, float *bhat) {
     constexpr int K = 2 * (P + 1), KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
     const int n = lane % 16, kq = lane / 16;
     const int j = j0 + n;
-    // source map of this tile: slot (dx+3) + 7 (dy+3) -> source cell or -1
-    for (int s = kq; s < 49; s += 4) smap[n][s] = -1;
-    wave_sync();
-    if (n < cnt) {
-        const uint32_t kj = c_key[j];
-        const int jx = (int)compact1by1(kj), jy = (int)compact1by1(kj >> 1);
-        const int m2 = n2[j];
-        for (int u = kq; u < m2; u += 4) {
-            const int sidx = l2[27 * j + u];
-            const uint32_t ks = c_key[sidx];
-            const int dx = (int)compact1by1(ks) - jx, dy = (int)compact1by1(ks >> 1) - jy;
-            smap[n][(dx + 3) + 7 * (dy + 3)] = sidx;
+    // source map of the tile: slot (dx+3) + 7 (dy+3) -> source cell or -1
+    for (int t = threadIdx.x; t < 16 * 49; t += blockDim.x) smap[t / 49][t % 49] = -1;
+    __syncthreads();
+    {
+        const int tn = threadIdx.x % 16, tu = threadIdx.x / 16;  // 16 entries of list 2 at a time
+        const int tj = j0 + tn;
+        if (tn < cnt) {
+            const uint32_t kj = c_key[tj];
+            const int jx = (int)compact1by1(kj), jy = (int)compact1by1(kj >> 1);
+            const int m2 = n2[tj];
+            for (int u = tu; u < m2; u += (int)blockDim.x / 16) {
+                const int sidx = l2[27 * tj + u];
+                const uint32_t ks = c_key[sidx];
+                const int dx = (int)compact1by1(ks) - jx, dy = (int)compact1by1(ks >> 1) - jy;
+                smap[tn][(dx + 3) + 7 * (dy + 3)] = sidx;
+            }
         }
     }
-    wave_sync();
+    __syncthreads();
     v4f acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -406,8 +414,9 @@ __device__ __forceinline__ void m2l_tile(int j0, int cnt, int lane, int (*smap)[
     for (int slot = 0; slot < 49; ++slot) {
         const int dx = slot % 7 - 3, dy = slot / 7 - 3;
         if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
-        const int sidx = smap[n][slot];
         const int oc = o++;
+        if (oc % kM2lWaves != w) continue;
+        const int sidx = smap[n][slot];
         if (__ballot(sidx >= 0) == 0) continue;  // no target of the tile uses this offset
         const float *src = ahat + (size_t)(sidx < 0 ? 0 : sidx) * kst;
         float b[KS];
@@ -424,12 +433,22 @@ __device__ __forceinline__ void m2l_tile(int j0, int cnt, int lane, int (*smap)[
                 acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ro[(ks * MT + mt) * kWave], b[ks],
                                                                acc[mt], 0, 0, 0);
     }
+    // sum of the wavefronts' partial results: red[w][(mt * 4 + v) * 64 + lane]
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[(w * 8 + mt * 4 + v) * kWave + lane] = acc[mt][v];
+    __syncthreads();
+    if (w != 0) return;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+            float sum = 0.f;
+#pragma unroll
+            for (int u = 0; u < kM2lWaves; ++u) sum += red[(u * 8 + mt * 4 + v) * kWave + lane];
             const int row = mt * 16 + 4 * kq + v;  // row of (Re b^_1, Im b^_1, Re b^_2, ...)
-            if (n < cnt && row < 2 * P) bhat[(size_t)j * kst + 2 + row] = acc[mt][v];
+            if (n < cnt && row < 2 * P) bhat[(size_t)j * kst + 2 + row] = sum;
         }
     }
 }
@@ -439,21 +458,23 @@ struct M2lTabs {
     const float *r[17];
 };
 
-// All tiles of every order in one launch: a wavefront per tile, dispatched on
-// the tile's order (tiles are grouped by order, so neighbours mostly agree).
-__global__ void __launch_bounds__(256)
+// All tiles of every order in one launch: a workgroup per tile, dispatched on
+// the tile's order (tiles are grouped by order).
+__global__ void __launch_bounds__(kWave *kM2lWaves)
     k_m2l_mfma(const int2 *tiles, const uint8_t *tile_p, int ntiles, const uint32_t *c_key,
                const int *l2, const uint8_t *n2, M2lTabs tabs, const float *ahat, int kst,
                float *bhat) {
-    __shared__ int smap[4][16][49];
+    __shared__ int smap[16][49];
+    __shared__ float red[kM2lWaves * 8 * kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int tile = blockIdx.x * 4 + w;
+    const int tile = blockIdx.x;
     if (tile >= ntiles) return;
     const int2 td = tiles[tile];
     const int p = tile_p[tile];
 #define M2L_CASE(P)                                                                          \
     case P:                                                                                  \
-        m2l_tile<P>(td.x, td.y, lane, smap[w], c_key, l2, n2, tabs.r[P], ahat, kst, bhat);    \
+        m2l_tile<P>(td.x, td.y, w, lane, smap, red, c_key, l2, n2, tabs.r[P], ahat, kst,      \
+                    bhat);                                                                   \
         break;
     switch (p) {
         M2L_CASE(1) M2L_CASE(2) M2L_CASE(3) M2L_CASE(4) M2L_CASE(5) M2L_CASE(6) M2L_CASE(7)
@@ -989,7 +1010,7 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     if (mx.ntiles > 0) {
         M2lTabs tabs;
         for (int p = 0; p <= 16; ++p) tabs.r[p] = mx.rptr_host[p];
-        k_m2l_mfma<<<div_up(mx.ntiles, 4), 256, 0, s>>>(mx.tiles, mx.tile_p, mx.ntiles, c_key,
+        k_m2l_mfma<<<mx.ntiles, kWave * kM2lWaves, 0, s>>>(mx.tiles, mx.tile_p, mx.ntiles, c_key,
                                                          (const int *)c.l2.p,
                                                          (const uint8_t *)c.n2.p, tabs, ahat, kst,
                                                          bhat);

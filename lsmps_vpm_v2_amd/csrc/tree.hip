@@ -335,38 +335,31 @@ __global__ void __launch_bounds__(kWave) k_reduce_split(const uint32_t *lvslot, 
     hdr->n_leaves_split = v[kLvCols - 1];
 }
 
-// Sliding-window maxima by the van Herk / Gil-Werman method: with blocks of
-// length B aligned at 0, the max over [t, t + B - 1] is max(h[t], g[t + B - 1])
-// where g / h are block-prefix / block-suffix maxima.  Entries outside the
-// valid range hold 0, which never raises a maximum of valid lcp values.  The
-// prefix / suffix maxima are segmented Hillis-Steele scans over the whole
-// workgroup (log2 B steps, ping-pong buffers); g and h return pointing at the
-// buffers that hold the results.
-__device__ __forceinline__ void block_prefix_suffix(const uint8_t *a, uint8_t *&g, uint8_t *&h,
-                                                    uint8_t *g2, uint8_t *h2, int len, int B) {
-    for (int x = threadIdx.x; x < len; x += blockDim.x) {
-        g[x] = a[x];
-        h[x] = a[x];
-    }
-    __syncthreads();
-    for (int dd = 1; dd < B; dd <<= 1) {
-        for (int x = threadIdx.x; x < len; x += blockDim.x) {
-            const int off = x % B;  // position inside its block
-            uint8_t gv = g[x], hv = h[x];
-            if (off >= dd) gv = max(gv, g[x - dd]);
-            if (off + dd < B && x + dd < len) hv = max(hv, h[x + dd]);
-            g2[x] = gv;
-            h2[x] = hv;
-        }
+// Sliding-window maxima by doubling (a sparse table): after step s the
+// buffer holds the maximum over [x, x + 2^s); a window of length B, with
+// 2^k <= B < 2^(k+1), is the larger of the two 2^k windows at its ends.
+// Entries outside the valid range hold 0, which never raises a maximum of
+// valid lcp values.  The source stays intact: the steps ping-pong between b
+// and c; returns the buffer holding the 2^k maxima.  A query's window lies
+// inside [0, len), so the entries the doubling leaves partial (x + 2^s
+// beyond len) are never read.
+__device__ __forceinline__ const uint8_t *block_window_max(const uint8_t *src, uint8_t *b,
+                                                           uint8_t *c, int len, int k) {
+    const uint8_t *a = src;
+    for (int st = 0; st < k; ++st) {
+        const int sft = 1 << st;
+        for (int x = threadIdx.x; x < len; x += blockDim.x)
+            b[x] = (x + sft < len) ? max(a[x], a[x + sft]) : a[x];
         __syncthreads();
-        uint8_t *t = g;
-        g = g2;
-        g2 = t;
-        t = h;
-        h = h2;
-        h2 = t;
+        a = b;
+        b = c;
+        c = const_cast<uint8_t *>(a);
     }
+    return a;
 }
+
+// floor(log2(v)) for v >= 1
+__device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 
 // Per particle: the deepest level whose cell is over the particle (W) or
 // source (S) threshold, hence the leaf level; and the number of cells whose
@@ -388,7 +381,7 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int wlen = W + kSplitBlock;
     const int wbase = i0 - W;
     const int wlast = n - 1 - W;  // last valid run start (< 0: no run of W+1 particles)
-    uint8_t *sw = lds, *gw = sw + wlen, *hw = gw + wlen, *gw2 = hw + wlen, *hw2 = gw2 + wlen;
+    uint8_t *sw = lds, *gw = sw + wlen, *hw = gw + wlen;
     for (int t = threadIdx.x; t < wlen; t += kSplitBlock) {
         const int j = wbase + t;
         sw[t] = (j >= 0 && j <= wlast) ? w[j] : 0;
@@ -398,26 +391,25 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int vbase = r_lo - S - 1;
     const int vlast = na - 1 - S;
     const int vlen = r_hi - vbase + 1;
-    uint8_t *sv = hw2 + wlen, *gv = sv + vlen, *hv = gv + vlen, *gv2 = hv + vlen, *hv2 = gv2 + vlen;
+    uint8_t *sv = hw + wlen, *gv = sv + vlen, *hv = gv + vlen;
     for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
         const int m = vbase + t;
         sv[t] = (m >= 0 && m <= vlast) ? v[m] : 0;
     }
     __syncthreads();
-    block_prefix_suffix(sw, gw, hw, gw2, hw2, wlen, W + 1);
-    if (S > 0) block_prefix_suffix(sv, gv, hv, gv2, hv2, vlen, S);
-    __syncthreads();
+    const int kw = ilog2(W + 1), ks = S > 0 ? ilog2(S) : 0;
+    const uint8_t *mw = block_window_max(sw, gw, hw, wlen, kw);
+    const uint8_t *mv = S > 0 ? block_window_max(sv, gv, hv, vlen, ks) : sv;
     int my_min = kMaxDepth + 1, my_max = 0, my_depth = 0;
     if (i < n) {
         const uint32_t ki = key[i];
         const int t = threadIdx.x;
-        int par_split = wlast >= 0 ? (int)max(hw[t], gw[t + W]) : -1;
-        int src_split = -1;
+        int par_split = -1, src_split = -1;
         const int r = (int)A[i];
-        if (S > 0) {
-            const int a = r - S - vbase;  // window [r - S, r - 1] in LDS coordinates
-            src_split = (int)max(hv[a], gv[a + S - 1]);
-        }
+        const int a = r - S - vbase;  // active window [r - S, r - 1] in LDS coordinates
+        // window [t, t + W] and active window [a, a + S - 1] from their end 2^k maxima
+        if (wlast >= 0) par_split = (int)max(mw[t], mw[t + W + 1 - (1 << kw)]);
+        if (S > 0) src_split = (int)max(mv[a], mv[a + S - (1 << ks)]);
         if (r <= vlast) {  // run starting at the first active at or after i
             const int e = min((int)sv[r - vbase], lcp_levels(ki, key[apos[r]]));
             src_split = max(src_split, e);
@@ -991,7 +983,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     k_lcp_v<<<div_up(n, 256), 256, 0, s>>>(key, apos, c.hdr_d, S, n, v);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
-    const size_t lds = 5 * (size_t)(W + kSplitBlock) + 5 * (size_t)(S + kSplitBlock + 2) + 16;
+    const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
                                           tp.max_level, leaf_lvl, cnt, lvslot);
