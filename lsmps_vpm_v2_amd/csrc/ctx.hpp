@@ -101,6 +101,7 @@ struct Ctx {
     DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
     DBuf tmp;  // scan / sort scratch
     DBuf bbox_part, lvslot;
+    DBuf bs_act, bs_cells, bs_leaf, bs_l13;  // producer block sums of scan inputs
     // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
     // resolved): the previous build's choice as the first guess of the next
     int sort_bits = 32;

@@ -286,7 +286,9 @@ __global__ void __launch_bounds__(kSubBlock)
     }
 }
 
-// M2M for the parents of levels Lhi .. 1 (top levels, few cells): one workgroup.
+// M2M for the parents of levels Lhi .. 2 (top levels, few cells): one workgroup.
+// Multipoles of levels 0 and 1 are never read (list-2 and list-3 cells are of
+// level >= 2), so the upward pass stops at level 2.
 __global__ void __launch_bounds__(kTopBlock)
     k_m2m_top(int Lhi, const DevHdr *hdr, const uint8_t *c_flags, const int *c_child,
               const LevelTab *lt, const float *m2m, int kst, float *ahat) {
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(kTopBlock)
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     load_tab(m2m, PM1, tab, lt, s_lp);
-    for (int L = Lhi; L >= 1; --L) {
+    for (int L = Lhi; L >= 2; --L) {
         for (int j = s_off[L] + w; j < s_off[L + 1]; j += kTopBlock / kWave)
             m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
         __syncthreads();
@@ -997,12 +999,12 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     if (K < maxL)
         k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_flags, c_child, mx.lt,
                                                mx.m2m, kst, ahat);
-    for (int L = K - 1; L >= std::max(T, 1); --L) {
+    for (int L = K - 1; L >= std::max(T, 2); --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         k_m2m_level<<<div_up(j1 - j0, kSubBlock / kWave), kSubBlock, 0, s>>>(
             L, j0, j1, c_flags, c_child, mx.lt, mx.m2m, kst, ahat);
     }
-    if (std::min(T, K) - 1 >= 1)
+    if (std::min(T, K) - 1 >= 2)
         k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_flags, c_child, mx.lt,
                                           mx.m2m, kst, ahat);
     mk.mark(3, s);

@@ -3,9 +3,14 @@
 // Scans: tile reduce -> down-sweep over 2048-element tiles (256 threads x
 // 8 items), wave64 shuffles inside a tile; the down-sweep block sums the tile
 // totals before it itself (up to kFusedSpine tiles), larger scans add a spine
-// launch.  Radix sort: 8-bit digits over 1024-element tiles; per-block digit histograms, a scan of the digit-major
-// histogram table, then a stable scatter whose in-tile ranks come from
-// 64-lane ballot "peer masks" (lanes of a wavefront holding the same digit).
+// launch.  When the kernel that produced the input already wrote the sums of
+// its runs of `per` elements (block sums), the reduce pass is skipped and the
+// scan is one launch.  Radix sort: 8-bit digits over 1024-element tiles;
+// per-block digit histograms, one launch that scans each digit's row of the
+// digit-major histogram table, then a stable scatter that adds the digit
+// bases (a block scan of the 256 digit totals) and whose in-tile ranks come
+// from 64-lane ballot "peer masks" (lanes of a wavefront holding the same
+// digit).
 #include "prim.hpp"
 
 namespace vpm {
@@ -136,6 +141,42 @@ __global__ void __launch_bounds__(kScanBlock)
     }
 }
 
+// Down-sweep from block sums of the producer: bsum[k] is the sum of
+// in[k per, (k+1) per); tile b (kScanTile elements, a multiple of per) adds
+// pre(b) = bsum[0 .. b kScanTile / per), summed by the block itself, to its
+// own scan.
+template <typename T>
+__global__ void __launch_bounds__(kScanBlock)
+    k_scan_down_bs(const T *in, uint32_t *out, int n, const uint32_t *bsum, int per, int nbs,
+                   uint32_t *total, int write_end) {
+    __shared__ uint32_t sm[kScanBlock / kWave];
+    const int kend = (int)min((long long)nbs, (long long)blockIdx.x * kScanTile / per);
+    uint32_t pre = 0, pre_all;
+    for (int k = threadIdx.x; k < kend; k += kScanBlock) pre += bsum[k];
+    block_excl_scan<kScanBlock>(pre, sm, pre_all);
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        v[j] = i < n ? (uint32_t)in[i] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t run = pre_all + block_excl_scan<kScanBlock>(s, sm, tot);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        if (total) *total = pre_all + tot;
+        if (write_end) out[n] = pre_all + tot;
+    }
+}
+
 __global__ void k_write_zero_total(uint32_t *out, int n, uint32_t *total, int write_end) {
     if (total) *total = 0;
     if (write_end) out[n] = 0;
@@ -163,6 +204,34 @@ void scan_impl(const T *in, uint32_t *out, int n, uint32_t *total, bool write_en
     VPM_LAUNCH_CHECK();
 }
 
+// Exclusive scan of each digit's row of the histogram table (block = digit),
+// in place; the digit's total goes to totals[digit].
+__global__ void __launch_bounds__(kRsBlock)
+    k_rs_rowscan(uint32_t *table, int nb, uint32_t *totals) {
+    __shared__ uint32_t sm[kRsBlock / kWave];
+    uint32_t *row = table + (size_t)blockIdx.x * nb;
+    constexpr int kItems = 4;
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kRsBlock * kItems) {
+        const int i0 = b0 + threadIdx.x * kItems;
+        uint32_t v[kItems], sum = 0;
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            v[u] = i0 + u < nb ? row[i0 + u] : 0u;
+            sum += v[u];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_excl_scan<kRsBlock>(sum, sm, tot);
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            if (i0 + u < nb) row[i0 + u] = run;
+            run += v[u];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 __global__ void __launch_bounds__(kRsBlock)
     k_rs_hist(const uint32_t *keys, int n, int shift, uint32_t *table, int nb) {
     __shared__ uint32_t h[256];
@@ -182,12 +251,16 @@ __global__ void __launch_bounds__(kRsBlock)
 // which is the index order; ranks are built in that order.
 __global__ void __launch_bounds__(kRsBlock)
     k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int n,
-                 int shift, const uint32_t *table, int nb) {
+                 int shift, const uint32_t *table, const uint32_t *totals, int nb) {
     constexpr int kW = kRsBlock / kWave;
     __shared__ uint32_t goff[256];
     __shared__ uint32_t wcnt[kW][256];
+    __shared__ uint32_t sm[kRsBlock / kWave];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    goff[t] = table[(size_t)t * nb + blockIdx.x];
+    // digit base (exclusive scan of the digit totals) + this tile's row prefix
+    uint32_t all;
+    const uint32_t dbase = block_excl_scan<kRsBlock>(totals[t], sm, all);
+    goff[t] = dbase + table[(size_t)t * nb + blockIdx.x];
 #pragma unroll
     for (int ww = 0; ww < kW; ++ww) wcnt[ww][t] = 0;
     __syncthreads();
@@ -245,9 +318,41 @@ void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
     scan_impl<uint8_t>(in, out, n, total, write_total_at_end, tmp, s);
 }
 
+template <typename T>
+void scan_bs_impl(const T *in, uint32_t *out, int n, uint32_t *bsum, int per, uint32_t *total,
+                  bool write_end, uint32_t *tmp, hipStream_t s) {
+    if (n <= 0) {
+        k_write_zero_total<<<1, 1, 0, s>>>(out, 0, total, write_end ? 1 : 0);
+        VPM_LAUNCH_CHECK();
+        return;
+    }
+    if (per <= 0 || kScanTile % per != 0) throw Error(kInternal, "scan block sums: bad run length");
+    const int nb = div_up(n, kScanTile), nbs = div_up(n, per);
+    if (nbs > kFusedSpine * 4) {
+        // very long inputs: the tile reduce (input read once more) beats a
+        // one-block spine over this many block sums
+        scan_impl<T>(in, out, n, total, write_end, tmp, s);
+        return;
+    }
+    k_scan_down_bs<T><<<nb, kScanBlock, 0, s>>>(in, out, n, bsum, per, nbs, total,
+                                                write_end ? 1 : 0);
+    VPM_LAUNCH_CHECK();
+}
+
+void scan_exclusive_bs_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
+                           uint32_t *total, bool write_total_at_end, uint32_t *tmp,
+                           hipStream_t s) {
+    scan_bs_impl<uint32_t>(in, out, n, bsum, per, total, write_total_at_end, tmp, s);
+}
+
+void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
+                          uint32_t *total, bool write_total_at_end, uint32_t *tmp, hipStream_t s) {
+    scan_bs_impl<uint8_t>(in, out, n, bsum, per, total, write_total_at_end, tmp, s);
+}
+
 size_t radix_tmp_words(long long n) {
     const long long nb = div_up(n > 0 ? n : 1, kRsTile);
-    return (size_t)(256 * nb) + scan_tmp_words(256 * nb);
+    return (size_t)(256 * nb) + 256;
 }
 
 bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
@@ -255,7 +360,7 @@ bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
     if (n <= 1) return false;
     const int nb = div_up(n, kRsTile);
     uint32_t *table = tmp;
-    uint32_t *scan_tmp = tmp + (size_t)256 * nb;
+    uint32_t *totals = tmp + (size_t)256 * nb;
     bool alt = false;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         const uint32_t *ki = alt ? keys_alt : keys;
@@ -263,8 +368,8 @@ bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
         uint32_t *ko = alt ? keys : keys_alt;
         uint32_t *vo = alt ? vals : vals_alt;
         k_rs_hist<<<nb, kRsBlock, 0, s>>>(ki, n, shift, table, nb);
-        scan_impl<uint32_t>(table, table, 256 * nb, nullptr, false, scan_tmp, s);
-        k_rs_scatter<<<nb, kRsBlock, 0, s>>>(ki, vi, ko, vo, n, shift, table, nb);
+        k_rs_rowscan<<<256, kRsBlock, 0, s>>>(table, nb, totals);
+        k_rs_scatter<<<nb, kRsBlock, 0, s>>>(ki, vi, ko, vo, n, shift, table, totals, nb);
         VPM_LAUNCH_CHECK();
         alt = !alt;
     }

@@ -15,6 +15,15 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tota
 void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
                        bool write_total_at_end, uint32_t *tmp, hipStream_t s);
 
+// The same, for an input whose producer wrote bsum[k] = sum of
+// in[k per, (k+1) per) (per divides the 2048-element scan tile): one launch
+// (the two-launch scan for inputs of more than 8192 runs).
+void scan_exclusive_bs_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
+                           uint32_t *total, bool write_total_at_end, uint32_t *tmp,
+                           hipStream_t s);
+void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
+                          uint32_t *total, bool write_total_at_end, uint32_t *tmp, hipStream_t s);
+
 // Scratch words needed by the radix sort for n elements.
 size_t radix_tmp_words(long long n);
 

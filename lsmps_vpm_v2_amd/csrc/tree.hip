@@ -62,6 +62,21 @@ __device__ __forceinline__ double cell_len(double R, int level) {
     return R / (double)(1u << level);
 }
 
+// Sum of v over a 256-thread block, written to *dst by thread 0 (every thread
+// of the block must call it: block sums of scan inputs, so that their scans
+// skip the reduce pass).
+__device__ __forceinline__ void block_sum_store(uint32_t v, uint32_t *dst) {
+    __shared__ uint32_t sw[256 / kWave];
+    for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) sw[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int u = 0; u < (int)blockDim.x / kWave; ++u) t += sw[u];
+        *dst = t;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 1. bounding box and root geometry
 // ---------------------------------------------------------------------------
@@ -192,19 +207,23 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
              const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
-             double *ys, double *qs, uint8_t *as) {
+             double *ys, double *qs, uint8_t *as, uint32_t *bs_act) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = perm[i];
-    xs[i] = x[o];
-    ys[i] = y[o];
-    if (q) {
-        qs[i] = q[o];
-    } else {
-        const double sz = ssz[o];
-        qs[i] = -omega[o] * (sz * sz) / (2 * M_PI);
+    uint32_t act = 0;
+    if (i < n) {
+        const uint32_t o = perm[i];
+        xs[i] = x[o];
+        ys[i] = y[o];
+        if (q) {
+            qs[i] = q[o];
+        } else {
+            const double sz = ssz[o];
+            qs[i] = -omega[o] * (sz * sz) / (2 * M_PI);
+        }
+        act = a[o] ? 1u : 0u;
+        as[i] = (uint8_t)act;
     }
-    as[i] = a[o] ? 1 : 0;
+    block_sum_store(act, bs_act + blockIdx.x);  // active particles per block
 }
 
 // ---------------------------------------------------------------------------
@@ -367,7 +386,8 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, uint32_t *lvslot) {
+            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, uint32_t *lvslot,
+            uint32_t *bs_cells) {
     extern __shared__ uint8_t lds[];
     // cells started per level (a particle whose lcp with its predecessor is d
     // starts the cells of levels d+1 .. its leaf level, the last a leaf)
@@ -401,6 +421,7 @@ __global__ void __launch_bounds__(kSplitBlock)
     const uint8_t *mw = block_window_max(sw, gw, hw, wlen, kw);
     const uint8_t *mv = S > 0 ? block_window_max(sv, gv, hv, vlen, ks) : sv;
     int my_min = kMaxDepth + 1, my_max = 0, my_depth = 0;
+    uint32_t my_cells = 0;
     if (i < n) {
         const uint32_t ki = key[i];
         const int t = threadIdx.x;
@@ -427,7 +448,8 @@ __global__ void __launch_bounds__(kSplitBlock)
         }
         leaf_lvl[i] = (uint8_t)lvl;
         const int di = d[i];
-        ccount[i] = lvl > di ? (uint32_t)(lvl - di) : 0u;
+        my_cells = lvl > di ? (uint32_t)(lvl - di) : 0u;
+        ccount[i] = my_cells;
         for (int L = di + 1; L <= lvl; ++L) atomicAdd(&lvc[L], 1u);
         if (lvl > di) atomicAdd(&lvc[kLvCols - 1], 1u);
         my_min = lvl;
@@ -462,6 +484,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         atomicMax(row + kSlotMax, (uint32_t)mx);
         if (dp) atomicOr(row + kSlotDepth, 1u);
     }
+    block_sum_store(my_cells, bs_cells + blockIdx.x);  // cells started per block
 }
 
 // ---------------------------------------------------------------------------
@@ -484,16 +507,14 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Cell attributes: range, counts, flags, index, centre (treeCell.hpp:68).
-__global__ void __launch_bounds__(256)
-    k_cells1(const uint32_t *lev_level, const uint32_t *lev_begin, int nc, const uint32_t *key,
-             const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
-             uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
-             uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
-             uint8_t *leaf01) {
+// Cell attributes: range, counts, flags, index, centre (treeCell.hpp:68) of
+// cell j; returns 1 for a leaf.
+__device__ __forceinline__ uint32_t cell_attrs(
+    int j, const uint32_t *lev_level, const uint32_t *lev_begin, const uint32_t *key,
+    const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr, uint8_t *c_level,
+    uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin, uint32_t *c_end, uint32_t *c_src,
+    double *c_cx, double *c_cy, int *c_child, uint8_t *leaf01) {
 #pragma clang fp contract(off)
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nc) return;
     const int L = (int)lev_level[j];
     const uint32_t b = lev_begin[j];
     uint32_t e, pre = 0;
@@ -540,6 +561,21 @@ __global__ void __launch_bounds__(256)
     c_cy[j] = hdr->pivot[1] + ((0.5 + (double)iy) * len);
     reinterpret_cast<int4 *>(c_child)[j] = make_int4(-1, -1, -1, -1);  // filled by k_cells2
     leaf01[j] = (f & kLeaf) ? 1 : 0;
+    return (f & kLeaf) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+    k_cells1(const uint32_t *lev_level, const uint32_t *lev_begin, int nc, const uint32_t *key,
+             const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
+             uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
+             uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
+             uint8_t *leaf01, uint32_t *bs_leaf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lf = 0;
+    if (j < nc)
+        lf = cell_attrs(j, lev_level, lev_begin, key, A, leaf_lvl, n, hdr, c_level, c_flags, c_key,
+                        c_begin, c_end, c_src, c_cx, c_cy, c_child, leaf01);
+    block_sum_store(lf, bs_leaf + blockIdx.x);  // leaves per block
 }
 
 __device__ __forceinline__ int find_cell(const uint32_t *c_key, const uint32_t *lev_off, int level,
@@ -785,7 +821,8 @@ __global__ void __launch_bounds__(256)
                  const double *c_cx, const double *c_cy, const int *c_child, const int *nbr,
                  const DevHdr *hdr, const uint8_t *orders, uint32_t *cnt1, uint32_t *cnt3,
                  uint2 *cntk, const uint32_t *off1, const uint32_t *off3, int *l1, int *l3,
-                 uint32_t cap1, uint32_t cap3, const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part,
+                 uint32_t cap1, uint32_t cap3, uint32_t *bs1, uint32_t *bs3,
+                 const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part,
                  ulonglong2 *part_terms) {
 #pragma clang fp contract(off)
     __shared__ int stk[kWalkDepth][256];
@@ -890,6 +927,10 @@ __global__ void __launch_bounds__(256)
         }
     }
     if (!FILL) {
+        // per-block sums of the list sizes (16 leaves per block) for the scans
+        block_sum_store(live ? n1 : 0u, bs1 + blockIdx.x);
+        __syncthreads();  // block_sum_store's LDS is reused
+        block_sum_store(live ? n3 : 0u, bs3 + blockIdx.x);
         uint32_t s1 = n1, s3 = n3;
 #pragma unroll
         for (int o = 1; o < kLeafLanes; o <<= 1) {
@@ -969,14 +1010,15 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     c.perm_sorted = perm;
     double *xs = c.xs.as<double>(n), *ys = c.ys.as<double>(n), *qs = c.qs.as<double>(n);
     uint8_t *as = c.acts.as<uint8_t>(n);
+    uint32_t *bs_act = c.bs_act.as<uint32_t>((size_t)div_up(n, 256) + 1);
     k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
-                                            ys, qs, as);
+                                            ys, qs, as, bs_act);
 
     // 4. leaf levels, cells per level and leaves
     uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
     k_lcp<<<div_up(n + 1, 256), 256, 0, s>>>(key, n, W, d, w);
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
-    scan_exclusive_u8(as, A, n, &c.hdr_d->n_active, true, tmp, s);
+    scan_exclusive_bs_u8(as, A, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
     uint32_t *apos = c.act_pos.as<uint32_t>(n);
     k_act_pos<<<div_up(n, 256), 256, 0, s>>>(as, A, n, apos);
     uint8_t *v = c.lcp_v.as<uint8_t>(n + 1);
@@ -985,11 +1027,13 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
     const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
+    uint32_t *bs_cells = c.bs_cells.as<uint32_t>((size_t)nbs + 1);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
-                                          tp.max_level, leaf_lvl, cnt, lvslot);
+                                          tp.max_level, leaf_lvl, cnt, lvslot, bs_cells);
     k_reduce_split<<<1, kWave, 0, s>>>(lvslot, c.hdr_d);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
-    scan_exclusive_u32(cnt, base, n, &c.hdr_d->n_cells_pre, true, tmp, s);
+    scan_exclusive_bs_u32(cnt, base, n, bs_cells, kSplitBlock, &c.hdr_d->n_cells_pre, true, tmp,
+                          s);
     VPM_LAUNCH_CHECK();
     // the one synchronisation of the build: sizes of every later launch
     sync_hdr(c);
@@ -1047,13 +1091,14 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     double *c_cx = c.c_cx.as<double>(nc), *c_cy = c.c_cy.as<double>(nc);
     int *c_parent = c.c_parent.as<int>(nc), *c_child = c.c_child.as<int>(4 * (size_t)nc);
     uint8_t *leaf01 = c.lcp_w.as<uint8_t>(std::max(n + 1, nc));
+    uint32_t *bs_leaf = c.bs_leaf.as<uint32_t>((size_t)div_up(nc, 256) + 1);
     k_cells1<<<div_up(nc, 256), 256, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
-                                             c_src, c_cx, c_cy, c_child, leaf01);
+                                             c_src, c_cx, c_cy, c_child, leaf01, bs_leaf);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
 
     uint32_t *lscan = c.leaf_flag_scan.as<uint32_t>(nc + 1);
-    scan_exclusive_u8(leaf01, lscan, nc, &c.hdr_d->n_leaves, true, tmp, s);
+    scan_exclusive_bs_u8(leaf01, lscan, nc, bs_leaf, 256, &c.hdr_d->n_leaves, true, tmp, s);
     uint32_t *leaf_idx = c.leaf_idx.as<uint32_t>(nc);
     k_leaf_idx<<<div_up(nc, 256), 256, 0, s>>>(c_flags, lscan, nc, leaf_idx);
     VPM_LAUNCH_CHECK();
@@ -1089,12 +1134,14 @@ void tree_lists(Ctx &c) {
     uint32_t *off1 = c.off1.as<uint32_t>(nl + 1), *off3 = c.off3.as<uint32_t>(nl + 1);
     const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
     uint2 *cntk = c.cntk.as<uint2>(9 * (size_t)nl + 1);
+    uint32_t *bs1 = c.bs_l13.as<uint32_t>(2 * (size_t)nbl + 2), *bs3 = bs1 + nbl + 1;
     k_leaf_lists<false><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
                                             nbr, c.hdr_d, c.orders, cnt1, cnt3, cntk, nullptr,
-                                            nullptr, nullptr, nullptr, 0, 0, c_begin, c_end,
-                                            c.hdr_d, nullptr, nullptr);
-    scan_exclusive_u32(cnt1, off1, nl, &c.hdr_d->tot1, true, tmp, s);
-    scan_exclusive_u32(cnt3, off3, nl, &c.hdr_d->tot3, true, tmp, s);
+                                            nullptr, nullptr, nullptr, 0, 0, bs1, bs3, c_begin,
+                                            c_end, c.hdr_d, nullptr, nullptr);
+    constexpr int kLeavesPerBlock = 256 / kLeafLanes;
+    scan_exclusive_bs_u32(cnt1, off1, nl, bs1, kLeavesPerBlock, &c.hdr_d->tot1, true, tmp, s);
+    scan_exclusive_bs_u32(cnt3, off3, nl, bs3, kLeavesPerBlock, &c.hdr_d->tot3, true, tmp, s);
     // Capacities without reading the totals back.  A leaf's list 1 holds
     // itself, at most 8 adjacent leaves at its level or coarser (one per
     // direction) and the finer adjacent leaves; each finer leaf is adjacent to
@@ -1107,7 +1154,8 @@ void tree_lists(Ctx &c) {
     k_leaf_lists<true><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
                                            nbr, c.hdr_d, c.orders, nullptr, nullptr, cntk, off1,
                                            off3, l1, l3, (uint32_t)ti.cap1, (uint32_t)ti.cap3,
-                                           c_begin, c_end, c.hdr_d, part, part + nbl);
+                                           nullptr, nullptr, c_begin, c_end, c.hdr_d, part,
+                                           part + nbl);
     k_reduce_part<<<1, 256, 0, s>>>(part, nbl, kPartLeafLists, c.hdr_d);
     k_reduce_part<<<1, 256, 0, s>>>(part + nbl, nbl, kPartLeafTerms, c.hdr_d);
     VPM_LAUNCH_CHECK();
