@@ -531,8 +531,9 @@ __global__ void __launch_bounds__(256)
 // List 4, mode 2: P2L from the active particles of coarser leaves,
 //   b^_m += -q/m (s_t / (z_j - z_t))^m.
 // One wavefront per target cell; lanes stride over the source particles and
-// keep the running powers for every order (<= 16) in registers; the orders
-// are reduced across lanes at the end.
+// keep the running sums of q w^m for every order (<= 16) in registers; the
+// -1/m factor is applied once per order after the lanes' sums are reduced
+// through LDS (lane l writes column l, lane m sums row m; stride 65).
 constexpr int kP2lWaves = 4;
 
 __global__ void __launch_bounds__(kWave *kP2lWaves)
@@ -540,9 +541,10 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
              const double *c_cx, const double *c_cy, const uint32_t *c_begin,
              const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
              const LevelTab *lt, int kst, float *bhat) {
+    __shared__ float2 red[kP2lWaves][16][kWave + 1];
     const int PM1 = kst / 2;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int j = j0 + blockIdx.x * kP2lWaves + threadIdx.x / kWave;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int j = j0 + blockIdx.x * kP2lWaves + w;
     if (j >= nc) return;
     const int m4 = n4[j];
     if (m4 == 0) return;
@@ -554,38 +556,43 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
 #pragma unroll
     for (int m = 0; m < 16; ++m) ar[m] = ai[m] = 0.f;
     for (int u = 0; u < m4; ++u) {
-        const int s = l4[8 * j + u];
-        const float dcx = (float)(c_cx[s] - cx), dcy = (float)(c_cy[s] - cy);
-        for (uint32_t i = c_begin[s] + lane; i < c_end[s]; i += kWave) {
+        const int sc = l4[8 * j + u];
+        const float dcx = (float)(c_cx[sc] - cx), dcy = (float)(c_cy[sc] - cy);
+        for (uint32_t i = c_begin[sc] + lane; i < c_end[sc]; i += kWave) {
             if (!as[i]) continue;
             const float2 o = oxy[i];
-            const cf w = cinv(cf{(dcx + o.x) * inv_st, (dcy + o.y) * inv_st});
-            const float q = qf[i];
-            cf pw = w;
+            const cf wz = cinv(cf{(dcx + o.x) * inv_st, (dcy + o.y) * inv_st});
+            cf pw = cscale(wz, qf[i]);
 #pragma unroll
             for (int m = 1; m <= 16; ++m) {
                 if (m <= p) {
-                    const float c = -q / (float)m;
-                    ar[m - 1] += c * pw.x;
-                    ai[m - 1] += c * pw.y;
-                    pw = cmul(pw, w);
+                    ar[m - 1] += pw.x;
+                    ai[m - 1] += pw.y;
+                    pw = cmul(pw, wz);
                 }
             }
         }
     }
 #pragma unroll
-    for (int m = 1; m <= 16; ++m) {
-        if (m <= p) {
-            float xr = ar[m - 1], xi = ai[m - 1];
-            for (int o = 1; o < kWave; o <<= 1) {
-                xr += __shfl_xor(xr, o, kWave);
-                xi += __shfl_xor(xi, o, kWave);
-            }
-            if (lane == m) {
-                const size_t idx = (size_t)j * PM1 + m;
-                stc(bhat, idx, cadd(ldc(bhat, idx), cf{xr, xi}));
-            }
+    for (int m = 1; m <= 16; ++m)
+        if (m <= p) red[w][m - 1][lane] = make_float2(ar[m - 1], ai[m - 1]);
+    wave_sync();
+    const int m = lane + 1;
+    if (m <= p) {
+        float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+#pragma unroll 4
+        for (int l = 0; l < kWave; l += 4) {
+            const float2 a0 = red[w][m - 1][l], a1 = red[w][m - 1][l + 1];
+            const float2 a2 = red[w][m - 1][l + 2], a3 = red[w][m - 1][l + 3];
+            s0.x += a0.x; s0.y += a0.y;
+            s1.x += a1.x; s1.y += a1.y;
+            s2.x += a2.x; s2.y += a2.y;
+            s3.x += a3.x; s3.y += a3.y;
         }
+        const float c = -1.f / (float)m;
+        const cf sum = {((s0.x + s1.x) + (s2.x + s3.x)) * c, ((s0.y + s1.y) + (s2.y + s3.y)) * c};
+        const size_t idx = (size_t)j * PM1 + m;
+        stc(bhat, idx, cadd(ldc(bhat, idx), sum));
     }
 }
 
