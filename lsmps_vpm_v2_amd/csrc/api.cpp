@@ -348,12 +348,13 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
     s.max_leaf_level = ti.max_level;
     s.n_list1 = ti.tot1;
     s.n_list3 = ti.tot3;
-    s.n_list2 = c.hdr_h->sum2;
-    s.n_list4 = c.hdr_h->sum4;
-    s.p2p_pairs = (int64_t)c.hdr_h->pairs;
-    s.m2p_evals = (int64_t)c.hdr_h->m2p;
-    s.m2p_terms = (int64_t)c.hdr_h->m2p_terms;
-    s.l2p_terms = (int64_t)c.hdr_h->l2p_terms;
+    const DevHdr &hs = *c.hdr_h;
+    s.n_list2 = (int64_t)stat_total(hs, kStSum2);
+    s.n_list4 = (int64_t)stat_total(hs, kStSum4);
+    s.p2p_pairs = (int64_t)stat_total(hs, kStPairs);
+    s.m2p_evals = (int64_t)stat_total(hs, kStM2p);
+    s.m2p_terms = (int64_t)stat_total(hs, kStM2pTerms);
+    s.l2p_terms = (int64_t)stat_total(hs, kStL2pTerms);
     if (h->o.timing) {
         s.ms_tree = span(h, 0, 1);
         s.ms_lists = span(h, 1, 2);

@@ -19,14 +19,28 @@
 
 namespace vpm {
 
+// Leaf-split statistics, accumulated by atomics into kSlotRows rows of one
+// cache line each (row = block mod kSlotRows) in the header and summed on the
+// host at the build's synchronisation: cells per level 0..16, leaves,
+// then 31 - min leaf level, max leaf level and the depth-overflow flag.
+constexpr int kLvCols = kMaxDepth + 2;
+constexpr int kSlotRows = 64;
+constexpr int kSlotStride = 32;
+constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCols + 2;
+
+// List statistics, accumulated by per-block atomics into kStatRows rows (row =
+// block mod kStatRows) and summed on the host: list-2 / list-4 entries, P2P
+// (target, source) pairs, (target, list-3 cell) evaluations, and the M2P /
+// L2P terms (sum of the order + 1 of each list-3 cell per target; the order
+// of each particle's leaf).
+enum StatCol { kStSum2, kStSum4, kStPairs, kStM2p, kStM2pTerms, kStL2pTerms, kStCols = 8 };
+constexpr int kStatRows = 64;
+
 // Header written by device kernels and read back by the host.
 struct DevHdr {
     double root_len;
     double pivot[2];
     double mn[2], mx[2];
-    unsigned long long pairs;    // P2P (target, source) pairs evaluated
-    unsigned long long m2p;      // (target, list-3 cell) evaluations
-    uint32_t sum2, sum4;         // list-2 / list-4 entries over all cells
     unsigned flags;
     uint32_t n_active;
     uint32_t n_cells_pre;        // cells excluding the root
@@ -38,20 +52,28 @@ struct DevHdr {
     uint8_t sel_pad[2];
     uint32_t sel_source;           // 0 fixed, 1 learned, 2 fallback to the fixed order
     float sel_pred_err, sel_pred_ms;
-    unsigned long long m2p_terms;  // sum over (particle, list-3 cell) of (order of the cell + 1)
-    unsigned long long l2p_terms;  // sum over particles of the order of their leaf
     uint32_t lev_cnt[kMaxDepth + 2];  // cells per level, counted by the leaf split
     uint32_t n_leaves_split;          // leaves, counted by the leaf split
     uint32_t pad[3];
+    unsigned long long stat[kStatRows][kStCols];
+    uint32_t split_rows[kSlotRows][kSlotStride];
 };
 
-// Leaf-split statistics, accumulated by atomics into kSlotRows rows of one
-// cache line each (row = block mod kSlotRows): cells per level 0..16, leaves,
-// then 31 - min leaf level, max leaf level and the depth-overflow flag.
-constexpr int kLvCols = kMaxDepth + 2;
-constexpr int kSlotRows = 64;
-constexpr int kSlotStride = 32;
-constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCols + 2;
+// What the build's synchronisation derives from the split rows, handed to the
+// device by value (the first kernel after it writes it into the header).
+struct SplitLevels {
+    uint32_t lev_off[kMaxDepth + 2];
+    uint32_t lev_cnt[kMaxDepth + 2];
+    uint32_t leaf_lvl_min, leaf_lvl_max;
+};
+
+// Host sum of one statistic column.
+inline unsigned long long stat_total(const DevHdr &h, int col) {
+    unsigned long long s = 0;
+    for (int r = 0; r < kStatRows; ++r) s += h.stat[r][col];
+    return s;
+}
+
 
 // Grow-only device buffer.
 struct DBuf {
@@ -100,7 +122,7 @@ struct Ctx {
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
     DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
     DBuf tmp;  // scan / sort scratch
-    DBuf bbox_part, lvslot;
+    DBuf bbox_part;
     DBuf bs_act, bs_cells, bs_leaf, bs_l13;  // producer block sums of scan inputs
     // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
     // resolved): the previous build's choice as the first guess of the next
@@ -119,7 +141,6 @@ struct Ctx {
     // mixed path: FP32 particle offsets and sources, tables, tiles
     DBuf oxy, qf;
     DBuf sel_part, sel_feats, sel_raw;  // level statistics of the selection
-    DBuf part;  // per-block partial results of the tree and list kernels
     // expansion order per level (levels 0..17), device resident: slot 0 holds
     // the learned selection (written by the selection kernel), slot 1 the
     // fixed orders (uploaded when they change); `orders` points at the slot

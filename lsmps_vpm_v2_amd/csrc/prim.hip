@@ -144,12 +144,25 @@ __global__ void __launch_bounds__(kScanBlock)
 // Down-sweep from block sums of the producer: bsum[k] is the sum of
 // in[k per, (k+1) per); tile b (kScanTile elements, a multiple of per) adds
 // pre(b) = bsum[0 .. b kScanTile / per), summed by the block itself, to its
-// own scan.
+// own scan.  out (if not null) receives the exclusive scan; compact (if not
+// null, for 0/1 inputs) receives i at position out[i] for every in[i] = 1.
+template <typename T>
+struct ScanBs {
+    const T *in;
+    uint32_t *out, *compact;
+    const uint32_t *bsum;
+    uint32_t *total;
+};
+
+// blockIdx.y selects one of up to two independent scans of the same length.
 template <typename T>
 __global__ void __launch_bounds__(kScanBlock)
-    k_scan_down_bs(const T *in, uint32_t *out, int n, const uint32_t *bsum, int per, int nbs,
-                   uint32_t *total, int write_end) {
+    k_scan_down_bs(ScanBs<T> a0, ScanBs<T> a1, int n, int per, int nbs, int write_end) {
     __shared__ uint32_t sm[kScanBlock / kWave];
+    const ScanBs<T> &a = blockIdx.y == 0 ? a0 : a1;
+    const T *in = a.in;
+    uint32_t *out = a.out, *compact = a.compact, *total = a.total;
+    const uint32_t *bsum = a.bsum;
     const int kend = (int)min((long long)nbs, (long long)blockIdx.x * kScanTile / per);
     uint32_t pre = 0, pre_all;
     for (int k = threadIdx.x; k < kend; k += kScanBlock) pre += bsum[k];
@@ -168,12 +181,15 @@ __global__ void __launch_bounds__(kScanBlock)
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         long long i = base + j;
-        if (i < n) out[i] = run;
+        if (i < n) {
+            if (out) out[i] = run;
+            if (compact && v[j]) compact[run] = (uint32_t)i;
+        }
         run += v[j];
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
         if (total) *total = pre_all + tot;
-        if (write_end) out[n] = pre_all + tot;
+        if (write_end && out) out[n] = pre_all + tot;
     }
 }
 
@@ -318,36 +334,69 @@ void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
     scan_impl<uint8_t>(in, out, n, total, write_total_at_end, tmp, s);
 }
 
+// Compaction of 0/1 flags into the positions of the ones (helper of the
+// long-input fallback): compact[scan[i]] = i where flag[i].
 template <typename T>
-void scan_bs_impl(const T *in, uint32_t *out, int n, uint32_t *bsum, int per, uint32_t *total,
-                  bool write_end, uint32_t *tmp, hipStream_t s) {
+__global__ void __launch_bounds__(256)
+    k_compact(const T *flag, const uint32_t *scan, int n, uint32_t *compact) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) compact[scan[i]] = (uint32_t)i;
+}
+
+template <typename T>
+void scan_bs_impl(const T *in, uint32_t *out, uint32_t *compact, int n, uint32_t *bsum, int per,
+                  uint32_t *total, bool write_end, uint32_t *tmp, hipStream_t s) {
     if (n <= 0) {
-        k_write_zero_total<<<1, 1, 0, s>>>(out, 0, total, write_end ? 1 : 0);
+        k_write_zero_total<<<1, 1, 0, s>>>(out, 0, total, (write_end && out) ? 1 : 0);
         VPM_LAUNCH_CHECK();
         return;
     }
     if (per <= 0 || kScanTile % per != 0) throw Error(kInternal, "scan block sums: bad run length");
+    if (!out && !compact) throw Error(kInternal, "scan block sums: no output");
     const int nb = div_up(n, kScanTile), nbs = div_up(n, per);
     if (nbs > kFusedSpine * 4) {
         // very long inputs: the tile reduce (input read once more) beats a
         // one-block spine over this many block sums
+        if (!out) throw Error(kInternal, "scan block sums: long compaction needs the scan output");
         scan_impl<T>(in, out, n, total, write_end, tmp, s);
+        if (compact) k_compact<T><<<div_up(n, 256), 256, 0, s>>>(in, out, n, compact);
+        VPM_LAUNCH_CHECK();
         return;
     }
-    k_scan_down_bs<T><<<nb, kScanBlock, 0, s>>>(in, out, n, bsum, per, nbs, total,
-                                                write_end ? 1 : 0);
+    const ScanBs<T> a{in, out, compact, bsum, total};
+    k_scan_down_bs<T><<<nb, kScanBlock, 0, s>>>(a, a, n, per, nbs, write_end ? 1 : 0);
     VPM_LAUNCH_CHECK();
 }
 
 void scan_exclusive_bs_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
                            uint32_t *total, bool write_total_at_end, uint32_t *tmp,
                            hipStream_t s) {
-    scan_bs_impl<uint32_t>(in, out, n, bsum, per, total, write_total_at_end, tmp, s);
+    scan_bs_impl<uint32_t>(in, out, nullptr, n, bsum, per, total, write_total_at_end, tmp, s);
 }
 
-void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
-                          uint32_t *total, bool write_total_at_end, uint32_t *tmp, hipStream_t s) {
-    scan_bs_impl<uint8_t>(in, out, n, bsum, per, total, write_total_at_end, tmp, s);
+void scan_exclusive_bs_u32_pair(const uint32_t *in0, uint32_t *out0, uint32_t *bsum0,
+                                uint32_t *total0, const uint32_t *in1, uint32_t *out1,
+                                uint32_t *bsum1, uint32_t *total1, int n, int per,
+                                bool write_total_at_end, uint32_t *tmp, hipStream_t s) {
+    const int nbs = n > 0 ? div_up(n, per) : 0;
+    if (n <= 0 || per <= 0 || kScanTile % per != 0 || nbs > kFusedSpine * 4) {
+        scan_bs_impl<uint32_t>(in0, out0, nullptr, n, bsum0, per, total0, write_total_at_end, tmp,
+                               s);
+        scan_bs_impl<uint32_t>(in1, out1, nullptr, n, bsum1, per, total1, write_total_at_end, tmp,
+                               s);
+        return;
+    }
+    const ScanBs<uint32_t> a0{in0, out0, nullptr, bsum0, total0};
+    const ScanBs<uint32_t> a1{in1, out1, nullptr, bsum1, total1};
+    k_scan_down_bs<uint32_t><<<dim3(div_up(n, kScanTile), 2), kScanBlock, 0, s>>>(
+        a0, a1, n, per, nbs, write_total_at_end ? 1 : 0);
+    VPM_LAUNCH_CHECK();
+}
+
+void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, uint32_t *compact, int n,
+                          uint32_t *bsum, int per, uint32_t *total, bool write_total_at_end,
+                          uint32_t *tmp, hipStream_t s) {
+    scan_bs_impl<uint8_t>(in, out, compact, n, bsum, per, total, write_total_at_end, tmp, s);
 }
 
 size_t radix_tmp_words(long long n) {

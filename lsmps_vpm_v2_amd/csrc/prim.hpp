@@ -21,8 +21,16 @@ void scan_exclusive_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *total,
 void scan_exclusive_bs_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
                            uint32_t *total, bool write_total_at_end, uint32_t *tmp,
                            hipStream_t s);
-void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, int n, uint32_t *bsum, int per,
-                          uint32_t *total, bool write_total_at_end, uint32_t *tmp, hipStream_t s);
+// Two independent such scans of the same length in one launch.
+void scan_exclusive_bs_u32_pair(const uint32_t *in0, uint32_t *out0, uint32_t *bsum0,
+                                uint32_t *total0, const uint32_t *in1, uint32_t *out1,
+                                uint32_t *bsum1, uint32_t *total1, int n, int per,
+                                bool write_total_at_end, uint32_t *tmp, hipStream_t s);
+// For 0/1 flags: out (may be null) gets the scan, compact (may be null) the
+// positions of the ones in order (compact[out[i]] = i where in[i] = 1).
+void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, uint32_t *compact, int n,
+                          uint32_t *bsum, int per, uint32_t *total, bool write_total_at_end,
+                          uint32_t *tmp, hipStream_t s);
 
 // Scratch words needed by the radix sort for n elements.
 size_t radix_tmp_words(long long n);

@@ -80,19 +80,16 @@ __device__ __forceinline__ void block_sum_store(uint32_t v, uint32_t *dst) {
 // ---------------------------------------------------------------------------
 // 1. bounding box and root geometry
 // ---------------------------------------------------------------------------
-// Block 0 also clears the device header and the split's statistic rows
-// (every later kernel of the build runs after this one on the stream).
+// Block 0 also clears the device header, statistic rows included (every later
+// kernel of the build runs after this one on the stream).
 __global__ void __launch_bounds__(256)
-    k_bbox_partial(const double *x, const double *y, int n, double *part, DevHdr *hdr,
-                   uint32_t *lvslot) {
+    k_bbox_partial(const double *x, const double *y, int n, double *part, DevHdr *hdr) {
     __shared__ double s[4][256];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
-    if (blockIdx.x == 0) {
+    if (blockIdx.x == 0)
         for (int u = threadIdx.x; u < (int)(sizeof(DevHdr) / 4); u += blockDim.x)
             reinterpret_cast<uint32_t *>(hdr)[u] = 0u;
-        for (int u = threadIdx.x; u < kSlotRows * kSlotStride; u += blockDim.x) lvslot[u] = 0u;
-    }
     double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
     int nf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -204,13 +201,20 @@ __global__ void __launch_bounds__(256)
 // Particles in Morton order.  The source strength is q[o], or, for the
 // velocity entry points (q == nullptr), -omega s^2 / (2 pi) formed here in the
 // operation order of velocity_calc.cpp:477 (pow(s, 2) is the exact square).
+// Also the level lcps of the sorted keys: d[i] = lcp(key[i-1], key[i])
+// (d[0] = d[n] = 0) and w[i] = lcp(key[i], key[i+W]).
 __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
              const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
-             double *ys, double *qs, uint8_t *as, uint32_t *bs_act) {
+             double *ys, double *qs, uint8_t *as, uint32_t *bs_act, const uint32_t *key, int W,
+             uint8_t *d, uint8_t *w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t act = 0;
     if (i < n) {
+        const uint32_t ki = key[i];
+        d[i] = i == 0 ? 0 : (uint8_t)lcp_levels(key[i - 1], ki);
+        if (i + W < n) w[i] = (uint8_t)lcp_levels(ki, key[i + W]);
+        if (i == n - 1) d[n] = 0;
         const uint32_t o = perm[i];
         xs[i] = x[o];
         ys[i] = y[o];
@@ -229,21 +233,6 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // 4. leaf levels from windowed LCPs
 // ---------------------------------------------------------------------------
-// d[i] = lcp(key[i-1], key[i]) (d[0] = d[n] = 0); w[j] = lcp(key[j], key[j+W]).
-__global__ void __launch_bounds__(256)
-    k_lcp(const uint32_t *key, int n, int W, uint8_t *d, uint8_t *w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    d[i] = (i == 0 || i == n) ? 0 : (uint8_t)lcp_levels(key[i - 1], key[i]);
-    if (i + W < n) w[i] = (uint8_t)lcp_levels(key[i], key[i + W]);
-}
-
-__global__ void __launch_bounds__(256)
-    k_act_pos(const uint8_t *as, const uint32_t *A, int n, uint32_t *apos) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && as[i]) apos[A[i]] = (uint32_t)i;
-}
-
 // v[m] = lcp(key[a_m], key[a_{m+S}]) over the sorted active particles.
 __global__ void __launch_bounds__(256)
     k_lcp_v(const uint32_t *key, const uint32_t *apos, const DevHdr *hdr, int S, int n,
@@ -254,105 +243,29 @@ __global__ void __launch_bounds__(256)
     v[m] = (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]]);
 }
 
-// Per-block partial results (two 64-bit words per block) and their one-block
-// reduction.  Global atomics on one header word from thousands of blocks
-// serialize across the chiplets; partials plus one small reduction do not.
-enum PartOp { kPartM2lLists = 1, kPartLeafLists = 2, kPartLeafTerms = 3 };
-
-// Block-wide sum of two 64-bit values; thread 0 writes part[blk].
-__device__ __forceinline__ void block_part(unsigned long long a, unsigned long long b,
-                                           ulonglong2 *part) {
-    __shared__ unsigned long long sa[16], sb[16];
-    for (int m = 1; m < kWave; m <<= 1) {
-        a += __shfl_xor(a, m, kWave);
-        b += __shfl_xor(b, m, kWave);
-    }
+// Block sums of up to four 64-bit statistics, added by thread 0 to the
+// header's statistic row of this block (every thread must call it).
+__device__ __forceinline__ void block_stats(DevHdr *hdr, int col0, int ncol,
+                                            unsigned long long v0, unsigned long long v1,
+                                            unsigned long long v2 = 0, unsigned long long v3 = 0) {
+    __shared__ unsigned long long sv[4][16];
+    unsigned long long v[4] = {v0, v1, v2, v3};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        for (int m = 1; m < kWave; m <<= 1) v[c] += __shfl_xor(v[c], m, kWave);
     const int w = threadIdx.x / kWave, nw = (blockDim.x + kWave - 1) / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        sa[w] = a;
-        sb[w] = b;
-    }
+    if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sv[c][w] = v[c];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int u = 1; u < nw; ++u) {
-            a += sa[u];
-            b += sb[u];
-        }
-        part[blockIdx.x] = make_ulonglong2(a, b);
-    }
-}
-
-__global__ void __launch_bounds__(256)
-    k_reduce_part(const ulonglong2 *part, int nblk, int op, DevHdr *hdr) {
-    unsigned long long a = 0ull, b = 0ull;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        const ulonglong2 v = part[i];
-        a += v.x;
-        b += v.y;
-    }
-    __shared__ unsigned long long sa[4], sb[4];
-    for (int m = 1; m < kWave; m <<= 1) {
-        a += __shfl_xor(a, m, kWave);
-        b += __shfl_xor(b, m, kWave);
-    }
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        sa[w] = a;
-        sb[w] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int u = 1; u < 4; ++u) {
-        a += sa[u];
-        b += sb[u];
-    }
-    if (op == kPartM2lLists) {
-        hdr->sum2 = (uint32_t)a;
-        hdr->sum4 = (uint32_t)b;
-    } else if (op == kPartLeafLists) {
-        hdr->pairs = a;
-        hdr->m2p = b;
-    } else {
-        hdr->m2p_terms = a;
-        hdr->l2p_terms = b;
+    if (threadIdx.x < ncol) {
+        unsigned long long t = 0;
+        for (int u = 0; u < nw; ++u) t += sv[threadIdx.x][u];
+        if (t) atomicAdd(&hdr->stat[blockIdx.x % kStatRows][col0 + threadIdx.x], t);
     }
 }
 
 constexpr int kSplitBlock = 256;
-
-// The split's statistic rows into the header: min / max leaf level (and the
-// depth-overflow flag), cells per level, leaves, and the level offsets of the
-// cell arrays (root = cell 0, then the levels in order).  One wavefront, lane
-// = row.
-__global__ void __launch_bounds__(kWave) k_reduce_split(const uint32_t *lvslot, DevHdr *hdr) {
-    const int lane = threadIdx.x;
-    const uint32_t *row = lvslot + (size_t)lane * kSlotStride;
-    uint32_t v[kLvCols + 3];
-#pragma unroll
-    for (int c = 0; c < kLvCols + 3; ++c) v[c] = row[c];
-    for (int m = 1; m < kWave; m <<= 1) {
-#pragma unroll
-        for (int c = 0; c < kLvCols; ++c) v[c] += __shfl_xor(v[c], m, kWave);
-        v[kSlotMinInv] = max(v[kSlotMinInv], (uint32_t)__shfl_xor(v[kSlotMinInv], m, kWave));
-        v[kSlotMax] = max(v[kSlotMax], (uint32_t)__shfl_xor(v[kSlotMax], m, kWave));
-        v[kSlotDepth] |= (uint32_t)__shfl_xor(v[kSlotDepth], m, kWave);
-    }
-    if (lane != 0) return;
-    hdr->leaf_lvl_min = 31u - v[kSlotMinInv];
-    hdr->leaf_lvl_max = v[kSlotMax];
-    if (v[kSlotDepth]) hdr->flags |= kFlagDepth;
-    uint32_t off = 1;  // the root is cell 0
-    hdr->lev_off[0] = 0;
-    hdr->lev_cnt[0] = 1;
-#pragma unroll
-    for (int L = 1; L <= kMaxDepth + 1; ++L) {
-        const uint32_t cnt = L <= kMaxDepth ? v[L] : 0u;
-        hdr->lev_off[L] = off;
-        hdr->lev_cnt[L] = cnt;
-        off += cnt;
-    }
-    hdr->n_leaves_split = v[kLvCols - 1];
-}
 
 // Sliding-window maxima by doubling (a sparse table): after step s the
 // buffer holds the maximum over [x, x + 2^s); a window of length B, with
@@ -386,7 +299,7 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, uint32_t *lvslot,
+            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, DevHdr *hdr_w,
             uint32_t *bs_cells) {
     extern __shared__ uint8_t lds[];
     // cells started per level (a particle whose lcp with its predecessor is d
@@ -469,7 +382,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         s_mm[wv][2] = my_depth;
     }
     __syncthreads();
-    uint32_t *row = lvslot + (size_t)(blockIdx.x % kSlotRows) * kSlotStride;
+    uint32_t *row = hdr_w->split_rows[blockIdx.x % kSlotRows];
     if (threadIdx.x < kLvCols) {
         const uint32_t c = lvc[threadIdx.x];
         if (c) atomicAdd(row + threadIdx.x, c);
@@ -492,11 +405,20 @@ __global__ void __launch_bounds__(kSplitBlock)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
     k_emit(const uint8_t *d, const uint8_t *leaf_lvl, const uint32_t *base, int n,
-           uint32_t *pre_level, uint32_t *pre_begin) {
+           uint32_t *pre_level, uint32_t *pre_begin, SplitLevels sl, DevHdr *hdr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         pre_level[0] = 0;
         pre_begin[0] = 0;
+    }
+    // level offsets / counts and the leaf-level range for the later kernels
+    if (blockIdx.x == 0 && threadIdx.x < kMaxDepth + 2) {
+        hdr->lev_off[threadIdx.x] = sl.lev_off[threadIdx.x];
+        hdr->lev_cnt[threadIdx.x] = sl.lev_cnt[threadIdx.x];
+        if (threadIdx.x == 0) {
+            hdr->leaf_lvl_min = sl.leaf_lvl_min;
+            hdr->leaf_lvl_max = sl.leaf_lvl_max;
+        }
     }
     if (i >= n) return;
     const int di = d[i], lv = leaf_lvl[i];
@@ -609,12 +531,6 @@ __global__ void __launch_bounds__(256)
     if (par >= 0) c_child[4 * par + qx + 2 * qy] = j;
 }
 
-__global__ void __launch_bounds__(256)
-    k_leaf_idx(const uint8_t *c_flags, const uint32_t *scan, int nc, uint32_t *leaf_idx) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nc && (c_flags[j] & kLeaf)) leaf_idx[scan[j]] = (uint32_t)j;
-}
-
 // ---------------------------------------------------------------------------
 // 6. neighbour table and lists
 // ---------------------------------------------------------------------------
@@ -717,7 +633,7 @@ __global__ void __launch_bounds__(256)
     k_m2l_lists(const uint8_t *c_level, const uint8_t *c_flags, const double *c_cx,
                 const double *c_cy, const int *c_parent, const int *c_child, const int *nbr,
                 const DevHdr *hdr, int j0, int nc, int *l2, uint8_t *n2, int *l4, uint8_t *n4,
-                DevHdr *hdr_w, ulonglong2 *part) {
+                DevHdr *hdr_w) {
 #pragma clang fp contract(off)
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = j0 + (int)(gid / kM2lLanes);
@@ -793,8 +709,8 @@ __global__ void __launch_bounds__(256)
             n4[j] = (uint8_t)tot4;
         }
     }
-    block_part((unsigned long long)(live ? m2 : 0), (unsigned long long)(live ? m4 : 0),
-                      part);
+    block_stats(hdr_w, kStSum2, 2, (unsigned long long)(live ? m2 : 0),
+                (unsigned long long)(live ? m4 : 0));
 }
 
 // Lists 1 and 3 of every leaf (treeCell.cpp:636-804), count or fill pass.
@@ -822,8 +738,7 @@ __global__ void __launch_bounds__(256)
                  const DevHdr *hdr, const uint8_t *orders, uint32_t *cnt1, uint32_t *cnt3,
                  uint2 *cntk, const uint32_t *off1, const uint32_t *off3, int *l1, int *l3,
                  uint32_t cap1, uint32_t cap3, uint32_t *bs1, uint32_t *bs3,
-                 const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w, ulonglong2 *part,
-                 ulonglong2 *part_terms) {
+                 const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w) {
 #pragma clang fp contract(off)
     __shared__ int stk[kWalkDepth][256];
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -951,9 +866,7 @@ __global__ void __launch_bounds__(256)
     }
     if (overflow) atomicOr(&hdr_w->flags, kFlagListOverflow);
     if (FILL) {
-        block_part(my_pairs, my_m2p, part);
-        __syncthreads();  // block_part's LDS is reused
-        block_part(my_m2p_terms, my_l2p_terms, part_terms);
+        block_stats(hdr_w, kStPairs, 4, my_pairs, my_m2p, my_m2p_terms, my_l2p_terms);
     }
 }
 
@@ -987,8 +900,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     // 1. bounding box (the first kernel also clears the header)
     const int nbb = std::min(1024, div_up(n, 256 * 8));
     double *part = c.bbox_part.as<double>(5 * (size_t)nbb);
-    uint32_t *lvslot = c.lvslot.as<uint32_t>((size_t)kSlotRows * kSlotStride);
-    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d, lvslot);
+    k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d);
     const double expansion = 1.0 + (tp.exp_tree / 100.0);
     k_bbox_final<<<1, 256, 0, s>>>(part, nbb, expansion, c.hdr_d);
     VPM_LAUNCH_CHECK();
@@ -1011,16 +923,14 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     double *xs = c.xs.as<double>(n), *ys = c.ys.as<double>(n), *qs = c.qs.as<double>(n);
     uint8_t *as = c.acts.as<uint8_t>(n);
     uint32_t *bs_act = c.bs_act.as<uint32_t>((size_t)div_up(n, 256) + 1);
+    uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
     k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
-                                            ys, qs, as, bs_act);
+                                            ys, qs, as, bs_act, key, W, d, w);
 
     // 4. leaf levels, cells per level and leaves
-    uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
-    k_lcp<<<div_up(n + 1, 256), 256, 0, s>>>(key, n, W, d, w);
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
-    scan_exclusive_bs_u8(as, A, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
     uint32_t *apos = c.act_pos.as<uint32_t>(n);
-    k_act_pos<<<div_up(n, 256), 256, 0, s>>>(as, A, n, apos);
+    scan_exclusive_bs_u8(as, A, apos, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
     uint8_t *v = c.lcp_v.as<uint8_t>(n + 1);
     k_lcp_v<<<div_up(n, 256), 256, 0, s>>>(key, apos, c.hdr_d, S, n, v);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
@@ -1029,16 +939,41 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     const int nbs = div_up(n, kSplitBlock);
     uint32_t *bs_cells = c.bs_cells.as<uint32_t>((size_t)nbs + 1);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
-                                          tp.max_level, leaf_lvl, cnt, lvslot, bs_cells);
-    k_reduce_split<<<1, kWave, 0, s>>>(lvslot, c.hdr_d);
+                                          tp.max_level, leaf_lvl, cnt, c.hdr_d, bs_cells);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
     scan_exclusive_bs_u32(cnt, base, n, bs_cells, kSplitBlock, &c.hdr_d->n_cells_pre, true, tmp,
                           s);
     VPM_LAUNCH_CHECK();
     // the one synchronisation of the build: sizes of every later launch
     sync_hdr(c);
-    const DevHdr &hh = *c.hdr_h;
+    DevHdr &hh = *c.hdr_h;
     if (hh.flags & (kFlagNonFinite | kFlagRange)) check_flags(hh);
+    // the split rows: leaf-level range, depth overflow, cells per level, leaves
+    SplitLevels sl{};
+    {
+        uint32_t col[kLvCols] = {}, min_inv = 0, lmax = 0, depth = 0;
+        for (int r = 0; r < kSlotRows; ++r) {
+            const uint32_t *row = hh.split_rows[r];
+            for (int cc = 0; cc < kLvCols; ++cc) col[cc] += row[cc];
+            min_inv = std::max(min_inv, row[kSlotMinInv]);
+            lmax = std::max(lmax, row[kSlotMax]);
+            depth |= row[kSlotDepth];
+        }
+        if (depth) hh.flags |= kFlagDepth;
+        hh.leaf_lvl_min = sl.leaf_lvl_min = 31u - min_inv;
+        hh.leaf_lvl_max = sl.leaf_lvl_max = lmax;
+        uint32_t off = 1;  // the root is cell 0
+        sl.lev_off[0] = 0;
+        sl.lev_cnt[0] = 1;
+        for (int L = 1; L <= kMaxDepth + 1; ++L) {
+            const uint32_t cnt_l = L <= kMaxDepth ? col[L] : 0u;
+            sl.lev_off[L] = off;
+            sl.lev_cnt[L] = cnt_l;
+            off += cnt_l;
+        }
+        hh.n_leaves_split = col[kLvCols - 1];
+        for (int L = 0; L < kMaxDepth + 2; ++L) hh.lev_off[L] = sl.lev_off[L];
+    }
     // A split on the sorted top bits is exact while its deepest leaf level is
     // at most bits / 2: a window's lcp is taken from its two end keys, which
     // bounds the whole window only where the keys are sorted, so every lcp
@@ -1080,7 +1015,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
 
     uint32_t *pl = c.pre_level.as<uint32_t>(nc), *pb = c.pre_slot.as<uint32_t>(nc);
     uint32_t *ll = c.lev_level.as<uint32_t>(nc), *lb = c.lev2pre.as<uint32_t>(nc);
-    k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb);
+    k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, sl, c.hdr_d);
     const bool alt2 = radix_sort_pairs(pl, pb, ll, lb, nc, 0, 8, tmp, s);
     const uint32_t *lev_level = alt2 ? ll : pl;
     const uint32_t *lev_begin = alt2 ? lb : pb;
@@ -1097,10 +1032,11 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
                                              c_src, c_cx, c_cy, c_child, leaf01, bs_leaf);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
 
-    uint32_t *lscan = c.leaf_flag_scan.as<uint32_t>(nc + 1);
-    scan_exclusive_bs_u8(leaf01, lscan, nc, bs_leaf, 256, &c.hdr_d->n_leaves, true, tmp, s);
+    // leaf cells in cell order (scan of the leaf flags, compacted)
     uint32_t *leaf_idx = c.leaf_idx.as<uint32_t>(nc);
-    k_leaf_idx<<<div_up(nc, 256), 256, 0, s>>>(c_flags, lscan, nc, leaf_idx);
+    uint32_t *lscan = c.leaf_flag_scan.as<uint32_t>(nc + 1);
+    scan_exclusive_bs_u8(leaf01, lscan, leaf_idx, nc, bs_leaf, 256, &c.hdr_d->n_leaves, false,
+                         tmp, s);
     VPM_LAUNCH_CHECK();
 }
 
@@ -1124,11 +1060,9 @@ void tree_lists(Ctx &c) {
     // and 1 have no lists 2 / 4 and nothing reads their entries
     const int nb2 = nc > j0 ? (int)div_up((long long)(nc - j0) * kM2lLanes, 256LL) : 0;
     const int nbl = (int)div_up((long long)nl * kLeafLanes, 256LL);
-    ulonglong2 *part = c.part.as<ulonglong2>(std::max(std::max(nb2, 2 * nbl), 1));
     if (nb2 > 0)
         k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
-                                        c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d, part);
-    k_reduce_part<<<1, 256, 0, s>>>(part, nb2, kPartM2lLists, c.hdr_d);
+                                        c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d);
     const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
     uint32_t *cnt1 = c.cnt1.as<uint32_t>(nl + 1), *cnt3 = c.cnt3.as<uint32_t>(nl + 1);
     uint32_t *off1 = c.off1.as<uint32_t>(nl + 1), *off3 = c.off3.as<uint32_t>(nl + 1);
@@ -1138,10 +1072,10 @@ void tree_lists(Ctx &c) {
     k_leaf_lists<false><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
                                             nbr, c.hdr_d, c.orders, cnt1, cnt3, cntk, nullptr,
                                             nullptr, nullptr, nullptr, 0, 0, bs1, bs3, c_begin,
-                                            c_end, c.hdr_d, nullptr, nullptr);
+                                            c_end, c.hdr_d);
     constexpr int kLeavesPerBlock = 256 / kLeafLanes;
-    scan_exclusive_bs_u32(cnt1, off1, nl, bs1, kLeavesPerBlock, &c.hdr_d->tot1, true, tmp, s);
-    scan_exclusive_bs_u32(cnt3, off3, nl, bs3, kLeavesPerBlock, &c.hdr_d->tot3, true, tmp, s);
+    scan_exclusive_bs_u32_pair(cnt1, off1, bs1, &c.hdr_d->tot1, cnt3, off3, bs3, &c.hdr_d->tot3, nl,
+                               kLeavesPerBlock, true, tmp, s);
     // Capacities without reading the totals back.  A leaf's list 1 holds
     // itself, at most 8 adjacent leaves at its level or coarser (one per
     // direction) and the finer adjacent leaves; each finer leaf is adjacent to
@@ -1154,10 +1088,7 @@ void tree_lists(Ctx &c) {
     k_leaf_lists<true><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
                                            nbr, c.hdr_d, c.orders, nullptr, nullptr, cntk, off1,
                                            off3, l1, l3, (uint32_t)ti.cap1, (uint32_t)ti.cap3,
-                                           nullptr, nullptr, c_begin, c_end, c.hdr_d, part,
-                                           part + nbl);
-    k_reduce_part<<<1, 256, 0, s>>>(part, nbl, kPartLeafLists, c.hdr_d);
-    k_reduce_part<<<1, 256, 0, s>>>(part + nbl, nbl, kPartLeafTerms, c.hdr_d);
+                                           nullptr, nullptr, c_begin, c_end, c.hdr_d);
     VPM_LAUNCH_CHECK();
 }
 
