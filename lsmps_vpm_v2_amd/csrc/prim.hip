@@ -154,6 +154,35 @@ struct ScanBs {
     uint32_t *total;
 };
 
+// kScanItems consecutive elements from base (aligned: base is a multiple of
+// kScanItems); vector loads when the whole run is in range.
+__device__ __forceinline__ void load_items(const uint8_t *in, long long base, int n,
+                                           uint32_t (&v)[kScanItems]) {
+    if (base + kScanItems <= n) {
+        const uint2 w = *reinterpret_cast<const uint2 *>(in + base);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[j] = (w.x >> (8 * j)) & 255u;
+            v[j + 4] = (w.y >> (8 * j)) & 255u;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = base + j < n ? (uint32_t)in[base + j] : 0u;
+    }
+}
+__device__ __forceinline__ void load_items(const uint32_t *in, long long base, int n,
+                                           uint32_t (&v)[kScanItems]) {
+    if (base + kScanItems <= n) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(in + base);
+        const uint4 b = *reinterpret_cast<const uint4 *>(in + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = base + j < n ? in[base + j] : 0u;
+    }
+}
+
 // blockIdx.y selects one of up to two independent scans of the same length.
 template <typename T>
 __global__ void __launch_bounds__(kScanBlock)
@@ -169,23 +198,28 @@ __global__ void __launch_bounds__(kScanBlock)
     block_excl_scan<kScanBlock>(pre, sm, pre_all);
     const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
+    load_items(in, base, n, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        long long i = base + j;
-        v[j] = i < n ? (uint32_t)in[i] : 0u;
-        s += v[j];
-    }
+    for (int j = 0; j < kScanItems; ++j) s += v[j];
     uint32_t tot;
     uint32_t run = pre_all + block_excl_scan<kScanBlock>(s, sm, tot);
+    uint32_t r[kScanItems];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        long long i = base + j;
-        if (i < n) {
-            if (out) out[i] = run;
-            if (compact && v[j]) compact[run] = (uint32_t)i;
-        }
+        r[j] = run;
+        if (compact && v[j] && base + j < n) compact[run] = (uint32_t)(base + j);
         run += v[j];
+    }
+    if (out) {
+        if (base + kScanItems <= n) {
+            *reinterpret_cast<uint4 *>(out + base) = make_uint4(r[0], r[1], r[2], r[3]);
+            *reinterpret_cast<uint4 *>(out + base + 4) = make_uint4(r[4], r[5], r[6], r[7]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j)
+                if (base + j < n) out[base + j] = r[j];
+        }
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
         if (total) *total = pre_all + tot;
