@@ -319,7 +319,7 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const Sour
 
 void read_hdr(Ctx &c) {
     VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
-    VPM_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c);
 }
 
 // Final synchronisation of a call: list totals and checks, the selection
@@ -644,6 +644,7 @@ int vpmfmm_destroy(vpmfmm_handle *h) {
     if (h->ctx.stream) (void)hipStreamSynchronize(h->ctx.stream);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    if (h->ctx.ev_sync) (void)hipEventDestroy(h->ctx.ev_sync);
     if (h->ctx.hdr_d) (void)hipFree(h->ctx.hdr_d);
     if (h->ctx.hdr_h) (void)hipHostFree(h->ctx.hdr_h);
     h->ctx.hdr_d = nullptr;

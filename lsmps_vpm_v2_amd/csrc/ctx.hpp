@@ -111,6 +111,7 @@ struct TreeInfo {
 
 struct Ctx {
     hipStream_t stream = nullptr;
+    hipEvent_t ev_sync = nullptr;  // completion marker of the per-solve synchronisations
     DevHdr *hdr_d = nullptr;  // device header
     DevHdr *hdr_h = nullptr;  // pinned host mirror
 
@@ -170,6 +171,11 @@ struct Sources {
 // Stage entry points (tree.hip).  Inputs are device pointers.
 void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &src,
                 const uint8_t *active, const TreeParams &tp);
+// Waits until the work queued on c.stream so far has completed by polling an
+// event (the host returns as soon as the GPU reaches it, without the
+// scheduler wake-up of a blocking synchronisation).
+void stream_wait(Ctx &c);
+
 // Lists of the tree in c, with no host synchronisation: sizes come from the
 // counts of tree_build and the list arrays are sized by proven bounds.
 void tree_lists(Ctx &c);

@@ -875,9 +875,19 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+void stream_wait(Ctx &c) {
+    if (!c.ev_sync) VPM_HIP(hipEventCreateWithFlags(&c.ev_sync, hipEventDisableTiming));
+    VPM_HIP(hipEventRecord(c.ev_sync, c.stream));
+    for (;;) {
+        const hipError_t e = hipEventQuery(c.ev_sync);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) VPM_HIP(e);
+    }
+}
+
 static void sync_hdr(Ctx &c) {
     VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
-    VPM_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c);
 }
 
 static void check_flags(const DevHdr &h) {
