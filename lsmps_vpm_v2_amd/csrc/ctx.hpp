@@ -24,7 +24,7 @@ namespace vpm {
 // host at the build's synchronisation: cells per level 0..16, leaves,
 // then 31 - min leaf level, max leaf level and the depth-overflow flag.
 constexpr int kLvCols = kMaxDepth + 2;
-constexpr int kSlotRows = 64;
+constexpr int kSlotRows = 16;
 constexpr int kSlotStride = 32;
 constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCols + 2;
 
@@ -34,7 +34,7 @@ constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCol
 // L2P terms (sum of the order + 1 of each list-3 cell per target; the order
 // of each particle's leaf).
 enum StatCol { kStSum2, kStSum4, kStPairs, kStM2p, kStM2pTerms, kStL2pTerms, kStCols = 8 };
-constexpr int kStatRows = 64;
+constexpr int kStatRows = 16;
 
 // Header written by device kernels and read back by the host.
 struct DevHdr {
