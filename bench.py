@@ -64,19 +64,23 @@ def eval_flops(st):
 def time_path(torch, solver, dev, steps, warmup):
     """Mean seconds per step and the per-step statistics; the selection of
     the first call (which chooses the leaf threshold on the learned path) is
-    kept on the solver as ``first_selection``."""
-    for k in range(warmup):
-        solver.velocity_device(*dev)
-        if k == 0:
-            solver.first_selection = solver.last_selection()
-    torch.cuda.synchronize()
-    per_step = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        solver.velocity_device(*dev)
-        per_step.append(solver.stats())
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+    kept on the solver as ``first_selection``.  The steps run on a stream of
+    their own (the solver then runs on it directly) and write into output
+    buffers allocated once, as a time-stepping code would."""
+    u, v = torch.empty_like(dev[0]), torch.empty_like(dev[0])
+    with torch.cuda.stream(torch.cuda.Stream(device=dev[0].device)):
+        for k in range(warmup):
+            solver.velocity_device(*dev, u=u, v=v)
+            if k == 0:
+                solver.first_selection = solver.last_selection()
+        torch.cuda.synchronize()
+        per_step = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.velocity_device(*dev, u=u, v=v)
+            per_step.append(solver.stats())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
     return dt, per_step
 
 

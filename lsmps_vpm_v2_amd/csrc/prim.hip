@@ -298,59 +298,82 @@ __global__ void __launch_bounds__(kRsBlock)
 }
 
 // Stable scatter: element order inside a tile is (item j, wave w, lane l),
-// which is the index order; ranks are built in that order.
+// which is the index order.  Each wavefront ranks its keys by digit with
+// 64-lane ballot peer masks (lanes holding the same digit); the per-digit
+// counts of every (item, wave) slot go to LDS, one exclusive scan per digit
+// over the slots (in index order) gives each key its rank inside the tile's
+// digit run.  The tile is then sorted by digit in LDS and written out in that
+// order, so consecutive threads store consecutive positions of a digit's run
+// in the output (the digit base is the scan of the digit totals plus this
+// tile's row prefix).
 __global__ void __launch_bounds__(kRsBlock)
     k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int n,
                  int shift, const uint32_t *table, const uint32_t *totals, int nb) {
     constexpr int kW = kRsBlock / kWave;
-    __shared__ uint32_t goff[256];
-    __shared__ uint32_t wcnt[kW][256];
+    constexpr int kSlots = kRsItems * kW;  // (item, wave) slots in index order
+    __shared__ uint32_t cnt[kSlots][256];
+    __shared__ uint32_t goff[256], lbase[256];
+    __shared__ uint32_t sk[kRsTile], sv[kRsTile];
     __shared__ uint32_t sm[kRsBlock / kWave];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    // digit base (exclusive scan of the digit totals) + this tile's row prefix
-    uint32_t all;
-    const uint32_t dbase = block_excl_scan<kRsBlock>(totals[t], sm, all);
-    goff[t] = dbase + table[(size_t)t * nb + blockIdx.x];
 #pragma unroll
-    for (int ww = 0; ww < kW; ++ww) wcnt[ww][t] = 0;
-    __syncthreads();
+    for (int q = 0; q < kSlots; ++q) cnt[q][t] = 0;
+    // digit base in the output: exclusive scan of the digit totals + row prefix
+    uint32_t all;
+    const uint32_t dbase = block_excl_scan<kRsBlock>(totals[t], sm, all);  // synchronises
+    const uint32_t gbase = dbase + table[(size_t)t * nb + blockIdx.x];
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
     const long long base = (long long)blockIdx.x * kRsTile;
+    const int tile_n = (int)min((long long)kRsTile, (long long)n - base);
+    uint32_t k[kRsItems], v[kRsItems], d[kRsItems], rank[kRsItems];
+    bool valid[kRsItems];
+#pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
         const long long i = base + (long long)j * kRsBlock + t;
-        const bool valid = i < n;
-        const uint32_t k = valid ? kin[i] : 0u;
-        const uint32_t v = valid ? vin[i] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+        valid[j] = i < n;
+        k[j] = valid[j] ? kin[i] : 0u;
+        v[j] = valid[j] ? vin[i] : 0u;
+        d[j] = (k[j] >> shift) & 255u;
+        uint64_t peers = __ballot(valid[j]);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint32_t bit = (d >> b) & 1u;
+        for (int bb = 0; bb < 8; ++bb) {
+            const uint32_t bit = (d[j] >> bb) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const uint32_t rank_w = (uint32_t)__popcll(peers & below);
-        if (valid && (peers & below) == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        uint32_t c[kW], sum = 0;
+        rank[j] = (uint32_t)__popcll(peers & below);
+        // the lowest lane of each digit group writes the group's count
+        if (valid[j] && (peers & below) == 0) cnt[j * kW + w][d[j]] = (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // thread t = digit: exclusive scan over the slots, then the tile-local
+    // digit bases (exclusive scan of the tile's digit counts)
+    uint32_t run = 0;
 #pragma unroll
-        for (int ww = 0; ww < kW; ++ww) {
-            c[ww] = wcnt[ww][t];
-            wcnt[ww][t] = sum;
-            sum += c[ww];
-        }
-        __syncthreads();
-        uint32_t dst = 0;
-        if (valid) dst = goff[d] + wcnt[w][d] + rank_w;
-        __syncthreads();
-        goff[t] += sum;
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t c = cnt[q][t];
+        cnt[q][t] = run;
+        run += c;
+    }
+    uint32_t tile_all;
+    const uint32_t lb = block_excl_scan<kRsBlock>(run, sm, tile_all);  // synchronises
+    lbase[t] = lb;
+    goff[t] = gbase;
+    __syncthreads();
 #pragma unroll
-        for (int ww = 0; ww < kW; ++ww) wcnt[ww][t] = 0;
-        __syncthreads();
-        if (valid) {
-            kout[dst] = k;
-            vout[dst] = v;
-        }
+    for (int j = 0; j < kRsItems; ++j) {
+        if (!valid[j]) continue;
+        const uint32_t pos = lbase[d[j]] + cnt[j * kW + w][d[j]] + rank[j];
+        sk[pos] = k[j];
+        sv[pos] = v[j];
+    }
+    __syncthreads();
+    for (int p = t; p < tile_n; p += kRsBlock) {
+        const uint32_t key = sk[p];
+        const uint32_t dd = (key >> shift) & 255u;
+        const uint32_t dst = goff[dd] + ((uint32_t)p - lbase[dd]);
+        kout[dst] = key;
+        vout[dst] = sv[p];
     }
 }
 

@@ -73,7 +73,8 @@ class FmmSolver:
         if st != 0:
             raise VpmFmmError(st, "vpmfmm_create", "invalid options or device")
         self._h = h
-        self._stream = None  # torch stream the handle runs on (created at the first device call)
+        self._stream = None  # torch stream the handle runs on (set at the first device call)
+        self._side = None    # stream used while torch runs on its legacy default stream
         if o.order_select == ORDER_LEARNED:
             from . import selector as _sel
             self.set_selector(_sel.load_model() if selector is None else selector)
@@ -186,13 +187,22 @@ class FmmSolver:
         self._stream = stream
 
     def _enter(self, t):
-        """Order the handle's stream after the work torch has queued for the
-        device tensors (the calls themselves return with their results
-        complete)."""
+        """Order the call after the work torch has queued for the device
+        tensors (the calls themselves return with their results complete).
+        On a stream the caller created, the handle runs on that stream
+        itself; on torch's legacy default stream it runs on a side stream
+        that first waits for it."""
         import torch
-        if self._stream is None:
-            self.set_stream(torch.cuda.Stream(device=t.device))
-        self._stream.wait_stream(torch.cuda.current_stream(t.device))
+        cur = torch.cuda.current_stream(t.device)
+        if cur.cuda_stream:
+            if self._stream is None or self._stream.cuda_stream != cur.cuda_stream:
+                self.set_stream(cur)
+            return
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=t.device)
+        if self._stream is None or self._stream.cuda_stream != self._side.cuda_stream:
+            self.set_stream(self._side)
+        self._side.wait_stream(cur)
 
     def stats(self) -> dict:
         s = Stats()
