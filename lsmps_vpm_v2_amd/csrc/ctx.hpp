@@ -128,6 +128,7 @@ struct Ctx {
     // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
     // resolved): the previous build's choice as the first guess of the next
     int sort_bits = 32;
+    size_t cell_cap = 0;  // cells the emission has room for before the count is known
 
     // cells
     DBuf pre_level, pre_slot, lev_level, lev2pre;
@@ -175,6 +176,10 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
 // event (the host returns as soon as the GPU reaches it, without the
 // scheduler wake-up of a blocking synchronisation).
 void stream_wait(Ctx &c);
+// The same in two steps: mark the point on the stream, queue more work, then
+// wait for the mark.
+void stream_mark(Ctx &c);
+void stream_mark_wait(Ctx &c);
 
 // Lists of the tree in c, with no host synchronisation: sizes come from the
 // counts of tree_build and the list arrays are sized by proven bounds.

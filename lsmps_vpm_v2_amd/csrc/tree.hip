@@ -405,25 +405,18 @@ __global__ void __launch_bounds__(kSplitBlock)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
     k_emit(const uint8_t *d, const uint8_t *leaf_lvl, const uint32_t *base, int n,
-           uint32_t *pre_level, uint32_t *pre_begin, SplitLevels sl, DevHdr *hdr) {
+           uint32_t *pre_level, uint32_t *pre_begin, uint32_t cap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         pre_level[0] = 0;
         pre_begin[0] = 0;
     }
-    // level offsets / counts and the leaf-level range for the later kernels
-    if (blockIdx.x == 0 && threadIdx.x < kMaxDepth + 2) {
-        hdr->lev_off[threadIdx.x] = sl.lev_off[threadIdx.x];
-        hdr->lev_cnt[threadIdx.x] = sl.lev_cnt[threadIdx.x];
-        if (threadIdx.x == 0) {
-            hdr->leaf_lvl_min = sl.leaf_lvl_min;
-            hdr->leaf_lvl_max = sl.leaf_lvl_max;
-        }
-    }
     if (i >= n) return;
     const int di = d[i], lv = leaf_lvl[i];
     uint32_t slot = 1 + base[i];
-    for (int L = di + 1; L <= lv; ++L, ++slot) {
+    // slots beyond the capacity are dropped; the host sees the cell count and
+    // emits again into grown buffers
+    for (int L = di + 1; L <= lv && slot < cap; ++L, ++slot) {
         pre_level[slot] = (uint32_t)L;
         pre_begin[slot] = (uint32_t)i;
     }
@@ -491,8 +484,17 @@ __global__ void __launch_bounds__(256)
              const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
              uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
              uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
-             uint8_t *leaf01, uint32_t *bs_leaf) {
+             uint8_t *leaf01, uint32_t *bs_leaf, SplitLevels sl, DevHdr *hdr_w) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // level offsets / counts and the leaf-level range for the later kernels
+    if (blockIdx.x == 0 && threadIdx.x < kMaxDepth + 2) {
+        hdr_w->lev_off[threadIdx.x] = sl.lev_off[threadIdx.x];
+        hdr_w->lev_cnt[threadIdx.x] = sl.lev_cnt[threadIdx.x];
+        if (threadIdx.x == 0) {
+            hdr_w->leaf_lvl_min = sl.leaf_lvl_min;
+            hdr_w->leaf_lvl_max = sl.leaf_lvl_max;
+        }
+    }
     uint32_t lf = 0;
     if (j < nc)
         lf = cell_attrs(j, lev_level, lev_begin, key, A, leaf_lvl, n, hdr, c_level, c_flags, c_key,
@@ -875,9 +877,12 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-void stream_wait(Ctx &c) {
+void stream_mark(Ctx &c) {
     if (!c.ev_sync) VPM_HIP(hipEventCreateWithFlags(&c.ev_sync, hipEventDisableTiming));
     VPM_HIP(hipEventRecord(c.ev_sync, c.stream));
+}
+
+void stream_mark_wait(Ctx &c) {
     for (;;) {
         const hipError_t e = hipEventQuery(c.ev_sync);
         if (e == hipSuccess) return;
@@ -885,9 +890,9 @@ void stream_wait(Ctx &c) {
     }
 }
 
-static void sync_hdr(Ctx &c) {
-    VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, c.stream));
-    stream_wait(c);
+void stream_wait(Ctx &c) {
+    stream_mark(c);
+    stream_mark_wait(c);
 }
 
 static void check_flags(const DevHdr &h) {
@@ -950,12 +955,20 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *bs_cells = c.bs_cells.as<uint32_t>((size_t)nbs + 1);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
                                           tp.max_level, leaf_lvl, cnt, c.hdr_d, bs_cells);
+    // The one synchronisation of the build: the header (split rows) is read
+    // back now; the cell-count scan and the cell emission need no host
+    // decision and run while the host waits for the copy and decides.
+    VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, s));
+    stream_mark(c);
     uint32_t *base = c.cbase.as<uint32_t>(n + 1);
     scan_exclusive_bs_u32(cnt, base, n, bs_cells, kSplitBlock, &c.hdr_d->n_cells_pre, true, tmp,
                           s);
+    // emission into buffers sized from the previous build (dropped beyond)
+    const size_t cap = std::max(c.cell_cap, (size_t)n / 4 + 1024);
+    uint32_t *pl = c.pre_level.as<uint32_t>(cap), *pb = c.pre_slot.as<uint32_t>(cap);
+    k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, (uint32_t)cap);
     VPM_LAUNCH_CHECK();
-    // the one synchronisation of the build: sizes of every later launch
-    sync_hdr(c);
+    stream_mark_wait(c);
     DevHdr &hh = *c.hdr_h;
     if (hh.flags & (kFlagNonFinite | kFlagRange)) check_flags(hh);
     // the split rows: leaf-level range, depth overflow, cells per level, leaves
@@ -1006,8 +1019,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
         }
     }
 
-    // 5. cells
-    const int nc = 1 + (int)hh.n_cells_pre;
+    // 5. cells (the cell count from the split rows; the scan's own total is
+    // checked against it at the end of the solve)
+    const int nc = (int)sl.lev_off[kMaxDepth + 1];
     TreeInfo &ti = c.info;
     ti.n = n;
     ti.n_active = (int)hh.n_active;
@@ -1020,12 +1034,18 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     for (int l = 0; l < kMaxDepth + 2; ++l) ti.lev_off[l] = hh.lev_off[l];
     ti.n_leaves = (int)hh.n_leaves_split;
     ti.tot1 = ti.tot3 = 0;
-    if ((int)hh.lev_off[kMaxDepth + 1] != nc || ti.n_leaves < 1 || ti.n_leaves > nc)
-        throw Error(kInternal, "cell counts of the leaf split disagree with the cell emission");
-
-    uint32_t *pl = c.pre_level.as<uint32_t>(nc), *pb = c.pre_slot.as<uint32_t>(nc);
+    if (ti.n_leaves < 1 || ti.n_leaves > nc)
+        throw Error(kInternal, "leaf count of the split out of range");
+    if ((size_t)nc > cap) {
+        // more cells than the emission had room for: let it finish, grow the
+        // buffers and emit again
+        VPM_HIP(hipStreamSynchronize(s));
+        pl = c.pre_level.as<uint32_t>(nc);
+        pb = c.pre_slot.as<uint32_t>(nc);
+        k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, (uint32_t)nc);
+    }
+    c.cell_cap = (size_t)nc + nc / 4;
     uint32_t *ll = c.lev_level.as<uint32_t>(nc), *lb = c.lev2pre.as<uint32_t>(nc);
-    k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, sl, c.hdr_d);
     const bool alt2 = radix_sort_pairs(pl, pb, ll, lb, nc, 0, 8, tmp, s);
     const uint32_t *lev_level = alt2 ? ll : pl;
     const uint32_t *lev_begin = alt2 ? lb : pb;
@@ -1039,7 +1059,8 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *bs_leaf = c.bs_leaf.as<uint32_t>((size_t)div_up(nc, 256) + 1);
     k_cells1<<<div_up(nc, 256), 256, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
-                                             c_src, c_cx, c_cy, c_child, leaf01, bs_leaf);
+                                             c_src, c_cx, c_cy, c_child, leaf01, bs_leaf, sl,
+                                             c.hdr_d);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
 
     // leaf cells in cell order (scan of the leaf flags, compacted)
@@ -1106,8 +1127,8 @@ void tree_lists_absorb(Ctx &c) {
     const DevHdr &hh = *c.hdr_h;
     TreeInfo &ti = c.info;
     check_flags(hh);
-    if ((int)hh.n_leaves != ti.n_leaves)
-        throw Error(kInternal, "leaf count of the cell build disagrees with the leaf split");
+    if ((int)hh.n_leaves != ti.n_leaves || (int)hh.n_cells_pre + 1 != ti.n_cells)
+        throw Error(kInternal, "cell or leaf count of the cell build disagrees with the leaf split");
     if ((size_t)hh.tot1 >= ti.cap1 || (size_t)hh.tot3 >= ti.cap3)
         throw Error(kInternal, "interaction lists exceed their bounds");
     ti.tot1 = (int)hh.tot1;

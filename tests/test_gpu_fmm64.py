@@ -111,3 +111,20 @@ def test_errors(gpu):
     f = _case("uniform", 500, 3)
     ex, ey = s.field(f["x"], f["y"], f["q"], f["active"])
     assert np.all(np.isfinite(ex)) and np.all(np.isfinite(ey))
+
+
+def test_tiny_leaf_threshold_many_cells(gpu, oracle):
+    """Leaf thresholds of one particle give about 1.3 cells per particle, more
+    than the cell emission has room for before the count is read back on a
+    fresh handle (n / 4 + 1024): the build grows its buffers and emits again.
+    Tree counts and the field equal the oracle's."""
+    f = _case("uniform", 6000, 13)
+    t = oracle.Tree(f["x"], f["y"], f["active"], oracle.opts(p=6, max_par=1, max_src=1))
+    s = FmmSolver(p=6, max_par=1, max_src=1)
+    ex, ey = s.field(f["x"], f["y"], f["q"], f["active"])
+    nc, nl, _, _ = t.counts()
+    assert nc > 6000 // 4 + 1024
+    assert s.tree_counts() == t.counts()
+    ex0, ey0 = oracle.fmm_field(f["x"], f["y"], f["q"], f["active"],
+                                oracle.opts(p=6, max_par=1, max_src=1))
+    assert rel_l2(ex, ey, ex0, ey0) <= TOL
