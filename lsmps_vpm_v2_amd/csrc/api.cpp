@@ -125,6 +125,7 @@ struct vpmfmm_handle_s {
     DBuf judge_a, judge_b, judge_part;
     vpmfmm_selection last_sel{};
     bool judged_reject_now = false;  // the judge of this call rejected the learned orders
+    bool concurrent_marks = false;   // the last run built the lists beside the upward pass
     MlpWork mlp;
     DBuf mlp_grad;
     // inspection cache of the last tree (host copies)
@@ -275,8 +276,10 @@ void reselect_threshold(vpmfmm_handle *h, int n, const double *x, const double *
     h->sel_cached = false;
 }
 
+// Tree and (learned path) orders; the lists too unless the caller builds them
+// itself (run() builds them concurrently with the upward pass).
 void build(vpmfmm_handle *h, int n, const double *x, const double *y, const Sources &q,
-           const uint8_t *a) {
+           const uint8_t *a, bool lists = true) {
     std::string msg;
     if (opts_check(h->o, msg)) throw Error(kBadArg, msg);
     if (n < 1) throw Error(kBadArg, "n must be >= 1");
@@ -312,9 +315,33 @@ void build(vpmfmm_handle *h, int n, const double *x, const double *y, const Sour
     h->sel_fresh = lrn && !reuse;
     h->ctx.orders = (lrn && !h->judge_reject) ? learned_slot(h) : fixed_slot(h);
     mark(h, 1);
-    tree_lists(h->ctx);
-    mark(h, 2);
+    h->concurrent_marks = false;
+    if (lists) {
+        tree_lists(h->ctx);
+        mark(h, 2);
+    }
     h->last_sel.threshold = tp.max_par;
+}
+
+// The mixed upward pass on the side stream while the list build runs on the
+// main stream; the main stream waits for it before M2L.
+void upward_with_lists(vpmfmm_handle *h, const Mixed32 &mx, const FmmMarks &mk) {
+    Ctx &c = h->ctx;
+    if (!c.side) VPM_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    if (!c.ev_fork) VPM_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    if (!c.ev_join) VPM_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    fmm32_prepare(c, mx);
+    VPM_HIP(hipEventRecord(c.ev_fork, c.stream));
+    VPM_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    // launches interleaved so that both streams have work early
+    tree_lists_begin(c);
+    fmm32_upward(c, mx, c.side, mk);
+    VPM_HIP(hipEventRecord(c.ev_join, c.side));
+    tree_lists_end(c);
+    mark(h, 2);
+    VPM_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    mark(h, 7);
+    h->concurrent_marks = true;
 }
 
 void read_hdr(Ctx &c) {
@@ -359,8 +386,9 @@ void finish_stats(vpmfmm_handle *h, bool fmm) {
         s.ms_tree = span(h, 0, 1);
         s.ms_lists = span(h, 1, 2);
         if (fmm) {
-            s.ms_upward = span(h, 2, 3);
-            s.ms_m2l = span(h, 3, 4);
+            // concurrent list build and upward pass: both from the fork
+            s.ms_upward = h->concurrent_marks ? span(h, 1, 3) : span(h, 2, 3);
+            s.ms_m2l = h->concurrent_marks ? span(h, 7, 4) : span(h, 3, 4);
             s.ms_downward = span(h, 4, 5);
             s.ms_eval = span(h, 5, 6);
             s.ms_total = span(h, 0, 6);
@@ -453,7 +481,8 @@ void report_selection(vpmfmm_handle *h, bool lrn_run) {
 void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Sources &q,
          const uint8_t *a, double *oa, double *ob, bool velocity) {
     if (!(q.q || (q.s && q.omega)) || !oa || !ob) throw Error(kBadArg, "null pointer");
-    build(h, n, x, y, q, a);
+    const bool mixed = h->o.precision == VPMFMM_MIXED;
+    build(h, n, x, y, q, a, !mixed);
     Ctx &c = h->ctx;
     FmmMarks mk{h->o.timing ? h->ev : nullptr};
     const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
@@ -464,7 +493,8 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Source
     if (h->o.precision == VPMFMM_MIXED) {
         lrn = learned(h) && !h->judge_reject;
         Mixed32 mx = mixed_setup(h);
-        fmm32_run(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
+        upward_with_lists(h, mx, mk);
+        fmm32_rest(c, mx, h->o.list4_mode, mask, oa, ob, velocity, h->o.u_inf, h->o.v_inf, mk);
         // the fixed-order judge (same tree): every fresh selection, and every
         // judge_every-th learned call
         const bool judge_due =
@@ -645,6 +675,12 @@ int vpmfmm_destroy(vpmfmm_handle *h) {
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->ctx.ev_sync) (void)hipEventDestroy(h->ctx.ev_sync);
+    if (h->ctx.ev_fork) (void)hipEventDestroy(h->ctx.ev_fork);
+    if (h->ctx.ev_join) (void)hipEventDestroy(h->ctx.ev_join);
+    if (h->ctx.side) {
+        (void)hipStreamSynchronize(h->ctx.side);
+        (void)hipStreamDestroy(h->ctx.side);
+    }
     if (h->ctx.hdr_d) (void)hipFree(h->ctx.hdr_d);
     if (h->ctx.hdr_h) (void)hipHostFree(h->ctx.hdr_h);
     h->ctx.hdr_d = nullptr;

@@ -112,6 +112,10 @@ struct TreeInfo {
 struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev_sync = nullptr;  // completion marker of the per-solve synchronisations
+    // second stream (default priority) for the upward pass, which runs
+    // concurrently with the list build; fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevHdr *hdr_d = nullptr;  // device header
     DevHdr *hdr_h = nullptr;  // pinned host mirror
 
@@ -184,6 +188,10 @@ void stream_mark_wait(Ctx &c);
 // Lists of the tree in c, with no host synchronisation: sizes come from the
 // counts of tree_build and the list arrays are sized by proven bounds.
 void tree_lists(Ctx &c);
+// The same in two parts (neighbour table and lists 2 / 4; lists 1 / 3), so
+// that other work can be enqueued between them.
+void tree_lists_begin(Ctx &c);
+void tree_lists_end(Ctx &c);
 // Host copy of the list totals and the checks that need the list build's
 // header (call after the header has been read back).
 void tree_lists_absorb(Ctx &c);

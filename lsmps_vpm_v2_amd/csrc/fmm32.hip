@@ -969,39 +969,45 @@ int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
     return ntiles;
 }
 
-void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
-               double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
-    hipStream_t s = c.stream;
-    const TreeInfo &ti = c.info;
-    const int nc = ti.n_cells, nl = ti.n_leaves, n = ti.n;
-    const int kst = 2 * (mx.pmax + 1), PM1 = mx.pmax + 1;
-    float *ahat = c.ak.as<float>((size_t)nc * kst), *bhat = c.bk.as<float>((size_t)nc * kst);
-    // no clearing needed: P2M / M2M write every coefficient a cell's readers use
-    // (zero for inactive cells), the list-2 M2L writes b^_1..p of every cell of
-    // level >= 2 before list 4 and L2L add to them
-    float2 *oxy = c.oxy.as<float2>(n);
-    float *qf = c.qf.as<float>(n);
-    const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
-    const uint32_t *c_key = (const uint32_t *)c.c_key.p;
-    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
-    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
-    const int *c_child = (const int *)c.c_child.p, *c_parent = (const int *)c.c_parent.p;
-    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
-    const uint8_t *as = (const uint8_t *)c.acts.p;
-
-    k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
-        leaf_idx, nl, c_level, c_flags, c_begin, c_end, c_cx, c_cy, (const double *)c.xs.p,
-        (const double *)c.ys.p, (const double *)c.qs.p, as, mx.lt, kst, oxy, qf, ahat);
-    // split levels: K = first level with >= kSubRoots cells (subtrees below),
-    // T = first level with >= kTopCells cells (one workgroup above T)
+// Split levels of the three-tier passes: K = first level with >= kSubRoots
+// cells (subtrees below), T = first level with >= kTopCells cells (one
+// workgroup above T).
+static void tier_levels(const TreeInfo &ti, int &K, int &T) {
     const int maxL = ti.max_level;
-    int K = maxL, T = maxL;
+    K = T = maxL;
     for (int L = maxL; L >= 1; --L) {
         const int cnt = (int)(ti.lev_off[L + 1] - ti.lev_off[L]);
         if (cnt >= kSubRoots) K = L;
         if (cnt >= kTopCells) T = L;
     }
     T = std::min(T, K);
+}
+
+// Upward pass (P2M, M2M) on stream s; the buffers come from fmm32_prepare.
+// No clearing is needed: P2M / M2M write every coefficient a cell's readers
+// use (zero for inactive cells), the list-2 M2L writes b^_1..p of every cell
+// of level >= 2 before list 4 and L2L add to them.
+void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) {
+    const TreeInfo &ti = c.info;
+    const int nl = ti.n_leaves;
+    const int kst = 2 * (mx.pmax + 1);
+    float *ahat = (float *)c.ak.p;
+    float2 *oxy = (float2 *)c.oxy.p;
+    float *qf = (float *)c.qf.p;
+    const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
+    const uint32_t *c_key = (const uint32_t *)c.c_key.p;
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
+    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
+    const int *c_child = (const int *)c.c_child.p;
+    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
+    const uint8_t *as = (const uint8_t *)c.acts.p;
+
+    k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
+        leaf_idx, nl, c_level, c_flags, c_begin, c_end, c_cx, c_cy, (const double *)c.xs.p,
+        (const double *)c.ys.p, (const double *)c.qs.p, as, mx.lt, kst, oxy, qf, ahat);
+    const int maxL = ti.max_level;
+    int K, T;
+    tier_levels(ti, K, T);
     const int nK = (int)(ti.lev_off[K + 1] - ti.lev_off[K]);
     if (K < maxL)
         k_m2m_subtree<<<nK, kSubBlock, 0, s>>>(K, maxL, c.hdr_d, c_key, c_flags, c_child, mx.lt,
@@ -1015,6 +1021,30 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
         k_m2m_top<<<1, kTopBlock, 0, s>>>(std::min(T, K) - 1, c.hdr_d, c_flags, c_child, mx.lt,
                                           mx.m2m, kst, ahat);
     mk.mark(3, s);
+    VPM_LAUNCH_CHECK();
+}
+
+void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
+                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
+    hipStream_t s = c.stream;
+    const TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves, n = ti.n;
+    const int kst = 2 * (mx.pmax + 1), PM1 = mx.pmax + 1;
+    float *ahat = (float *)c.ak.p, *bhat = (float *)c.bk.p;
+    const float2 *oxy = (const float2 *)c.oxy.p;
+    const float *qf = (const float *)c.qf.p;
+    const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
+    const uint32_t *c_key = (const uint32_t *)c.c_key.p;
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
+    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
+    const int *c_child = (const int *)c.c_child.p;
+    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
+    const uint8_t *as = (const uint8_t *)c.acts.p;
+    const int maxL = ti.max_level;
+    int K, T;
+    tier_levels(ti, K, T);
+    const int nK = (int)(ti.lev_off[K + 1] - ti.lev_off[K]);
+    (void)n;
     // M2L list 2 on MFMA: every tile in one launch
     if (mx.ntiles > 0) {
         M2lTabs tabs;
@@ -1058,6 +1088,22 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
     k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(ea, nl);
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
+}
+
+void fmm32_prepare(Ctx &c, const Mixed32 &mx) {
+    const TreeInfo &ti = c.info;
+    const size_t kst = 2 * (size_t)(mx.pmax + 1);
+    c.ak.as<float>((size_t)ti.n_cells * kst);
+    c.bk.as<float>((size_t)ti.n_cells * kst);
+    c.oxy.as<float2>(ti.n);
+    c.qf.as<float>(ti.n);
+}
+
+void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
+               double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
+    fmm32_prepare(c, mx);
+    fmm32_upward(c, mx, c.stream, mk);
+    fmm32_rest(c, mx, list4_mode, stage_mask, out_a, out_b, velocity, u_inf, v_inf, mk);
 }
 
 }  // namespace vpm

@@ -36,6 +36,14 @@ int mixed_tile_count(const Ctx &c);
 int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
                       uint8_t *tile_p);
 
+// The mixed FMM of the tree in c: fmm32_prepare (host: sizes the expansion
+// and particle buffers), fmm32_upward (P2M, M2M; on any stream, it needs the
+// tree only), fmm32_rest (M2L, list 4, L2L, evaluation on c.stream; needs
+// the lists and the upward pass).  fmm32_run does all three on c.stream.
+void fmm32_prepare(Ctx &c, const Mixed32 &mx);
+void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk);
+void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
+                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 

@@ -1072,16 +1072,20 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
 }
 
 void tree_lists(Ctx &c) {
+    tree_lists_begin(c);
+    tree_lists_end(c);
+}
+
+void tree_lists_begin(Ctx &c) {
     hipStream_t s = c.stream;
     TreeInfo &ti = c.info;
-    const int nc = ti.n_cells, nl = ti.n_leaves;
+    const int nc = ti.n_cells;
     if (!c.orders) throw Error(kInternal, "expansion orders not set before the list build");
     uint8_t *c_level = (uint8_t *)c.c_level.p, *c_flags = (uint8_t *)c.c_flags.p;
     uint32_t *c_key = (uint32_t *)c.c_key.p;
     double *c_cx = (double *)c.c_cx.p, *c_cy = (double *)c.c_cy.p;
     int *c_parent = (int *)c.c_parent.p, *c_child = (int *)c.c_child.p;
     int *nbr = c.c_nbr.as<int>(9 * (size_t)nc);
-    uint32_t *tmp = (uint32_t *)c.tmp.p;
     k_nbr<<<div_up(9LL * nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, nbr);
 
     int *l2 = c.l2.as<int>(27 * (size_t)nc), *l4 = c.l4.as<int>(8 * (size_t)nc);
@@ -1090,10 +1094,22 @@ void tree_lists(Ctx &c) {
     // n2 / n4 of every cell of level >= 2 are written by k_m2l_lists; levels 0
     // and 1 have no lists 2 / 4 and nothing reads their entries
     const int nb2 = nc > j0 ? (int)div_up((long long)(nc - j0) * kM2lLanes, 256LL) : 0;
-    const int nbl = (int)div_up((long long)nl * kLeafLanes, 256LL);
     if (nb2 > 0)
         k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
                                         c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d);
+    VPM_LAUNCH_CHECK();
+}
+
+void tree_lists_end(Ctx &c) {
+    hipStream_t s = c.stream;
+    TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves;
+    uint8_t *c_level = (uint8_t *)c.c_level.p, *c_flags = (uint8_t *)c.c_flags.p;
+    double *c_cx = (double *)c.c_cx.p, *c_cy = (double *)c.c_cy.p;
+    int *c_child = (int *)c.c_child.p;
+    const int *nbr = (const int *)c.c_nbr.p;
+    uint32_t *tmp = (uint32_t *)c.tmp.p;
+    const int nbl = (int)div_up((long long)nl * kLeafLanes, 256LL);
     const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
     uint32_t *cnt1 = c.cnt1.as<uint32_t>(nl + 1), *cnt3 = c.cnt3.as<uint32_t>(nl + 1);
     uint32_t *off1 = c.off1.as<uint32_t>(nl + 1), *off3 = c.off3.as<uint32_t>(nl + 1);
