@@ -16,7 +16,9 @@ ratio reported.  Errors: relative L2 of the rotational velocity against the
 FP64 direct sum on the GPU (20,000 seeded targets, all sources) and against
 the reference's own CPU FMM (all particles).  The CPU baseline is that
 reference FMM (oracle/_ref, built from the reference sources) on this host's
-cores.
+cores.  ``inviscid_loop`` (not the metric) times the loop step of
+vpmfmm_step_device on the same field: Euler advection, then the velocity at
+the moved particles, a new tree every step.
 """
 from __future__ import annotations
 
@@ -82,6 +84,26 @@ def time_path(torch, solver, dev, steps, warmup):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
     return dt, per_step
+
+
+def time_loop(torch, solver, dev, steps, warmup, dt=2e-3):
+    """Mean milliseconds per step of the inviscid loop (vpmfmm_step_device:
+    Euler advection with the current velocity, then the velocity at the moved
+    particles) on copies of the positions, so each step builds a new tree."""
+    x, y = dev[0].clone(), dev[1].clone()
+    u, v = torch.empty_like(x), torch.empty_like(x)
+    with torch.cuda.stream(torch.cuda.Stream(device=x.device)):
+        solver.velocity_device(x, y, dev[2], dev[3], dev[4], u=u, v=v)
+        for _ in range(warmup):
+            solver.step_device(x, y, dev[2], dev[3], dev[4], u, v, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.step_device(x, y, dev[2], dev[3], dev[4], u, v, dt)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+    return {"ms_per_step": sec * 1e3, "steps": steps, "dt": dt, "scheme": "euler",
+            "particles_moved_per_s": x.numel() / sec}
 
 
 def rel_l2(u, v, ur, vr):
@@ -151,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="skip the inviscid-loop timing (advection + velocity per step)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -193,6 +217,12 @@ def main():
             f"ms {sel['pred_ms']:.3f}")
     u, v = solver.velocity_device(*dev)
     u_h, v_h = u.cpu().numpy() - solver.opts.u_inf, v.cpu().numpy() - solver.opts.v_inf
+
+    loop = None
+    if not args.no_loop:
+        loop = time_loop(torch, solver, dev, args.steps, args.warmup)
+        log(f"[bench] inviscid loop: {loop['ms_per_step']:.3f} ms/step "
+            f"(advection + velocity, tree rebuilt every step)")
 
     out_fp64 = None
     if args.path != "fp64" and not args.no_fp64:
@@ -255,6 +285,8 @@ def main():
         if first and first["reselected"]:
             # once per run (and when n changes by > 10 %): not part of a step
             out["selection"]["threshold_choice_ms"] = first["ms_reselect"]
+    if loop:
+        out["inviscid_loop"] = loop
     if out_fp64:
         out["fp64_path"] = {k: val for k, val in out_fp64.items() if not k.startswith("_")}
         out["speedup_vs_fp64"] = out["value"] / out_fp64["value"]

@@ -165,6 +165,29 @@ int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const d
                            const double *s, const double *omega, const uint8_t *active,
                            double *u, double *v);
 
+/* Advection schemes of advection::main_advection (main_advection.cpp:6-39). */
+typedef enum vpmfmm_advect_scheme {
+    VPMFMM_ADVECT_EULER = 0, /* advection_euler, :44-61 (the scheme the reference compiles) */
+    VPMFMM_ADVECT_RK2 = 1    /* advection_rk2, :64-83 (scales the current velocity only)    */
+} vpmfmm_advect_scheme;
+
+/* Advection sub-step, device pointers, in place on x, y
+ * (advection::main_advection, main_advection.cpp:6-83):
+ *   EULER: x += u dt, y += v dt;
+ *   RK2:   x += (u + 2 u/2 + 2 u/4 + u/4) dt / 6, the same for y with v.
+ * Rounded as the reference's host loop (no FMA contraction). */
+int vpmfmm_advect_device(vpmfmm_handle *h, int32_t n, double *x, double *y, const double *u,
+                         const double *v, double dt, int32_t scheme);
+
+/* One step of the reference's iteration loop restricted to the velocity path
+ * (main.cpp:296-429 without penalization, diffusion and remeshing): the
+ * advection sub-step with the current (u, v) (main.cpp:373), then the new
+ * velocity at the moved particles into (u, v) (get_velocity, main.cpp:410).
+ * Device pointers; x, y, u, v are updated in place. */
+int vpmfmm_step_device(vpmfmm_handle *h, int32_t n, double *x, double *y, const double *s,
+                       const double *omega, const uint8_t *active, double *u, double *v,
+                       double dt, int32_t scheme);
+
 /* FP64 O(N^2) direct sum of the same kernel on the GPU, device pointers: the
  * field at targets tidx[0..nt) (indices into x/y; j == i skipped), or at all
  * n particles when tidx is NULL.  src_mask (may be NULL) selects sources.

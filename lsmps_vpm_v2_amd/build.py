@@ -40,6 +40,7 @@ SOURCES = {
     "fmm64.hip": [],
     "fmm32.hip": [],
     "select.hip": [],
+    "advect.hip": ["-ffp-contract=off"],
     "api.cpp": [],
 }
 

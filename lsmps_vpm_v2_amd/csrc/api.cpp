@@ -768,6 +768,36 @@ int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *
     });
 }
 
+static_assert(kAdvectEuler == VPMFMM_ADVECT_EULER && kAdvectRk2 == VPMFMM_ADVECT_RK2,
+              "advection scheme codes");
+
+int vpmfmm_advect_device(vpmfmm_handle *h, int32_t n, double *x, double *y, const double *u,
+                         const double *v, double dt, int32_t scheme) {
+    return guarded(h, [&] {
+        if (n < 1 || !x || !y || !u || !v) throw Error(kBadArg, "bad size or null pointer");
+        if (scheme != VPMFMM_ADVECT_EULER && scheme != VPMFMM_ADVECT_RK2)
+            throw Error(kBadArg, "unknown advection scheme");
+        VPM_HIP(hipSetDevice(h->o.device));
+        advect(n, x, y, u, v, dt, scheme, h->ctx.stream);
+        VPM_HIP(hipStreamSynchronize(h->ctx.stream));
+    });
+}
+
+int vpmfmm_step_device(vpmfmm_handle *h, int32_t n, double *x, double *y, const double *s,
+                       const double *omega, const uint8_t *active, double *u, double *v,
+                       double dt, int32_t scheme) {
+    return guarded(h, [&] {
+        if (n < 1 || !x || !y || !s || !omega || !u || !v)
+            throw Error(kBadArg, "bad size or null pointer");
+        if (scheme != VPMFMM_ADVECT_EULER && scheme != VPMFMM_ADVECT_RK2)
+            throw Error(kBadArg, "unknown advection scheme");
+        VPM_HIP(hipSetDevice(h->o.device));
+        // same stream: the tree build reads the moved positions
+        advect(n, x, y, u, v, dt, scheme, h->ctx.stream);
+        run(h, n, x, y, Sources{nullptr, s, omega}, active, u, v, true);
+    });
+}
+
 int vpmfmm_direct_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
                          const double *q, const uint8_t *src_mask, int32_t nt, const int32_t *tidx,
                          double *ex, double *ey) {

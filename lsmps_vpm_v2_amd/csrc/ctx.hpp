@@ -211,6 +211,12 @@ struct FmmMarks {
 void fmm64_run(Ctx &c, int p, int list4_mode, int stage_mask, double *out_a, double *out_b,
                bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 
+// Particle advection sub-step (advect.hip): Euler x += u dt, or the
+// reference's velocity-scaling "rk2" update.
+constexpr int kAdvectEuler = 0, kAdvectRk2 = 1;
+void advect(int n, double *x, double *y, const double *u, const double *v, double dt, int scheme,
+            hipStream_t s);
+
 // FP64 direct sum on the GPU (accuracy reference, not part of the FMM).
 void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,
               int nt, const int *tidx, double *ex, double *ey, hipStream_t s);

@@ -254,6 +254,27 @@ class FmmSolver:
                                                    _dptr(v)), "vpmfmm_velocity_device")
         return u, v
 
+    _SCHEMES = {"euler": 0, "rk2": 1}
+
+    def advect_device(self, x, y, u, v, dt, scheme="euler"):
+        """Advection sub-step in place on device tensors x, y
+        (main_advection.cpp:6-83; "euler" is the scheme the reference compiles)."""
+        n = x.numel()
+        self._enter(x)
+        self._check(self._L.vpmfmm_advect_device(self._h, n, _dptr(x), _dptr(y), _dptr(u),
+                                                 _dptr(v), float(dt), self._SCHEMES[scheme]),
+                    "vpmfmm_advect_device")
+
+    def step_device(self, x, y, s, omega, active, u, v, dt, scheme="euler"):
+        """One loop step on device tensors: advect x, y with (u, v), then the
+        velocity at the moved particles into (u, v) (main.cpp:373, 410)."""
+        n = x.numel()
+        self._enter(x)
+        self._check(self._L.vpmfmm_step_device(self._h, n, _dptr(x), _dptr(y), _dptr(s),
+                                               _dptr(omega), _dptr(active), _dptr(u), _dptr(v),
+                                               float(dt), self._SCHEMES[scheme]),
+                    "vpmfmm_step_device")
+
     def direct_device(self, x, y, q, src_mask=None, targets=None):
         """FP64 direct sum on the GPU at `targets` (int32 device tensor) or all."""
         n = x.numel()

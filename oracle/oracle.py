@@ -264,6 +264,40 @@ def velocity(x, y, s, omega, active, o: OrcOpts | None = None, u_inf=1.0, v_inf=
     return u, v
 
 
+def advect(x, y, u, v, dt, scheme="euler"):
+    """Advection sub-step, returning new (x, y) (main_advection.cpp:44-83).
+
+    "euler": x + u dt (:44-61, the scheme the reference compiles).  "rk2":
+    x + (u + 2 u1 + 2 u2 + u3) dt / 6 with u1 = u/2, u2 = u1/2, u3 = u2
+    (:64-83), evaluated left to right as the reference's loop does; numpy
+    rounds every operation once (no FMA), like the reference's host build."""
+    x, y, u, v = _d(x), _d(y), _d(u), _d(v)
+    if scheme == "rk2":
+        u1 = u / 2
+        u2 = u1 / 2
+        u3 = u2
+        v1 = v / 2
+        v2 = v1 / 2
+        v3 = v2
+        return x + (u + 2 * u1 + 2 * u2 + u3) * dt / 6, y + (v + 2 * v1 + 2 * v2 + v3) * dt / 6
+    if scheme != "euler":
+        raise ValueError(scheme)
+    return x + u * dt, y + v * dt
+
+
+def inviscid_loop(x, y, s, omega, active, dt, steps, o: OrcOpts | None = None, u_inf=1.0,
+                  v_inf=0.0, scheme="euler"):
+    """The reference's loop restricted to the velocity path (main.cpp:296-429:
+    advection at :373, the velocity solve at :410), from the velocity at the
+    initial positions.  Returns (x, y, u, v) after `steps` steps."""
+    u, v = velocity(x, y, s, omega, active, o, u_inf, v_inf)
+    x, y = _d(x), _d(y)
+    for _ in range(steps):
+        x, y = advect(x, y, u, v, dt, scheme)
+        u, v = velocity(x, y, s, omega, active, o, u_inf, v_inf)
+    return x, y, u, v
+
+
 def direct_field(x, y, q, targets=None, src_mask=None):
     """Singular FP64 direct sum (j == i skipped) at `targets` (indices) or all."""
     L = lib()
