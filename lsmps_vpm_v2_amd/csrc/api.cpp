@@ -14,11 +14,25 @@
 
 namespace vpm {
 
+// Buffers replaced by larger ones during a call.  hipFree waits for the
+// device, so freeing them at once would stall the launches already queued;
+// they are freed when the call returns (free_retired, from guarded()).
+static thread_local std::vector<void *> g_retired;
+
+static void free_retired() {
+    for (void *q : g_retired) (void)hipFree(q);
+    g_retired.clear();
+}
+
 void *DBuf::get(size_t bytes) {
     if (bytes <= cap && p) return p;
-    release();
-    size_t nb = std::max(bytes, cap + cap / 2);
+    // headroom, so that sizes that creep up from call to call (a moving
+    // particle field) regrow rarely
+    size_t nb = std::max(bytes + bytes / 4, cap + cap / 2);
     nb = (nb + 255) & ~(size_t)255;
+    if (p) g_retired.push_back(p);
+    p = nullptr;
+    cap = 0;
     if (hipMalloc(&p, nb) != hipSuccess) {
         p = nullptr;
         cap = 0;
@@ -165,17 +179,19 @@ int opts_check(const vpmfmm_opts &o, std::string &msg) {
 template <class F>
 int guarded(vpmfmm_handle *h, F &&f) {
     if (!h) return VPMFMM_BAD_ARG;
+    int st = VPMFMM_OK;
     try {
         h->err.clear();
         f();
-        return VPMFMM_OK;
     } catch (const Error &e) {
         h->err = e.what();
-        return e.code;
+        st = e.code;
     } catch (const std::exception &e) {
         h->err = e.what();
-        return VPMFMM_INTERNAL;
+        st = VPMFMM_INTERNAL;
     }
+    free_retired();
+    return st;
 }
 
 void ensure_tables(vpmfmm_handle *h) {
