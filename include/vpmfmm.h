@@ -69,7 +69,15 @@ typedef struct vpmfmm_opts {
                                p on the same tree and compares; a difference above the
                                model's target returns the fixed-order result and keeps
                                the fixed order until the next re-selection; -1 = never  */
-    int32_t reserved[4];
+    int32_t icutoff;        /* near-field (P2P) kernel: 0 = singular, the type-2 FMM
+                               (fmm2D.cpp:164-173); 1 / 2 / 3 = the high-order
+                               algebraic / Gaussian / super-Gaussian regularization of
+                               regul_func_2d (regularization_function_2d.cpp:11-37,
+                               Pars::icutoff) at sij^2 = (si^2 + sj^2) / 4, as the
+                               type-1 near field (subroutine_all_fmm_2d.cpp:594-627):
+                               pairs at r = 0 add nothing.  The far field stays
+                               singular.  > 0 needs the sizes: velocity entries only */
+    int32_t reserved[3];
 } vpmfmm_opts;
 
 /* Expansion-order source of the mixed path. */

@@ -42,7 +42,7 @@ class Opts(C.Structure):
         ("u_inf", C.c_double), ("v_inf", C.c_double), ("device", C.c_int32),
         ("timing", C.c_int32), ("stage_mask", C.c_int32), ("order_select", C.c_int32),
         ("reselect_every", C.c_int32), ("judge_every", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("icutoff", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

@@ -28,7 +28,7 @@ void *DBuf::get(size_t bytes) {
     if (bytes <= cap && p) return p;
     // headroom, so that sizes that creep up from call to call (a moving
     // particle field) regrow rarely
-    size_t nb = std::max(bytes + bytes / 4, cap + cap / 2);
+    size_t nb = std::max(bytes + bytes / 2, cap + cap / 2);
     nb = (nb + 255) & ~(size_t)255;
     if (p) g_retired.push_back(p);
     p = nullptr;
@@ -172,6 +172,7 @@ int opts_check(const vpmfmm_opts &o, std::string &msg) {
         msg = "order_select must be 0 (fixed) or 1 (learned)";
     else if (o.reselect_every < 0) msg = "reselect_every must be >= 0";
     else if (o.judge_every < -1) msg = "judge_every must be >= -1";
+    else if (o.icutoff < 0 || o.icutoff > 3) msg = "icutoff must be in [0, 3]";
     else return VPMFMM_OK;
     return VPMFMM_BAD_ARG;
 }
@@ -497,6 +498,10 @@ void report_selection(vpmfmm_handle *h, bool lrn_run) {
 void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Sources &q,
          const uint8_t *a, double *oa, double *ob, bool velocity) {
     if (!(q.q || (q.s && q.omega)) || !oa || !ob) throw Error(kBadArg, "null pointer");
+    if (h->o.icutoff > 0 && !q.s)
+        throw Error(kBadArg, "the regularized near field (icutoff > 0) needs the particle "
+                             "sizes: use a velocity entry point");
+    h->ctx.icut = h->o.icutoff;
     const bool mixed = h->o.precision == VPMFMM_MIXED;
     build(h, n, x, y, q, a, !mixed);
     Ctx &c = h->ctx;

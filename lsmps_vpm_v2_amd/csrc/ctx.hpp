@@ -123,6 +123,10 @@ struct Ctx {
     DBuf x_in, y_in, q_in, a_in, ex_out, ey_out;  // staging for host-pointer calls
     DBuf key_a, key_b, val_a, val_b;
     DBuf xs, ys, qs, acts;
+    // near-field kernel of this call (0 singular, 1..3 the regularized
+    // kernels of regul_func_2d) and, when regularized, the sorted sizes
+    int icut = 0;
+    DBuf ss;
     uint32_t *key_sorted = nullptr;   // points into key_a or key_b after the sort
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
     DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;

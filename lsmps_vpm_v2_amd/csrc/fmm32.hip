@@ -649,16 +649,44 @@ __device__ __forceinline__ float4 far_source() {
 
 // P2P of one target over the staged sources g, g + G, ... (st = stage + g,
 // nu = padded round length / G).
-template <int G, bool SELF>
-__device__ __forceinline__ void p2p_round(const float4 *st, int nu, float2 ti, uint32_t i,
-                                          float &ex, float &ey) {
+// Regularization factor of a near-field pair in FP32 (regul_func_2d,
+// regularization_function_2d.cpp:11-37) at rho^2 = r^2 / sij^2.  Where the
+// reference's 1 - exp form cancels (small h = rho^2 / 2), the Taylor series
+// of the same function is used: Gaussian 1 - e^-h below h = 1/8, super-
+// Gaussian 1 - (1 + h) e^-h below h = 1/2 (truncation under 5e-7 relative).
+template <int ICUT>
+__device__ __forceinline__ float regul32(float rho2) {
+    const float x = fminf(rho2, 1e15f);  // padding sources: far, finite
+    if (ICUT == 1) return x * (x + 2.f) * __builtin_amdgcn_rcpf((x + 1.f) * (x + 1.f));
+    const float h = 0.5f * x;
+    const float e = __expf(-h);
+    if (ICUT == 2) {
+        const float ser = h * (1.f - h * (0.5f - h * (1.f / 6 - h * (1.f / 24 - h * (1.f / 120)))));
+        return h < 0.125f ? ser : 1.f - e;
+    }
+    const float ser =
+        h * h * (0.5f - h * (1.f / 3 - h * (1.f / 8 - h * (1.f / 30 -
+        h * (1.f / 144 - h * (1.f / 840 - h * (1.f / 5760)))))));
+    return h < 0.5f ? ser : 1.f - (1.f + h) * e;
+}
+
+template <int G, bool SELF, int ICUT>
+__device__ __forceinline__ void p2p_round(const float4 *st, const float *sst, int nu, float2 ti,
+                                          float ts4, uint32_t i, float &ex, float &ey) {
 #pragma unroll 4
     for (int u = 0; u < nu; ++u) {
         const float4 sv = st[u * G];
         const float dx = ti.x - sv.x, dy = ti.y - sv.y;
         const float r2 = dx * dx + dy * dy;
         float inv = __builtin_amdgcn_rcpf(r2);
-        if (SELF) inv = (__float_as_uint(sv.w) == i) ? 0.f : inv;
+        if (ICUT == 0) {
+            if (SELF) inv = (__float_as_uint(sv.w) == i) ? 0.f : inv;
+        } else {
+            // sij^2 = (si^2 + sj^2) / 4 (both terms staged as s^2 / 4);
+            // pairs at r = 0 (the target itself included) add nothing
+            const float rho2 = r2 * __builtin_amdgcn_rcpf(ts4 + sst[u * G]);
+            inv = r2 > 0.f ? regul32<ICUT>(rho2) * inv : 0.f;
+        }
         const float qr = sv.z * inv;
         ex += qr * dx;
         ey += qr * dy;
@@ -676,6 +704,7 @@ struct EvalArgs {
     const int *l3;
     const float2 *oxy;
     const float *qf;
+    const double *ss;  // sorted sizes (regularized near field only)
     const uint32_t *perm;
     const LevelTab *lt;
     const float *ahat, *bhat;
@@ -684,10 +713,10 @@ struct EvalArgs {
     double *out_a, *out_b;
 };
 
-template <int G>
+template <int G, int ICUT>
 __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, float4 *stg,
-                                          uint32_t *s_off, uint32_t *s_beg, float *s_dx,
-                                          float *s_dy) {
+                                          float *sst, uint32_t *s_off, uint32_t *s_beg,
+                                          float *s_dx, float *s_dy) {
     constexpr int LG = kWave / G;
     const int PM1 = A.kst / 2;
     const int g = lane / LG, tl = lane % LG;
@@ -704,6 +733,7 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
         const uint32_t i = tb + tl;
         const bool valid = i < e;
         const float2 ti = valid ? A.oxy[i] : make_float2(0.f, 0.f);
+        const float ts4 = (ICUT && valid) ? (float)(0.25 * A.ss[i] * A.ss[i]) : 0.f;
         // each staging round sums in FP32; the round totals add in FP64, so
         // leaves with very many sources do not lose the FP32 accumulation
         double exd = 0.0, eyd = 0.0;
@@ -716,13 +746,15 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
                         const uint32_t src = base + (uint32_t)r;
                         const float2 o = A.oxy[src];
                         stg[r] = make_float4(o.x, o.y, A.qf[src], __uint_as_float(src));
+                        if (ICUT) sst[r] = (float)(0.25 * A.ss[src] * A.ss[src]);
                     } else {
                         stg[r] = far_source();
+                        if (ICUT) sst[r] = 0.f;
                     }
                 }
                 wave_sync();
                 float ex = 0.f, ey = 0.f;
-                p2p_round<G, true>(stg + g, fillp / G, ti, i, ex, ey);
+                p2p_round<G, true, ICUT>(stg + g, sst + g, fillp / G, ti, ts4, i, ex, ey);
                 exd += (double)ex;
                 eyd += (double)ey;
                 wave_sync();
@@ -765,6 +797,7 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
                 for (int r = lane; r < fillp; r += kWave) {
                     if (r >= fill) {
                         stg[r] = far_source();
+                        if (ICUT) sst[r] = 0.f;
                         continue;
                     }
                     const uint32_t gi = base + (uint32_t)r;
@@ -780,10 +813,11 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
                     const float2 o = A.oxy[src];
                     stg[r] = make_float4(o.x + s_dx[lo], o.y + s_dy[lo], A.qf[src],
                                          __uint_as_float(src));
+                    if (ICUT) sst[r] = (float)(0.25 * A.ss[src] * A.ss[src]);
                 }
                 wave_sync();
                 float ex = 0.f, ey = 0.f;
-                p2p_round<G, false>(stg + g, fillp / G, ti, i, ex, ey);
+                p2p_round<G, false, ICUT>(stg + g, sst + g, fillp / G, ti, ts4, i, ex, ey);
                 exd += (double)ex;
                 eyd += (double)ey;
                 wave_sync();
@@ -835,8 +869,10 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
     }
 }
 
+template <int ICUT>
 __global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl) {
     __shared__ float4 stage[kEvalWaves][kStage];
+    __shared__ float stage_s[ICUT ? kEvalWaves : 1][ICUT ? kStage : 1];
     __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
     __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -844,14 +880,15 @@ __global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl
     if (t >= nl) return;
     const int j = (int)A.leaf_idx[t];
     const int nt = (int)(A.c_end[j] - A.c_begin[j]);
+    float *sst = stage_s[ICUT ? w : 0];
     if (nt <= 8)
-        eval_leaf<8>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+        eval_leaf<8, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else if (nt <= 16)
-        eval_leaf<4>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+        eval_leaf<4, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else if (nt <= 32)
-        eval_leaf<2>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+        eval_leaf<2, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else
-        eval_leaf<1>(A, t, lane, stage[w], s_off[w], s_beg[w], s_dx[w], s_dy[w]);
+        eval_leaf<1, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
 }
 
 }  // namespace
@@ -1083,9 +1120,15 @@ void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, doubl
     mk.mark(5, s);
     EvalArgs ea{leaf_idx, c_level, c_flags, c_begin, c_end, c_cx, c_cy,
                 (const uint32_t *)c.off1.p, (const int *)c.l1.p, (const uint32_t *)c.off3.p,
-                (const int *)c.l3.p, oxy, qf, c.perm_sorted, mx.lt, ahat, bhat, kst,
-                stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a, out_b};
-    k_eval32<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(ea, nl);
+                (const int *)c.l3.p, oxy, qf, (const double *)c.ss.p, c.perm_sorted, mx.lt,
+                ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a, out_b};
+    const int nb = div_up(nl, kEvalWaves), nth = kWave * kEvalWaves;
+    switch (c.icut) {
+        case 1: k_eval32<1><<<nb, nth, 0, s>>>(ea, nl); break;
+        case 2: k_eval32<2><<<nb, nth, 0, s>>>(ea, nl); break;
+        case 3: k_eval32<3><<<nb, nth, 0, s>>>(ea, nl); break;
+        default: k_eval32<0><<<nb, nth, 0, s>>>(ea, nl); break;
+    }
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }

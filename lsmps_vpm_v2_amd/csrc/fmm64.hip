@@ -181,14 +181,32 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(y, e, y);
 }
 
+// Regularization factor of a near-field pair, 2 pi q of regul_func_2d
+// (regularization_function_2d.cpp:11-37) at rho^2 = r^2 / sij^2, in the
+// reference's form: 1 high-order algebraic, 2 Gaussian, 3 super-Gaussian.
+template <int ICUT>
+__device__ __forceinline__ double regul64(double rho2) {
+    if (ICUT == 1) return (rho2 * (rho2 + 2.0)) / ((rho2 + 1.0) * (rho2 + 1.0));
+    const double qt = -0.5 * rho2;
+    if (ICUT == 2) return 1.0 - exp(qt);
+    return 1.0 - (1.0 - qt) * exp(qt);
+}
+
+// ICUT 0: the singular P2P of fmm2D (fmm2D.cpp:164-173); a pair at r = 0
+// raises DUPLICATE_POINTS.  ICUT 1..3: the type-1 near field of direct_sum
+// (subroutine_all_fmm_2d.cpp:594-627): each pair term times regul64 at
+// sij^2 = (si^2 + sj^2) / 4 (ss: the sorted sizes); pairs at r = 0 add
+// nothing.
+template <int ICUT>
 __global__ void __launch_bounds__(kWave *kEvalWaves)
     k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
              const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
              const uint32_t *off3, const int *l3, const double *xs, const double *ys,
-             const double *qs, const uint32_t *perm, const double2 *ak, const double2 *bk,
-             int velocity, double u_inf, double v_inf, int mask, double *out_a, double *out_b,
-             unsigned *flags) {
+             const double *qs, const double *ss, const uint32_t *perm, const double2 *ak,
+             const double2 *bk, int velocity, double u_inf, double v_inf, int mask,
+             double *out_a, double *out_b, unsigned *flags) {
     __shared__ double4 stage[kEvalWaves][kStage64];
+    __shared__ double stage_s[ICUT ? kEvalWaves : 1][ICUT ? kStage64 : 1];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int t = blockIdx.x * kEvalWaves + w;
     if (t >= nl) return;
@@ -206,6 +224,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
         const uint32_t i = tb + tl;
         const bool valid = i < e;
         const double xi = valid ? xs[i] : 0.0, yi = valid ? ys[i] : 0.0;
+        const double si2 = (ICUT && valid) ? ss[i] * ss[i] : 0.0;
         double ex = 0.0, ey = 0.0;
         // P2P over every particle of the list-1 cells, j != i (fmm2D.cpp:135-179)
         uint32_t u = a1, pos = a1 < z1 ? c_begin[l1[a1]] : 0u;
@@ -217,6 +236,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 for (int r = lane; r < take; r += kWave) {
                     const uint32_t src = pos + r;
                     stage[w][fill + r] = make_double4(xs[src], ys[src], qs[src], (double)src);
+                    if (ICUT) stage_s[w][fill + r] = ss[src] * ss[src];
                 }
                 fill += take;
                 pos += take;
@@ -228,9 +248,15 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 const double4 sv = stage[w][r];
                 const double dx = xi - sv.x, dy = yi - sv.y;
                 const double r2 = dx * dx + dy * dy;
-                if (valid && (uint32_t)sv.w != i) {
+                if (ICUT == 0 && valid && (uint32_t)sv.w != i) {
                     dup |= r2 == 0.0;
                     const double qr = sv.z * rcp64(r2);
+                    ex += qr * dx;
+                    ey += qr * dy;
+                }
+                if (ICUT != 0 && valid && r2 > 0.0) {
+                    const double sij2 = 0.25 * (si2 + stage_s[w][r]);
+                    const double qr = sv.z * regul64<ICUT>(r2 / sij2) * rcp64(r2);
                     ex += qr * dx;
                     ey += qr * dy;
                 }
@@ -360,11 +386,20 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
                                                                       c_cx, c_cy, bk);
     }
     mk.mark(5, s);
-    k_eval64<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
-        leaf_idx, nl, P, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
-        (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs,
-        c.perm_sorted, ak, bk,
-        velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b, &c.hdr_d->flags);
+    const double *ss = (const double *)c.ss.p;
+    auto eval = [&](auto kern) {
+        kern<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
+            leaf_idx, nl, P, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
+            (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs, ss,
+            c.perm_sorted, ak, bk, velocity ? 1 : 0, u_inf, v_inf, stage_mask, out_a, out_b,
+            &c.hdr_d->flags);
+    };
+    switch (c.icut) {
+        case 1: eval(k_eval64<1>); break;
+        case 2: eval(k_eval64<2>); break;
+        case 3: eval(k_eval64<3>); break;
+        default: eval(k_eval64<0>); break;
+    }
     mk.mark(6, s);
     VPM_LAUNCH_CHECK();
 }

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
              const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
              double *ys, double *qs, uint8_t *as, uint32_t *bs_act, const uint32_t *key, int W,
-             uint8_t *d, uint8_t *w) {
+             uint8_t *d, uint8_t *w, double *ss) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t act = 0;
     if (i < n) {
@@ -224,6 +224,7 @@ __global__ void __launch_bounds__(256)
             const double sz = ssz[o];
             qs[i] = -omega[o] * (sz * sz) / (2 * M_PI);
         }
+        if (ss) ss[i] = ssz[o];  // sizes, for a regularized near field
         act = a[o] ? 1u : 0u;
         as[i] = (uint8_t)act;
     }
@@ -939,8 +940,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint8_t *as = c.acts.as<uint8_t>(n);
     uint32_t *bs_act = c.bs_act.as<uint32_t>((size_t)div_up(n, 256) + 1);
     uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
+    double *ss = (c.icut > 0 && src.s) ? c.ss.as<double>(n) : nullptr;
     k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
-                                            ys, qs, as, bs_act, key, W, d, w);
+                                            ys, qs, as, bs_act, key, W, d, w, ss);
 
     // 4. leaf levels, cells per level and leaves
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);

@@ -52,11 +52,12 @@ class FmmSolver:
     def __init__(self, p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
                  precision="fp64", binom_mode=1, list4_mode=1, u_inf=1.0, v_inf=0.0,
                  device=0, timing=False, stage_mask=0, order="fixed", reselect_every=0,
-                 selector=None):
+                 selector=None, icutoff=0):
         """order = "learned" (mixed path only) installs ``selector`` (a
         ``selector.Model`` or a model file path; None: the packaged model) and
         lets it pick p per level and the leaf threshold, with ``p`` as the
-        fixed-order fallback."""
+        fixed-order fallback.  icutoff 1 / 2 / 3 regularizes the near field
+        as the reference's Pars::icutoff (regul_func_2d; velocity calls only)."""
         self._L = _lib.load()
         o = _lib.default_opts()
         o.p, o.max_par, o.max_src, o.exp_tree = p, max_par, max_src, exp_tree
@@ -67,6 +68,7 @@ class FmmSolver:
         o.stage_mask = stage_mask
         o.order_select = _ORDER[order]
         o.reselect_every = reselect_every
+        o.icutoff = icutoff
         self._o = o
         h = C.c_void_p()
         st = self._L.vpmfmm_create(C.byref(o), C.byref(h))

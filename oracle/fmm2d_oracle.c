@@ -50,6 +50,7 @@ void orc_default_opts(orc_opts *o) {
     o->binom_mode = 1;   /* reference helper */
     o->list4_mode = 1;   /* M2L_P2_METHOD_TYPE 1, fmm2D.cpp:7 */
     o->stage_mask = 0;
+    o->icutoff = 0;      /* the type-2 FMM's P2P is singular, fmm2D.cpp:164-173 */
 }
 
 static int opts_ok(const orc_opts *o) {
@@ -63,7 +64,21 @@ static int opts_ok(const orc_opts *o) {
     if (o->binom_mode != 0 && o->binom_mode != 1) return 0;
     if (o->list4_mode != 1 && o->list4_mode != 2) return 0;
     if (o->stage_mask < 0 || o->stage_mask > 7) return 0;
+    if (o->icutoff < 0 || o->icutoff > 3) return 0;
     return 1;
+}
+
+/* Regularization factor of a near-field pair: 2 pi q of regul_func_2d
+ * (regularization_function_2d.cpp:11-37) for the pair distance rij and the
+ * pair size sij (the reference's direct_sum passes sqrt of both,
+ * subroutine_all_fmm_2d.cpp:614-623, and regul_func_2d squares them again). */
+static double regul_factor(int icutoff, double rij, double sij) {
+    const double rij2 = pow(rij, 2), sij2 = pow(sij, 2);
+    const double rho2 = rij2 / sij2;
+    if (icutoff == 1) return (rho2 * (rho2 + 2.0)) / pow(rho2 + 1.0, 2);
+    const double qt = -rho2 / 2.0;
+    if (icutoff == 2) return 1.0 - exp(qt);
+    return 1.0 - (1 - qt) * exp(qt);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -650,8 +665,9 @@ static int cmp_i64(const void *a, const void *b) {
 /* Core of fmm2D::setupFMM + calcField.  With ids/ak_out/bk_out non-NULL the
  * per-cell expansions are also returned, cells sorted by ID. */
 static int fmm_run(int n, const double *x, const double *y, const double *q,
-                   const uint8_t *active, const orc_opts *o, double *ex, double *ey,
-                   int64_t *ids, double *ak_out, double *bk_out) {
+                   const double *sz, const uint8_t *active, const orc_opts *o, double *ex,
+                   double *ey, int64_t *ids, double *ak_out, double *bk_out) {
+    if (o && o->icutoff > 0 && !sz) return ORC_BAD_ARG;
     orc_tree *t = NULL;
     int st = orc_tree_build(&t, n, x, y, active, o);
     if (st) return st;
@@ -775,6 +791,17 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
                     if (i == j) continue;
                     double dx = x[i] - x[j], dy = y[i] - y[j];
                     double r2 = dx * dx + dy * dy;
+                    if (o->icutoff > 0) {
+                        /* the type-1 near field, direct_sum
+                           (subroutine_all_fmm_2d.cpp:594-627): sij^2 =
+                           (si^2 + sj^2) / 4, pairs at r = 0 skipped */
+                        if (!(r2 > 0.0)) continue;
+                        double sij2 = 0.25 * (pow(sz[i], 2) + pow(sz[j], 2));
+                        double f = regul_factor(o->icutoff, sqrt(r2), sqrt(sij2));
+                        ex[i] += f * q[j] * dx / r2;
+                        ey[i] += f * q[j] * dy / r2;
+                        continue;
+                    }
                     ex[i] += q[j] * dx / r2;
                     ey[i] += q[j] * dy / r2;
                 }
@@ -808,7 +835,7 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
 
 int orc_fmm_field(int n, const double *x, const double *y, const double *q,
                   const uint8_t *active, const orc_opts *o, double *ex, double *ey) {
-    return fmm_run(n, x, y, q, active, o, ex, ey, NULL, NULL, NULL);
+    return fmm_run(n, x, y, q, NULL, active, o, ex, ey, NULL, NULL, NULL);
 }
 
 int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
@@ -817,7 +844,7 @@ int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
     double *ex = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
     double *ey = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
     if (!ex || !ey) { free(ex); free(ey); return ORC_NOMEM; }
-    int st = fmm_run(n, x, y, q, active, o, ex, ey, ids, ak, bk);
+    int st = fmm_run(n, x, y, q, NULL, active, o, ex, ey, ids, ak, bk);
     free(ex); free(ey);
     return st;
 }
@@ -827,12 +854,12 @@ int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
 int orc_velocity(int n, const double *x, const double *y, const double *s,
                  const double *omega, const uint8_t *active, const orc_opts *o,
                  double u_inf, double v_inf, double *u, double *v) {
-    double *q = (double *)malloc(sizeof(double) * n);
+    double *q = (double *)calloc(n > 0 ? n : 1, sizeof(double));
     double *ex = (double *)malloc(sizeof(double) * n);
     double *ey = (double *)malloc(sizeof(double) * n);
     if (!q || !ex || !ey) { free(q); free(ex); free(ey); return ORC_NOMEM; }
     for (int i = 0; i < n; ++i) q[i] = -omega[i] * pow(s[i], 2.0) / (2 * M_PI);
-    int st = orc_fmm_field(n, x, y, q, active, o, ex, ey);
+    int st = fmm_run(n, x, y, q, s, active, o, ex, ey, NULL, NULL, NULL);
     if (st == ORC_OK)
         for (int i = 0; i < n; ++i) { u[i] = ey[i] + u_inf; v[i] = -ex[i] + v_inf; }
     free(q); free(ex); free(ey);

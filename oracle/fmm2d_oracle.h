@@ -50,6 +50,10 @@ typedef struct {
     int    list4_mode; /* 1: M2L from the list-4 leaf multipole (reference),
                           2: P2L from the list-4 leaf's active particles            */
     int    stage_mask; /* evaluation parts: 1 P2P, 2 M2P (list 3), 4 L2P; 0 = all  */
+    int    icutoff;    /* near-field (P2P) kernel, regul_func_2d
+                          (regularization_function_2d.cpp:11-37): 0 singular,
+                          1 high-order algebraic, 2 Gaussian, 3 super-Gaussian;
+                          > 0 needs the particle sizes (orc_velocity)            */
 } orc_opts;
 
 /* Status codes (shared numbering with the product C-ABI). */

@@ -40,6 +40,7 @@ class OrcOpts(C.Structure):
         ("binom_mode", C.c_int),
         ("list4_mode", C.c_int),
         ("stage_mask", C.c_int),
+        ("icutoff", C.c_int),
     ]
 
 
@@ -139,9 +140,47 @@ def ref():
 # options
 # --------------------------------------------------------------------------
 def opts(p=10, max_par=75, max_src=50, exp_tree=1.0, min_level=1, max_level=0,
-         binom_mode=1, list4_mode=1, stage_mask=0) -> OrcOpts:
+         binom_mode=1, list4_mode=1, stage_mask=0, icutoff=0) -> OrcOpts:
     return OrcOpts(p, max_par, max_src, exp_tree, min_level, max_level, binom_mode,
-                   list4_mode, stage_mask)
+                   list4_mode, stage_mask, icutoff)
+
+
+def regul_factor(icutoff, rho2):
+    """2 pi q of regul_func_2d (regularization_function_2d.cpp:11-37) as a
+    function of rho^2 = r^2 / sij^2: 0 singular (1), 1 high-order algebraic,
+    2 Gaussian, 3 super-Gaussian."""
+    rho2 = np.asarray(rho2, dtype=np.float64)
+    if icutoff == 0:
+        return np.ones_like(rho2)
+    if icutoff == 1:
+        return rho2 * (rho2 + 2.0) / (rho2 + 1.0) ** 2
+    qt = -rho2 / 2.0
+    if icutoff == 2:
+        return 1.0 - np.exp(qt)
+    return 1.0 - (1.0 - qt) * np.exp(qt)
+
+
+def regularized_direct_velocity(x, y, s, omega, icutoff, targets=None, r_min=0.0):
+    """Regularized Biot-Savart direct sum (biotsavart_direct_2d.cpp:5-41):
+    u_i -= dy gz_j q / r^2, v_i += dx gz_j q / r^2 over all sources j with
+    r > r_min, gz = omega s^2, q = regul_factor / (2 pi) at
+    sij = sqrt(si^2 + sj^2) / 2.  Rotational part only (no free stream)."""
+    x, y, s, w = _d(x), _d(y), _d(s), _d(omega)
+    gz = w * s ** 2
+    idx = np.arange(len(x)) if targets is None else np.asarray(targets)
+    u, v = np.zeros(len(idx)), np.zeros(len(idx))
+    for c0 in range(0, len(idx), 512):
+        t = idx[c0:c0 + 512]
+        dx = x[t, None] - x[None, :]
+        dy = y[t, None] - y[None, :]
+        r2 = dx * dx + dy * dy
+        sij2 = 0.25 * (s[t, None] ** 2 + s[None, :] ** 2)
+        keep = r2 > r_min * r_min
+        with np.errstate(divide="ignore", invalid="ignore"):
+            f = np.where(keep, regul_factor(icutoff, r2 / sij2) / (2 * np.pi) / r2, 0.0)
+        u[c0:c0 + 512] = -np.sum(dy * gz[None, :] * f, axis=1)
+        v[c0:c0 + 512] = np.sum(dx * gz[None, :] * f, axis=1)
+    return u, v
 
 
 class OracleError(RuntimeError):
