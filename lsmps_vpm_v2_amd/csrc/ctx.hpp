@@ -129,7 +129,7 @@ struct Ctx {
     DBuf ss;
     uint32_t *key_sorted = nullptr;   // points into key_a or key_b after the sort
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
-    DBuf lcp_d, lcp_w, act_scan, act_pos, lcp_v, leaf_lvl, ccount, cbase, leaf_of;
+    DBuf lcp_d, lcp_w, act_scan, act_pos, leaf_lvl, ccount, cbase, leaf_of;
     DBuf tmp;  // scan / sort scratch
     DBuf bbox_part;
     DBuf bs_act, bs_cells, bs_leaf, bs_l13;  // producer block sums of scan inputs

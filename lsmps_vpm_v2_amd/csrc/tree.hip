@@ -125,12 +125,20 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// treeCell.cpp:1079-1092: R = max side * (1 + expTree/100), pivot = (max + min - R)/2.
-// One block reduces the per-block partials; thread 0 forms the root geometry.
+// ---------------------------------------------------------------------------
+// 2-3. root geometry and Morton keys, sort, gather
+// ---------------------------------------------------------------------------
+// Root geometry, treeCell.cpp:1079-1092: R = max side * (1 + expTree/100),
+// pivot = (max + min - R)/2, from the bounding-box partials; every block
+// reduces them itself (a few KB, read from L2), so no launch separates the
+// box from the keys.  Block 0 records the geometry and the flags in the
+// header.  Then the Morton keys of a grid-stride range of particles.
 __global__ void __launch_bounds__(256)
-    k_bbox_final(const double *part, int nb, double expansion, DevHdr *hdr) {
+    k_keys(const double *x, const double *y, int n, const double *part, int nb,
+           double expansion, DevHdr *hdr, uint32_t *key, uint32_t *val) {
 #pragma clang fp contract(off)
     __shared__ double s[4][256];
+    __shared__ double geo[3];
     __shared__ int sbad;
     if (threadIdx.x == 0) sbad = 0;
     double mn0 = part[0], mx0 = part[1], mn1 = part[2], mx1 = part[3];
@@ -157,45 +165,42 @@ __global__ void __launch_bounds__(256)
         }
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
-    const double mn[2] = {s[0][0], s[2][0]}, mx[2] = {s[1][0], s[3][0]};
-    double R = 0.0;
-    for (int d = 0; d < 2; ++d) {
-        double side = fabs(mx[d] - mn[d]);
-        if (side > R) R = side;
+    if (threadIdx.x == 0) {
+        const double mn[2] = {s[0][0], s[2][0]}, mx[2] = {s[1][0], s[3][0]};
+        double R = 0.0;
+        for (int d = 0; d < 2; ++d) {
+            double side = fabs(mx[d] - mn[d]);
+            if (side > R) R = side;
+        }
+        R *= expansion;
+        geo[0] = R;
+        for (int d = 0; d < 2; ++d) geo[1 + d] = (mx[d] + mn[d] - R) / 2.0;
+        if (blockIdx.x == 0) {
+            hdr->root_len = R;
+            for (int d = 0; d < 2; ++d) {
+                hdr->mn[d] = mn[d];
+                hdr->mx[d] = mx[d];
+                hdr->pivot[d] = geo[1 + d];
+            }
+            unsigned f = 0;
+            if (sbad) f |= kFlagNonFinite;
+            if (!(R > 0.0)) f |= kFlagRange;
+            hdr->flags |= f;
+            hdr->leaf_lvl_min = kMaxDepth + 1;
+            hdr->leaf_lvl_max = 0;
+        }
     }
-    R *= expansion;
-    hdr->root_len = R;
-    for (int d = 0; d < 2; ++d) {
-        hdr->mn[d] = mn[d];
-        hdr->mx[d] = mx[d];
-        hdr->pivot[d] = (mx[d] + mn[d] - R) / 2.0;
+    __syncthreads();
+    const double R = geo[0], px = geo[1], py = geo[2];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double tx = (x[i] - px) / R, ty = (y[i] - py) / R;
+        const double sx = tx * 65536.0, sy = ty * 65536.0;
+        uint32_t ux = 0, uy = 0;
+        if (sx >= 0.0 && sx < 65536.0) ux = (uint32_t)sx;
+        if (sy >= 0.0 && sy < 65536.0) uy = (uint32_t)sy;
+        key[i] = part1by1(ux) | (part1by1(uy) << 1);
+        val[i] = (uint32_t)i;
     }
-    unsigned f = 0;
-    if (sbad) f |= kFlagNonFinite;
-    if (!(R > 0.0)) f |= kFlagRange;
-    hdr->flags |= f;
-    hdr->leaf_lvl_min = kMaxDepth + 1;
-    hdr->leaf_lvl_max = 0;
-}
-
-// ---------------------------------------------------------------------------
-// 2-3. Morton keys, sort, gather
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-    k_keys(const double *x, const double *y, int n, const DevHdr *hdr, uint32_t *key,
-           uint32_t *val) {
-#pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double R = hdr->root_len, px = hdr->pivot[0], py = hdr->pivot[1];
-    const double tx = (x[i] - px) / R, ty = (y[i] - py) / R;
-    const double sx = tx * 65536.0, sy = ty * 65536.0;
-    uint32_t ux = 0, uy = 0;
-    if (sx >= 0.0 && sx < 65536.0) ux = (uint32_t)sx;
-    if (sy >= 0.0 && sy < 65536.0) uy = (uint32_t)sy;
-    key[i] = part1by1(ux) | (part1by1(uy) << 1);
-    val[i] = (uint32_t)i;
 }
 
 // Particles in Morton order.  The source strength is q[o], or, for the
@@ -234,16 +239,6 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // 4. leaf levels from windowed LCPs
 // ---------------------------------------------------------------------------
-// v[m] = lcp(key[a_m], key[a_{m+S}]) over the sorted active particles.
-__global__ void __launch_bounds__(256)
-    k_lcp_v(const uint32_t *key, const uint32_t *apos, const DevHdr *hdr, int S, int n,
-            uint8_t *v) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const long long na = hdr->n_active;
-    if (m >= n || (long long)m + S >= na) return;
-    v[m] = (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]]);
-}
-
 // Block sums of up to four 64-bit statistics, added by thread 0 to the
 // header's statistic row of this block (every thread must call it).
 __device__ __forceinline__ void block_stats(DevHdr *hdr, int col0, int ncol,
@@ -299,7 +294,7 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 // first particle it is.
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
-            const uint32_t *apos, const uint8_t *v, const DevHdr *hdr, int n, int W, int S,
+            const uint32_t *apos, const DevHdr *hdr, int n, int W, int S,
             int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, DevHdr *hdr_w,
             uint32_t *bs_cells) {
     extern __shared__ uint8_t lds[];
@@ -326,9 +321,11 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int vlast = na - 1 - S;
     const int vlen = r_hi - vbase + 1;
     uint8_t *sv = hw + wlen, *gv = sv + vlen, *hv = gv + vlen;
+    // v[m] = lcp(key of active m, key of active m + S), formed here from the
+    // compacted active positions
     for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
         const int m = vbase + t;
-        sv[t] = (m >= 0 && m <= vlast) ? v[m] : 0;
+        sv[t] = (m >= 0 && m <= vlast) ? (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]]) : 0;
     }
     __syncthreads();
     const int kw = ilog2(W + 1), ks = S > 0 ? ilog2(S) : 0;
@@ -918,7 +915,6 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     double *part = c.bbox_part.as<double>(5 * (size_t)nbb);
     k_bbox_partial<<<nbb, 256, 0, s>>>(x, y, n, part, c.hdr_d);
     const double expansion = 1.0 + (tp.exp_tree / 100.0);
-    k_bbox_final<<<1, 256, 0, s>>>(part, nbb, expansion, c.hdr_d);
     VPM_LAUNCH_CHECK();
 
     // 2-3. keys, sort, gather
@@ -927,7 +923,8 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     const size_t tmp_words =
         std::max(radix_tmp_words(n), scan_tmp_words((long long)n + 1) + 64) + 64;
     uint32_t *tmp = c.tmp.as<uint32_t>(std::max(tmp_words, radix_tmp_words(16LL * n + 16)));
-    k_keys<<<div_up(n, 256), 256, 0, s>>>(x, y, n, c.hdr_d, ka, va);
+    k_keys<<<std::min(512, div_up(n, 256)), 256, 0, s>>>(x, y, n, part, nbb, expansion, c.hdr_d,
+                                                       ka, va);
     // the top sort_bits of the key (levels <= sort_bits / 2 resolved), a guess
     // from the previous build checked after the split
     const int bits = c.sort_bits;
@@ -948,14 +945,12 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
     uint32_t *apos = c.act_pos.as<uint32_t>(n);
     scan_exclusive_bs_u8(as, A, apos, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
-    uint8_t *v = c.lcp_v.as<uint8_t>(n + 1);
-    k_lcp_v<<<div_up(n, 256), 256, 0, s>>>(key, apos, c.hdr_d, S, n, v);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     uint32_t *cnt = c.ccount.as<uint32_t>(n);
     const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
     uint32_t *bs_cells = c.bs_cells.as<uint32_t>((size_t)nbs + 1);
-    k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, v, c.hdr_d, n, W, S, tp.min_level,
+    k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, c.hdr_d, n, W, S, tp.min_level,
                                           tp.max_level, leaf_lvl, cnt, c.hdr_d, bs_cells);
     // The one synchronisation of the build: the header (split rows) is read
     // back now; the cell-count scan and the cell emission need no host
