@@ -350,9 +350,11 @@ void upward_with_lists(vpmfmm_handle *h, const Mixed32 &mx, const FmmMarks &mk) 
     fmm32_prepare(c, mx);
     VPM_HIP(hipEventRecord(c.ev_fork, c.stream));
     VPM_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-    // launches interleaved so that both streams have work early
+    // launches interleaved so that both streams have work early, the upward
+    // pass (the longer chain) first
+    fmm32_p2m(c, mx, c.side);
     tree_lists_begin(c);
-    fmm32_upward(c, mx, c.side, mk);
+    fmm32_m2m(c, mx, c.side, mk);
     VPM_HIP(hipEventRecord(c.ev_join, c.side));
     tree_lists_end(c);
     mark(h, 2);

@@ -1024,24 +1024,24 @@ static void tier_levels(const TreeInfo &ti, int &K, int &T) {
 // No clearing is needed: P2M / M2M write every coefficient a cell's readers
 // use (zero for inactive cells), the list-2 M2L writes b^_1..p of every cell
 // of level >= 2 before list 4 and L2L add to them.
-void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) {
+void fmm32_p2m(Ctx &c, const Mixed32 &mx, hipStream_t s) {
+    const int nl = c.info.n_leaves;
+    k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
+        (const uint32_t *)c.leaf_idx.p, nl, (const uint8_t *)c.c_level.p,
+        (const uint8_t *)c.c_flags.p, (const uint32_t *)c.c_begin.p, (const uint32_t *)c.c_end.p,
+        (const double *)c.c_cx.p, (const double *)c.c_cy.p, (const double *)c.xs.p,
+        (const double *)c.ys.p, (const double *)c.qs.p, (const uint8_t *)c.acts.p, mx.lt,
+        2 * (mx.pmax + 1), (float2 *)c.oxy.p, (float *)c.qf.p, (float *)c.ak.p);
+    VPM_LAUNCH_CHECK();
+}
+
+void fmm32_m2m(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) {
     const TreeInfo &ti = c.info;
-    const int nl = ti.n_leaves;
     const int kst = 2 * (mx.pmax + 1);
     float *ahat = (float *)c.ak.p;
-    float2 *oxy = (float2 *)c.oxy.p;
-    float *qf = (float *)c.qf.p;
-    const uint8_t *c_level = (const uint8_t *)c.c_level.p, *c_flags = (const uint8_t *)c.c_flags.p;
+    const uint8_t *c_flags = (const uint8_t *)c.c_flags.p;
     const uint32_t *c_key = (const uint32_t *)c.c_key.p;
-    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
-    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
     const int *c_child = (const int *)c.c_child.p;
-    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
-    const uint8_t *as = (const uint8_t *)c.acts.p;
-
-    k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(
-        leaf_idx, nl, c_level, c_flags, c_begin, c_end, c_cx, c_cy, (const double *)c.xs.p,
-        (const double *)c.ys.p, (const double *)c.qs.p, as, mx.lt, kst, oxy, qf, ahat);
     const int maxL = ti.max_level;
     int K, T;
     tier_levels(ti, K, T);
@@ -1059,6 +1059,11 @@ void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) 
                                           mx.m2m, kst, ahat);
     mk.mark(3, s);
     VPM_LAUNCH_CHECK();
+}
+
+void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) {
+    fmm32_p2m(c, mx, s);
+    fmm32_m2m(c, mx, s, mk);
 }
 
 void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,

@@ -42,6 +42,9 @@ int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
 // the lists and the upward pass).  fmm32_run does all three on c.stream.
 void fmm32_prepare(Ctx &c, const Mixed32 &mx);
 void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk);
+// its two parts: P2M (one launch), then M2M up the tree
+void fmm32_p2m(Ctx &c, const Mixed32 &mx, hipStream_t s);
+void fmm32_m2m(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk);
 void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                 double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
