@@ -10,7 +10,9 @@ lists, order selection (learned path), every FMM stage and the write-back of
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--path learned|mixed|fp64]
 
 Prints one JSON line.  ``value`` = particles / (seconds per step) of the
-headline path (default: the learned-order FP32/MFMA path).  The plain FP64
+headline path (default: the learned-order FP32/MFMA path), timed without
+stage events; the stage times and the evaluation kernel's duration come
+from a second solver of the same options that records them.  The plain FP64
 path (reference options: p = 10, 75/50 leaf rule) is timed beside it and the
 ratio reported.  Errors: relative L2 of the rotational velocity against the
 FP64 direct sum on the GPU (20,000 seeded targets, all sources) and against
@@ -204,11 +206,18 @@ def main():
     torch.cuda.synchronize()
 
     opts = PATHS[args.path]
-    solver = FmmSolver(timing=True, **opts)
+    # the headline: the production configuration (no stage events)
+    solver = FmmSolver(**opts)
     dt, stats = time_path(torch, solver, dev, args.steps, args.warmup)
+    # stage times and the evaluation kernel's duration from a second solver
+    # that records HIP events around the stages (they cost about 30 us per
+    # solve, so they stay out of the headline)
+    solver_t = FmmSolver(timing=True, **opts)
+    dt_t, stats = time_path(torch, solver_t, dev, args.steps, args.warmup)
     st = stats[-1]
     sel = solver.last_selection()
-    log(f"[bench] {args.path}: {dt * 1e3:.3f} ms/step, stages(ms) tree {st['ms_tree']:.3f} "
+    log(f"[bench] {args.path}: {dt * 1e3:.3f} ms/step ({dt_t * 1e3:.3f} with stage events), "
+        f"stages(ms) tree {st['ms_tree']:.3f} "
         f"lists {st['ms_lists']:.3f} up {st['ms_upward']:.3f} m2l {st['ms_m2l']:.3f} "
         f"down {st['ms_downward']:.3f} eval {st['ms_eval']:.3f}")
     if args.path == "learned":
@@ -226,8 +235,10 @@ def main():
 
     out_fp64 = None
     if args.path != "fp64" and not args.no_fp64:
-        s64 = FmmSolver(timing=True, **PATHS["fp64"])
-        dt64, st64 = time_path(torch, s64, dev, args.steps, args.warmup)
+        s64 = FmmSolver(**PATHS["fp64"])
+        dt64, _ = time_path(torch, s64, dev, args.steps, args.warmup)
+        _, st64 = time_path(torch, FmmSolver(timing=True, **PATHS["fp64"]), dev, args.steps,
+                            args.warmup)
         u64, v64 = s64.velocity_device(*dev)
         out_fp64 = {"value": n / dt64, "ms_per_step": dt64 * 1e3,
                     "stage_ms": {k: float(np.mean([s[k] for s in st64])) for k in
@@ -249,6 +260,7 @@ def main():
 
     stage_ms = {k: float(np.mean([s[k] for s in stats])) for k in
                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_eval")}
+    stage_ms["ms_per_step_with_events"] = dt_t * 1e3
     # dominant kernel: the evaluation kernel (P2P + M2P + L2P), vector-ALU bound
     eval_ms = stage_ms["ms_eval"]
     flops = eval_flops(st)
@@ -261,7 +273,7 @@ def main():
                 "compute_unit": "vector ALU (" + ("FP64" if fp64 else "FP32") + ")",
                 "flops_per_launch": flops, "ms_per_launch": eval_ms,
                 "timing": "HIP events on the solver stream around the kernel, mean over the "
-                          "timed steps",
+                          "steps of a second solver that records stage events",
                 "traffic_source": traffic_src}
 
     cfg = {k: v for k, v in opts.items()}
