@@ -51,7 +51,8 @@ PATHS = {
     "learned": dict(precision="mixed", p=12, max_par=48, max_src=48, binom_mode=0, list4_mode=2,
                     order="learned"),
 }
-KERNEL = {"fp64": "k_eval64", "mixed": "k_eval32", "learned": "k_eval32"}
+# the evaluation kernel of each path (template argument 0: the singular near field)
+KERNEL = {"fp64": "k_eval64<0>", "mixed": "k_eval32<0>", "learned": "k_eval32<0>"}
 
 
 def log(msg):
@@ -158,7 +159,7 @@ def cpu_baseline(field, reps=3):
 def load_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{workload}_pmc.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
