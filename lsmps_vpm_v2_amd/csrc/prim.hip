@@ -20,10 +20,7 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-constexpr int kRsBlock = 256;   // = number of digit values
-constexpr int kRsItems = 4;
 constexpr int kFusedSpine = 2048;  // tiles up to which the down-sweep forms its own prefix
-constexpr int kRsTile = kRsBlock * kRsItems;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -462,7 +459,8 @@ size_t radix_tmp_words(long long n) {
 }
 
 bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
-                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s) {
+                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s,
+                      bool first_counts) {
     if (n <= 1) return false;
     const int nb = div_up(n, kRsTile);
     uint32_t *table = tmp;
@@ -473,7 +471,8 @@ bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
         const uint32_t *vi = alt ? vals_alt : vals;
         uint32_t *ko = alt ? keys : keys_alt;
         uint32_t *vo = alt ? vals : vals_alt;
-        k_rs_hist<<<nb, kRsBlock, 0, s>>>(ki, n, shift, table, nb);
+        if (!(first_counts && shift == begin_bit))
+            k_rs_hist<<<nb, kRsBlock, 0, s>>>(ki, n, shift, table, nb);
         k_rs_rowscan<<<256, kRsBlock, 0, s>>>(table, nb, totals);
         k_rs_scatter<<<nb, kRsBlock, 0, s>>>(ki, vi, ko, vo, n, shift, table, totals, nb);
         VPM_LAUNCH_CHECK();

@@ -35,11 +35,21 @@ void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, uint32_t *compact, i
 // Scratch words needed by the radix sort for n elements.
 size_t radix_tmp_words(long long n);
 
+// Radix tiles: kRsTile keys per workgroup of kRsBlock threads, element
+// base + j kRsBlock + thread for j < kRsItems.  The per-tile digit counts of a
+// pass live in tmp[digit * tiles + tile] (tiles = radix_tiles(n)).
+constexpr int kRsBlock = 256;  // = number of digit values
+constexpr int kRsItems = 4;
+constexpr int kRsTile = kRsBlock * kRsItems;
+inline int radix_tiles(long long n) { return (int)((n + kRsTile - 1) / kRsTile); }
+
 // Stable LSD radix sort of (key, val) pairs on key bits [begin_bit, end_bit),
 // 8 bits per pass.  Buffers ping-pong between (keys, vals) and
 // (keys_alt, vals_alt); the return value tells where the result lies
-// (false: keys/vals, true: keys_alt/vals_alt).
+// (false: keys/vals, true: keys_alt/vals_alt).  first_counts: the kernel that
+// wrote the keys also wrote the first pass's per-tile digit counts into tmp.
 bool radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
-                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s);
+                      int n, int begin_bit, int end_bit, uint32_t *tmp, hipStream_t s,
+                      bool first_counts = false);
 
 }  // namespace vpm

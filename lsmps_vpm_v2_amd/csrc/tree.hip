@@ -132,15 +132,20 @@ __global__ void __launch_bounds__(256)
 // pivot = (max + min - R)/2, from the bounding-box partials; every block
 // reduces them itself (a few KB, read from L2), so no launch separates the
 // box from the keys.  Block 0 records the geometry and the flags in the
-// header.  Then the Morton keys of a grid-stride range of particles.
-__global__ void __launch_bounds__(256)
+// header.  Then the Morton keys of one radix tile (kRsTile particles, the
+// radix sort's element order) and the tile's counts of the first sort digit
+// (key >> shift) & 255, so the sort's first pass needs no counting launch.
+__global__ void __launch_bounds__(kRsBlock)
     k_keys(const double *x, const double *y, int n, const double *part, int nb,
-           double expansion, DevHdr *hdr, uint32_t *key, uint32_t *val) {
+           double expansion, DevHdr *hdr, uint32_t *key, uint32_t *val, int shift,
+           uint32_t *table, int tiles) {
 #pragma clang fp contract(off)
-    __shared__ double s[4][256];
+    __shared__ double s[4][kRsBlock / kWave];
     __shared__ double geo[3];
+    __shared__ uint32_t h[256];
     __shared__ int sbad;
     if (threadIdx.x == 0) sbad = 0;
+    h[threadIdx.x] = 0;
     double mn0 = part[0], mx0 = part[1], mn1 = part[2], mx1 = part[3];
     int bad = 0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -150,23 +155,29 @@ __global__ void __launch_bounds__(256)
         mx1 = fmax(mx1, part[5 * b + 3]);
         bad |= part[5 * b + 4] != 0.0;
     }
-    s[0][threadIdx.x] = mn0;
-    s[1][threadIdx.x] = mx0;
-    s[2][threadIdx.x] = mn1;
-    s[3][threadIdx.x] = mx1;
+    for (int m = 1; m < kWave; m <<= 1) {
+        mn0 = fmin(mn0, __shfl_xor(mn0, m, kWave));
+        mx0 = fmax(mx0, __shfl_xor(mx0, m, kWave));
+        mn1 = fmin(mn1, __shfl_xor(mn1, m, kWave));
+        mx1 = fmax(mx1, __shfl_xor(mx1, m, kWave));
+    }
+    const int wv = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s[0][wv] = mn0;
+        s[1][wv] = mx0;
+        s[2][wv] = mn1;
+        s[3][wv] = mx1;
+    }
     __syncthreads();
     if (bad) atomicOr(&sbad, 1);
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + o]);
-            s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + o]);
-            s[2][threadIdx.x] = fmin(s[2][threadIdx.x], s[2][threadIdx.x + o]);
-            s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + o]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        const double mn[2] = {s[0][0], s[2][0]}, mx[2] = {s[1][0], s[3][0]};
+        double mn[2] = {s[0][0], s[2][0]}, mx[2] = {s[1][0], s[3][0]};
+        for (int u = 1; u < kRsBlock / kWave; ++u) {
+            mn[0] = fmin(mn[0], s[0][u]);
+            mx[0] = fmax(mx[0], s[1][u]);
+            mn[1] = fmin(mn[1], s[2][u]);
+            mx[1] = fmax(mx[1], s[3][u]);
+        }
         double R = 0.0;
         for (int d = 0; d < 2; ++d) {
             double side = fabs(mx[d] - mn[d]);
@@ -182,25 +193,35 @@ __global__ void __launch_bounds__(256)
                 hdr->mx[d] = mx[d];
                 hdr->pivot[d] = geo[1 + d];
             }
-            unsigned f = 0;
-            if (sbad) f |= kFlagNonFinite;
-            if (!(R > 0.0)) f |= kFlagRange;
-            hdr->flags |= f;
-            hdr->leaf_lvl_min = kMaxDepth + 1;
-            hdr->leaf_lvl_max = 0;
         }
     }
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        unsigned f = 0;
+        if (sbad) f |= kFlagNonFinite;
+        if (!(geo[0] > 0.0)) f |= kFlagRange;
+        hdr->flags |= f;
+        hdr->leaf_lvl_min = kMaxDepth + 1;
+        hdr->leaf_lvl_max = 0;
+    }
     const double R = geo[0], px = geo[1], py = geo[2];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int base = blockIdx.x * kRsTile;
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j) {
+        const int i = base + j * kRsBlock + threadIdx.x;
+        if (i >= n) break;
         const double tx = (x[i] - px) / R, ty = (y[i] - py) / R;
         const double sx = tx * 65536.0, sy = ty * 65536.0;
         uint32_t ux = 0, uy = 0;
         if (sx >= 0.0 && sx < 65536.0) ux = (uint32_t)sx;
         if (sy >= 0.0 && sy < 65536.0) uy = (uint32_t)sy;
-        key[i] = part1by1(ux) | (part1by1(uy) << 1);
+        const uint32_t k = part1by1(ux) | (part1by1(uy) << 1);
+        key[i] = k;
         val[i] = (uint32_t)i;
+        atomicAdd(&h[(k >> shift) & 255u], 1u);
     }
+    __syncthreads();
+    table[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
 // Particles in Morton order.  The source strength is q[o], or, for the
@@ -923,12 +944,13 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     const size_t tmp_words =
         std::max(radix_tmp_words(n), scan_tmp_words((long long)n + 1) + 64) + 64;
     uint32_t *tmp = c.tmp.as<uint32_t>(std::max(tmp_words, radix_tmp_words(16LL * n + 16)));
-    k_keys<<<std::min(512, div_up(n, 256)), 256, 0, s>>>(x, y, n, part, nbb, expansion, c.hdr_d,
-                                                       ka, va);
     // the top sort_bits of the key (levels <= sort_bits / 2 resolved), a guess
     // from the previous build checked after the split
     const int bits = c.sort_bits;
-    const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 32 - bits, 32, tmp, s);
+    const int tiles = radix_tiles(n);
+    k_keys<<<tiles, kRsBlock, 0, s>>>(x, y, n, part, nbb, expansion, c.hdr_d, ka, va, 32 - bits,
+                                      tmp, tiles);
+    const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 32 - bits, 32, tmp, s, true);
     uint32_t *key = alt ? kb : ka;
     uint32_t *perm = alt ? vb : va;
     c.key_sorted = key;
