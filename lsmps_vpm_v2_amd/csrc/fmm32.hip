@@ -654,20 +654,33 @@ __device__ __forceinline__ float4 far_source() {
 // reference's 1 - exp form cancels (small h = rho^2 / 2), the Taylor series
 // of the same function is used: Gaussian 1 - e^-h below h = 1/8, super-
 // Gaussian 1 - (1 + h) e^-h below h = 1/2 (truncation under 5e-7 relative).
+// Such close pairs are rare, so the series is only evaluated by a wavefront
+// that has one (a wave-uniform branch).  Far padding sources can give
+// rho^2 = inf: the Gaussian takes it (e = 0, factor 1); the other two clamp
+// it, so that (x + 1)^2 and (1 + h) e stay finite.
 template <int ICUT>
 __device__ __forceinline__ float regul32(float rho2) {
-    const float x = fminf(rho2, 1e15f);  // padding sources: far, finite
-    if (ICUT == 1) return x * (x + 2.f) * __builtin_amdgcn_rcpf((x + 1.f) * (x + 1.f));
-    const float h = 0.5f * x;
+    if (ICUT == 1) {
+        const float x = fminf(rho2, 1e15f);
+        return x * (x + 2.f) * __builtin_amdgcn_rcpf((x + 1.f) * (x + 1.f));
+    }
+    const float h = 0.5f * (ICUT == 3 ? fminf(rho2, 1e15f) : rho2);
     const float e = __expf(-h);
     if (ICUT == 2) {
-        const float ser = h * (1.f - h * (0.5f - h * (1.f / 6 - h * (1.f / 24 - h * (1.f / 120)))));
-        return h < 0.125f ? ser : 1.f - e;
+        float f = 1.f - e;
+        const bool near = h < 0.125f;
+        if (__ballot(near))
+            f = near ? h * (1.f - h * (0.5f - h * (1.f / 6 - h * (1.f / 24 - h * (1.f / 120)))))
+                     : f;
+        return f;
     }
-    const float ser =
-        h * h * (0.5f - h * (1.f / 3 - h * (1.f / 8 - h * (1.f / 30 -
-        h * (1.f / 144 - h * (1.f / 840 - h * (1.f / 5760)))))));
-    return h < 0.5f ? ser : 1.f - (1.f + h) * e;
+    float f = 1.f - (1.f + h) * e;
+    const bool near = h < 0.5f;
+    if (__ballot(near))
+        f = near ? h * h * (0.5f - h * (1.f / 3 - h * (1.f / 8 - h * (1.f / 30 -
+                   h * (1.f / 144 - h * (1.f / 840 - h * (1.f / 5760)))))))
+                 : f;
+    return f;
 }
 
 template <int G, bool SELF, int ICUT>
