@@ -347,17 +347,25 @@ void upward_with_lists(vpmfmm_handle *h, const Mixed32 &mx, const FmmMarks &mk) 
     if (!c.side) VPM_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
     if (!c.ev_fork) VPM_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     if (!c.ev_join) VPM_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    if (!c.ev_p2m) VPM_HIP(hipEventCreateWithFlags(&c.ev_p2m, hipEventDisableTiming));
     fmm32_prepare(c, mx);
     VPM_HIP(hipEventRecord(c.ev_fork, c.stream));
     VPM_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
     // launches interleaved so that both streams have work early, the upward
     // pass (the longer chain) first
     fmm32_p2m(c, mx, c.side);
+    VPM_HIP(hipEventRecord(c.ev_p2m, c.side));
     tree_lists_begin(c);
     fmm32_m2m(c, mx, c.side, mk);
     VPM_HIP(hipEventRecord(c.ev_join, c.side));
     tree_lists_end(c);
     mark(h, 2);
+    // list 4 by P2L on the main stream once P2M has formed the offsets, beside
+    // the rest of the upward pass
+    if (h->o.list4_mode == 2) {
+        VPM_HIP(hipStreamWaitEvent(c.stream, c.ev_p2m, 0));
+        fmm32_p2l(c, mx, c.stream);
+    }
     VPM_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     mark(h, 7);
     h->concurrent_marks = true;
@@ -700,6 +708,7 @@ int vpmfmm_destroy(vpmfmm_handle *h) {
     if (h->ctx.ev_sync) (void)hipEventDestroy(h->ctx.ev_sync);
     if (h->ctx.ev_fork) (void)hipEventDestroy(h->ctx.ev_fork);
     if (h->ctx.ev_join) (void)hipEventDestroy(h->ctx.ev_join);
+    if (h->ctx.ev_p2m) (void)hipEventDestroy(h->ctx.ev_p2m);
     if (h->ctx.side) {
         (void)hipStreamSynchronize(h->ctx.side);
         (void)hipStreamDestroy(h->ctx.side);

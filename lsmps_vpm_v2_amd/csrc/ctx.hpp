@@ -115,7 +115,7 @@ struct Ctx {
     // second stream (default priority) for the upward pass, which runs
     // concurrently with the list build; fork / join events
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_p2m = nullptr;
     DevHdr *hdr_d = nullptr;  // device header
     DevHdr *hdr_h = nullptr;  // pinned host mirror
 
@@ -150,6 +150,7 @@ struct Ctx {
 
     // mixed path: FP32 particle offsets and sources, tables, tiles
     DBuf oxy, qf;
+    DBuf b4;  // list-4 P2L contributions (mixed path, list4_mode 2)
     DBuf sel_part, sel_feats, sel_raw;  // level statistics of the selection
     // expansion order per level (levels 0..17), device resident: slot 0 holds
     // the learned selection (written by the selection kernel), slot 1 the

@@ -386,7 +386,8 @@ template <int P>
 __device__ __forceinline__ void m2l_tile(int j0, int cnt, int w, int lane, int (*smap)[49],
                                          float *red, const uint32_t *c_key, const int *l2,
                                          const uint8_t *n2, const float *R, const float *ahat,
-                                         int kst, float *bhat) {
+                                         int kst, const uint8_t *n4, const float *b4,
+                                         float *bhat) {
     constexpr int K = 2 * (P + 1), KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
     const int n = lane % 16, kq = lane / 16;
     const int j = j0 + n;
@@ -450,7 +451,11 @@ __device__ __forceinline__ void m2l_tile(int j0, int cnt, int w, int lane, int (
 #pragma unroll
             for (int u = 0; u < kM2lWaves; ++u) sum += red[(u * 8 + mt * 4 + v) * kWave + lane];
             const int row = mt * 16 + 4 * kq + v;  // row of (Re b^_1, Im b^_1, Re b^_2, ...)
-            if (n < cnt && row < 2 * P) bhat[(size_t)j * kst + 2 + row] = sum;
+            if (n < cnt && row < 2 * P) {
+                const size_t idx = (size_t)j * kst + 2 + row;
+                // list 4 by P2L, formed beforehand into b4 (cells with a list 4)
+                bhat[idx] = (b4 && n4[j]) ? sum + b4[idx] : sum;
+            }
         }
     }
 }
@@ -465,7 +470,7 @@ struct M2lTabs {
 __global__ void __launch_bounds__(kWave *kM2lWaves)
     k_m2l_mfma(const int2 *tiles, const uint8_t *tile_p, int ntiles, const uint32_t *c_key,
                const int *l2, const uint8_t *n2, M2lTabs tabs, const float *ahat, int kst,
-               float *bhat) {
+               const uint8_t *n4, const float *b4, float *bhat) {
     __shared__ int smap[16][49];
     __shared__ float red[kM2lWaves * 8 * kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -475,8 +480,8 @@ __global__ void __launch_bounds__(kWave *kM2lWaves)
     const int p = tile_p[tile];
 #define M2L_CASE(P)                                                                          \
     case P:                                                                                  \
-        m2l_tile<P>(td.x, td.y, w, lane, smap, red, c_key, l2, n2, tabs.r[P], ahat, kst,      \
-                    bhat);                                                                   \
+        m2l_tile<P>(td.x, td.y, w, lane, smap, red, c_key, l2, n2, tabs.r[P], ahat, kst, n4,  \
+                    b4, bhat);                                                               \
         break;
     switch (p) {
         M2L_CASE(1) M2L_CASE(2) M2L_CASE(3) M2L_CASE(4) M2L_CASE(5) M2L_CASE(6) M2L_CASE(7)
@@ -529,7 +534,9 @@ __global__ void __launch_bounds__(256)
 }
 
 // List 4, mode 2: P2L from the active particles of coarser leaves,
-//   b^_m += -q/m (s_t / (z_j - z_t))^m.
+//   b4_m = -q/m (s_t / (z_j - z_t))^m  (cells with a list 4 only),
+// into its own buffer: it needs only the particles and list 4, so it runs
+// before M2L (beside the upward pass), and the M2L epilogue adds it.
 // One wavefront per target cell; lanes stride over the source particles and
 // keep the running sums of q w^m for every order (<= 16) in registers; the
 // -1/m factor is applied once per order after the lanes' sums are reduced
@@ -540,7 +547,7 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
     k_p2l_32(int j0, int nc, const uint8_t *c_level, const int *l4, const uint8_t *n4,
              const double *c_cx, const double *c_cy, const uint32_t *c_begin,
              const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
-             const LevelTab *lt, int kst, float *bhat) {
+             const LevelTab *lt, int kst, float *b4) {
     __shared__ float2 red[kP2lWaves][16][kWave + 1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -591,8 +598,7 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
         }
         const float c = -1.f / (float)m;
         const cf sum = {((s0.x + s1.x) + (s2.x + s3.x)) * c, ((s0.y + s1.y) + (s2.y + s3.y)) * c};
-        const size_t idx = (size_t)j * PM1 + m;
-        stc(bhat, idx, cadd(ldc(bhat, idx), sum));
+        stc(b4, (size_t)j * PM1 + m, sum);
     }
 }
 
@@ -1079,6 +1085,18 @@ void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk) 
     fmm32_m2m(c, mx, s, mk);
 }
 
+void fmm32_p2l(Ctx &c, const Mixed32 &mx, hipStream_t s) {
+    const TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, j2 = (int)ti.lev_off[2];
+    if (nc <= j2) return;
+    k_p2l_32<<<div_up(nc - j2, kP2lWaves), kWave * kP2lWaves, 0, s>>>(
+        j2, nc, (const uint8_t *)c.c_level.p, (const int *)c.l4.p, (const uint8_t *)c.n4.p,
+        (const double *)c.c_cx.p, (const double *)c.c_cy.p, (const uint32_t *)c.c_begin.p,
+        (const uint32_t *)c.c_end.p, (const float2 *)c.oxy.p, (const float *)c.qf.p,
+        (const uint8_t *)c.acts.p, mx.lt, 2 * (mx.pmax + 1), (float *)c.b4.p);
+    VPM_LAUNCH_CHECK();
+}
+
 void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                 double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
     hipStream_t s = c.stream;
@@ -1104,22 +1122,16 @@ void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, doubl
     if (mx.ntiles > 0) {
         M2lTabs tabs;
         for (int p = 0; p <= 16; ++p) tabs.r[p] = mx.rptr_host[p];
-        k_m2l_mfma<<<mx.ntiles, kWave * kM2lWaves, 0, s>>>(mx.tiles, mx.tile_p, mx.ntiles, c_key,
-                                                         (const int *)c.l2.p,
-                                                         (const uint8_t *)c.n2.p, tabs, ahat, kst,
-                                                         bhat);
+        k_m2l_mfma<<<mx.ntiles, kWave * kM2lWaves, 0, s>>>(
+            mx.tiles, mx.tile_p, mx.ntiles, c_key, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
+            tabs, ahat, kst, (const uint8_t *)c.n4.p,
+            list4_mode == 2 ? (const float *)c.b4.p : nullptr, bhat);
     }
     const int j2 = (int)ti.lev_off[2];
-    if (nc > j2) {
-        if (list4_mode == 2)
-            k_p2l_32<<<div_up(nc - j2, kP2lWaves), kWave * kP2lWaves, 0, s>>>(
-                j2, nc, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy,
-                c_begin, c_end, oxy, qf, as, mx.lt, kst, bhat);
-        else
-            k_m2l4_32<<<div_up((long long)(nc - j2) * (PM1 - 1), 256), 256, 0, s>>>(
-                j2, nc, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, mx.lt,
-                mx.tm2l, ahat, kst, bhat);
-    }
+    if (nc > j2 && list4_mode != 2)
+        k_m2l4_32<<<div_up((long long)(nc - j2) * (PM1 - 1), 256), 256, 0, s>>>(
+            j2, nc, c_level, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, mx.lt,
+            mx.tm2l, ahat, kst, bhat);
     mk.mark(4, s);
     if (maxL >= 3) {
         // parents 2 .. T-1 (one workgroup), T .. K-1 (a launch per level), K .. (subtrees)
@@ -1156,6 +1168,7 @@ void fmm32_prepare(Ctx &c, const Mixed32 &mx) {
     const size_t kst = 2 * (size_t)(mx.pmax + 1);
     c.ak.as<float>((size_t)ti.n_cells * kst);
     c.bk.as<float>((size_t)ti.n_cells * kst);
+    c.b4.as<float>((size_t)ti.n_cells * kst);
     c.oxy.as<float2>(ti.n);
     c.qf.as<float>(ti.n);
 }
@@ -1164,6 +1177,7 @@ void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double
                double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
     fmm32_prepare(c, mx);
     fmm32_upward(c, mx, c.stream, mk);
+    if (list4_mode == 2) fmm32_p2l(c, mx, c.stream);
     fmm32_rest(c, mx, list4_mode, stage_mask, out_a, out_b, velocity, u_inf, v_inf, mk);
 }
 

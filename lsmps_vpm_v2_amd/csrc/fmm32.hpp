@@ -39,12 +39,16 @@ int mixed_level_setup(Ctx &c, const uint8_t *orders, LevelTab *lt, int2 *tiles,
 // The mixed FMM of the tree in c: fmm32_prepare (host: sizes the expansion
 // and particle buffers), fmm32_upward (P2M, M2M; on any stream, it needs the
 // tree only), fmm32_rest (M2L, list 4, L2L, evaluation on c.stream; needs
-// the lists and the upward pass).  fmm32_run does all three on c.stream.
+// the lists and the upward pass, and, for list4_mode 2, fmm32_p2l).
+// fmm32_run does all of them on c.stream.
 void fmm32_prepare(Ctx &c, const Mixed32 &mx);
 void fmm32_upward(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk);
 // its two parts: P2M (one launch), then M2M up the tree
 void fmm32_p2m(Ctx &c, const Mixed32 &mx, hipStream_t s);
 void fmm32_m2m(Ctx &c, const Mixed32 &mx, hipStream_t s, const FmmMarks &mk);
+// List 4 by P2L (list4_mode 2) into c.b4: after the lists and P2M (it needs
+// the particle offsets), before fmm32_rest, whose M2L adds it.
+void fmm32_p2l(Ctx &c, const Mixed32 &mx, hipStream_t s);
 void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
                 double *out_b, bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
 void fmm32_run(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, double *out_a,
