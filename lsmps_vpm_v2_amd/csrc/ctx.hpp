@@ -129,17 +129,17 @@ struct Ctx {
     DBuf ss;
     uint32_t *key_sorted = nullptr;   // points into key_a or key_b after the sort
     uint32_t *perm_sorted = nullptr;  // sorted slot -> caller index (val_a or val_b)
-    DBuf lcp_d, lcp_w, act_scan, act_pos, leaf_lvl, ccount, cbase, leaf_of;
+    DBuf lcp_d, lcp_w, act_scan, act_pos, leaf_lvl, lv_blk, lv_tot, leaf_of;
     DBuf tmp;  // scan / sort scratch
     DBuf bbox_part;
-    DBuf bs_act, bs_cells, bs_leaf, bs_l13;  // producer block sums of scan inputs
+    DBuf bs_act, bs_leaf, bs_l13;  // producer block sums of scan inputs
     // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
     // resolved): the previous build's choice as the first guess of the next
     int sort_bits = 32;
     size_t cell_cap = 0;  // cells the emission has room for before the count is known
 
     // cells
-    DBuf pre_level, pre_slot, lev_level, lev2pre;
+    DBuf lev_level, lev_begin;
     DBuf c_level, c_flags, c_key, c_begin, c_end, c_src, c_cx, c_cy, c_parent, c_child, c_nbr;
     DBuf leaf_flag_scan, leaf_idx;
     DBuf l2, n2, l4, n4;

@@ -316,8 +316,8 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 __global__ void __launch_bounds__(kSplitBlock)
     k_split(const uint32_t *key, const uint8_t *d, const uint8_t *w, const uint32_t *A,
             const uint32_t *apos, const DevHdr *hdr, int n, int W, int S,
-            int min_level, int max_level, uint8_t *leaf_lvl, uint32_t *ccount, DevHdr *hdr_w,
-            uint32_t *bs_cells) {
+            int min_level, int max_level, uint8_t *leaf_lvl, DevHdr *hdr_w, uint32_t *blk_lv,
+            int nbs) {
     extern __shared__ uint8_t lds[];
     // cells started per level (a particle whose lcp with its predecessor is d
     // starts the cells of levels d+1 .. its leaf level, the last a leaf)
@@ -353,7 +353,6 @@ __global__ void __launch_bounds__(kSplitBlock)
     const uint8_t *mw = block_window_max(sw, gw, hw, wlen, kw);
     const uint8_t *mv = S > 0 ? block_window_max(sv, gv, hv, vlen, ks) : sv;
     int my_min = kMaxDepth + 1, my_max = 0, my_depth = 0;
-    uint32_t my_cells = 0;
     if (i < n) {
         const uint32_t ki = key[i];
         const int t = threadIdx.x;
@@ -380,8 +379,6 @@ __global__ void __launch_bounds__(kSplitBlock)
         }
         leaf_lvl[i] = (uint8_t)lvl;
         const int di = d[i];
-        my_cells = lvl > di ? (uint32_t)(lvl - di) : 0u;
-        ccount[i] = my_cells;
         for (int L = di + 1; L <= lvl; ++L) atomicAdd(&lvc[L], 1u);
         if (lvl > di) atomicAdd(&lvc[kLvCols - 1], 1u);
         my_min = lvl;
@@ -401,6 +398,8 @@ __global__ void __launch_bounds__(kSplitBlock)
         s_mm[wv][2] = my_depth;
     }
     __syncthreads();
+    // cells this block starts per level, for the level-major emission
+    if (threadIdx.x <= kMaxDepth) blk_lv[(size_t)threadIdx.x * nbs + blockIdx.x] = lvc[threadIdx.x];
     uint32_t *row = hdr_w->split_rows[blockIdx.x % kSlotRows];
     if (threadIdx.x < kLvCols) {
         const uint32_t c = lvc[threadIdx.x];
@@ -416,28 +415,86 @@ __global__ void __launch_bounds__(kSplitBlock)
         atomicMax(row + kSlotMax, (uint32_t)mx);
         if (dp) atomicOr(row + kSlotDepth, 1u);
     }
-    block_sum_store(my_cells, bs_cells + blockIdx.x);  // cells started per block
 }
 
 // ---------------------------------------------------------------------------
 // 5. cells
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-    k_emit(const uint8_t *d, const uint8_t *leaf_lvl, const uint32_t *base, int n,
-           uint32_t *pre_level, uint32_t *pre_begin, uint32_t cap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        pre_level[0] = 0;
-        pre_begin[0] = 0;
+__global__ void __launch_bounds__(256) k_lv_scan(uint32_t *blk_lv, int nbs, uint32_t *lv_tot) {
+    // exclusive scan, in place, of level blockIdx.x's per-block cell counts
+    __shared__ uint32_t sw[256 / kWave];
+    uint32_t *row = blk_lv + (size_t)blockIdx.x * nbs;
+    const int per = (nbs + 255) / 256;
+    const int b0 = threadIdx.x * per, b1 = min(b0 + per, nbs);
+    uint32_t sum = 0;
+    for (int b = b0; b < b1; ++b) sum += row[b];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t inc = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += y;
     }
-    if (i >= n) return;
-    const int di = d[i], lv = leaf_lvl[i];
-    uint32_t slot = 1 + base[i];
-    // slots beyond the capacity are dropped; the host sees the cell count and
-    // emits again into grown buffers
-    for (int L = di + 1; L <= lv && slot < cap; ++L, ++slot) {
-        pre_level[slot] = (uint32_t)L;
-        pre_begin[slot] = (uint32_t)i;
+    if (lane == kWave - 1) sw[w] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (int u = 0; u < w; ++u) run += sw[u];
+    for (int b = b0; b < b1; ++b) {
+        const uint32_t v = row[b];
+        row[b] = run;
+        run += v;
+    }
+    if (threadIdx.x == 255) lv_tot[blockIdx.x] = run;
+}
+
+// Cells in level-major order, Morton order inside a level (the order the
+// reference's level-by-level build gives): particle i starts the cells of
+// levels d[i]+1 .. leaf_lvl[i], each with first particle i.  A level-L cell
+// goes to lev_off[L] + (cells of level L started by earlier blocks, from
+// k_lv_scan) + (by earlier particles of this block, from wave ballots).  The
+// blocks are k_split's.  Slots at or beyond cap are dropped (the host sees
+// the cell count and emits again into grown buffers); block 0 writes the
+// root and the total cell count started by particles.
+__global__ void __launch_bounds__(kSplitBlock)
+    k_emit(const uint8_t *d, const uint8_t *leaf_lvl, int n, const uint32_t *blk_lv, int nbs,
+           const uint32_t *lv_tot, uint32_t *lev_level, uint32_t *lev_begin, uint32_t cap,
+           DevHdr *hdr_w) {
+    constexpr int kW = kSplitBlock / kWave;
+    __shared__ uint32_t s_base[kMaxDepth + 1];
+    __shared__ uint32_t s_w[kMaxDepth + 1][kW];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    if (tid <= kMaxDepth) {
+        uint32_t off = 1;  // the root is cell 0
+        for (int l = 1; l < tid; ++l) off += lv_tot[l];
+        s_base[tid] = off + blk_lv[(size_t)tid * nbs + blockIdx.x];
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        lev_level[0] = 0;
+        lev_begin[0] = 0;
+        uint32_t tot = 0;
+        for (int l = 1; l <= kMaxDepth; ++l) tot += lv_tot[l];
+        hdr_w->n_cells_pre = tot;
+    }
+    const int i = blockIdx.x * kSplitBlock + tid;
+    const int di = i < n ? (int)d[i] : 0, lv = i < n ? (int)leaf_lvl[i] : 0;
+    const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+#pragma unroll
+    for (int L = 1; L <= kMaxDepth; ++L) {
+        const uint64_t m = __ballot(di < L && L <= lv);
+        if (lane == 0) s_w[L][w] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int L = 1; L <= kMaxDepth; ++L) {
+        const bool mine = di < L && L <= lv;
+        const uint64_t m = __ballot(mine);
+        if (mine) {
+            uint32_t pos = s_base[L] + (uint32_t)__popcll(m & below);
+            for (int u = 0; u < w; ++u) pos += s_w[L][u];
+            if (pos < cap) {
+                lev_level[pos] = (uint32_t)L;
+                lev_begin[pos] = (uint32_t)i;
+            }
+        }
     }
 }
 
@@ -968,24 +1025,23 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *apos = c.act_pos.as<uint32_t>(n);
     scan_exclusive_bs_u8(as, A, apos, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
-    uint32_t *cnt = c.ccount.as<uint32_t>(n);
     const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
-    uint32_t *bs_cells = c.bs_cells.as<uint32_t>((size_t)nbs + 1);
+    uint32_t *blk_lv = c.lv_blk.as<uint32_t>((size_t)(kMaxDepth + 1) * nbs);
+    uint32_t *lv_tot = c.lv_tot.as<uint32_t>(kMaxDepth + 1);
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, c.hdr_d, n, W, S, tp.min_level,
-                                          tp.max_level, leaf_lvl, cnt, c.hdr_d, bs_cells);
+                                          tp.max_level, leaf_lvl, c.hdr_d, blk_lv, nbs);
     // The one synchronisation of the build: the header (split rows) is read
-    // back now; the cell-count scan and the cell emission need no host
+    // back now; the per-level scan and the cell emission need no host
     // decision and run while the host waits for the copy and decides.
     VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, s));
     stream_mark(c);
-    uint32_t *base = c.cbase.as<uint32_t>(n + 1);
-    scan_exclusive_bs_u32(cnt, base, n, bs_cells, kSplitBlock, &c.hdr_d->n_cells_pre, true, tmp,
-                          s);
+    k_lv_scan<<<kMaxDepth + 1, 256, 0, s>>>(blk_lv, nbs, lv_tot);
     // emission into buffers sized from the previous build (dropped beyond)
     const size_t cap = std::max(c.cell_cap, (size_t)n / 4 + 1024);
-    uint32_t *pl = c.pre_level.as<uint32_t>(cap), *pb = c.pre_slot.as<uint32_t>(cap);
-    k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, (uint32_t)cap);
+    uint32_t *lev_level = c.lev_level.as<uint32_t>(cap), *lev_begin = c.lev_begin.as<uint32_t>(cap);
+    k_emit<<<nbs, kSplitBlock, 0, s>>>(d, leaf_lvl, n, blk_lv, nbs, lv_tot, lev_level, lev_begin,
+                                       (uint32_t)cap, c.hdr_d);
     VPM_LAUNCH_CHECK();
     stream_mark_wait(c);
     DevHdr &hh = *c.hdr_h;
@@ -1059,15 +1115,12 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
         // more cells than the emission had room for: let it finish, grow the
         // buffers and emit again
         VPM_HIP(hipStreamSynchronize(s));
-        pl = c.pre_level.as<uint32_t>(nc);
-        pb = c.pre_slot.as<uint32_t>(nc);
-        k_emit<<<div_up(n, 256), 256, 0, s>>>(d, leaf_lvl, base, n, pl, pb, (uint32_t)nc);
+        lev_level = c.lev_level.as<uint32_t>(nc);
+        lev_begin = c.lev_begin.as<uint32_t>(nc);
+        k_emit<<<nbs, kSplitBlock, 0, s>>>(d, leaf_lvl, n, blk_lv, nbs, lv_tot, lev_level,
+                                           lev_begin, (uint32_t)nc, c.hdr_d);
     }
     c.cell_cap = (size_t)nc + nc / 4;
-    uint32_t *ll = c.lev_level.as<uint32_t>(nc), *lb = c.lev2pre.as<uint32_t>(nc);
-    const bool alt2 = radix_sort_pairs(pl, pb, ll, lb, nc, 0, 8, tmp, s);
-    const uint32_t *lev_level = alt2 ? ll : pl;
-    const uint32_t *lev_begin = alt2 ? lb : pb;
 
     uint8_t *c_level = c.c_level.as<uint8_t>(nc), *c_flags = c.c_flags.as<uint8_t>(nc);
     uint32_t *c_key = c.c_key.as<uint32_t>(nc), *c_begin = c.c_begin.as<uint32_t>(nc);
