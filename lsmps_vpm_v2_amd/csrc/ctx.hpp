@@ -132,7 +132,7 @@ struct Ctx {
     DBuf lcp_d, lcp_w, act_scan, act_pos, leaf_lvl, lv_blk, lv_tot, leaf_of;
     DBuf tmp;  // scan / sort scratch
     DBuf bbox_part;
-    DBuf bs_act, bs_leaf, bs_l13;  // producer block sums of scan inputs
+    DBuf bs_act, bs_l13;  // producer block sums of scan inputs
     // key bits the particle sort covers (top bits; levels <= sort_bits / 2 are
     // resolved): the previous build's choice as the first guess of the next
     int sort_bits = 32;
@@ -141,7 +141,7 @@ struct Ctx {
     // cells
     DBuf lev_level, lev_begin;
     DBuf c_level, c_flags, c_key, c_begin, c_end, c_src, c_cx, c_cy, c_parent, c_child, c_nbr;
-    DBuf leaf_flag_scan, leaf_idx;
+    DBuf leaf_idx;
     DBuf l2, n2, l4, n4;
     DBuf cnt1, cnt3, cntk, off1, off3, l1, l3;
 

@@ -321,8 +321,9 @@ __global__ void __launch_bounds__(kSplitBlock)
     extern __shared__ uint8_t lds[];
     // cells started per level (a particle whose lcp with its predecessor is d
     // starts the cells of levels d+1 .. its leaf level, the last a leaf)
-    __shared__ uint32_t lvc[kLvCols];
+    __shared__ uint32_t lvc[kLvCols], lvl_leaf[kMaxDepth + 1];
     if (threadIdx.x < kLvCols) lvc[threadIdx.x] = 0;
+    if (threadIdx.x <= kMaxDepth) lvl_leaf[threadIdx.x] = 0;
     const int i0 = blockIdx.x * kSplitBlock;
     const int i = i0 + threadIdx.x;
     const int ilast = min(i0 + kSplitBlock, n) - 1;
@@ -380,7 +381,10 @@ __global__ void __launch_bounds__(kSplitBlock)
         leaf_lvl[i] = (uint8_t)lvl;
         const int di = d[i];
         for (int L = di + 1; L <= lvl; ++L) atomicAdd(&lvc[L], 1u);
-        if (lvl > di) atomicAdd(&lvc[kLvCols - 1], 1u);
+        if (lvl > di) {
+            atomicAdd(&lvc[kLvCols - 1], 1u);
+            atomicAdd(&lvl_leaf[lvl], 1u);
+        }
         my_min = lvl;
         my_max = lvl;
     }
@@ -398,8 +402,11 @@ __global__ void __launch_bounds__(kSplitBlock)
         s_mm[wv][2] = my_depth;
     }
     __syncthreads();
-    // cells this block starts per level, for the level-major emission
-    if (threadIdx.x <= kMaxDepth) blk_lv[(size_t)threadIdx.x * nbs + blockIdx.x] = lvc[threadIdx.x];
+    // cells and leaves this block starts per level, for the level-major emission
+    if (threadIdx.x <= kMaxDepth) {
+        blk_lv[(size_t)threadIdx.x * nbs + blockIdx.x] = lvc[threadIdx.x];
+        blk_lv[(size_t)(kMaxDepth + 1 + threadIdx.x) * nbs + blockIdx.x] = lvl_leaf[threadIdx.x];
+    }
     uint32_t *row = hdr_w->split_rows[blockIdx.x % kSlotRows];
     if (threadIdx.x < kLvCols) {
         const uint32_t c = lvc[threadIdx.x];
@@ -421,7 +428,8 @@ __global__ void __launch_bounds__(kSplitBlock)
 // 5. cells
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_lv_scan(uint32_t *blk_lv, int nbs, uint32_t *lv_tot) {
-    // exclusive scan, in place, of level blockIdx.x's per-block cell counts
+    // exclusive scan, in place, of one row of per-block counts: rows 0 ..
+    // kMaxDepth cells per level, the next kMaxDepth + 1 rows leaves per level
     __shared__ uint32_t sw[256 / kWave];
     uint32_t *row = blk_lv + (size_t)blockIdx.x * nbs;
     const int per = (nbs + 255) / 256;
@@ -448,31 +456,41 @@ __global__ void __launch_bounds__(256) k_lv_scan(uint32_t *blk_lv, int nbs, uint
 
 // Cells in level-major order, Morton order inside a level (the order the
 // reference's level-by-level build gives): particle i starts the cells of
-// levels d[i]+1 .. leaf_lvl[i], each with first particle i.  A level-L cell
-// goes to lev_off[L] + (cells of level L started by earlier blocks, from
-// k_lv_scan) + (by earlier particles of this block, from wave ballots).  The
-// blocks are k_split's.  Slots at or beyond cap are dropped (the host sees
-// the cell count and emits again into grown buffers); block 0 writes the
-// root and the total cell count started by particles.
+// levels d[i]+1 .. leaf_lvl[i], each with first particle i, the last a leaf.
+// A level-L cell goes to lev_off[L] + (cells of level L started by earlier
+// blocks, from k_lv_scan) + (by earlier particles of this block, from wave
+// ballots); a leaf's cell index goes to leaf_idx in the same way (leaves in
+// cell order).  The blocks are k_split's.  Slots at or beyond cap are dropped
+// (the host sees the cell count and emits again into grown buffers); block 0
+// writes the root and the cell and leaf totals.
 __global__ void __launch_bounds__(kSplitBlock)
     k_emit(const uint8_t *d, const uint8_t *leaf_lvl, int n, const uint32_t *blk_lv, int nbs,
-           const uint32_t *lv_tot, uint32_t *lev_level, uint32_t *lev_begin, uint32_t cap,
-           DevHdr *hdr_w) {
+           const uint32_t *lv_tot, uint32_t *lev_level, uint32_t *lev_begin, uint32_t *leaf_idx,
+           uint32_t cap, DevHdr *hdr_w) {
     constexpr int kW = kSplitBlock / kWave;
-    __shared__ uint32_t s_base[kMaxDepth + 1];
-    __shared__ uint32_t s_w[kMaxDepth + 1][kW];
+    constexpr int kL = kMaxDepth + 1;  // row offset of the leaf counts
+    __shared__ uint32_t s_base[kMaxDepth + 1], s_lbase[kMaxDepth + 1];
+    __shared__ uint32_t s_w[kMaxDepth + 1][kW], s_lw[kMaxDepth + 1][kW];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
     if (tid <= kMaxDepth) {
-        uint32_t off = 1;  // the root is cell 0
-        for (int l = 1; l < tid; ++l) off += lv_tot[l];
+        uint32_t off = 1, loff = 0;  // the root is cell 0
+        for (int l = 1; l < tid; ++l) {
+            off += lv_tot[l];
+            loff += lv_tot[kL + l];
+        }
         s_base[tid] = off + blk_lv[(size_t)tid * nbs + blockIdx.x];
+        s_lbase[tid] = loff + blk_lv[(size_t)(kL + tid) * nbs + blockIdx.x];
     }
     if (blockIdx.x == 0 && tid == 0) {
         lev_level[0] = 0;
         lev_begin[0] = 0;
-        uint32_t tot = 0;
-        for (int l = 1; l <= kMaxDepth; ++l) tot += lv_tot[l];
+        uint32_t tot = 0, ltot = 0;
+        for (int l = 1; l <= kMaxDepth; ++l) {
+            tot += lv_tot[l];
+            ltot += lv_tot[kL + l];
+        }
         hdr_w->n_cells_pre = tot;
+        hdr_w->n_leaves = ltot;
     }
     const int i = blockIdx.x * kSplitBlock + tid;
     const int di = i < n ? (int)d[i] : 0, lv = i < n ? (int)leaf_lvl[i] : 0;
@@ -480,19 +498,29 @@ __global__ void __launch_bounds__(kSplitBlock)
 #pragma unroll
     for (int L = 1; L <= kMaxDepth; ++L) {
         const uint64_t m = __ballot(di < L && L <= lv);
-        if (lane == 0) s_w[L][w] = (uint32_t)__popcll(m);
+        const uint64_t ml = __ballot(di < L && L == lv);
+        if (lane == 0) {
+            s_w[L][w] = (uint32_t)__popcll(m);
+            s_lw[L][w] = (uint32_t)__popcll(ml);
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int L = 1; L <= kMaxDepth; ++L) {
         const bool mine = di < L && L <= lv;
         const uint64_t m = __ballot(mine);
+        const uint64_t ml = __ballot(mine && L == lv);
         if (mine) {
             uint32_t pos = s_base[L] + (uint32_t)__popcll(m & below);
             for (int u = 0; u < w; ++u) pos += s_w[L][u];
             if (pos < cap) {
                 lev_level[pos] = (uint32_t)L;
                 lev_begin[pos] = (uint32_t)i;
+                if (L == lv) {  // the leaf over particle i
+                    uint32_t r = s_lbase[L] + (uint32_t)__popcll(ml & below);
+                    for (int u = 0; u < w; ++u) r += s_lw[L][u];
+                    leaf_idx[r] = pos;
+                }
             }
         }
     }
@@ -504,7 +532,7 @@ __device__ __forceinline__ uint32_t cell_attrs(
     int j, const uint32_t *lev_level, const uint32_t *lev_begin, const uint32_t *key,
     const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr, uint8_t *c_level,
     uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin, uint32_t *c_end, uint32_t *c_src,
-    double *c_cx, double *c_cy, int *c_child, uint8_t *leaf01) {
+    double *c_cx, double *c_cy, int *c_child) {
 #pragma clang fp contract(off)
     const int L = (int)lev_level[j];
     const uint32_t b = lev_begin[j];
@@ -551,7 +579,6 @@ __device__ __forceinline__ uint32_t cell_attrs(
     c_cx[j] = hdr->pivot[0] + ((0.5 + (double)ix) * len);
     c_cy[j] = hdr->pivot[1] + ((0.5 + (double)iy) * len);
     reinterpret_cast<int4 *>(c_child)[j] = make_int4(-1, -1, -1, -1);  // filled by k_cells2
-    leaf01[j] = (f & kLeaf) ? 1 : 0;
     return (f & kLeaf) ? 1u : 0u;
 }
 
@@ -560,7 +587,7 @@ __global__ void __launch_bounds__(256)
              const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
              uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
              uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
-             uint8_t *leaf01, uint32_t *bs_leaf, SplitLevels sl, DevHdr *hdr_w) {
+             SplitLevels sl, DevHdr *hdr_w) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     // level offsets / counts and the leaf-level range for the later kernels
     if (blockIdx.x == 0 && threadIdx.x < kMaxDepth + 2) {
@@ -571,11 +598,9 @@ __global__ void __launch_bounds__(256)
             hdr_w->leaf_lvl_max = sl.leaf_lvl_max;
         }
     }
-    uint32_t lf = 0;
     if (j < nc)
-        lf = cell_attrs(j, lev_level, lev_begin, key, A, leaf_lvl, n, hdr, c_level, c_flags, c_key,
-                        c_begin, c_end, c_src, c_cx, c_cy, c_child, leaf01);
-    block_sum_store(lf, bs_leaf + blockIdx.x);  // leaves per block
+        cell_attrs(j, lev_level, lev_begin, key, A, leaf_lvl, n, hdr, c_level, c_flags, c_key,
+                   c_begin, c_end, c_src, c_cx, c_cy, c_child);
 }
 
 __device__ __forceinline__ int find_cell(const uint32_t *c_key, const uint32_t *lev_off, int level,
@@ -1000,7 +1025,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *va = c.val_a.as<uint32_t>(n), *vb = c.val_b.as<uint32_t>(n);
     const size_t tmp_words =
         std::max(radix_tmp_words(n), scan_tmp_words((long long)n + 1) + 64) + 64;
-    uint32_t *tmp = c.tmp.as<uint32_t>(std::max(tmp_words, radix_tmp_words(16LL * n + 16)));
+    uint32_t *tmp = c.tmp.as<uint32_t>(tmp_words);
     // the top sort_bits of the key (levels <= sort_bits / 2 resolved), a guess
     // from the previous build checked after the split
     const int bits = c.sort_bits;
@@ -1027,8 +1052,8 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
-    uint32_t *blk_lv = c.lv_blk.as<uint32_t>((size_t)(kMaxDepth + 1) * nbs);
-    uint32_t *lv_tot = c.lv_tot.as<uint32_t>(kMaxDepth + 1);
+    uint32_t *blk_lv = c.lv_blk.as<uint32_t>((size_t)2 * (kMaxDepth + 1) * nbs);
+    uint32_t *lv_tot = c.lv_tot.as<uint32_t>(2 * (kMaxDepth + 1));
     k_split<<<nbs, kSplitBlock, lds, s>>>(key, d, w, A, apos, c.hdr_d, n, W, S, tp.min_level,
                                           tp.max_level, leaf_lvl, c.hdr_d, blk_lv, nbs);
     // The one synchronisation of the build: the header (split rows) is read
@@ -1036,12 +1061,13 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     // decision and run while the host waits for the copy and decides.
     VPM_HIP(hipMemcpyAsync(c.hdr_h, c.hdr_d, sizeof(DevHdr), hipMemcpyDeviceToHost, s));
     stream_mark(c);
-    k_lv_scan<<<kMaxDepth + 1, 256, 0, s>>>(blk_lv, nbs, lv_tot);
+    k_lv_scan<<<2 * (kMaxDepth + 1), 256, 0, s>>>(blk_lv, nbs, lv_tot);
     // emission into buffers sized from the previous build (dropped beyond)
     const size_t cap = std::max(c.cell_cap, (size_t)n / 4 + 1024);
     uint32_t *lev_level = c.lev_level.as<uint32_t>(cap), *lev_begin = c.lev_begin.as<uint32_t>(cap);
+    uint32_t *leaf_idx = c.leaf_idx.as<uint32_t>(cap);
     k_emit<<<nbs, kSplitBlock, 0, s>>>(d, leaf_lvl, n, blk_lv, nbs, lv_tot, lev_level, lev_begin,
-                                       (uint32_t)cap, c.hdr_d);
+                                       leaf_idx, (uint32_t)cap, c.hdr_d);
     VPM_LAUNCH_CHECK();
     stream_mark_wait(c);
     DevHdr &hh = *c.hdr_h;
@@ -1117,8 +1143,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
         VPM_HIP(hipStreamSynchronize(s));
         lev_level = c.lev_level.as<uint32_t>(nc);
         lev_begin = c.lev_begin.as<uint32_t>(nc);
+        leaf_idx = c.leaf_idx.as<uint32_t>(nc);
         k_emit<<<nbs, kSplitBlock, 0, s>>>(d, leaf_lvl, n, blk_lv, nbs, lv_tot, lev_level,
-                                           lev_begin, (uint32_t)nc, c.hdr_d);
+                                           lev_begin, leaf_idx, (uint32_t)nc, c.hdr_d);
     }
     c.cell_cap = (size_t)nc + nc / 4;
 
@@ -1127,19 +1154,10 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *c_end = c.c_end.as<uint32_t>(nc), *c_src = c.c_src.as<uint32_t>(nc);
     double *c_cx = c.c_cx.as<double>(nc), *c_cy = c.c_cy.as<double>(nc);
     int *c_parent = c.c_parent.as<int>(nc), *c_child = c.c_child.as<int>(4 * (size_t)nc);
-    uint8_t *leaf01 = c.lcp_w.as<uint8_t>(std::max(n + 1, nc));
-    uint32_t *bs_leaf = c.bs_leaf.as<uint32_t>((size_t)div_up(nc, 256) + 1);
     k_cells1<<<div_up(nc, 256), 256, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
-                                             c_src, c_cx, c_cy, c_child, leaf01, bs_leaf, sl,
-                                             c.hdr_d);
+                                             c_src, c_cx, c_cy, c_child, sl, c.hdr_d);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
-
-    // leaf cells in cell order (scan of the leaf flags, compacted)
-    uint32_t *leaf_idx = c.leaf_idx.as<uint32_t>(nc);
-    uint32_t *lscan = c.leaf_flag_scan.as<uint32_t>(nc + 1);
-    scan_exclusive_bs_u8(leaf01, lscan, leaf_idx, nc, bs_leaf, 256, &c.hdr_d->n_leaves, false,
-                         tmp, s);
     VPM_LAUNCH_CHECK();
 }
 
