@@ -495,8 +495,21 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int i = blockIdx.x * kSplitBlock + tid;
     const int di = i < n ? (int)d[i] : 0, lv = i < n ? (int)leaf_lvl[i] : 0;
     const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
-#pragma unroll
-    for (int L = 1; L <= kMaxDepth; ++L) {
+    // per-wave counts start at zero: a wave visits only the levels its own
+    // particles start (a few of the 16)
+    if (tid < (kMaxDepth + 1) * kW) {
+        (&s_w[0][0])[tid] = 0;
+        (&s_lw[0][0])[tid] = 0;
+    }
+    int lo = lv > di ? di + 1 : kMaxDepth + 1, hi = lv > di ? lv : 0;
+    for (int m = 1; m < kWave; m <<= 1) {
+        lo = min(lo, __shfl_xor(lo, m, kWave));
+        hi = max(hi, __shfl_xor(hi, m, kWave));
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
+    __syncthreads();
+    for (int L = lo; L <= hi; ++L) {
         const uint64_t m = __ballot(di < L && L <= lv);
         const uint64_t ml = __ballot(di < L && L == lv);
         if (lane == 0) {
@@ -505,8 +518,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int L = 1; L <= kMaxDepth; ++L) {
+    for (int L = lo; L <= hi; ++L) {
         const bool mine = di < L && L <= lv;
         const uint64_t m = __ballot(mine);
         const uint64_t ml = __ballot(mine && L == lv);
