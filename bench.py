@@ -184,7 +184,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              f"decomposition is a design (DESIGN.md), not a measured path.", file=sys.stderr)
+              f"decomposition (Morton-range partition, top-tree all-gather, halo exchange) "
+              f"is designed, not built or measured.", file=sys.stderr)
         sys.exit(2)
 
     import torch
