@@ -730,6 +730,7 @@ struct EvalArgs {
     int kst, mask, velocity;
     double u_inf, v_inf;
     double *out_a, *out_b;
+    unsigned *flags;  // header flags (DUPLICATE_POINTS)
 };
 
 template <int G, int ICUT>
@@ -756,6 +757,7 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
         // each staging round sums in FP32; the round totals add in FP64, so
         // leaves with very many sources do not lose the FP32 accumulation
         double exd = 0.0, eyd = 0.0;
+        bool dup = false;
         if ((A.mask & 1) && own_src) {  // P2P from the leaf's own particles
             for (uint32_t base = b; base < e; base += kStage) {
                 const int fill = (int)min<uint32_t>((uint32_t)kStage, e - base);
@@ -774,6 +776,9 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
                 wave_sync();
                 float ex = 0.f, ey = 0.f;
                 p2p_round<G, true, ICUT>(stg + g, sst + g, fillp / G, ti, ts4, i, ex, ey);
+                // two particles at one position (only ever in one leaf) give
+                // r = 0 and a non-finite sum: the singular kernel's error
+                if (ICUT == 0) dup |= valid && !(isfinite(ex) && isfinite(ey));
                 exd += (double)ex;
                 eyd += (double)ey;
                 wave_sync();
@@ -885,6 +890,7 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
                 A.out_b[o] = eyd;
             }
         }
+        if (dup) atomicOr(A.flags, kFlagDuplicate);
     }
 }
 
@@ -1151,7 +1157,8 @@ void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, doubl
     EvalArgs ea{leaf_idx, c_level, c_flags, c_begin, c_end, c_cx, c_cy,
                 (const uint32_t *)c.off1.p, (const int *)c.l1.p, (const uint32_t *)c.off3.p,
                 (const int *)c.l3.p, oxy, qf, (const double *)c.ss.p, c.perm_sorted, mx.lt,
-                ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a, out_b};
+                ahat, bhat, kst, stage_mask, velocity ? 1 : 0, u_inf, v_inf, out_a, out_b,
+                &c.hdr_d->flags};
     const int nb = div_up(nl, kEvalWaves), nth = kWave * kEvalWaves;
     switch (c.icut) {
         case 1: k_eval32<1><<<nb, nth, 0, s>>>(ea, nl); break;

@@ -108,3 +108,43 @@ def test_single_particle_is_a_degenerate_domain(gpu):
     x, y, q, a = _field(100, 3)
     ex, ey = s.field(x, y, q, a)
     assert np.all(np.isfinite(ex)) and np.all(np.isfinite(ey))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_duplicate_points_raise(gpu, oracle, precision):
+    """Two active particles at one position: the singular P2P has r = 0 (the
+    reference would divide by zero).  Both paths report DUPLICATE_POINTS."""
+    x, y, q, a = _field(2000, 21)
+    x[7], y[7] = x[3], y[3]
+    kw = dict(p=10) if precision == "fp64" else dict(p=10, precision="mixed", binom_mode=0,
+                                                     list4_mode=2)
+    s = FmmSolver(**kw)
+    with pytest.raises(VpmFmmError) as e:
+        s.field(x, y, q, a)
+    assert e.value.status == "DUPLICATE_POINTS"
+    # the handle still works on a clean field
+    x2, y2, q2, a2 = _field(2000, 22)
+    ex0, ey0 = oracle.fmm_field(x2, y2, q2, a2, oracle.opts(
+        p=10, **({} if precision == "fp64" else dict(binom_mode=0, list4_mode=2))))
+    ex, ey = s.field(x2, y2, q2, a2)
+    _check(ex, ey, ex0, ey0, 1e-10 if precision == "fp64" else
+           5e-6 + 2 * _fp32_floor(oracle, x2, y2, q2, a2))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_depth_overflow_raises(gpu, oracle, precision):
+    """100 particles inside a 1e-9 square with the 75-particle leaf rule need
+    more than 16 levels: TREE_DEPTH_EXCEEDED, as the oracle reports."""
+    x, y, q, a = _field(1000, 23)
+    rng = np.random.default_rng(24)
+    x = np.concatenate([x, 0.3 + 1e-9 * rng.uniform(0, 1, 100)])
+    y = np.concatenate([y, 0.2 + 1e-9 * rng.uniform(0, 1, 100)])
+    q = np.concatenate([q, rng.normal(0, 1, 100)])
+    a = np.ones(len(x), np.uint8)
+    with pytest.raises(oracle.OracleError):
+        oracle.fmm_field(x, y, q, a, oracle.opts(p=10))
+    kw = dict(p=10) if precision == "fp64" else dict(p=10, precision="mixed", binom_mode=0,
+                                                     list4_mode=2)
+    with pytest.raises(VpmFmmError) as e:
+        FmmSolver(**kw).field(x, y, q, a)
+    assert e.value.status == "TREE_DEPTH_EXCEEDED"
