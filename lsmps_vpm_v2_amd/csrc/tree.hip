@@ -343,16 +343,24 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int vlast = na - 1 - S;
     const int vlen = r_hi - vbase + 1;
     uint8_t *sv = hw + wlen, *gv = sv + vlen, *hv = gv + vlen;
-    // v[m] = lcp(key of active m, key of active m + S), formed here from the
-    // compacted active positions
-    for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
-        const int m = vbase + t;
-        sv[t] = (m >= 0 && m <= vlast) ? (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]]) : 0;
+    const int kw = ilog2(W + 1), ks = S > 0 ? ilog2(S) : 0;
+    // Every particle active and S == W: the active positions are the
+    // identity, v[m] = w[m], so the active window is the particle window
+    // shifted by one (sv[a] = sw[a - 1]) and, with equal doubling depths, its
+    // maxima are those of the particle window: one table instead of two.
+    const bool same = na == n && S == W && kw == ks;
+    if (!same) {
+        // v[m] = lcp(key of active m, key of active m + S), formed here from
+        // the compacted active positions
+        for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
+            const int m = vbase + t;
+            sv[t] = (m >= 0 && m <= vlast) ? (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]])
+                                           : 0;
+        }
     }
     __syncthreads();
-    const int kw = ilog2(W + 1), ks = S > 0 ? ilog2(S) : 0;
     const uint8_t *mw = block_window_max(sw, gw, hw, wlen, kw);
-    const uint8_t *mv = S > 0 ? block_window_max(sv, gv, hv, vlen, ks) : sv;
+    const uint8_t *mv = (S > 0 && !same) ? block_window_max(sv, gv, hv, vlen, ks) : sv;
     int my_min = kMaxDepth + 1, my_max = 0, my_depth = 0;
     if (i < n) {
         const uint32_t ki = key[i];
@@ -362,14 +370,21 @@ __global__ void __launch_bounds__(kSplitBlock)
         const int a = r - S - vbase;  // active window [r - S, r - 1] in LDS coordinates
         // window [t, t + W] and active window [a, a + S - 1] from their end 2^k maxima
         if (wlast >= 0) par_split = (int)max(mw[t], mw[t + W + 1 - (1 << kw)]);
-        if (S > 0) src_split = (int)max(mv[a], mv[a + S - (1 << ks)]);
+        if (S > 0)
+            src_split = same ? (int)max(mw[a - 1], mw[a - 1 + S - (1 << ks)])
+                             : (int)max(mv[a], mv[a + S - (1 << ks)]);
         if (r <= vlast) {  // run starting at the first active at or after i
-            const int e = min((int)sv[r - vbase], lcp_levels(ki, key[apos[r]]));
+            const int vr = same ? (int)sw[r - vbase - 1] : (int)sv[r - vbase];
+            const int e = min(vr, lcp_levels(ki, key[apos[r]]));
             src_split = max(src_split, e);
         }
         const int m0 = r - S - 1;
         if (m0 >= 0 && m0 <= vlast) {  // run ending at the last active before i
-            const int e = min((int)sv[m0 - vbase], lcp_levels(key[apos[r - 1]], ki));
+            // (v[m0] for the block's first particle lies just before the window)
+            const int vm = !same ? (int)sv[m0 - vbase]
+                           : m0 - vbase >= 1 ? (int)sw[m0 - vbase - 1]
+                                             : lcp_levels(key[m0], key[m0 + S]);
+            const int e = min(vm, lcp_levels(key[apos[r - 1]], ki));
             src_split = max(src_split, e);
         }
         int lvl = max(max(par_split, src_split) + 1, max(min_level, 1));

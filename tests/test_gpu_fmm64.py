@@ -55,10 +55,15 @@ def test_field_modes_match_oracle(gpu, oracle, binom_mode, list4_mode):
     assert rel_l2(ex, ey, ex0, ey0) <= TOL
 
 
-def test_tree_and_lists_bit_exact(gpu, oracle):
-    f = _case("inactive", 4000, 5)
-    t = oracle.Tree(f["x"], f["y"], f["active"])
-    s = FmmSolver()
+@pytest.mark.parametrize("kind,max_par,max_src", [("inactive", 75, 50), ("clusters", 48, 48),
+                                                  ("clusters", 64, 64), ("uniform", 32, 32),
+                                                  ("uniform", 63, 63)])
+def test_tree_and_lists_bit_exact(gpu, oracle, kind, max_par, max_src):
+    """Reference leaf rule (75/50, some inactive) and the equal thresholds of
+    the learned path on all-active fields."""
+    f = _case(kind, 4000, 5)
+    t = oracle.Tree(f["x"], f["y"], f["active"], oracle.opts(max_par=max_par, max_src=max_src))
+    s = FmmSolver(max_par=max_par, max_src=max_src)
     s.build_tree(f["x"], f["y"], f["active"])
     assert s.tree_counts() == t.counts()
     co, cg = t.cells(), s.tree_cells()
