@@ -33,7 +33,11 @@ constexpr int kSlotMinInv = kLvCols, kSlotMax = kLvCols + 1, kSlotDepth = kLvCol
 // (target, source) pairs, (target, list-3 cell) evaluations, and the M2P /
 // L2P terms (sum of the order + 1 of each list-3 cell per target; the order
 // of each particle's leaf).
-enum StatCol { kStSum2, kStSum4, kStPairs, kStM2p, kStM2pTerms, kStL2pTerms, kStCols = 8 };
+enum StatCol {
+    kStSum2, kStSum4, kStPairs, kStM2p, kStM2pTerms, kStL2pTerms,
+    kStInactive,  // particles with active = 0 (the gather counts them)
+    kStCols = 8
+};
 constexpr int kStatRows = 16;
 
 // Header written by device kernels and read back by the host.

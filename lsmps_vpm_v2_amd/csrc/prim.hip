@@ -183,8 +183,27 @@ __device__ __forceinline__ void load_items(const uint32_t *in, long long base, i
 // blockIdx.y selects one of up to two independent scans of the same length.
 template <typename T>
 __global__ void __launch_bounds__(kScanBlock)
-    k_scan_down_bs(ScanBs<T> a0, ScanBs<T> a1, int n, int per, int nbs, int write_end) {
+    k_scan_down_bs(ScanBs<T> a0, ScanBs<T> a1, int n, int per, int nbs, int write_end,
+                   const unsigned long long *skip, int skip_n, int skip_stride) {
     __shared__ uint32_t sm[kScanBlock / kWave];
+    if (skip) {
+        // the producer counted the zeros of a 0/1 input: none means out and
+        // compact are the identity, which the caller then uses unwritten
+        __shared__ int s_skip;
+        if (threadIdx.x == 0) {
+            unsigned long long z = 0;
+            for (int k = 0; k < skip_n; ++k) z += skip[(size_t)k * skip_stride];
+            s_skip = z == 0;
+        }
+        __syncthreads();
+        if (s_skip) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                if (a0.total) *a0.total = (uint32_t)n;
+                if (write_end && a0.out) a0.out[n] = (uint32_t)n;
+            }
+            return;
+        }
+    }
     const ScanBs<T> &a = blockIdx.y == 0 ? a0 : a1;
     const T *in = a.in;
     uint32_t *out = a.out, *compact = a.compact, *total = a.total;
@@ -399,7 +418,8 @@ __global__ void __launch_bounds__(256)
 
 template <typename T>
 void scan_bs_impl(const T *in, uint32_t *out, uint32_t *compact, int n, uint32_t *bsum, int per,
-                  uint32_t *total, bool write_end, uint32_t *tmp, hipStream_t s) {
+                  uint32_t *total, bool write_end, uint32_t *tmp, hipStream_t s,
+                  const unsigned long long *skip = nullptr, int skip_n = 0, int skip_stride = 0) {
     if (n <= 0) {
         k_write_zero_total<<<1, 1, 0, s>>>(out, 0, total, (write_end && out) ? 1 : 0);
         VPM_LAUNCH_CHECK();
@@ -418,7 +438,8 @@ void scan_bs_impl(const T *in, uint32_t *out, uint32_t *compact, int n, uint32_t
         return;
     }
     const ScanBs<T> a{in, out, compact, bsum, total};
-    k_scan_down_bs<T><<<nb, kScanBlock, 0, s>>>(a, a, n, per, nbs, write_end ? 1 : 0);
+    k_scan_down_bs<T><<<nb, kScanBlock, 0, s>>>(a, a, n, per, nbs, write_end ? 1 : 0, skip,
+                                                skip_n, skip_stride);
     VPM_LAUNCH_CHECK();
 }
 
@@ -443,14 +464,16 @@ void scan_exclusive_bs_u32_pair(const uint32_t *in0, uint32_t *out0, uint32_t *b
     const ScanBs<uint32_t> a0{in0, out0, nullptr, bsum0, total0};
     const ScanBs<uint32_t> a1{in1, out1, nullptr, bsum1, total1};
     k_scan_down_bs<uint32_t><<<dim3(div_up(n, kScanTile), 2), kScanBlock, 0, s>>>(
-        a0, a1, n, per, nbs, write_total_at_end ? 1 : 0);
+        a0, a1, n, per, nbs, write_total_at_end ? 1 : 0, nullptr, 0, 0);
     VPM_LAUNCH_CHECK();
 }
 
 void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, uint32_t *compact, int n,
                           uint32_t *bsum, int per, uint32_t *total, bool write_total_at_end,
-                          uint32_t *tmp, hipStream_t s) {
-    scan_bs_impl<uint8_t>(in, out, compact, n, bsum, per, total, write_total_at_end, tmp, s);
+                          uint32_t *tmp, hipStream_t s, const unsigned long long *zeros,
+                          int zeros_n, int zeros_stride) {
+    scan_bs_impl<uint8_t>(in, out, compact, n, bsum, per, total, write_total_at_end, tmp, s,
+                          zeros, zeros_n, zeros_stride);
 }
 
 size_t radix_tmp_words(long long n) {

@@ -28,9 +28,13 @@ void scan_exclusive_bs_u32_pair(const uint32_t *in0, uint32_t *out0, uint32_t *b
                                 bool write_total_at_end, uint32_t *tmp, hipStream_t s);
 // For 0/1 flags: out (may be null) gets the scan, compact (may be null) the
 // positions of the ones in order (compact[out[i]] = i where in[i] = 1).
+// zeros (may be null): zeros_n device counters, zeros_stride apart, whose sum
+// is the number of zero flags; when it is 0, only the total (n, and out[n])
+// may be written and the caller reads out and compact as the identity.
 void scan_exclusive_bs_u8(const uint8_t *in, uint32_t *out, uint32_t *compact, int n,
                           uint32_t *bsum, int per, uint32_t *total, bool write_total_at_end,
-                          uint32_t *tmp, hipStream_t s);
+                          uint32_t *tmp, hipStream_t s, const unsigned long long *zeros = nullptr,
+                          int zeros_n = 0, int zeros_stride = 0);
 
 // Scratch words needed by the radix sort for n elements.
 size_t radix_tmp_words(long long n);

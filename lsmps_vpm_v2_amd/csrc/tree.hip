@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
              const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
              double *ys, double *qs, uint8_t *as, uint32_t *bs_act, const uint32_t *key, int W,
-             uint8_t *d, uint8_t *w, double *ss) {
+             uint8_t *d, uint8_t *w, double *ss, DevHdr *hdr_w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t act = 0;
     if (i < n) {
@@ -254,6 +254,12 @@ __global__ void __launch_bounds__(256)
         act = a[o] ? 1u : 0u;
         as[i] = (uint8_t)act;
     }
+    // inactive particles per wavefront into the block's statistic row (no
+    // atomic at all when every particle is active)
+    const uint64_t inact = __ballot(i < n && !act);
+    if (inact && (threadIdx.x & (kWave - 1)) == 0)
+        atomicAdd(&hdr_w->stat[blockIdx.x % kStatRows][kStInactive],
+                  (unsigned long long)__popcll(inact));
     block_sum_store(act, bs_act + blockIdx.x);  // active particles per block
 }
 
@@ -328,6 +334,11 @@ __global__ void __launch_bounds__(kSplitBlock)
     const int i = i0 + threadIdx.x;
     const int ilast = min(i0 + kSplitBlock, n) - 1;
     const int na = (int)hdr->n_active;
+    // every particle active: the active ranks and positions are the identity
+    // (the scan then leaves A and apos unwritten)
+    const bool all_act = na == n;
+    auto rank = [&](int x) { return all_act ? x : (int)A[x]; };
+    auto apos_at = [&](int m) { return all_act ? (uint32_t)m : apos[m]; };
     // runs of W+1 particles: start j in [i - W, i]; LDS index j - (i0 - W)
     const int wlen = W + kSplitBlock;
     const int wbase = i0 - W;
@@ -338,7 +349,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         sw[t] = (j >= 0 && j <= wlast) ? w[j] : 0;
     }
     // runs of S+1 active particles: start m in [r - S, r - 1]; LDS index m - vbase
-    const int r_lo = (int)A[i0], r_hi = (int)A[ilast];
+    const int r_lo = rank(i0), r_hi = rank(ilast);
     const int vbase = r_lo - S - 1;
     const int vlast = na - 1 - S;
     const int vlen = r_hi - vbase + 1;
@@ -348,14 +359,15 @@ __global__ void __launch_bounds__(kSplitBlock)
     // identity, v[m] = w[m], so the active window is the particle window
     // shifted by one (sv[a] = sw[a - 1]) and, with equal doubling depths, its
     // maxima are those of the particle window: one table instead of two.
-    const bool same = na == n && S == W && kw == ks;
+    const bool same = all_act && S == W && kw == ks;
     if (!same) {
         // v[m] = lcp(key of active m, key of active m + S), formed here from
         // the compacted active positions
         for (int t = threadIdx.x; t < vlen; t += kSplitBlock) {
             const int m = vbase + t;
-            sv[t] = (m >= 0 && m <= vlast) ? (uint8_t)lcp_levels(key[apos[m]], key[apos[m + S]])
-                                           : 0;
+            sv[t] = (m >= 0 && m <= vlast)
+                        ? (uint8_t)lcp_levels(key[apos_at(m)], key[apos_at(m + S)])
+                        : 0;
         }
     }
     __syncthreads();
@@ -366,7 +378,7 @@ __global__ void __launch_bounds__(kSplitBlock)
         const uint32_t ki = key[i];
         const int t = threadIdx.x;
         int par_split = -1, src_split = -1;
-        const int r = (int)A[i];
+        const int r = rank(i);
         const int a = r - S - vbase;  // active window [r - S, r - 1] in LDS coordinates
         // window [t, t + W] and active window [a, a + S - 1] from their end 2^k maxima
         if (wlast >= 0) par_split = (int)max(mw[t], mw[t + W + 1 - (1 << kw)]);
@@ -375,7 +387,7 @@ __global__ void __launch_bounds__(kSplitBlock)
                              : (int)max(mv[a], mv[a + S - (1 << ks)]);
         if (r <= vlast) {  // run starting at the first active at or after i
             const int vr = same ? (int)sw[r - vbase - 1] : (int)sv[r - vbase];
-            const int e = min(vr, lcp_levels(ki, key[apos[r]]));
+            const int e = min(vr, lcp_levels(ki, key[apos_at(r)]));
             src_split = max(src_split, e);
         }
         const int m0 = r - S - 1;
@@ -384,7 +396,7 @@ __global__ void __launch_bounds__(kSplitBlock)
             const int vm = !same ? (int)sv[m0 - vbase]
                            : m0 - vbase >= 1 ? (int)sw[m0 - vbase - 1]
                                              : lcp_levels(key[m0], key[m0 + S]);
-            const int e = min(vm, lcp_levels(key[apos[r - 1]], ki));
+            const int e = min(vm, lcp_levels(key[apos_at(r - 1)], ki));
             src_split = max(src_split, e);
         }
         int lvl = max(max(par_split, src_split) + 1, max(min_level, 1));
@@ -590,7 +602,8 @@ __device__ __forceinline__ uint32_t cell_attrs(
         }
         e = (uint32_t)hi;
     }
-    const uint32_t src = A[e] - A[b];
+    // every particle active: the scan left A unwritten (the identity)
+    const uint32_t src = hdr->n_active == (uint32_t)n ? e - b : A[e] - A[b];
     unsigned f = 0;
     if (L > 0 && (int)leaf_lvl[b] == L) f |= kLeaf;
     if (src > 0) f |= kActive;
@@ -1070,12 +1083,14 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
     double *ss = (c.icut > 0 && src.s) ? c.ss.as<double>(n) : nullptr;
     k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
-                                            ys, qs, as, bs_act, key, W, d, w, ss);
+                                            ys, qs, as, bs_act, key, W, d, w, ss, c.hdr_d);
 
     // 4. leaf levels, cells per level and leaves
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
     uint32_t *apos = c.act_pos.as<uint32_t>(n);
-    scan_exclusive_bs_u8(as, A, apos, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s);
+    // (every particle active: A and apos are the identity and stay unwritten)
+    scan_exclusive_bs_u8(as, A, apos, n, bs_act, 256, &c.hdr_d->n_active, true, tmp, s,
+                         &c.hdr_d->stat[0][kStInactive], kStatRows, kStCols);
     uint8_t *leaf_lvl = c.leaf_lvl.as<uint8_t>(n);
     const size_t lds = 3 * (size_t)(W + kSplitBlock) + 3 * (size_t)(S + kSplitBlock + 2) + 16;
     const int nbs = div_up(n, kSplitBlock);
