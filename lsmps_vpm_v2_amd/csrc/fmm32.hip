@@ -692,7 +692,9 @@ __device__ __forceinline__ float regul32(float rho2) {
 template <int G, bool SELF, int ICUT>
 __device__ __forceinline__ void p2p_round(const float4 *st, const float *sst, int nu, float2 ti,
                                           float ts4, uint32_t i, float &ex, float &ey) {
-#pragma unroll 4
+    // (the regularized kernels' wave-uniform branch keeps their loop rolled)
+    constexpr int kUnroll = ICUT ? 1 : 4;
+#pragma unroll kUnroll
     for (int u = 0; u < nu; ++u) {
         const float4 sv = st[u * G];
         const float dx = ti.x - sv.x, dy = ti.y - sv.y;
@@ -1118,7 +1120,6 @@ void fmm32_rest(Ctx &c, const Mixed32 &mx, int list4_mode, int stage_mask, doubl
     const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
     const int *c_child = (const int *)c.c_child.p;
     const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
-    const uint8_t *as = (const uint8_t *)c.acts.p;
     const int maxL = ti.max_level;
     int K, T;
     tier_levels(ti, K, T);

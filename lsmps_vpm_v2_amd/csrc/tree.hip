@@ -54,9 +54,6 @@ __device__ __forceinline__ int lcp_levels(uint32_t a, uint32_t b) {
     uint32_t x = a ^ b;
     return x == 0 ? kMaxDepth : (__clz(x) >> 1);
 }
-__device__ __forceinline__ uint32_t prefix_at(uint32_t key, int level) {
-    return level == 0 ? 0u : key >> (32 - 2 * level);
-}
 // Cell side at a level: R / 2^L (exact).
 __device__ __forceinline__ double cell_len(double R, int level) {
     return R / (double)(1u << level);
