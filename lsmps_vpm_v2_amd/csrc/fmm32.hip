@@ -222,11 +222,20 @@ __device__ __forceinline__ void l2l_wave(int j, int pp, int pc, int lane, int PM
     wave_sync();
 }
 
-// Table (4 PM1 x PM1 complex) and the order of every level into LDS.
+// Table (4 PM1 x PM1 complex, stride PM1) and the order of every level into
+// LDS.  Only the entries (q, m, k) with m, k up to the largest order of levels
+// lo .. hi are loaded: the passes over those levels read no others.
 __device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab,
-                                         const LevelTab *lt, int *s_lp) {
+                                         const LevelTab *lt, int *s_lp, int lo, int hi) {
+    int pm = 0;
+    for (int L = lo; L <= hi; ++L) pm = max(pm, lt->p[L]);
+    const int P1 = min(pm, PM1 - 1) + 1;
     const float2 *src = reinterpret_cast<const float2 *>(g);
-    for (int k = threadIdx.x; k < 4 * PM1 * PM1; k += blockDim.x) tab[k] = src[k];
+    for (int u = threadIdx.x; u < 4 * P1 * P1; u += blockDim.x) {
+        const int q = u / (P1 * P1), r = u - q * P1 * P1, m = r / P1, k = r - m * P1;
+        const int idx = (q * PM1 + m) * PM1 + k;
+        tab[idx] = src[idx];
+    }
     if (threadIdx.x < kMaxDepth + 2) s_lp[threadIdx.x] = lt->p[threadIdx.x];
     __syncthreads();
 }
@@ -277,7 +286,7 @@ __global__ void __launch_bounds__(kSubBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab, lt, s_lp);
+    load_tab(m2m, PM1, tab, lt, s_lp, K, maxL);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
     for (int L = maxL - 1; L >= K; --L) {
         for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
@@ -297,7 +306,7 @@ __global__ void __launch_bounds__(kTopBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kTopBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab, lt, s_lp);
+    load_tab(m2m, PM1, tab, lt, s_lp, 2, Lhi + 1);
     for (int L = Lhi; L >= 2; --L) {
         for (int j = s_off[L] + w; j < s_off[L + 1]; j += kTopBlock / kWave)
             m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
@@ -313,7 +322,7 @@ __global__ void __launch_bounds__(kSubBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sa[kSubBlock / kWave][4 * kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(m2m, PM1, tab, lt, s_lp);
+    load_tab(m2m, PM1, tab, lt, s_lp, L, L + 1);
     const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
     if (j < j1) {
         m2m_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sa[w], ahat);
@@ -327,7 +336,7 @@ __global__ void __launch_bounds__(kSubBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab, lt, s_lp);
+    load_tab(l2l, PM1, tab, lt, s_lp, L, L + 1);
     const int j = j0 + blockIdx.x * (kSubBlock / kWave) + w;
     if (j < j1) {
         l2l_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sb[w], bhat);
@@ -343,7 +352,7 @@ __global__ void __launch_bounds__(kTopBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kTopBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab, lt, s_lp);
+    load_tab(l2l, PM1, tab, lt, s_lp, 2, Lhi + 1);
     for (int L = 2; L <= Lhi; ++L) {
         for (int j = s_off[L] + w; j < s_off[L + 1]; j += kTopBlock / kWave)
             l2l_wave(j, s_lp[L], s_lp[L + 1], lane, PM1, c_flags, c_child, tab, sb[w], bhat);
@@ -361,7 +370,7 @@ __global__ void __launch_bounds__(kSubBlock)
     __shared__ float2 tab[4 * kMaxPM1 * kMaxPM1], sb[kSubBlock / kWave][kMaxPM1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    load_tab(l2l, PM1, tab, lt, s_lp);
+    load_tab(l2l, PM1, tab, lt, s_lp, max(K, 2), maxL);
     subtree_ranges(K, maxL, hdr, c_key, s_lo, s_hi);
     for (int L = max(K, 2); L <= maxL - 1; ++L) {
         for (int j = s_lo[L] + w; j < s_hi[L]; j += kSubBlock / kWave)
