@@ -562,43 +562,52 @@ __global__ void __launch_bounds__(kSplitBlock)
     }
 }
 
-// Cell attributes: range, counts, flags, index, centre (treeCell.hpp:68) of
-// cell j; returns 1 for a leaf.
-__device__ __forceinline__ uint32_t cell_attrs(
-    int j, const uint32_t *lev_level, const uint32_t *lev_begin, const uint32_t *key,
-    const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr, uint8_t *c_level,
-    uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin, uint32_t *c_end, uint32_t *c_src,
-    double *c_cx, double *c_cy, int *c_child) {
-#pragma clang fp contract(off)
-    const int L = (int)lev_level[j];
-    const uint32_t b = lev_begin[j];
-    uint32_t e, pre = 0;
-    if (L == 0) {
-        e = (uint32_t)n;
-    } else {
-        const int sh = 32 - 2 * L;
-        pre = key[b] >> sh;
-        long long lo = b, hi, step = 1;
-        for (;;) {
-            long long nx = lo + step;
-            if (nx >= n) {
-                hi = n;
-                break;
-            }
-            if ((key[nx] >> sh) != pre) {
-                hi = nx;
-                break;
-            }
-            lo = nx;
-            step <<= 1;
+// End of the particle range that starts at b and shares the key prefix pre
+// (key >> sh): the first index after b whose prefix differs, or n.  The 64
+// lanes of a wavefront search together, one load per lane per round: a
+// 64-way gallop from b (strides 1, 64, 4096, ...) brackets the end, then
+// 64-way refinement narrows the bracket to one index, about 2 log64 of the
+// range in dependent loads.  b, sh and pre are wavefront-uniform and every
+// lane of the wavefront is active.
+__device__ __forceinline__ uint32_t range_end_wave(const uint32_t *key, int n, uint32_t b, int sh,
+                                                   uint32_t pre, int lane) {
+    // invariant: key[lo] has the prefix; the first index without it is <= hi
+    long long lo = b, hi, stride = 1;
+    for (;;) {
+        const long long pos = lo + stride * (lane + 1);
+        bool out = pos >= n;
+        if (!out) out = (key[pos] >> sh) != pre;
+        const uint64_t m = __ballot(out);
+        if (m) {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            hi = min((long long)n, lo + stride * (f + 1));
+            lo += stride * f;
+            break;
         }
-        while (hi - lo > 1) {
-            long long mid = lo + ((hi - lo) >> 1);
-            if ((key[mid] >> sh) == pre) lo = mid;
-            else hi = mid;
-        }
-        e = (uint32_t)hi;
+        lo += stride * kWave;
+        stride *= kWave;
     }
+    while (hi - lo > 1) {
+        const long long st = (hi - lo + kWave - 1) / kWave;  // lane 63 reaches hi
+        const long long pos = lo + st * (lane + 1);
+        bool out = pos >= hi;
+        if (!out) out = (key[pos] >> sh) != pre;
+        const uint64_t m = __ballot(out);
+        const int f = __ffsll((unsigned long long)m) - 1;
+        hi = min(hi, lo + st * (f + 1));
+        lo += st * f;
+    }
+    return (uint32_t)hi;
+}
+
+// Cell attributes: range, counts, flags, index, centre (treeCell.hpp:68) of
+// cell j at level L with particle range [b, e) and level prefix pre.
+__device__ __forceinline__ void cell_attrs(
+    int j, int L, uint32_t b, uint32_t e, uint32_t pre, const uint32_t *A,
+    const uint8_t *leaf_lvl, int n, const DevHdr *hdr, uint8_t *c_level, uint8_t *c_flags,
+    uint32_t *c_key, uint32_t *c_begin, uint32_t *c_end, uint32_t *c_src, double *c_cx,
+    double *c_cy, int *c_child) {
+#pragma clang fp contract(off)
     // every particle active: the scan left A unwritten (the identity)
     const uint32_t src = hdr->n_active == (uint32_t)n ? e - b : A[e] - A[b];
     unsigned f = 0;
@@ -616,16 +625,19 @@ __device__ __forceinline__ uint32_t cell_attrs(
     c_cx[j] = hdr->pivot[0] + ((0.5 + (double)ix) * len);
     c_cy[j] = hdr->pivot[1] + ((0.5 + (double)iy) * len);
     reinterpret_cast<int4 *>(c_child)[j] = make_int4(-1, -1, -1, -1);  // filled by k_cells2
-    return (f & kLeaf) ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256)
+constexpr int kCells1Block = 256;
+
+__global__ void __launch_bounds__(kCells1Block)
     k_cells1(const uint32_t *lev_level, const uint32_t *lev_begin, int nc, const uint32_t *key,
              const uint32_t *A, const uint8_t *leaf_lvl, int n, const DevHdr *hdr,
              uint8_t *c_level, uint8_t *c_flags, uint32_t *c_key, uint32_t *c_begin,
              uint32_t *c_end, uint32_t *c_src, double *c_cx, double *c_cy, int *c_child,
              SplitLevels sl, DevHdr *hdr_w) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per cell (j is wavefront-uniform)
+    const int j = blockIdx.x * (kCells1Block / kWave) + (int)(threadIdx.x / kWave);
+    const int lane = (int)(threadIdx.x & (kWave - 1));
     // level offsets / counts and the leaf-level range for the later kernels
     if (blockIdx.x == 0 && threadIdx.x < kMaxDepth + 2) {
         hdr_w->lev_off[threadIdx.x] = sl.lev_off[threadIdx.x];
@@ -635,9 +647,18 @@ __global__ void __launch_bounds__(256)
             hdr_w->leaf_lvl_max = sl.leaf_lvl_max;
         }
     }
-    if (j < nc)
-        cell_attrs(j, lev_level, lev_begin, key, A, leaf_lvl, n, hdr, c_level, c_flags, c_key,
-                   c_begin, c_end, c_src, c_cx, c_cy, c_child);
+    if (j >= nc) return;
+    const int L = (int)lev_level[j];
+    const uint32_t b = lev_begin[j];
+    uint32_t e = (uint32_t)n, pre = 0;
+    if (L > 0) {
+        const int sh = 32 - 2 * L;
+        pre = key[b] >> sh;
+        e = range_end_wave(key, n, b, sh, pre, lane);
+    }
+    if (lane == 0)
+        cell_attrs(j, L, b, e, pre, A, leaf_lvl, n, hdr, c_level, c_flags, c_key, c_begin, c_end,
+                   c_src, c_cx, c_cy, c_child);
 }
 
 __device__ __forceinline__ int find_cell(const uint32_t *c_key, const uint32_t *lev_off, int level,
@@ -1193,7 +1214,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint32_t *c_end = c.c_end.as<uint32_t>(nc), *c_src = c.c_src.as<uint32_t>(nc);
     double *c_cx = c.c_cx.as<double>(nc), *c_cy = c.c_cy.as<double>(nc);
     int *c_parent = c.c_parent.as<int>(nc), *c_child = c.c_child.as<int>(4 * (size_t)nc);
-    k_cells1<<<div_up(nc, 256), 256, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
+    k_cells1<<<div_up(nc, kCells1Block / kWave), kCells1Block, 0, s>>>(lev_level, lev_begin, nc, key, A, leaf_lvl, n,
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
                                              c_src, c_cx, c_cy, c_child, sl, c.hdr_d);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
