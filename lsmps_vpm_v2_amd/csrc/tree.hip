@@ -695,61 +695,16 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // 6. neighbour table and lists
 // ---------------------------------------------------------------------------
-// Index of the cell with key `pre` in the sorted range [lo, hi), or -1;
-// galloping search from `start` (neighbours lie close in Morton order).
-__device__ __forceinline__ int find_near(const uint32_t *key, int lo, int hi, int start,
-                                         uint32_t pre) {
-    const uint32_t ks = key[start];
-    if (ks == pre) return start;
-    int a, b;
-    if (ks < pre) {
-        int prev = start, step = 1;
-        for (;;) {
-            const int nx = start + step;
-            if (nx >= hi) {
-                a = prev + 1;
-                b = hi;
-                break;
-            }
-            if (key[nx] >= pre) {
-                a = prev + 1;
-                b = nx;
-                break;
-            }
-            prev = nx;
-            step <<= 1;
-        }
-    } else {
-        int prev = start, step = 1;
-        for (;;) {
-            const int nx = start - step;
-            if (nx < lo) {
-                a = lo;
-                b = prev;
-                break;
-            }
-            if (key[nx] < pre) {
-                a = nx + 1;
-                b = prev;
-                break;
-            }
-            prev = nx;
-            step <<= 1;
-        }
-    }
-    while (a < b) {  // lower bound in [a, b]; key[b] >= pre or b == hi
-        const int m = (a + b) >> 1;
-        if (key[m] < pre) a = m + 1;
-        else b = m;
-    }
-    return (a < hi && key[a] == pre) ? a : -1;
-}
-
 // nbr[9j + k], k = (dx+1) + 3 (dy+1) (treeCell.cpp:593-622 order): the
 // deepest existing cell at level <= L containing the level-L cell at
-// (ix+dx, iy+dy), or -1 outside [0, 2^L)^2.  One thread per (cell, k).
+// (ix+dx, iy+dy), or -1 outside [0, 2^L)^2.  One thread per (cell, k): up the
+// parent links to the lowest common ancestor of the cell and that position,
+// then down the child slots toward the position while the child exists (the
+// cells containing a position form one chain from the root, so the walk
+// ends at the deepest of them).
 __global__ void __launch_bounds__(256)
-    k_nbr(const uint8_t *c_level, const uint32_t *c_key, int nc, const DevHdr *hdr, int *nbr) {
+    k_nbr(const uint8_t *c_level, const uint32_t *c_key, const int *c_parent, const int *c_child,
+          int nc, int *nbr) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= 9LL * nc) return;
     const int j = (int)(idx / 9), k = (int)(idx % 9);
@@ -768,13 +723,16 @@ __global__ void __launch_bounds__(256)
     const int X = ix + (k % 3) - 1, Y = iy + (k / 3) - 1;
     int res = -1;
     if (X >= 0 && X < cnt && Y >= 0 && Y < cnt) {
-        const uint32_t p = part1by1((uint32_t)X) | (part1by1((uint32_t)Y) << 1);
-        res = find_near(c_key, (int)hdr->lev_off[L], (int)hdr->lev_off[L + 1], j, p);
-        for (int l = L - 1; res < 0 && l >= 0; --l) {
-            const int sh = L - l;
-            const uint32_t q = part1by1((uint32_t)X >> sh) | (part1by1((uint32_t)Y >> sh) << 1);
-            res = l == 0 ? 0 : find_cell(c_key, hdr->lev_off, l, q);
+        int up = 1;  // (X, Y) != (ix, iy), and both are 0 at level 0
+        while ((ix >> up) != (X >> up) || (iy >> up) != (Y >> up)) ++up;
+        int cur = j;
+        for (int u = 0; u < up && cur >= 0; ++u) cur = c_parent[cur];
+        for (int sh = up - 1; sh >= 0 && cur >= 0; --sh) {
+            const int ch = c_child[4 * cur + ((X >> sh) & 1) + 2 * ((Y >> sh) & 1)];
+            if (ch < 0) break;
+            cur = ch;
         }
+        res = cur;
     }
     nbr[9 * j + k] = res;
 }
@@ -1236,7 +1194,7 @@ void tree_lists_begin(Ctx &c) {
     double *c_cx = (double *)c.c_cx.p, *c_cy = (double *)c.c_cy.p;
     int *c_parent = (int *)c.c_parent.p, *c_child = (int *)c.c_child.p;
     int *nbr = c.c_nbr.as<int>(9 * (size_t)nc);
-    k_nbr<<<div_up(9LL * nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, nbr);
+    k_nbr<<<div_up(9LL * nc, 256), 256, 0, s>>>(c_level, c_key, c_parent, c_child, nc, nbr);
 
     int *l2 = c.l2.as<int>(27 * (size_t)nc), *l4 = c.l4.as<int>(8 * (size_t)nc);
     uint8_t *n2 = c.n2.as<uint8_t>(nc), *n4 = c.n4.as<uint8_t>(nc);
