@@ -227,8 +227,10 @@ __device__ __forceinline__ void l2l_wave(int j, int pp, int pc, int lane, int PM
 // lo .. hi are loaded: the passes over those levels read no others.
 __device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab,
                                          const LevelTab *lt, int *s_lp, int lo, int hi) {
+    if (threadIdx.x < kMaxDepth + 2) s_lp[threadIdx.x] = lt->p[threadIdx.x];
+    __syncthreads();
     int pm = 0;
-    for (int L = lo; L <= hi; ++L) pm = max(pm, lt->p[L]);
+    for (int L = lo; L <= hi; ++L) pm = max(pm, s_lp[L]);
     const int P1 = min(pm, PM1 - 1) + 1;
     const float2 *src = reinterpret_cast<const float2 *>(g);
     for (int u = threadIdx.x; u < 4 * P1 * P1; u += blockDim.x) {
@@ -236,7 +238,6 @@ __device__ __forceinline__ void load_tab(const float *g, int PM1, float2 *tab,
         const int idx = (q * PM1 + m) * PM1 + k;
         tab[idx] = src[idx];
     }
-    if (threadIdx.x < kMaxDepth + 2) s_lp[threadIdx.x] = lt->p[threadIdx.x];
     __syncthreads();
 }
 
@@ -254,25 +255,60 @@ constexpr int kTopBlock = 1024;
 constexpr int kSubRoots = 1024;  // subtree roots wanted at the split level K
 constexpr int kTopCells = 64;    // levels with fewer cells go to the one-workgroup kernel
 
-__device__ int lower_key(const uint32_t *c_key, int lo, int hi, unsigned long long v) {
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((unsigned long long)c_key[mid] < v) lo = mid + 1;
-        else hi = mid;
+// Lower bound of v in c_key[lo, hi), found by a group of kKeyGroup consecutive
+// lanes together: each round the group tests the last key of each of
+// kKeyGroup equal chunks of the range (one load per lane) and keeps the chunk
+// where the bound lies, so about log16 of the range in dependent loads.  All
+// lanes of the wavefront call it; each group may search its own range.
+constexpr int kKeyGroup = 16;
+
+__device__ int lower_key_group(const uint32_t *c_key, int lo, int hi, unsigned long long v) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int gl = lane & (kKeyGroup - 1), gbase = lane - gl;
+    for (;;) {
+        const bool live = lo < hi;  // the bound lies in [lo, hi]
+        if (!__ballot(live)) break;
+        const int st = live ? (hi - lo + kKeyGroup - 1) / kKeyGroup : 0;
+        const int p = lo + st * (gl + 1) - 1;
+        bool ge = true;
+        if (live && p < hi) ge = (unsigned long long)c_key[p] >= v;
+        const uint64_t m = (__ballot(ge) >> gbase) & ((1ull << kKeyGroup) - 1);
+        if (live) {
+            if (m == 0) {  // every tested key is below v, the last one at hi - 1
+                lo = hi;
+            } else {
+                const int f = __ffsll((unsigned long long)m) - 1;
+                hi = min(hi, lo + st * (f + 1) - 1);
+                lo += st * f;
+            }
+        }
     }
     return lo;
 }
 
+// Index range [s_lo[L], s_hi[L]) of the descendants at level L = K .. maxL of
+// the level-K cell of this workgroup: two lower bounds per level, one
+// lane group each.
 __device__ __forceinline__ void subtree_ranges(int K, int maxL, const DevHdr *hdr,
                                                const uint32_t *c_key, int *s_lo, int *s_hi) {
     const int r = (int)hdr->lev_off[K] + blockIdx.x;
     const unsigned long long kr = c_key[r];
-    const int L = K + (int)threadIdx.x;
-    if (L <= maxL) {
-        const int sh = 2 * (L - K);
-        const int lo = (int)hdr->lev_off[L], hi = (int)hdr->lev_off[L + 1];
-        s_lo[L] = lower_key(c_key, lo, hi, kr << sh);
-        s_hi[L] = lower_key(c_key, lo, hi, (kr + 1ull) << sh);
+    const int nsearch = 2 * (maxL - K + 1);
+    for (int s0 = 0; s0 < nsearch; s0 += kSubBlock / kKeyGroup) {
+        const int si = s0 + (int)threadIdx.x / kKeyGroup;
+        int lo = 0, hi = 0, L = K;
+        unsigned long long v = 0;
+        if (si < nsearch) {
+            L = K + si / 2;
+            lo = (int)hdr->lev_off[L];
+            hi = (int)hdr->lev_off[L + 1];
+            v = (kr + (unsigned long long)(si & 1)) << (2 * (L - K));
+        }
+        const int pos = lower_key_group(c_key, lo, hi, v);
+        if (si < nsearch && (threadIdx.x & (kKeyGroup - 1)) == 0) {
+            if (si & 1) s_hi[L] = pos;
+            else s_lo[L] = pos;
+        }
     }
     __syncthreads();
 }
