@@ -93,14 +93,33 @@ def main():
         save("resume_state_00500.npz", **{c: d[c] for c in d.dtype.names})
         save("ref_resume_00500.npz", **fields(f, (10, 12)))
 
-    # config 1 (99,856 particles): reference field at p = 10 on every 97th particle
+    config1()
+
+
+def config1():
+    """Config 1 (99,856 particles), reference field at p = 10 on every 97th
+    particle: under the reference's 75/50 leaf rule (leaves at level 6) and
+    with the leaf rule 100/100, which puts every leaf at tree level 5."""
     f = W.uniform_vortices()
-    ex, ey, _ = O.ref_fmm_field(f["x"], f["y"], f["q"], f["active"], p=10)
-    idx = np.arange(0, f["n"], 97)
-    save("ref_uniform_vortices_100k.npz", idx=idx, ex=ex[idx], ey=ey[idx],
-         ex_norm=np.array(np.linalg.norm(ex)), ey_norm=np.array(np.linalg.norm(ey)),
-         x_sum=np.array(f["x"].sum()), q_sum=np.array(f["q"].sum()))
+    for name, mp, ms in (("ref_uniform_vortices_100k.npz", 75, 50),
+                         ("ref_uniform_vortices_100k_level5.npz", 100, 100)):
+        ex, ey, _ = O.ref_fmm_field(f["x"], f["y"], f["q"], f["active"], p=10, max_par=mp,
+                                    max_src=ms)
+        nc, nl, lo, hi = O.RefTree(f["x"], f["y"], f["active"], p=10, max_par=mp,
+                                   max_src=ms).counts()
+        idx = np.arange(0, f["n"], 97)
+        save(name, idx=idx, ex=ex[idx], ey=ey[idx],
+             ex_norm=np.array(np.linalg.norm(ex)), ey_norm=np.array(np.linalg.norm(ey)),
+             x_sum=np.array(f["x"].sum()), q_sum=np.array(f["q"].sum()),
+             leaf_rule=np.array([mp, ms], np.int32),
+             tree_counts=np.array([nc, nl, lo, hi], np.int64))
 
 
 if __name__ == "__main__":
-    main()
+    # `make_golden.py config1` writes only the config-1 fixtures
+    if sys.argv[1:] == ["config1"]:
+        if not O.ref_available():
+            sys.exit("oracle/_ref/libref_fmm2d.so is missing: run `make -C oracle ref` first")
+        config1()
+    else:
+        main()

@@ -1,16 +1,20 @@
 """The benchmark configurations at their full sizes on the GPU.
 
 1. 99,856-particle uniform field, FP64 path, p 10: against the reference
-   FMM's golden values (strided subset + norm) to 1e-10.
+   FMM's golden values (strided subset + norm) to 1e-10, under the
+   reference's 75/50 leaf rule (leaves at level 6) and at tree level 5 (the
+   leaf rule 100/100, which the reference build puts at level 5 throughout,
+   and the 75/50 rule with the tree capped at level 5: the same leaves).
 2. 779,280-particle flat plate: FP64 and mixed paths against the FP64 direct
-   sum (20k seeded targets, all sources).
+   sum (20k seeded targets, all sources), and the FP64 path with the tree
+   capped at level 6.
 3. The same field with the tree capped at levels 5, 6, 7 and p in {8, 12, 16}
    on the mixed path: error per p against the direct sum.
 4. 2M-particle boundary layer with shed vortices: the learned path.
 5. 8M-particle von Karman street: the learned path, error on a 100k seeded
    target sample (all 8M sources).
-The direct sum is the GPU FP64 one (vpmfmm_direct_device), itself checked
-against the CPU oracle in test_gpu_fmm64.py."""
+The direct sum is the GPU FP64 one (vpmfmm_direct_device), itself pinned to
+the CPU oracle's direct sum in test_gpu_direct.py."""
 import os
 
 import numpy as np
@@ -56,6 +60,51 @@ def test_config1_uniform_100k_matches_reference_golden(gpu):
     print(f"config 1: rel-L2 vs reference golden {err:.2e}")
     assert err <= 1e-10
     assert abs(np.linalg.norm(ex) - float(g["ex_norm"])) <= 1e-10 * float(g["ex_norm"])
+
+
+@pytest.mark.parametrize("rule", [dict(max_par=100, max_src=100), dict(max_level=5)],
+                         ids=["leaf_rule_100", "level_cap_5"])
+def test_config1_uniform_100k_tree_level5_matches_reference_golden(gpu, rule):
+    from lsmps_vpm_v2_amd import FmmSolver, workloads as W
+    import torch
+    f = W.uniform_vortices()
+    g = np.load(os.path.join(GOLD, "ref_uniform_vortices_100k_level5.npz"))
+    assert list(g["tree_counts"][2:]) == [5, 5]  # the reference build's leaf levels
+    dev = _dev(f)
+    q = torch.from_numpy(f["q"]).cuda()
+    s = FmmSolver(p=10, **rule)
+    ex, ey = s.field_device(dev[0], dev[1], q, dev[4])
+    st = s.stats()
+    ex, ey = ex.cpu().numpy(), ey.cpu().numpy()
+    i = g["idx"]
+    err = np.sqrt(np.sum((ex[i] - g["ex"]) ** 2 + (ey[i] - g["ey"]) ** 2) /
+                  np.sum(g["ex"] ** 2 + g["ey"] ** 2))
+    print(f"config 1 at tree level 5 ({rule}): {st['n_cells']} cells, {st['n_leaves']} "
+          f"leaves, leaf levels {st['min_leaf_level']}-{st['max_leaf_level']}, rel-L2 vs "
+          f"reference golden {err:.2e}")
+    assert (st["n_cells"], st["n_leaves"]) == tuple(g["tree_counts"][:2])
+    assert st["min_leaf_level"] == st["max_leaf_level"] == 5
+    assert err <= 1e-10
+    assert abs(np.linalg.norm(ex) - float(g["ex_norm"])) <= 1e-10 * float(g["ex_norm"])
+    assert abs(np.linalg.norm(ey) - float(g["ey_norm"])) <= 1e-10 * float(g["ey_norm"])
+
+
+def test_config2_flat_plate_780k_fp64_tree_level6(gpu):
+    """Config 2 names tree level 6: the FP64 path (reference options, p 10)
+    with the tree capped at level 6 (thousands of particles per leaf, so the
+    near field dominates) against the FP64 direct sum."""
+    from lsmps_vpm_v2_amd import FmmSolver, workloads as W
+    f = W.boundary_layer()
+    dev = _dev(f)
+    tidx, ur, vr = _direct(f, dev, 20000)
+    s = FmmSolver(p=10, max_level=6, timing=True)
+    e = _err(*s.velocity_device(*dev), tidx, ur, vr)
+    st = s.stats()
+    print(f"config 2 FP64 at tree level 6: rel-L2 {e:.2e}, leaf levels "
+          f"{st['min_leaf_level']}-{st['max_leaf_level']}, {st['n_leaves']} leaves, "
+          f"{st['p2p_pairs']:.3e} P2P pairs, {st['ms_total']:.1f} ms")
+    assert st["max_leaf_level"] == 6
+    assert e <= 2e-5
 
 
 def test_config2_flat_plate_780k_fp64_and_mixed(gpu):
