@@ -133,6 +133,8 @@ typedef struct vpmfmm_stats {
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_eval;
     int64_t m2p_terms;   /* sum over (particle, list-3 cell) of (order of the cell + 1) */
     int64_t l2p_terms;   /* sum over particles of the order of their leaf             */
+    int64_t scratch_bytes; /* device scratch the last call took from the array of
+                              vpmfmm_velocity_device_noalloc (0 for the other calls)   */
 } vpmfmm_stats;
 
 typedef struct vpmfmm_handle_s vpmfmm_handle;
@@ -172,6 +174,26 @@ int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *
 int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
                            const double *s, const double *omega, const uint8_t *active,
                            double *u, double *v);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vpmfmm_velocity_device with every scratch buffer (sorted
+ * particles, cells, lists, expansions, tables, judge buffers) carved from one
+ * static device array of VPMFMM_NOALLOC_SCRATCH_BYTES instead of the handle's
+ * grow-only buffers.  The call makes no hipMalloc / hipFree (the handle, its
+ * streams and events were made by vpmfmm_create, which also resolves the
+ * array's address on the handle's device).  Limits: n <= VPMFMM_NOALLOC_MAX_N,
+ * and a tree whose buffers exceed the array returns BAD_ARG (a uniform-like
+ * field at the maximum n takes about 110 MiB; vpmfmm_stats.scratch_bytes
+ * reports the amount).  Each device holds one array for the life of the
+ * process, shared by every handle on it: calls on one device that overlap in
+ * time (two host threads, or two handles on different streams) race on it.
+ * The tree is not kept for the inspection functions after such a call. */
+#define VPMFMM_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define VPMFMM_NOALLOC_MAX_N (1 << 20)
+int vpmfmm_velocity_device_noalloc(vpmfmm_handle *h, int32_t n, const double *x,
+                                   const double *y, const double *s, const double *omega,
+                                   const uint8_t *active, double *u, double *v);
 
 /* Advection schemes of advection::main_advection (main_advection.cpp:6-39). */
 typedef enum vpmfmm_advect_scheme {

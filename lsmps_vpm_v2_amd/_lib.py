@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libvpmfmm.so")
 EXPORTS = (
     "vpmfmm_default_opts", "vpmfmm_create", "vpmfmm_destroy", "vpmfmm_set_opts",
     "vpmfmm_get_opts", "vpmfmm_set_stream", "vpmfmm_field", "vpmfmm_field_device",
-    "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_advect_device",
+    "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_velocity_device_noalloc",
+    "vpmfmm_advect_device",
     "vpmfmm_step_device", "vpmfmm_direct_device",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
@@ -32,6 +33,8 @@ STATUS = {
 FP64, MIXED = 0, 1
 ORDER_FIXED, ORDER_LEARNED = 0, 1
 SEL_FEATURES, SEL_HIDDEN, SEL_PARAMS = 9, 64, 4930
+# limits of vpmfmm_velocity_device_noalloc (include/vpmfmm.h)
+NOALLOC_SCRATCH_BYTES, NOALLOC_MAX_N = 256 << 20, 1 << 20
 
 
 class Opts(C.Structure):
@@ -79,7 +82,7 @@ class Stats(C.Structure):
         ("ms_total", C.c_double),
         ("ms_tree", C.c_double), ("ms_lists", C.c_double), ("ms_upward", C.c_double),
         ("ms_m2l", C.c_double), ("ms_downward", C.c_double), ("ms_eval", C.c_double),
-        ("m2p_terms", C.c_int64), ("l2p_terms", C.c_int64),
+        ("m2p_terms", C.c_int64), ("l2p_terms", C.c_int64), ("scratch_bytes", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -121,7 +124,7 @@ def load() -> C.CDLL:
     L.vpmfmm_set_stream.argtypes = [_vp, _vp]
     for name in ("vpmfmm_field", "vpmfmm_field_device"):
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]
-    for name in ("vpmfmm_velocity", "vpmfmm_velocity_device"):
+    for name in ("vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_velocity_device_noalloc"):
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.vpmfmm_advect_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_double,
                                        C.c_int32]

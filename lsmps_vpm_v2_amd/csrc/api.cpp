@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,7 +25,60 @@ static void free_retired() {
     g_retired.clear();
 }
 
+// Bump allocator over the device scratch array of the no-allocation entry
+// point.  Each buffer asked for during the call takes a fresh 256-byte
+// aligned block (a buffer that grows takes another); nothing is freed until
+// the arena closes, when every buffer gets its own allocation back.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    std::vector<DBuf *> touched;
+};
+static thread_local Arena *g_arena = nullptr;
+
+// The scratch array of the no-allocation entry point.  Host code never uses
+// its name as an address: each device holds its own copy, whose address is
+// resolved once per device (scratch_address) and kept.
+static __device__ __align__(256) unsigned char scratch[VPMFMM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev`, which must be current.
+static void *scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error(kBadArg, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        VPM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Error(kHipError, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 void *DBuf::get(size_t bytes) {
+    if (g_arena) {
+        Arena &a = *g_arena;
+        if (in_arena && bytes <= cap) return p;
+        const size_t nb = (bytes + 255) & ~(size_t)255;
+        if (nb > a.size - a.used)
+            throw Error(kBadArg, "the no-allocation scratch (" + std::to_string(a.size >> 20) +
+                                     " MiB) is too small for this tree: " +
+                                     std::to_string((a.used + nb) >> 20) +
+                                     " MiB asked; use vpmfmm_velocity_device");
+        if (!in_arena) {
+            heap_p = p;
+            heap_cap = cap;
+            in_arena = true;
+            a.touched.push_back(this);
+        }
+        p = a.base + a.used;
+        cap = nb;
+        a.used += nb;
+        return p;
+    }
     if (bytes <= cap && p) return p;
     // headroom, so that sizes that creep up from call to call (a moving
     // particle field) regrow rarely
@@ -44,9 +98,11 @@ void *DBuf::get(size_t bytes) {
 }
 
 void DBuf::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
+    void *own = in_arena ? heap_p : p;  // never an address inside the scratch array
+    if (own) (void)hipFree(own);
+    p = heap_p = nullptr;
+    cap = heap_cap = 0;
+    in_arena = false;
 }
 
 // fmm2D::binComb (reference src/FMM/fmm2D.cpp:28-58): product n..max+1, then
@@ -142,6 +198,7 @@ struct vpmfmm_handle_s {
     bool concurrent_marks = false;   // the last run built the lists beside the upward pass
     MlpWork mlp;
     DBuf mlp_grad;
+    void *scratch = nullptr;  // this device's no-allocation scratch array
     // inspection cache of the last tree (host copies)
     bool insp = false;
     std::vector<uint8_t> h_level, h_flags;
@@ -566,6 +623,64 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Source
     report_selection(h, lrn);
 }
 
+// The no-allocation call: every buffer the call asks for is carved from the
+// device's scratch array.  The handle's caches of device-resident data (the
+// FP64 and mixed tables, the fixed-order slot, the learned orders) refer to
+// its own buffers, so they are re-uploaded or re-selected inside the arena;
+// afterwards the buffers and those caches are put back.  The learned orders
+// are selected afresh by the next call (the cache then describes this call's
+// tree), and the tree is not kept for inspection.
+class ArenaScope {
+public:
+    explicit ArenaScope(vpmfmm_handle *h) : h_(h) {
+        a_.base = static_cast<char *>(h->scratch);
+        a_.size = VPMFMM_NOALLOC_SCRATCH_BYTES;
+        a_.touched.reserve(96);
+        tab_p_ = h->tab_p;
+        tab_binom_ = h->tab_binom;
+        mx_pmax_ = h->mx_pmax;
+        std::memcpy(mx_rptr_, h->mx_rptr_host, sizeof(mx_rptr_));
+        fix_valid_ = h->fix_valid;
+        std::memcpy(fix_orders_, h->fix_orders, sizeof(fix_orders_));
+        h->tab_p = h->tab_binom = -1;
+        h->mx_pmax = -1;
+        h->fix_valid = false;
+        h->sel_cached = false;
+        g_arena = &a_;
+    }
+    ~ArenaScope() {
+        g_arena = nullptr;
+        for (DBuf *b : a_.touched) {
+            b->p = b->heap_p;
+            b->cap = b->heap_cap;
+            b->heap_p = nullptr;
+            b->heap_cap = 0;
+            b->in_arena = false;
+        }
+        h_->tab_p = tab_p_;
+        h_->tab_binom = tab_binom_;
+        h_->mx_pmax = mx_pmax_;
+        std::memcpy(h_->mx_rptr_host, mx_rptr_, sizeof(mx_rptr_));
+        h_->fix_valid = fix_valid_;
+        std::memcpy(h_->fix_orders, fix_orders_, sizeof(fix_orders_));
+        h_->sel_cached = false;
+        h_->insp = false;
+        h_->exp_p = -1;
+        h_->ctx.info.n_cells = 0;  // inspection: no tree in the handle's buffers
+        h_->ctx.key_sorted = h_->ctx.perm_sorted = nullptr;
+        h_->ctx.orders = nullptr;
+    }
+    size_t used() const { return a_.used; }
+
+private:
+    vpmfmm_handle *h_;
+    Arena a_;
+    int tab_p_, tab_binom_, mx_pmax_;
+    const float *mx_rptr_[17];
+    bool fix_valid_;
+    uint8_t fix_orders_[kMaxDepth + 2];
+};
+
 // Host-pointer staging.
 struct Staged {
     const double *x, *y, *q;
@@ -690,6 +805,14 @@ int vpmfmm_create(const vpmfmm_opts *o, vpmfmm_handle **out) {
         VPM_HIP(hipMalloc(&h->ctx.hdr_d, sizeof(DevHdr)));
         VPM_HIP(hipHostMalloc(&h->ctx.hdr_h, sizeof(DevHdr), hipHostMallocDefault));
         for (auto &e : h->ev) VPM_HIP(hipEventCreate(&e));
+        // streams and events of every path, made here so that no call makes any
+        Ctx &c = h->ctx;
+        VPM_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+        VPM_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+        VPM_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+        VPM_HIP(hipEventCreateWithFlags(&c.ev_p2m, hipEventDisableTiming));
+        VPM_HIP(hipEventCreateWithFlags(&c.ev_sync, hipEventDisableTiming));
+        h->scratch = scratch_address(oo.device);
     });
     if (st != VPMFMM_OK) {
         vpmfmm_destroy(h);
@@ -774,6 +897,39 @@ int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const d
         if (n < 1 || !s || !omega) throw Error(kBadArg, "bad size or null pointer");
         VPM_HIP(hipSetDevice(h->o.device));
         run(h, n, x, y, Sources{nullptr, s, omega}, active, u, v, true);
+    });
+}
+
+int vpmfmm_velocity_device_noalloc(vpmfmm_handle *h, int32_t n, const double *x,
+                                   const double *y, const double *s, const double *omega,
+                                   const uint8_t *active, double *u, double *v) {
+    return guarded(h, [&] {
+        if (n < 1 || !s || !omega) throw Error(kBadArg, "bad size or null pointer");
+        if (n > VPMFMM_NOALLOC_MAX_N)
+            throw Error(kBadArg, "n = " + std::to_string(n) +
+                                     " exceeds the no-allocation limit of " +
+                                     std::to_string(VPMFMM_NOALLOC_MAX_N) +
+                                     " particles; use vpmfmm_velocity_device");
+        if (!h->scratch) throw Error(kInternal, "no scratch address on this handle");
+        VPM_HIP(hipSetDevice(h->o.device));
+        size_t used = 0;
+        try {
+            ArenaScope arena(h);
+            try {
+                run(h, n, x, y, Sources{nullptr, s, omega}, active, u, v, true);
+            } catch (...) {
+                // launches of a failed call may still read or write the
+                // scratch: let them finish before the array is used again
+                (void)hipStreamSynchronize(h->ctx.stream);
+                (void)hipStreamSynchronize(h->ctx.side);
+                throw;
+            }
+            used = arena.used();
+        } catch (...) {
+            h->st = vpmfmm_stats{};
+            throw;
+        }
+        h->st.scratch_bytes = (int64_t)used;
     });
 }
 

@@ -79,10 +79,16 @@ inline unsigned long long stat_total(const DevHdr &h, int col) {
 }
 
 
-// Grow-only device buffer.
+// Grow-only device buffer.  While a scratch arena is active on the calling
+// thread (the no-allocation entry point), get() carves from the arena instead
+// of allocating, and the buffer's own allocation (heap_p, heap_cap) is put
+// back when the arena closes.
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool in_arena = false;
+    void *heap_p = nullptr;
+    size_t heap_cap = 0;
     void *get(size_t bytes);
     template <class T>
     T *as(size_t count) {

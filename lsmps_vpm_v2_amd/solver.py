@@ -256,6 +256,17 @@ class FmmSolver:
                                                    _dptr(v)), "vpmfmm_velocity_device")
         return u, v
 
+    def velocity_device_noalloc(self, x, y, s, omega, active, u, v):
+        """velocity_device with every scratch buffer in the library's static
+        device array (vpmfmm_velocity_device_noalloc): no device allocation,
+        n <= NOALLOC_MAX_N; the caller passes the outputs."""
+        n = x.numel()
+        self._enter(x)
+        self._check(self._L.vpmfmm_velocity_device_noalloc(
+            self._h, n, _dptr(x), _dptr(y), _dptr(s), _dptr(omega), _dptr(active), _dptr(u),
+            _dptr(v)), "vpmfmm_velocity_device_noalloc")
+        return u, v
+
     _SCHEMES = {"euler": 0, "rk2": 1}
 
     def advect_device(self, x, y, u, v, dt, scheme="euler"):
