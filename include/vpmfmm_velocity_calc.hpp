@@ -18,6 +18,10 @@
  * parameter so the header compiles against the reference's struct Particle
  * (Utils.hpp:77-159) or any type with the same members.  Define
  * VPMFMM_NO_GLOBAL_VELOCITYCALC to keep the class inside namespace vpmfmm.
+ * The whole reference program must see this one class: include it through
+ * vpmfmm_reference_velocity_calc.hpp, which takes the place of the
+ * reference's src/velocity_calculation/velocity_calc.hpp, the header both of
+ * its callers include (main.cpp:44, penalization_calc.cpp:3).
  */
 #ifndef VPMFMM_VELOCITY_CALC_HPP
 #define VPMFMM_VELOCITY_CALC_HPP
@@ -26,6 +30,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "vpmfmm.h"
@@ -89,13 +94,56 @@ private:
     vpmfmm_handle *h_ = nullptr;
 };
 
+inline bool same_opts(const vpmfmm_opts &a, const vpmfmm_opts &b) {
+    return a.p == b.p && a.max_par == b.max_par && a.max_src == b.max_src &&
+           a.exp_tree == b.exp_tree && a.min_level == b.min_level &&
+           a.max_level == b.max_level && a.precision == b.precision &&
+           a.binom_mode == b.binom_mode && a.list4_mode == b.list4_mode &&
+           a.u_inf == b.u_inf && a.v_inf == b.v_inf && a.device == b.device &&
+           a.timing == b.timing && a.stage_mask == b.stage_mask &&
+           a.order_select == b.order_select && a.reselect_every == b.reselect_every &&
+           a.judge_every == b.judge_every && a.icutoff == b.icutoff;
+}
+
+// Options of every default-constructed VelocityCalc: the reference defaults
+// (P_max 10, 75/50 leaf rule, expTree 1 %, global.cpp:303-313; u_inf 1,
+// v_inf 0, global.cpp:164-165) until set_default_options replaces them.  They
+// are read at an object's first velocity call, so the reference's
+// `VelocityCalc velocity_tool;` (main.cpp:72) and the one that
+// penalization::no_slip_iterative makes per call (penalization_calc.cpp:363)
+// both follow one set_default_options call made before the time loop.
+inline vpmfmm_opts &default_options_slot() {
+    static vpmfmm_opts o = [] {
+        vpmfmm_opts d;
+        vpmfmm_default_opts(&d);
+        return d;
+    }();
+    return o;
+}
+inline void set_default_options(const vpmfmm_opts &o) { default_options_slot() = o; }
+inline vpmfmm_opts default_options() { return default_options_slot(); }
+
+// One solver per host thread and option set, made at its first use and kept
+// for the life of the process (never destroyed: the HIP runtime may already
+// be gone when static destructors run).  Every VelocityCalc of that thread
+// with those options shares it, so an object made per call reuses the
+// device buffers, tables and learned selection of the first.
+inline Solver &shared_solver(const vpmfmm_opts &o) {
+    thread_local std::vector<std::pair<vpmfmm_opts, Solver *>> *cache =
+        new std::vector<std::pair<vpmfmm_opts, Solver *>>();
+    for (auto &e : *cache)
+        if (same_opts(e.first, o)) return *e.second;
+    Solver *s = new Solver(&o);
+    cache->emplace_back(o, s);
+    return *s;
+}
+
 // Drop-in velocity manager (FMM branch).
 class VelocityCalc {
 public:
-    // Reference defaults: P_max 10, 75/50 leaf rule, expTree 1 % (global.cpp:303-313),
-    // u_inf 1, v_inf 0 (global.cpp:164-165).
-    VelocityCalc() : solver_(nullptr) {}
-    explicit VelocityCalc(const vpmfmm_opts &o) : solver_(&o) {}
+    // The options of set_default_options (the reference defaults if never set).
+    VelocityCalc() = default;
+    explicit VelocityCalc(const vpmfmm_opts &o) : opts_(o), explicit_opts_(true) {}
 
     template <class ParticleT, class GridT>
     void get_velocity(ParticleT &particle, const GridT & /*base_grid*/, const int step) {
@@ -116,9 +164,9 @@ public:
         particle.v.clear();
         particle.u.resize(n, 0.0);
         particle.v.resize(n, 0.0);
-        solver_.velocity(n, particle.x.data(), particle.y.data(), particle.s.data(),
-                         particle.vorticity.data(), active_.data(), particle.u.data(),
-                         particle.v.data());
+        solver().velocity(n, particle.x.data(), particle.y.data(), particle.s.data(),
+                          particle.vorticity.data(), active_.data(), particle.u.data(),
+                          particle.v.data());
     }
 
     // The reference's LSMPS Poisson test entry (velocity_calc.hpp:46) is not
@@ -128,10 +176,16 @@ public:
         throw std::logic_error("VelocityCalc::LSMPS_poisson_2d is not provided by vpmfmm");
     }
 
-    Solver &solver() { return solver_; }
+    // The shared solver of this object's options (resolved at first use).
+    Solver &solver() {
+        if (!solver_) solver_ = &shared_solver(explicit_opts_ ? opts_ : default_options());
+        return *solver_;
+    }
 
 private:
-    Solver solver_;
+    vpmfmm_opts opts_{};
+    bool explicit_opts_ = false;
+    Solver *solver_ = nullptr;
     std::vector<uint8_t> active_;
 };
 

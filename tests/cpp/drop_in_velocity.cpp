@@ -66,6 +66,31 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "non-finite input did not raise std::range_error\n");
         return 1;
     }
+    // a VelocityCalc made per call, as penalization::no_slip_iterative does
+    // (penalization_calc.cpp:363, 402), shares the first one's solver (and
+    // its device buffers) and gives the same velocity
+    {
+        VelocityCalc per_call;
+        Particle q = p;
+        per_call.get_velocity(q, 0);
+        if (&per_call.solver() != &velocity_tool.solver()) {
+            std::fprintf(stderr, "a per-call VelocityCalc made a solver of its own\n");
+            return 1;
+        }
+        for (int i = 0; i < n; ++i)
+            if (q.u[i] != p.u[i] || q.v[i] != p.v[i]) {
+                std::fprintf(stderr, "per-call velocity differs at %d\n", i);
+                return 1;
+            }
+        vpmfmm_opts o;
+        vpmfmm_default_opts(&o);
+        o.p = 12;
+        VelocityCalc other(o);
+        if (&other.solver() == &velocity_tool.solver()) {
+            std::fprintf(stderr, "different options share a solver\n");
+            return 1;
+        }
+    }
     FILE *f = std::fopen(argv[2], "wb");
     if (!f) return 1;
     const double nd = n;
