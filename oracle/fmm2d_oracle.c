@@ -618,6 +618,21 @@ void orc_m2l(const double *aks, double dx, double dy, int p, int binom_mode, dou
     free(wp);
 }
 
+/* fmm2D.cpp:463-467 (list 2) and 504-508 (list 4), the order-0 row that only
+ * the potential reads: b_0 += a_0 log(z_t - z_s) - sum_k a_k/k (-1)^k d^-k,
+ * d = z_s - z_t.  log is the principal branch of std::log: log|z| + i arg z,
+ * with z_t - z_s = -d exactly. */
+void orc_m2l0(const double *aks, double dx, double dy, int p, double *bk) {
+    cx_t w = cinv(cx(dx, dy)), pw = w;
+    cx_t s = cmul(cx(aks[0], aks[1]), cx(log(hypot(dx, dy)), atan2(-dy, -dx)));
+    for (int k = 1; k <= p; ++k) {
+        double sg = (k & 1) ? -1.0 : 1.0;
+        s = csub(s, cscale(cmul(cx(aks[2 * k], aks[2 * k + 1]), pw), sg / (double)k));
+        pw = cmul(pw, w);
+    }
+    bk[0] += s.re; bk[1] += s.im;
+}
+
 /* fmm2D.cpp:637-643 with d = z_child - z_parent (binComb(k, p) wraps from P = 18). */
 void orc_l2l(const double *bkp, double dx, double dy, int p, int binom_mode, double *bkc) {
     cx_t d = cx(dx, dy);
@@ -666,8 +681,9 @@ static int cmp_i64(const void *a, const void *b) {
  * per-cell expansions are also returned, cells sorted by ID. */
 static int fmm_run(int n, const double *x, const double *y, const double *q,
                    const double *sz, const uint8_t *active, const orc_opts *o, double *ex,
-                   double *ey, int64_t *ids, double *ak_out, double *bk_out) {
+                   double *ey, int64_t *ids, double *ak_out, double *bk_out, double *phi) {
     if (o && o->icutoff > 0 && !sz) return ORC_BAD_ARG;
+    if (phi && o && o->icutoff > 0) return ORC_BAD_ARG;  /* no regularized potential */
     orc_tree *t = NULL;
     int st = orc_tree_build(&t, n, x, y, active, o);
     if (st) return st;
@@ -729,6 +745,8 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
         for (int i = 0; i < l2.n; ++i) {
             int sidx = h_find(t, l2.v[i]);
             orc_cell *s = &t->cells[sidx];
+            if (phi)
+                orc_m2l0(&ak[(size_t)sidx * W], s->cx - c->cx, s->cy - c->cy, P, &bk[cidx * W]);
             orc_m2l(&ak[(size_t)sidx * W], s->cx - c->cx, s->cy - c->cy, P, o->binom_mode,
                     &bk[cidx * W]);
         }
@@ -736,6 +754,9 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
             int sidx = h_find(t, l4.v[i]);
             orc_cell *s = &t->cells[sidx];
             if (o->list4_mode == 1) {
+                if (phi)
+                    orc_m2l0(&ak[(size_t)sidx * W], s->cx - c->cx, s->cy - c->cy, P,
+                             &bk[cidx * W]);
                 orc_m2l(&ak[(size_t)sidx * W], s->cx - c->cx, s->cy - c->cy, P,
                         o->binom_mode, &bk[cidx * W]);
             } else {
@@ -743,6 +764,11 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
                 for (int j = 0; j < s->pcnt; ++j) {
                     int pj = t->plist[s->poff + j];
                     if (!active[pj]) continue;
+                    if (phi) { /* b_0 += q log(z_t - z_j) (:540) */
+                        double dxx = c->cx - x[pj], dyy = c->cy - y[pj];
+                        bk[cidx * W] += q[pj] * log(hypot(dxx, dyy));
+                        bk[cidx * W + 1] += q[pj] * atan2(dyy, dxx);
+                    }
                     cx_t w = cinv(cx(x[pj] - c->cx, y[pj] - c->cy)), pw = w;
                     for (int m = 1; m <= P; ++m) {
                         bk[cidx * W + 2 * m] -= q[pj] * pw.re / (double)m;
@@ -776,6 +802,8 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
     const int mask = o->stage_mask ? o->stage_mask : 7;
     const int do_p2p = mask & 1, do_m2p = mask & 2, do_l2p = mask & 4;
     for (int i = 0; i < n; ++i) { ex[i] = 0.0; ey[i] = 0.0; }
+    if (phi)
+        for (int i = 0; i < n; ++i) phi[i] = 0.0;
     idvec l1 = {0}, l3 = {0};
     for (int li = 0; li < t->n_leaves; ++li) {
         orc_cell *c = &t->cells[t->leaf_list[li]];
@@ -791,6 +819,10 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
                     if (i == j) continue;
                     double dx = x[i] - x[j], dy = y[i] - y[j];
                     double r2 = dx * dx + dy * dy;
+                    if (phi) { /* potDirSum (fmm2D.cpp:113-115) */
+                        phi[i] += q[j] * log(r2) / 2.0;
+                        continue;
+                    }
                     if (o->icutoff > 0) {
                         /* the type-1 near field, direct_sum
                            (subroutine_all_fmm_2d.cpp:594-627): sij^2 =
@@ -807,7 +839,35 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
                 }
             }
         }
-        for (int ii = 0; ii < c->pcnt; ++ii) {
+        for (int ii = 0; phi && ii < c->pcnt; ++ii) {
+            /* calcPotential per target (fmm2D.cpp:817-880): M2P over list 3,
+               a_0 log(z - zc) - sum a_p (z - zc)^-p / p, then L2P, b_0 +
+               sum b_p (z - z0)^p; the real parts */
+            int i = t->plist[c->poff + ii];
+            for (int a = 0; a < (do_m2p ? l3.n : 0); ++a) {
+                int sidx = h_find(t, l3.v[a]);
+                orc_cell *s = &t->cells[sidx];
+                const double *as = &ak[(size_t)sidx * W];
+                double dx = x[i] - s->cx, dy = y[i] - s->cy;
+                cx_t w = cinv(cx(dx, dy)), pw = w;
+                cx_t f = cmul(cx(as[0], as[1]), cx(log(hypot(dx, dy)), atan2(dy, dx)));
+                for (int k = 1; k <= P; ++k) {
+                    f = csub(f, cscale(cmul(cx(as[2 * k], as[2 * k + 1]), pw), 1.0 / (double)k));
+                    pw = cmul(pw, w);
+                }
+                phi[i] += f.re;
+            }
+            if (do_l2p) {
+                const double *bs = &bk[(size_t)(c - t->cells) * W];
+                cx_t z = cx(x[i] - c->cx, y[i] - c->cy), pw = z, f = cx(bs[0], bs[1]);
+                for (int k = 1; k <= P; ++k) {
+                    f = cadd(f, cmul(cx(bs[2 * k], bs[2 * k + 1]), pw));
+                    pw = cmul(pw, z);
+                }
+                phi[i] += f.re;
+            }
+        }
+        for (int ii = 0; !phi && ii < c->pcnt; ++ii) {
             int i = t->plist[c->poff + ii];
             for (int a = 0; a < (do_m2p ? l3.n : 0); ++a) {       /* M2P over list 3 */
                 int sidx = h_find(t, l3.v[a]);
@@ -835,7 +895,17 @@ static int fmm_run(int n, const double *x, const double *y, const double *q,
 
 int orc_fmm_field(int n, const double *x, const double *y, const double *q,
                   const uint8_t *active, const orc_opts *o, double *ex, double *ey) {
-    return fmm_run(n, x, y, q, NULL, active, o, ex, ey, NULL, NULL, NULL);
+    return fmm_run(n, x, y, q, NULL, active, o, ex, ey, NULL, NULL, NULL, NULL);
+}
+
+int orc_fmm_potential(int n, const double *x, const double *y, const double *q,
+                      const uint8_t *active, const orc_opts *o, double *phi) {
+    double *ex = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *ey = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    if (!ex || !ey) { free(ex); free(ey); return ORC_NOMEM; }
+    int st = fmm_run(n, x, y, q, NULL, active, o, ex, ey, NULL, NULL, NULL, phi);
+    free(ex); free(ey);
+    return st;
 }
 
 int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
@@ -844,7 +914,7 @@ int orc_fmm_expansions(int n, const double *x, const double *y, const double *q,
     double *ex = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
     double *ey = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
     if (!ex || !ey) { free(ex); free(ey); return ORC_NOMEM; }
-    int st = fmm_run(n, x, y, q, NULL, active, o, ex, ey, ids, ak, bk);
+    int st = fmm_run(n, x, y, q, NULL, active, o, ex, ey, ids, ak, bk, NULL);
     free(ex); free(ey);
     return st;
 }
@@ -859,7 +929,7 @@ int orc_velocity(int n, const double *x, const double *y, const double *s,
     double *ey = (double *)malloc(sizeof(double) * n);
     if (!q || !ex || !ey) { free(q); free(ex); free(ey); return ORC_NOMEM; }
     for (int i = 0; i < n; ++i) q[i] = -omega[i] * pow(s[i], 2.0) / (2 * M_PI);
-    int st = fmm_run(n, x, y, q, s, active, o, ex, ey, NULL, NULL, NULL);
+    int st = fmm_run(n, x, y, q, s, active, o, ex, ey, NULL, NULL, NULL, NULL);
     if (st == ORC_OK)
         for (int i = 0; i < n; ++i) { u[i] = ey[i] + u_inf; v[i] = -ex[i] + v_inf; }
     free(q); free(ex); free(ey);
@@ -886,5 +956,24 @@ void orc_direct_field(int n, const double *x, const double *y, const double *q,
             sy += q[j] * dy / r2;
         }
         ex[t] = sx; ey[t] = sy;
+    }
+}
+
+/* potDirSum's kernel summed over every source (fmm2D.cpp:113-115):
+ * phi_i = sum_{j != i} q_j log(r^2) / 2.  OpenMP over targets. */
+void orc_direct_potential(int n, const double *x, const double *y, const double *q,
+                          const uint8_t *src_mask, int nt, const int *tidx, double *phi) {
+    int m = tidx ? nt : n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int t = 0; t < m; ++t) {
+        int i = tidx ? tidx[t] : t;
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            if (src_mask && !src_mask[j]) continue;
+            double dx = x[i] - x[j], dy = y[i] - y[j];
+            s += q[j] * log(dx * dx + dy * dy) / 2.0;
+        }
+        phi[t] = s;
     }
 }

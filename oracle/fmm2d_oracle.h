@@ -114,6 +114,18 @@ int  orc_velocity(int n, const double *x, const double *y, const double *s,
                   const double *omega, const uint8_t *active, const orc_opts *o,
                   double u_inf, double v_inf, double *u, double *v);
 
+/* Reference-semantics FMM potential (fmm2D::calcPotential, fmm2D.cpp:681-896):
+ * phi_i = sum_{j != i} q_j log r_ij, from P2P (potDirSum, :77-122), M2P over
+ * list 3 and L2P with the order-0 local coefficient b_0 that the M2L of
+ * setupFMM forms (:463-467, 504-508; P2L :540-548).  icutoff must be 0. */
+int  orc_fmm_potential(int n, const double *x, const double *y, const double *q,
+                       const uint8_t *active, const orc_opts *o, double *phi);
+
+/* Direct potential sum phi_i = sum_j q_j log(r_ij^2) / 2 (j == i and, with
+ * src_mask, masked sources skipped) at targets tidx or all n particles. */
+void orc_direct_potential(int n, const double *x, const double *y, const double *q,
+                          const uint8_t *src_mask, int nt, const int *tidx, double *phi);
+
 /* Singular FP64 direct sum at targets tidx[0..nt) (indices into the source
  * arrays, so j == i is skipped), or at all n particles when tidx == NULL.
  * src_mask (may be NULL) restricts the sources.  OpenMP over targets. */
@@ -133,6 +145,8 @@ void orc_m2m(const double *ak_child, double dx, double dy, int p, int binom_mode
              double *ak_parent_acc);
 void orc_m2l(const double *ak_src, double dx, double dy, int p, int binom_mode,
              double *bk_acc /* d = z_src - z_tgt */);
+/* the order-0 local coefficient of the potential (b_0 += ..., d = z_src - z_tgt) */
+void orc_m2l0(const double *ak_src, double dx, double dy, int p, double *bk_acc);
 void orc_l2l(const double *bk_parent, double dx, double dy, int p, int binom_mode,
              double *bk_child_acc);
 void orc_m2p(const double *ak, double dx, double dy, int p, double *ex, double *ey);

@@ -90,6 +90,11 @@ def lib():
                                          _lp, _dp, _dp]
         L.orc_direct_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.c_int, _ip, _dp, _dp]
         L.orc_direct_field.restype = None
+        L.orc_fmm_potential.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.POINTER(OrcOpts), _dp]
+        L.orc_direct_potential.argtypes = [C.c_int, _dp, _dp, _dp, _up, C.c_int, _ip, _dp]
+        L.orc_direct_potential.restype = None
+        L.orc_m2l0.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_m2l0.restype = None
         L.orc_binom_ref.argtypes = [C.c_int, C.c_int]
         L.orc_binom_exact.argtypes = [C.c_int, C.c_int]
         L.orc_binom_exact.restype = C.c_double
@@ -132,6 +137,7 @@ def ref():
         L.ref_int_list.argtypes = [C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
         L.ref_ext_list.argtypes = [C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
         L.ref_fmm_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, _dp, _dp, _dp]
+        L.ref_fmm_potential.argtypes = [C.c_int, _dp, _dp, _dp, _up, _dp]
         _ref = L
     return _ref
 
@@ -337,6 +343,34 @@ def inviscid_loop(x, y, s, omega, active, dt, steps, o: OrcOpts | None = None, u
     return x, y, u, v
 
 
+def fmm_potential(x, y, q, active, o: OrcOpts | None = None):
+    """fmm2D::calcPotential + get_Potential (fmm2D.cpp:681-896): phi per particle."""
+    L = lib()
+    o = o or opts()
+    x, y, q, a = _d(x), _d(y), _d(q), _u8(active)
+    phi = np.zeros(len(x))
+    _check(L.orc_fmm_potential(len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
+                               C.byref(o), _ptr(phi, _dp)), "orc_fmm_potential")
+    return phi
+
+
+def direct_potential(x, y, q, targets=None, src_mask=None):
+    """phi_i = sum_{j != i} q_j log(r_ij^2) / 2 at `targets` (indices) or all."""
+    L = lib()
+    x, y, q = _d(x), _d(y), _d(q)
+    n = len(x)
+    if targets is None:
+        nt, tp = n, None
+    else:
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        nt, tp = len(t), _ptr(t, _ip)
+    phi = np.zeros(nt)
+    m = None if src_mask is None else _ptr(_u8(src_mask), _up)
+    L.orc_direct_potential(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), m, nt, tp,
+                           _ptr(phi, _dp))
+    return phi
+
+
 def direct_field(x, y, q, targets=None, src_mask=None):
     """Singular FP64 direct sum (j == i skipped) at `targets` (indices) or all."""
     L = lib()
@@ -409,6 +443,17 @@ def ref_fmm_field(x, y, q, active, p=10, max_par=75, max_src=50, exp_tree=1.0,
     _check(R.ref_fmm_field(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
                            _ptr(ex, _dp), _ptr(ey, _dp), _ptr(sec, _dp)), "ref_fmm_field")
     return ex, ey, (float(sec[0]), float(sec[1]))
+
+
+def ref_fmm_potential(x, y, q, active, p=10, max_par=75, max_src=50, exp_tree=1.0):
+    """Potential from the reference's own fmm2D::calcPotential."""
+    R = ref()
+    R.ref_set_params(p, max_par, max_src, exp_tree)
+    x, y, q, a = _d(x), _d(y), _d(q), _u8(active)
+    phi = np.zeros(len(x))
+    _check(R.ref_fmm_potential(len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(q, _dp), _ptr(a, _up),
+                               _ptr(phi, _dp)), "ref_fmm_potential")
+    return phi
 
 
 def ref_binom(n, k) -> int:

@@ -227,4 +227,26 @@ int ref_fmm_field(int n, const double *x, const double *y, const double *q,
     return 0;
 }
 
+// The same tree + fmm2D::calcPotential and get_Potential
+// (fmm2D.cpp:681-896, 1137-1140), the potential variant of the path.
+int ref_fmm_potential(int n, const double *x, const double *y, const double *q,
+                      const uint8_t *active, double *phi) {
+    std::vector<std::vector<double>> pos;
+    std::vector<bool> mark;
+    pack(n, x, y, active, pos, mark);
+    std::vector<double> src(q, q + n);
+    try {
+        MuteStdout mute;
+        TreeCell tree;
+        tree.initializeTree(pos, mark);
+        fmm2D fmm;
+        fmm.calcPotential(tree, pos, mark, src);
+        std::vector<double> f = fmm.get_Potential();
+        for (int i = 0; i < n; ++i) phi[i] = f[i];
+    } catch (const std::exception &) {
+        return 6;
+    }
+    return 0;
+}
+
 }  // extern "C"

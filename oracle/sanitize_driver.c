@@ -116,6 +116,12 @@ static int run_field(int n, int clusters, double inactive) {
             CHECK(finite_all(ex, n) && finite_all(ey, n), "finite velocity");
         }
         o.icutoff = 0;
+        for (int l4m = 1; l4m <= 2; ++l4m) { /* the potential (calcPotential) */
+            o.list4_mode = l4m;
+            CHECK(orc_fmm_potential(n, x, y, q, a, &o, ex) == ORC_OK, "orc_fmm_potential");
+            CHECK(finite_all(ex, n), "finite potential");
+        }
+        o.list4_mode = 1;
         /* the expansions of every cell (count from a tree of the same options) */
         orc_tree *t2 = NULL;
         CHECK(orc_tree_build(&t2, n, x, y, a, &o) == ORC_OK, "orc_tree_build (expansions)");
@@ -132,10 +138,13 @@ static int run_field(int n, int clusters, double inactive) {
     CHECK(finite_all(ex, n), "direct (all)");
     orc_direct_field(n, x, y, q, a, n / 3, tidx, ex, ey);
     CHECK(finite_all(ex, n / 3), "direct (targets, mask)");
+    orc_direct_potential(n, x, y, q, a, n / 3, tidx, ex);
+    CHECK(finite_all(ex, n / 3), "direct potential");
     /* stage entry points */
     double ak[2 * 17] = {0}, bk[2 * 17] = {0}, fx = 0, fy = 0;
     orc_p2m(n < 50 ? n : 50, x, y, q, 0.1, -0.2, 16, ak);
     orc_m2m(ak, 0.05, -0.03, 16, 1, bk);
+    orc_m2l0(ak, 3.0, 1.0, 16, bk);
     orc_m2l(ak, 3.0, 1.0, 16, 1, bk);
     orc_l2l(bk, 0.02, 0.01, 16, 0, ak);
     orc_m2p(ak, 2.0, 1.5, 16, &fx, &fy);

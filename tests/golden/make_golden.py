@@ -94,6 +94,19 @@ def main():
         save("ref_resume_00500.npz", **fields(f, (10, 12)))
 
     config1()
+    potential()
+
+
+def potential():
+    """fmm2D::calcPotential of the reference build on the 1k uniform and 5k
+    cluster inputs of ref_uniform_1k.npz / ref_clusters_5k.npz, p in 8, 10, 12
+    (75/50 leaf rule, the reference's int32 binomials)."""
+    for name in ("uniform_1k", "clusters_5k"):
+        g = np.load(os.path.join(HERE, f"ref_{name}.npz"))
+        out = {}
+        for p in (8, 10, 12):
+            out[f"phi_p{p}"] = O.ref_fmm_potential(g["x"], g["y"], g["q"], g["active"], p=p)
+        save(f"ref_potential_{name}.npz", **out)
 
 
 def config1():
@@ -116,10 +129,10 @@ def config1():
 
 
 if __name__ == "__main__":
-    # `make_golden.py config1` writes only the config-1 fixtures
-    if sys.argv[1:] == ["config1"]:
+    # `make_golden.py config1` / `potential` writes only those fixtures
+    if sys.argv[1:] in (["config1"], ["potential"]):
         if not O.ref_available():
             sys.exit("oracle/_ref/libref_fmm2d.so is missing: run `make -C oracle ref` first")
-        config1()
+        config1() if sys.argv[1] == "config1" else potential()
     else:
         main()
