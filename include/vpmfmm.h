@@ -165,6 +165,19 @@ int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
 int vpmfmm_field_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
                         const double *q, const uint8_t *active, double *ex, double *ey);
 
+/* Potential of the same kernel, host pointers:
+ *   phi_i = sum_{j != i} q_j log r_ij
+ * with the reference's FMM semantics: fmm2D::calcPotential + get_Potential
+ * (fmm2D.cpp:681-896, 1137-1140; caller pressure_poisson.cpp:216-219).  FP64
+ * whatever opts.precision (the order, leaf rule, binomials and list-4 mode of
+ * the options apply); singular kernel only (icutoff 0).  Two particles of a
+ * list-1 pair at one position give DUPLICATE_POINTS (log 0). */
+int vpmfmm_potential(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                     const double *q, const uint8_t *active, double *phi);
+/* Same with device pointers. */
+int vpmfmm_potential_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                            const double *q, const uint8_t *active, double *phi);
+
 /* Velocity of VelocityCalc::velocity_fmm_2d + get_velocity's free stream
  * (velocity_calc.cpp:465-481, 572-575, 226-230):
  *   q = -omega s^2 / (2 pi),  u = Ey + u_inf,  v = -Ex + v_inf. */
@@ -263,9 +276,10 @@ int vpmfmm_tree_lists(vpmfmm_handle *h, int64_t id, int64_t *l1, int32_t *n1, in
 int vpmfmm_set_level_orders(vpmfmm_handle *h, const int32_t *p, int32_t nlev);
 
 /* Multipole (ak) and local (bk) coefficients of every cell after the last
- * FP64 field/velocity call, cells sorted by ID: ids[n_cells],
- * ak/bk[n_cells * 2(p+1)] as (re, im) pairs.  b_0 is not formed (it never
- * reaches the field).  Stage-level parity hook (P2M, M2M, M2L + L2L). */
+ * FP64 field/velocity/potential call, cells sorted by ID: ids[n_cells],
+ * ak/bk[n_cells * 2(p+1)] as (re, im) pairs.  b_0 is formed by the potential
+ * calls only (it never reaches the field).  Stage-level parity hook (P2M,
+ * M2M, M2L + L2L). */
 int vpmfmm_tree_expansions(vpmfmm_handle *h, int64_t *ids, double *ak, double *bk);
 
 /* ---- learned order / tree-level selection (no counterpart in the reference,

@@ -16,7 +16,8 @@ EXPORTS = (
     "vpmfmm_get_opts", "vpmfmm_set_stream", "vpmfmm_field", "vpmfmm_field_device",
     "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_velocity_device_noalloc",
     "vpmfmm_advect_device",
-    "vpmfmm_step_device", "vpmfmm_direct_device",
+    "vpmfmm_step_device", "vpmfmm_direct_device", "vpmfmm_potential",
+    "vpmfmm_potential_device",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
     "vpmfmm_tree_expansions", "vpmfmm_set_level_orders", "vpmfmm_set_selector",
@@ -126,6 +127,8 @@ def load() -> C.CDLL:
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in ("vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_velocity_device_noalloc"):
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    for name in ("vpmfmm_potential", "vpmfmm_potential_device"):
+        getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]
     L.vpmfmm_advect_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_double,
                                        C.c_int32]
     L.vpmfmm_step_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

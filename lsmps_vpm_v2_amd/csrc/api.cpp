@@ -681,6 +681,31 @@ private:
     uint8_t fix_orders_[kMaxDepth + 2];
 };
 
+// The potential (fmm2D::calcPotential, fmm2D.cpp:681-896): the FP64 FMM with
+// the handle's order, leaf rule, binomials and list-4 mode whatever its
+// precision (the reference computes it in FP64 only), singular kernel only.
+void run_potential(vpmfmm_handle *h, int n, const double *x, const double *y, const double *q,
+                   const uint8_t *a, double *phi) {
+    if (!q || !phi) throw Error(kBadArg, "null pointer");
+    if (h->o.icutoff != 0)
+        throw Error(kBadArg, "the potential has the singular kernel only (icutoff 0)");
+    struct Restore {
+        vpmfmm_handle *h;
+        vpmfmm_opts o;
+        ~Restore() { h->o = o; }
+    } restore{h, h->o};
+    h->o.precision = VPMFMM_FP64;
+    h->ctx.icut = 0;
+    build(h, n, x, y, Sources{q}, a, true);
+    ensure_tables(h);
+    const int mask = h->o.stage_mask ? h->o.stage_mask : 7;
+    fmm64_run(h->ctx, h->o.p, h->o.list4_mode, mask, phi, nullptr, false, 0.0, 0.0,
+              FmmMarks{h->o.timing ? h->ev : nullptr}, true);
+    h->exp_p = h->o.p;
+    finish_stats(h, true);
+    report_selection(h, false);
+}
+
 // Host-pointer staging.
 struct Staged {
     const double *x, *y, *q;
@@ -887,6 +912,24 @@ int vpmfmm_field(vpmfmm_handle *h, int32_t n, const double *x, const double *y, 
         double *oa = c.ex_out.as<double>(n), *ob = c.ey_out.as<double>(n);
         run(h, n, s.x, s.y, Sources{s.q}, s.a, oa, ob, false);
         stage_out(h, n, ex, ey);
+    });
+}
+
+int vpmfmm_potential_device(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                            const double *q, const uint8_t *active, double *phi) {
+    return guarded(h, [&] { run_potential(h, n, x, y, q, active, phi); });
+}
+
+int vpmfmm_potential(vpmfmm_handle *h, int32_t n, const double *x, const double *y,
+                     const double *q, const uint8_t *active, double *phi) {
+    return guarded(h, [&] {
+        if (!phi) throw Error(kBadArg, "null pointer");
+        Staged st = stage_in(h, n, x, y, q, active);
+        Ctx &c = h->ctx;
+        double *dphi = c.ex_out.as<double>(n);
+        run_potential(h, n, st.x, st.y, st.q, st.a, dphi);
+        VPM_HIP(hipMemcpyAsync(phi, dphi, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
     });
 }
 

@@ -222,9 +222,12 @@ struct FmmMarks {
 
 // FP64 FMM (fmm64.hip): expansions and evaluation.  Output either the field
 // (ex, ey) or the velocity (u = ey + u_inf, v = -ex + v_inf), scattered to the
-// caller's particle order.
+// caller's particle order; with `potential`, the potential phi = sum q log r
+// into out_a (fmm2D::calcPotential: the local coefficient b_0 is formed and
+// carried down, out_b and velocity are unused).
 void fmm64_run(Ctx &c, int p, int list4_mode, int stage_mask, double *out_a, double *out_b,
-               bool velocity, double u_inf, double v_inf, const FmmMarks &mk);
+               bool velocity, double u_inf, double v_inf, const FmmMarks &mk,
+               bool potential = false);
 
 // Particle advection sub-step (advect.hip): Euler x += u dt, or the
 // reference's velocity-scaling "rk2" update.

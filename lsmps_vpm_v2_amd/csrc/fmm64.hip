@@ -51,31 +51,66 @@ __device__ __forceinline__ cd ld(const double2 *a, size_t i) {
 }
 __device__ __forceinline__ void st(double2 *a, size_t i, cd v) { a[i] = make_double2(v.x, v.y); }
 
-// P2M: one wavefront per leaf, lane p forms a_p = sum q (z - zc)^p over the
-// leaf's active particles (fmm2D.cpp:237-284, with the sign and 1/p folded
-// into the translation tables).
-constexpr int kUpWaves = 4;
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+    return __shfl_xor(v, m, kWave);
+}
 
-__global__ void __launch_bounds__(kWave *kUpWaves)
+// Lanes of one wavefront read what other lanes of it wrote to LDS.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// P2M (fmm2D.cpp:237-284, with the sign and 1/p folded into the translation
+// tables): four leaves per wavefront, 16 lanes each.  A leaf's lanes stage 16
+// of its particles at a time in LDS as (z - zc, q), zero strength for
+// inactive particles and leaves; lane r then adds q (z - zc)^p for its own
+// order p = k0 + r over the staged particles, in particle order (the
+// reference's loop order), with no cross-lane reduction.  Orders beyond 15
+// take further chunks of 16.
+constexpr int kP2m64Waves = 4;
+constexpr int kP2mLanes = 16;                    // lanes per leaf
+constexpr int kP2mLeaves = kWave / kP2mLanes;    // leaves per wavefront
+
+__global__ void __launch_bounds__(kWave *kP2m64Waves)
     k_p2m64(const uint32_t *leaf_idx, int nl, int P, const uint8_t *c_flags,
             const uint32_t *c_begin, const uint32_t *c_end, const double *c_cx,
             const double *c_cy, const double *xs, const double *ys, const double *qs,
             const uint8_t *as, double2 *ak) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int t = blockIdx.x * kUpWaves + threadIdx.x / kWave;
-    if (t >= nl) return;
+    __shared__ double4 stg[kP2m64Waves][kWave];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int g = lane / kP2mLanes, r = lane % kP2mLanes;
+    const long long t = ((long long)blockIdx.x * kP2m64Waves + w) * kP2mLeaves + g;
+    const bool has = t < nl;
     const int P1 = P + 1;
-    const int j = (int)leaf_idx[t];
-    if (lane >= P1) return;
-    cd acc = {0.0, 0.0};
-    if (c_flags[j] & kActive) {
-        const double cx = c_cx[j], cy = c_cy[j];
-        for (uint32_t i = c_begin[j]; i < c_end[j]; ++i) {
-            if (!as[i]) continue;
-            acc = cadd(acc, cscale(cpowi(cd{xs[i] - cx, ys[i] - cy}, lane), qs[i]));
+    const int j = has ? (int)leaf_idx[t] : 0;
+    const bool act = has && (c_flags[j] & kActive) != 0;
+    const uint32_t b = has ? c_begin[j] : 0u, cnt = has ? c_end[j] - b : 0u;
+    const double cx = has ? c_cx[j] : 0.0, cy = has ? c_cy[j] : 0.0;
+    // rounds of 16 particles: the largest of the wavefront's leaves (uniform)
+    uint32_t mx = cnt;
+    for (int m = kP2mLanes; m < kWave; m <<= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, m, kWave));
+    mx = (uint32_t)__builtin_amdgcn_readfirstlane((int)mx);
+    for (int k0 = 0; k0 < P1; k0 += kP2mLanes) {
+        const int p = k0 + r;
+        cd acc = {0.0, 0.0};
+        for (uint32_t base = 0; base < mx; base += kP2mLanes) {
+            double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+            const uint32_t i = b + base + (uint32_t)r;
+            if (act && base + (uint32_t)r < cnt && as[i])
+                v = make_double4(xs[i] - cx, ys[i] - cy, qs[i], 0.0);
+            stg[w][lane] = v;
+            wave_sync();
+            if (p < P1) {
+                for (int u = 0; u < kP2mLanes; ++u) {
+                    const double4 sv = stg[w][g * kP2mLanes + u];
+                    if (sv.z != 0.0) acc = cadd(acc, cscale(cpowi(cd{sv.x, sv.y}, p), sv.z));
+                }
+            }
+            wave_sync();
         }
+        if (has && p < P1) st(ak, (size_t)j * P1 + p, acc);
     }
-    st(ak, (size_t)j * P1 + lane, acc);
 }
 
 // M2M for the non-leaf cells of one level: thread per (parent, p),
@@ -104,17 +139,27 @@ __global__ void __launch_bounds__(256)
     st(ak, (size_t)j * P1 + p, acc);
 }
 
-// M2L for every cell of level >= 2 from lists 2 and 4.  tab[p][k] holds the
-// coefficient of a_k w^(p+k), w = 1/(zs - zt).
+// Principal complex logarithm, as std::log: log|z| + i arg z.
+__device__ __forceinline__ cd clog(cd z) { return {log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
+
+// M2L for every cell of level >= 2 from lists 2 and 4 (fmm2D.cpp:409-556):
+// thread per (cell, order).  tab[p][k] holds the coefficient of a_k w^(p+k),
+// w = 1/(zs - zt), so b_p = w^p sum_k tab[p][k] a_k w^k (Horner in w).  With
+// POT the order-0 row of the potential is formed too (fmm2D.cpp:463-467,
+// 504-508): b_0 = a_0 log(z_t - z_s) - sum_{k>=1} a_k (-1)^k / k w^k, and by
+// P2L b_0 += q log(z_t - z_i) (:540).  With PT > 0 (orders P <= PT) a
+// source's coefficients are loaded into registers at once before the Horner
+// sweep, rather than one dependent load per step.
+template <bool POT, int PT>
 __global__ void __launch_bounds__(256)
     k_m2l64(int j0, int nc, int P, int list4_mode, const double *tab, const int *l2,
             const uint8_t *n2, const int *l4, const uint8_t *n4, const double *c_cx,
             const double *c_cy, const uint32_t *c_begin, const uint32_t *c_end, const double *xs,
             const double *ys, const double *qs, const uint8_t *as, const double2 *ak, double2 *bk) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int P1 = P + 1;
-    if (idx >= (long long)(nc - j0) * P) return;
-    const int j = j0 + (int)(idx / P), p = 1 + (int)(idx % P);
+    const int P1 = P + 1, NP = POT ? P1 : P;  // orders formed: 0..P (potential) or 1..P
+    if (idx >= (long long)(nc - j0) * NP) return;
+    const int j = j0 + (int)(idx / NP), p = (POT ? 0 : 1) + (int)(idx % NP);
     const double cx = c_cx[j], cy = c_cy[j];
     const double *row = tab + (size_t)p * P1;
     cd acc = {0.0, 0.0};
@@ -123,17 +168,39 @@ __global__ void __launch_bounds__(256)
         const int sidx = u < m2 ? l2[27 * j + u] : l4[8 * j + (u - m2)];
         const cd w = cinv(cd{c_cx[sidx] - cx, c_cy[sidx] - cy});
         const size_t b = (size_t)sidx * P1;
-        cd s = cscale(ld(ak, b + P), row[P]);
-        for (int k = P - 1; k >= 0; --k) s = cadd(cmul(s, w), cscale(ld(ak, b + k), row[k]));
+        if (POT && p == 0) {
+            cd sw = {0.0, 0.0};
+            for (int k = P; k >= 1; --k)
+                sw = cmul(cadd(sw, cscale(ld(ak, b + k), ((k & 1) ? 1.0 : -1.0) / k)), w);
+            acc = cadd(acc, cadd(sw, cmul(ld(ak, b), clog(cd{cx - c_cx[sidx], cy - c_cy[sidx]}))));
+            continue;
+        }
+        cd s;
+        if constexpr (PT > 0) {
+            cd a[PT + 1];
+#pragma unroll
+            for (int k = 0; k <= PT; ++k) a[k] = k <= P ? ld(ak, b + k) : cd{0.0, 0.0};
+            s = cscale(a[PT], PT <= P ? row[PT] : 0.0);
+#pragma unroll
+            for (int k = PT - 1; k >= 0; --k)
+                if (k <= P) s = cadd(cmul(s, w), cscale(a[k], row[k]));
+        } else {
+            s = cscale(ld(ak, b + P), row[P]);
+            for (int k = P - 1; k >= 0; --k) s = cadd(cmul(s, w), cscale(ld(ak, b + k), row[k]));
+        }
         acc = cadd(acc, cmul(s, cpowi(w, p)));
     }
     if (list4_mode == 2) {
         // P2L from the active particles of each list-4 leaf (fmm2D.cpp:526-553)
-        const double inv_p = 1.0 / (double)p;
+        const double inv_p = p > 0 ? 1.0 / (double)p : 0.0;
         for (int u = 0; u < m4; ++u) {
             const int sidx = l4[8 * j + u];
             for (uint32_t i = c_begin[sidx]; i < c_end[sidx]; ++i) {
                 if (!as[i]) continue;
+                if (POT && p == 0) {
+                    acc = cadd(acc, cscale(clog(cd{cx - xs[i], cy - ys[i]}), qs[i]));
+                    continue;
+                }
                 const cd w = cinv(cd{xs[i] - cx, ys[i] - cy});
                 acc = cadd(acc, cscale(cpowi(w, p), -qs[i] * inv_p));
             }
@@ -143,14 +210,16 @@ __global__ void __launch_bounds__(256)
 }
 
 // L2L for the cells of one level (>= 3): each child adds its parent's shifted
-// local expansion.  tab[p][k] = C(k, p).
+// local expansion.  tab[p][k] = C(k, p).  Orders 1..P, and 0 for the
+// potential (fmm2D.cpp:637-643 runs p from 0).
+template <bool POT>
 __global__ void __launch_bounds__(256)
     k_l2l64(int j0, int j1, int P, const double *tab, const int *c_parent, const double *c_cx,
             const double *c_cy, double2 *bk) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int P1 = P + 1;
-    if (idx >= (long long)(j1 - j0) * P) return;
-    const int j = j0 + (int)(idx / P), p = 1 + (int)(idx % P);
+    const int P1 = P + 1, NP = POT ? P1 : P;
+    if (idx >= (long long)(j1 - j0) * NP) return;
+    const int j = j0 + (int)(idx / NP), p = (POT ? 0 : 1) + (int)(idx % NP);
     const int par = c_parent[j];
     const cd d = {c_cx[j] - c_cx[par], c_cy[j] - c_cy[par]};
     const size_t b = (size_t)par * P1;
@@ -168,9 +237,7 @@ __global__ void __launch_bounds__(256)
 constexpr int kEvalWaves = 4;
 constexpr int kStage64 = 256;
 
-__device__ __forceinline__ double shfl_xor_d(double v, int m) {
-    return __shfl_xor(v, m, kWave);
-}
+
 
 // 1/x to full double precision: hardware estimate and two Newton steps.
 __device__ __forceinline__ double rcp64(double x) {
@@ -304,6 +371,91 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
     if (dup) atomicOr(flags, kFlagDuplicate);
 }
 
+// Potential evaluation (fmm2D::calcPotential, fmm2D.cpp:681-896), one
+// wavefront per leaf as k_eval64: P2P phi += q log(r^2) / 2 over every
+// particle of the list-1 cells, j != i (potDirSum, :77-122); M2P over list 3,
+// Re[a_0 log(z - zc) - sum_p a_p (z - zc)^-p / p] (:817-834); L2P
+// Re[b_0 + sum_p b_p (z - z0)^p] (:854-866).  A pair at r = 0 raises
+// DUPLICATE_POINTS (log 0).
+__global__ void __launch_bounds__(kWave *kEvalWaves)
+    k_pot64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
+            const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
+            const uint32_t *off3, const int *l3, const double *xs, const double *ys,
+            const double *qs, const uint32_t *perm, const double2 *ak, const double2 *bk,
+            int mask, double *phi, unsigned *flags) {
+    __shared__ double4 stage[kEvalWaves][kStage64];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kEvalWaves + w;
+    if (t >= nl) return;
+    const int j = (int)leaf_idx[t];
+    const uint32_t b = c_begin[j], e = c_end[j];
+    const int nt = (int)(e - b);
+    const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
+    const int LG = kWave / G;
+    const int g = lane / LG, tl = lane % LG;
+    const int P1 = P + 1;
+    const double lcx = c_cx[j], lcy = c_cy[j];
+    const uint32_t a1 = off1[t], z1 = off1[t + 1], a3 = off3[t], z3 = off3[t + 1];
+    bool dup = false;
+    for (uint32_t tb = b; tb < e; tb += LG) {
+        const uint32_t i = tb + tl;
+        const bool valid = i < e;
+        const double xi = valid ? xs[i] : 0.0, yi = valid ? ys[i] : 0.0;
+        double ph = 0.0;
+        uint32_t u = a1, pos = a1 < z1 ? c_begin[l1[a1]] : 0u;
+        while ((mask & 1) && u < z1) {
+            int fill = 0;
+            while (u < z1 && fill < kStage64) {
+                const uint32_t se = c_end[l1[u]];
+                const int take = (int)min<uint32_t>(se - pos, (uint32_t)(kStage64 - fill));
+                for (int r = lane; r < take; r += kWave) {
+                    const uint32_t src = pos + r;
+                    stage[w][fill + r] = make_double4(xs[src], ys[src], qs[src], (double)src);
+                }
+                fill += take;
+                pos += take;
+                if (pos >= se && ++u < z1) pos = c_begin[l1[u]];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int r = g; r < fill; r += G) {
+                const double4 sv = stage[w][r];
+                const double dx = xi - sv.x, dy = yi - sv.y;
+                const double r2 = dx * dx + dy * dy;
+                if (valid && (uint32_t)sv.w != i) {
+                    dup |= r2 == 0.0;
+                    ph += sv.z * log(r2) / 2.0;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
+            const int sc = l3[v];
+            const cd zr = {xi - c_cx[sc], yi - c_cy[sc]};
+            const cd wv = cinv(zr);
+            const size_t sbk = (size_t)sc * P1;
+            // sum_{p>=1} a_p w^p / p by Horner, then a_0 log(z - zc) minus it
+            cd f = {0.0, 0.0};
+            for (int p = P; p >= 1; --p) f = cmul(cadd(f, cscale(ld(ak, sbk + p), 1.0 / p)), wv);
+            const cd a0l = cmul(ld(ak, sbk), clog(zr));
+            if (valid) ph += a0l.x - f.x;
+        }
+        for (int m = LG; m < kWave; m <<= 1) ph += shfl_xor_d(ph, m);
+        if (valid && g == 0) {
+            if (mask & 4) {
+                const cd z = {xi - lcx, yi - lcy};
+                const size_t lb = (size_t)j * P1;
+                cd f = ld(bk, lb + P);
+                for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, z), ld(bk, lb + p));
+                ph += f.x;
+            }
+            phi[perm[i]] = ph;
+        }
+    }
+    if (dup) atomicOr(flags, kFlagDuplicate);
+}
+
 // FP64 direct sum with LDS source tiles (accuracy reference only).
 __global__ void __launch_bounds__(256)
     k_direct64(int n, const double *x, const double *y, const double *q, const uint8_t *mask,
@@ -343,7 +495,7 @@ __global__ void __launch_bounds__(256)
 }  // namespace
 
 void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, double *out_b,
-               bool velocity, double u_inf, double v_inf, const FmmMarks &mk) {
+               bool velocity, double u_inf, double v_inf, const FmmMarks &mk, bool potential) {
     hipStream_t s = c.stream;
     const TreeInfo &ti = c.info;
     const int nc = ti.n_cells, nl = ti.n_leaves, P1 = P + 1;
@@ -361,7 +513,7 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     const double *t_m2l = (const double *)c.tab_m2l.p, *t_m2m = (const double *)c.tab_m2m.p,
                  *t_l2l = (const double *)c.tab_l2l.p;
 
-    k_p2m64<<<div_up(nl, kUpWaves), kWave * kUpWaves, 0, s>>>(
+    k_p2m64<<<div_up(nl, kP2m64Waves * kP2mLeaves), kWave * kP2m64Waves, 0, s>>>(
         leaf_idx, nl, P, c_flags, c_begin, c_end, c_cx, c_cy, xs, ys, qs, as, ak);
     for (int L = ti.max_level - 1; L >= 1; --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
@@ -372,8 +524,11 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     mk.mark(3, s);
     // M2L: every cell of level >= 2
     const int j2 = (int)ti.lev_off[2];
+    const int NP = potential ? P1 : P;  // orders formed per cell
     if (nc > j2)
-        k_m2l64<<<div_up((long long)(nc - j2) * P, 256), 256, 0, s>>>(
+        (potential ? (P <= 16 ? k_m2l64<true, 16> : k_m2l64<true, 0>)
+                   : (P <= 16 ? k_m2l64<false, 16> : k_m2l64<false, 0>))<<<
+            div_up((long long)(nc - j2) * NP, 256), 256, 0, s>>>(
             j2, nc, P, list4_mode, t_m2l, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
             (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
             as, ak, bk);
@@ -382,10 +537,20 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     for (int L = 3; L <= ti.max_level; ++L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
-        k_l2l64<<<div_up((long long)(j1 - j0) * P, 256), 256, 0, s>>>(j0, j1, P, t_l2l, c_parent,
-                                                                      c_cx, c_cy, bk);
+        (potential ? k_l2l64<true> : k_l2l64<false>)<<<div_up((long long)(j1 - j0) * NP, 256), 256,
+                                                          0, s>>>(j0, j1, P, t_l2l, c_parent,
+                                                                  c_cx, c_cy, bk);
     }
     mk.mark(5, s);
+    if (potential) {
+        k_pot64<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
+            leaf_idx, nl, P, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
+            (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs,
+            c.perm_sorted, ak, bk, stage_mask, out_a, &c.hdr_d->flags);
+        mk.mark(6, s);
+        VPM_LAUNCH_CHECK();
+        return;
+    }
     const double *ss = (const double *)c.ss.p;
     auto eval = [&](auto kern) {
         kern<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(

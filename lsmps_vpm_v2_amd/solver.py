@@ -17,7 +17,9 @@ Host arrays are numpy; device entry points take torch tensors on the GPU
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -221,6 +223,26 @@ class FmmSolver:
                                          _ptr(ex), _ptr(ey)), "vpmfmm_field")
         return ex, ey
 
+    def potential(self, x, y, q, active):
+        """phi of fmm2D::calcPotential for host arrays (FP64, singular kernel)."""
+        x, y, q, a = _f64(x), _f64(y), _f64(q), _u8(active)
+        n = len(x)
+        phi = np.empty(n)
+        self._check(self._L.vpmfmm_potential(self._h, n, _ptr(x), _ptr(y), _ptr(q), _ptr(a),
+                                             _ptr(phi)), "vpmfmm_potential")
+        return phi
+
+    def potential_device(self, x, y, q, active, phi=None):
+        """phi = sum_{j != i} q_j log r_ij on device tensors (float64 / uint8)."""
+        n = x.numel()
+        if phi is None:
+            phi = x.new_empty(n)
+        self._enter(x)
+        self._check(self._L.vpmfmm_potential_device(self._h, n, _dptr(x), _dptr(y), _dptr(q),
+                                                    _dptr(active), _dptr(phi)),
+                    "vpmfmm_potential_device")
+        return phi
+
     def velocity(self, x, y, s, omega, active):
         """(u, v) of VelocityCalc::get_velocity (FMM branch) for host arrays."""
         x, y, s, w, a = _f64(x), _f64(y), _f64(s), _f64(omega), _u8(active)
@@ -422,9 +444,14 @@ class TreeCell:
         self.rootLength = 0.0
         self.pivotCoord = np.zeros(2)
 
+    # the tree's own knobs (TreeCell reads Pars::par_count_max, src_count_max
+    # and expTree; treeCell.hpp:174-185), applied by fmm2D on this tree
+    TREE_KNOBS = ("max_par", "max_src", "exp_tree", "min_level", "max_level")
+
     def initializeTree(self, parPos, activeFlag):
         pos = np.asarray(parPos, dtype=np.float64)
         self._n = len(pos)
+        self._fingerprint = _fingerprint(pos[:, 0], pos[:, 1], activeFlag)
         self._solver.build_tree(pos[:, 0], pos[:, 1], activeFlag)
         nc, nl, lo, hi = self._solver.tree_counts()
         self.cellCount, self.min_level, self.max_level = nc, lo, hi
@@ -449,17 +476,56 @@ class TreeCell:
         return l2, l4
 
 
+def _fingerprint(x, y, active):
+    h = hashlib.blake2b(digest_size=16)
+    for a in (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64),
+              np.ascontiguousarray(active, np.uint8)):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
 class fmm2D:  # noqa: N801 -- reference class name
-    """``fmm2D::calcField`` / ``get_Field_x`` / ``get_Field_y``."""
+    """``fmm2D::calcField`` / ``calcPotential`` and their getters
+    (fmm2D.hpp:59-67).  The expansion order and binomials are this object's
+    (Pars::P_max); the tree is the given ``cellTree``: the library builds the
+    tree inside each call, deterministically, so the call runs with the
+    tree's leaf rule on the positions the tree was built from (others raise
+    ValueError, where the reference would mix the two)."""
 
     def __init__(self, solver: FmmSolver | None = None, **solver_opts):
         self._solver = solver or FmmSolver(**solver_opts)
         self._ex = np.zeros(0)
         self._ey = np.zeros(0)
+        self._phi = np.zeros(0)
+
+    @contextlib.contextmanager
+    def _on_tree(self, cellTree, x, y, activeFlag):
+        if cellTree is None:
+            yield
+            return
+        if getattr(cellTree, "_fingerprint", None) != _fingerprint(x, y, activeFlag):
+            raise ValueError("fmm2D: the tree was built from other positions or active flags "
+                             "(call cellTree.initializeTree on these particles first)")
+        keep = {k: getattr(self._solver.opts, k) for k in TreeCell.TREE_KNOBS}
+        self._solver.set_opts(**{k: getattr(cellTree._solver.opts, k)
+                                 for k in TreeCell.TREE_KNOBS})
+        try:
+            yield
+        finally:
+            self._solver.set_opts(**keep)
 
     def calcField(self, cellTree: TreeCell | None, parPos, activeFlag, srcVal):
         pos = np.asarray(parPos, dtype=np.float64)
-        self._ex, self._ey = self._solver.field(pos[:, 0], pos[:, 1], srcVal, activeFlag)
+        with self._on_tree(cellTree, pos[:, 0], pos[:, 1], activeFlag):
+            self._ex, self._ey = self._solver.field(pos[:, 0], pos[:, 1], srcVal, activeFlag)
+
+    def calcPotential(self, cellTree: TreeCell | None, parPos, activeFlag, srcVal):
+        pos = np.asarray(parPos, dtype=np.float64)
+        with self._on_tree(cellTree, pos[:, 0], pos[:, 1], activeFlag):
+            self._phi = self._solver.potential(pos[:, 0], pos[:, 1], srcVal, activeFlag)
+
+    def get_Potential(self):
+        return self._phi.copy()
 
     def get_Field_x(self):
         return self._ex.copy()
