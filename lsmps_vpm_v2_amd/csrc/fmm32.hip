@@ -917,7 +917,10 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
             eyd += __shfl_xor(eyd, m, kWave);
         }
         if (valid && g == 0) {
-            if (A.mask & 4) {
+            // local expansions exist from level 2 down (M2L and L2L form no
+            // other; b^ of a shallower leaf is never written): a leaf of level
+            // 0 or 1 -- a tree one level deep -- has no far field
+            if ((A.mask & 4) && L >= 2) {
                 // L2P: F = (1/s) sum_m m b^_m zeta^(m-1)
                 const cf z = {ti.x * inv_s, ti.y * inv_s};
                 const size_t cb = (size_t)j * PM1;
@@ -948,6 +951,8 @@ __global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl
     __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
     __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    // (an XCD-contiguous leaf order cut the L2 misses but unbalanced the XCDs:
+    // the dense band's leaves landed on a few of them, 0.129 -> 0.140 ms)
     const int t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kEvalWaves + w));
     if (t >= nl) return;
     const int j = (int)A.leaf_idx[t];
@@ -1095,7 +1100,8 @@ static void tier_levels(const TreeInfo &ti, int &K, int &T) {
 // Upward pass (P2M, M2M) on stream s; the buffers come from fmm32_prepare.
 // No clearing is needed: P2M / M2M write every coefficient a cell's readers
 // use (zero for inactive cells), the list-2 M2L writes b^_1..p of every cell
-// of level >= 2 before list 4 and L2L add to them.
+// of level >= 2 before list 4 and L2L add to them, and the evaluation reads
+// b^ only of leaves of level >= 2.
 void fmm32_p2m(Ctx &c, const Mixed32 &mx, hipStream_t s) {
     const int nl = c.info.n_leaves;
     k_p2m32<<<div_up(nl, kP2mWaves), kWave * kP2mWaves, 0, s>>>(

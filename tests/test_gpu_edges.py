@@ -148,3 +148,24 @@ def test_depth_overflow_raises(gpu, oracle, precision):
     with pytest.raises(VpmFmmError) as e:
         FmmSolver(**kw).field(x, y, q, a)
     assert e.value.status == "TREE_DEPTH_EXCEEDED"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_level_tree_after_a_large_solve_on_one_handle(gpu, oracle, precision):
+    """A tree one level deep (leaves at level 1) has no far field.  The mixed
+    path forms local coefficients from level 2 down only, so its L2P must not
+    read the level-1 leaves' coefficients: on a handle whose buffers still
+    hold a large solve's coefficients those are stale, not zero."""
+    from lsmps_vpm_v2_amd import workloads as W
+    kw = dict(p=10) if precision == "fp64" else dict(p=10, precision="mixed", binom_mode=0,
+                                                      list4_mode=2)
+    s = FmmSolver(**kw)
+    big = W.random_clusters(50000, seed=61)
+    s.field(big["x"], big["y"], big["q"], big["active"])
+    for n in (2, 3, 5, 40):
+        x, y, q, a = _field(n, 300 + n)
+        ex, ey = s.field(x, y, q, a)
+        assert s.stats()["max_leaf_level"] == 1
+        ex0, ey0 = oracle.direct_field(x, y, q)
+        tol = 1e-10 if precision == "fp64" else 5e-6 + 2 * _fp32_floor(oracle, x, y, q, a)
+        _check(ex, ey, ex0, ey0, tol)
