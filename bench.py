@@ -66,6 +66,73 @@ def eval_flops(st):
     return 11.0 * st["p2p_pairs"] + 8.0 * st["m2p_terms"] + 8.0 * st["l2p_terms"]
 
 
+def stage_work(solver, orders, fp64):
+    """Algorithmic work of each stage on the solver's last tree, counted from
+    its cells and lists (the inspection API) and the orders used per level:
+    P2M 8(p+1) flops per active particle of a leaf; M2M 8(p+1)(p+2)/2 per
+    active child link into a parent of level >= 1 (FP64) or >= 2 (mixed: its
+    upward pass stops there); L2L the same per child link from a parent of
+    level >= 2; M2L 8(p+1)^2 per list-2 translation (and per list-4 one on the
+    FP64 path, whose M2L stage does list 4 by M2L); list build: each list
+    entry written once (4 B) and each cell record read once (30 B)."""
+    cells = solver.tree_cells()
+    lev = cells["level"].astype(np.int64)
+    leaf = cells["is_leaf"].astype(bool)
+    act = cells["is_active"].astype(bool)
+    p = np.asarray(orders, np.int64)
+    pl = p[lev]
+    n2 = np.zeros(len(lev), np.int64)
+    n4 = np.zeros(len(lev), np.int64)
+    for j in np.nonzero(lev >= 2)[0]:
+        _, l2, _, l4 = solver.tree_lists(cells["id"][j], cap=256)
+        n2[j], n4[j] = len(l2), len(l4)
+    par = np.maximum(lev - 1, 0)
+    pp = p[par]
+    tri = 8.0 * (pp + 1) * (pp + 2) / 2.0
+    up_floor = 1 if fp64 else 2
+    p2m = 8.0 * float(np.sum(cells["src_num"][leaf & act] * (pl[leaf & act] + 1)))
+    m2m = float(np.sum(tri[(lev >= 1) & (par >= up_floor) & act]))
+    l2l = float(np.sum(tri[(lev >= 1) & (par >= 2)]))
+    m2l = 8.0 * float(np.sum((n2 + (n4 if fp64 else 0)) * (pl + 1) ** 2))
+    return {"p2m_flops": p2m, "m2m_flops": m2m, "l2l_flops": l2l, "m2l_flops": m2l,
+            "list2_translations": int(n2.sum()), "list4_entries": int(n4.sum()),
+            "n_cells": int(len(lev))}
+
+
+def stage_rooflines(st, stage_ms, work, n, fp64):
+    """Per-stage roofline entries and the time-weighted overall fraction
+    sum_s t_s (achieved_s / peak_s) / sum_s t_s.  Bounds: the tree build and
+    the list build by HBM (algorithmic bytes: 48 B per particle for the
+    binning, read x, y, q and write them sorted; the list bytes of
+    stage_work); M2L on the mixed path by FP32 MFMA; the other stages by the
+    vector ALU (FP32 or FP64 peak)."""
+    vpeak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
+    lists_bytes = 4.0 * (st["n_list1"] + st["n_list2"] + st["n_list3"] + st["n_list4"]) + \
+        30.0 * st["n_cells"]
+    spec = [
+        ("tree", "ms_tree", "hbm", 48.0 * n, HBM_PEAK_GBS, "GB/s"),
+        ("lists", "ms_lists", "hbm", lists_bytes, HBM_PEAK_GBS, "GB/s"),
+        ("upward (P2M + M2M)", "ms_upward", "valu", work["p2m_flops"] + work["m2m_flops"],
+         vpeak, "TFLOP/s"),
+        ("M2L (list 2" + (" + list 4)" if fp64 else ")"), "ms_m2l", "valu" if fp64 else "mfma",
+         work["m2l_flops"], vpeak, "TFLOP/s"),
+        ("downward (L2L)", "ms_downward", "valu", work["l2l_flops"], vpeak, "TFLOP/s"),
+        ("evaluation (P2P + M2P + L2P)", "ms_eval", "valu", eval_flops(st), vpeak, "TFLOP/s"),
+    ]
+    out, num, den = [], 0.0, 0.0
+    for name, key, bound, work_s, peak, unit in spec:
+        ms = stage_ms[key]
+        scale = 1e9 if unit == "GB/s" else 1e12
+        achieved = work_s / (ms * 1e-3) / scale if ms > 0 else 0.0
+        frac = achieved / peak
+        out.append({"stage": name, "bound": bound, "ms": ms, "work": work_s,
+                    "work_unit": "bytes" if unit == "GB/s" else "flops", "achieved": achieved,
+                    "peak": peak, "unit": unit, "frac": frac})
+        num += ms * frac
+        den += ms
+    return out, (num / den if den > 0 else 0.0)
+
+
 def time_path(torch, solver, dev, steps, warmup):
     """Mean seconds per step and the per-step statistics; the selection of
     the first call (which chooses the leaf threshold on the learned path) is
@@ -109,6 +176,25 @@ def time_loop(torch, solver, dev, steps, warmup, dt=2e-3):
             "particles_moved_per_s": x.numel() / sec}
 
 
+def time_host_path(FmmSolver, opts, field, steps, warmup):
+    """The drop-in's own call: vpmfmm_velocity with host (pageable numpy)
+    arrays, as VelocityCalc::get_velocity makes it -- the copies of x, y, s,
+    omega, active in (33 B per particle) and u, v out (16 B) included."""
+    s = FmmSolver(**opts)
+    host = [np.ascontiguousarray(field[k]) for k in ("x", "y", "s", "omega", "active")]
+    for _ in range(max(warmup, 1)):
+        s.velocity(*host)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.velocity(*host)
+    sec = (time.perf_counter() - t0) / steps
+    n = field["n"]
+    return {"value": n / sec, "unit": UNIT, "ms_per_step": sec * 1e3, "steps": steps,
+            "bytes_copied_per_step": 49 * n,
+            "what": "vpmfmm_velocity (host pointers, pageable memory): the H2D copies of "
+                    "x, y, s, omega, active, the solve, the D2H copies of u, v"}
+
+
 def rel_l2(u, v, ur, vr):
     return float(np.sqrt(np.sum((u - ur) ** 2 + (v - vr) ** 2) / np.sum(ur ** 2 + vr ** 2)))
 
@@ -122,28 +208,47 @@ def direct_reference(torch, solver, field, dev, n_targets=20000):
     return tidx, ey.cpu().numpy(), -ex.cpu().numpy()
 
 
-def cpu_baseline(field, reps=3):
-    """Reference FMM (oracle/_ref) on the host threads; falls back to the C
-    restatement (single thread) if the reference build is absent.  Returns
-    (baseline dict, reference field (ex, ey) or None)."""
-    from oracle import oracle as O
-    n = field["n"]
-    cores = os.cpu_count() or 1
+def host_cores():
+    """(threads used, physical cores from lscpu or None): the threads are the
+    process's OpenMP share (OMP_NUM_THREADS, else the CPUs it may run on)."""
+    import subprocess
+    phys = None
     try:
-        cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        pairs = {ln for ln in out.splitlines() if ln and not ln.startswith("#")}
+        phys = len(pairs) or None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    threads = len(os.sched_getaffinity(0))
+    try:
+        threads = int(os.environ.get("OMP_NUM_THREADS", threads))
     except ValueError:
         pass
+    return max(1, threads), phys
+
+
+def cpu_baseline(field, reps=5, single_thread=True):
+    """Reference FMM (oracle/_ref) on the host threads: one warm-up, then the
+    median of `reps` solves of the full field; and one single-thread solve.
+    Falls back to the C restatement (single thread) if the reference build is
+    absent.  Returns (baseline dict, reference field (ex, ey) or None)."""
+    from oracle import oracle as O
+    n = field["n"]
+    threads, phys = host_cores()
     times, ref = [], None
     if O.ref_available():
+        O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"], threads=threads)
         for _ in range(reps):
             t0 = time.perf_counter()
             ref = O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"],
-                                  threads=cores)
+                                  threads=threads)
             times.append(time.perf_counter() - t0)
-        kind, used = "reference", cores
+        kind, used = "reference", threads
         sample = (f"reference TreeCell::initializeTree + fmm2D::calcField built from the "
                   f"reference sources (g++ -O2 -fopenmp), the full {n}-particle field, p=10, "
-                  f"75/50 leaf rule, median of {reps} solves")
+                  f"75/50 leaf rule, {threads} OpenMP threads, median of {reps} solves after "
+                  f"one warm-up")
     else:
         t0 = time.perf_counter()
         O.fmm_field(field["x"], field["y"], field["q"], field["active"], O.opts())
@@ -152,8 +257,36 @@ def cpu_baseline(field, reps=3):
         sample = f"C restatement, {n} particles, single thread, 1 solve"
     t = statistics.median(times)
     out = {"value": n / t, "unit": UNIT, "cores": used, "kind": kind, "sample": sample,
-           "seconds_per_solve": t}
+           "seconds_per_solve": t, "solves_s": times, "host_physical_cores": phys}
+    if kind == "reference" and single_thread:
+        t0 = time.perf_counter()
+        O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"], threads=1)
+        t1 = time.perf_counter() - t0
+        out["single_thread"] = {"value": n / t1, "seconds_per_solve": t1, "cores": 1,
+                                "sample": "the same solve on 1 thread, once"}
     return out, (None if ref is None else (ref[0], ref[1]))
+
+
+def best_fixed(torch, FmmSolver, dev, tidx, ur, vr, u_inf, steps, warmup,
+               orders=(6, 7, 8, 9, 10), thresholds=(32, 48, 64, 96, 128), target=1e-4):
+    """The fixed-order mixed path swept over uniform orders x leaf thresholds
+    (max_par = max_src): each setting's time per step (same timing as the
+    headline) and rel-L2 against the FP64 direct sum at the same targets.
+    Returns the fastest setting that meets `target` and the whole table."""
+    table = []
+    for t in thresholds:
+        for p in orders:
+            s = FmmSolver(precision="mixed", p=p, max_par=t, max_src=t, binom_mode=0,
+                          list4_mode=2)
+            dt, _ = time_path(torch, s, dev, steps, warmup)
+            u, v = s.velocity_device(*dev)
+            uu = u.cpu().numpy()[tidx] - u_inf
+            vv = v.cpu().numpy()[tidx]
+            table.append({"p": p, "threshold": t, "ms_per_step": dt * 1e3,
+                          "rel_l2_vs_fp64_direct": rel_l2(uu, vv, ur, vr)})
+    ok = [r for r in table if r["rel_l2_vs_fp64_direct"] <= target]
+    best = min(ok, key=lambda r: r["ms_per_step"]) if ok else None
+    return best, table
 
 
 def load_traffic(kernel, workload):
@@ -178,6 +311,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--no-loop", action="store_true",
                     help="skip the inviscid-loop timing (advection + velocity per step)")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip timing the host-pointer entry point (copies included)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the sweep of fixed orders x leaf thresholds (best fixed setting)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -226,8 +363,18 @@ def main():
         log(f"[bench] selection: source {sel['source']} threshold {sel['threshold']} "
             f"p {sel['p'][:st['max_leaf_level'] + 1]} predicted err {sel['pred_err']:.2e} "
             f"ms {sel['pred_ms']:.3f}")
+    fp64 = opts["precision"] == "fp64"
+    sel_t = solver_t.last_selection()
+    orders = sel_t["p"] if not fp64 else [opts["p"]] * 17
+    work = stage_work(solver_t, orders, fp64)
     u, v = solver.velocity_device(*dev)
     u_h, v_h = u.cpu().numpy() - solver.opts.u_inf, v.cpu().numpy() - solver.opts.v_inf
+
+    host = None
+    if not args.no_host:
+        host = time_host_path(FmmSolver, opts, field, max(3, args.steps // 4), 1)
+        log(f"[bench] host-pointer path (vpmfmm_velocity, copies included): "
+            f"{host['ms_per_step']:.3f} ms/step")
 
     loop = None
     if not args.no_loop:
@@ -239,19 +386,22 @@ def main():
     if args.path != "fp64" and not args.no_fp64:
         s64 = FmmSolver(**PATHS["fp64"])
         dt64, _ = time_path(torch, s64, dev, args.steps, args.warmup)
-        _, st64 = time_path(torch, FmmSolver(timing=True, **PATHS["fp64"]), dev, args.steps,
-                            args.warmup)
+        s64t = FmmSolver(timing=True, **PATHS["fp64"])
+        _, st64 = time_path(torch, s64t, dev, args.steps, args.warmup)
         u64, v64 = s64.velocity_device(*dev)
-        out_fp64 = {"value": n / dt64, "ms_per_step": dt64 * 1e3,
-                    "stage_ms": {k: float(np.mean([s[k] for s in st64])) for k in
-                                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward",
-                                  "ms_eval")},
+        sms64 = {k: float(np.mean([s[k] for s in st64])) for k in
+                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_eval")}
+        w64 = stage_work(s64t, [PATHS["fp64"]["p"]] * 17, True)
+        stg64, wt64 = stage_rooflines(st64[-1], sms64, w64, n, True)
+        out_fp64 = {"value": n / dt64, "ms_per_step": dt64 * 1e3, "stage_ms": sms64,
+                    "stage_roofline": stg64, "time_weighted_frac": wt64,
                     "config": PATHS["fp64"],
                     "_uv": (u64.cpu().numpy() - 1.0, v64.cpu().numpy())}
         log(f"[bench] fp64: {dt64 * 1e3:.3f} ms/step; speedup of {args.path} "
             f"{dt64 / dt:.2f}x")
 
     err = None
+    sweep = None
     if not args.no_accuracy:
         tidx, ur, vr = direct_reference(torch, solver, field, dev)
         err = rel_l2(u_h[tidx], v_h[tidx], ur, vr)
@@ -259,6 +409,19 @@ def main():
         if out_fp64:
             uu, vv = out_fp64["_uv"]
             out_fp64["rel_l2_vs_fp64_direct"] = rel_l2(uu[tidx], vv[tidx], ur, vr)
+        if args.path != "fp64" and not args.no_sweep:
+            best, table = best_fixed(torch, FmmSolver, dev, tidx, ur, vr, solver.opts.u_inf,
+                                     args.steps, args.warmup)
+            sweep = {"best_fixed": best, "table": table,
+                     "what": "fixed-order mixed path, uniform p in 6..10 x leaf threshold "
+                             "(max_par = max_src) in {32, 48, 64, 96, 128}; the fastest "
+                             "setting with rel-L2 <= 1e-4 against the FP64 direct sum"}
+            if best:
+                sweep["headline_speedup_vs_best_fixed"] = best["ms_per_step"] / (dt * 1e3)
+                log(f"[bench] best fixed setting: p {best['p']} threshold {best['threshold']} "
+                    f"{best['ms_per_step']:.3f} ms/step (rel-L2 "
+                    f"{best['rel_l2_vs_fp64_direct']:.2e}); {args.path} is "
+                    f"{sweep['headline_speedup_vs_best_fixed']:.3f}x that")
 
     stage_ms = {k: float(np.mean([s[k] for s in stats])) for k in
                 ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_eval")}
@@ -266,19 +429,34 @@ def main():
     # dominant kernel: the evaluation kernel (P2P + M2P + L2P), vector-ALU bound
     eval_ms = stage_ms["ms_eval"]
     flops = eval_flops(st)
-    fp64 = opts["precision"] == "fp64"
     peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
     achieved = flops / (eval_ms * 1e-3) / 1e12
     traffic, traffic_src = load_traffic(KERNEL[args.path], field["name"])
+    stages, weighted = stage_rooflines(st, stage_ms, work, n, fp64)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "kernel": KERNEL[args.path],
+                "bound_note": "compute bound on the vector ALU, not on HBM; 'mfma' is the "
+                              "compute-bound label of this field, and the peak is the MI355X "
+                              + ("FP64" if fp64 else "FP32") + " vector figure",
                 "compute_unit": "vector ALU (" + ("FP64" if fp64 else "FP32") + ")",
                 "flops_per_launch": flops, "ms_per_launch": eval_ms,
                 "timing": "HIP events on the solver stream around the kernel, mean over the "
                           "steps of a second solver that records stage events",
-                "traffic_source": traffic_src}
+                "traffic_source": traffic_src,
+                "stages": stages, "time_weighted_frac": weighted,
+                "stage_work": work}
+    log("[bench] stage rooflines: " + ", ".join(
+        f"{e['stage'].split(' ')[0]} {e['achieved']:.2f} {e['unit']} ({e['frac']:.3f})"
+        for e in stages) + f"; time-weighted {weighted:.3f}")
 
     cfg = {k: v for k, v in opts.items()}
+    if args.path == "learned":
+        # the options hold the fallback; the tree and orders actually used
+        cfg = {k: v for k, v in cfg.items() if k not in ("p", "max_par", "max_src")}
+        cfg["fallback"] = {"p": opts["p"], "max_par": opts["max_par"],
+                           "max_src": opts["max_src"]}
+        cfg["leaf_threshold"] = sel["threshold"]
+        cfg["p_per_level"] = sel["p"][:st["max_leaf_level"] + 1]
     out = {
         "metric": METRIC, "value": n / dt, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
@@ -299,11 +477,15 @@ def main():
         if first and first["reselected"]:
             # once per run (and when n changes by > 10 %): not part of a step
             out["selection"]["threshold_choice_ms"] = first["ms_reselect"]
+    if host:
+        out["host_path"] = host
     if loop:
         out["inviscid_loop"] = loop
     if out_fp64:
         out["fp64_path"] = {k: val for k, val in out_fp64.items() if not k.startswith("_")}
         out["speedup_vs_fp64"] = out["value"] / out_fp64["value"]
+    if sweep:
+        out["fixed_order_sweep"] = sweep
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
         base, ref = cpu_baseline(field)
