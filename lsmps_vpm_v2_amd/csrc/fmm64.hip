@@ -209,6 +209,102 @@ __global__ void __launch_bounds__(256)
     st(bk, (size_t)j * P1 + p, acc);
 }
 
+// M2L by one wavefront per target cell, for orders P <= 16 (the same sums as
+// k_m2l64).  The wave stages its M2L sources' coefficients a_k and
+// w = 1/(zs - zt) in LDS once (the list-2 entries, then the list-4 ones
+// with list4_mode 1); lane l = g NP + i (NP = P orders, P + 1 for the
+// potential) forms order p = i (+1 without the potential) over the sources
+// u = g, g + G, ... (G = 64 / NP groups), and the groups' partial sums are
+// added in group order.  Every lane works, and a source's coefficients are
+// read from HBM / L2 once per target instead of once per (target, order).
+// P2L (list4_mode 2) splits the list-4 particles over the groups the same way.
+constexpr int kM2lwWaves = 4;
+constexpr int kM2lwSrc = 27 + 8;  // list 2 <= 27, list 4 <= 8
+constexpr int kM2lwP1 = 17;       // orders up to 16
+
+template <bool POT>
+__global__ void __launch_bounds__(kWave *kM2lwWaves)
+    k_m2l64w(int j0, int nc, int P, int list4_mode, const double *tab, const int *l2,
+             const uint8_t *n2, const int *l4, const uint8_t *n4, const double *c_cx,
+             const double *c_cy, const uint32_t *c_begin, const uint32_t *c_end,
+             const double *xs, const double *ys, const double *qs, const uint8_t *as,
+             const double2 *ak, double2 *bk) {
+    __shared__ double2 sak[kM2lwWaves][kM2lwSrc * kM2lwP1];
+    __shared__ double2 sw[kM2lwWaves][kM2lwSrc], slg[kM2lwWaves][kM2lwSrc];
+    __shared__ double2 red[kM2lwWaves][kWave];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int j = j0 + blockIdx.x * kM2lwWaves + w;
+    if (j >= nc) return;  // uniform over the wavefront
+    const int P1 = P + 1, NP = POT ? P1 : P, G = kWave / NP;
+    const int g = lane / NP, i = lane % NP, p = (POT ? 0 : 1) + i;
+    const double cx = c_cx[j], cy = c_cy[j];
+    const int m2 = n2[j], m4 = n4[j];
+    const int ns = m2 + (list4_mode == 1 ? m4 : 0);
+    // staging: w and (potential) log(z_t - z_s) per source, then the a_k rows
+    if (lane < ns) {
+        const int sidx = lane < m2 ? l2[27 * j + lane] : l4[8 * j + (lane - m2)];
+        const cd wv = cinv(cd{c_cx[sidx] - cx, c_cy[sidx] - cy});
+        sw[w][lane] = make_double2(wv.x, wv.y);
+        if (POT) {
+            const cd lg = clog(cd{cx - c_cx[sidx], cy - c_cy[sidx]});
+            slg[w][lane] = make_double2(lg.x, lg.y);
+        }
+    }
+    for (int r = lane; r < ns * P1; r += kWave) {
+        const int u = r / P1, k = r % P1;
+        const int sidx = u < m2 ? l2[27 * j + u] : l4[8 * j + (u - m2)];
+        sak[w][u * kM2lwP1 + k] = ak[(size_t)sidx * P1 + k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    cd acc = {0.0, 0.0};
+    if (g < G) {
+        const double *row = tab + (size_t)p * P1;
+        for (int u = g; u < ns; u += G) {
+            const double2 w2 = sw[w][u];
+            const cd wv = {w2.x, w2.y};
+            const double2 *a = &sak[w][u * kM2lwP1];
+            if (POT && p == 0) {
+                cd sm = {0.0, 0.0};
+                for (int k = P; k >= 1; --k)
+                    sm = cmul(cadd(sm, cscale(ld(a, k), ((k & 1) ? 1.0 : -1.0) / k)), wv);
+                const double2 lg = slg[w][u];
+                acc = cadd(acc, cadd(sm, cmul(ld(a, 0), cd{lg.x, lg.y})));
+                continue;
+            }
+            cd sm = cscale(ld(a, P), row[P]);
+            for (int k = P - 1; k >= 0; --k) sm = cadd(cmul(sm, wv), cscale(ld(a, k), row[k]));
+            acc = cadd(acc, cmul(sm, cpowi(wv, p)));
+        }
+        if (list4_mode == 2) {
+            // P2L (fmm2D.cpp:526-553): the list-4 leaves' active particles,
+            // taken in turn by the groups
+            const double inv_p = p > 0 ? 1.0 / (double)p : 0.0;
+            int q = 0;
+            for (int u = 0; u < m4; ++u) {
+                const int sidx = l4[8 * j + u];
+                for (uint32_t pi = c_begin[sidx]; pi < c_end[sidx]; ++pi, ++q) {
+                    if (q % G != g || !as[pi]) continue;
+                    if (POT && p == 0) {
+                        acc = cadd(acc, cscale(clog(cd{cx - xs[pi], cy - ys[pi]}), qs[pi]));
+                        continue;
+                    }
+                    const cd wv = cinv(cd{xs[pi] - cx, ys[pi] - cy});
+                    acc = cadd(acc, cscale(cpowi(wv, p), -qs[pi] * inv_p));
+                }
+            }
+        }
+    }
+    red[w][lane] = make_double2(acc.x, acc.y);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < NP) {
+        cd sum = {0.0, 0.0};
+        for (int gg = 0; gg < G; ++gg) sum = cadd(sum, ld(red[w], gg * NP + lane));
+        st(bk, (size_t)j * P1 + (POT ? 0 : 1) + lane, sum);
+    }
+}
+
 // L2L for the cells of one level (>= 3): each child adds its parent's shifted
 // local expansion.  tab[p][k] = C(k, p).  Orders 1..P, and 0 for the
 // potential (fmm2D.cpp:637-643 runs p from 0).
@@ -525,9 +621,14 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     // M2L: every cell of level >= 2
     const int j2 = (int)ti.lev_off[2];
     const int NP = potential ? P1 : P;  // orders formed per cell
-    if (nc > j2)
-        (potential ? (P <= 16 ? k_m2l64<true, 16> : k_m2l64<true, 0>)
-                   : (P <= 16 ? k_m2l64<false, 16> : k_m2l64<false, 0>))<<<
+    if (nc > j2 && P < kM2lwP1)
+        (potential ? k_m2l64w<true> : k_m2l64w<false>)<<<div_up(nc - j2, kM2lwWaves),
+                                                         kWave * kM2lwWaves, 0, s>>>(
+            j2, nc, P, list4_mode, t_m2l, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
+            (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
+            as, ak, bk);
+    else if (nc > j2)
+        (potential ? k_m2l64<true, 0> : k_m2l64<false, 0>)<<<
             div_up((long long)(nc - j2) * NP, 256), 256, 0, s>>>(
             j2, nc, P, list4_mode, t_m2l, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
             (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
