@@ -66,9 +66,10 @@ typedef struct vpmfmm_opts {
                                (0 = on the first call and when n changes by > 10%)      */
     int32_t judge_every;    /* learned: the call that makes a fresh order selection (and,
                                for k > 0, every k-th call) also solves at the fixed order
-                               p on the same tree and compares; a difference above the
-                               model's target returns the fixed-order result and keeps
-                               the fixed order until the next re-selection; -1 = never  */
+                               p on the same tree and compares; a difference above 0.9 x
+                               the model's target (the rest is left for the fixed order's
+                               own error) returns the fixed-order result and keeps the
+                               fixed order until the next re-selection; -1 = never      */
     int32_t icutoff;        /* near-field (P2P) kernel: 0 = singular, the type-2 FMM
                                (fmm2D.cpp:164-173); 1 / 2 / 3 = the high-order
                                algebraic / Gaussian / super-Gaussian regularization of

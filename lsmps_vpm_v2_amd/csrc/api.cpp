@@ -211,6 +211,7 @@ struct vpmfmm_handle_s {
 namespace {
 
 constexpr int kMaxP = 40;
+constexpr float kJudgeShare = 0.9f;  // share of the target the judged difference may take
 constexpr int kMaxThreshold = 4096;
 
 int opts_check(const vpmfmm_opts &o, std::string &msg) {
@@ -603,7 +604,12 @@ void run(vpmfmm_handle *h, int n, const double *x, const double *y, const Source
                 const double a0 = velocity ? h->o.u_inf : 0.0, b0 = velocity ? h->o.v_inf : 0.0;
                 ls.judge_err = (float)rel_diff64(nn, oa, ob, ja, jb, a0, b0,
                                                  h->judge_part.as<double>(1024), c.stream);
-                if (!(ls.judge_err <= h->sel.target_err)) {
+                // The learned error against the exact sum is at most the judged
+                // difference plus the fixed order's own error; the judge keeps
+                // kJudgeShare of the target for the first, leaving the rest for
+                // the second (p = 12 measures 1.5e-6, 0.015 of a 1e-4 target,
+                // on the 780k flat plate)
+                if (!(ls.judge_err <= kJudgeShare * h->sel.target_err)) {
                     h->judge_reject = true;
                     h->judged_reject_now = true;
                     VPM_HIP(hipMemcpyAsync(oa, ja, sizeof(double) * nn, hipMemcpyDeviceToDevice,

@@ -204,9 +204,13 @@ def test_out_of_range_features_fall_back_to_fixed_order(gpu):
 
 def test_trained_selector_gains_over_fixed_order(gpu):
     """The packaged model on fields it was not trained on: error <= 1e-4
-    against the FP64 direct sum, and at least 5 % faster than the same solver
-    without the network (its fallback: fixed order 12, leaf rule 75/50) on the
-    same field."""
+    against the FP64 direct sum; at least 5 % faster than the same solver
+    without the network (its fallback: fixed order 12, leaf rule 75/50) and
+    than that order with the hand-tuned 48/48 rule; and within 5 % of the best
+    fixed setting an exhaustive sweep finds (uniform p 7-9 x thresholds 48,
+    64, 96 meeting 1e-4), which the network reaches without the sweep.  Times
+    are the median of 15 single solves (each call returns with its result
+    complete)."""
     import time
     import torch
     from lsmps_vpm_v2_amd import FmmSolver, selector as S, workloads as W
@@ -226,22 +230,30 @@ def test_trained_selector_gains_over_fixed_order(gpu):
             return float(np.sqrt(np.sum((uu - ur) ** 2 + (vv - vr) ** 2) /
                                  np.sum(ur ** 2 + vr ** 2)))
 
-        def timed(s, reps=10):
-            s.velocity_device(*dev)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
+        def timed(s, reps=15):
+            for _ in range(2):
                 u, v = s.velocity_device(*dev)
             torch.cuda.synchronize()
-            return (time.perf_counter() - t0) / reps, u, v
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                u, v = s.velocity_device(*dev)
+                ts.append(time.perf_counter() - t0)
+            return float(np.median(ts)), u, v
 
         # the method without the network: the learned solver's own fallback,
         # fixed order 12 with the handle's leaf rule (the reference's 75/50)
         kw = dict(p=12, precision="mixed", binom_mode=0, list4_mode=2)
-        fixed = FmmSolver(**kw)
-        t_fix, u, v = timed(fixed)
+        t_fix, u, v = timed(FmmSolver(**kw))
         e_fix = err_of(u, v)
-        t_48, _, _ = timed(FmmSolver(max_par=48, max_src=48, **kw))  # a hand-tuned rule, for reference
+        t_48, _, _ = timed(FmmSolver(max_par=48, max_src=48, **kw))
+        best = None
+        for p in (7, 8, 9):
+            for t in (48, 64, 96):
+                tt, u, v = timed(FmmSolver(p=p, max_par=t, max_src=t, precision="mixed",
+                                           binom_mode=0, list4_mode=2), reps=7)
+                if err_of(u, v) <= 1e-4 and (best is None or tt < best[0]):
+                    best = (tt, p, t)
         learned = FmmSolver(order="learned", **kw)
         t_lrn, u, v = timed(learned)
         e_lrn = err_of(u, v)
@@ -250,16 +262,15 @@ def test_trained_selector_gains_over_fixed_order(gpu):
         learned.velocity_device(*dev)
         judged = learned.last_selection()
         print(f"{f['name']}: fixed p=12 (75/50) {t_fix * 1e3:.3f} ms err {e_fix:.2e} "
-              f"[48/48: {t_48 * 1e3:.3f} ms]; learned "
+              f"[48/48: {t_48 * 1e3:.3f} ms]; best fixed p {best[1]} threshold {best[2]} "
+              f"{best[0] * 1e3:.3f} ms; learned "
               f"{t_lrn * 1e3:.3f} ms err {e_lrn:.2e} (pred {sel['pred_err']:.2e}, judge "
               f"{judged['judge_err']:.2e}), threshold {sel['threshold']}, p {sel['p'][:12]}, "
               f"source {sel['source']}")
         assert sel["source"] == 1 and judged["source"] == 1
-        assert e_lrn <= 1e-4 and judged["judge_err"] <= 1e-4
-        # the orders only change the upward pass, M2L and the downward pass
-        # (about a third of a solve at these sizes), so the saving is bounded;
-        # measured: 14 % (2M shed vortices), 7 % (780k boundary layer)
-        assert t_lrn <= 0.95 * t_fix
+        assert e_lrn <= 1e-4 and judged["judge_err"] <= 0.9e-4
+        assert t_lrn <= 0.95 * t_fix and t_lrn <= 0.95 * t_48
+        assert t_lrn <= 1.05 * best[0]
 
 
 def test_fixed_order_judge_rejects_a_bad_model(gpu):
