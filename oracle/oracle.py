@@ -44,6 +44,17 @@ class OrcOpts(C.Structure):
     ]
 
 
+class T1Opts(C.Structure):
+    """Options of the type-1 oracle (fmm2d_type1_oracle.h)."""
+    _fields_ = [
+        ("n_s", C.c_int),
+        ("lmax", C.c_int),
+        ("ndp", C.c_int),
+        ("icutoff", C.c_int),
+        ("l2l_mode", C.c_int),
+    ]
+
+
 def _d(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -106,6 +117,23 @@ def lib():
         L.orc_l2l.argtypes = [_dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         L.orc_m2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_l2p.argtypes = [_dp, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_t1_default_opts.argtypes = [C.POINTER(T1Opts)]
+        L.orc_t1_bico.argtypes = [C.c_int, _ip]
+        L.orc_t1_bico.restype = C.c_double
+        L.orc_t1_l2l_table.argtypes = [C.c_int, C.c_int, _dp, _ip]
+        L.orc_t1_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _dp, C.POINTER(T1Opts)]
+        L.orc_t1_free.argtypes = [C.c_void_p]
+        for name in ("orc_t1_nboxes", "orc_t1_depth", "orc_t1_nlost"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.orc_t1_root.argtypes = [C.c_void_p, _dp]
+        L.orc_t1_boxes.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _dp]
+        L.orc_t1_box_particles.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        L.orc_t1_list.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int]
+        L.orc_t1_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(T1Opts), _dp, _dp]
+        L.orc_t1_direct.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _ip, _dp, _dp]
+        for name in ("orc_t1_default_opts", "orc_t1_l2l_table", "orc_t1_free", "orc_t1_root",
+                     "orc_t1_boxes", "orc_t1_direct"):
+            getattr(L, name).restype = None
         for name in ("orc_p2m", "orc_m2m", "orc_m2l", "orc_l2l", "orc_m2p", "orc_l2p",
                      "orc_tree_free", "orc_tree_geometry", "orc_tree_cells",
                      "orc_tree_leaves", "orc_default_opts"):
@@ -138,6 +166,8 @@ def ref():
         L.ref_ext_list.argtypes = [C.c_int64, _lp, _ip, _lp, _ip, C.c_int]
         L.ref_fmm_field.argtypes = [C.c_int, _dp, _dp, _dp, _up, _dp, _dp, _dp]
         L.ref_fmm_potential.argtypes = [C.c_int, _dp, _dp, _dp, _up, _dp]
+        L.ref_type1_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, _dp, _dp]
         _ref = L
     return _ref
 
@@ -458,3 +488,136 @@ def ref_fmm_potential(x, y, q, active, p=10, max_par=75, max_src=50, exp_tree=1.
 
 def ref_binom(n, k) -> int:
     return int(ref().ref_binom(n, k))
+
+
+# --------------------------------------------------------------------------
+# type-1 ("old") FMM: biotsavart_fmm_2d and its subroutines
+# --------------------------------------------------------------------------
+T1_STATUS = {1: "bad argument", 2: "non-finite input", 5: "out of memory",
+             7: "domain too small for the absolute box tolerance"}
+
+
+def t1_opts(n_s=50, lmax=7, ndp=10, icutoff=2, l2l_mode=1) -> T1Opts:
+    """Defaults: global.cpp:316-328; l2l_mode 1 is the shift as the reference
+    runs it."""
+    return T1Opts(n_s, lmax, ndp, icutoff, l2l_mode)
+
+
+def t1_bico(n, k):
+    """(value, k after the call) of the reference's bico."""
+    kk = C.c_int(k)
+    c = lib().orc_t1_bico(n, C.byref(kk))
+    return c, kk.value
+
+
+def t1_l2l_table(ndp, l2l_mode=1):
+    m = ndp + 1
+    coef = np.zeros(m * m)
+    expo = np.zeros(m * m, np.int32)
+    lib().orc_t1_l2l_table(ndp, l2l_mode, _ptr(coef, _dp), _ptr(expo, _ip))
+    return coef.reshape(m, m), expo.reshape(m, m)
+
+
+class T1Tree:
+    """The type-1 box hierarchy (hierarchy_mesh) and its lists."""
+
+    def __init__(self, x, y, o: T1Opts | None = None):
+        L = lib()
+        self._x, self._y = _d(x), _d(y)
+        self.o = o or t1_opts()
+        h = C.c_void_p()
+        st = L.orc_t1_build(C.byref(h), len(self._x), _ptr(self._x, _dp), _ptr(self._y, _dp),
+                            C.byref(self.o))
+        if st:
+            raise OracleError(f"orc_t1_build: {T1_STATUS.get(st, st)}")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_t1_free(self.h)
+            self.h = None
+
+    @property
+    def nboxes(self):
+        return lib().orc_t1_nboxes(self.h)
+
+    @property
+    def depth(self):
+        return lib().orc_t1_depth(self.h)
+
+    @property
+    def nlost(self):
+        return lib().orc_t1_nlost(self.h)
+
+    def root(self):
+        g = np.zeros(5)
+        lib().orc_t1_root(self.h, _ptr(g, _dp))
+        return g
+
+    def boxes(self):
+        nb = self.nboxes
+        out = {k: np.zeros(nb, np.int32) for k in ("level", "ix", "iy", "count", "nchild",
+                                                   "parent")}
+        bnd = np.zeros((nb, 4))
+        lib().orc_t1_boxes(self.h, *(_ptr(out[k], _ip) for k in ("level", "ix", "iy", "count",
+                                                                "nchild", "parent")),
+                           _ptr(bnd, _dp))
+        out["bounds"] = bnd
+        return out
+
+    def particles(self, b):
+        cap = len(self._x)
+        buf = np.zeros(max(cap, 1), np.int32)
+        m = lib().orc_t1_box_particles(self.h, int(b), _ptr(buf, _ip), cap)
+        return buf[:m].copy()
+
+    def list(self, b, which):
+        cap = 1024
+        while True:
+            buf = np.zeros(cap, np.int32)
+            m = lib().orc_t1_list(self.h, int(b), int(which), _ptr(buf, _ip), cap)
+            if m < 0:
+                raise OracleError("orc_t1_list failed")
+            if m <= cap:
+                return buf[:m].copy()
+            cap = m
+
+
+def t1_velocity(x, y, s, g, o: T1Opts | None = None):
+    """Rotational velocity of the type-1 FMM (no free stream)."""
+    x, y, s, g = _d(x), _d(y), _d(s), _d(g)
+    n = len(x)
+    u, v = np.zeros(n), np.zeros(n)
+    o = o or t1_opts()
+    st = lib().orc_t1_velocity(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(g, _dp),
+                               C.byref(o), _ptr(u, _dp), _ptr(v, _dp))
+    if st:
+        raise OracleError(f"orc_t1_velocity: {T1_STATUS.get(st, st)}")
+    return u, v
+
+
+def t1_direct(x, y, s, g, icutoff=2, targets=None):
+    """Direct sum of the type-1 pair kernel over every pair at r > 0."""
+    x, y, s, g = _d(x), _d(y), _d(s), _d(g)
+    n = len(x)
+    if targets is None:
+        m, tp = n, None
+    else:
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        m, tp = len(t), _ptr(t, _ip)
+    u, v = np.zeros(m), np.zeros(m)
+    lib().orc_t1_direct(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(g, _dp), int(icutoff),
+                        m, tp, _ptr(u, _dp), _ptr(v, _dp))
+    return u, v
+
+
+def ref_type1_velocity(x, y, s, g, n_s=50, lmax=7, ndp=10, icutoff=2):
+    """The reference's own biotsavart_fmm_2d (rotational part)."""
+    R = ref()
+    x, y, s, g = _d(x), _d(y), _d(s), _d(g)
+    n = len(x)
+    u, v = np.zeros(n), np.zeros(n)
+    _check(R.ref_type1_velocity(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(g, _dp),
+                                int(n_s), int(lmax), int(ndp), int(icutoff), _ptr(u, _dp),
+                                _ptr(v, _dp)), "ref_type1_velocity")
+    return u, v

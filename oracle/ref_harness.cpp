@@ -1,7 +1,9 @@
 // ref_harness.cpp -- C entry points around the reference's own FMM path.
 //
 // TEST INFRASTRUCTURE ONLY.  Compiled (with the reference's TreeCell and
-// fmm2D sources, straight from the reference tree) into
+// fmm2D sources and its type-1 FMM sources biotsavart_fmm_2d.cpp,
+// subroutine_all_fmm_2d.cpp and regularization_function_2d.cpp, straight
+// from the reference tree) into
 // oracle/_ref/libref_fmm2d.so by oracle/Makefile.  It replays the packing of
 // VelocityCalc::velocity_fmm_2d (velocity_calc.cpp:465-481) and calls
 // TreeCell::initializeTree (treeCell.cpp:1043) and fmm2D::calcField
@@ -34,8 +36,10 @@
 #define private public
 #include "src/FMM/fmm2D.hpp"
 #undef private
+#include "src/velocity_calculation/velocity_biot_savart.hpp"
 
 extern "C" void ref_pars_set(int p, int max_par, int max_src, double exp_tree);
+extern "C" void ref_pars_set_type1(int n_s, int lmax, int ndp, int icutoff);
 
 namespace {
 
@@ -245,6 +249,37 @@ int ref_fmm_potential(int n, const double *x, const double *y, const double *q,
         for (int i = 0; i < n; ++i) phi[i] = f[i];
     } catch (const std::exception &) {
         return 6;
+    }
+    return 0;
+}
+
+// The type-1 path: VelocityCalc::velocity_old (velocity_calc.cpp:390-440)
+// copies the particles into two Particle sets (targets and sources, the same
+// data) and calls velocity_biot_savart::biotsavart_fmm_2d with
+// n_inter = 1; u, v are the rotational part (no free stream).
+int ref_type1_velocity(int n, const double *x, const double *y, const double *s,
+                       const double *g, int n_s, int lmax, int ndp, int icutoff, double *u,
+                       double *v) {
+    ref_pars_set_type1(n_s, lmax, ndp, icutoff);
+    Particle pi, pj;
+    pi.num = pj.num = n;
+    pi.x.assign(x, x + n);
+    pi.y.assign(y, y + n);
+    pi.s.assign(s, s + n);
+    pi.gz.assign(g, g + n);
+    pi.u.assign(n, 0.0);
+    pi.v.assign(n, 0.0);
+    pj = pi;
+    try {
+        MuteStdout mute;
+        velocity_biot_savart f;
+        f.biotsavart_fmm_2d(pi, pj, icutoff, n_s, 1, ndp);
+    } catch (const std::exception &) {
+        return 6;
+    }
+    for (int i = 0; i < n; ++i) {
+        u[i] = pi.u[i];
+        v[i] = pi.v[i];
     }
     return 0;
 }

@@ -10,6 +10,7 @@
 #include <stdlib.h>
 
 #include "fmm2d_oracle.h"
+#include "fmm2d_type1_oracle.h"
 
 static uint64_t g_state = 0x243F6A8885A308D3ull;
 
@@ -152,6 +153,41 @@ static int run_field(int n, int clusters, double inactive) {
     CHECK(isfinite(fx) && isfinite(fy), "stage entry points");
     for (int nn = 0; nn < 64; ++nn)
         for (int k = 0; k <= nn + 1; ++k) (void)orc_binom_ref(nn, k), (void)orc_binom_exact(nn, k);
+    /* the type-1 path: velocity (every kernel, both L2L modes), tree and
+     * lists, direct sum */
+    for (int ic = 0; ic <= 3; ++ic) {
+        orc_t1_opts t1;
+        orc_t1_default_opts(&t1);
+        t1.icutoff = ic;
+        t1.n_s = 5 + 10 * ic;
+        t1.lmax = 4 + ic;
+        t1.ndp = 3 + 3 * ic;
+        t1.l2l_mode = ic % 2;
+        CHECK(orc_t1_velocity(n, x, y, s, q, &t1, ex, ey) == 0, "type-1 velocity");
+        CHECK(finite_all(ex, n) && finite_all(ey, n), "type-1 velocity finite");
+    }
+    {
+        orc_t1_opts t1;
+        orc_t1_default_opts(&t1);
+        t1.n_s = 4;
+        orc_t1_tree *tt = NULL;
+        CHECK(orc_t1_build(&tt, n, x, y, &t1) == 0, "type-1 tree");
+        const int nb = orc_t1_nboxes(tt);
+        int *lst = (int *)malloc(sizeof(int) * 4096);
+        int *lv = (int *)malloc(sizeof(int) * (nb + 1));
+        double *bd = (double *)malloc(sizeof(double) * 4 * (nb + 1));
+        orc_t1_boxes(tt, lv, NULL, NULL, NULL, NULL, NULL, bd);
+        for (int b = 0; b < nb; ++b)
+            for (int w = 1; w <= 4; ++w) CHECK(orc_t1_list(tt, b, w, lst, 4096) >= 0, "type-1 list");
+        if (nb > 0) CHECK(orc_t1_box_particles(tt, 0, lst, 4096) > 0, "type-1 box particles");
+        free(lst); free(lv); free(bd);
+        orc_t1_free(tt);
+        double coef[121];
+        int expo[121];
+        orc_t1_l2l_table(10, 1, coef, expo);
+        orc_t1_direct(n, x, y, s, q, 2, n / 3, tidx, ex, ey);
+        CHECK(finite_all(ex, n / 3), "type-1 direct");
+    }
     free(x); free(y); free(q); free(s); free(w); free(a); free(ex); free(ey); free(tidx);
     return 0;
 }

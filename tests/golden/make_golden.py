@@ -5,8 +5,10 @@ Run in a container that has the reference sources (for oracle/_ref) :
     make -C oracle ref && python tests/golden/make_golden.py
 
 Every output is data: inputs and the reference FMM's outputs (fields, tree,
-lists, binomial helper values), computed by the reference's own TreeCell and
-fmm2D sources compiled into oracle/_ref/libref_fmm2d.so.  The resume-state
+lists, binomial helper values, type-1 velocities), computed by the
+reference's own TreeCell / fmm2D sources and its type-1 sources
+(biotsavart_fmm_2d.cpp, subroutine_all_fmm_2d.cpp,
+regularization_function_2d.cpp) compiled into oracle/_ref/libref_fmm2d.so.  The resume-state
 fixture is the reference's input file input/resume_data/particle_state_00500.csv
 converted to npz (same columns, same values).
 """
@@ -95,6 +97,37 @@ def main():
 
     config1()
     potential()
+    type1()
+
+
+# (n_s, lmax, ndp, icutoff) of the type-1 fixtures: the reference's defaults
+# (global.cpp:316-328) first, then deeper and shallower trees, every kernel
+T1_CONFIGS = ((50, 7, 10, 2), (20, 8, 6, 0), (10, 6, 14, 3), (30, 9, 12, 1))
+
+
+def type1():
+    """The reference's type-1 FMM (biotsavart_fmm_2d, rotational velocity) on
+    a uniform field, a clustered field at large coordinates and the
+    reference's own resume state (its x, y, s and gz columns), for each
+    (n_s, lmax, ndp, icutoff) of T1_CONFIGS."""
+    g = np.random.Generator(np.random.PCG64(201))
+    n = 2000
+    uni = dict(x=g.uniform(-1, 1, n), y=g.uniform(-1, 1, n), s=0.02 * (1 + g.uniform(0, 1, n)),
+               g=1e-3 * g.normal(0, 1, n))
+    c = W.random_clusters(6000, seed=202)
+    clu = dict(x=13 * c["x"] + 7, y=13 * c["y"] - 40, s=0.05 * (1 + g.uniform(0, 1, 6000)),
+               g=c["q"])
+    cases = {"type1_uniform_2k": uni, "type1_clusters_6k_far": clu}
+    d = np.load(os.path.join(HERE, "resume_state_00500.npz"))
+    cases["type1_resume_00500"] = dict(x=d["xp"], y=d["yp"], s=d["sp"], g=d["gpz"])
+    for name, f in cases.items():
+        out = dict(x=f["x"], y=f["y"], s=f["s"], g=f["g"],
+                   configs=np.array(T1_CONFIGS, np.int32))
+        for i, (ns, lm, nd, ic) in enumerate(T1_CONFIGS):
+            u, v = O.ref_type1_velocity(f["x"], f["y"], f["s"], f["g"], n_s=ns, lmax=lm, ndp=nd,
+                                        icutoff=ic)
+            out[f"u{i}"], out[f"v{i}"] = u, v
+        save(f"ref_{name}.npz", **out)
 
 
 def potential():
@@ -129,10 +162,10 @@ def config1():
 
 
 if __name__ == "__main__":
-    # `make_golden.py config1` / `potential` writes only those fixtures
-    if sys.argv[1:] in (["config1"], ["potential"]):
+    # `make_golden.py config1` / `potential` / `type1` writes only those fixtures
+    if sys.argv[1:] in (["config1"], ["potential"], ["type1"]):
         if not O.ref_available():
             sys.exit("oracle/_ref/libref_fmm2d.so is missing: run `make -C oracle ref` first")
-        config1() if sys.argv[1] == "config1" else potential()
+        {"config1": config1, "potential": potential, "type1": type1}[sys.argv[1]]()
     else:
         main()
