@@ -32,7 +32,9 @@ typedef enum vpmfmm_status {
     VPMFMM_OUT_OF_MEMORY = 5,
     VPMFMM_INTERNAL = 6,
     VPMFMM_DUPLICATE_POINTS = 7,     /* two particles of one list-1 pair coincide */
-    VPMFMM_HIP_ERROR = 8
+    VPMFMM_HIP_ERROR = 8,
+    VPMFMM_OUTSIDE_BOXES = 9         /* type-1: a particle outside every box of the
+                                        reference's hierarchy (see vpmfmm_velocity_type1) */
 } vpmfmm_status;
 
 typedef enum vpmfmm_precision {
@@ -208,6 +210,53 @@ int vpmfmm_velocity_device(vpmfmm_handle *h, int32_t n, const double *x, const d
 int vpmfmm_velocity_device_noalloc(vpmfmm_handle *h, int32_t n, const double *x,
                                    const double *y, const double *s, const double *omega,
                                    const uint8_t *active, double *u, double *v);
+
+/* Type-1 ("old") FMM options: Pars::n_s, lmax, ndp, icutoff of the
+ * reference's global.cpp:316-328 (defaults 50, 7, 10, 2). */
+typedef struct vpmfmm_type1_opts {
+    int32_t n_s;       /* boxes holding more than n_s particles split, 1..4096        */
+    int32_t lmax;      /* deepest level, 2..16                                        */
+    int32_t ndp;       /* multipole terms (orders 0..ndp-1); the local expansion has
+                          ndp + 1 terms (orders 0..ndp), 1..40                        */
+    int32_t icutoff;   /* near field: 0 singular, 1 high-order algebraic, 2 Gaussian,
+                          3 super-Gaussian (regul_func_2d); pairs at r = 0 add nothing */
+    int32_t l2l_mode;  /* 1 = the L2L shift as the reference runs it: its bico helper
+                          rewrites the k argument in place (subroutine_all_fmm_2d.cpp:
+                          580-583), which changes the binomial and exponent of later
+                          terms (biotsavart_fmm_2d.cpp:299-307) and holds the accuracy
+                          near 4e-4 whatever ndp; 0 = the exact shift                  */
+    int32_t reserved[3];
+} vpmfmm_type1_opts;
+
+void vpmfmm_type1_default_opts(vpmfmm_type1_opts *o);
+
+/* Replaces VelocityCalc::velocity_old (velocity_calc.cpp:390-440,
+ * VELOCITY_2D_TYPE 1) -> velocity_biot_savart::biotsavart_fmm_2d
+ * (biotsavart_fmm_2d.cpp:4-501) with its box hierarchy and lists
+ * (subroutine_all_fmm_2d.cpp:97-556), followed by get_velocity's free stream
+ * (velocity_calc.cpp:226-230): u, v of every particle from its position, size
+ * s (Particle::s, read for icutoff > 0; may be NULL for icutoff 0) and vortex
+ * strength gz (Particle::gz = vorticity s^2), every particle a target and a
+ * source:
+ *   u = u_inf - sum_j f(r, s_ij) (y_i - y_j) gz_j / (2 pi r^2),
+ *   v = v_inf + sum_j f(r, s_ij) (x_i - x_j) gz_j / (2 pi r^2)
+ * over the near-field pairs at r > 0 (f of the icutoff kernel at
+ * s_ij^2 = (s_i^2 + s_j^2) / 4) and the far field by the type-1 expansions.
+ * The boxes, their bounds and every list decision are the reference's (its
+ * rounded box bounds and absolute 1e-15 box tolerance).  The reference drops a
+ * particle that lies on the upper bound of its level-1 boxes (all particles on
+ * one horizontal or vertical line leave the padding at zero) or between a
+ * split box's rounded child bounds; such input returns VPMFMM_OUTSIDE_BOXES.
+ * All particles at one point give u_inf, v_inf (the reference's empty boxes).
+ * FP64 throughout; u_inf, v_inf and device come from the handle's options,
+ * everything else from *o.  Host pointers for vpmfmm_velocity_type1, device
+ * pointers (results on the handle's stream) for the _device form. */
+int vpmfmm_velocity_type1(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t n,
+                          const double *x, const double *y, const double *s, const double *gz,
+                          double *u, double *v);
+int vpmfmm_velocity_type1_device(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t n,
+                                 const double *x, const double *y, const double *s,
+                                 const double *gz, double *u, double *v);
 
 /* Advection schemes of advection::main_advection (main_advection.cpp:6-39). */
 typedef enum vpmfmm_advect_scheme {

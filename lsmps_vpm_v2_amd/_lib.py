@@ -17,7 +17,8 @@ EXPORTS = (
     "vpmfmm_velocity", "vpmfmm_velocity_device", "vpmfmm_velocity_device_noalloc",
     "vpmfmm_advect_device",
     "vpmfmm_step_device", "vpmfmm_direct_device", "vpmfmm_potential",
-    "vpmfmm_potential_device",
+    "vpmfmm_potential_device", "vpmfmm_type1_default_opts", "vpmfmm_velocity_type1",
+    "vpmfmm_velocity_type1_device",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
     "vpmfmm_tree_expansions", "vpmfmm_set_level_orders", "vpmfmm_set_selector",
@@ -28,7 +29,7 @@ EXPORTS = (
 STATUS = {
     0: "OK", 1: "BAD_ARG", 2: "NONFINITE_INPUT", 3: "DEGENERATE_DOMAIN",
     4: "TREE_DEPTH_EXCEEDED", 5: "OUT_OF_MEMORY", 6: "INTERNAL", 7: "DUPLICATE_POINTS",
-    8: "HIP_ERROR",
+    8: "HIP_ERROR", 9: "OUTSIDE_BOXES",
 }
 
 FP64, MIXED = 0, 1
@@ -47,6 +48,14 @@ class Opts(C.Structure):
         ("timing", C.c_int32), ("stage_mask", C.c_int32), ("order_select", C.c_int32),
         ("reselect_every", C.c_int32), ("judge_every", C.c_int32),
         ("icutoff", C.c_int32), ("reserved", C.c_int32 * 3),
+    ]
+
+
+class Type1Opts(C.Structure):
+    """vpmfmm_type1_opts: the reference's type-1 FMM knobs (global.cpp:316-328)."""
+    _fields_ = [
+        ("n_s", C.c_int32), ("lmax", C.c_int32), ("ndp", C.c_int32), ("icutoff", C.c_int32),
+        ("l2l_mode", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -129,6 +138,11 @@ def load() -> C.CDLL:
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in ("vpmfmm_potential", "vpmfmm_potential_device"):
         getattr(L, name).argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]
+    L.vpmfmm_type1_default_opts.argtypes = [C.POINTER(Type1Opts)]
+    L.vpmfmm_type1_default_opts.restype = None
+    for name in ("vpmfmm_velocity_type1", "vpmfmm_velocity_type1_device"):
+        getattr(L, name).argtypes = [_vp, C.POINTER(Type1Opts), C.c_int32, _vp, _vp, _vp, _vp,
+                                     _vp, _vp]
     L.vpmfmm_advect_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_double,
                                        C.c_int32]
     L.vpmfmm_step_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

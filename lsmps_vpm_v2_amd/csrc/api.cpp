@@ -206,6 +206,10 @@ struct vpmfmm_handle_s {
     std::vector<int> h_l1, h_l3, h_l2, h_l4;
     std::vector<uint8_t> h_n2, h_n4;
     std::vector<int64_t> h_id;
+    // type-1 tables on the device (ndp, l2l_mode they were built for) and the
+    // per-level orders of a type-1 tree (list statistics)
+    int t1_nd = -1, t1_mode = -1;
+    uint8_t t1_orders[kMaxDepth + 2] = {};
 };
 
 namespace {
@@ -712,6 +716,126 @@ void run_potential(vpmfmm_handle *h, int n, const double *x, const double *y, co
     report_selection(h, false);
 }
 
+// bico (subroutine_all_fmm_2d.cpp:560-591): nCk in double, formed as the
+// reference forms it; k is rewritten to n - k where the reference rewrites it
+// (its k is a reference parameter).
+double t1_bico(int n, int &k) {
+    if (k < 0 || k > n || n < 0) return 0.0;
+    if (k == 0 || k == n || n == 0) return 1.0;
+    if (k == 1 || k == n - 1) return (double)n;
+    if (k > n / 2) k = n - k;
+    const int n_k = n - k;
+    double c = 1.0;
+    for (int i = 1; i <= k; ++i) c = c * ((double)(n_k + i) / (double)i);
+    return c;
+}
+
+int t1_opts_check(const vpmfmm_type1_opts &t, std::string &msg) {
+    if (t.n_s < 1 || t.n_s > kMaxThreshold) msg = "n_s must be in [1, 4096]";
+    else if (t.lmax < 2 || t.lmax > kMaxDepth) msg = "lmax must be in [2, 16]";
+    else if (t.ndp < 1 || t.ndp > kMaxP) msg = "ndp must be in [1, 40]";
+    else if (t.icutoff < 0 || t.icutoff > 3) msg = "icutoff must be in [0, 3]";
+    else if (t.l2l_mode != 0 && t.l2l_mode != 1) msg = "l2l_mode must be 0 or 1";
+    else return VPMFMM_OK;
+    return VPMFMM_BAD_ARG;
+}
+
+// Type-1 tables: M2M at order ndp - 1 (exact binomials), M2L
+// tab[l][j] = (-1)^(j+1) C(l + j, j) (biotsavart_fmm_2d.cpp:222-231), and
+// the L2L shift as run (biotsavart_fmm_2d.cpp:299-307: l is handed to bico by
+// reference, so after a rewrite the sweep goes on with the new value in both
+// the binomial and the exponent) or exact.
+void ensure_t1_tables(vpmfmm_handle *h, int nd, int mode) {
+    if (h->t1_nd == nd && h->t1_mode == mode) return;
+    Ctx &c = h->ctx;
+    std::vector<double> m2l, m2m, l2l;
+    fill_tables(nd - 1, 0, m2l, m2m, l2l);
+    const int nl = nd + 1;
+    std::vector<double> tm2l((size_t)nl * nd), coef((size_t)nl * nl, 0.0);
+    std::vector<int> expo((size_t)nl * nl, 0);
+    for (int l = 0; l <= nd; ++l)
+        for (int j = 0; j < nd; ++j) tm2l[(size_t)l * nd + j] = ((j & 1) ? 1.0 : -1.0) * exact_binom(l + j, j);
+    for (int l = 0; l <= nd; ++l) {
+        int ll = l;
+        for (int kp = l; kp <= nd; ++kp) {
+            double cb;
+            if (mode == 1) {
+                cb = t1_bico(kp, ll);
+            } else {
+                cb = exact_binom(kp, l);
+            }
+            coef[(size_t)l * nl + kp] = cb;
+            expo[(size_t)l * nl + kp] = kp - ll;
+        }
+    }
+    auto up = [&](DBuf &b, const void *src, size_t bytes) {
+        VPM_HIP(hipMemcpyAsync(b.get(bytes), src, bytes, hipMemcpyHostToDevice, c.stream));
+    };
+    up(c.t1_m2m, m2m.data(), m2m.size() * sizeof(double));
+    up(c.t1_m2l, tm2l.data(), tm2l.size() * sizeof(double));
+    up(c.t1_l2l_c, coef.data(), coef.size() * sizeof(double));
+    up(c.t1_l2l_e, expo.data(), expo.size() * sizeof(int));
+    VPM_HIP(hipStreamSynchronize(c.stream));
+    h->t1_nd = nd;
+    h->t1_mode = mode;
+}
+
+// The type-1 velocity (VelocityCalc::velocity_old -> biotsavart_fmm_2d) on
+// device pointers: the type-1 tree (every particle active), its lists with
+// the reference's box tests, the FP64 type-1 expansions.
+void run_type1(vpmfmm_handle *h, const vpmfmm_type1_opts &t, int n, const double *x,
+               const double *y, const double *sz, const double *g, double *u, double *v) {
+    std::string msg;
+    if (t1_opts_check(t, msg)) throw Error(kBadArg, msg);
+    if (n < 1) throw Error(kBadArg, "n must be >= 1");
+    if (!x || !y || !g || !u || !v || (t.icutoff > 0 && !sz)) throw Error(kBadArg, "null pointer");
+    VPM_HIP(hipSetDevice(h->o.device));
+    Ctx &c = h->ctx;
+    h->insp = false;
+    h->exp_p = -1;
+    h->last_sel = vpmfmm_selection{};
+    c.icut = t.icutoff;
+    uint8_t *ones = c.t1_ones.as<uint8_t>(n);
+    VPM_HIP(hipMemsetAsync(ones, 1, (size_t)n, c.stream));
+    Sources src;
+    src.q = g;
+    src.s = t.icutoff > 0 ? sz : nullptr;
+    src.qscale = -1.0 / (2.0 * M_PI);  // u = Ey, v = -Ex of q = -gz / (2 pi)
+    const TreeParams tp{t.n_s, t.n_s, 1.0, 1, t.lmax, true};
+    mark(h, 0);
+    try {
+        tree_build(c, n, x, y, src, ones, tp);
+    } catch (const Error &e) {
+        if (e.code != kDegenerate) throw;
+        // every particle at one point: the reference's quadrants are empty
+        // (db = 0), so the rotational part is zero
+        c.info.n_cells = 0;
+        fill_pair(u, v, n, h->o.u_inf, h->o.v_inf, c.stream);
+        VPM_HIP(hipStreamSynchronize(c.stream));
+        h->st = vpmfmm_stats{};
+        h->st.n = h->st.n_active = n;
+        return;
+    }
+    for (int L = 0; L < kMaxDepth + 2; ++L) h->t1_orders[L] = (uint8_t)t.ndp;
+    uint8_t *slot = c.orders_buf.as<uint8_t>(64) + 32;
+    VPM_HIP(hipMemcpyAsync(slot, h->t1_orders, sizeof(h->t1_orders), hipMemcpyHostToDevice,
+                           c.stream));
+    h->fix_valid = false;  // the fixed-order slot now holds ndp
+    c.orders = slot;
+    mark(h, 1);
+    h->concurrent_marks = false;
+    tree_lists(c);
+    mark(h, 2);
+    ensure_t1_tables(h, t.ndp, t.l2l_mode);
+    fmm_t1_run(c, t.ndp, u, v, h->o.u_inf, h->o.v_inf, FmmMarks{h->o.timing ? h->ev : nullptr});
+    h->sel_fresh = false;
+    finish_stats(h, true);
+    vpmfmm_selection &ls = h->last_sel;
+    ls.pred_err = ls.pred_ms = ls.judge_err = -1.f;
+    for (int L = 0; L <= kMaxDepth; ++L) ls.p[L] = t.ndp;
+    ls.threshold = t.n_s;
+}
+
 // Host-pointer staging.
 struct Staged {
     const double *x, *y, *q;
@@ -1001,6 +1125,49 @@ int vpmfmm_velocity(vpmfmm_handle *h, int32_t n, const double *x, const double *
         run(h, n, dx, dy, Sources{nullptr, ds, dw}, da, ds, dw, true);
         VPM_HIP(hipMemcpyAsync(u, ds, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipMemcpyAsync(v, dw, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+void vpmfmm_type1_default_opts(vpmfmm_type1_opts *o) {
+    if (!o) return;
+    *o = vpmfmm_type1_opts{};
+    o->n_s = 50;      // global.cpp:326
+    o->lmax = 7;      // global.cpp:316
+    o->ndp = 10;      // global.cpp:327
+    o->icutoff = 2;   // global.cpp:328
+    o->l2l_mode = 1;  // the reference as it runs
+}
+
+int vpmfmm_velocity_type1_device(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t n,
+                                 const double *x, const double *y, const double *s,
+                                 const double *gz, double *u, double *v) {
+    return guarded(h, [&] {
+        if (!o) throw Error(kBadArg, "null options");
+        run_type1(h, *o, n, x, y, s, gz, u, v);
+    });
+}
+
+int vpmfmm_velocity_type1(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t n,
+                          const double *x, const double *y, const double *s, const double *gz,
+                          double *u, double *v) {
+    return guarded(h, [&] {
+        if (!o) throw Error(kBadArg, "null options");
+        if (n < 1 || !x || !y || !gz || !u || !v || (o->icutoff > 0 && !s))
+            throw Error(kBadArg, "bad size or null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        Ctx &c = h->ctx;
+        double *dx = c.x_in.as<double>(n), *dy = c.y_in.as<double>(n);
+        double *ds = c.ex_out.as<double>(n), *dg = c.ey_out.as<double>(n);
+        VPM_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        if (o->icutoff > 0)
+            VPM_HIP(hipMemcpyAsync(ds, s, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        VPM_HIP(hipMemcpyAsync(dg, gz, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        // the outputs reuse the s / gz staging (read by the tree build first)
+        run_type1(h, *o, n, dx, dy, o->icutoff > 0 ? ds : nullptr, dg, ds, dg);
+        VPM_HIP(hipMemcpyAsync(u, ds, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipMemcpyAsync(v, dg, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipStreamSynchronize(c.stream));
     });
 }

@@ -29,6 +29,7 @@ enum Status : int {
     kInternal = 6,
     kDuplicate = 7,
     kHipError = 8,
+    kOutsideBoxes = 9,
 };
 
 struct Error : std::runtime_error {
@@ -55,6 +56,7 @@ enum DevFlag : unsigned {
     kFlagDuplicate = 4u,
     kFlagListOverflow = 8u,
     kFlagRange = 16u,
+    kFlagT1Lost = 32u,  // type-1 tree: a particle outside every box of its level
 };
 
 }  // namespace vpm

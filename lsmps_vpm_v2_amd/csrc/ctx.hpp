@@ -98,11 +98,15 @@ struct DBuf {
     ~DBuf() { release(); }
 };
 
-// Tree-build parameters (reference knobs plus the level extensions).
+// Tree-build parameters (reference knobs plus the level extensions).  With
+// type1 the root and the particle keys are those of the type-1 box hierarchy
+// (hierarchy_mesh, subroutine_all_fmm_2d.cpp:97-287): exp_tree is unused and
+// max_level is its lmax.
 struct TreeParams {
     int max_par, max_src;
     double exp_tree;
     int min_level, max_level;
+    bool type1 = false;
 };
 
 // Host copy of what the tree build produced.
@@ -133,6 +137,11 @@ struct Ctx {
     DBuf x_in, y_in, q_in, a_in, ex_out, ey_out;  // staging for host-pointer calls
     DBuf key_a, key_b, val_a, val_b;
     DBuf xs, ys, qs, acts;
+    // type-1 tree (TreeParams::type1): box bounds per cell as the reference
+    // rounds them (xmin, xmax, ymin, ymax); the list kernels then apply the
+    // reference's box tests to them
+    bool t1 = false;
+    DBuf c_bnd;
     // near-field kernel of this call (0 singular, 1..3 the regularized
     // kernels of regul_func_2d) and, when regularized, the sorted sizes
     int icut = 0;
@@ -157,6 +166,10 @@ struct Ctx {
 
     // expansions
     DBuf ak, bk, tab_m2l, tab_m2m, tab_l2l;
+    // type-1 tables: M2M (order ndp - 1, as tab_m2m), M2L (ndp + 1) x ndp,
+    // L2L coefficients and exponents (ndp + 1)^2, and the all-active flags
+    // of a type-1 call
+    DBuf t1_m2m, t1_m2l, t1_l2l_c, t1_l2l_e, t1_ones;
 
     // mixed path: FP32 particle offsets and sources, tables, tiles
     DBuf oxy, qf;
@@ -186,6 +199,7 @@ struct Sources {
     const double *q = nullptr;
     const double *s = nullptr;
     const double *omega = nullptr;
+    double qscale = 1.0;  // the build forms q[i] * qscale from q
 };
 
 // Stage entry points (tree.hip).  Inputs are device pointers.
@@ -229,11 +243,22 @@ void fmm64_run(Ctx &c, int p, int list4_mode, int stage_mask, double *out_a, dou
                bool velocity, double u_inf, double v_inf, const FmmMarks &mk,
                bool potential = false);
 
+// Type-1 FMM (fmm64.hip, biotsavart_fmm_2d.cpp:93-446) on a type-1 tree:
+// multipoles of orders 0..ndp-1 (P2M, M2M), locals of orders 0..ndp from
+// M2L (list 2) and P2L (list 4), the L2L shift of the tables (coefficient,
+// exponent) in t1_l2l_c / t1_l2l_e, evaluation with the near-field kernel
+// c.icut; u = Ey + u_inf, v = -Ex + v_inf.
+void fmm_t1_run(Ctx &c, int ndp, double *u, double *v, double u_inf, double v_inf,
+                const FmmMarks &mk);
+
 // Particle advection sub-step (advect.hip): Euler x += u dt, or the
 // reference's velocity-scaling "rk2" update.
 constexpr int kAdvectEuler = 0, kAdvectRk2 = 1;
 void advect(int n, double *x, double *y, const double *u, const double *v, double dt, int scheme,
             hipStream_t s);
+
+// a[i] = va, b[i] = vb for i < n.
+void fill_pair(double *a, double *b, int n, double va, double vb, hipStream_t s);
 
 // FP64 direct sum on the GPU (accuracy reference, not part of the FMM).
 void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,

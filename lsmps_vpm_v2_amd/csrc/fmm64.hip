@@ -31,6 +31,7 @@ struct cd {
 };
 __device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ __forceinline__ cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
 __device__ __forceinline__ cd cscale(cd a, double s) { return {a.x * s, a.y * s}; }
 __device__ __forceinline__ cd cinv(cd a) {
     const double r = 1.0 / (a.x * a.x + a.y * a.y);
@@ -360,7 +361,11 @@ __device__ __forceinline__ double regul64(double rho2) {
 // (subroutine_all_fmm_2d.cpp:594-627): each pair term times regul64 at
 // sij^2 = (si^2 + sj^2) / 4 (ss: the sorted sizes); pairs at r = 0 add
 // nothing.
-template <int ICUT>
+// T1: the type-1 FMM (biotsavart_fmm_2d.cpp:321-419): pairs at r = 0 add
+// nothing whatever the kernel; the multipoles hold orders 0..P-1 (stride P)
+// and the local expansions orders 0..P (stride P + 1) in the Cauchy form
+// F = sum_l b_l (z - z0)^l.
+template <int ICUT, bool T1 = false>
 __global__ void __launch_bounds__(kWave *kEvalWaves)
     k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
              const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
@@ -411,8 +416,13 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 const double4 sv = stage[w][r];
                 const double dx = xi - sv.x, dy = yi - sv.y;
                 const double r2 = dx * dx + dy * dy;
-                if (ICUT == 0 && valid && (uint32_t)sv.w != i) {
+                if (ICUT == 0 && !T1 && valid && (uint32_t)sv.w != i) {
                     dup |= r2 == 0.0;
+                    const double qr = sv.z * rcp64(r2);
+                    ex += qr * dx;
+                    ey += qr * dy;
+                }
+                if (ICUT == 0 && T1 && valid && r2 > 0.0) {
                     const double qr = sv.z * rcp64(r2);
                     ex += qr * dx;
                     ey += qr * dy;
@@ -428,12 +438,13 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             __builtin_amdgcn_wave_barrier();
         }
         // M2P over list 3, split across groups
+        const int PM = T1 ? P - 1 : P;  // highest multipole order
         for (uint32_t v = a3 + g; (mask & 2) && v < z3; v += G) {
             const int sc = l3[v];
             const cd wv = cinv(cd{xi - c_cx[sc], yi - c_cy[sc]});
-            const size_t sbk = (size_t)sc * P1;
-            cd f = ld(ak, sbk + P);
-            for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, wv), ld(ak, sbk + p));
+            const size_t sbk = (size_t)sc * (PM + 1);
+            cd f = ld(ak, sbk + PM);
+            for (int p = PM - 1; p >= 0; --p) f = cadd(cmul(f, wv), ld(ak, sbk + p));
             f = cmul(f, wv);
             if (valid) {
                 ex += f.x;
@@ -449,8 +460,15 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 // L2P
                 const cd z = {xi - lcx, yi - lcy};
                 const size_t lb = (size_t)j * P1;
-                cd f = cscale(ld(bk, lb + P), (double)P);
-                for (int p = P - 1; p >= 1; --p) f = cadd(cmul(f, z), cscale(ld(bk, lb + p), (double)p));
+                cd f;
+                if (T1) {
+                    f = ld(bk, lb + P);
+                    for (int p = P - 1; p >= 0; --p) f = cadd(cmul(f, z), ld(bk, lb + p));
+                } else {
+                    f = cscale(ld(bk, lb + P), (double)P);
+                    for (int p = P - 1; p >= 1; --p)
+                        f = cadd(cmul(f, z), cscale(ld(bk, lb + p), (double)p));
+                }
                 ex += f.x;
                 ey -= f.y;
             }
@@ -550,6 +568,76 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
         }
     }
     if (dup) atomicOr(flags, kFlagDuplicate);
+}
+
+// Type-1 M2L and P2L (biotsavart_fmm_2d.cpp:199-271) for every cell of level
+// >= 1, thread per (cell, order l = 0..ND), with multipoles a_0..a_{ND-1}
+// (stride ND) of strengths q = -g / (2 pi):
+//   b_l = sum_{list 2} w^(l+1) sum_j tab[l][j] a_j w^j,   w = 1 / (zs - zt),
+//         tab[l][j] = (-1)^(j+1) C(l + j, j)              (Horner in w)
+//       - sum_{list 4} sum_i q_i w_i^(l+1),               w_i = 1 / (z_i - zt)
+// (the reference's list 4 is always P2L from the leaf's particles).  Cells of
+// level 1 have no lists (their locals are zero, as in the reference).
+__global__ void __launch_bounds__(256)
+    k_m2l_t1(int j0, int nc, int ND, const double *tab, const uint8_t *c_level, const int *l2,
+             const uint8_t *n2, const int *l4, const uint8_t *n4, const double *c_cx,
+             const double *c_cy, const uint32_t *c_begin, const uint32_t *c_end, const double *xs,
+             const double *ys, const double *qs, const double2 *ak, double2 *bk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NL = ND + 1;
+    if (idx >= (long long)(nc - j0) * NL) return;
+    const int j = j0 + (int)(idx / NL), l = (int)(idx % NL);
+    cd acc = {0.0, 0.0};
+    if (c_level[j] >= 2) {
+        const double cx = c_cx[j], cy = c_cy[j];
+        const double *row = tab + (size_t)l * ND;
+        const int m2 = n2[j], m4 = n4[j];
+        for (int u = 0; u < m2; ++u) {
+            const int sidx = l2[27 * j + u];
+            const cd w = cinv(cd{c_cx[sidx] - cx, c_cy[sidx] - cy});
+            const size_t b = (size_t)sidx * ND;
+            cd sm = cscale(ld(ak, b + ND - 1), row[ND - 1]);
+            for (int k = ND - 2; k >= 0; --k) sm = cadd(cmul(sm, w), cscale(ld(ak, b + k), row[k]));
+            acc = cadd(acc, cmul(sm, cpowi(w, l + 1)));
+        }
+        for (int u = 0; u < m4; ++u) {
+            const int sidx = l4[8 * j + u];
+            for (uint32_t i = c_begin[sidx]; i < c_end[sidx]; ++i) {
+                const cd w = cinv(cd{xs[i] - cx, ys[i] - cy});
+                acc = csub(acc, cscale(cpowi(w, l + 1), qs[i]));
+            }
+        }
+    }
+    st(bk, (size_t)j * NL + l, acc);
+}
+
+// Type-1 L2L (biotsavart_fmm_2d.cpp:283-312) for the cells of one level
+// (>= 3; level-1 locals are zero): thread per (cell, l),
+//   b_l += sum_{k = l..ND} coef[l][k] d^expo[l][k] b_k(parent),  d = zc - zp,
+// with the table of the shift as the reference runs it (or the exact one).
+// Along a row the exponent rises by one per term except at most once, where
+// it is formed afresh.
+__global__ void __launch_bounds__(256)
+    k_l2l_t1(int j0, int j1, int ND, const double *coef, const int *expo, const int *c_parent,
+             const double *c_cx, const double *c_cy, double2 *bk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NL = ND + 1;
+    if (idx >= (long long)(j1 - j0) * NL) return;
+    const int j = j0 + (int)(idx / NL), l = (int)(idx % NL);
+    const int par = c_parent[j];
+    const cd d = {c_cx[j] - c_cx[par], c_cy[j] - c_cy[par]};
+    const size_t b = (size_t)par * NL;
+    cd s = {0.0, 0.0}, pw = {1.0, 0.0};
+    int e_prev = 0;
+    for (int k = l; k <= ND; ++k) {
+        const int e = expo[l * NL + k];
+        if (e == e_prev + 1) pw = cmul(pw, d);
+        else if (e != e_prev) pw = cpowi(d, e);
+        e_prev = e;
+        s = cadd(s, cscale(cmul(pw, ld(bk, b + k)), coef[l * NL + k]));
+    }
+    const size_t o = (size_t)j * NL + l;
+    st(bk, o, cadd(ld(bk, o), s));
 }
 
 // FP64 direct sum with LDS source tiles (accuracy reference only).
@@ -670,6 +758,66 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     VPM_LAUNCH_CHECK();
 }
 
+void fmm_t1_run(Ctx &c, int ND, double *u, double *v, double u_inf, double v_inf,
+                const FmmMarks &mk) {
+    hipStream_t s = c.stream;
+    const TreeInfo &ti = c.info;
+    const int nc = ti.n_cells, nl = ti.n_leaves, NL = ND + 1, PM = ND - 1;
+    double2 *ak = c.ak.as<double2>((size_t)nc * ND), *bk = c.bk.as<double2>((size_t)nc * NL);
+    VPM_HIP(hipMemsetAsync(ak, 0, sizeof(double2) * (size_t)nc * ND, s));
+    const uint8_t *c_flags = (const uint8_t *)c.c_flags.p, *c_level = (const uint8_t *)c.c_level.p;
+    const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
+    const double *c_cx = (const double *)c.c_cx.p, *c_cy = (const double *)c.c_cy.p;
+    const int *c_child = (const int *)c.c_child.p, *c_parent = (const int *)c.c_parent.p;
+    const double *xs = (const double *)c.xs.p, *ys = (const double *)c.ys.p,
+                 *qs = (const double *)c.qs.p;
+    const uint8_t *as = (const uint8_t *)c.acts.p;
+    const uint32_t *leaf_idx = (const uint32_t *)c.leaf_idx.p;
+
+    // upward: P2M and M2M at order PM = ND - 1 (biotsavart_fmm_2d.cpp:123-188;
+    // the exact shift C(p, k), the reference's bico values to rounding)
+    k_p2m64<<<div_up(nl, kP2m64Waves * kP2mLeaves), kWave * kP2m64Waves, 0, s>>>(
+        leaf_idx, nl, PM, c_flags, c_begin, c_end, c_cx, c_cy, xs, ys, qs, as, ak);
+    for (int L = ti.max_level - 1; L >= 1; --L) {
+        const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
+        if (j1 <= j0) continue;
+        k_m2m64<<<div_up((long long)(j1 - j0) * ND, 256), 256, 0, s>>>(
+            j0, j1, PM, (const double *)c.t1_m2m.p, c_flags, c_cx, c_cy, c_child, ak);
+    }
+    mk.mark(3, s);
+    // M2L and P2L: every cell of level >= 1 (all of bk written)
+    const int j1 = (int)ti.lev_off[1];
+    if (nc > j1)
+        k_m2l_t1<<<div_up((long long)(nc - j1) * NL, 256), 256, 0, s>>>(
+            j1, nc, ND, (const double *)c.t1_m2l.p, c_level, (const int *)c.l2.p,
+            (const uint8_t *)c.n2.p, (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy,
+            c_begin, c_end, xs, ys, qs, ak, bk);
+    mk.mark(4, s);
+    for (int L = 3; L <= ti.max_level; ++L) {
+        const int j0 = (int)ti.lev_off[L], jz = (int)ti.lev_off[L + 1];
+        if (jz <= j0) continue;
+        k_l2l_t1<<<div_up((long long)(jz - j0) * NL, 256), 256, 0, s>>>(
+            j0, jz, ND, (const double *)c.t1_l2l_c.p, (const int *)c.t1_l2l_e.p, c_parent, c_cx,
+            c_cy, bk);
+    }
+    mk.mark(5, s);
+    const double *ss = (const double *)c.ss.p;
+    auto eval = [&](auto kern) {
+        kern<<<div_up(nl, kEvalWaves), kWave * kEvalWaves, 0, s>>>(
+            leaf_idx, nl, ND, c_begin, c_end, c_cx, c_cy, (const uint32_t *)c.off1.p,
+            (const int *)c.l1.p, (const uint32_t *)c.off3.p, (const int *)c.l3.p, xs, ys, qs, ss,
+            c.perm_sorted, ak, bk, 1, u_inf, v_inf, 7, u, v, &c.hdr_d->flags);
+    };
+    switch (c.icut) {
+        case 1: eval(k_eval64<1, true>); break;
+        case 2: eval(k_eval64<2, true>); break;
+        case 3: eval(k_eval64<3, true>); break;
+        default: eval(k_eval64<0, true>); break;
+    }
+    mk.mark(6, s);
+    VPM_LAUNCH_CHECK();
+}
+
 // Partial sums of |(a, b) - (ja, jb)|^2 and |(ja - a0, jb - b0)|^2 per block.
 __global__ void __launch_bounds__(256)
     k_diff_norms(int n, const double *a, const double *b, const double *ja, const double *jb,
@@ -716,6 +864,20 @@ double rel_diff64(int n, const double *a, const double *b, const double *ja, con
         den += h[2 * k + 1];
     }
     return den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
+}
+
+__global__ void __launch_bounds__(256)
+    k_fill_pair(double *a, double *b, int n, double va, double vb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = va;
+        b[i] = vb;
+    }
+}
+
+void fill_pair(double *a, double *b, int n, double va, double vb, hipStream_t s) {
+    k_fill_pair<<<div_up(n, 256), 256, 0, s>>>(a, b, n, va, vb);
+    VPM_LAUNCH_CHECK();
 }
 
 void direct64(int n, const double *x, const double *y, const double *q, const uint8_t *src_mask,

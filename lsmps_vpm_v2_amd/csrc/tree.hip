@@ -125,6 +125,33 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // 2-3. root geometry and Morton keys, sort, gather
 // ---------------------------------------------------------------------------
+// Type-1 key bits of one coordinate (hierarchy_mesh and amount_inbox_new,
+// subroutine_all_fmm_2d.cpp:175-263, 635-678): the level-1 box is
+// [lo + b db, lo + (b + 1) db); below it a box [l, h) has the children
+// [l + b d, l + (b + 1) d), d = (h - l) / 2, each half-open, so a coordinate
+// takes child 0 below l + d and child 1 below l + 2 d.  The bounds are formed
+// by the reference's own additions (contraction off), so the path is the
+// reference's to the last bit.  Returns false if the coordinate falls outside
+// the level-1 boxes or past the rounded upper bound of a split box's children.
+__device__ __forceinline__ bool t1_path(double x, double lo, double db, int lmax,
+                                        uint32_t &bits) {
+#pragma clang fp contract(off)
+    uint32_t b = x < lo + 1.0 * db ? 0u : 1u;
+    double l = lo + (double)b * db, h = lo + (double)(b + 1) * db;
+    bool ok = x >= lo && x < h;
+    bits = b;
+    for (int lv = 2; lv <= lmax; ++lv) {
+        const double d = (h - l) / 2.0;
+        b = x < l + 1.0 * d ? 0u : 1u;
+        const double nl = l + (double)b * d, nh = l + (double)(b + 1) * d;
+        ok = ok && x < nh;
+        l = nl;
+        h = nh;
+        bits = (bits << 1) | b;
+    }
+    return ok;
+}
+
 // Root geometry, treeCell.cpp:1079-1092: R = max side * (1 + expTree/100),
 // pivot = (max + min - R)/2, from the bounding-box partials; every block
 // reduces them itself (a few KB, read from L2), so no launch separates the
@@ -132,9 +159,15 @@ __global__ void __launch_bounds__(256)
 // header.  Then the Morton keys of one radix tile (kRsTile particles, the
 // radix sort's element order) and the tile's counts of the first sort digit
 // (key >> shift) & 255, so the sort's first pass needs no counting launch.
+// With T1, the type-1 root (hierarchy_mesh, subroutine_all_fmm_2d.cpp:
+// 120-170: the box padded by dr 1e-5 on each side, dr = shorter side /
+// 2^lmax, then squared about its centre on the longer side; db = half that
+// side) recorded as pivot = (xmin, ymin), root_len = 2 db, and the keys of
+// the type-1 paths (t1_path) in the same bit layout.
+template <bool T1>
 __global__ void __launch_bounds__(kRsBlock)
     k_keys(const double *x, const double *y, int n, const double *part, int nb,
-           double expansion, DevHdr *hdr, uint32_t *key, uint32_t *val, int shift,
+           double expansion, int lmax, DevHdr *hdr, uint32_t *key, uint32_t *val, int shift,
            uint32_t *table, int tiles) {
 #pragma clang fp contract(off)
     __shared__ double s[4][kRsBlock / kWave];
@@ -175,16 +208,41 @@ __global__ void __launch_bounds__(kRsBlock)
             mn[1] = fmin(mn[1], s[2][u]);
             mx[1] = fmax(mx[1], s[3][u]);
         }
-        double R = 0.0;
-        for (int d = 0; d < 2; ++d) {
-            double side = fabs(mx[d] - mn[d]);
-            if (side > R) R = side;
+        if (T1) {
+            double xmin = mn[0], xmax = mx[0], ymin = mn[1], ymax = mx[1];
+            const double tlout = 1.0e-5;
+            const double sx = fabs(xmax - xmin), sy = fabs(ymax - ymin);
+            const double dr = (sy < sx ? sy : sx) / (double)(1u << lmax);
+            xmin = xmin - dr * tlout;
+            xmax = xmax + dr * tlout;
+            ymin = ymin - dr * tlout;
+            ymax = ymax + dr * tlout;
+            const double rdx = fabs(xmax - xmin), rdy = fabs(ymax - ymin);
+            double db;
+            if (rdy >= rdx) {
+                const double cent = (xmin + xmax) / 2.0;
+                xmin = cent - rdy / 2.0;
+                db = rdy / 2.0;
+            } else {
+                const double cent = (ymin + ymax) / 2.0;
+                ymin = cent - rdx / 2.0;
+                db = rdx / 2.0;
+            }
+            geo[0] = 2.0 * db;
+            geo[1] = xmin;
+            geo[2] = ymin;
+        } else {
+            double R = 0.0;
+            for (int d = 0; d < 2; ++d) {
+                double side = fabs(mx[d] - mn[d]);
+                if (side > R) R = side;
+            }
+            R *= expansion;
+            geo[0] = R;
+            for (int d = 0; d < 2; ++d) geo[1 + d] = (mx[d] + mn[d] - R) / 2.0;
         }
-        R *= expansion;
-        geo[0] = R;
-        for (int d = 0; d < 2; ++d) geo[1 + d] = (mx[d] + mn[d] - R) / 2.0;
         if (blockIdx.x == 0) {
-            hdr->root_len = R;
+            hdr->root_len = geo[0];
             for (int d = 0; d < 2; ++d) {
                 hdr->mn[d] = mn[d];
                 hdr->mx[d] = mx[d];
@@ -197,31 +255,42 @@ __global__ void __launch_bounds__(kRsBlock)
         unsigned f = 0;
         if (sbad) f |= kFlagNonFinite;
         if (!(geo[0] > 0.0)) f |= kFlagRange;
-        hdr->flags |= f;
+        atomicOr(&hdr->flags, f);  // other blocks may raise kFlagT1Lost meanwhile
         hdr->leaf_lvl_min = kMaxDepth + 1;
         hdr->leaf_lvl_max = 0;
     }
     const double R = geo[0], px = geo[1], py = geo[2];
     const int base = blockIdx.x * kRsTile;
+    bool lost = false;
 #pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
         const int i = base + j * kRsBlock + threadIdx.x;
         if (i >= n) break;
-        const double tx = (x[i] - px) / R, ty = (y[i] - py) / R;
-        const double sx = tx * 65536.0, sy = ty * 65536.0;
         uint32_t ux = 0, uy = 0;
-        if (sx >= 0.0 && sx < 65536.0) ux = (uint32_t)sx;
-        if (sy >= 0.0 && sy < 65536.0) uy = (uint32_t)sy;
+        if (T1) {
+            const double db = R / 2.0;
+            const bool okx = t1_path(x[i], px, db, lmax, ux);
+            const bool oky = t1_path(y[i], py, db, lmax, uy);
+            lost |= !(okx && oky);
+            ux <<= 16 - lmax;
+            uy <<= 16 - lmax;
+        } else {
+            const double tx = (x[i] - px) / R, ty = (y[i] - py) / R;
+            const double sx = tx * 65536.0, sy = ty * 65536.0;
+            if (sx >= 0.0 && sx < 65536.0) ux = (uint32_t)sx;
+            if (sy >= 0.0 && sy < 65536.0) uy = (uint32_t)sy;
+        }
         const uint32_t k = part1by1(ux) | (part1by1(uy) << 1);
         key[i] = k;
         val[i] = (uint32_t)i;
         atomicAdd(&h[(k >> shift) & 255u], 1u);
     }
+    if (T1 && lost) atomicOr(&hdr->flags, kFlagT1Lost);
     __syncthreads();
     table[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Particles in Morton order.  The source strength is q[o], or, for the
+// Particles in Morton order.  The source strength is q[o] qscale, or, for the
 // velocity entry points (q == nullptr), -omega s^2 / (2 pi) formed here in the
 // operation order of velocity_calc.cpp:477 (pow(s, 2) is the exact square).
 // Also the level lcps of the sorted keys: d[i] = lcp(key[i-1], key[i])
@@ -230,7 +299,7 @@ __global__ void __launch_bounds__(256)
     k_gather(const uint32_t *perm, const double *x, const double *y, const double *q,
              const double *ssz, const double *omega, const uint8_t *a, int n, double *xs,
              double *ys, double *qs, uint8_t *as, uint32_t *bs_act, const uint32_t *key, int W,
-             uint8_t *d, uint8_t *w, double *ss, DevHdr *hdr_w) {
+             uint8_t *d, uint8_t *w, double *ss, double qscale, DevHdr *hdr_w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t act = 0;
     if (i < n) {
@@ -242,7 +311,7 @@ __global__ void __launch_bounds__(256)
         xs[i] = x[o];
         ys[i] = y[o];
         if (q) {
-            qs[i] = q[o];
+            qs[i] = q[o] * qscale;
         } else {
             const double sz = ssz[o];
             qs[i] = -omega[o] * (sz * sz) / (2 * M_PI);
@@ -692,9 +761,69 @@ __global__ void __launch_bounds__(256)
     if (par >= 0) c_child[4 * par + qx + 2 * qy] = j;
 }
 
+// Type-1 box bounds of every cell (hierarchy_mesh, subroutine_all_fmm_2d.cpp:
+// 179-182, 229-240): from the root (pivot, side root_len = 2 db) down the
+// cell's path with the reference's additions, in the order the reference
+// forms them (t1_path); the centre (max + min) / 2 (:201-202, 258-259)
+// replaces the pivot-based centre of cell_attrs.
+__global__ void __launch_bounds__(256)
+    k_t1_bounds(const uint8_t *c_level, const uint32_t *c_key, int nc, const DevHdr *hdr,
+                double4 *bnd, double *c_cx, double *c_cy) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nc) return;
+    const int L = c_level[j];
+    const double db = hdr->root_len / 2.0;
+    double lo[2] = {hdr->pivot[0], hdr->pivot[1]}, hi[2] = {lo[0] + 2.0 * db, lo[1] + 2.0 * db};
+    if (L > 0) {
+        const uint32_t pre = c_key[j];
+        const uint32_t ix[2] = {compact1by1(pre), compact1by1(pre >> 1)};
+        for (int d = 0; d < 2; ++d) {
+            const double o = lo[d];
+            uint32_t b = (ix[d] >> (L - 1)) & 1u;
+            double l = o + (double)b * db, h = o + (double)(b + 1) * db;
+            for (int lv = 2; lv <= L; ++lv) {
+                b = (ix[d] >> (L - lv)) & 1u;
+                const double dd = (h - l) / 2.0;
+                const double nl = l + (double)b * dd, nh = l + (double)(b + 1) * dd;
+                l = nl;
+                h = nh;
+            }
+            lo[d] = l;
+            hi[d] = h;
+        }
+    }
+    bnd[j] = make_double4(lo[0], hi[0], lo[1], hi[1]);
+    c_cx[j] = (hi[0] + lo[0]) / 2.0;
+    c_cy[j] = (hi[1] + lo[1]) / 2.0;
+}
+
 // ---------------------------------------------------------------------------
 // 6. neighbour table and lists
 // ---------------------------------------------------------------------------
+// The box tests of the type-1 lists (list_one .. list_four,
+// subroutine_all_fmm_2d.cpp:291-556) on bounds (xmin, xmax, ymin, ymax),
+// against the absolute tolerance Pars::tol2 = 1e-15 (global.cpp:325):
+// a touches b when an edge of one lies within tol2 of an edge of the other
+// and the ranges overlap (within tol2) in the other direction; a is well
+// separated from b when a gap above tol2 parts them in some direction.
+constexpr double kT1Tol = 1.0e-15;
+
+__device__ __forceinline__ bool t1_touch(const double4 a, const double4 b) {
+#pragma clang fp contract(off)
+    const double dx_xn = fabs(a.y - b.x), dx_nx = fabs(a.x - b.y);
+    const double dy_xn = fabs(a.w - b.z), dy_nx = fabs(a.z - b.w);
+    return (((dy_xn <= kT1Tol) || (dy_nx <= kT1Tol)) && (a.x <= b.y + kT1Tol) &&
+            (a.y >= b.x - kT1Tol)) ||
+           (((dx_xn <= kT1Tol) || (dx_nx <= kT1Tol)) && (a.z <= b.w + kT1Tol) &&
+            (a.w >= b.z - kT1Tol));
+}
+__device__ __forceinline__ bool t1_sep(const double4 a, const double4 b) {
+#pragma clang fp contract(off)
+    return (a.x > b.y + kT1Tol) || (a.y < b.x - kT1Tol) || (a.z > b.w + kT1Tol) ||
+           (a.w < b.z - kT1Tol);
+}
+
 // nbr[9j + k], k = (dx+1) + 3 (dy+1) (treeCell.cpp:593-622 order): the
 // deepest existing cell at level <= L containing the level-L cell at
 // (ix+dx, iy+dy), or -1 outside [0, 2^L)^2.  One thread per (cell, k): up the
@@ -746,13 +875,19 @@ __global__ void __launch_bounds__(256)
 //    occurrence among the eight lanes and if separated from the cell.
 // The lanes' entries are compacted with a prefix sum over the eight lanes, in
 // lane order and child order within a lane.
+// With T1 the candidates are the same (the cells a type-1 box test can pass
+// lie within one cell of the parent) and the reference's type-1 tests decide
+// (list_two, list_four): a non-leaf neighbour contributes only if it touches
+// the parent, its children only if well separated from the cell; a list-4
+// leaf must touch the parent and be well separated from the cell.
 constexpr int kM2lLanes = 8;
 
+template <bool T1>
 __global__ void __launch_bounds__(256)
     k_m2l_lists(const uint8_t *c_level, const uint8_t *c_flags, const double *c_cx,
                 const double *c_cy, const int *c_parent, const int *c_child, const int *nbr,
-                const DevHdr *hdr, int j0, int nc, int *l2, uint8_t *n2, int *l4, uint8_t *n4,
-                DevHdr *hdr_w) {
+                const double4 *bnd, const DevHdr *hdr, int j0, int nc, int *l2, uint8_t *n2,
+                int *l4, uint8_t *n4, DevHdr *hdr_w) {
 #pragma clang fp contract(off)
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = j0 + (int)(gid / kM2lLanes);
@@ -775,12 +910,14 @@ __global__ void __launch_bounds__(256)
             if (fa & kActive) {
                 if (c_level[A] == L - 1 && !(fa & kLeaf)) {
                     const double md2 = s + kListEps;
+                    const bool adj = !T1 || t1_touch(bnd[A], bnd[c_parent[j]]);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) ch[q] = c_child[4 * A + q];
+                    for (int q = 0; q < 4; ++q) ch[q] = adj ? c_child[4 * A + q] : -1;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         if (ch[q] < 0 || !(c_flags[ch[q]] & kActive)) continue;
-                        keep[q] = fabs(c_cx[ch[q]] - cx) > md2 || fabs(c_cy[ch[q]] - cy) > md2;
+                        keep[q] = T1 ? t1_sep(bnd[ch[q]], bnd[j])
+                                     : fabs(c_cx[ch[q]] - cx) > md2 || fabs(c_cy[ch[q]] - cy) > md2;
                     }
                 } else if (fa & kLeaf) {
                     cand = A;
@@ -797,8 +934,12 @@ __global__ void __launch_bounds__(256)
     }
     bool sep4 = false;
     if (cand >= 0 && !dup) {
-        const double md = ((cell_len(R, c_level[cand]) + s) / 2.0) + kListEps;
-        sep4 = fabs(c_cx[cand] - cx) > md || fabs(c_cy[cand] - cy) > md;
+        if (T1) {
+            sep4 = t1_touch(bnd[cand], bnd[c_parent[j]]) && t1_sep(bnd[cand], bnd[j]);
+        } else {
+            const double md = ((cell_len(R, c_level[cand]) + s) / 2.0) + kListEps;
+            sep4 = fabs(c_cx[cand] - cx) > md || fabs(c_cy[cand] - cy) > md;
+        }
     }
     const int m2 = (int)keep[0] + (int)keep[1] + (int)keep[2] + (int)keep[3];
     const int m4 = sep4 ? 1 : 0;
@@ -847,14 +988,22 @@ __global__ void __launch_bounds__(256)
 // fill pass writes at the leaf offset plus the prefix of the lanes' counts and
 // also counts the P2P pairs and M2P / L2P terms the lists imply (statistics),
 // reduced per block into per-block partials.
+// With T1 the walk visits the same cells (every cell a type-1 box test can
+// pass is adjacent to the leaf or a child of an adjacent cell) and the
+// reference's type-1 tests decide (list_one, list_three): a leaf joins list
+// 1 if it touches the leaf (the leaf itself always, first); the children of a
+// visited non-leaf that touches the leaf join list 3 if well separated from
+// it.  Adjacent non-leaf children are walked whatever the tests say, since
+// the reference tests every cell on its own.
 constexpr int kLeafLanes = 16;
 constexpr int kWalkDepth = 24;
 
-template <bool FILL>
+template <bool FILL, bool T1>
 __global__ void __launch_bounds__(256)
     k_leaf_lists(const uint32_t *leaf_idx, int nl, const uint8_t *c_level, const uint8_t *c_flags,
                  const double *c_cx, const double *c_cy, const int *c_child, const int *nbr,
-                 const DevHdr *hdr, const uint8_t *orders, uint32_t *cnt1, uint32_t *cnt3,
+                 const double4 *bnd, const DevHdr *hdr, const uint8_t *orders, uint32_t *cnt1,
+                 uint32_t *cnt3,
                  uint2 *cntk, const uint32_t *off1, const uint32_t *off3, int *l1, int *l3,
                  uint32_t cap1, uint32_t cap3, uint32_t *bs1, uint32_t *bs3,
                  const uint32_t *c_begin, const uint32_t *c_end, DevHdr *hdr_w) {
@@ -919,12 +1068,21 @@ __global__ void __launch_bounds__(256)
         }
         ++n1;
     };
+    auto emit3 = [&](int c) {
+        if (FILL) {
+            if (o3 + n3 < cap3) l3[o3 + n3] = c;
+            else overflow = true;
+            m2p_w += (unsigned long long)orders[c_level[c]] + 1ull;
+        }
+        ++n3;
+    };
+    const double4 bj = T1 && live ? bnd[j] : make_double4(0.0, 0.0, 0.0, 0.0);
     if (live && A >= 0) {
         if (c_level[A] < L) {
-            if (cand >= 0 && !dup) emit1(A);
+            if (cand >= 0 && !dup && (!T1 || t1_touch(bnd[A], bj))) emit1(A);
         } else if (fa & kActive) {
             if (fa & kLeaf) {
-                emit1(A);
+                if (!T1 || A == j || t1_touch(bnd[A], bj)) emit1(A);
             } else {
                 const int me = threadIdx.x;
                 int sp = 0;
@@ -932,6 +1090,7 @@ __global__ void __launch_bounds__(256)
                 while (sp > 0) {
                     const int B = stk[--sp][me];
                     const double adj = ((s + (cell_len(R, c_level[B]) / 2.0)) / 2.0) + kListEps;
+                    const bool tb = !T1 || t1_touch(bnd[B], bj);
                     int chq[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) chq[q] = c_child[4 * B + q];
@@ -944,16 +1103,17 @@ __global__ void __launch_bounds__(256)
                         const bool is_adj =
                             fabs(c_cx[ch] - cx) <= adj && fabs(c_cy[ch] - cy) <= adj;
                         if (is_adj) {
-                            if (fc & kLeaf) emit1(ch);
-                            else if (sp < kWalkDepth) stk[sp++][me] = ch;
-                            else overflow = true;
-                        } else {
-                            if (FILL) {
-                                if (o3 + n3 < cap3) l3[o3 + n3] = ch;
-                                else overflow = true;
-                                m2p_w += (unsigned long long)orders[c_level[ch]] + 1ull;
+                            if (fc & kLeaf) {
+                                if (!T1 || t1_touch(bnd[ch], bj)) emit1(ch);
+                            } else if (sp < kWalkDepth) {
+                                stk[sp++][me] = ch;
+                            } else {
+                                overflow = true;
                             }
-                            ++n3;
+                            // bounds rounded apart by more than tol2
+                            if (T1 && tb && t1_sep(bnd[ch], bj)) emit3(ch);
+                        } else if (!T1 || (tb && t1_sep(bnd[ch], bj))) {
+                            emit3(ch);
                         }
                     }
                 }
@@ -1015,6 +1175,11 @@ void stream_wait(Ctx &c) {
 static void check_flags(const DevHdr &h) {
     if (h.flags & kFlagNonFinite) throw Error(kNonFinite, "non-finite particle coordinate");
     if (h.flags & kFlagRange) throw Error(kDegenerate, "all particles at one point (root length 0)");
+    if (h.flags & kFlagT1Lost)
+        throw Error(kOutsideBoxes,
+                    "a particle lies outside every box of the type-1 hierarchy (on the upper "
+                    "bound of the level-1 boxes or between a split box's rounded child bounds); "
+                    "the reference drops such a particle");
     if (h.flags & kFlagDepth)
         throw Error(kDepth, "tree deeper than 16 levels (more coincident particles than the leaf thresholds)");
     if (h.flags & kFlagListOverflow) throw Error(kInternal, "list traversal stack overflow");
@@ -1046,8 +1211,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     // from the previous build checked after the split
     const int bits = c.sort_bits;
     const int tiles = radix_tiles(n);
-    k_keys<<<tiles, kRsBlock, 0, s>>>(x, y, n, part, nbb, expansion, c.hdr_d, ka, va, 32 - bits,
-                                      tmp, tiles);
+    c.t1 = tp.type1;
+    (tp.type1 ? k_keys<true> : k_keys<false>)<<<tiles, kRsBlock, 0, s>>>(
+        x, y, n, part, nbb, expansion, tp.max_level, c.hdr_d, ka, va, 32 - bits, tmp, tiles);
     const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 32 - bits, 32, tmp, s, true);
     uint32_t *key = alt ? kb : ka;
     uint32_t *perm = alt ? vb : va;
@@ -1059,7 +1225,8 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     uint8_t *d = c.lcp_d.as<uint8_t>(n + 1), *w = c.lcp_w.as<uint8_t>(n + 1);
     double *ss = (c.icut > 0 && src.s) ? c.ss.as<double>(n) : nullptr;
     k_gather<<<div_up(n, 256), 256, 0, s>>>(perm, x, y, src.q, src.s, src.omega, active, n, xs,
-                                            ys, qs, as, bs_act, key, W, d, w, ss, c.hdr_d);
+                                            ys, qs, as, bs_act, key, W, d, w, ss, src.qscale,
+                                            c.hdr_d);
 
     // 4. leaf levels, cells per level and leaves
     uint32_t *A = c.act_scan.as<uint32_t>(n + 1);
@@ -1089,7 +1256,7 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
     VPM_LAUNCH_CHECK();
     stream_mark_wait(c);
     DevHdr &hh = *c.hdr_h;
-    if (hh.flags & (kFlagNonFinite | kFlagRange)) check_flags(hh);
+    if (hh.flags & (kFlagNonFinite | kFlagRange | kFlagT1Lost)) check_flags(hh);
     // the split rows: leaf-level range, depth overflow, cells per level, leaves
     SplitLevels sl{};
     {
@@ -1176,6 +1343,9 @@ void tree_build(Ctx &c, int n, const double *x, const double *y, const Sources &
                                              c.hdr_d, c_level, c_flags, c_key, c_begin, c_end,
                                              c_src, c_cx, c_cy, c_child, sl, c.hdr_d);
     k_cells2<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d, c_parent, c_child);
+    if (tp.type1)
+        k_t1_bounds<<<div_up(nc, 256), 256, 0, s>>>(c_level, c_key, nc, c.hdr_d,
+                                                    c.c_bnd.as<double4>(nc), c_cx, c_cy);
     VPM_LAUNCH_CHECK();
 }
 
@@ -1202,9 +1372,11 @@ void tree_lists_begin(Ctx &c) {
     // n2 / n4 of every cell of level >= 2 are written by k_m2l_lists; levels 0
     // and 1 have no lists 2 / 4 and nothing reads their entries
     const int nb2 = nc > j0 ? (int)div_up((long long)(nc - j0) * kM2lLanes, 256LL) : 0;
+    const double4 *bnd = c.t1 ? (const double4 *)c.c_bnd.p : nullptr;
     if (nb2 > 0)
-        k_m2l_lists<<<nb2, 256, 0, s>>>(c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr,
-                                        c.hdr_d, j0, nc, l2, n2, l4, n4, c.hdr_d);
+        (c.t1 ? k_m2l_lists<true> : k_m2l_lists<false>)<<<nb2, 256, 0, s>>>(
+            c_level, c_flags, c_cx, c_cy, c_parent, c_child, nbr, bnd, c.hdr_d, j0, nc, l2, n2,
+            l4, n4, c.hdr_d);
     VPM_LAUNCH_CHECK();
 }
 
@@ -1224,10 +1396,10 @@ void tree_lists_end(Ctx &c) {
     const uint32_t *c_begin = (const uint32_t *)c.c_begin.p, *c_end = (const uint32_t *)c.c_end.p;
     uint2 *cntk = c.cntk.as<uint2>(9 * (size_t)nl + 1);
     uint32_t *bs1 = c.bs_l13.as<uint32_t>(2 * (size_t)nbl + 2), *bs3 = bs1 + nbl + 1;
-    k_leaf_lists<false><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
-                                            nbr, c.hdr_d, c.orders, cnt1, cnt3, cntk, nullptr,
-                                            nullptr, nullptr, nullptr, 0, 0, bs1, bs3, c_begin,
-                                            c_end, c.hdr_d);
+    const double4 *bnd = c.t1 ? (const double4 *)c.c_bnd.p : nullptr;
+    (c.t1 ? k_leaf_lists<false, true> : k_leaf_lists<false, false>)<<<nbl, 256, 0, s>>>(
+        leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child, nbr, bnd, c.hdr_d, c.orders, cnt1,
+        cnt3, cntk, nullptr, nullptr, nullptr, nullptr, 0, 0, bs1, bs3, c_begin, c_end, c.hdr_d);
     constexpr int kLeavesPerBlock = 256 / kLeafLanes;
     scan_exclusive_bs_u32_pair(cnt1, off1, bs1, &c.hdr_d->tot1, cnt3, off3, bs3, &c.hdr_d->tot3, nl,
                                kLeavesPerBlock, true, tmp, s);
@@ -1240,10 +1412,10 @@ void tree_lists_end(Ctx &c) {
     ti.cap1 = 17 * (size_t)nl + 1;
     ti.cap3 = 8 * (size_t)nc + 1;
     int *l1 = c.l1.as<int>(ti.cap1), *l3 = c.l3.as<int>(ti.cap3);
-    k_leaf_lists<true><<<nbl, 256, 0, s>>>(leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child,
-                                           nbr, c.hdr_d, c.orders, nullptr, nullptr, cntk, off1,
-                                           off3, l1, l3, (uint32_t)ti.cap1, (uint32_t)ti.cap3,
-                                           nullptr, nullptr, c_begin, c_end, c.hdr_d);
+    (c.t1 ? k_leaf_lists<true, true> : k_leaf_lists<true, false>)<<<nbl, 256, 0, s>>>(
+        leaf_idx, nl, c_level, c_flags, c_cx, c_cy, c_child, nbr, bnd, c.hdr_d, c.orders, nullptr,
+        nullptr, cntk, off1, off3, l1, l3, (uint32_t)ti.cap1, (uint32_t)ti.cap3, nullptr, nullptr,
+        c_begin, c_end, c.hdr_d);
     VPM_LAUNCH_CHECK();
 }
 

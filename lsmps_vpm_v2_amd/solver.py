@@ -7,7 +7,9 @@ it reads the same:
 * ``Particle``      -- the fields of ``struct Particle`` the path touches
                        (reference Utils.hpp:77-159);
 * ``VelocityCalc``  -- ``get_velocity(particle, base_grid, step)``
-                       (velocity_calc.hpp:42, velocity_calc.cpp:149-251);
+                       (velocity_calc.hpp:42, velocity_calc.cpp:149-251), its
+                       FMM (type 2) and type-1 FMM branches;
+* ``velocity_biot_savart`` -- ``biotsavart_fmm_2d`` (biotsavart_fmm_2d.cpp:4);
 * ``TreeCell``      -- ``initializeTree`` / ``updateTree`` / ``intList`` /
                        ``extList`` and the tree summary (treeCell.hpp:83-195);
 * ``fmm2D``         -- ``calcField`` and ``get_Field_x/y`` (fmm2D.hpp:15-68).
@@ -252,6 +254,42 @@ class FmmSolver:
                                             _ptr(a), _ptr(u), _ptr(v)), "vpmfmm_velocity")
         return u, v
 
+    @staticmethod
+    def type1_opts(n_s=50, lmax=7, ndp=10, icutoff=2, l2l_mode=1) -> "_lib.Type1Opts":
+        """vpmfmm_type1_opts; defaults are the reference's global.cpp:316-328
+        (l2l_mode 1: the L2L shift as the reference runs it)."""
+        o = _lib.Type1Opts()
+        o.n_s, o.lmax, o.ndp, o.icutoff, o.l2l_mode = n_s, lmax, ndp, icutoff, l2l_mode
+        return o
+
+    def velocity_type1(self, x, y, s, gz, **type1):
+        """(u, v) of the type-1 FMM (VelocityCalc::velocity_old ->
+        biotsavart_fmm_2d, plus the handle's free stream) for host arrays;
+        keyword arguments are the fields of type1_opts."""
+        o = self.type1_opts(**type1)
+        x, y, g = _f64(x), _f64(y), _f64(gz)
+        s = _f64(s) if s is not None else None
+        n = len(x)
+        u, v = np.empty(n), np.empty(n)
+        self._check(self._L.vpmfmm_velocity_type1(self._h, C.byref(o), n, _ptr(x), _ptr(y),
+                                                  _ptr(s) if s is not None else None, _ptr(g),
+                                                  _ptr(u), _ptr(v)), "vpmfmm_velocity_type1")
+        return u, v
+
+    def velocity_type1_device(self, x, y, s, gz, u=None, v=None, **type1):
+        """velocity_type1 on device tensors (s may be None for icutoff 0)."""
+        o = self.type1_opts(**type1)
+        n = x.numel()
+        if u is None:
+            u = x.new_empty(n)
+        if v is None:
+            v = x.new_empty(n)
+        self._enter(x)
+        self._check(self._L.vpmfmm_velocity_type1_device(
+            self._h, C.byref(o), n, _dptr(x), _dptr(y), _dptr(s) if s is not None else None,
+            _dptr(gz), _dptr(u), _dptr(v)), "vpmfmm_velocity_type1_device")
+        return u, v
+
     # ----------------------------------------------------------- device paths
     def field_device(self, x, y, q, active, ex=None, ey=None):
         """Device tensors in, device tensors out (float64 / uint8, contiguous)."""
@@ -398,7 +436,8 @@ class FmmSolver:
 # ---------------------------------------------------------------------------
 @dataclass
 class Particle:
-    """Fields of the reference ``Particle`` used by the velocity path."""
+    """Fields of the reference ``Particle`` used by the velocity path
+    (``gz``, the vortex strength vorticity s^2, feeds the type-1 branch)."""
     x: np.ndarray
     y: np.ndarray
     s: np.ndarray
@@ -406,6 +445,7 @@ class Particle:
     isActive: np.ndarray
     u: np.ndarray = field(default_factory=lambda: np.zeros(0))
     v: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    gz: np.ndarray | None = None
 
     @property
     def num(self) -> int:
@@ -413,9 +453,15 @@ class Particle:
 
 
 class VelocityCalc:
-    """``VelocityCalc`` with the FMM branch (VELOCITY_2D_TYPE 2) on the GPU."""
+    """``VelocityCalc`` with the FMM branches on the GPU: VELOCITY_2D_TYPE 2
+    (``velocity_fmm_2d``, the default) and 1 (``velocity_old``, the type-1
+    FMM, with the type-1 knobs n_s, lmax, ndp, icutoff, l2l_mode)."""
 
-    def __init__(self, **solver_opts):
+    def __init__(self, velocity_type: int = 2, type1: dict | None = None, **solver_opts):
+        if velocity_type not in (1, 2):
+            raise ValueError("velocity_type must be 1 (type-1 FMM) or 2 (FMM)")
+        self.velocity_type = velocity_type
+        self.type1 = dict(type1 or {})
         self._solver = FmmSolver(**solver_opts)
 
     @property
@@ -425,11 +471,55 @@ class VelocityCalc:
     def get_velocity(self, particle: Particle, base_grid=None, step: int = 0) -> None:
         """velocity_calc.cpp:149-251: p.u, p.v = rotational part + free stream.
 
-        ``base_grid`` is accepted and unused, as on the reference's FMM branch;
-        ``step`` is accepted for signature parity (the tree is rebuilt every
-        call, as TreeCell::updateTree does, treeCell.cpp:1245-1249)."""
+        ``base_grid`` is accepted and unused, as on the reference's FMM
+        branches; ``step`` is accepted for signature parity (the tree is
+        rebuilt every call, as TreeCell::updateTree does,
+        treeCell.cpp:1245-1249).  Type 1 reads ``gz`` (vorticity s^2 when it
+        is None, redistribute_particles.cpp:186) and ignores ``isActive``
+        (velocity_calc.cpp:404-424: every particle is a source)."""
+        if self.velocity_type == 1:
+            gz = particle.gz if particle.gz is not None else \
+                np.asarray(particle.vorticity) * np.asarray(particle.s) ** 2
+            particle.u, particle.v = self._solver.velocity_type1(
+                particle.x, particle.y, particle.s, gz, **self.type1)
+            return
         particle.u, particle.v = self._solver.velocity(
             particle.x, particle.y, particle.s, particle.vorticity, particle.isActive)
+
+
+class velocity_biot_savart:  # noqa: N801 -- reference class name
+    """``velocity_biot_savart`` (velocity_biot_savart.hpp:12-63): the type-1
+    FMM ``biotsavart_fmm_2d(pi, pj, icutoff, n_s, n_inter, ndp)``
+    (biotsavart_fmm_2d.cpp:4).  Like the reference, targets and sources are
+    one particle set (its box lists index pj with pi's particles; velocity_old
+    passes two copies of the same particles), n_inter 1 is the only mode, and
+    the result is the rotational part alone, written to pi.u, pi.v.  The
+    near-field kernel is ``icutoff`` (the reference reads Pars::icutoff, the
+    same value in velocity_old); lmax and the L2L mode are constructor knobs
+    (Pars::lmax = 7; l2l_mode 1 is the reference's shift)."""
+
+    def __init__(self, lmax: int = 7, l2l_mode: int = 1, solver: FmmSolver | None = None):
+        self.lmax, self.l2l_mode = lmax, l2l_mode
+        self._solver = solver or FmmSolver(u_inf=0.0, v_inf=0.0)
+
+    def biotsavart_fmm_2d(self, pi: Particle, pj: Particle, icutoff: int = 2, n_s: int = 50,
+                          n_inter: int = 1, ndp: int = 10) -> None:
+        if n_inter != 1:
+            raise ValueError("n_inter must be 1 (targets and sources are one particle set)")
+        if pj is not pi and not (np.array_equal(pi.x, pj.x) and np.array_equal(pi.y, pj.y)):
+            raise ValueError("targets and sources must be the same particles")
+        gz = pj.gz if pj.gz is not None else np.asarray(pj.vorticity) * np.asarray(pj.s) ** 2
+        o = self._solver.opts
+        free = (o.u_inf, o.v_inf)
+        if free != (0.0, 0.0):
+            self._solver.set_opts(u_inf=0.0, v_inf=0.0)
+        try:
+            pi.u, pi.v = self._solver.velocity_type1(pj.x, pj.y, pj.s, gz, n_s=n_s,
+                                                     lmax=self.lmax, ndp=ndp, icutoff=icutoff,
+                                                     l2l_mode=self.l2l_mode)
+        finally:
+            if free != (0.0, 0.0):
+                self._solver.set_opts(u_inf=free[0], v_inf=free[1])
 
 
 class TreeCell:

@@ -23,7 +23,7 @@ def _ensure_built():
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(vpmfmm_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(vpmfmm_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declarations_are_exported():
