@@ -2,8 +2,9 @@
  * the reference's velocity manager.
  *
  * Replaces class VelocityCalc (src/velocity_calculation/velocity_calc.hpp:15-47)
- * for VELOCITY_2D_TYPE = FMM: get_velocity(particle, base_grid, step) and
- * get_velocity(particle, step) (velocity_calc.cpp:149-251) compute
+ * for VELOCITY_2D_TYPE = FMM (2) and the type-1 FMM (1, set_default_velocity_type):
+ * get_velocity(particle, base_grid, step) and
+ * get_velocity(particle, step) (velocity_calc.cpp:149-251) compute, on the FMM branch,
  *   u = E_y + u_inf, v = -E_x + v_inf,  E_i = sum_{j != i} q_j (x_i - x_j) / r_ij^2,
  *   q_j = -omega_j s_j^2 / (2 pi)
  * (velocity_calc.cpp:455-578, fmm2D.cpp) on the GPU through include/vpmfmm.h,
@@ -49,6 +50,7 @@ inline void throw_status(int st, const char *where, vpmfmm_handle *h) {
     case VPMFMM_DEGENERATE_DOMAIN:
     case VPMFMM_TREE_DEPTH_EXCEEDED:
     case VPMFMM_DUPLICATE_POINTS:
+    case VPMFMM_OUTSIDE_BOXES:
         throw std::range_error(msg);
     default:
         throw std::runtime_error(msg);
@@ -83,6 +85,12 @@ public:
     void field(int32_t n, const double *x, const double *y, const double *q,
                const uint8_t *active, double *ex, double *ey) {
         throw_status(vpmfmm_field(h_, n, x, y, q, active, ex, ey), "vpmfmm_field", h_);
+    }
+    // The type-1 FMM (u, v include the free stream).
+    void velocity_type1(const vpmfmm_type1_opts &t, int32_t n, const double *x, const double *y,
+                        const double *s, const double *gz, double *u, double *v) {
+        throw_status(vpmfmm_velocity_type1(h_, &t, n, x, y, s, gz, u, v),
+                     "vpmfmm_velocity_type1", h_);
     }
     vpmfmm_stats stats() const {
         vpmfmm_stats s;
@@ -123,6 +131,30 @@ inline vpmfmm_opts &default_options_slot() {
 inline void set_default_options(const vpmfmm_opts &o) { default_options_slot() = o; }
 inline vpmfmm_opts default_options() { return default_options_slot(); }
 
+// The branch of get_velocity (the reference's VELOCITY_2D_TYPE,
+// velocity_calc.cpp:25, 170-198): 2 = the FMM branch (velocity_fmm_2d, the
+// default here), 1 = the type-1 FMM (velocity_old) with the type-1 options
+// (Pars::n_s, lmax, ndp, icutoff; the reference's global.cpp:316-328 unless
+// given).  Read at each get_velocity call of a default-constructed object.
+struct VelocityBranch {
+    int type = 2;
+    vpmfmm_type1_opts type1{};
+};
+inline VelocityBranch &default_branch_slot() {
+    static VelocityBranch b = [] {
+        VelocityBranch d;
+        vpmfmm_type1_default_opts(&d.type1);
+        return d;
+    }();
+    return b;
+}
+inline void set_default_velocity_type(int type, const vpmfmm_type1_opts *type1 = nullptr) {
+    if (type != 1 && type != 2)
+        throw std::invalid_argument("set_default_velocity_type: type must be 1 or 2");
+    default_branch_slot().type = type;
+    if (type1) default_branch_slot().type1 = *type1;
+}
+
 // One solver per host thread and option set, made at its first use and kept
 // for the life of the process (never destroyed: the HIP runtime may already
 // be gone when static destructors run).  Every VelocityCalc of that thread
@@ -152,6 +184,11 @@ public:
 
     template <class ParticleT>
     void get_velocity(ParticleT &particle, const int /*step*/) {
+        const VelocityBranch br = default_branch_slot();
+        if (br.type == 1) {
+            velocity_old(particle, br.type1);
+            return;
+        }
         const int n = particle.num;
         if (n < 1) throw std::invalid_argument("VelocityCalc::get_velocity: no particles");
         if ((int)particle.x.size() < n || (int)particle.y.size() < n ||
@@ -167,6 +204,24 @@ public:
         solver().velocity(n, particle.x.data(), particle.y.data(), particle.s.data(),
                           particle.vorticity.data(), active_.data(), particle.u.data(),
                           particle.v.data());
+    }
+
+    // VelocityCalc::velocity_old (velocity_calc.cpp:390-440) and get_velocity's
+    // free stream: every particle (isActive is not read) with its size s and
+    // vortex strength gz (Particle::gz) through the type-1 FMM.
+    template <class ParticleT>
+    void velocity_old(ParticleT &particle, const vpmfmm_type1_opts &t) {
+        const int n = particle.num;
+        if (n < 1) throw std::invalid_argument("VelocityCalc::get_velocity: no particles");
+        if ((int)particle.x.size() < n || (int)particle.y.size() < n ||
+            (int)particle.s.size() < n || (int)particle.gz.size() < n)
+            throw std::invalid_argument("VelocityCalc::get_velocity: particle arrays shorter than num");
+        particle.u.clear();
+        particle.v.clear();
+        particle.u.resize(n, 0.0);
+        particle.v.resize(n, 0.0);
+        solver().velocity_type1(t, n, particle.x.data(), particle.y.data(), particle.s.data(),
+                                particle.gz.data(), particle.u.data(), particle.v.data());
     }
 
     // The reference's LSMPS Poisson test entry (velocity_calc.hpp:46) is not

@@ -5,7 +5,8 @@
 // std::vector<bool> isActive; int num).
 //
 //   drop_in_velocity <n> <out.bin>
-// writes n, then x, y, s, omega, active (as double), u, v (n doubles each).
+// writes n, then x, y, s, omega, active (as double), u, v, and the type-1
+// branch's gz, u, v (n doubles each).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -15,7 +16,7 @@
 #include "vpmfmm_velocity_calc.hpp"
 
 struct Particle {
-    std::vector<double> x, y, u, v, s, vorticity;
+    std::vector<double> x, y, u, v, gz, s, vorticity;
     std::vector<bool> isActive;
     int num = 0;
 };
@@ -44,6 +45,7 @@ int main(int argc, char **argv) {
         p.s.push_back(0.01);
         p.vorticity.push_back(std::sin(3.0 * p.x.back()) * std::cos(2.0 * p.y.back()) * 50.0);
         p.isActive.push_back(rnd() < 0.9);
+        p.gz.push_back(p.vorticity.back() * p.s.back() * p.s.back());
     }
     GridNode grid;
     VelocityCalc velocity_tool;
@@ -91,13 +93,26 @@ int main(int argc, char **argv) {
             return 1;
         }
     }
+    // the type-1 branch (VELOCITY_2D_TYPE 1, velocity_old): every particle,
+    // the reference's type-1 options, gz as the strength
+    Particle t1 = p;
+    vpmfmm::set_default_velocity_type(1);
+    try {
+        VelocityCalc old_tool;
+        old_tool.get_velocity(t1, grid, 0);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "type-1 get_velocity failed: %s\n", e.what());
+        return 1;
+    }
+    vpmfmm::set_default_velocity_type(2);
     FILE *f = std::fopen(argv[2], "wb");
     if (!f) return 1;
     const double nd = n;
     std::fwrite(&nd, 8, 1, f);
     std::vector<double> a(n);
     for (int i = 0; i < n; ++i) a[i] = p.isActive[i] ? 1.0 : 0.0;
-    for (const std::vector<double> *v : {&p.x, &p.y, &p.s, &p.vorticity, &a, &p.u, &p.v})
+    for (const std::vector<double> *v : {&p.x, &p.y, &p.s, &p.vorticity, &a, &p.u, &p.v,
+                                         &p.gz, &t1.u, &t1.v})
         std::fwrite(v->data(), 8, n, f);
     std::fclose(f);
     std::printf("drop_in_velocity: n=%d u[0]=%.17g v[0]=%.17g\n", n, p.u[0], p.v[0]);

@@ -1,6 +1,7 @@
 """The C++ drop-in (include/vpmfmm_velocity_calc.hpp) on the GPU: the
 reference's call get_velocity(particle, grid, step) through a compiled C++
-program, checked against the oracle's velocity to 1e-10."""
+program, on the FMM branch and the type-1 branch, checked against the
+oracle's velocities to 1e-10."""
 import subprocess
 
 import numpy as np
@@ -20,9 +21,15 @@ def test_cpp_drop_in_velocity_matches_oracle(gpu, oracle, tmp_path):
     assert r.returncode == 0
     raw = np.fromfile(out, dtype=np.float64)
     assert int(raw[0]) == n
-    x, y, s, w, a, u, v = raw[1:].reshape(7, n)
+    x, y, s, w, a, u, v, gz, u1, v1 = raw[1:].reshape(10, n)
     uo, vo = oracle.velocity(x, y, s, w, a.astype(np.uint8), oracle.opts())
     scale = np.sqrt(np.mean((uo - 1.0) ** 2 + vo ** 2))
     err = np.max(np.hypot(u - uo, v - vo)) / scale
     print(f"drop-in vs oracle: max |d(u,v)| / rms = {err:.2e}")
     assert err <= 1e-10
+    # the type-1 branch (VELOCITY_2D_TYPE 1): the reference's type-1 options,
+    # every particle a source, plus the free stream (1, 0)
+    ut, vt = oracle.t1_velocity(x, y, s, gz, oracle.t1_opts())
+    err1 = np.max(np.hypot(u1 - 1.0 - ut, v1 - vt)) / np.sqrt(np.mean(ut ** 2 + vt ** 2))
+    print(f"drop-in type-1 vs oracle: max |d(u,v)| / rms = {err1:.2e}")
+    assert err1 <= 1e-10
