@@ -44,6 +44,17 @@ class OrcOpts(C.Structure):
     ]
 
 
+class O3Opts(C.Structure):
+    """Options of the 3-D oracle (fmm3d_oracle.h)."""
+    _fields_ = [
+        ("crit_num", C.c_int),
+        ("exp_tree", C.c_double),
+        ("center", C.c_double * 3),
+        ("support", C.c_double),
+        ("max_depth", C.c_int),
+    ]
+
+
 class T1Opts(C.Structure):
     """Options of the type-1 oracle (fmm2d_type1_oracle.h)."""
     _fields_ = [
@@ -131,6 +142,27 @@ def lib():
         L.orc_t1_list.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int]
         L.orc_t1_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(T1Opts), _dp, _dp]
         L.orc_t1_direct.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, _ip, _dp, _dp]
+        L.orc3_default_opts.argtypes = [C.POINTER(O3Opts)]
+        L.orc3_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _dp, _dp, _up,
+                                 C.POINTER(O3Opts)]
+        L.orc3_free.argtypes = [C.c_void_p]
+        L.orc3_ncells.argtypes = [C.c_void_p]
+        L.orc3_depth.argtypes = [C.c_void_p]
+        L.orc3_cells.argtypes = [C.c_void_p, _ip, _lp, _ip, _up, _up, _dp, _dp]
+        L.orc3_leaf_particles.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        L.orc3_velocity_alpha.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _up,
+                                          C.POINTER(O3Opts), _dp, _dp, _dp]
+        L.orc3_velocity.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _up,
+                                    C.POINTER(O3Opts), C.c_double, C.c_double, C.c_double,
+                                    _dp, _dp, _dp]
+        L.orc3_m2p_mul.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc3_p2m.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc3_m2m.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double]
+        L.orc3_direct.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _up, C.c_int, _ip,
+                                  _dp, _dp, _dp]
+        for name in ("orc3_default_opts", "orc3_free", "orc3_cells", "orc3_m2p_mul",
+                     "orc3_p2m", "orc3_m2m", "orc3_direct"):
+            getattr(L, name).restype = None
         for name in ("orc_t1_default_opts", "orc_t1_l2l_table", "orc_t1_free", "orc_t1_root",
                      "orc_t1_boxes", "orc_t1_direct"):
             getattr(L, name).restype = None
@@ -621,3 +653,111 @@ def ref_type1_velocity(x, y, s, g, n_s=50, lmax=7, ndp=10, icutoff=2):
                                 int(n_s), int(lmax), int(ndp), int(icutoff), _ptr(u, _dp),
                                 _ptr(v, _dp)), "ref_type1_velocity")
     return u, v
+
+
+# --------------------------------------------------------------------------
+# the 3-D vector FMM (fmm3d_oracle.c; reference fast3DFMM.cpp)
+# --------------------------------------------------------------------------
+def o3_opts(crit_num=50, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
+            max_depth=21) -> O3Opts:
+    o = O3Opts()
+    lib().orc3_default_opts(C.byref(o))
+    o.crit_num, o.exp_tree, o.support, o.max_depth = crit_num, exp_tree, support, max_depth
+    for d in range(3):
+        o.center[d] = center[d]
+    return o
+
+
+class O3Tree:
+    """The octree of FMMCell::generateTree (fast3DFMM.cpp:485-745): cells in
+    creation order with level, path key, count, leaf, active, centre, size."""
+
+    def __init__(self, x, y, z, active, o: O3Opts | None = None):
+        L = lib()
+        x, y, z, a = _d(x), _d(y), _d(z), _u8(active)
+        o = o or o3_opts()
+        self._h = C.c_void_p()
+        _check(L.orc3_build(C.byref(self._h), len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                            _ptr(a, _up), C.byref(o)), "orc3_build")
+        nc = L.orc3_ncells(self._h)
+        self.depth = L.orc3_depth(self._h)
+        self.level = np.zeros(nc, np.int32)
+        self.key = np.zeros(nc, np.int64)
+        self.count = np.zeros(nc, np.int32)
+        self.leaf = np.zeros(nc, np.uint8)
+        self.active = np.zeros(nc, np.uint8)
+        self.centre = np.zeros((nc, 3))
+        self.size = np.zeros(nc)
+        L.orc3_cells(self._h, _ptr(self.level, _ip), _ptr(self.key, _lp), _ptr(self.count, _ip),
+                     _ptr(self.leaf, _up), _ptr(self.active, _up), _ptr(self.centre, _dp),
+                     _ptr(self.size, _dp))
+
+    def particles(self, c):
+        out = np.zeros(max(int(self.count[c]), 1), np.int32)
+        k = lib().orc3_leaf_particles(self._h, int(c), _ptr(out, _ip), len(out))
+        return out[:max(k, 0)].copy()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc3_free(self._h)
+            self._h = None
+
+
+def o3_velocity_alpha(x, y, z, ax, ay, az, active, o: O3Opts | None = None):
+    """Rotational velocity from vortex strengths (fmm3D::calcVelocityFast)."""
+    L = lib()
+    x, y, z, ax, ay, az, a = _d(x), _d(y), _d(z), _d(ax), _d(ay), _d(az), _u8(active)
+    n = len(x)
+    u, v, w = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(L.orc3_velocity_alpha(n, *(_ptr(t, _dp) for t in (x, y, z, ax, ay, az)), _ptr(a, _up),
+                                 C.byref(o or o3_opts()), _ptr(u, _dp), _ptr(v, _dp),
+                                 _ptr(w, _dp)), "orc3_velocity_alpha")
+    return u, v, w
+
+
+def o3_velocity(x, y, z, s, wx, wy, wz, active, o: O3Opts | None = None, u_inf=0.0,
+                v_inf=0.0, w_inf=0.0):
+    """Velocity of every particle as VelocityCalc::velocity_fmm_3d_fast plus the
+    free stream (velocity_calc.cpp:2384-2525, 221-236)."""
+    L = lib()
+    arrs = [_d(t) for t in (x, y, z, s, wx, wy, wz)]
+    a = _u8(active)
+    n = len(arrs[0])
+    u, v, w = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(L.orc3_velocity(n, *(_ptr(t, _dp) for t in arrs), _ptr(a, _up),
+                           C.byref(o or o3_opts()), u_inf, v_inf, w_inf, _ptr(u, _dp),
+                           _ptr(v, _dp), _ptr(w, _dp)), "orc3_velocity")
+    return u, v, w
+
+
+def o3_direct(x, y, z, ax, ay, az, active=None, targets=None):
+    """Direct 3-D Biot-Savart sum over active sources (the target skipped)."""
+    L = lib()
+    x, y, z, ax, ay, az = _d(x), _d(y), _d(z), _d(ax), _d(ay), _d(az)
+    a = _u8(active) if active is not None else None
+    t = np.ascontiguousarray(targets, np.int32) if targets is not None else None
+    m = len(t) if t is not None else len(x)
+    u, v, w = np.zeros(m), np.zeros(m), np.zeros(m)
+    L.orc3_direct(len(x), *(_ptr(q, _dp) for q in (x, y, z, ax, ay, az)),
+                  _ptr(a, _up) if a is not None else None, m,
+                  _ptr(t, _ip) if t is not None else None, _ptr(u, _dp), _ptr(v, _dp),
+                  _ptr(w, _dp))
+    return u, v, w
+
+
+def o3_m2p_mul(dx, dy, dz):
+    mx, my, mz = np.zeros(10), np.zeros(10), np.zeros(10)
+    lib().orc3_m2p_mul(dx, dy, dz, _ptr(mx, _dp), _ptr(my, _dp), _ptr(mz, _dp))
+    return mx, my, mz
+
+
+def o3_p2m(mp, src, dx, dy, dz):
+    mp = _d(mp).copy()
+    lib().orc3_p2m(_ptr(mp, _dp), src, dx, dy, dz)
+    return mp
+
+
+def o3_m2m(mp, mpc, dx, dy, dz):
+    mp, mpc = _d(mp).copy(), _d(mpc)
+    lib().orc3_m2m(_ptr(mp, _dp), _ptr(mpc, _dp), dx, dy, dz)
+    return mp

@@ -11,6 +11,7 @@
 
 #include "fmm2d_oracle.h"
 #include "fmm2d_type1_oracle.h"
+#include "fmm3d_oracle.h"
 
 static uint64_t g_state = 0x243F6A8885A308D3ull;
 
@@ -187,6 +188,34 @@ static int run_field(int n, int clusters, double inactive) {
         orc_t1_l2l_table(10, 1, coef, expo);
         orc_t1_direct(n, x, y, s, q, 2, n / 3, tidx, ex, ey);
         CHECK(finite_all(ex, n / 3), "type-1 direct");
+    }
+    /* the 3-D path: the (x, y, q) field lifted to 3-D with z from the sizes,
+     * tree, velocity at two leaf thresholds, direct sum */
+    {
+        double *z = (double *)malloc(sizeof(double) * n);
+        double *ez = (double *)malloc(sizeof(double) * n);
+        for (int i = 0; i < n; ++i) z[i] = 50.0 * s[i] - 0.5 + 0.3 * x[i];
+        for (int crit = 3; crit <= 50; crit += 47) {
+            orc3_opts o3;
+            orc3_default_opts(&o3);
+            o3.crit_num = crit;
+            orc3_tree *t3 = NULL;
+            CHECK(orc3_build(&t3, n, x, y, z, a, &o3) == 0, "3-D tree");
+            const int nc = orc3_ncells(t3);
+            int *lv = (int *)malloc(sizeof(int) * nc), *lst = (int *)malloc(sizeof(int) * n);
+            int64_t *key = (int64_t *)malloc(sizeof(int64_t) * nc);
+            double *cen = (double *)malloc(sizeof(double) * 3 * nc);
+            orc3_cells(t3, lv, key, NULL, NULL, NULL, cen, NULL);
+            for (int c = 0; c < nc; ++c) CHECK(orc3_leaf_particles(t3, c, lst, n) >= 0, "3-D leaf");
+            free(lv); free(lst); free(key); free(cen);
+            orc3_free(t3);
+            CHECK(orc3_velocity(n, x, y, z, s, q, w, q, a, &o3, 1.0, 0.0, 0.5, ex, ey, ez) == 0,
+                  "3-D velocity");
+            CHECK(finite_all(ex, n) && finite_all(ey, n) && finite_all(ez, n), "3-D velocity finite");
+        }
+        orc3_direct(n, x, y, z, q, w, q, a, n / 3, tidx, ex, ey, ez);
+        CHECK(finite_all(ez, n / 3), "3-D direct");
+        free(z); free(ez);
     }
     free(x); free(y); free(q); free(s); free(w); free(a); free(ex); free(ey); free(tidx);
     return 0;
