@@ -258,6 +258,79 @@ int vpmfmm_velocity_type1_device(vpmfmm_handle *h, const vpmfmm_type1_opts *o, i
                                  const double *x, const double *y, const double *s,
                                  const double *gz, double *u, double *v);
 
+/* ------------------------------------------------------------------------
+ * The 3-D vector FMM (the reference's default 3-D velocity, VELOCITY_3D_TYPE
+ * 1: VelocityCalc::velocity_fmm_3d_fast, velocity_calc.cpp:2384-2525 ->
+ * fmm3D::calcVelocityFast, src/FMM/fast3DFMM.cpp:32-92, helpers
+ * src/FMM/fmm3D.cpp:84-210).  FP64.
+ *
+ * Octree of FMMCell::generateTree (fast3DFMM.cpp:485-745); order-2 Cartesian
+ * moments of the vortex strengths alpha = omega s^3 / (4 pi); every particle
+ * is a target; a cell whose centre is farther than support * side from the
+ * target adds its moments' term, a nearer leaf adds the direct sum
+ *   u += alpha_j x (x_i - x_j) / |x_i - x_j|^3
+ * over its active sources (the target itself excluded).  As in the
+ * reference, the walk starts at the children of the level-1 cells, so the
+ * sources of a level-1 leaf (an octant of the root holding fewer than
+ * crit_num particles) are never summed.
+ * ---------------------------------------------------------------------- */
+typedef struct vpmfmm_fmm3d_opts {
+    int32_t crit_num;   /* a cell with fewer particles is a leaf (Pars::src_count_max,
+                           global.cpp:313: 50), 1..4096                               */
+    int32_t max_depth;  /* refuse deeper trees, 1..20 (the reference has no bound;
+                           at least crit_num coincident particles never separate)    */
+    double exp_tree;    /* root expansion in percent (Pars::expTree = 1.0)           */
+    double center[3];   /* a point the root box always holds (Pars::x/y/zcenter = 0) */
+    double support;     /* opening factor (SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3: 2.0)  */
+    double u_inf, v_inf, w_inf;  /* free stream added to every velocity (get_velocity,
+                           velocity_calc.cpp:221-236)                                */
+    int32_t reserved[4];
+} vpmfmm_fmm3d_opts;
+
+void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o);
+
+/* (u, v, w) of every particle from positions (x, y, z), sizes s, vorticity
+ * (wx, wy, wz) and active flags (inactive particles are targets, not
+ * sources).  Host pointers for vpmfmm_velocity3d, device pointers (on the
+ * handle's device) for vpmfmm_velocity3d_device.  VPMFMM_NONFINITE_INPUT for
+ * a non-finite coordinate, VPMFMM_TREE_DEPTH_EXCEEDED for a tree deeper than
+ * max_depth. */
+int vpmfmm_velocity3d(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n, const double *x,
+                      const double *y, const double *z, const double *s, const double *wx,
+                      const double *wy, const double *wz, const uint8_t *active, double *u,
+                      double *v, double *w);
+int vpmfmm_velocity3d_device(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n,
+                             const double *x, const double *y, const double *z, const double *s,
+                             const double *wx, const double *wy, const double *wz,
+                             const uint8_t *active, double *u, double *v, double *w);
+
+/* The FMM tool's own entry (fmm3D::calcVelocityFast, fast3DFMM.cpp:32-92):
+ * the rotational velocity alone (the free stream of *o is not added) from the
+ * vortex strengths (ax, ay, az) themselves.  Host pointers. */
+int vpmfmm_velocity3d_alpha(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n,
+                            const double *x, const double *y, const double *z, const double *ax,
+                            const double *ay, const double *az, const uint8_t *active, double *u,
+                            double *v, double *w);
+
+/* What the last 3-D solve built and evaluated. */
+typedef struct vpmfmm_fmm3d_stats {
+    int32_t n, n_active, n_cells, n_leaves, depth;  /* cells exclude the root      */
+    int32_t sorted_levels;     /* octree levels the particle sort resolved (10, 20) */
+    double root_centre[3], root_side;
+    uint64_t pairs;            /* direct (target, source) pairs                     */
+    uint64_t m2p;              /* (target, far cell) moment terms                   */
+} vpmfmm_fmm3d_stats;
+
+int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st);
+
+/* The cells of the last 3-D solve (n_cells entries each, any pointer may be
+ * NULL), depth-first: level, path key (3 bits per level, octant x + 2y + 4z,
+ * level 1 highest), particle count, leaf, active, centre (3 per cell), side.
+ * vpmfmm_fmm3d_perm: the particles in the tree's order (n entries). */
+int vpmfmm_fmm3d_cells(vpmfmm_handle *h, int32_t *level, int64_t *key, int32_t *count,
+                       uint8_t *leaf, uint8_t *active, double *centre3, double *side);
+int vpmfmm_fmm3d_perm(vpmfmm_handle *h, int32_t *perm);
+
 /* Advection schemes of advection::main_advection (main_advection.cpp:6-39). */
 typedef enum vpmfmm_advect_scheme {
     VPMFMM_ADVECT_EULER = 0, /* advection_euler, :44-61 (the scheme the reference compiles) */

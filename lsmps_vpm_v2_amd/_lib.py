@@ -18,7 +18,9 @@ EXPORTS = (
     "vpmfmm_advect_device",
     "vpmfmm_step_device", "vpmfmm_direct_device", "vpmfmm_potential",
     "vpmfmm_potential_device", "vpmfmm_type1_default_opts", "vpmfmm_velocity_type1",
-    "vpmfmm_velocity_type1_device",
+    "vpmfmm_velocity_type1_device", "vpmfmm_fmm3d_default_opts", "vpmfmm_velocity3d",
+    "vpmfmm_velocity3d_device", "vpmfmm_velocity3d_alpha", "vpmfmm_fmm3d_last_stats",
+    "vpmfmm_fmm3d_cells", "vpmfmm_fmm3d_perm",
     "vpmfmm_last_stats", "vpmfmm_last_error", "vpmfmm_build_tree", "vpmfmm_tree_counts",
     "vpmfmm_tree_geometry", "vpmfmm_tree_cells", "vpmfmm_tree_leaves", "vpmfmm_tree_lists",
     "vpmfmm_tree_expansions", "vpmfmm_set_level_orders", "vpmfmm_set_selector",
@@ -57,6 +59,30 @@ class Type1Opts(C.Structure):
         ("n_s", C.c_int32), ("lmax", C.c_int32), ("ndp", C.c_int32), ("icutoff", C.c_int32),
         ("l2l_mode", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
+
+
+class Fmm3dOpts(C.Structure):
+    """vpmfmm_fmm3d_opts: the 3-D treecode's knobs (fast3DFMM.cpp, fmm3D.cpp:3,
+    global.cpp:140-166, 303, 313)."""
+    _fields_ = [
+        ("crit_num", C.c_int32), ("max_depth", C.c_int32), ("exp_tree", C.c_double),
+        ("center", C.c_double * 3), ("support", C.c_double), ("u_inf", C.c_double),
+        ("v_inf", C.c_double), ("w_inf", C.c_double), ("reserved", C.c_int32 * 4),
+    ]
+
+
+class Fmm3dStats(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("n_active", C.c_int32), ("n_cells", C.c_int32),
+        ("n_leaves", C.c_int32), ("depth", C.c_int32), ("sorted_levels", C.c_int32),
+        ("root_centre", C.c_double * 3), ("root_side", C.c_double), ("pairs", C.c_uint64),
+        ("m2p", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["root_centre"] = tuple(self.root_centre)
+        return d
 
 
 class Selector(C.Structure):
@@ -143,6 +169,14 @@ def load() -> C.CDLL:
     for name in ("vpmfmm_velocity_type1", "vpmfmm_velocity_type1_device"):
         getattr(L, name).argtypes = [_vp, C.POINTER(Type1Opts), C.c_int32, _vp, _vp, _vp, _vp,
                                      _vp, _vp]
+    L.vpmfmm_fmm3d_default_opts.argtypes = [C.POINTER(Fmm3dOpts)]
+    L.vpmfmm_fmm3d_default_opts.restype = None
+    for name in ("vpmfmm_velocity3d", "vpmfmm_velocity3d_device"):
+        getattr(L, name).argtypes = [_vp, C.POINTER(Fmm3dOpts), C.c_int32] + [_vp] * 11
+    L.vpmfmm_velocity3d_alpha.argtypes = [_vp, C.POINTER(Fmm3dOpts), C.c_int32] + [_vp] * 10
+    L.vpmfmm_fmm3d_last_stats.argtypes = [_vp, C.POINTER(Fmm3dStats)]
+    L.vpmfmm_fmm3d_cells.argtypes = [_vp] * 8
+    L.vpmfmm_fmm3d_perm.argtypes = [_vp, _vp]
     L.vpmfmm_advect_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_double,
                                        C.c_int32]
     L.vpmfmm_step_device.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

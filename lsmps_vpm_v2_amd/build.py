@@ -3,7 +3,8 @@
 Usage: ``python -m lsmps_vpm_v2_amd.build`` (or ``python lsmps_vpm_v2_amd/build.py``).
 Every source is compiled for gfx950 only, with packed-FP32 instruction
 selection disabled; tree.hip additionally disables FMA contraction so cell
-geometry rounds exactly as the reference's FP64 arithmetic.
+geometry rounds exactly as the reference's FP64 arithmetic; so do advect.hip
+and fmm3d.hip (whose sums opt back in to contraction block by block).
 """
 from __future__ import annotations
 
@@ -41,6 +42,7 @@ SOURCES = {
     "fmm32.hip": [],
     "select.hip": [],
     "advect.hip": ["-ffp-contract=off"],
+    "fmm3d.hip": ["-ffp-contract=off"],
     "api.cpp": [],
 }
 

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -11,6 +12,7 @@
 #include "../../include/vpmfmm.h"
 #include "ctx.hpp"
 #include "fmm32.hpp"
+#include "fmm3d.hpp"
 #include "select.hpp"
 
 namespace vpm {
@@ -210,6 +212,8 @@ struct vpmfmm_handle_s {
     // per-level orders of a type-1 tree (list statistics)
     int t1_nd = -1, t1_mode = -1;
     uint8_t t1_orders[kMaxDepth + 2] = {};
+    // the 3-D treecode's buffers (made by its first call)
+    std::unique_ptr<Ctx3> c3;
 };
 
 namespace {
@@ -1169,6 +1173,142 @@ int vpmfmm_velocity_type1(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t 
         VPM_HIP(hipMemcpyAsync(u, ds, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipMemcpyAsync(v, dg, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o) {
+    if (!o) return;
+    *o = vpmfmm_fmm3d_opts{};
+    o->crit_num = 50;        // FMMCell::critNum = Pars::src_count_max (global.cpp:313)
+    o->max_depth = kFmm3Depth;
+    o->exp_tree = 1.0;       // Pars::expTree (global.cpp:303)
+    o->support = 2.0;        // SUPPORT_RADIUS_FACTOR (fmm3D.cpp:3)
+}
+
+namespace {
+Fmm3Params fmm3_params(const vpmfmm_fmm3d_opts &o) {
+    Fmm3Params p;
+    p.crit_num = o.crit_num;
+    p.max_depth = o.max_depth;
+    p.exp_tree = o.exp_tree;
+    for (int d = 0; d < 3; ++d) p.center[d] = o.center[d];
+    p.support = o.support;
+    p.u_inf = o.u_inf;
+    p.v_inf = o.v_inf;
+    p.w_inf = o.w_inf;
+    return p;
+}
+
+Ctx3 &ctx3(vpmfmm_handle *h) {
+    if (!h->c3) h->c3.reset(new Ctx3());
+    return *h->c3;
+}
+}  // namespace
+
+int vpmfmm_velocity3d_device(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n,
+                             const double *x, const double *y, const double *z, const double *s,
+                             const double *wx, const double *wy, const double *wz,
+                             const uint8_t *active, double *u, double *v, double *w) {
+    return guarded(h, [&] {
+        if (!o) throw Error(kBadArg, "null options");
+        VPM_HIP(hipSetDevice(h->o.device));
+        if (!s) throw Error(kBadArg, "null pointer");
+        fmm3d_velocity(ctx3(h), h->ctx.stream, n, x, y, z, s, wx, wy, wz, active, fmm3_params(*o),
+                       u, v, w);
+    });
+}
+
+namespace {
+// Host-pointer 3-D solve: s null passes (wx, wy, wz) as the strengths and
+// adds no free stream.
+void velocity3d_host(vpmfmm_handle *h, const vpmfmm_fmm3d_opts &o, int32_t n, const double *x,
+                     const double *y, const double *z, const double *s, const double *wx,
+                     const double *wy, const double *wz, const uint8_t *active, double *u,
+                     double *v, double *w) {
+    {
+        if (n < 1 || !x || !y || !z || !wx || !wy || !wz || !active || !u || !v || !w)
+            throw Error(kBadArg, "bad size or null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        Ctx &c = h->ctx;
+        const size_t b = sizeof(double) * (size_t)n;
+        // staging: seven inputs, then the three outputs in the first three slots
+        double *st = c.x_in.as<double>(7 * (size_t)n);
+        uint8_t *da = c.a_in.as<uint8_t>(n);
+        const double *src[7] = {x, y, z, s, wx, wy, wz};
+        for (int k = 0; k < 7; ++k)
+            if (src[k])
+                VPM_HIP(hipMemcpyAsync(st + (size_t)k * n, src[k], b, hipMemcpyHostToDevice,
+                                       c.stream));
+        VPM_HIP(hipMemcpyAsync(da, active, n, hipMemcpyHostToDevice, c.stream));
+        double *out = c.ex_out.as<double>(3 * (size_t)n);
+        Fmm3Params p = fmm3_params(o);
+        if (!s) p.u_inf = p.v_inf = p.w_inf = 0.0;
+        fmm3d_velocity(ctx3(h), c.stream, n, st, st + n, st + 2 * (size_t)n,
+                       s ? st + 3 * (size_t)n : nullptr, st + 4 * (size_t)n, st + 5 * (size_t)n,
+                       st + 6 * (size_t)n, da, p, out, out + n, out + 2 * (size_t)n);
+        VPM_HIP(hipMemcpyAsync(u, out, b, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipMemcpyAsync(v, out + n, b, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipMemcpyAsync(w, out + 2 * (size_t)n, b, hipMemcpyDeviceToHost, c.stream));
+        VPM_HIP(hipStreamSynchronize(c.stream));
+    }
+}
+}  // namespace
+
+int vpmfmm_velocity3d(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n, const double *x,
+                      const double *y, const double *z, const double *s, const double *wx,
+                      const double *wy, const double *wz, const uint8_t *active, double *u,
+                      double *v, double *w) {
+    return guarded(h, [&] {
+        if (!o) throw Error(kBadArg, "null options");
+        if (!s) throw Error(kBadArg, "null pointer");
+        velocity3d_host(h, *o, n, x, y, z, s, wx, wy, wz, active, u, v, w);
+    });
+}
+
+int vpmfmm_velocity3d_alpha(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n,
+                            const double *x, const double *y, const double *z, const double *ax,
+                            const double *ay, const double *az, const uint8_t *active, double *u,
+                            double *v, double *w) {
+    return guarded(h, [&] {
+        if (!o) throw Error(kBadArg, "null options");
+        velocity3d_host(h, *o, n, x, y, z, nullptr, ax, ay, az, active, u, v, w);
+    });
+}
+
+int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st) {
+    return guarded(h, [&] {
+        if (!st) throw Error(kBadArg, "null pointer");
+        *st = vpmfmm_fmm3d_stats{};
+        if (!h->c3) return;
+        const Fmm3Info &fi = h->c3->info;
+        st->n = fi.n;
+        st->n_active = fi.n_active;
+        st->n_cells = fi.n_cells;
+        st->n_leaves = fi.n_leaves;
+        st->depth = fi.depth;
+        st->sorted_levels = fi.sorted_levels;
+        for (int d = 0; d < 3; ++d) st->root_centre[d] = fi.root_c[d];
+        st->root_side = fi.root_len;
+        st->pairs = fi.pairs;
+        st->m2p = fi.m2p;
+    });
+}
+
+int vpmfmm_fmm3d_cells(vpmfmm_handle *h, int32_t *level, int64_t *key, int32_t *count,
+                       uint8_t *leaf, uint8_t *active, double *centre3, double *side) {
+    return guarded(h, [&] {
+        if (!h->c3) throw Error(kBadArg, "no 3-D solve has run on this handle");
+        VPM_HIP(hipSetDevice(h->o.device));
+        fmm3d_cells(*h->c3, h->ctx.stream, level, key, count, leaf, active, centre3, side);
+    });
+}
+
+int vpmfmm_fmm3d_perm(vpmfmm_handle *h, int32_t *perm) {
+    return guarded(h, [&] {
+        if (!h->c3) throw Error(kBadArg, "no 3-D solve has run on this handle");
+        if (!perm) throw Error(kBadArg, "null pointer");
+        VPM_HIP(hipSetDevice(h->o.device));
+        fmm3d_perm(*h->c3, h->ctx.stream, perm);
     });
 }
 

@@ -1,0 +1,745 @@
+// fmm3d.hip -- the 3-D vector treecode of the reference's default 3-D path
+// (VELOCITY_3D_TYPE 1, src/FMM/fast3DFMM.cpp with the fmm3D.cpp helpers) on
+// gfx950, FP64.
+//
+// Reference semantics kept exactly:
+//   * octree (FMMCell::generateTree, fast3DFMM.cpp:485-745): root box = the
+//     particles' box together with the domain centre, side = the longest
+//     extent times (1 + expTree / 100), centre = the box midpoint; a particle
+//     goes to octant (x > xc) + 2 (y > yc) + 4 (z > zc) of its cell; a child's
+//     centre is the parent's plus (size / 2) (bit - 1/2); a particle settles
+//     at the first level where its cell holds fewer than critNum particles;
+//     a cell is active when a particle of its subtree is active.  The keys
+//     below replay these comparisons and additions per particle, so every
+//     cell, centre and size is the reference's, bit for bit.
+//   * moments: the 10 order-2 Cartesian moments of each vorticity component
+//     about the cell centre, d = centre - particle, over active particles
+//     (P2M_calc, fmm3D.cpp:84-112).  The reference forms them at the leaves
+//     and shifts them upward (M2M_calc :114-145); the order-2 shift is exact,
+//     so here each cell shifts its leaves' moments directly (equal up to
+//     rounding, one launch instead of one per level).
+//   * evaluation (evaluateFMM, fast3DFMM.cpp:318-484): every particle is a
+//     target; the traversal starts at the children of the level-1 cells (a
+//     level-1 leaf's sources are never summed); an inactive cell is skipped
+//     with its subtree; a cell is far when |r|^2 > (2 size)^2 (the same
+//     rounded comparison) and then adds its M2P term (M2P_mul_calc,
+//     fmm3D.cpp:147-210); a near non-leaf cell is opened; a near leaf adds
+//     the direct Biot-Savart sum over its active sources, the target itself
+//     excluded.
+//
+// Mapping: cells are emitted in depth-first pre-order with a skip pointer
+// (the next cell after the subtree), so a traversal is a loop over one index.
+// One wavefront takes 64 consecutive sorted targets and walks the tree once
+// for all of them: a cell is visited when some lane has to test it, and each
+// lane keeps one bit per level saying whether it opened the cell of that
+// level on the current path (a lane tests a cell only when its bit of the
+// parent's level is set).  Cell records, moments and source particles are
+// wave-uniform loads (scalar cache).
+//
+// This file is compiled without FMA contraction: cell centres and the
+// opening test round as the reference's; the arithmetic of the sums opts
+// back in (contract(fast) blocks).
+#include <cmath>
+
+#include "fmm3d.hpp"
+#include "prim.hpp"
+
+namespace vpm {
+namespace {
+
+constexpr int kHalf = kFmm3Depth / 2;  // levels per key word
+constexpr unsigned kF3NonFinite = 1u, kF3Deeper = 2u, kF3Depth = 4u;
+constexpr int kMomStride = 32;
+constexpr int kEvalBlock = 256;
+
+struct __align__(16) Par3 {
+    double x, y, z, ax, ay, az, act, pad;
+};
+
+// meta: level (bits 0-7), leaf (bit 8), active (bit 9)
+struct __align__(16) Cell3 {
+    double cx, cy, cz, rc2;
+    uint32_t begin, end, skip, meta;
+};
+
+__device__ __forceinline__ uint64_t pfx(uint64_t k, int L) { return k >> (3 * (kFmm3Depth - L)); }
+
+// -------------------------------------------------------------------------
+// bounding box and root
+// -------------------------------------------------------------------------
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
+
+__global__ void __launch_bounds__(256)
+    k3_bbox(const double *__restrict__ x, const double *__restrict__ y,
+            const double *__restrict__ z, int n, double *__restrict__ part) {
+    __shared__ double sm[4][7];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double bad = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)gridDim.x * 256) {
+        const double p[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!isfinite(p[d])) {
+                bad = 1.0;
+                continue;
+            }
+            mn[d] = dmin(mn[d], p[d]);
+            mx[d] = dmax(mx[d], p[d]);
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = dmin(mn[d], __shfl_xor(mn[d], m, kWave));
+            mx[d] = dmax(mx[d], __shfl_xor(mx[d], m, kWave));
+        }
+        bad = dmax(bad, __shfl_xor(bad, m, kWave));
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        for (int d = 0; d < 3; ++d) {
+            sm[w][d] = mn[d];
+            sm[w][3 + d] = mx[d];
+        }
+        sm[w][6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double r = sm[0][threadIdx.x];
+        for (int q = 1; q < 4; ++q) {
+            const double o = sm[q][threadIdx.x];
+            r = (threadIdx.x < 3) ? dmin(r, o) : dmax(r, o);
+        }
+        part[(size_t)blockIdx.x * 8 + threadIdx.x] = r;
+    }
+}
+
+// Root of FMMCell::generateTree (fast3DFMM.cpp:507-531): the box starts at
+// the domain centre, side = max extent * (1 + expTree / 100).  Also clears
+// the header's counters.
+__global__ void __launch_bounds__(256)
+    k3_root(const double *__restrict__ part, int nb, double c0, double c1, double c2,
+            double exp_tree, Fmm3Hdr *__restrict__ h) {
+    __shared__ double sm[4][7];
+    double mn[3] = {c0, c1, c2}, mx[3] = {c0, c1, c2}, bad = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = dmin(mn[d], part[(size_t)b * 8 + d]);
+            mx[d] = dmax(mx[d], part[(size_t)b * 8 + 3 + d]);
+        }
+        bad = dmax(bad, part[(size_t)b * 8 + 6]);
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = dmin(mn[d], __shfl_xor(mn[d], m, kWave));
+            mx[d] = dmax(mx[d], __shfl_xor(mx[d], m, kWave));
+        }
+        bad = dmax(bad, __shfl_xor(bad, m, kWave));
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        for (int d = 0; d < 3; ++d) {
+            sm[w][d] = mn[d];
+            sm[w][3 + d] = mx[d];
+        }
+        sm[w][6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; ++q) {
+            for (int d = 0; d < 3; ++d) {
+                sm[0][d] = dmin(sm[0][d], sm[q][d]);
+                sm[0][3 + d] = dmax(sm[0][3 + d], sm[q][3 + d]);
+            }
+            sm[0][6] = dmax(sm[0][6], sm[q][6]);
+        }
+        double root = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double len = sm[0][3 + d] - sm[0][d];
+            root = (len < root) ? root : len;  // std::max(len, root)
+            h->root_c[d] = 0.5 * (sm[0][3 + d] + sm[0][d]);
+        }
+        root *= (1 + (exp_tree / 100));
+        h->root_len = root;
+        h->flags = sm[0][6] != 0.0 ? kF3NonFinite : 0u;
+        h->depth = 0;
+        h->n_leaves = 0;
+        h->pairs = 0;
+        h->m2p = 0;
+    }
+}
+
+// -------------------------------------------------------------------------
+// keys, sort, gather
+// -------------------------------------------------------------------------
+// The particle's octant path, level by level from the root (the octant from
+// the parent centre, then the child centre from the parent's).  hi holds
+// levels 1..10, lo levels 11..20; sk receives the word the first sort pass
+// orders (hi for a 10-level sort, lo for the first of a 20-level one).
+__global__ void __launch_bounds__(256)
+    k3_keys(const double *__restrict__ x, const double *__restrict__ y,
+            const double *__restrict__ z, int n, const Fmm3Hdr *__restrict__ h,
+            uint32_t *__restrict__ khi, uint32_t *__restrict__ klo, uint32_t *__restrict__ sk,
+            uint32_t *__restrict__ sv, int deep) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double cx = h->root_c[0], cy = h->root_c[1], cz = h->root_c[2], sz = h->root_len;
+    const double px = x[i], py = y[i], pz = z[i];
+    uint32_t hi = 0, lo = 0;
+    for (int L = 1; L <= kFmm3Depth; ++L) {
+        const uint32_t o = (px > cx ? 1u : 0u) | (py > cy ? 2u : 0u) | (pz > cz ? 4u : 0u);
+        if (L <= kHalf)
+            hi = (hi << 3) | o;
+        else
+            lo = (lo << 3) | o;
+        const double cl = sz / 2.0;
+        cx = cx + cl * ((double)(o & 1u) - 0.5);
+        cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
+        cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
+        sz = cl;
+    }
+    khi[i] = hi;
+    klo[i] = lo;
+    sk[i] = deep ? lo : hi;
+    sv[i] = (uint32_t)i;
+}
+
+// Second pass of a 20-level sort: the high words in the order of the first.
+__global__ void __launch_bounds__(256)
+    k3_regather(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ khi, int n,
+                uint32_t *__restrict__ sk, uint32_t *__restrict__ sv) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = perm[j];
+    sk[j] = khi[i];
+    sv[j] = i;
+}
+
+// Sorted particle records, 60-bit keys, active flags.  The vortex strength
+// is formed as velocity_fmm_3d_fast does (velocity_calc.cpp:2429-2431):
+// omega * s^3 / (4 pi); without sizes (s null) (wx, wy, wz) are the
+// strengths.
+__global__ void __launch_bounds__(256)
+    k3_gather(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ khi,
+              const uint32_t *__restrict__ klo, const double *__restrict__ x,
+              const double *__restrict__ y, const double *__restrict__ z,
+              const double *__restrict__ s, const double *__restrict__ wx,
+              const double *__restrict__ wy, const double *__restrict__ wz,
+              const uint8_t *__restrict__ active, int n, Par3 *__restrict__ P,
+              uint64_t *__restrict__ key, uint8_t *__restrict__ act8) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = perm[j];
+    key[j] = ((uint64_t)khi[i] << 30) | (uint64_t)klo[i];
+    const bool a = active[i] != 0;
+    Par3 r;
+    r.x = x[i];
+    r.y = y[i];
+    r.z = z[i];
+    if (s) {
+        const double s3 = pow(s[i], 3.0);
+        const double k4pi = 4 * M_PI;
+        r.ax = wx[i] * s3 / k4pi;
+        r.ay = wy[i] * s3 / k4pi;
+        r.az = wz[i] * s3 / k4pi;
+    } else {  // the strengths themselves (calcVelocityFast's alpha)
+        r.ax = wx[i];
+        r.ay = wy[i];
+        r.az = wz[i];
+    }
+    r.act = a ? 1.0 : 0.0;
+    r.pad = 0.0;
+    P[j] = r;
+    act8[j] = a ? 1 : 0;
+}
+
+// -------------------------------------------------------------------------
+// leaf levels and cell emission
+// -------------------------------------------------------------------------
+// settle[i]: the first level whose cell around particle i holds fewer than
+// crit particles (the cell's range from two binary searches inside the
+// parent's range); mprev[i]: levels shared with particle i - 1; starts[i]:
+// the cells particle i opens in pre-order (levels mprev + 1 .. settle).
+__global__ void __launch_bounds__(256)
+    k3_settle(const uint64_t *__restrict__ key, int n, int D, int crit,
+              uint8_t *__restrict__ settle, uint8_t *__restrict__ mprev,
+              uint32_t *__restrict__ starts, Fmm3Hdr *__restrict__ h) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int st = 0, mp = 0;
+    bool unsettled = false;
+    if (i < n) {
+        const uint64_t ki = key[i];
+        int lo = 0, hi = n;
+        for (int L = 1; L <= D; ++L) {
+            const uint64_t p = pfx(ki, L);
+            int a = lo, b = i;
+            while (a < b) {
+                const int m = (a + b) >> 1;
+                if (pfx(key[m], L) < p) a = m + 1;
+                else b = m;
+            }
+            int e0 = i + 1, e1 = hi;
+            while (e0 < e1) {
+                const int m = (e0 + e1) >> 1;
+                if (pfx(key[m], L) > p) e1 = m;
+                else e0 = m + 1;
+            }
+            if (e0 - a < crit) {
+                st = L;
+                break;
+            }
+            lo = a;
+            hi = e0;
+        }
+        if (i > 0) {
+            const uint64_t d = key[i - 1] ^ ki;
+            mp = d ? (kFmm3Depth - 1) - (63 - __clzll((long long)d)) / 3 : D;
+            mp = min(mp, D);
+        }
+        unsettled = st == 0;
+        settle[i] = (uint8_t)st;
+        mprev[i] = (uint8_t)mp;
+        starts[i] = st > mp ? (uint32_t)(st - mp) : 0u;
+    }
+    int dmax_ = st;
+    for (int m = 1; m < kWave; m <<= 1) dmax_ = max(dmax_, __shfl_xor(dmax_, m, kWave));
+    const uint64_t leaves = __ballot(i < n && st > mp);
+    const bool any_unsettled = __ballot(unsettled) != 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicMax(&h->depth, (uint32_t)dmax_);
+        if (leaves) atomicAdd(&h->n_leaves, (uint32_t)__popcll(leaves));
+        if (any_unsettled) atomicOr(&h->flags, D < kFmm3Depth ? kF3Deeper : kF3Depth);
+    }
+}
+
+// Cells of particle i (levels mprev + 1 .. settle) at pre-order positions
+// pos[i] ...: centre (replayed from the root), (support size)^2, particle
+// range, skip pointer (the first cell of the first particle past the range),
+// level, leaf, active (an active particle in the range).
+__global__ void __launch_bounds__(256)
+    k3_emit(const uint64_t *__restrict__ key, const uint8_t *__restrict__ settle,
+            const uint8_t *__restrict__ mprev, const uint32_t *__restrict__ pos,
+            const uint32_t *__restrict__ A, int n, uint32_t ncell,
+            const Fmm3Hdr *__restrict__ h, double support, Cell3 *__restrict__ cells) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int st = settle[i], mp = mprev[i];
+    if (st <= mp) return;
+    const uint64_t ki = key[i];
+    double cx = h->root_c[0], cy = h->root_c[1], cz = h->root_c[2], sz = h->root_len;
+    uint32_t p = pos[i];
+    int hi = n;
+    for (int L = 1; L <= st; ++L) {
+        const uint32_t o = (uint32_t)(pfx(ki, L) & 7u);
+        const double cl = sz / 2.0;
+        cx = cx + cl * ((double)(o & 1u) - 0.5);
+        cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
+        cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
+        sz = cl;
+        if (L <= mp) continue;
+        const uint64_t pf = pfx(ki, L);
+        int e0 = i + 1, e1 = hi;
+        while (e0 < e1) {
+            const int m = (e0 + e1) >> 1;
+            if (pfx(key[m], L) > pf) e1 = m;
+            else e0 = m + 1;
+        }
+        hi = e0;
+        Cell3 c;
+        c.cx = cx;
+        c.cy = cy;
+        c.cz = cz;
+        const double rck = sz * support;  // R_check (fast3DFMM.cpp:376)
+        c.rc2 = rck * rck;
+        c.begin = (uint32_t)i;
+        c.end = (uint32_t)e0;
+        c.skip = e0 < n ? pos[e0] : ncell;
+        const bool act = A[e0] > A[i];
+        c.meta = (uint32_t)L | (L == st ? 0x100u : 0u) | (act ? 0x200u : 0u);
+        if (p < ncell) cells[p] = c;
+        ++p;
+    }
+}
+
+// -------------------------------------------------------------------------
+// moments
+// -------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// One wavefront per leaf of level >= 2 (the level-1 cells are never tested):
+// P2M of its active particles.
+__global__ void __launch_bounds__(256)
+    k3_p2m(const Cell3 *__restrict__ cells, int ncell, const Par3 *__restrict__ P,
+           double *__restrict__ mom) {
+#pragma clang fp contract(fast)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + threadIdx.x / kWave));
+    if (c >= ncell) return;
+    const Cell3 C = cells[c];
+    const int L = (int)(C.meta & 0xffu);
+    if (L < 2 || (C.meta & 0x300u) != 0x300u) return;
+    double m[30];
+#pragma unroll
+    for (int k = 0; k < 30; ++k) m[k] = 0.0;
+    for (uint32_t j = C.begin + lane; j < C.end; j += kWave) {
+        const Par3 q = P[j];
+        if (q.act == 0.0) continue;
+        const double dx = C.cx - q.x, dy = C.cy - q.y, dz = C.cz - q.z;
+        const double t[10] = {1.0,          dx,           dy,           dz,      dx * dx * 0.5,
+                              dy * dy * 0.5, dz * dz * 0.5, dx * dy,      dy * dz, dx * dz};
+        const double a[3] = {q.ax, q.ay, q.az};
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+            for (int k = 0; k < 10; ++k) m[cc * 10 + k] += a[cc] * t[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 30; ++k) m[k] = wave_sum(m[k]);
+    if (lane < 30) {
+        double v = m[0];
+#pragma unroll
+        for (int k = 1; k < 30; ++k) v = lane == k ? m[k] : v;
+        mom[(size_t)c * kMomStride + lane] = v;
+    }
+}
+
+// One wavefront per non-leaf cell of level >= 2: its leaves' moments shifted
+// to its centre (the order-2 shift of M2M_calc, applied from each leaf).
+__global__ void __launch_bounds__(256)
+    k3_m2m(const Cell3 *__restrict__ cells, int ncell, double *__restrict__ mom) {
+#pragma clang fp contract(fast)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + threadIdx.x / kWave));
+    if (c >= ncell) return;
+    const Cell3 C = cells[c];
+    const int L = (int)(C.meta & 0xffu);
+    if (L < 2 || (C.meta & 0x300u) != 0x200u) return;
+    double m[30];
+#pragma unroll
+    for (int k = 0; k < 30; ++k) m[k] = 0.0;
+    for (uint32_t d = (uint32_t)c + 1 + lane; d < C.skip; d += kWave) {
+        const Cell3 D = cells[d];
+        if ((D.meta & 0x300u) != 0x300u) continue;
+        const double dx = C.cx - D.cx, dy = C.cy - D.cy, dz = C.cz - D.cz;
+        const double *mc = mom + (size_t)d * kMomStride;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const double *q = mc + cc * 10;
+            double *o = m + cc * 10;
+            const double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            o[0] += q0;
+            o[1] += q1 + q0 * dx;
+            o[2] += q2 + q0 * dy;
+            o[3] += q3 + q0 * dz;
+            o[4] += q[4] + q0 * dx * dx * 0.5 + q1 * dx;
+            o[5] += q[5] + q0 * dy * dy * 0.5 + q2 * dy;
+            o[6] += q[6] + q0 * dz * dz * 0.5 + q3 * dz;
+            o[7] += q[7] + q0 * dx * dy + q1 * dy + q2 * dx;
+            o[8] += q[8] + q0 * dy * dz + q2 * dz + q3 * dy;
+            o[9] += q[9] + q0 * dx * dz + q1 * dz + q3 * dx;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 30; ++k) m[k] = wave_sum(m[k]);
+    if (lane < 30) {
+        double v = m[0];
+#pragma unroll
+        for (int k = 1; k < 30; ++k) v = lane == k ? m[k] : v;
+        mom[(size_t)c * kMomStride + lane] = v;
+    }
+}
+
+// -------------------------------------------------------------------------
+// evaluation
+// -------------------------------------------------------------------------
+// |r|^2 as the reference rounds it (no contraction: the opening test must
+// take the same side).
+__device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// The far-cell term: the moments against the Taylor coefficients of
+// r / |r|^3 at the target (M2P_mul_calc), crossed into the velocity.
+__device__ __forceinline__ void m2p_term(const double *__restrict__ M, double dx, double dy,
+                                         double dz, double R2, double &vu, double &vv,
+                                         double &vw) {
+#pragma clang fp contract(fast)
+    const double ir = rsqrt(R2);
+    const double i2 = ir * ir, i3 = i2 * ir, i5 = i3 * i2, i7 = i5 * i2;
+    const double a = 3.0 * i5, b = 15.0 * i7;
+    const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    const double xy = dx * dy, yz = dy * dz, xz = dx * dz, bxyz = b * xy * dz;
+    const double mx[10] = {dx * i3,          i3 - a * xx,          -a * xy,
+                           -a * xz,          (b * xx - 3.0 * a) * dx, (b * yy - a) * dx,
+                           (b * zz - a) * dx, (b * xx - a) * dy,   bxyz,
+                           (b * xx - a) * dz};
+    const double my[10] = {dy * i3,          -a * xy,           i3 - a * yy,
+                           -a * yz,          (b * xx - a) * dy, (b * yy - 3.0 * a) * dy,
+                           (b * zz - a) * dy, (b * yy - a) * dx, (b * yy - a) * dz,
+                           bxyz};
+    const double mz[10] = {dz * i3,           -a * xz,           -a * yz,
+                           i3 - a * zz,       (b * xx - a) * dz, (b * yy - a) * dz,
+                           (b * zz - 3.0 * a) * dz, bxyz,        (b * zz - a) * dy,
+                           (b * zz - a) * dx};
+    double su = 0.0, sv = 0.0, sw = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        const double Vx = M[k], Vy = M[10 + k], Vz = M[20 + k];
+        su += Vy * mz[k] - Vz * my[k];
+        sv += Vz * mx[k] - Vx * mz[k];
+        sw += Vx * my[k] - Vy * mx[k];
+    }
+    vu += su;
+    vv += sv;
+    vw += sw;
+}
+
+__global__ void __launch_bounds__(kEvalBlock)
+    k3_eval(const Cell3 *__restrict__ cells, int ncell, const double *__restrict__ mom,
+            const Par3 *__restrict__ P, const uint32_t *__restrict__ perm, int n, double u_inf,
+            double v_inf, double w_inf, double *__restrict__ u, double *__restrict__ v,
+            double *__restrict__ w, Fmm3Hdr *__restrict__ h) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave =
+        __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kEvalBlock / kWave) + threadIdx.x / kWave));
+    const int i = wave * kWave + lane;
+    if (wave * kWave >= n) return;
+    const bool valid = i < n;
+    double xp = 0.0, yp = 0.0, zp = 0.0;
+    if (valid) {
+        xp = P[i].x;
+        yp = P[i].y;
+        zp = P[i].z;
+    }
+    double vu = 0.0, vv = 0.0, vw = 0.0;
+    uint32_t npair = 0, nfar = 0;
+    uint32_t open = 0;  // bit L: this lane opened the level-L cell of the current path
+    int c = 0;
+    while (c < ncell) {
+        const Cell3 C = cells[c];
+        const int L = (int)(C.meta & 0xffu);
+        if (L == 1) {
+            // level-1 cells are always opened and never tested; a level-1
+            // leaf is passed over
+            if (C.meta & 0x100u) {
+                c = (int)C.skip;
+            } else {
+                open = valid ? 2u : 0u;
+                c = c + 1;
+            }
+            continue;
+        }
+        if (!(C.meta & 0x200u)) {
+            c = (int)C.skip;
+            continue;
+        }
+        const bool test = (open >> (L - 1)) & 1u;
+        const double dx = xp - C.cx, dy = yp - C.cy, dz = zp - C.cz;
+        const double R2 = r2_exact(dx, dy, dz);
+        const bool far = test && R2 > C.rc2;
+        const bool near = test && !(R2 > C.rc2);
+        if (far) {
+            m2p_term(mom + (size_t)c * kMomStride, dx, dy, dz, R2, vu, vv, vw);
+            ++nfar;
+        }
+        const bool any_near = __ballot(near) != 0;
+        if (C.meta & 0x100u) {
+            if (any_near) {
+#pragma clang fp contract(fast)
+                for (uint32_t j = C.begin; j < C.end; ++j) {
+                    const Par3 q = P[j];
+                    if (q.act == 0.0) continue;
+                    if (near && j != (uint32_t)i) {
+                        const double ex = xp - q.x, ey = yp - q.y, ez = zp - q.z;
+                        const double r2 = ex * ex + ey * ey + ez * ez;
+                        const double ir = rsqrt(r2);
+                        const double i3 = ir * ir * ir;
+                        vu += (q.ay * ez - q.az * ey) * i3;
+                        vv += (q.az * ex - q.ax * ez) * i3;
+                        vw += (q.ax * ey - q.ay * ex) * i3;
+                        ++npair;
+                    }
+                }
+            }
+            c = (int)C.skip;
+        } else {
+            open = near ? (open | (1u << L)) : (open & ~(1u << L));
+            c = any_near ? c + 1 : (int)C.skip;
+        }
+    }
+    if (valid) {
+        const uint32_t o = perm[i];
+        u[o] = vu + u_inf;
+        v[o] = vv + v_inf;
+        w[o] = vw + w_inf;
+    }
+    unsigned long long sp = npair, sf = nfar;
+    for (int m = 1; m < kWave; m <<= 1) {
+        sp += __shfl_xor(sp, m, kWave);
+        sf += __shfl_xor(sf, m, kWave);
+    }
+    if (lane == 0) {
+        atomicAdd(&h->pairs, sp);
+        atomicAdd(&h->m2p, sf);
+    }
+}
+
+}  // namespace
+
+Ctx3::~Ctx3() {
+    if (hdr_h) (void)hipHostFree(hdr_h);
+}
+
+void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double *y,
+                    const double *z, const double *sz, const double *wx, const double *wy,
+                    const double *wz, const uint8_t *active, const Fmm3Params &p, double *u,
+                    double *v, double *w) {
+    if (n < 1) throw Error(kBadArg, "n must be >= 1");
+    if (!x || !y || !z || !wx || !wy || !wz || !active || !u || !v || !w)
+        throw Error(kBadArg, "null pointer");
+    if (p.crit_num < 1 || p.crit_num > 4096) throw Error(kBadArg, "crit_num must be in [1, 4096]");
+    if (p.max_depth < 1 || p.max_depth > kFmm3Depth)
+        throw Error(kBadArg, "max_depth must be in [1, 20]");
+    if (!(p.exp_tree >= 0.0) || !std::isfinite(p.exp_tree)) throw Error(kBadArg, "exp_tree must be >= 0");
+    if (!(p.support > 0.0) || !std::isfinite(p.support)) throw Error(kBadArg, "support must be > 0");
+    for (double q : {p.center[0], p.center[1], p.center[2], p.u_inf, p.v_inf, p.w_inf})
+        if (!std::isfinite(q)) throw Error(kBadArg, "centre and free stream must be finite");
+    if (!c.hdr_h) VPM_HIP(hipHostMalloc(&c.hdr_h, sizeof(Fmm3Hdr), hipHostMallocDefault));
+    c.hdr_d = c.hdr_buf.as<Fmm3Hdr>(1);
+    Fmm3Hdr *hd = c.hdr_d;
+    c.last = p;
+
+    const int nb = div_up(n, 256);
+    const int nbb = std::min(1024, div_up(n, 256 * 8));
+    double *part = c.bbox.as<double>(8 * (size_t)nbb);
+    k3_bbox<<<nbb, 256, 0, s>>>(x, y, z, n, part);
+    VPM_LAUNCH_CHECK();
+
+    uint32_t *khi = c.khi.as<uint32_t>(n), *klo = c.klo.as<uint32_t>(n);
+    uint32_t *ka = c.key_a.as<uint32_t>(n), *kb = c.key_b.as<uint32_t>(n);
+    uint32_t *va = c.val_a.as<uint32_t>(n), *vb = c.val_b.as<uint32_t>(n);
+    const size_t tmp_words = std::max(radix_tmp_words(n), scan_tmp_words((long long)n + 1)) + 64;
+    uint32_t *tmp = c.tmp.as<uint32_t>(tmp_words);
+    Par3 *P = c.par.as<Par3>(n);
+    uint64_t *key = c.key64.as<uint64_t>(n);
+    uint8_t *act8 = c.act8.as<uint8_t>(n);
+    uint32_t *A = c.act_scan.as<uint32_t>((size_t)n + 1);
+    uint8_t *settle = c.settle.as<uint8_t>(n), *mprev = c.mprev.as<uint8_t>(n);
+    uint32_t *starts = c.starts.as<uint32_t>(n), *pos = c.pos.as<uint32_t>((size_t)n + 1);
+
+    // The sort resolves 10 levels when the tree is that shallow, else 20; the
+    // choice is a function of the tree alone (a build sorted the other way is
+    // repeated), so the order inside a leaf, and every rounding after it,
+    // depends on the input only.
+    int D = c.levels_guess;
+    for (int attempt = 0;; ++attempt) {
+        const int deep = D > kHalf;
+        // (the root kernel also clears the header's flags and counters)
+        k3_root<<<1, 256, 0, s>>>(part, nbb, p.center[0], p.center[1], p.center[2], p.exp_tree,
+                                  hd);
+        k3_keys<<<nb, 256, 0, s>>>(x, y, z, n, hd, khi, klo, ka, va, deep);
+        const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 0, 3 * kHalf, tmp, s);
+        uint32_t *perm = alt ? vb : va;
+        if (deep) {
+            uint32_t *k2 = alt ? ka : kb, *v2 = alt ? va : vb;
+            uint32_t *k3 = alt ? kb : ka, *v3 = alt ? vb : va;
+            k3_regather<<<nb, 256, 0, s>>>(perm, khi, n, k2, v2);
+            const bool alt2 = radix_sort_pairs(k2, v2, k3, v3, n, 0, 3 * kHalf, tmp, s);
+            perm = alt2 ? v3 : v2;
+        }
+        c.perm = perm;
+        k3_gather<<<nb, 256, 0, s>>>(perm, khi, klo, x, y, z, sz, wx, wy, wz, active, n, P, key,
+                                     act8);
+        k3_settle<<<nb, 256, 0, s>>>(key, n, D, p.crit_num, settle, mprev, starts, hd);
+        scan_exclusive_u32(starts, pos, n, &hd->n_cells, true, tmp, s);
+        scan_exclusive_u8(act8, A, n, &hd->n_active, true, tmp, s);
+        VPM_LAUNCH_CHECK();
+        VPM_HIP(hipMemcpyAsync(c.hdr_h, hd, sizeof(Fmm3Hdr), hipMemcpyDeviceToHost, s));
+        VPM_HIP(hipStreamSynchronize(s));
+        const Fmm3Hdr &hh = *c.hdr_h;
+        if (hh.flags & kF3NonFinite) throw Error(kNonFinite, "non-finite particle coordinate");
+        if (hh.flags & kF3Depth)
+            throw Error(kDepth, "octree deeper than 20 levels (at least crit_num coincident particles?)");
+        const int want = ((hh.flags & kF3Deeper) || (int)hh.depth > kHalf) ? kFmm3Depth : kHalf;
+        if (want != D && attempt == 0) {
+            D = want;
+            c.levels_guess = want;
+            continue;
+        }
+        if (want != D) throw Error(kInternal, "3-D sort depth did not settle");
+        break;
+    }
+    const Fmm3Hdr hh = *c.hdr_h;
+    if ((int)hh.depth > p.max_depth) throw Error(kDepth, "octree deeper than max_depth");
+    const uint32_t ncell = hh.n_cells;
+    Fmm3Info &fi = c.info;
+    fi.n = n;
+    fi.n_active = (int)hh.n_active;
+    fi.n_cells = (int)ncell;
+    fi.n_leaves = (int)hh.n_leaves;
+    fi.depth = (int)hh.depth;
+    for (int d = 0; d < 3; ++d) fi.root_c[d] = hh.root_c[d];
+    fi.root_len = hh.root_len;
+    fi.sorted_levels = D;
+
+    Cell3 *cells = c.cells.as<Cell3>(ncell);
+    double *mom = c.mom.as<double>((size_t)ncell * kMomStride);
+    k3_emit<<<nb, 256, 0, s>>>(key, settle, mprev, pos, A, n, ncell, hd, p.support, cells);
+    const int cb = div_up(ncell, 4);
+    if (ncell > 0) {
+        k3_p2m<<<cb, 256, 0, s>>>(cells, (int)ncell, P, mom);
+        k3_m2m<<<cb, 256, 0, s>>>(cells, (int)ncell, mom);
+    }
+    const int waves = div_up(n, kWave);
+    k3_eval<<<div_up(waves, kEvalBlock / kWave), kEvalBlock, 0, s>>>(
+        cells, (int)ncell, mom, P, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, hd);
+    VPM_LAUNCH_CHECK();
+    VPM_HIP(hipMemcpyAsync(c.hdr_h, hd, sizeof(Fmm3Hdr), hipMemcpyDeviceToHost, s));
+    VPM_HIP(hipStreamSynchronize(s));
+    fi.pairs = c.hdr_h->pairs;
+    fi.m2p = c.hdr_h->m2p;
+}
+
+void fmm3d_cells(Ctx3 &c, hipStream_t s, int32_t *level, int64_t *key, int32_t *count,
+                 uint8_t *leaf, uint8_t *active, double *centre3, double *size) {
+    const int nc = c.info.n_cells, n = c.info.n;
+    if (n == 0) throw Error(kBadArg, "no 3-D solve has run on this handle");
+    std::vector<Cell3> cl((size_t)nc);
+    std::vector<uint64_t> k((size_t)n);
+    if (nc)
+        VPM_HIP(hipMemcpyAsync(cl.data(), c.cells.p, sizeof(Cell3) * nc, hipMemcpyDeviceToHost, s));
+    VPM_HIP(hipMemcpyAsync(k.data(), c.key64.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+    VPM_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nc; ++j) {
+        const Cell3 &q = cl[(size_t)j];
+        const int L = (int)(q.meta & 0xffu);
+        if (level) level[j] = L;
+        if (key) key[j] = (int64_t)(k[q.begin] >> (3 * (kFmm3Depth - L)));
+        if (count) count[j] = (int32_t)(q.end - q.begin);
+        if (leaf) leaf[j] = (q.meta >> 8) & 1u;
+        if (active) active[j] = (q.meta >> 9) & 1u;
+        if (centre3) {
+            centre3[3 * j] = q.cx;
+            centre3[3 * j + 1] = q.cy;
+            centre3[3 * j + 2] = q.cz;
+        }
+        if (size) size[j] = std::ldexp(c.info.root_len, -L);
+    }
+}
+
+void fmm3d_perm(Ctx3 &c, hipStream_t s, int32_t *perm) {
+    if (c.info.n == 0 || !c.perm) throw Error(kBadArg, "no 3-D solve has run on this handle");
+    VPM_HIP(hipMemcpyAsync(perm, c.perm, sizeof(uint32_t) * c.info.n, hipMemcpyDeviceToHost, s));
+    VPM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace vpm

@@ -50,6 +50,18 @@ def _dptr(t):
     return C.c_void_p(int(t.data_ptr()))
 
 
+def _torch_first():
+    """torch carries its own HIP runtime, which finds no GPU once the
+    library's runtime has initialised the device in the same process; when
+    torch is installed and sees a GPU, it initialises first."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.device_count() > 0:
+        torch.cuda.init()
+
+
 class FmmSolver:
     """One solver handle: persistent device buffers, one HIP stream."""
 
@@ -62,6 +74,7 @@ class FmmSolver:
         lets it pick p per level and the leaf threshold, with ``p`` as the
         fixed-order fallback.  icutoff 1 / 2 / 3 regularizes the near field
         as the reference's Pars::icutoff (regul_func_2d; velocity calls only)."""
+        _torch_first()
         self._L = _lib.load()
         o = _lib.default_opts()
         o.p, o.max_par, o.max_src, o.exp_tree = p, max_par, max_src, exp_tree
@@ -290,6 +303,84 @@ class FmmSolver:
             _dptr(gz), _dptr(u), _dptr(v)), "vpmfmm_velocity_type1_device")
         return u, v
 
+    # ------------------------------------------------------------- 3-D path
+    @staticmethod
+    def fmm3d_opts(crit_num=50, max_depth=20, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
+                   u_inf=0.0, v_inf=0.0, w_inf=0.0) -> "_lib.Fmm3dOpts":
+        """vpmfmm_fmm3d_opts; defaults are the reference's (critNum =
+        src_count_max 50, expTree 1.0, centre 0, SUPPORT_RADIUS_FACTOR 2.0,
+        w_inf 0: global.cpp:140-166, 303, 313; fmm3D.cpp:3)."""
+        o = _lib.Fmm3dOpts()
+        o.crit_num, o.max_depth, o.exp_tree, o.support = crit_num, max_depth, exp_tree, support
+        for d in range(3):
+            o.center[d] = center[d]
+        o.u_inf, o.v_inf, o.w_inf = u_inf, v_inf, w_inf
+        return o
+
+    def velocity3d(self, x, y, z, s, wx, wy, wz, active, **fmm3d):
+        """(u, v, w) of the 3-D treecode (velocity_fmm_3d_fast plus the free
+        stream of the keyword options) for host arrays."""
+        o = self.fmm3d_opts(**fmm3d)
+        arrs = [_f64(t) for t in (x, y, z, s, wx, wy, wz)]
+        a = _u8(active)
+        n = len(arrs[0])
+        u, v, w = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self._L.vpmfmm_velocity3d(self._h, C.byref(o), n, *(_ptr(t) for t in arrs),
+                                              _ptr(a), _ptr(u), _ptr(v), _ptr(w)),
+                    "vpmfmm_velocity3d")
+        return u, v, w
+
+    def velocity3d_alpha(self, x, y, z, ax, ay, az, active, **fmm3d):
+        """Rotational (u, v, w) from the vortex strengths themselves
+        (fmm3D::calcVelocityFast) for host arrays."""
+        o = self.fmm3d_opts(**fmm3d)
+        arrs = [_f64(t) for t in (x, y, z, ax, ay, az)]
+        a = _u8(active)
+        n = len(arrs[0])
+        u, v, w = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self._L.vpmfmm_velocity3d_alpha(self._h, C.byref(o), n,
+                                                    *(_ptr(t) for t in arrs), _ptr(a), _ptr(u),
+                                                    _ptr(v), _ptr(w)), "vpmfmm_velocity3d_alpha")
+        return u, v, w
+
+    def velocity3d_device(self, x, y, z, s, wx, wy, wz, active, u=None, v=None, w=None,
+                          **fmm3d):
+        """velocity3d on device tensors (float64 / uint8, contiguous)."""
+        o = self.fmm3d_opts(**fmm3d)
+        n = x.numel()
+        u = x.new_empty(n) if u is None else u
+        v = x.new_empty(n) if v is None else v
+        w = x.new_empty(n) if w is None else w
+        self._enter(x)
+        self._check(self._L.vpmfmm_velocity3d_device(
+            self._h, C.byref(o), n, *(_dptr(t) for t in (x, y, z, s, wx, wy, wz, active)),
+            _dptr(u), _dptr(v), _dptr(w)), "vpmfmm_velocity3d_device")
+        return u, v, w
+
+    def fmm3d_stats(self) -> dict:
+        st = _lib.Fmm3dStats()
+        self._check(self._L.vpmfmm_fmm3d_last_stats(self._h, C.byref(st)),
+                    "vpmfmm_fmm3d_last_stats")
+        return st.as_dict()
+
+    def fmm3d_cells(self) -> dict:
+        """Cells of the last 3-D solve in depth-first order."""
+        nc = self.fmm3d_stats()["n_cells"]
+        out = {"level": np.zeros(nc, np.int32), "key": np.zeros(nc, np.int64),
+               "count": np.zeros(nc, np.int32), "leaf": np.zeros(nc, np.uint8),
+               "active": np.zeros(nc, np.uint8), "centre": np.zeros((nc, 3)),
+               "size": np.zeros(nc)}
+        self._check(self._L.vpmfmm_fmm3d_cells(
+            self._h, *(_ptr(out[k]) for k in ("level", "key", "count", "leaf", "active",
+                                              "centre", "size"))), "vpmfmm_fmm3d_cells")
+        return out
+
+    def fmm3d_perm(self) -> np.ndarray:
+        n = self.fmm3d_stats()["n"]
+        perm = np.zeros(n, np.int32)
+        self._check(self._L.vpmfmm_fmm3d_perm(self._h, _ptr(perm)), "vpmfmm_fmm3d_perm")
+        return perm
+
     # ----------------------------------------------------------- device paths
     def field_device(self, x, y, q, active, ex=None, ey=None):
         """Device tensors in, device tensors out (float64 / uint8, contiguous)."""
@@ -446,6 +537,12 @@ class Particle:
     u: np.ndarray = field(default_factory=lambda: np.zeros(0))
     v: np.ndarray = field(default_factory=lambda: np.zeros(0))
     gz: np.ndarray | None = None
+    # 3-D fields (DIM 3): z, the vorticity vector, the z velocity
+    z: np.ndarray | None = None
+    vortx: np.ndarray | None = None
+    vorty: np.ndarray | None = None
+    vortz: np.ndarray | None = None
+    w: np.ndarray = field(default_factory=lambda: np.zeros(0))
 
     @property
     def num(self) -> int:
@@ -455,14 +552,26 @@ class Particle:
 class VelocityCalc:
     """``VelocityCalc`` with the FMM branches on the GPU: VELOCITY_2D_TYPE 2
     (``velocity_fmm_2d``, the default) and 1 (``velocity_old``, the type-1
-    FMM, with the type-1 knobs n_s, lmax, ndp, icutoff, l2l_mode)."""
+    FMM, with the type-1 knobs n_s, lmax, ndp, icutoff, l2l_mode); with
+    ``dim=3``, VELOCITY_3D_TYPE 1 (``velocity_fmm_3d_fast``, the knobs of
+    FmmSolver.fmm3d_opts; u_inf and v_inf default to the solver's)."""
 
-    def __init__(self, velocity_type: int = 2, type1: dict | None = None, **solver_opts):
-        if velocity_type not in (1, 2):
+    def __init__(self, velocity_type: int | None = None, type1: dict | None = None,
+                 dim: int = 2, fmm3d: dict | None = None, **solver_opts):
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        if velocity_type is None:  # VELOCITY_2D_TYPE 2, VELOCITY_3D_TYPE 1
+            velocity_type = 2 if dim == 2 else 1
+        if dim == 2 and velocity_type not in (1, 2):
             raise ValueError("velocity_type must be 1 (type-1 FMM) or 2 (FMM)")
+        if dim == 3 and velocity_type != 1:
+            raise ValueError("the 3-D velocity type must be 1 (velocity_fmm_3d_fast)")
+        self.dim = dim
         self.velocity_type = velocity_type
         self.type1 = dict(type1 or {})
         self._solver = FmmSolver(**solver_opts)
+        o = self._solver.opts
+        self.fmm3d = {"u_inf": o.u_inf, "v_inf": o.v_inf, **(fmm3d or {})}
 
     @property
     def solver(self) -> FmmSolver:
@@ -477,6 +586,11 @@ class VelocityCalc:
         treeCell.cpp:1245-1249).  Type 1 reads ``gz`` (vorticity s^2 when it
         is None, redistribute_particles.cpp:186) and ignores ``isActive``
         (velocity_calc.cpp:404-424: every particle is a source)."""
+        if self.dim == 3:
+            particle.u, particle.v, particle.w = self._solver.velocity3d(
+                particle.x, particle.y, particle.z, particle.s, particle.vortx, particle.vorty,
+                particle.vortz, particle.isActive, **self.fmm3d)
+            return
         if self.velocity_type == 1:
             gz = particle.gz if particle.gz is not None else \
                 np.asarray(particle.vorticity) * np.asarray(particle.s) ** 2
@@ -622,3 +736,35 @@ class fmm2D:  # noqa: N801 -- reference class name
 
     def get_Field_y(self):
         return self._ey.copy()
+
+
+class fmm3D:  # noqa: N801 -- reference class name
+    """``fmm3D``'s fast entry (fast3DFMM.hpp, fast3DFMM.cpp:32-92):
+    ``calcVelocityFast(parPos, activeFlag, targetFlag, alphaX, alphaY,
+    alphaZ)`` then ``get_Field_x/y/z()``.  As in the reference, every particle
+    is a target (``targetFlag`` is accepted and unused, fast3DFMM.cpp:341),
+    inactive particles are not sources, the result is the rotational part
+    alone, and the octree and opening knobs are the reference's
+    (``critNum`` = src_count_max, expTree, the domain centre,
+    SUPPORT_RADIUS_FACTOR), overridable as keyword arguments."""
+
+    def __init__(self, solver: FmmSolver | None = None, **fmm3d):
+        self._solver = solver or FmmSolver()
+        self.fmm3d = dict(fmm3d)
+        self.parField_x = self.parField_y = self.parField_z = np.zeros(0)
+
+    def calcVelocityFast(self, parPos, activeFlag, targetFlag, alphaX, alphaY, alphaZ):  # noqa: N802
+        P = np.asarray(parPos, dtype=np.float64)
+        if P.ndim != 2 or P.shape[1] != 3:
+            raise ValueError("parPos must be (n, 3)")
+        self.parField_x, self.parField_y, self.parField_z = self._solver.velocity3d_alpha(
+            P[:, 0], P[:, 1], P[:, 2], alphaX, alphaY, alphaZ, activeFlag, **self.fmm3d)
+
+    def get_Field_x(self):  # noqa: N802
+        return self.parField_x
+
+    def get_Field_y(self):  # noqa: N802
+        return self.parField_y
+
+    def get_Field_z(self):  # noqa: N802
+        return self.parField_z

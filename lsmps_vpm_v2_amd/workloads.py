@@ -201,3 +201,42 @@ def subsample(field: dict, n_targets: int, seed: int = 7) -> np.ndarray:
     if n_targets >= n:
         return np.arange(n, dtype=np.int32)
     return np.sort(rng.choice(n, n_targets, replace=False)).astype(np.int32)
+
+
+def vortex_ring(n_target: int = 780_000, radius: float = 1.0, sigma: float = 0.1,
+                gamma: float = 1.0, cutoff: float = 3.0, jitter: float = 0.0, seed: int = 6,
+                active_frac: float = 1.0) -> dict:
+    """3-D vortex ring: particles on a cubic lattice (spacing h, size s = h)
+    inside the torus of ring radius ``radius`` and tube radius cutoff*sigma
+    about the z = 0 circle, with the Gaussian core vorticity
+    gamma / (pi sigma^2) exp(-d^2 / sigma^2) along the azimuth (d = distance
+    to the core circle).  h is chosen so that about ``n_target`` particles
+    fall inside; ``jitter`` (in units of h) perturbs the lattice.  Returns x,
+    y, z, s, wx, wy, wz, active and n."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = cutoff * sigma
+    vol = 2.0 * np.pi * radius * np.pi * a * a
+    h = (vol / n_target) ** (1.0 / 3.0)
+    ext = radius + a
+    g = np.arange(-ext, ext + 0.5 * h, h)
+    gz = np.arange(-a, a + 0.5 * h, h)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rho = np.hypot(X, Y)
+    ring = np.abs(rho - radius) <= a
+    xs, ys = X[ring], Y[ring]
+    xs, zs = np.broadcast_arrays(xs[:, None], gz[None, :])
+    ys = np.broadcast_to(ys[:, None], xs.shape)
+    x, y, z = xs.ravel().copy(), ys.ravel().copy(), zs.ravel().copy()
+    d2 = (np.hypot(x, y) - radius) ** 2 + z * z
+    keep = d2 <= a * a
+    x, y, z, d2 = x[keep], y[keep], z[keep], d2[keep]
+    if jitter:
+        x += jitter * h * rng.uniform(-0.5, 0.5, len(x))
+        y += jitter * h * rng.uniform(-0.5, 0.5, len(x))
+        z += jitter * h * rng.uniform(-0.5, 0.5, len(x))
+    r = np.hypot(x, y)
+    om = gamma / (np.pi * sigma * sigma) * np.exp(-d2 / (sigma * sigma))
+    n = len(x)
+    act = (rng.uniform(0, 1, n) < active_frac).astype(np.uint8)
+    return dict(x=x, y=y, z=z, s=np.full(n, h), wx=-om * y / r, wy=om * x / r,
+                wz=np.zeros(n), active=act, n=n, h=h, name=f"vortex_ring_{n}")

@@ -36,6 +36,24 @@ def test_header_declarations_are_exported():
     assert sorted(_lib.EXPORTS) == names
 
 
+def test_binding_argument_counts_match_header():
+    """Every function the binding declares argtypes for takes as many
+    arguments as its prototype in the header."""
+    _lib = _ensure_built()
+    L = _lib.load()
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    checked = 0
+    for name in declared_functions():
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", txt)
+        params = [a for a in m.group(1).split(",") if a.strip() and a.strip() != "void"]
+        fn = getattr(L, name)
+        if fn.argtypes is None:
+            continue
+        assert len(fn.argtypes) == len(params), name
+        checked += 1
+    assert checked >= 30
+
+
 def test_exported_symbols_are_c_linkage():
     _lib = _ensure_built()
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
@@ -61,7 +79,9 @@ def test_struct_layouts_match_header(tmp_path):
     _lib = _ensure_built()
     probe = tmp_path / "probe.c"
     structs = [("vpmfmm_opts", _lib.Opts), ("vpmfmm_stats", _lib.Stats),
-               ("vpmfmm_selector", _lib.Selector), ("vpmfmm_selection", _lib.Selection)]
+               ("vpmfmm_selector", _lib.Selector), ("vpmfmm_selection", _lib.Selection),
+               ("vpmfmm_type1_opts", _lib.Type1Opts), ("vpmfmm_fmm3d_opts", _lib.Fmm3dOpts),
+               ("vpmfmm_fmm3d_stats", _lib.Fmm3dStats)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vpmfmm.h"',
              "int main(void) {"]
     for cname, _ in structs:
