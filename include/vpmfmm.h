@@ -319,6 +319,9 @@ typedef struct vpmfmm_fmm3d_stats {
     double root_centre[3], root_side;
     uint64_t pairs;            /* direct (target, source) pairs                     */
     uint64_t m2p;              /* (target, far cell) moment terms                   */
+    uint64_t wave_pairs;       /* wavefront steps of the direct sums (64 lanes each;
+                                  pairs / (64 wave_pairs) is the lane use)          */
+    uint64_t wave_m2p;         /* wavefront steps of the moment terms               */
 } vpmfmm_fmm3d_stats;
 
 int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st);

@@ -1291,6 +1291,8 @@ int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st) {
         st->root_side = fi.root_len;
         st->pairs = fi.pairs;
         st->m2p = fi.m2p;
+        st->wave_pairs = fi.wave_pairs;
+        st->wave_m2p = fi.wave_m2p;
     });
 }
 

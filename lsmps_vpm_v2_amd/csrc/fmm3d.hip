@@ -56,10 +56,19 @@ struct __align__(16) Par3 {
     double x, y, z, ax, ay, az, act, pad;
 };
 
-// meta: level (bits 0-7), leaf (bit 8), active (bit 9)
+// meta: level (bits 0-7), leaf (bit 8), active (bit 9); [begin, end): the
+// cell's particles in tree order, [abegin, aend): its active ones in the
+// compacted source array
 struct __align__(16) Cell3 {
     double cx, cy, cz, rc2;
     uint32_t begin, end, skip, meta;
+    uint32_t abegin, aend, pad0, pad1;
+};
+
+// An active source in tree order: position, tree index (for the self test),
+// strength.
+struct __align__(16) Src3 {
+    double x, y, z, idx, ax, ay, az, pad;
 };
 
 __device__ __forceinline__ uint64_t pfx(uint64_t k, int L) { return k >> (3 * (kFmm3Depth - L)); }
@@ -171,6 +180,8 @@ __global__ void __launch_bounds__(256)
         h->n_leaves = 0;
         h->pairs = 0;
         h->m2p = 0;
+        h->wave_pairs = 0;
+        h->wave_m2p = 0;
     }
 }
 
@@ -261,59 +272,96 @@ __global__ void __launch_bounds__(256)
 // -------------------------------------------------------------------------
 // leaf levels and cell emission
 // -------------------------------------------------------------------------
-// settle[i]: the first level whose cell around particle i holds fewer than
-// crit particles (the cell's range from two binary searches inside the
-// parent's range); mprev[i]: levels shared with particle i - 1; starts[i]:
-// the cells particle i opens in pre-order (levels mprev + 1 .. settle).
+__device__ __forceinline__ int common_levels(uint64_t a, uint64_t b, int D) {
+    const uint64_t d = a ^ b;
+    const int m = d ? (kFmm3Depth - 1) - (63 - __clzll((long long)d)) / 3 : D;
+    return min(m, D);
+}
+
+// lcpw[j] (j <= n - crit): levels shared by the ends of the window of crit
+// sorted particles starting at j.  The cell of particle i at level L holds
+// crit particles or more exactly when some window containing i has
+// lcpw >= L (the keys are sorted, so a window's ends bound it).
 __global__ void __launch_bounds__(256)
-    k3_settle(const uint64_t *__restrict__ key, int n, int D, int crit,
-              uint8_t *__restrict__ settle, uint8_t *__restrict__ mprev,
-              uint32_t *__restrict__ starts, Fmm3Hdr *__restrict__ h) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    k3_lcpw(const uint64_t *__restrict__ key, int n, int D, int crit,
+            uint8_t *__restrict__ lcpw) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j > n - crit) return;
+    lcpw[j] = (uint8_t)common_levels(key[j], key[j + crit - 1], D);
+}
+
+// settle[i]: the first level whose cell around particle i holds fewer than
+// crit particles (1 + the largest lcpw of the windows containing i);
+// mprev[i]: levels shared with particle i - 1; starts[i]: the cells particle
+// i opens in pre-order (levels mprev + 1 .. settle).
+__global__ void __launch_bounds__(256)
+    k3_settle(const uint64_t *__restrict__ key, const uint8_t *__restrict__ lcpw, int n, int D,
+              int crit, uint8_t *__restrict__ settle, uint8_t *__restrict__ mprev,
+              uint32_t *__restrict__ starts, uint2 *__restrict__ bstat,
+              Fmm3Hdr *__restrict__ h) {
+    extern __shared__ uint8_t s_w[];
+    const int i0 = blockIdx.x * 256;
+    const int i = i0 + threadIdx.x;
+    // windows j in [i0 - crit + 1, i0 + 255] clipped to [0, n - crit]
+    const int j_lo = max(0, i0 - crit + 1), j_hi = min(i0 + 255, n - crit);
+    for (int j = j_lo + (int)threadIdx.x; j <= j_hi; j += 256) s_w[j - j_lo] = lcpw[j];
+    __syncthreads();
     int st = 0, mp = 0;
     bool unsettled = false;
     if (i < n) {
-        const uint64_t ki = key[i];
-        int lo = 0, hi = n;
-        for (int L = 1; L <= D; ++L) {
-            const uint64_t p = pfx(ki, L);
-            int a = lo, b = i;
-            while (a < b) {
-                const int m = (a + b) >> 1;
-                if (pfx(key[m], L) < p) a = m + 1;
-                else b = m;
-            }
-            int e0 = i + 1, e1 = hi;
-            while (e0 < e1) {
-                const int m = (e0 + e1) >> 1;
-                if (pfx(key[m], L) > p) e1 = m;
-                else e0 = m + 1;
-            }
-            if (e0 - a < crit) {
-                st = L;
-                break;
-            }
-            lo = a;
-            hi = e0;
-        }
-        if (i > 0) {
-            const uint64_t d = key[i - 1] ^ ki;
-            mp = d ? (kFmm3Depth - 1) - (63 - __clzll((long long)d)) / 3 : D;
-            mp = min(mp, D);
-        }
-        unsettled = st == 0;
+        int mx = 0;
+        const int a = max(j_lo, i - crit + 1), b = min(i, j_hi);
+        for (int j = a; j <= b; ++j) mx = max(mx, (int)s_w[j - j_lo]);
+        unsettled = mx >= D;
+        st = unsettled ? 0 : mx + 1;
+        if (i > 0) mp = common_levels(key[i - 1], key[i], D);
         settle[i] = (uint8_t)st;
         mprev[i] = (uint8_t)mp;
         starts[i] = st > mp ? (uint32_t)(st - mp) : 0u;
     }
+    // per-block depth and leaf count (k3_settle_stats sums them); the rare
+    // unsettled flag is the only atomic
+    __shared__ uint32_t s_d[4], s_l[4];
     int dmax_ = st;
     for (int m = 1; m < kWave; m <<= 1) dmax_ = max(dmax_, __shfl_xor(dmax_, m, kWave));
     const uint64_t leaves = __ballot(i < n && st > mp);
     const bool any_unsettled = __ballot(unsettled) != 0;
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicMax(&h->depth, (uint32_t)dmax_);
-        if (leaves) atomicAdd(&h->n_leaves, (uint32_t)__popcll(leaves));
+        s_d[threadIdx.x / kWave] = (uint32_t)dmax_;
+        s_l[threadIdx.x / kWave] = (uint32_t)__popcll(leaves);
         if (any_unsettled) atomicOr(&h->flags, D < kFmm3Depth ? kF3Deeper : kF3Depth);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        bstat[blockIdx.x] = make_uint2(max(max(s_d[0], s_d[1]), max(s_d[2], s_d[3])),
+                                       s_l[0] + s_l[1] + s_l[2] + s_l[3]);
+}
+
+__global__ void __launch_bounds__(1024)
+    k3_settle_stats(const uint2 *__restrict__ bstat, int nb, Fmm3Hdr *__restrict__ h) {
+    __shared__ uint32_t sd[16], sl[16];
+    uint32_t d = 0, l = 0;
+    for (int q = threadIdx.x; q < nb; q += 1024) {
+        d = max(d, bstat[q].x);
+        l += bstat[q].y;
+    }
+    for (int m = 1; m < kWave; m <<= 1) {
+        d = max(d, (uint32_t)__shfl_xor((int)d, m, kWave));
+        l += (uint32_t)__shfl_xor((int)l, m, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sd[threadIdx.x / kWave] = d;
+        sl[threadIdx.x / kWave] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t td = 0, tl = 0;
+        for (int q = 0; q < 16; ++q) {
+            td = max(td, sd[q]);
+            tl += sl[q];
+        }
+        h->depth = td;
+        h->n_leaves = tl;
     }
 }
 
@@ -359,11 +407,33 @@ __global__ void __launch_bounds__(256)
         c.begin = (uint32_t)i;
         c.end = (uint32_t)e0;
         c.skip = e0 < n ? pos[e0] : ncell;
+        c.abegin = A[i];
+        c.aend = A[e0];
+        c.pad0 = c.pad1 = 0u;
         const bool act = A[e0] > A[i];
         c.meta = (uint32_t)L | (L == st ? 0x100u : 0u) | (act ? 0x200u : 0u);
         if (p < ncell) cells[p] = c;
         ++p;
     }
+}
+
+// The active particles in tree order (A: exclusive scan of the flags).
+__global__ void __launch_bounds__(256)
+    k3_compact(const Par3 *__restrict__ P, const uint32_t *__restrict__ A, int n,
+               Src3 *__restrict__ S) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n || A[j + 1] == A[j]) return;
+    const Par3 q = P[j];
+    Src3 r;
+    r.x = q.x;
+    r.y = q.y;
+    r.z = q.z;
+    r.idx = (double)j;
+    r.ax = q.ax;
+    r.ay = q.ay;
+    r.az = q.az;
+    r.pad = 0.0;
+    S[A[j]] = r;
 }
 
 // -------------------------------------------------------------------------
@@ -375,14 +445,13 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One wavefront per leaf of level >= 2 (the level-1 cells are never tested):
+// One thread per leaf of level >= 2 (the level-1 cells are never tested):
 // P2M of its active particles.
 __global__ void __launch_bounds__(256)
-    k3_p2m(const Cell3 *__restrict__ cells, int ncell, const Par3 *__restrict__ P,
+    k3_p2m(const Cell3 *__restrict__ cells, int ncell, const Src3 *__restrict__ S,
            double *__restrict__ mom) {
 #pragma clang fp contract(fast)
-    const int lane = threadIdx.x & (kWave - 1);
-    const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + threadIdx.x / kWave));
+    const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= ncell) return;
     const Cell3 C = cells[c];
     const int L = (int)(C.meta & 0xffu);
@@ -390,9 +459,8 @@ __global__ void __launch_bounds__(256)
     double m[30];
 #pragma unroll
     for (int k = 0; k < 30; ++k) m[k] = 0.0;
-    for (uint32_t j = C.begin + lane; j < C.end; j += kWave) {
-        const Par3 q = P[j];
-        if (q.act == 0.0) continue;
+    for (uint32_t j = C.abegin; j < C.aend; ++j) {
+        const Src3 q = S[j];
         const double dx = C.cx - q.x, dy = C.cy - q.y, dz = C.cz - q.z;
         const double t[10] = {1.0,          dx,           dy,           dz,      dx * dx * 0.5,
                               dy * dy * 0.5, dz * dz * 0.5, dx * dy,      dy * dz, dx * dz};
@@ -402,14 +470,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int k = 0; k < 10; ++k) m[cc * 10 + k] += a[cc] * t[k];
     }
+    double *o = mom + (size_t)c * kMomStride;
 #pragma unroll
-    for (int k = 0; k < 30; ++k) m[k] = wave_sum(m[k]);
-    if (lane < 30) {
-        double v = m[0];
-#pragma unroll
-        for (int k = 1; k < 30; ++k) v = lane == k ? m[k] : v;
-        mom[(size_t)c * kMomStride + lane] = v;
-    }
+    for (int k = 0; k < 30; k += 2) *reinterpret_cast<double2 *>(o + k) = make_double2(m[k], m[k + 1]);
 }
 
 // One wavefront per non-leaf cell of level >= 2: its leaves' moments shifted
@@ -461,10 +524,27 @@ __global__ void __launch_bounds__(256)
 // -------------------------------------------------------------------------
 // evaluation
 // -------------------------------------------------------------------------
+constexpr int kEvalWaves = kEvalBlock / kWave;
+
+// Lanes of one wavefront read what other lanes of it wrote to LDS.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // |r|^2 as the reference rounds it (no contraction: the opening test must
 // take the same side).
 __device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
     return dx * dx + dy * dy + dz * dz;
+}
+
+// 1 / sqrt(x) for finite x > 0: the hardware estimate y0 refined by the
+// binomial series (1 - t)^(-1/2) = 1 + t/2 + 3 t^2/8 + ..., t = 1 - x y0^2.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+#pragma clang fp contract(fast)
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double t = fma(-x * y0, y0, 1.0);
+    return fma(y0 * t, fma(t, 0.375, 0.5), y0);
 }
 
 // The far-cell term: the moments against the Taylor coefficients of
@@ -473,7 +553,7 @@ __device__ __forceinline__ void m2p_term(const double *__restrict__ M, double dx
                                          double dz, double R2, double &vu, double &vv,
                                          double &vw) {
 #pragma clang fp contract(fast)
-    const double ir = rsqrt(R2);
+    const double ir = rsqrt_pos(R2);
     const double i2 = ir * ir, i3 = i2 * ir, i5 = i3 * i2, i7 = i5 * i2;
     const double a = 3.0 * i5, b = 15.0 * i7;
     const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
@@ -503,25 +583,45 @@ __device__ __forceinline__ void m2p_term(const double *__restrict__ M, double dx
     vw += sw;
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
+// b mod 8); this order hands each XCD one contiguous run of the targets, so
+// the cells and sources around them stay in that XCD's L2 (7.22 -> 6.82 ms
+// for the 780k-particle vortex ring).  A tail of fewer than 8 per XCD keeps
+// the plain order.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int per = nb / kXcds;
+    if (b >= per * kXcds) return b;
+    return (b % kXcds) * per + b / kXcds;
+}
+
+// A near leaf's active sources are read as wave-uniform loads (scalar
+// cache), four per batch.
 __global__ void __launch_bounds__(kEvalBlock)
     k3_eval(const Cell3 *__restrict__ cells, int ncell, const double *__restrict__ mom,
-            const Par3 *__restrict__ P, const uint32_t *__restrict__ perm, int n, double u_inf,
-            double v_inf, double w_inf, double *__restrict__ u, double *__restrict__ v,
-            double *__restrict__ w, Fmm3Hdr *__restrict__ h) {
+            const Par3 *__restrict__ P, const Src3 *__restrict__ S,
+            const uint32_t *__restrict__ perm, int n, double u_inf, double v_inf, double w_inf,
+            double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+            unsigned long long *__restrict__ wstat) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave =
-        __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kEvalBlock / kWave) + threadIdx.x / kWave));
+    const int wv = threadIdx.x / kWave;
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int wave = __builtin_amdgcn_readfirstlane(blk * kEvalWaves + wv);
     const int i = wave * kWave + lane;
     if (wave * kWave >= n) return;
     const bool valid = i < n;
     double xp = 0.0, yp = 0.0, zp = 0.0;
+    bool my_act = false;
     if (valid) {
         xp = P[i].x;
         yp = P[i].y;
         zp = P[i].z;
+        my_act = P[i].act != 0.0;
     }
+    const double di = (double)i;
     double vu = 0.0, vv = 0.0, vw = 0.0;
     uint32_t npair = 0, nfar = 0;
+    uint32_t wfar = 0, wpair = 0;  // wave-level steps (uniform)
     uint32_t open = 0;  // bit L: this lane opened the level-L cell of the current path
     int c = 0;
     while (c < ncell) {
@@ -547,27 +647,34 @@ __global__ void __launch_bounds__(kEvalBlock)
         const double R2 = r2_exact(dx, dy, dz);
         const bool far = test && R2 > C.rc2;
         const bool near = test && !(R2 > C.rc2);
-        if (far) {
-            m2p_term(mom + (size_t)c * kMomStride, dx, dy, dz, R2, vu, vv, vw);
-            ++nfar;
+        if (__ballot(far)) {
+            ++wfar;
+            if (far) {
+                m2p_term(mom + (size_t)c * kMomStride, dx, dy, dz, R2, vu, vv, vw);
+                ++nfar;
+            }
         }
         const bool any_near = __ballot(near) != 0;
         if (C.meta & 0x100u) {
             if (any_near) {
+                const uint32_t ab = C.abegin, ae = C.aend;
+                wpair += ae - ab;
+                if (near) {
 #pragma clang fp contract(fast)
-                for (uint32_t j = C.begin; j < C.end; ++j) {
-                    const Par3 q = P[j];
-                    if (q.act == 0.0) continue;
-                    if (near && j != (uint32_t)i) {
+#pragma unroll 4
+                    for (uint32_t j = ab; j < ae; ++j) {
+                        const Src3 q = S[j];
                         const double ex = xp - q.x, ey = yp - q.y, ez = zp - q.z;
                         const double r2 = ex * ex + ey * ey + ez * ez;
-                        const double ir = rsqrt(r2);
-                        const double i3 = ir * ir * ir;
+                        const double ir = rsqrt_pos(r2);
+                        const double i3 = q.idx != di ? ir * ir * ir : 0.0;
                         vu += (q.ay * ez - q.az * ey) * i3;
                         vv += (q.az * ex - q.ax * ez) * i3;
                         vw += (q.ax * ey - q.ay * ex) * i3;
-                        ++npair;
                     }
+                    // the target itself is a source of its own leaf when active
+                    const bool self = my_act && (uint32_t)i >= C.begin && (uint32_t)i < C.end;
+                    npair += ae - ab - (self ? 1u : 0u);
                 }
             }
             c = (int)C.skip;
@@ -582,14 +689,44 @@ __global__ void __launch_bounds__(kEvalBlock)
         v[o] = vv + v_inf;
         w[o] = vw + w_inf;
     }
+    // work counts per wavefront (summed by k3_eval_stats; same-address
+    // atomics from every wavefront would serialise)
     unsigned long long sp = npair, sf = nfar;
     for (int m = 1; m < kWave; m <<= 1) {
         sp += __shfl_xor(sp, m, kWave);
         sf += __shfl_xor(sf, m, kWave);
     }
     if (lane == 0) {
-        atomicAdd(&h->pairs, sp);
-        atomicAdd(&h->m2p, sf);
+        ulonglong4 r;
+        r.x = sp;
+        r.y = sf;
+        r.z = wpair;
+        r.w = wfar;
+        reinterpret_cast<ulonglong4 *>(wstat)[wave] = r;
+    }
+}
+
+// Sum of the per-wavefront work counts into the header.
+__global__ void __launch_bounds__(1024)
+    k3_eval_stats(const unsigned long long *__restrict__ wstat, int nw,
+                  Fmm3Hdr *__restrict__ h) {
+    __shared__ unsigned long long sm[16][4];
+    unsigned long long a[4] = {0, 0, 0, 0};
+    for (int q = threadIdx.x; q < nw; q += 1024)
+        for (int k = 0; k < 4; ++k) a[k] += wstat[4 * (size_t)q + k];
+    for (int m = 1; m < kWave; m <<= 1)
+        for (int k = 0; k < 4; ++k) a[k] += __shfl_xor(a[k], m, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int k = 0; k < 4; ++k) sm[threadIdx.x / kWave][k] = a[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 16; ++q)
+            for (int k = 0; k < 4; ++k) t[k] += sm[q][k];
+        h->pairs = t[0];
+        h->m2p = t[1];
+        h->wave_pairs = t[2];
+        h->wave_m2p = t[3];
     }
 }
 
@@ -659,7 +796,13 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
         c.perm = perm;
         k3_gather<<<nb, 256, 0, s>>>(perm, khi, klo, x, y, z, sz, wx, wy, wz, active, n, P, key,
                                      act8);
-        k3_settle<<<nb, 256, 0, s>>>(key, n, D, p.crit_num, settle, mprev, starts, hd);
+        uint8_t *lcpw = c.lcpw.as<uint8_t>(n);
+        if (n >= p.crit_num)
+            k3_lcpw<<<div_up(n - p.crit_num + 1, 256), 256, 0, s>>>(key, n, D, p.crit_num, lcpw);
+        uint2 *bstat = c.bstat.as<uint2>(nb);
+        k3_settle<<<nb, 256, (size_t)(256 + p.crit_num), s>>>(key, lcpw, n, D, p.crit_num,
+                                                                 settle, mprev, starts, bstat, hd);
+        k3_settle_stats<<<1, 1024, 0, s>>>(bstat, nb, hd);
         scan_exclusive_u32(starts, pos, n, &hd->n_cells, true, tmp, s);
         scan_exclusive_u8(act8, A, n, &hd->n_active, true, tmp, s);
         VPM_LAUNCH_CHECK();
@@ -693,20 +836,26 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
 
     Cell3 *cells = c.cells.as<Cell3>(ncell);
     double *mom = c.mom.as<double>((size_t)ncell * kMomStride);
+    Src3 *S = c.src.as<Src3>((size_t)std::max(1u, hh.n_active));
     k3_emit<<<nb, 256, 0, s>>>(key, settle, mprev, pos, A, n, ncell, hd, p.support, cells);
+    k3_compact<<<nb, 256, 0, s>>>(P, A, n, S);
     const int cb = div_up(ncell, 4);
     if (ncell > 0) {
-        k3_p2m<<<cb, 256, 0, s>>>(cells, (int)ncell, P, mom);
+        k3_p2m<<<div_up(ncell, 256), 256, 0, s>>>(cells, (int)ncell, S, mom);
         k3_m2m<<<cb, 256, 0, s>>>(cells, (int)ncell, mom);
     }
     const int waves = div_up(n, kWave);
-    k3_eval<<<div_up(waves, kEvalBlock / kWave), kEvalBlock, 0, s>>>(
-        cells, (int)ncell, mom, P, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, hd);
+    unsigned long long *wstat = c.wstat.as<unsigned long long>(4 * (size_t)waves);
+    k3_eval<<<div_up(waves, kEvalWaves), kEvalBlock, 0, s>>>(
+        cells, (int)ncell, mom, P, S, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, wstat);
+    k3_eval_stats<<<1, 1024, 0, s>>>(wstat, waves, hd);
     VPM_LAUNCH_CHECK();
     VPM_HIP(hipMemcpyAsync(c.hdr_h, hd, sizeof(Fmm3Hdr), hipMemcpyDeviceToHost, s));
     VPM_HIP(hipStreamSynchronize(s));
     fi.pairs = c.hdr_h->pairs;
     fi.m2p = c.hdr_h->m2p;
+    fi.wave_pairs = c.hdr_h->wave_pairs;
+    fi.wave_m2p = c.hdr_h->wave_m2p;
 }
 
 void fmm3d_cells(Ctx3 &c, hipStream_t s, int32_t *level, int64_t *key, int32_t *count,

@@ -41,12 +41,14 @@ struct Fmm3Hdr {
     uint32_t n_leaves;
     uint32_t pad;
     unsigned long long pairs, m2p;  // evaluation work: direct pairs, far-cell terms
+    // wavefront steps: sources staged for near lanes, far cells with a far lane
+    unsigned long long wave_pairs, wave_m2p;
 };
 
 struct Fmm3Info {
     int n = 0, n_active = 0, n_cells = 0, n_leaves = 0, depth = 0;
     double root_c[3] = {0, 0, 0}, root_len = 0;
-    unsigned long long pairs = 0, m2p = 0;
+    unsigned long long pairs = 0, m2p = 0, wave_pairs = 0, wave_m2p = 0;
     int sorted_levels = 0;  // levels the particle sort resolved (10 or 20)
 };
 
@@ -54,7 +56,8 @@ struct Ctx3 {
     Fmm3Hdr *hdr_d = nullptr;
     Fmm3Hdr *hdr_h = nullptr;  // pinned mirror
     DBuf hdr_buf, bbox, khi, klo, key_a, key_b, val_a, val_b, tmp;
-    DBuf par, key64, act8, act_scan, settle, mprev, starts, pos, cells, mom;
+    DBuf par, src, key64, act8, act_scan, lcpw, settle, mprev, starts, pos, cells, mom;
+    DBuf bstat, wstat;
     uint32_t *perm = nullptr;  // sorted slot -> caller index (val_a or val_b)
     int levels_guess = 10;     // the previous build's sort depth
     Fmm3Info info;
