@@ -275,15 +275,17 @@ int vpmfmm_velocity_type1_device(vpmfmm_handle *h, const vpmfmm_type1_opts *o, i
  * crit_num particles) are never summed.
  * ---------------------------------------------------------------------- */
 typedef struct vpmfmm_fmm3d_opts {
-    int32_t crit_num;   /* a cell with fewer particles is a leaf (Pars::src_count_max,
-                           global.cpp:313: 50), 1..4096                               */
+    int32_t crit_num;   /* a cell with fewer particles is a leaf (Pars::src_count_max
+                           of the DIM == 3 parameters, global.cpp:779: 30), 1..4096   */
     int32_t max_depth;  /* refuse deeper trees, 1..20 (the reference has no bound;
                            at least crit_num coincident particles never separate)    */
-    double exp_tree;    /* root expansion in percent (Pars::expTree = 1.0)           */
-    double center[3];   /* a point the root box always holds (Pars::x/y/zcenter = 0) */
+    double exp_tree;    /* root expansion in percent (Pars::expTree, global.cpp:769:
+                           1.0)                                                       */
+    double center[3];   /* a point the root box always holds (Pars::x/y/zcenter,
+                           global.cpp:609-611: 0)                                     */
     double support;     /* opening factor (SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3: 2.0)  */
     double u_inf, v_inf, w_inf;  /* free stream added to every velocity (get_velocity,
-                           velocity_calc.cpp:221-236)                                */
+                           velocity_calc.cpp:221-236; global.cpp:633-635: 1, 0, 0)   */
     int32_t reserved[4];
 } vpmfmm_fmm3d_opts;
 

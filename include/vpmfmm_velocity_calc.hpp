@@ -19,6 +19,11 @@
  * parameter so the header compiles against the reference's struct Particle
  * (Utils.hpp:77-159) or any type with the same members.  Define
  * VPMFMM_NO_GLOBAL_VELOCITYCALC to keep the class inside namespace vpmfmm.
+ * With DIM defined as 3 before this header (the reference's global.hpp:85),
+ * get_velocity takes the reference's 3-D branch instead: VELOCITY_3D_TYPE 1,
+ * velocity_fmm_3d_fast (velocity_calc.cpp:201-236, 2384-2525), reading
+ * particle.num, x, y, z, s, vortx, vorty, vortz, isActive and writing u, v, w
+ * (free stream and treecode knobs from set_default_fmm3d_options).
  * The whole reference program must see this one class: include it through
  * vpmfmm_reference_velocity_calc.hpp, which takes the place of the
  * reference's src/velocity_calculation/velocity_calc.hpp, the header both of
@@ -35,6 +40,14 @@
 #include <vector>
 
 #include "vpmfmm.h"
+
+// The dimension get_velocity serves: the reference's DIM (global.hpp:85) when
+// it is defined before this header, else 2.
+#if defined(DIM)
+#define VPMFMM_DIM DIM
+#else
+#define VPMFMM_DIM 2
+#endif
 
 namespace vpmfmm {
 
@@ -91,6 +104,13 @@ public:
                         const double *s, const double *gz, double *u, double *v) {
         throw_status(vpmfmm_velocity_type1(h_, &t, n, x, y, s, gz, u, v),
                      "vpmfmm_velocity_type1", h_);
+    }
+    // The 3-D treecode (u, v, w include the free stream of o).
+    void velocity3d(const vpmfmm_fmm3d_opts &o, int32_t n, const double *x, const double *y,
+                    const double *z, const double *s, const double *wx, const double *wy,
+                    const double *wz, const uint8_t *active, double *u, double *v, double *w) {
+        throw_status(vpmfmm_velocity3d(h_, &o, n, x, y, z, s, wx, wy, wz, active, u, v, w),
+                     "vpmfmm_velocity3d", h_);
     }
     vpmfmm_stats stats() const {
         vpmfmm_stats s;
@@ -155,6 +175,20 @@ inline void set_default_velocity_type(int type, const vpmfmm_type1_opts *type1 =
     if (type1) default_branch_slot().type1 = *type1;
 }
 
+// Options of the 3-D branch: the reference's DIM == 3 parameters (critNum =
+// src_count_max 30, expTree 1 %, centre 0, free stream (1, 0, 0):
+// global.cpp:609-611, 633-635, 769, 779; SUPPORT_RADIUS_FACTOR 2, fmm3D.cpp:3)
+// until set_default_fmm3d_options replaces them.  Read at each 3-D call.
+inline vpmfmm_fmm3d_opts &default_fmm3d_slot() {
+    static vpmfmm_fmm3d_opts o = [] {
+        vpmfmm_fmm3d_opts d;
+        vpmfmm_fmm3d_default_opts(&d);
+        return d;
+    }();
+    return o;
+}
+inline void set_default_fmm3d_options(const vpmfmm_fmm3d_opts &o) { default_fmm3d_slot() = o; }
+
 // One solver per host thread and option set, made at its first use and kept
 // for the life of the process (never destroyed: the HIP runtime may already
 // be gone when static destructors run).  Every VelocityCalc of that thread
@@ -183,7 +217,44 @@ public:
     }
 
     template <class ParticleT>
-    void get_velocity(ParticleT &particle, const int /*step*/) {
+    void get_velocity(ParticleT &particle, const int step) {
+#if VPMFMM_DIM == 3
+        velocity_fmm_3d_fast(particle, step);
+#else
+        (void)step;
+        get_velocity_2d(particle);
+#endif
+    }
+
+    // The 3-D branch (VELOCITY_3D_TYPE 1): velocity_fmm_3d_fast plus the free
+    // stream (velocity_calc.cpp:201-236); every particle is a target, the
+    // active ones are the sources.
+    template <class ParticleT>
+    void velocity_fmm_3d_fast(ParticleT &particle, const int /*step*/) {
+        const int n = particle.num;
+        if (n < 1) throw std::invalid_argument("VelocityCalc::get_velocity: no particles");
+        if ((int)particle.x.size() < n || (int)particle.y.size() < n ||
+            (int)particle.z.size() < n || (int)particle.s.size() < n ||
+            (int)particle.vortx.size() < n || (int)particle.vorty.size() < n ||
+            (int)particle.vortz.size() < n || (int)particle.isActive.size() < n)
+            throw std::invalid_argument("VelocityCalc::get_velocity: particle arrays shorter than num");
+        active_.resize(n);
+        for (int i = 0; i < n; ++i) active_[i] = particle.isActive[i] ? 1 : 0;
+        particle.u.clear();
+        particle.v.clear();
+        particle.w.clear();
+        particle.u.resize(n, 0.0);
+        particle.v.resize(n, 0.0);
+        particle.w.resize(n, 0.0);
+        solver().velocity3d(default_fmm3d_slot(), n, particle.x.data(), particle.y.data(),
+                            particle.z.data(), particle.s.data(), particle.vortx.data(),
+                            particle.vorty.data(), particle.vortz.data(), active_.data(),
+                            particle.u.data(), particle.v.data(), particle.w.data());
+    }
+
+private:
+    template <class ParticleT>
+    void get_velocity_2d(ParticleT &particle) {
         const VelocityBranch br = default_branch_slot();
         if (br.type == 1) {
             velocity_old(particle, br.type1);
@@ -206,6 +277,7 @@ public:
                           particle.v.data());
     }
 
+public:
     // VelocityCalc::velocity_old (velocity_calc.cpp:390-440) and get_velocity's
     // free stream: every particle (isActive is not read) with its size s and
     // vortex strength gz (Particle::gz) through the type-1 FMM.

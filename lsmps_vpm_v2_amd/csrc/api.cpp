@@ -1179,10 +1179,12 @@ int vpmfmm_velocity_type1(vpmfmm_handle *h, const vpmfmm_type1_opts *o, int32_t 
 void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o) {
     if (!o) return;
     *o = vpmfmm_fmm3d_opts{};
-    o->crit_num = 50;        // FMMCell::critNum = Pars::src_count_max (global.cpp:313)
+    // the reference's DIM == 3 parameter block (global.cpp:474-875)
+    o->crit_num = 30;        // FMMCell::critNum = Pars::src_count_max (global.cpp:779)
     o->max_depth = kFmm3Depth;
-    o->exp_tree = 1.0;       // Pars::expTree (global.cpp:303)
+    o->exp_tree = 1.0;       // Pars::expTree (global.cpp:769)
     o->support = 2.0;        // SUPPORT_RADIUS_FACTOR (fmm3D.cpp:3)
+    o->u_inf = 1.0;          // Pars::u_inf, v_inf, w_inf (global.cpp:633-635)
 }
 
 namespace {

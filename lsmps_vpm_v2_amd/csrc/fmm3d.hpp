@@ -22,7 +22,7 @@ namespace vpm {
 constexpr int kFmm3Depth = 20;
 
 struct Fmm3Params {
-    int crit_num = 50;       // a cell with fewer particles is a leaf
+    int crit_num = 30;       // a cell with fewer particles is a leaf
     int max_depth = kFmm3Depth;
     double exp_tree = 1.0;   // root expansion in percent
     double center[3] = {0.0, 0.0, 0.0};  // a point the root box always holds

@@ -305,11 +305,12 @@ class FmmSolver:
 
     # ------------------------------------------------------------- 3-D path
     @staticmethod
-    def fmm3d_opts(crit_num=50, max_depth=20, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
-                   u_inf=0.0, v_inf=0.0, w_inf=0.0) -> "_lib.Fmm3dOpts":
-        """vpmfmm_fmm3d_opts; defaults are the reference's (critNum =
-        src_count_max 50, expTree 1.0, centre 0, SUPPORT_RADIUS_FACTOR 2.0,
-        w_inf 0: global.cpp:140-166, 303, 313; fmm3D.cpp:3)."""
+    def fmm3d_opts(crit_num=30, max_depth=20, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
+                   u_inf=1.0, v_inf=0.0, w_inf=0.0) -> "_lib.Fmm3dOpts":
+        """vpmfmm_fmm3d_opts; defaults are the reference's DIM == 3 parameters
+        (critNum = src_count_max 30, expTree 1.0, centre 0, free stream
+        (1, 0, 0): global.cpp:609-611, 633-635, 769, 779) and
+        SUPPORT_RADIUS_FACTOR 2.0 (fmm3D.cpp:3)."""
         o = _lib.Fmm3dOpts()
         o.crit_num, o.max_depth, o.exp_tree, o.support = crit_num, max_depth, exp_tree, support
         for d in range(3):

@@ -19,9 +19,10 @@
 #define O3_TERMS 10
 
 void orc3_default_opts(orc3_opts *o) {
-    o->crit_num = 50;      /* FMMCell::critNum = Pars::src_count_max (global.cpp:313) */
-    o->exp_tree = 1.0;     /* global.cpp:303 */
-    o->center[0] = o->center[1] = o->center[2] = 0.0;  /* global.cpp:140-142 */
+    /* the reference's DIM == 3 parameter block (global.cpp:474-875) */
+    o->crit_num = 30;      /* FMMCell::critNum = Pars::src_count_max (global.cpp:779) */
+    o->exp_tree = 1.0;     /* global.cpp:769 */
+    o->center[0] = o->center[1] = o->center[2] = 0.0;  /* global.cpp:609-611 */
     o->support = 2.0;      /* SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3 */
     o->max_depth = 21;
 }

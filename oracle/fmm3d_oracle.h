@@ -29,7 +29,8 @@ extern "C" {
 #endif
 
 typedef struct {
-    int crit_num;       /* a cell with fewer particles is a leaf (Pars::src_count_max = 50) */
+    int crit_num;       /* a cell with fewer particles is a leaf (Pars::src_count_max of
+                           the DIM == 3 parameters = 30)                                   */
     double exp_tree;    /* root expansion in percent (Pars::expTree = 1.0)                 */
     double center[3];   /* the box always contains this point (Pars::x/y/zcenter = 0)      */
     double support;     /* opening factor: a cell is far when r > support * size

@@ -658,7 +658,7 @@ def ref_type1_velocity(x, y, s, g, n_s=50, lmax=7, ndp=10, icutoff=2):
 # --------------------------------------------------------------------------
 # the 3-D vector FMM (fmm3d_oracle.c; reference fast3DFMM.cpp)
 # --------------------------------------------------------------------------
-def o3_opts(crit_num=50, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
+def o3_opts(crit_num=30, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
             max_depth=21) -> O3Opts:
     o = O3Opts()
     lib().orc3_default_opts(C.byref(o))

@@ -143,20 +143,23 @@ def test_product_never_imports_the_oracle():
                 assert "fmm2d_oracle" not in src and "libref_fmm2d" not in src, fn
 
 
-def build_drop_in(tmp_path):
-    """Compile tests/cpp/drop_in_velocity.cpp against the C++ drop-in header."""
+def build_drop_in(tmp_path, name="drop_in_velocity"):
+    """Compile tests/cpp/<name>.cpp against the C++ drop-in header."""
     _lib = _ensure_built()
-    exe = tmp_path / "drop_in_velocity"
+    exe = tmp_path / name
     libdir = os.path.dirname(_lib.LIB_PATH)
     subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I",
                     os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "cpp", "drop_in_velocity.cpp"), f"-L{libdir}",
+                    os.path.join(ROOT, "tests", "cpp", name + ".cpp"), f"-L{libdir}",
                     "-lvpmfmm", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     return exe
 
 
 def test_cpp_drop_in_header_compiles(tmp_path):
     """The reference's call site get_velocity(particle, grid, step) compiles
-    against include/vpmfmm_velocity_calc.hpp with the reference's member types."""
+    against include/vpmfmm_velocity_calc.hpp with the reference's member types,
+    with DIM 2 and with DIM 3."""
     exe = build_drop_in(tmp_path)
     assert exe.exists()
+    exe3 = build_drop_in(tmp_path, "drop_in_velocity3d")
+    assert exe3.exists()

@@ -1,7 +1,7 @@
 """The C++ drop-in (include/vpmfmm_velocity_calc.hpp) on the GPU: the
 reference's call get_velocity(particle, grid, step) through a compiled C++
-program, on the FMM branch and the type-1 branch, checked against the
-oracle's velocities to 1e-10."""
+program, on the FMM branch, the type-1 branch and (compiled with DIM 3) the
+3-D branch, checked against the oracle's velocities to 1e-10."""
 import subprocess
 
 import numpy as np
@@ -33,3 +33,22 @@ def test_cpp_drop_in_velocity_matches_oracle(gpu, oracle, tmp_path):
     err1 = np.max(np.hypot(u1 - 1.0 - ut, v1 - vt)) / np.sqrt(np.mean(ut ** 2 + vt ** 2))
     print(f"drop-in type-1 vs oracle: max |d(u,v)| / rms = {err1:.2e}")
     assert err1 <= 1e-10
+
+
+def test_cpp_drop_in_3d_matches_oracle(gpu, oracle, tmp_path):
+    exe = build_drop_in(tmp_path, "drop_in_velocity3d")
+    out = tmp_path / "uvw.bin"
+    n = 8000
+    r = subprocess.run([str(exe), str(n), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(out, dtype=np.float64)
+    assert int(raw[0]) == n
+    x, y, z, s, wx, wy, wz, a, u, v, w = raw[1:].reshape(11, n)
+    # the reference's DIM == 3 defaults: critNum 30, free stream (1, 0, 0)
+    uo, vo, wo = oracle.o3_velocity(x, y, z, s, wx, wy, wz, a.astype(np.uint8),
+                                    oracle.o3_opts(), 1.0, 0.0, 0.0)
+    scale = np.sqrt(np.mean((uo - 1.0) ** 2 + vo ** 2 + wo ** 2))
+    err = np.max(np.sqrt((u - uo) ** 2 + (v - vo) ** 2 + (w - wo) ** 2)) / scale
+    print(f"3-D drop-in vs oracle: max |d(u,v,w)| / rms = {err:.2e}")
+    assert err <= 1e-10

@@ -44,7 +44,10 @@ def _cloud(n, seed, kind="uniform", spread=0.08):
     return p[0].copy(), p[1].copy(), p[2].copy(), s, w[0].copy(), w[1].copy(), w[2].copy()
 
 
-def _oracle(x, y, z, s, wx, wy, wz, act, crit=50, support=2.0, free=(0.0, 0.0, 0.0)):
+FREE = (1.0, 0.0, 0.0)  # the default free stream (global.cpp:633-635)
+
+
+def _oracle(x, y, z, s, wx, wy, wz, act, crit=30, support=2.0, free=FREE):
     from oracle import oracle as O
     o = O.o3_opts(crit_num=crit, support=support)
     return O.o3_velocity(x, y, z, s, wx, wy, wz, act, o, *free)
@@ -126,7 +129,7 @@ def test_vortex_ring_matches_oracle(solver):
     got = solver.velocity3d(f["x"], f["y"], f["z"], f["s"], f["wx"], f["wy"], f["wz"],
                             f["active"])
     ref = _oracle(f["x"], f["y"], f["z"], f["s"], f["wx"], f["wy"], f["wz"], f["active"])
-    e = _err(got, ref)
+    e = _err(got, ref, FREE)
     print(f"  err {e:.2e}")
     assert e <= TOL
     # the ring moves along +z: the mean axial velocity over the core is positive
@@ -174,8 +177,8 @@ def test_device_entry_and_determinism(solver):
 
 def test_small_and_level1_leaf_cases(solver):
     from oracle import oracle as O
-    x, y, z, s, wx, wy, wz = _cloud(40, 17)
-    act = np.ones(40, np.uint8)
+    x, y, z, s, wx, wy, wz = _cloud(25, 17)
+    act = np.ones(25, np.uint8)
     print("3-D fewer than crit_num particles", flush=True)
     u, v, w = solver.velocity3d(x, y, z, s, wx, wy, wz, act, u_inf=0.25, w_inf=-1.0)
     assert np.all(u == 0.25) and np.all(v == 0.0) and np.all(w == -1.0)
@@ -187,10 +190,10 @@ def test_small_and_level1_leaf_cases(solver):
     act = np.ones(len(keep), np.uint8)
     print("3-D level-1 leaf", flush=True)
     got = solver.velocity3d(*cols, act)
-    ref = O.o3_velocity(*cols, act)
-    assert _err(got, ref) <= TOL
+    ref = O.o3_velocity(*cols, act, O.o3_opts(), *FREE)
+    assert _err(got, ref, FREE) <= TOL
     one = solver.velocity3d(x[:1], y[:1], z[:1], s[:1], wx[:1], wy[:1], wz[:1],
-                            np.ones(1, np.uint8), v_inf=2.0)
+                            np.ones(1, np.uint8), u_inf=0.0, v_inf=2.0)
     assert one[0][0] == 0.0 and one[1][0] == 2.0 and one[2][0] == 0.0
 
 
@@ -216,7 +219,7 @@ def test_error_paths(solver):
     # the handle still works
     got = solver.velocity3d(x, y, z, s, wx, wy, wz, act)
     ref = _oracle(x, y, z, s, wx, wy, wz, act)
-    assert _err(got, ref) <= TOL
+    assert _err(got, ref, FREE) <= TOL
 
 
 def test_velocity_calc_dim3(solver):

@@ -179,7 +179,7 @@ def test_octree_invariants(kind, n, crit):
 
 def _level1_leaf_particles(T):
     out = []
-    for c in np.flatnonzero((T.level == 1) & (T.count < 50)):
+    for c in np.flatnonzero((T.level == 1) & (T.leaf == 1)):
         out.extend(T.particles(c).tolist())
     return np.array(out, int)
 
@@ -217,8 +217,8 @@ def test_level1_leaf_sources_are_not_summed():
     du, dv, dw = O.o3_direct(x, y, z, ax, ay, az, src.astype(np.uint8))
     assert _rel(np.r_[u, v, w], np.r_[du, dv, dw]) < 1e-13
     # fewer particles than crit_num: every level-1 cell is a leaf, nothing is summed
-    u, v, w = O.o3_velocity_alpha(x[:40], y[:40], z[:40], ax[:40], ay[:40], az[:40],
-                                  act[:40])
+    u, v, w = O.o3_velocity_alpha(x[:25], y[:25], z[:25], ax[:25], ay[:25], az[:25],
+                                  act[:25])
     assert not np.any(u) and not np.any(v) and not np.any(w)
 
 
