@@ -20,7 +20,9 @@ the reference's own CPU FMM (all particles).  The CPU baseline is that
 reference FMM (oracle/_ref, built from the reference sources) on this host's
 cores.  ``inviscid_loop`` (not the metric) times the loop step of
 vpmfmm_step_device on the same field: Euler advection, then the velocity at
-the moved particles, a new tree every step.
+the moved particles, a new tree every step.  ``fmm3d_path`` (not the
+metric) times the 3-D treecode on a 778,873-particle vortex ring
+(``--fmm3d-only`` prints that line alone).
 """
 from __future__ import annotations
 
@@ -289,6 +291,109 @@ def best_fixed(torch, FmmSolver, dev, tidx, ur, vr, u_inf, steps, warmup,
     return best, table
 
 
+# Flops of the 3-D evaluation kernel as it computes them (an FMA counts 2):
+# a direct pair 26 (offsets 3, |r|^2 5, 1/|r| 1, |r|^-3 2, cross product 9,
+# scale and sum 6); a moment term 198 (1/|r| 1, powers 4, two scales 2,
+# products 8, the 30 Taylor multipliers 60, the contraction 120, sum 3).
+FMM3D_FLOPS_PAIR = 26.0
+FMM3D_FLOPS_M2P = 198.0
+FMM3D_METRIC = "3-D treecode particle-velocities/sec at 780k particles (vortex ring)"
+
+
+def fmm3d_section(torch, steps, warmup, cpu, accuracy):
+    """The 3-D treecode (the reference's VELOCITY_3D_TYPE 1,
+    velocity_fmm_3d_fast) on the 778,873-particle vortex ring
+    (workloads.vortex_ring) with the reference's DIM 3 options: time per
+    solve with inputs in HBM, stage times and the evaluation kernel's
+    roofline from a solver that records HIP events, lane use of the
+    wavefront walk, the error against the 3-D direct sum (2,000 seeded
+    targets, oracle), and the oracle on the host cores (the CPU baseline,
+    whose velocities also check the GPU's to 1e-10)."""
+    from lsmps_vpm_v2_amd import FmmSolver, workloads
+    f = workloads.vortex_ring()
+    n = f["n"]
+    log(f"[bench] 3-D workload {f['name']} n={n}")
+    dev = [torch.from_numpy(np.ascontiguousarray(f[k])).cuda()
+           for k in ("x", "y", "z", "s", "wx", "wy", "wz")]
+    act = torch.from_numpy(f["active"]).cuda()
+    outs = [dev[0].new_empty(n) for _ in range(3)]
+    s = FmmSolver()
+    for _ in range(warmup):
+        s.velocity3d_device(*dev, act, *outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.velocity3d_device(*dev, act, *outs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = s.fmm3d_stats()
+    s_t = FmmSolver(timing=True)
+    ms = {"ms_tree": [], "ms_moments": [], "ms_eval": []}
+    for k in range(warmup + steps):
+        s_t.velocity3d_device(*dev, act, *outs)
+        if k >= warmup:
+            stt = s_t.fmm3d_stats()
+            for key in ms:
+                ms[key].append(stt[key])
+    stage_ms = {k: float(np.mean(v)) for k, v in ms.items()}
+    flops = FMM3D_FLOPS_PAIR * st["pairs"] + FMM3D_FLOPS_M2P * st["m2p"]
+    achieved = flops / (stage_ms["ms_eval"] * 1e-3) / 1e12
+    u, v, w = (t.cpu().numpy() for t in outs)
+    log(f"[bench] 3-D: {dt * 1e3:.3f} ms/step; tree {stage_ms['ms_tree']:.3f} moments "
+        f"{stage_ms['ms_moments']:.3f} eval {stage_ms['ms_eval']:.3f} ms; eval "
+        f"{achieved:.2f} TFLOP/s ({achieved / FP64_PEAK_TFLOPS:.3f} of FP64)")
+    out = {
+        "metric": FMM3D_METRIC, "value": n / dt, "unit": UNIT, "ms_per_step": dt * 1e3,
+        "steps": steps, "warmup": warmup, "dtype": "f64",
+        "data": "synthetic (seeded vortex ring: Gaussian core, lattice inside the torus)",
+        "config": {"workload": f["name"], "n": n, "crit_num": 30, "exp_tree": 1.0,
+                   "support": 2.0, "n_cells": st["n_cells"], "n_leaves": st["n_leaves"],
+                   "depth": st["depth"], "sorted_levels": st["sorted_levels"]},
+        "stage_ms": stage_ms,
+        "work": {"pairs": st["pairs"], "m2p": st["m2p"],
+                 "lane_use_pairs": st["pairs"] / (64.0 * max(1, st["wave_pairs"])),
+                 "lane_use_m2p": st["m2p"] / (64.0 * max(1, st["wave_m2p"]))},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k3_eval", "flops_per_launch": flops,
+                     "ms_per_launch": stage_ms["ms_eval"],
+                     "flops_note": f"{FMM3D_FLOPS_PAIR:.0f} per direct pair, "
+                                   f"{FMM3D_FLOPS_M2P:.0f} per moment term",
+                     "timing": "HIP events on the solver stream around the kernel, mean over "
+                               "the steps of a second solver that records them"},
+    }
+    if accuracy or cpu:
+        from oracle import oracle as O
+        al = [f[k] * f["s"] ** 3 / (4 * np.pi) for k in ("wx", "wy", "wz")]
+        rot = np.stack([u - 1.0, v, w], 1)
+        if accuracy:
+            rng = np.random.Generator(np.random.PCG64(7))
+            tidx = np.sort(rng.choice(n, 2000, replace=False)).astype(np.int32)
+            du, dv, dw = O.o3_direct(f["x"], f["y"], f["z"], *al, f["active"], targets=tidx)
+            d = np.stack([du, dv, dw], 1)
+            out["rel_l2_vs_fp64_direct"] = float(np.sqrt(((rot[tidx] - d) ** 2).sum() /
+                                                         (d ** 2).sum()))
+            log(f"[bench] 3-D rel-L2 vs the direct sum (2k targets): "
+                f"{out['rel_l2_vs_fp64_direct']:.3e}")
+        if cpu:
+            threads, phys = host_cores()
+            t0 = time.perf_counter()
+            ou, ov, ow = O.o3_velocity(f["x"], f["y"], f["z"], f["s"], f["wx"], f["wy"],
+                                       f["wz"], f["active"], O.o3_opts(), 1.0, 0.0, 0.0)
+            tc = time.perf_counter() - t0
+            orot = np.stack([ou - 1.0, ov, ow], 1)
+            sc = np.sqrt(np.mean((orot ** 2).sum(1)))
+            out["max_diff_vs_oracle"] = float(np.max(np.linalg.norm(rot - orot, axis=1)) / sc)
+            out["cpu_baseline"] = {
+                "value": n / tc, "unit": UNIT, "cores": threads, "kind": "port",
+                "sample": f"the C restatement (oracle/fmm3d_oracle.c, OpenMP over targets) on "
+                          f"the full {n}-particle ring, {threads} threads, one solve",
+                "seconds_per_solve": tc, "host_physical_cores": phys}
+            log(f"[bench] 3-D cpu baseline {tc:.3f} s/solve on {threads} threads; GPU vs "
+                f"oracle max |d| / rms = {out['max_diff_vs_oracle']:.2e}")
+    return out
+
+
 def load_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size)."""
@@ -315,6 +420,10 @@ def main():
                     help="skip timing the host-pointer entry point (copies included)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the sweep of fixed orders x leaf thresholds (best fixed setting)")
+    ap.add_argument("--no-fmm3d", action="store_true",
+                    help="skip the 3-D treecode section (vortex ring)")
+    ap.add_argument("--fmm3d-only", action="store_true",
+                    help="time only the 3-D treecode and print its line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -337,6 +446,14 @@ def main():
         print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "copies": 5}))
         return
     from lsmps_vpm_v2_amd import FmmSolver, workloads
+
+    if args.fmm3d_only:
+        o3 = fmm3d_section(torch, args.steps, args.warmup, not args.no_cpu_baseline,
+                           not args.no_accuracy)
+        o3.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+                   "vs_baseline": None})
+        print(json.dumps(o3), flush=True)
+        return
 
     field = workloads.boundary_layer()
     n = field["n"]
@@ -499,6 +616,9 @@ def main():
                 uu, vv = out_fp64["_uv"]
                 out["fp64_path"]["rel_l2_vs_reference_fmm"] = rel_l2(uu, vv, ur_all, vr_all)
             log(f"[bench] rel-L2 vs reference FMM: {out['rel_l2_vs_reference_fmm']:.3e}")
+    if not args.no_fmm3d:
+        out["fmm3d_path"] = fmm3d_section(torch, args.steps, args.warmup,
+                                          not args.no_cpu_baseline, not args.no_accuracy)
     print(json.dumps(out), flush=True)
 
 

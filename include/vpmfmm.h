@@ -324,6 +324,10 @@ typedef struct vpmfmm_fmm3d_stats {
     uint64_t wave_pairs;       /* wavefront steps of the direct sums (64 lanes each;
                                   pairs / (64 wave_pairs) is the lane use)          */
     uint64_t wave_m2p;         /* wavefront steps of the moment terms               */
+    double ms_tree;            /* stage times of a handle made with timing = 1 (HIP
+                                  events; -1 otherwise): octree and compacted sources, */
+    double ms_moments;         /* moments,                                           */
+    double ms_eval;            /* evaluation kernel                                  */
 } vpmfmm_fmm3d_stats;
 
 int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st);

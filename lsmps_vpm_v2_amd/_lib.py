@@ -77,6 +77,7 @@ class Fmm3dStats(C.Structure):
         ("n_leaves", C.c_int32), ("depth", C.c_int32), ("sorted_levels", C.c_int32),
         ("root_centre", C.c_double * 3), ("root_side", C.c_double), ("pairs", C.c_uint64),
         ("m2p", C.c_uint64), ("wave_pairs", C.c_uint64), ("wave_m2p", C.c_uint64),
+        ("ms_tree", C.c_double), ("ms_moments", C.c_double), ("ms_eval", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -1216,7 +1216,7 @@ int vpmfmm_velocity3d_device(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32
         VPM_HIP(hipSetDevice(h->o.device));
         if (!s) throw Error(kBadArg, "null pointer");
         fmm3d_velocity(ctx3(h), h->ctx.stream, n, x, y, z, s, wx, wy, wz, active, fmm3_params(*o),
-                       u, v, w);
+                       u, v, w, h->o.timing ? h->ev : nullptr);
     });
 }
 
@@ -1247,7 +1247,8 @@ void velocity3d_host(vpmfmm_handle *h, const vpmfmm_fmm3d_opts &o, int32_t n, co
         if (!s) p.u_inf = p.v_inf = p.w_inf = 0.0;
         fmm3d_velocity(ctx3(h), c.stream, n, st, st + n, st + 2 * (size_t)n,
                        s ? st + 3 * (size_t)n : nullptr, st + 4 * (size_t)n, st + 5 * (size_t)n,
-                       st + 6 * (size_t)n, da, p, out, out + n, out + 2 * (size_t)n);
+                       st + 6 * (size_t)n, da, p, out, out + n, out + 2 * (size_t)n,
+                       h->o.timing ? h->ev : nullptr);
         VPM_HIP(hipMemcpyAsync(u, out, b, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipMemcpyAsync(v, out + n, b, hipMemcpyDeviceToHost, c.stream));
         VPM_HIP(hipMemcpyAsync(w, out + 2 * (size_t)n, b, hipMemcpyDeviceToHost, c.stream));
@@ -1295,6 +1296,9 @@ int vpmfmm_fmm3d_last_stats(vpmfmm_handle *h, vpmfmm_fmm3d_stats *st) {
         st->m2p = fi.m2p;
         st->wave_pairs = fi.wave_pairs;
         st->wave_m2p = fi.wave_m2p;
+        st->ms_tree = fi.ms_tree;
+        st->ms_moments = fi.ms_moments;
+        st->ms_eval = fi.ms_eval;
     });
 }
 

@@ -739,7 +739,7 @@ Ctx3::~Ctx3() {
 void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double *y,
                     const double *z, const double *sz, const double *wx, const double *wy,
                     const double *wz, const uint8_t *active, const Fmm3Params &p, double *u,
-                    double *v, double *w) {
+                    double *v, double *w, hipEvent_t *ev) {
     if (n < 1) throw Error(kBadArg, "n must be >= 1");
     if (!x || !y || !z || !wx || !wy || !wz || !active || !u || !v || !w)
         throw Error(kBadArg, "null pointer");
@@ -758,6 +758,7 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     const int nb = div_up(n, 256);
     const int nbb = std::min(1024, div_up(n, 256 * 8));
     double *part = c.bbox.as<double>(8 * (size_t)nbb);
+    if (ev) VPM_HIP(hipEventRecord(ev[0], s));
     k3_bbox<<<nbb, 256, 0, s>>>(x, y, z, n, part);
     VPM_LAUNCH_CHECK();
 
@@ -839,15 +840,18 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     Src3 *S = c.src.as<Src3>((size_t)std::max(1u, hh.n_active));
     k3_emit<<<nb, 256, 0, s>>>(key, settle, mprev, pos, A, n, ncell, hd, p.support, cells);
     k3_compact<<<nb, 256, 0, s>>>(P, A, n, S);
+    if (ev) VPM_HIP(hipEventRecord(ev[1], s));
     const int cb = div_up(ncell, 4);
     if (ncell > 0) {
         k3_p2m<<<div_up(ncell, 256), 256, 0, s>>>(cells, (int)ncell, S, mom);
         k3_m2m<<<cb, 256, 0, s>>>(cells, (int)ncell, mom);
     }
+    if (ev) VPM_HIP(hipEventRecord(ev[2], s));
     const int waves = div_up(n, kWave);
     unsigned long long *wstat = c.wstat.as<unsigned long long>(4 * (size_t)waves);
     k3_eval<<<div_up(waves, kEvalWaves), kEvalBlock, 0, s>>>(
         cells, (int)ncell, mom, P, S, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, wstat);
+    if (ev) VPM_HIP(hipEventRecord(ev[3], s));
     k3_eval_stats<<<1, 1024, 0, s>>>(wstat, waves, hd);
     VPM_LAUNCH_CHECK();
     VPM_HIP(hipMemcpyAsync(c.hdr_h, hd, sizeof(Fmm3Hdr), hipMemcpyDeviceToHost, s));
@@ -856,6 +860,16 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     fi.m2p = c.hdr_h->m2p;
     fi.wave_pairs = c.hdr_h->wave_pairs;
     fi.wave_m2p = c.hdr_h->wave_m2p;
+    fi.ms_tree = fi.ms_moments = fi.ms_eval = -1.0;
+    if (ev) {
+        float a = 0.f, b = 0.f, e = 0.f;
+        VPM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        VPM_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        VPM_HIP(hipEventElapsedTime(&e, ev[2], ev[3]));
+        fi.ms_tree = a;
+        fi.ms_moments = b;
+        fi.ms_eval = e;
+    }
 }
 
 void fmm3d_cells(Ctx3 &c, hipStream_t s, int32_t *level, int64_t *key, int32_t *count,

@@ -49,6 +49,9 @@ struct Fmm3Info {
     int n = 0, n_active = 0, n_cells = 0, n_leaves = 0, depth = 0;
     double root_c[3] = {0, 0, 0}, root_len = 0;
     unsigned long long pairs = 0, m2p = 0, wave_pairs = 0, wave_m2p = 0;
+    // stage times (HIP events; -1 when the solve ran without them): the
+    // octree with the compacted sources, the moments, the evaluation kernel
+    double ms_tree = -1, ms_moments = -1, ms_eval = -1;
     int sorted_levels = 0;  // levels the particle sort resolved (10 or 20)
 };
 
@@ -69,10 +72,11 @@ struct Ctx3 {
 // the vorticity (wx, wy, wz) and size s (alpha = omega s^3 / (4 pi)), plus the
 // free stream.  With s null, (wx, wy, wz) are the strengths alpha.  Throws
 // Error on bad input (non-finite coordinates, a tree deeper than max_depth).
+// ev (4 events, may be null) times the stages.
 void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double *y,
                     const double *z, const double *sz, const double *wx, const double *wy,
                     const double *wz, const uint8_t *active, const Fmm3Params &p, double *u,
-                    double *v, double *w);
+                    double *v, double *w, hipEvent_t *ev = nullptr);
 
 // The cells of the last solve (host arrays of n_cells entries, any may be
 // null): level, path key (octant x + 2y + 4z per level, level 1 highest),
