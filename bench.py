@@ -264,6 +264,7 @@ def cpu_baseline(field, reps=5, single_thread=True):
         t0 = time.perf_counter()
         O.ref_fmm_field(field["x"], field["y"], field["q"], field["active"], threads=1)
         t1 = time.perf_counter() - t0
+        O.set_threads(threads)  # the OpenMP setting is per calling thread: restore it
         out["single_thread"] = {"value": n / t1, "seconds_per_solve": t1, "cores": 1,
                                 "sample": "the same solve on 1 thread, once"}
     return out, (None if ref is None else (ref[0], ref[1]))
@@ -377,6 +378,7 @@ def fmm3d_section(torch, steps, warmup, cpu, accuracy):
                 f"{out['rel_l2_vs_fp64_direct']:.3e}")
         if cpu:
             threads, phys = host_cores()
+            O.set_threads(threads)
             t0 = time.perf_counter()
             ou, ov, ow = O.o3_velocity(f["x"], f["y"], f["z"], f["s"], f["wx"], f["wy"],
                                        f["wz"], f["active"], O.o3_opts(), 1.0, 0.0, 0.0)

@@ -693,11 +693,6 @@ __global__ void __launch_bounds__(256)
 constexpr int kEvalWaves = 4;
 constexpr int kStage = 256;  // sources staged per wavefront per round
 
-// A wave-uniform value read by every lane (from LDS) into a scalar register.
-__device__ __forceinline__ uint32_t uni(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-
 // Padding source: r^2 ~ 1e36 stays finite and q = 0 contributes exactly 0.
 __device__ __forceinline__ float4 far_source() {
     return make_float4(1e18f, 1e18f, 0.f, __uint_as_float(0xffffffffu));
@@ -787,8 +782,8 @@ struct EvalArgs {
 
 template <int G, int ICUT>
 __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, float4 *stg,
-                                          float *sst, uint32_t *s_off, uint32_t *s_end,
-                                          uint32_t *s_beg, float *s_dx, float *s_dy) {
+                                          float *sst, uint32_t *s_off, uint32_t *s_beg,
+                                          float *s_dx, float *s_dy) {
     constexpr int LG = kWave / G;
     const int PM1 = A.kst / 2;
     const int g = lane / LG, tl = lane % LG;
@@ -837,81 +832,67 @@ __device__ __forceinline__ void eval_leaf(const EvalArgs &A, int t, int lane, fl
             }
         }
         // P2P from the other list-1 cells.  Up to 64 cells at a time: lane k
-        // fetches cell k's range and centre shift (one round of loads) and a
-        // wave scan gives each cell its offset in the concatenated source
-        // list.  A round stages the whole cells that fit in kStage (a cell
-        // longer than that is cut), cell by cell: lane r takes row r of each
-        // cell, the cell's range and shift read wave-uniformly from LDS, so a
-        // staged source costs a load and a store instead of a search over the
-        // offsets.
-        auto round = [&](uint32_t fill) {
-            const int fillp = ((int)fill + G - 1) / G * G;
-            for (int r = (int)fill + lane; r < fillp; r += kWave) {
-                stg[r] = far_source();
-                if (ICUT) sst[r] = 0.f;
-            }
-            wave_sync();
-            float ex = 0.f, ey = 0.f;
-            p2p_round<G, false, ICUT>(stg + g, sst + g, fillp / G, ti, ts4, i, ex, ey);
-            exd += (double)ex;
-            eyd += (double)ey;
-            wave_sync();
-        };
-        auto stage1 = [&](int slot, uint32_t src, float sdx, float sdy) {
-            const float2 o = A.oxy[src];
-            stg[slot] = make_float4(o.x + sdx, o.y + sdy, A.qf[src], __uint_as_float(src));
-            if (ICUT) sst[slot] = (float)(0.25 * A.ss[src] * A.ss[src]);
-        };
+        // fetches cell k's range and centre shift (one round of loads), a wave
+        // scan gives each cell its offset in the concatenated source list, and
+        // the sources are staged in rounds of kStage, each lane locating its
+        // source's cell by binary search over the offsets.
         for (uint32_t c0 = a1; (A.mask & 1) && c0 < z1; c0 += kWave) {
             const int ncell = (int)min<uint32_t>(kWave, z1 - c0);
-            {
-                uint32_t cnt = 0, beg = 0;
-                float dcx = 0.f, dcy = 0.f;
-                if (lane < ncell) {
-                    const int sc = A.l1[c0 + lane];
-                    if (sc != j) {  // the own leaf was done above
-                        beg = A.c_begin[sc];
-                        cnt = A.c_end[sc] - beg;
-                        dcx = (float)(A.c_cx[sc] - tcx);
-                        dcy = (float)(A.c_cy[sc] - tcy);
+            uint32_t cnt = 0, beg = 0;
+            float dcx = 0.f, dcy = 0.f;
+            if (lane < ncell) {
+                const int sc = A.l1[c0 + lane];
+                if (sc != j) {  // the own leaf was done above
+                    beg = A.c_begin[sc];
+                    cnt = A.c_end[sc] - beg;
+                    dcx = (float)(A.c_cx[sc] - tcx);
+                    dcy = (float)(A.c_cy[sc] - tcy);
+                }
+            }
+            uint32_t inc = cnt;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, kWave);
+                if (lane >= o) inc += y;
+            }
+            // wave-uniform: the round loops run on the scalar unit
+            const uint32_t total =
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(inc, kWave - 1, kWave));
+            s_off[lane] = inc - cnt;
+            s_beg[lane] = beg;
+            s_dx[lane] = dcx;
+            s_dy[lane] = dcy;
+            wave_sync();
+            for (uint32_t base = 0; base < total; base += kStage) {
+                const int fill = (int)min<uint32_t>((uint32_t)kStage, total - base);
+                const int fillp = (fill + G - 1) / G * G;
+                for (int r = lane; r < fillp; r += kWave) {
+                    if (r >= fill) {
+                        stg[r] = far_source();
+                        if (ICUT) sst[r] = 0.f;
+                        continue;
                     }
+                    const uint32_t gi = base + (uint32_t)r;
+                    // last cell whose offset is <= gi (empty cells share the
+                    // offset of the next one and are never the last)
+                    int lo = 0, hi = ncell - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= gi) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint32_t src = s_beg[lo] + (gi - s_off[lo]);
+                    const float2 o = A.oxy[src];
+                    stg[r] = make_float4(o.x + s_dx[lo], o.y + s_dy[lo], A.qf[src],
+                                         __uint_as_float(src));
+                    if (ICUT) sst[r] = (float)(0.25 * A.ss[src] * A.ss[src]);
                 }
-                uint32_t inc = cnt;
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const uint32_t y = __shfl_up(inc, o, kWave);
-                    if (lane >= o) inc += y;
-                }
-                // the cells' ranges and shifts in LDS, read back wave-uniformly
-                // (so they hold no registers during the pair loop)
-                s_off[lane] = inc - cnt;
-                s_end[lane] = inc;
-                s_beg[lane] = beg;
-                s_dx[lane] = dcx;
-                s_dy[lane] = dcy;
+                wave_sync();
+                float ex = 0.f, ey = 0.f;
+                p2p_round<G, false, ICUT>(stg + g, sst + g, fillp / G, ti, ts4, i, ex, ey);
+                exd += (double)ex;
+                eyd += (double)ey;
                 wave_sync();
             }
-            // windows [gs, ge) of the concatenated list, at most kStage long,
-            // ending at a cell end where one fits (a longer cell is cut)
-            const uint32_t total = uni(s_end[ncell - 1]);
-            int ca = 0;  // first cell that reaches into the window
-            for (uint32_t gs = 0; gs < total;) {
-                const int cb =
-                    (int)__popcll(__ballot(lane < ncell && s_end[lane] <= gs + kStage));
-                const uint32_t ge = cb > ca ? uni(s_end[cb - 1]) : gs + kStage;
-                for (int k = ca; k < ncell; ++k) {
-                    const uint32_t o = uni(s_off[k]);
-                    if (o >= ge) break;
-                    const uint32_t lo = max(o, gs), hi = min(uni(s_end[k]), ge);
-                    const uint32_t src0 = uni(s_beg[k]) + (lo - o);
-                    const float dxk = s_dx[k], dyk = s_dy[k];
-                    for (uint32_t r = (uint32_t)lane; r < hi - lo; r += kWave)
-                        stage1((int)(lo - gs + r), src0 + r, dxk, dyk);
-                }
-                round(ge - gs);
-                if (cb > ca) ca = cb;
-                gs = ge;
-            }
-            wave_sync();  // the next chunk rewrites the cell table
         }
         // M2P over list 3: F = (1/s) sum a^_k xi^-(k+1), xi = (z - zc)/s
         for (uint32_t v = a3 + g; (A.mask & 2) && v < z3; v += G) {
@@ -967,8 +948,7 @@ template <int ICUT>
 __global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl) {
     __shared__ float4 stage[kEvalWaves][kStage];
     __shared__ float stage_s[ICUT ? kEvalWaves : 1][ICUT ? kStage : 1];
-    __shared__ uint32_t s_off[kEvalWaves][kWave], s_end[kEvalWaves][kWave],
-        s_beg[kEvalWaves][kWave];
+    __shared__ uint32_t s_off[kEvalWaves][kWave], s_beg[kEvalWaves][kWave];
     __shared__ float s_dx[kEvalWaves][kWave], s_dy[kEvalWaves][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     // (an XCD-contiguous leaf order cut the L2 misses but unbalanced the XCDs:
@@ -979,17 +959,13 @@ __global__ void __launch_bounds__(kWave *kEvalWaves) k_eval32(EvalArgs A, int nl
     const int nt = (int)(A.c_end[j] - A.c_begin[j]);
     float *sst = stage_s[ICUT ? w : 0];
     if (nt <= 8)
-        eval_leaf<8, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_end[w], s_beg[w],
-                               s_dx[w], s_dy[w]);
+        eval_leaf<8, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else if (nt <= 16)
-        eval_leaf<4, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_end[w], s_beg[w],
-                               s_dx[w], s_dy[w]);
+        eval_leaf<4, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else if (nt <= 32)
-        eval_leaf<2, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_end[w], s_beg[w],
-                               s_dx[w], s_dy[w]);
+        eval_leaf<2, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
     else
-        eval_leaf<1, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_end[w], s_beg[w],
-                               s_dx[w], s_dy[w]);
+        eval_leaf<1, ICUT>(A, t, lane, stage[w], sst, s_off[w], s_beg[w], s_dx[w], s_dy[w]);
 }
 
 }  // namespace

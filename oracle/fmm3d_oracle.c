@@ -13,6 +13,7 @@
 #include "fmm3d_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -480,4 +481,8 @@ void orc3_direct(int n, const double *x, const double *y, const double *z, const
         v[a] = vv;
         w[a] = vw;
     }
+}
+
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
 }

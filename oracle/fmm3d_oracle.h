@@ -80,6 +80,10 @@ void orc3_m2p_mul(double dx, double dy, double dz, double *mx, double *my, doubl
 void orc3_p2m(double *mp, double src, double dx, double dy, double dz);
 void orc3_m2m(double *mp, const double *mpc, double dx, double dy, double dz);
 
+/* OpenMP threads of the calling thread's later parallel regions (the
+ * oracle's and any other library's sharing the OpenMP runtime). */
+void orc_set_threads(int n);
+
 /* Direct 3-D Biot-Savart sum u_i = sum_{j != i, active j} alpha_j x (x_i - x_j) / r^3
  * at targets tidx (NULL = all).  OpenMP over targets. */
 void orc3_direct(int n, const double *x, const double *y, const double *z, const double *ax,

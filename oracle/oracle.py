@@ -160,6 +160,8 @@ def lib():
         L.orc3_m2m.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double]
         L.orc3_direct.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _up, C.c_int, _ip,
                                   _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
         for name in ("orc3_default_opts", "orc3_free", "orc3_cells", "orc3_m2p_mul",
                      "orc3_p2m", "orc3_m2m", "orc3_direct"):
             getattr(L, name).restype = None
@@ -658,6 +660,12 @@ def ref_type1_velocity(x, y, s, g, n_s=50, lmax=7, ndp=10, icutoff=2):
 # --------------------------------------------------------------------------
 # the 3-D vector FMM (fmm3d_oracle.c; reference fast3DFMM.cpp)
 # --------------------------------------------------------------------------
+def set_threads(n: int) -> None:
+    """OpenMP threads of this thread's later oracle calls (and of the
+    reference build's, which shares the OpenMP runtime)."""
+    lib().orc_set_threads(int(n))
+
+
 def o3_opts(crit_num=30, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
             max_depth=21) -> O3Opts:
     o = O3Opts()
