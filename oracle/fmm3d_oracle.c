@@ -1,6 +1,7 @@
 /*
- * fmm3d_oracle.c -- CPU oracle for the reference's default 3-D velocity path
- * (fast3DFMM.cpp with the fmm3D.cpp helpers).  See fmm3d_oracle.h.
+ * fmm3d_oracle.c -- CPU oracle for the reference's two 3-D velocity paths
+ * (type 1: fast3DFMM.cpp; type 2: fmm3D::calcVelocity over the 3-D TreeCell;
+ * both with the fmm3D.cpp helpers).  See fmm3d_oracle.h.
  *
  * TEST INFRASTRUCTURE ONLY (the checker).
  *
@@ -24,8 +25,11 @@ void orc3_default_opts(orc3_opts *o) {
     o->crit_num = 30;      /* FMMCell::critNum = Pars::src_count_max (global.cpp:779) */
     o->exp_tree = 1.0;     /* global.cpp:769 */
     o->center[0] = o->center[1] = o->center[2] = 0.0;  /* global.cpp:609-611 */
-    o->support = 2.0;      /* SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3 */
+    o->support = 2.0;      /* SUPPORT_RADIUS_FACTOR, fast3DFMM.cpp:4 and fmm3D.cpp:3 */
     o->max_depth = 21;
+    o->type = 1;           /* VELOCITY_3D_TYPE, velocity_calc.cpp:31 */
+    o->max_par = 50;       /* TreeCell::max_particle = Pars::par_count_max (global.cpp:778) */
+    o->max_src = 30;       /* TreeCell::max_source = Pars::src_count_max (global.cpp:779) */
 }
 
 struct orc3_tree {
@@ -33,7 +37,8 @@ struct orc3_tree {
     orc3_opts o;
     int *level, *parent, *npar, *chd, *flag, *pbeg, *pcnt;
     int64_t *key;
-    uint8_t *active;
+    uint8_t *active, *leaf;
+    int *nsrc;       /* active particles (type 2) */
     double *xc, *yc, *zc, *size;
     int *lev_start;  /* cells of level L: [lev_start[L], lev_start[L + 1]) */
     int *pool;       /* leaf particle lists */
@@ -56,6 +61,8 @@ static int o3_grow(orc3_tree *t) {
     O3_RE(pcnt, int, 1);
     O3_RE(key, int64_t, 1);
     O3_RE(active, uint8_t, 1);
+    O3_RE(leaf, uint8_t, 1);
+    O3_RE(nsrc, int, 1);
     O3_RE(xc, double, 1);
     O3_RE(yc, double, 1);
     O3_RE(zc, double, 1);
@@ -76,6 +83,8 @@ void orc3_free(orc3_tree *t) {
     free(t->pcnt);
     free(t->key);
     free(t->active);
+    free(t->leaf);
+    free(t->nsrc);
     free(t->xc);
     free(t->yc);
     free(t->zc);
@@ -98,6 +107,8 @@ static int o3_new_cell(orc3_tree *t, int lev, int par, int64_t key, double xc, d
     t->pcnt[c] = 0;
     t->key[c] = key;
     t->active[c] = 0;
+    t->leaf[c] = 0;
+    t->nsrc[c] = 0;
     t->xc[c] = xc;
     t->yc[c] = yc;
     t->zc[c] = zc;
@@ -106,12 +117,10 @@ static int o3_new_cell(orc3_tree *t, int lev, int par, int64_t key, double xc, d
 }
 
 /* FMMCell::generateTree, fast3DFMM.cpp:485-745. */
-int orc3_build(orc3_tree **out, int n, const double *x, const double *y, const double *z,
-               const uint8_t *active, const orc3_opts *o) {
+static int o3_build_t1(orc3_tree **out, int n, const double *x, const double *y,
+                       const double *z, const uint8_t *active, const orc3_opts *o) {
     *out = NULL;
-    if (!o || n < 0 || o->crit_num < 1 || o->max_depth < 1 || o->max_depth > 21 ||
-        (n > 0 && (!x || !y || !z || !active)))
-        return 1;
+    if (o->crit_num < 1) return 1;
     for (int i = 0; i < n; ++i)
         if (!isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i])) return 2;
     orc3_tree *t = (orc3_tree *)calloc(1, sizeof(orc3_tree));
@@ -195,6 +204,7 @@ int orc3_build(orc3_tree **out, int n, const double *x, const double *y, const d
     if (!st) {
         t->depth = lev;
         t->lev_start[lev + 1] = t->ncell;
+        for (int c = 1; c < t->ncell; ++c) t->leaf[c] = t->npar[c] < o->crit_num;
         /* leaf particle lists in index order */
         int off = 0;
         for (int c = 0; c < t->ncell; ++c) {
@@ -229,6 +239,215 @@ int orc3_build(orc3_tree **out, int n, const double *x, const double *y, const d
     return 0;
 }
 
+/* Cells of one level keyed by their index triple (open addressing). */
+typedef struct {
+    uint64_t *k;
+    int *v;
+    size_t cap;
+} o3_hash;
+
+static uint64_t o3_mix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+static int o3_hash_init(o3_hash *h, size_t want) {
+    size_t cap = 1024;
+    while (cap < 2 * want) cap <<= 1;
+    h->k = (uint64_t *)malloc(sizeof(uint64_t) * cap);
+    h->v = (int *)malloc(sizeof(int) * cap);
+    if (!h->k || !h->v) return 0;
+    h->cap = cap;
+    for (size_t i = 0; i < cap; ++i) h->v[i] = -1;
+    return 1;
+}
+
+static int *o3_hash_slot(o3_hash *h, uint64_t key) {
+    size_t i = o3_mix(key) & (h->cap - 1);
+    while (h->v[i] >= 0 && h->k[i] != key) i = (i + 1) & (h->cap - 1);
+    h->k[i] = key;
+    return &h->v[i];
+}
+
+static void o3_hash_free(o3_hash *h) {
+    free(h->k);
+    free(h->v);
+}
+
+/* TreeCell::initializeTree (treeCell.cpp:1043-1243) in 3-D: the root box of
+ * the particles alone (min / max from the first particle on, :1066-1077),
+ * side = the longest extent * (1 + expTree / 100) (:1080-1088), pivot =
+ * (max + min - side) / 2 (:1091); at each level a particle's cell index is
+ * (int)((x - pivot) / cellSize) per axis (coord2index, :290-298; cellSize =
+ * side / 2^level, :101-104), the cells created in particle order; a new cell
+ * is a leaf when srcParNum <= max_source and parNum <= max_particle, active
+ * when srcParNum != 0 (:1185-1199); the root is never a leaf.  The cell
+ * centre is pivot + (0.5 + index) * length (Cell, treeCell.hpp:68).  Parent
+ * and children are linked by index arithmetic (get_childID, :378-412). */
+static int o3_build_t2(orc3_tree **out, int n, const double *x, const double *y,
+                       const double *z, const uint8_t *active, const orc3_opts *o) {
+    *out = NULL;
+    if (o->max_par < 0 || o->max_src < 0 || n < 1) return 1;
+    const double *p[3] = {x, y, z};
+    double mn[3], mx[3], piv[3];
+    for (int d = 0; d < 3; ++d) mn[d] = mx[d] = p[d][0];
+    for (int i = 1; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            if (mn[d] > p[d][i]) mn[d] = p[d][i];
+            if (mx[d] < p[d][i]) mx[d] = p[d][i];
+        }
+    double root = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double sz = fabs(mx[d] - mn[d]);
+        if (sz > root) root = sz;
+    }
+    root *= 1.0 + (o->exp_tree / 100.0);
+    if (!(root > 0.0)) return 3;  /* every particle at one point */
+    for (int d = 0; d < 3; ++d) piv[d] = (mx[d] + mn[d] - root) / 2.0;
+    orc3_tree *t = (orc3_tree *)calloc(1, sizeof(orc3_tree));
+    if (!t) return 5;
+    t->n = n;
+    t->o = *o;
+    t->lev_start = (int *)malloc(sizeof(int) * (size_t)(o->max_depth + 3));
+    t->pool = (int *)malloc(sizeof(int) * (size_t)n);
+    int *pcell = (int *)malloc(sizeof(int) * (size_t)n);
+    int *ix = (int *)malloc(sizeof(int) * 3 * (size_t)n);
+    uint8_t *done = (uint8_t *)calloc((size_t)n, 1);
+    int nsrc_all = 0;
+    for (int i = 0; i < n; ++i) nsrc_all += active[i] != 0;
+    if (!t->lev_start || !t->pool || !pcell || !ix || !done ||
+        o3_new_cell(t, 0, -1, 0, piv[0] + 0.5 * root, piv[1] + 0.5 * root, piv[2] + 0.5 * root,
+                    root) < 0) {
+        free(pcell);
+        free(ix);
+        free(done);
+        orc3_free(t);
+        return 5;
+    }
+    t->npar[0] = n;
+    t->nsrc[0] = nsrc_all;
+    t->active[0] = nsrc_all != 0;
+    t->lev_start[0] = 0;
+    int unsettled = n, lev = 0, st = 0;
+    while (unsettled > 0) {
+        ++lev;
+        if (lev > o->max_depth) {
+            st = 4;
+            break;
+        }
+        t->lev_start[lev] = t->ncell;
+        const double cs = root / pow(2.0, lev);
+        o3_hash hsh;
+        if (!o3_hash_init(&hsh, (size_t)unsettled)) {
+            st = 5;
+            break;
+        }
+        o3_hash hpar;
+        const int pb = t->lev_start[lev - 1], pe = t->lev_start[lev];
+        if (!o3_hash_init(&hpar, (size_t)(pe - pb))) {
+            o3_hash_free(&hsh);
+            st = 5;
+            break;
+        }
+        /* the previous level's cells by index, for the parent links */
+        for (int c = pb; c < pe; ++c) {
+            const uint64_t k = (uint64_t)(uint32_t)ix[3 * (size_t)t->pbeg[c]] |
+                               ((uint64_t)(uint32_t)ix[3 * (size_t)t->pbeg[c] + 1] << 21) |
+                               ((uint64_t)(uint32_t)ix[3 * (size_t)t->pbeg[c] + 2] << 42);
+            if (lev > 1) *o3_hash_slot(&hpar, k) = c;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (done[i]) continue;
+            int id[3];
+            for (int d = 0; d < 3; ++d) id[d] = (int)((p[d][i] - piv[d]) / cs);
+            const uint64_t k = (uint64_t)(uint32_t)id[0] | ((uint64_t)(uint32_t)id[1] << 21) |
+                               ((uint64_t)(uint32_t)id[2] << 42);
+            int *slot = o3_hash_slot(&hsh, k);
+            if (*slot < 0) {
+                const double len = cs;
+                int par = 0;
+                if (lev > 1) {
+                    const uint64_t kp = (uint64_t)(uint32_t)(id[0] / 2) |
+                                        ((uint64_t)(uint32_t)(id[1] / 2) << 21) |
+                                        ((uint64_t)(uint32_t)(id[2] / 2) << 42);
+                    par = *o3_hash_slot(&hpar, kp);
+                }
+                const int oc = (id[0] & 1) | ((id[1] & 1) << 1) | ((id[2] & 1) << 2);
+                const int c = par < 0 ? -1
+                                      : o3_new_cell(t, lev, par, t->key[par] * 8 + oc,
+                                                    piv[0] + ((0.5 + id[0]) * len),
+                                                    piv[1] + ((0.5 + id[1]) * len),
+                                                    piv[2] + ((0.5 + id[2]) * len), len);
+                if (c < 0) {
+                    st = par < 0 ? 6 : 5;
+                    break;
+                }
+                t->chd[8 * par + oc] = c;
+                t->flag[par] |= 1 << oc;
+                t->pbeg[c] = i;  /* a particle of the cell, for its index triple */
+                *slot = c;
+            }
+            const int c = *slot;
+            pcell[i] = c;
+            for (int d = 0; d < 3; ++d) ix[3 * (size_t)i + d] = id[d];
+            t->npar[c]++;
+            if (active[i]) t->nsrc[c]++;
+        }
+        o3_hash_free(&hsh);
+        o3_hash_free(&hpar);
+        if (st) break;
+        for (int c = t->lev_start[lev]; c < t->ncell; ++c) {
+            t->leaf[c] = t->nsrc[c] <= o->max_src && t->npar[c] <= o->max_par;
+            t->active[c] = t->nsrc[c] != 0;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (done[i] || !t->leaf[pcell[i]]) continue;
+            t->pcnt[pcell[i]]++;
+            done[i] = 1;
+            --unsettled;
+        }
+    }
+    if (!st) {
+        t->depth = lev;
+        t->lev_start[lev + 1] = t->ncell;
+        int off = 0;
+        for (int c = 0; c < t->ncell; ++c) {
+            t->pbeg[c] = off;
+            off += t->pcnt[c];
+            t->pcnt[c] = 0;
+        }
+        for (int i = 0; i < n; ++i) {
+            const int c = pcell[i];
+            t->pool[t->pbeg[c] + t->pcnt[c]++] = i;
+        }
+    }
+    free(pcell);
+    free(ix);
+    free(done);
+    if (st) {
+        orc3_free(t);
+        return st;
+    }
+    *out = t;
+    return 0;
+}
+
+int orc3_build(orc3_tree **out, int n, const double *x, const double *y, const double *z,
+               const uint8_t *active, const orc3_opts *o) {
+    *out = NULL;
+    if (!o || n < 0 || o->max_depth < 1 || o->max_depth > 21 || (o->type != 1 && o->type != 2) ||
+        (n > 0 && (!x || !y || !z || !active)))
+        return 1;
+    for (int i = 0; i < n; ++i)
+        if (!isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i])) return 2;
+    return o->type == 1 ? o3_build_t1(out, n, x, y, z, active, o)
+                        : o3_build_t2(out, n, x, y, z, active, o);
+}
+
 int orc3_ncells(const orc3_tree *t) { return t->ncell; }
 int orc3_depth(const orc3_tree *t) { return t->depth; }
 
@@ -238,7 +457,7 @@ void orc3_cells(const orc3_tree *t, int *level, int64_t *key, int *count, uint8_
         if (level) level[c] = t->level[c];
         if (key) key[c] = t->key[c];
         if (count) count[c] = t->npar[c];
-        if (leaf) leaf[c] = t->npar[c] < t->o.crit_num;
+        if (leaf) leaf[c] = t->leaf[c];
         if (active) active[c] = t->active[c];
         if (centre3) {
             centre3[3 * c] = t->xc[c];
@@ -352,8 +571,8 @@ int orc3_velocity_alpha(int n, const double *x, const double *y, const double *z
     }
     const double *al[3] = {ax, ay, az};
     /* P2M at active leaves over their active particles (:187-219) */
-    for (int c = 0; c < nc; ++c) {
-        if (t->npar[c] >= t->o.crit_num || !t->active[c]) continue;
+    for (int c = 1; c < nc; ++c) {
+        if (!t->leaf[c] || !t->active[c]) continue;
         for (int q = 0; q < t->pcnt[c]; ++q) {
             const int i = t->pool[t->pbeg[c] + q];
             if (!active[i]) continue;
@@ -364,7 +583,7 @@ int orc3_velocity_alpha(int n, const double *x, const double *y, const double *z
     /* M2M, levels maxLevel-1 .. 1 (:245-287) */
     for (int L = t->depth - 1; L > 0; --L)
         for (int c = t->lev_start[L]; c < t->lev_start[L + 1]; ++c) {
-            if (!t->active[c] || t->npar[c] < t->o.crit_num) continue;
+            if (!t->active[c] || t->leaf[c]) continue;
             for (int oc = 0; oc < 8; ++oc) {
                 if (((t->flag[c] >> oc) % 2) == 0) continue;
                 const int ch = t->chd[8 * c + oc];
@@ -375,12 +594,24 @@ int orc3_velocity_alpha(int n, const double *x, const double *y, const double *z
                              dx, dy, dz);
             }
         }
-    /* evaluation (:318-484): from the children of the level-1 cells, a cell
-     * is far (M2P) beyond support * size, else opened (non-leaf) or summed
-     * directly (leaf, active sources only, the target itself skipped);
-     * inactive cells are skipped with their subtrees */
+    /* evaluation (type 1: evaluateFMM, fast3DFMM.cpp:318-484; type 2:
+     * fmm3D::calcVelocity with VELOCITY_CALCULATION_TYPE 2, fmm3D.cpp:
+     * 1136-1196): from the children of the level-1 cells, a cell is far
+     * (M2P) beyond support * size, else opened (non-leaf) or summed directly
+     * (leaf; the target itself skipped; type 1 sums its active particles,
+     * type 2 all of them); inactive cells are skipped with their subtrees.
+     * Type 2 asks the level-1 cells for their children with get_childID,
+     * which throws for a leaf (treeCell.cpp:384-388): that is status 6. */
     const int depth = t->depth;
     const double sup = t->o.support;
+    const int t2 = t->o.type == 2;
+    if (t2 && depth >= 2)
+        for (int oc = 0; oc < 8; ++oc)
+            if (((t->flag[0] >> oc) % 2) && t->leaf[t->chd[oc]]) {
+                free(m);
+                orc3_free(t);
+                return 6;
+            }
 #pragma omp parallel
     {
         int *stk = (int *)malloc(sizeof(int) * (size_t)(8 * (depth + 2) + 16));
@@ -418,12 +649,12 @@ int orc3_velocity_alpha(int n, const double *x, const double *y, const double *z
                             vv += Vz[k] * mx[k] - Vx[k] * mz[k];
                             vw += Vx[k] * my[k] - Vy[k] * mx[k];
                         }
-                    } else if (t->flag[ch] != 0) {
+                    } else if (!t->leaf[ch]) {
                         open[no++] = ch;
                     } else {
                         for (int q = 0; q < t->pcnt[ch]; ++q) {
                             const int j = t->pool[t->pbeg[ch] + q];
-                            if (!active[j] || j == i) continue;
+                            if ((!t2 && !active[j]) || j == i) continue;
                             o3_pair(xp, yp, zp, x, y, z, ax, ay, az, j, &vu, &vv, &vw);
                         }
                     }

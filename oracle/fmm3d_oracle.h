@@ -1,12 +1,17 @@
 /*
- * fmm3d_oracle.h -- CPU oracle for the reference's default 3-D velocity
- * path: VelocityCalc::velocity_fmm_3d_fast (velocity_calc.cpp:2383-2525,
- * VELOCITY_3D_TYPE 1) -> fmm3D::calcVelocityFast (fast3DFMM.cpp:32-110):
+ * fmm3d_oracle.h -- CPU oracle for the reference's 3-D velocity paths.
+ * Type 1 (the default): VelocityCalc::velocity_fmm_3d_fast
+ * (velocity_calc.cpp:2384-2525, VELOCITY_3D_TYPE 1) ->
+ * fmm3D::calcVelocityFast (fast3DFMM.cpp:32-92):
  *   - octree                FMMCell::generateTree  fast3DFMM.cpp:485-745
  *   - P2M, M2M              fastFMM3Dutils::multipoleCalc  :164-317,
  *                           fmm3D::P2M_calc / M2M_calc  fmm3D.cpp:84-145
  *   - evaluation            fastFMM3Dutils::evaluateFMM  :318-484,
  *                           fmm3D::M2P_mul_calc  fmm3D.cpp:147-210
+ * Type 2: VelocityCalc::velocity_fmm_3d (velocity_calc.cpp:2172-2383,
+ * VELOCITY_3D_TYPE 2) -> TreeCell::initializeTree in 3-D
+ * (treeCell.cpp:1043-1243) and fmm3D::calcVelocity (fmm3D.cpp:1064-1205 with
+ * setupVelocityFMM :585-700, VELOCITY_CALCULATION_TYPE 2, :24).
  *
  * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, never by the
  * product library.
@@ -29,13 +34,18 @@ extern "C" {
 #endif
 
 typedef struct {
-    int crit_num;       /* a cell with fewer particles is a leaf (Pars::src_count_max of
-                           the DIM == 3 parameters = 30)                                   */
+    int crit_num;       /* type 1: a cell with fewer particles is a leaf (Pars::src_count_max
+                           of the DIM == 3 parameters = 30)                                */
     double exp_tree;    /* root expansion in percent (Pars::expTree = 1.0)                 */
     double center[3];   /* the box always contains this point (Pars::x/y/zcenter = 0)      */
     double support;     /* opening factor: a cell is far when r > support * size
                            (SUPPORT_RADIUS_FACTOR = 2.0, fmm3D.cpp:3)                     */
     int max_depth;      /* refuse deeper trees (the reference has no bound)               */
+    int type;           /* 1: fast3DFMM (FMMCell octree, VELOCITY_3D_TYPE 1, the default);
+                           2: fmm3D::calcVelocity over the 3-D TreeCell (VELOCITY_3D_TYPE 2) */
+    int max_par, max_src; /* type 2: a cell is a leaf when it holds at most max_par
+                           particles and max_src active ones (Pars::par_count_max 50,
+                           src_count_max 30)                                               */
 } orc3_opts;
 
 void orc3_default_opts(orc3_opts *o);
@@ -43,8 +53,10 @@ void orc3_default_opts(orc3_opts *o);
 /* Tree (opaque). */
 typedef struct orc3_tree orc3_tree;
 
-/* Returns 0, 1 bad arguments, 2 non-finite input, 4 depth beyond max_depth,
- * 5 out of memory. */
+/* Returns 0, 1 bad arguments, 2 non-finite input, 3 every particle at one
+ * point (type 2), 4 depth beyond max_depth, 5 out of memory.  The velocity
+ * calls also return 6 for a type-2 tree with a level-1 leaf, where the
+ * reference's get_childID throws. */
 int  orc3_build(orc3_tree **out, int n, const double *x, const double *y, const double *z,
                 const uint8_t *active, const orc3_opts *o);
 void orc3_free(orc3_tree *t);

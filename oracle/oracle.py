@@ -52,6 +52,9 @@ class O3Opts(C.Structure):
         ("center", C.c_double * 3),
         ("support", C.c_double),
         ("max_depth", C.c_int),
+        ("type", C.c_int),
+        ("max_par", C.c_int),
+        ("max_src", C.c_int),
     ]
 
 
@@ -667,10 +670,11 @@ def set_threads(n: int) -> None:
 
 
 def o3_opts(crit_num=30, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
-            max_depth=21) -> O3Opts:
+            max_depth=21, type=1, max_par=50, max_src=30) -> O3Opts:
     o = O3Opts()
     lib().orc3_default_opts(C.byref(o))
     o.crit_num, o.exp_tree, o.support, o.max_depth = crit_num, exp_tree, support, max_depth
+    o.type, o.max_par, o.max_src = type, max_par, max_src
     for d in range(3):
         o.center[d] = center[d]
     return o

@@ -213,6 +213,22 @@ static int run_field(int n, int clusters, double inactive) {
                   "3-D velocity");
             CHECK(finite_all(ex, n) && finite_all(ey, n) && finite_all(ez, n), "3-D velocity finite");
         }
+        {
+            /* type 2: the TreeCell octree (a level-1 leaf beside deeper
+             * cells is status 6, as the reference throws there) */
+            orc3_opts o3;
+            orc3_default_opts(&o3);
+            o3.type = 2;
+            orc3_tree *t3 = NULL;
+            const int bs = orc3_build(&t3, n, x, y, z, a, &o3);
+            CHECK(bs == 0 || bs == 3, "3-D type-2 tree");
+            if (t3) orc3_free(t3);
+            const int vs = orc3_velocity(n, x, y, z, s, q, w, q, a, &o3, 1.0, 0.0, 0.5, ex, ey, ez);
+            CHECK(vs == 0 || vs == 3 || vs == 6, "3-D type-2 velocity");
+            if (vs == 0)
+                CHECK(finite_all(ex, n) && finite_all(ey, n) && finite_all(ez, n),
+                      "3-D type-2 velocity finite");
+        }
         orc3_direct(n, x, y, z, q, w, q, a, n / 3, tidx, ex, ey, ez);
         CHECK(finite_all(ez, n / 3), "3-D direct");
         free(z); free(ez);

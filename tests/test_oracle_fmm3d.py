@@ -260,3 +260,103 @@ def test_bad_inputs():
     zs = np.r_[z, np.full(60, 0.5)]
     with pytest.raises(O.OracleError):
         O.O3Tree(xs, ys, zs, np.ones(260, np.uint8), O.o3_opts(max_depth=12))
+
+
+# ---------------------------------------------------------------------------
+# type 2: fmm3D::calcVelocity over the 3-D TreeCell (VELOCITY_3D_TYPE 2)
+# ---------------------------------------------------------------------------
+def _t2(**kw):
+    return O.o3_opts(type=2, **kw)
+
+
+@pytest.mark.parametrize("kind,n", [("uniform", 4000), ("clusters", 8000)])
+def test_type2_treecell_invariants(kind, n):
+    """TreeCell::initializeTree in 3-D against numpy: the root box of the
+    particles alone, side x 1.01, pivot (max + min - side) / 2; every
+    particle's cell index per level (int)((x - pivot) / (side / 2^L)); cell
+    centres pivot + (0.5 + index) side / 2^L; the leaf rule (at most 50
+    particles and 30 active ones, the root never a leaf); active = an active
+    particle inside."""
+    x, y, z, *_ = _cloud(n, 31, kind)
+    act = (np.random.default_rng(4).random(n) < 0.7).astype(np.uint8)
+    T = O.O3Tree(x, y, z, act, _t2())
+    P = np.stack([x, y, z], 1)
+    mn, mx = P.min(0), P.max(0)
+    side = max(np.abs(mx - mn)) * (1.0 + 1.0 / 100.0)
+    piv = (mx + mn - side) / 2.0
+    assert T.size[0] == side
+    leaf = T.leaf.astype(bool)
+    assert not leaf[0]
+    where = {(int(l), int(k)): c for c, (l, k) in enumerate(zip(T.level, T.key))}
+    seen = np.zeros(n, int)
+    for c in range(1, len(T.level)):
+        L, k = int(T.level[c]), int(T.key[c])
+        cs = side / 2.0 ** L
+        assert T.size[c] == cs
+        idx = np.zeros(3, np.int64)
+        for lev in range(1, L + 1):
+            oc = (k >> (3 * (L - lev))) & 7
+            idx = 2 * idx + np.array([oc & 1, (oc >> 1) & 1, (oc >> 2) & 1])
+        np.testing.assert_array_equal(T.centre[c], piv + (0.5 + idx) * cs)
+        if leaf[c]:
+            ids = T.particles(c)
+            seen[ids] += 1
+            np.testing.assert_array_equal(((P[ids] - piv) / cs).astype(np.int64),
+                                          np.tile(idx, (len(ids), 1)))
+            assert len(ids) == T.count[c]
+            assert len(ids) <= 50 and act[ids].sum() <= 30
+            assert bool(T.active[c]) == bool(act[ids].any())
+        par = where[(L - 1, k >> 3)]
+        assert not leaf[par]
+    assert np.all(seen == 1)
+    # a non-leaf cell is over one of the two limits
+    for c in np.flatnonzero(~leaf)[1:]:
+        L, k = int(T.level[c]), int(T.key[c])
+        sub = [d for d in np.flatnonzero(leaf) if T.level[d] > L and
+               (int(T.key[d]) >> (3 * (int(T.level[d]) - L))) == k]
+        ids = np.concatenate([T.particles(d) for d in sub])
+        assert len(ids) == T.count[c]
+        assert len(ids) > 50 or act[ids].sum() > 30
+
+
+def test_type2_all_open_equals_direct_sum_with_inactive_sources():
+    """With every cell opened, type 2 sums every particle of the active
+    leaves directly, inactive ones included (fmm3D.cpp:1172-1186 reads no
+    active mark), and nothing of the inactive leaves."""
+    x, y, z, ax, ay, az = _cloud(5000, 37)
+    act = np.ones(5000, np.uint8)
+    act[np.random.default_rng(2).random(5000) < 0.3] = 0
+    act[(x > 0.6) & (y > 0.6)] = 0  # some leaves wholly inactive
+    o = _t2(support=1e6)
+    u, v, w = O.o3_velocity_alpha(x, y, z, ax, ay, az, act, o)
+    T = O.O3Tree(x, y, z, act, o)
+    src = np.zeros(5000, np.uint8)
+    for c in np.flatnonzero(T.leaf.astype(bool) & T.active.astype(bool)):
+        src[T.particles(c)] = 1
+    assert src.sum() > act.sum()  # inactive particles of active leaves are sources
+    du, dv, dw = O.o3_direct(x, y, z, ax, ay, az, src)
+    assert _rel(np.r_[u, v, w], np.r_[du, dv, dw]) < 1e-13
+
+
+def test_type2_treecode_error_and_level1_leaf():
+    x, y, z, ax, ay, az = _cloud(20000, 41)
+    act = np.ones(20000, np.uint8)
+    du, dv, dw = O.o3_direct(x, y, z, ax, ay, az, act)
+    u, v, w = O.o3_velocity_alpha(x, y, z, ax, ay, az, act, _t2())
+    e2 = _rel(np.r_[u, v, w], np.r_[du, dv, dw])
+    u1, v1, w1 = O.o3_velocity_alpha(x, y, z, ax, ay, az, act, O.o3_opts())
+    e1 = _rel(np.r_[u1, v1, w1], np.r_[du, dv, dw])
+    assert e2 < 2e-3 and e1 < 2e-3, (e1, e2)
+    # a level-1 leaf beside deeper cells: the reference's get_childID throws
+    # (treeCell.cpp:384-388) when the walk asks the leaf for its children
+    few = np.flatnonzero((x > 0) & (y > 0) & (z > 0))
+    keep = np.setdiff1d(np.arange(20000), few[20:])
+    with pytest.raises(O.OracleError):
+        O.o3_velocity_alpha(*(t[keep] for t in (x, y, z, ax, ay, az)), act[keep], _t2())
+    # a tree one level deep: the walk never starts, the rotational part is zero
+    u, v, w = O.o3_velocity_alpha(x[:40], y[:40], z[:40], ax[:40], ay[:40], az[:40], act[:40],
+                                  _t2())
+    assert not np.any(u) and not np.any(v) and not np.any(w)
+    # every particle at one point: no root box
+    with pytest.raises(O.OracleError):
+        O.O3Tree(np.zeros(60), np.zeros(60), np.zeros(60), np.ones(60, np.uint8), _t2())
