@@ -33,8 +33,10 @@ typedef enum vpmfmm_status {
     VPMFMM_INTERNAL = 6,
     VPMFMM_DUPLICATE_POINTS = 7,     /* two particles of one list-1 pair coincide */
     VPMFMM_HIP_ERROR = 8,
-    VPMFMM_OUTSIDE_BOXES = 9         /* type-1: a particle outside every box of the
+    VPMFMM_OUTSIDE_BOXES = 9,        /* type-1: a particle outside every box of the
                                         reference's hierarchy (see vpmfmm_velocity_type1) */
+    VPMFMM_LEVEL1_LEAF = 10          /* 3-D type 2: a level-1 cell is a leaf while the tree
+                                        goes deeper (the reference's get_childID throws)   */
 } vpmfmm_status;
 
 typedef enum vpmfmm_precision {
@@ -262,7 +264,10 @@ int vpmfmm_velocity_type1_device(vpmfmm_handle *h, const vpmfmm_type1_opts *o, i
  * The 3-D vector FMM (the reference's default 3-D velocity, VELOCITY_3D_TYPE
  * 1: VelocityCalc::velocity_fmm_3d_fast, velocity_calc.cpp:2384-2525 ->
  * fmm3D::calcVelocityFast, src/FMM/fast3DFMM.cpp:32-92, helpers
- * src/FMM/fmm3D.cpp:84-210).  FP64.
+ * src/FMM/fmm3D.cpp:84-210; and, with opts.type 2, VELOCITY_3D_TYPE 2:
+ * VelocityCalc::velocity_fmm_3d, velocity_calc.cpp:2172-2383 ->
+ * TreeCell::initializeTree + fmm3D::calcVelocity, treeCell.cpp:1043-1243,
+ * fmm3D.cpp:585-700, 1064-1205).  FP64.
  *
  * Octree of FMMCell::generateTree (fast3DFMM.cpp:485-745); order-2 Cartesian
  * moments of the vortex strengths alpha = omega s^3 / (4 pi); every particle
@@ -286,17 +291,27 @@ typedef struct vpmfmm_fmm3d_opts {
     double support;     /* opening factor (SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3: 2.0)  */
     double u_inf, v_inf, w_inf;  /* free stream added to every velocity (get_velocity,
                            velocity_calc.cpp:221-236; global.cpp:633-635: 1, 0, 0)   */
-    int32_t reserved[4];
+    int32_t type;       /* VELOCITY_3D_TYPE (velocity_calc.cpp:31): 1 (default)
+                           velocity_fmm_3d_fast, the FMMCell octree; 2 velocity_fmm_3d,
+                           fmm3D::calcVelocity over the 3-D TreeCell (root box of the
+                           particles alone, truncated cell indices, the leaf rule below,
+                           every particle of a near leaf summed, inactive ones too)   */
+    int32_t max_par;    /* type 2: a cell is a leaf with at most max_par particles     */
+    int32_t max_src;    /*   and at most max_src active ones (Pars::par_count_max 50,
+                           src_count_max 30, global.cpp:778-779), 0..4096             */
+    int32_t reserved;
 } vpmfmm_fmm3d_opts;
 
 void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o);
 
 /* (u, v, w) of every particle from positions (x, y, z), sizes s, vorticity
- * (wx, wy, wz) and active flags (inactive particles are targets, not
- * sources).  Host pointers for vpmfmm_velocity3d, device pointers (on the
- * handle's device) for vpmfmm_velocity3d_device.  VPMFMM_NONFINITE_INPUT for
- * a non-finite coordinate, VPMFMM_TREE_DEPTH_EXCEEDED for a tree deeper than
- * max_depth. */
+ * (wx, wy, wz) and active flags (inactive particles are targets; type 1
+ * never sums them, type 2 sums them in the near field).  Host pointers for
+ * vpmfmm_velocity3d, device pointers (on the handle's device) for
+ * vpmfmm_velocity3d_device.  VPMFMM_NONFINITE_INPUT for a non-finite
+ * coordinate, VPMFMM_TREE_DEPTH_EXCEEDED for a tree deeper than max_depth,
+ * VPMFMM_DEGENERATE_DOMAIN (type 2) for every particle at one point,
+ * VPMFMM_LEVEL1_LEAF (type 2) where the reference throws. */
 int vpmfmm_velocity3d(vpmfmm_handle *h, const vpmfmm_fmm3d_opts *o, int32_t n, const double *x,
                       const double *y, const double *z, const double *s, const double *wx,
                       const double *wy, const double *wz, const uint8_t *active, double *u,

@@ -64,6 +64,7 @@ inline void throw_status(int st, const char *where, vpmfmm_handle *h) {
     case VPMFMM_TREE_DEPTH_EXCEEDED:
     case VPMFMM_DUPLICATE_POINTS:
     case VPMFMM_OUTSIDE_BOXES:
+    case VPMFMM_LEVEL1_LEAF:
         throw std::range_error(msg);
     default:
         throw std::runtime_error(msg);

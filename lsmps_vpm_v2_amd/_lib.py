@@ -31,7 +31,7 @@ EXPORTS = (
 STATUS = {
     0: "OK", 1: "BAD_ARG", 2: "NONFINITE_INPUT", 3: "DEGENERATE_DOMAIN",
     4: "TREE_DEPTH_EXCEEDED", 5: "OUT_OF_MEMORY", 6: "INTERNAL", 7: "DUPLICATE_POINTS",
-    8: "HIP_ERROR", 9: "OUTSIDE_BOXES",
+    8: "HIP_ERROR", 9: "OUTSIDE_BOXES", 10: "LEVEL1_LEAF",
 }
 
 FP64, MIXED = 0, 1
@@ -67,7 +67,8 @@ class Fmm3dOpts(C.Structure):
     _fields_ = [
         ("crit_num", C.c_int32), ("max_depth", C.c_int32), ("exp_tree", C.c_double),
         ("center", C.c_double * 3), ("support", C.c_double), ("u_inf", C.c_double),
-        ("v_inf", C.c_double), ("w_inf", C.c_double), ("reserved", C.c_int32 * 4),
+        ("v_inf", C.c_double), ("w_inf", C.c_double), ("type", C.c_int32),
+        ("max_par", C.c_int32), ("max_src", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -1185,6 +1185,9 @@ void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o) {
     o->exp_tree = 1.0;       // Pars::expTree (global.cpp:769)
     o->support = 2.0;        // SUPPORT_RADIUS_FACTOR (fmm3D.cpp:3)
     o->u_inf = 1.0;          // Pars::u_inf, v_inf, w_inf (global.cpp:633-635)
+    o->type = 1;             // VELOCITY_3D_TYPE (velocity_calc.cpp:31)
+    o->max_par = 50;         // TreeCell: Pars::par_count_max (global.cpp:778)
+    o->max_src = 30;         //           Pars::src_count_max (global.cpp:779)
 }
 
 namespace {
@@ -1198,6 +1201,9 @@ Fmm3Params fmm3_params(const vpmfmm_fmm3d_opts &o) {
     p.u_inf = o.u_inf;
     p.v_inf = o.v_inf;
     p.w_inf = o.w_inf;
+    p.type = o.type;
+    p.max_par = o.max_par;
+    p.max_src = o.max_src;
     return p;
 }
 

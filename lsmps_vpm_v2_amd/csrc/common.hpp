@@ -30,6 +30,7 @@ enum Status : int {
     kDuplicate = 7,
     kHipError = 8,
     kOutsideBoxes = 9,
+    kLevel1Leaf = 10,
 };
 
 struct Error : std::runtime_error {

@@ -1,6 +1,10 @@
-// fmm3d.hip -- the 3-D vector treecode of the reference's default 3-D path
-// (VELOCITY_3D_TYPE 1, src/FMM/fast3DFMM.cpp with the fmm3D.cpp helpers) on
-// gfx950, FP64.
+// fmm3d.hip -- the 3-D vector treecodes of the reference on gfx950, FP64:
+// the default 3-D path (VELOCITY_3D_TYPE 1, src/FMM/fast3DFMM.cpp with the
+// fmm3D.cpp helpers) and the second one (VELOCITY_3D_TYPE 2,
+// fmm3D::calcVelocity over the 3-D TreeCell, fmm3D.cpp:585-700, 1064-1205;
+// treeCell.cpp:1043-1243), which shares the walk and the moments and differs
+// in the octree (root box, cell index rule, leaf rule) and in summing every
+// particle of a near leaf, inactive ones included.  Type 1 below unless said.
 //
 // Reference semantics kept exactly:
 //   * octree (FMMCell::generateTree, fast3DFMM.cpp:485-745): root box = the
@@ -48,7 +52,7 @@ namespace vpm {
 namespace {
 
 constexpr int kHalf = kFmm3Depth / 2;  // levels per key word
-constexpr unsigned kF3NonFinite = 1u, kF3Deeper = 2u, kF3Depth = 4u;
+constexpr unsigned kF3NonFinite = 1u, kF3Deeper = 2u, kF3Depth = 4u, kF3Degenerate = 8u;
 constexpr int kMomStride = 32;
 constexpr int kEvalBlock = 256;
 
@@ -126,14 +130,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Root of FMMCell::generateTree (fast3DFMM.cpp:507-531): the box starts at
-// the domain centre, side = max extent * (1 + expTree / 100).  Also clears
-// the header's counters.
+// The root.  Type 1, FMMCell::generateTree (fast3DFMM.cpp:507-531): the
+// box starts at the domain centre, side = max extent * (1 + expTree / 100),
+// centre = the box midpoint.  Type 2, TreeCell::initializeTree
+// (treeCell.cpp:1066-1091): the box of the particles alone, side = the
+// longest extent * (1 + expTree / 100), pivot (lower corner) = (max + min -
+// side) / 2.  Also clears the header's counters.
 __global__ void __launch_bounds__(256)
     k3_root(const double *__restrict__ part, int nb, double c0, double c1, double c2,
-            double exp_tree, Fmm3Hdr *__restrict__ h) {
+            double exp_tree, int type, Fmm3Hdr *__restrict__ h) {
     __shared__ double sm[4][7];
+    const bool t2 = type == 2;
     double mn[3] = {c0, c1, c2}, mx[3] = {c0, c1, c2}, bad = 0.0;
+    if (t2)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = INFINITY;
+            mx[d] = -INFINITY;
+        }
     for (int b = threadIdx.x; b < nb; b += 256) {
         for (int d = 0; d < 3; ++d) {
             mn[d] = dmin(mn[d], part[(size_t)b * 8 + d]);
@@ -168,14 +181,30 @@ __global__ void __launch_bounds__(256)
             sm[0][6] = dmax(sm[0][6], sm[q][6]);
         }
         double root = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const double len = sm[0][3 + d] - sm[0][d];
-            root = (len < root) ? root : len;  // std::max(len, root)
-            h->root_c[d] = 0.5 * (sm[0][3 + d] + sm[0][d]);
+        unsigned f = sm[0][6] != 0.0 ? kF3NonFinite : 0u;
+        if (!t2) {
+            for (int d = 0; d < 3; ++d) {
+                const double len = sm[0][3 + d] - sm[0][d];
+                root = (len < root) ? root : len;  // std::max(len, root)
+                h->root_c[d] = 0.5 * (sm[0][3 + d] + sm[0][d]);
+                h->pivot[d] = 0.0;
+            }
+            root *= (1 + (exp_tree / 100));
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                const double side = fabs(sm[0][3 + d] - sm[0][d]);
+                if (side > root) root = side;
+            }
+            root *= 1.0 + (exp_tree / 100.0);
+            for (int d = 0; d < 3; ++d) {
+                h->pivot[d] = (sm[0][3 + d] + sm[0][d] - root) / 2.0;
+                h->root_c[d] = h->pivot[d] + (0.5 * root);  // the root cell's centre
+            }
+            if (!(root > 0.0)) f |= kF3Degenerate;
         }
-        root *= (1 + (exp_tree / 100));
         h->root_len = root;
-        h->flags = sm[0][6] != 0.0 ? kF3NonFinite : 0u;
+        h->flags = f;
+        h->min_settle = 0;
         h->depth = 0;
         h->n_leaves = 0;
         h->pairs = 0;
@@ -192,27 +221,53 @@ __global__ void __launch_bounds__(256)
 // the parent centre, then the child centre from the parent's).  hi holds
 // levels 1..10, lo levels 11..20; sk receives the word the first sort pass
 // orders (hi for a 10-level sort, lo for the first of a 20-level one).
+//
+// Type 2 (TreeCell): the level-L index is (int)((x - pivot) / (side / 2^L))
+// (coord2index, treeCell.cpp:290-298).  side / 2^L is exact and scaling by a
+// power of two commutes with rounding, so that index is the top L bits of
+// t = (x - pivot) / side taken to 20 binary places: one division per axis,
+// and every level's index nests in the next.
 __global__ void __launch_bounds__(256)
     k3_keys(const double *__restrict__ x, const double *__restrict__ y,
             const double *__restrict__ z, int n, const Fmm3Hdr *__restrict__ h,
             uint32_t *__restrict__ khi, uint32_t *__restrict__ klo, uint32_t *__restrict__ sk,
-            uint32_t *__restrict__ sv, int deep) {
+            uint32_t *__restrict__ sv, int deep, int type) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    double cx = h->root_c[0], cy = h->root_c[1], cz = h->root_c[2], sz = h->root_len;
     const double px = x[i], py = y[i], pz = z[i];
     uint32_t hi = 0, lo = 0;
-    for (int L = 1; L <= kFmm3Depth; ++L) {
-        const uint32_t o = (px > cx ? 1u : 0u) | (py > cy ? 2u : 0u) | (pz > cz ? 4u : 0u);
-        if (L <= kHalf)
-            hi = (hi << 3) | o;
-        else
-            lo = (lo << 3) | o;
-        const double cl = sz / 2.0;
-        cx = cx + cl * ((double)(o & 1u) - 0.5);
-        cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
-        cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
-        sz = cl;
+    if (type == 2) {
+        const double R = h->root_len;
+        uint32_t id[3];
+        const double p[3] = {px, py, pz};
+        for (int d = 0; d < 3; ++d) {
+            const double t = (p[d] - h->pivot[d]) / R;
+            const double sc = t * (double)(1u << kFmm3Depth);
+            id[d] = sc >= 0.0 && sc < (double)(1u << kFmm3Depth) ? (uint32_t)sc : 0u;
+        }
+        for (int L = 1; L <= kFmm3Depth; ++L) {
+            const int sh = kFmm3Depth - L;
+            const uint32_t o = ((id[0] >> sh) & 1u) | (((id[1] >> sh) & 1u) << 1) |
+                               (((id[2] >> sh) & 1u) << 2);
+            if (L <= kHalf)
+                hi = (hi << 3) | o;
+            else
+                lo = (lo << 3) | o;
+        }
+    } else {
+        double cx = h->root_c[0], cy = h->root_c[1], cz = h->root_c[2], sz = h->root_len;
+        for (int L = 1; L <= kFmm3Depth; ++L) {
+            const uint32_t o = (px > cx ? 1u : 0u) | (py > cy ? 2u : 0u) | (pz > cz ? 4u : 0u);
+            if (L <= kHalf)
+                hi = (hi << 3) | o;
+            else
+                lo = (lo << 3) | o;
+            const double cl = sz / 2.0;
+            cx = cx + cl * ((double)(o & 1u) - 0.5);
+            cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
+            cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
+            sz = cl;
+        }
     }
     khi[i] = hi;
     klo[i] = lo;
@@ -278,6 +333,32 @@ __device__ __forceinline__ int common_levels(uint64_t a, uint64_t b, int D) {
     return min(m, D);
 }
 
+// Per-block depth, leaf count and shallowest leaf level (k3_settle_stats
+// combines them); the rare unsettled flag is the only atomic.
+__device__ __forceinline__ void settle_block_stats(bool valid, int st, int mp, bool unsettled,
+                                                   int D, uint4 *__restrict__ bstat,
+                                                   Fmm3Hdr *__restrict__ h) {
+    __shared__ uint32_t s_d[4], s_l[4], s_m[4];
+    int dmax_ = st, dmin_ = valid && st > 0 ? st : 255;
+    for (int m = 1; m < kWave; m <<= 1) {
+        dmax_ = max(dmax_, __shfl_xor(dmax_, m, kWave));
+        dmin_ = min(dmin_, __shfl_xor(dmin_, m, kWave));
+    }
+    const uint64_t leaves = __ballot(valid && st > mp);
+    const bool any_unsettled = __ballot(unsettled) != 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_d[threadIdx.x / kWave] = (uint32_t)dmax_;
+        s_l[threadIdx.x / kWave] = (uint32_t)__popcll(leaves);
+        s_m[threadIdx.x / kWave] = (uint32_t)dmin_;
+        if (any_unsettled) atomicOr(&h->flags, D < kFmm3Depth ? kF3Deeper : kF3Depth);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        bstat[blockIdx.x] = make_uint4(max(max(s_d[0], s_d[1]), max(s_d[2], s_d[3])),
+                                       s_l[0] + s_l[1] + s_l[2] + s_l[3],
+                                       min(min(s_m[0], s_m[1]), min(s_m[2], s_m[3])), 0u);
+}
+
 // lcpw[j] (j <= n - crit): levels shared by the ends of the window of crit
 // sorted particles starting at j.  The cell of particle i at level L holds
 // crit particles or more exactly when some window containing i has
@@ -297,7 +378,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k3_settle(const uint64_t *__restrict__ key, const uint8_t *__restrict__ lcpw, int n, int D,
               int crit, uint8_t *__restrict__ settle, uint8_t *__restrict__ mprev,
-              uint32_t *__restrict__ starts, uint2 *__restrict__ bstat,
+              uint32_t *__restrict__ starts, uint4 *__restrict__ bstat,
               Fmm3Hdr *__restrict__ h) {
     extern __shared__ uint8_t s_w[];
     const int i0 = blockIdx.x * 256;
@@ -319,75 +400,126 @@ __global__ void __launch_bounds__(256)
         mprev[i] = (uint8_t)mp;
         starts[i] = st > mp ? (uint32_t)(st - mp) : 0u;
     }
-    // per-block depth and leaf count (k3_settle_stats sums them); the rare
-    // unsettled flag is the only atomic
-    __shared__ uint32_t s_d[4], s_l[4];
-    int dmax_ = st;
-    for (int m = 1; m < kWave; m <<= 1) dmax_ = max(dmax_, __shfl_xor(dmax_, m, kWave));
-    const uint64_t leaves = __ballot(i < n && st > mp);
-    const bool any_unsettled = __ballot(unsettled) != 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        s_d[threadIdx.x / kWave] = (uint32_t)dmax_;
-        s_l[threadIdx.x / kWave] = (uint32_t)__popcll(leaves);
-        if (any_unsettled) atomicOr(&h->flags, D < kFmm3Depth ? kF3Deeper : kF3Depth);
+    settle_block_stats(i < n, st, mp, unsettled, D, bstat, h);
+}
+
+// Type 2 (TreeCell): the first level whose cell around particle i holds at
+// most max_par particles and max_src active ones (the cell's range from two
+// binary searches inside the parent's range; its active count from the
+// active scan A).
+__global__ void __launch_bounds__(256)
+    k3_settle_t2(const uint64_t *__restrict__ key, const uint32_t *__restrict__ A, int n, int D,
+                 int max_par, int max_src, uint8_t *__restrict__ settle,
+                 uint8_t *__restrict__ mprev, uint32_t *__restrict__ starts,
+                 uint4 *__restrict__ bstat, Fmm3Hdr *__restrict__ h) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int st = 0, mp = 0;
+    bool unsettled = false;
+    if (i < n) {
+        const uint64_t ki = key[i];
+        int lo = 0, hi = n;
+        for (int L = 1; L <= D; ++L) {
+            const uint64_t p = pfx(ki, L);
+            int a = lo, b = i;
+            while (a < b) {
+                const int m = (a + b) >> 1;
+                if (pfx(key[m], L) < p) a = m + 1;
+                else b = m;
+            }
+            int e0 = i + 1, e1 = hi;
+            while (e0 < e1) {
+                const int m = (e0 + e1) >> 1;
+                if (pfx(key[m], L) > p) e1 = m;
+                else e0 = m + 1;
+            }
+            if (e0 - a <= max_par && (int)(A[e0] - A[a]) <= max_src) {
+                st = L;
+                break;
+            }
+            lo = a;
+            hi = e0;
+        }
+        if (i > 0) mp = common_levels(key[i - 1], ki, D);
+        unsettled = st == 0;
+        settle[i] = (uint8_t)st;
+        mprev[i] = (uint8_t)mp;
+        starts[i] = st > mp ? (uint32_t)(st - mp) : 0u;
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        bstat[blockIdx.x] = make_uint2(max(max(s_d[0], s_d[1]), max(s_d[2], s_d[3])),
-                                       s_l[0] + s_l[1] + s_l[2] + s_l[3]);
+    settle_block_stats(i < n, st, mp, unsettled, D, bstat, h);
 }
 
 __global__ void __launch_bounds__(1024)
-    k3_settle_stats(const uint2 *__restrict__ bstat, int nb, Fmm3Hdr *__restrict__ h) {
-    __shared__ uint32_t sd[16], sl[16];
-    uint32_t d = 0, l = 0;
+    k3_settle_stats(const uint4 *__restrict__ bstat, int nb, Fmm3Hdr *__restrict__ h) {
+    __shared__ uint32_t sd[16], sl[16], sm[16];
+    uint32_t d = 0, l = 0, mn = 255;
     for (int q = threadIdx.x; q < nb; q += 1024) {
         d = max(d, bstat[q].x);
         l += bstat[q].y;
+        mn = min(mn, bstat[q].z);
     }
     for (int m = 1; m < kWave; m <<= 1) {
         d = max(d, (uint32_t)__shfl_xor((int)d, m, kWave));
         l += (uint32_t)__shfl_xor((int)l, m, kWave);
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, m, kWave));
     }
     if ((threadIdx.x & (kWave - 1)) == 0) {
         sd[threadIdx.x / kWave] = d;
         sl[threadIdx.x / kWave] = l;
+        sm[threadIdx.x / kWave] = mn;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t td = 0, tl = 0;
+        uint32_t td = 0, tl = 0, tm = 255;
         for (int q = 0; q < 16; ++q) {
             td = max(td, sd[q]);
             tl += sl[q];
+            tm = min(tm, sm[q]);
         }
         h->depth = td;
         h->n_leaves = tl;
+        h->min_settle = tm;
     }
 }
 
 // Cells of particle i (levels mprev + 1 .. settle) at pre-order positions
-// pos[i] ...: centre (replayed from the root), (support size)^2, particle
-// range, skip pointer (the first cell of the first particle past the range),
-// level, leaf, active (an active particle in the range).
+// pos[i] ...: centre, (support size)^2, particle range, skip pointer (the
+// first cell of the first particle past the range), level, leaf, active (an
+// active particle in the range).  The centre is replayed from the root
+// (type 1: parent + (size / 2)(bit - 1/2)) or formed from the level's index
+// (type 2: pivot + (0.5 + index) size, Cell::Cell, treeCell.hpp:68).  The
+// direct-sum sources of a cell are its active particles (type 1) or all of
+// them (type 2, fmm3D.cpp:1172-1186 reads no active mark).
 __global__ void __launch_bounds__(256)
     k3_emit(const uint64_t *__restrict__ key, const uint8_t *__restrict__ settle,
             const uint8_t *__restrict__ mprev, const uint32_t *__restrict__ pos,
             const uint32_t *__restrict__ A, int n, uint32_t ncell,
-            const Fmm3Hdr *__restrict__ h, double support, Cell3 *__restrict__ cells) {
+            const Fmm3Hdr *__restrict__ h, double support, int type,
+            Cell3 *__restrict__ cells) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int st = settle[i], mp = mprev[i];
     if (st <= mp) return;
+    const bool t2 = type == 2;
     const uint64_t ki = key[i];
     double cx = h->root_c[0], cy = h->root_c[1], cz = h->root_c[2], sz = h->root_len;
+    uint32_t ix = 0, iy = 0, iz = 0;
     uint32_t p = pos[i];
     int hi = n;
     for (int L = 1; L <= st; ++L) {
         const uint32_t o = (uint32_t)(pfx(ki, L) & 7u);
         const double cl = sz / 2.0;
-        cx = cx + cl * ((double)(o & 1u) - 0.5);
-        cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
-        cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
+        if (t2) {
+            ix = 2u * ix + (o & 1u);
+            iy = 2u * iy + ((o >> 1) & 1u);
+            iz = 2u * iz + ((o >> 2) & 1u);
+            cx = h->pivot[0] + ((0.5 + (double)ix) * cl);
+            cy = h->pivot[1] + ((0.5 + (double)iy) * cl);
+            cz = h->pivot[2] + ((0.5 + (double)iz) * cl);
+        } else {
+            cx = cx + cl * ((double)(o & 1u) - 0.5);
+            cy = cy + cl * ((double)((o >> 1) & 1u) - 0.5);
+            cz = cz + cl * ((double)((o >> 2) & 1u) - 0.5);
+        }
         sz = cl;
         if (L <= mp) continue;
         const uint64_t pf = pfx(ki, L);
@@ -407,8 +539,8 @@ __global__ void __launch_bounds__(256)
         c.begin = (uint32_t)i;
         c.end = (uint32_t)e0;
         c.skip = e0 < n ? pos[e0] : ncell;
-        c.abegin = A[i];
-        c.aend = A[e0];
+        c.abegin = t2 ? (uint32_t)i : A[i];
+        c.aend = t2 ? (uint32_t)e0 : A[e0];
         c.pad0 = c.pad1 = 0u;
         const bool act = A[e0] > A[i];
         c.meta = (uint32_t)L | (L == st ? 0x100u : 0u) | (act ? 0x200u : 0u);
@@ -417,12 +549,14 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// The active particles in tree order (A: exclusive scan of the flags).
+// The direct-sum sources in tree order: the active particles (A: exclusive
+// scan of the flags) or, with all, every particle; pad holds the active
+// flag (P2M reads active particles only).
 __global__ void __launch_bounds__(256)
-    k3_compact(const Par3 *__restrict__ P, const uint32_t *__restrict__ A, int n,
+    k3_compact(const Par3 *__restrict__ P, const uint32_t *__restrict__ A, int n, int all,
                Src3 *__restrict__ S) {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= n || A[j + 1] == A[j]) return;
+    if (j >= n || (!all && A[j + 1] == A[j])) return;
     const Par3 q = P[j];
     Src3 r;
     r.x = q.x;
@@ -432,8 +566,8 @@ __global__ void __launch_bounds__(256)
     r.ax = q.ax;
     r.ay = q.ay;
     r.az = q.az;
-    r.pad = 0.0;
-    S[A[j]] = r;
+    r.pad = q.act;
+    S[all ? (uint32_t)j : A[j]] = r;
 }
 
 // -------------------------------------------------------------------------
@@ -461,6 +595,7 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < 30; ++k) m[k] = 0.0;
     for (uint32_t j = C.abegin; j < C.aend; ++j) {
         const Src3 q = S[j];
+        if (q.pad == 0.0) continue;
         const double dx = C.cx - q.x, dy = C.cy - q.y, dz = C.cz - q.z;
         const double t[10] = {1.0,          dx,           dy,           dz,      dx * dx * 0.5,
                               dy * dy * 0.5, dz * dz * 0.5, dx * dy,      dy * dz, dx * dz};
@@ -602,7 +737,7 @@ __global__ void __launch_bounds__(kEvalBlock)
             const Par3 *__restrict__ P, const Src3 *__restrict__ S,
             const uint32_t *__restrict__ perm, int n, double u_inf, double v_inf, double w_inf,
             double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
-            unsigned long long *__restrict__ wstat) {
+            int src_all, unsigned long long *__restrict__ wstat) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -672,8 +807,10 @@ __global__ void __launch_bounds__(kEvalBlock)
                         vv += (q.az * ex - q.ax * ez) * i3;
                         vw += (q.ax * ey - q.ay * ex) * i3;
                     }
-                    // the target itself is a source of its own leaf when active
-                    const bool self = my_act && (uint32_t)i >= C.begin && (uint32_t)i < C.end;
+                    // the target itself is a source of its own leaf (when
+                    // active, unless every particle is a source)
+                    const bool self = (src_all || my_act) && (uint32_t)i >= C.begin &&
+                                      (uint32_t)i < C.end;
                     npair += ae - ab - (self ? 1u : 0u);
                 }
             }
@@ -743,10 +880,18 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     if (n < 1) throw Error(kBadArg, "n must be >= 1");
     if (!x || !y || !z || !wx || !wy || !wz || !active || !u || !v || !w)
         throw Error(kBadArg, "null pointer");
-    if (p.crit_num < 1 || p.crit_num > 4096) throw Error(kBadArg, "crit_num must be in [1, 4096]");
+    if (p.type != 1 && p.type != 2) throw Error(kBadArg, "type must be 1 or 2");
+    const bool t2 = p.type == 2;
+    if (!t2 && (p.crit_num < 1 || p.crit_num > 4096))
+        throw Error(kBadArg, "crit_num must be in [1, 4096]");
+    if (t2 && (p.max_par < 0 || p.max_par > 4096 || p.max_src < 0 || p.max_src > 4096))
+        throw Error(kBadArg, "max_par and max_src must be in [0, 4096]");
     if (p.max_depth < 1 || p.max_depth > kFmm3Depth)
         throw Error(kBadArg, "max_depth must be in [1, 20]");
     if (!(p.exp_tree >= 0.0) || !std::isfinite(p.exp_tree)) throw Error(kBadArg, "exp_tree must be >= 0");
+    // with no expansion the type-2 box's upper face holds a particle whose
+    // index is out of range (the reference's index2ID throws there)
+    if (t2 && !(p.exp_tree > 0.0)) throw Error(kBadArg, "exp_tree must be > 0 for type 2");
     if (!(p.support > 0.0) || !std::isfinite(p.support)) throw Error(kBadArg, "support must be > 0");
     for (double q : {p.center[0], p.center[1], p.center[2], p.u_inf, p.v_inf, p.w_inf})
         if (!std::isfinite(q)) throw Error(kBadArg, "centre and free stream must be finite");
@@ -783,8 +928,8 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
         const int deep = D > kHalf;
         // (the root kernel also clears the header's flags and counters)
         k3_root<<<1, 256, 0, s>>>(part, nbb, p.center[0], p.center[1], p.center[2], p.exp_tree,
-                                  hd);
-        k3_keys<<<nb, 256, 0, s>>>(x, y, z, n, hd, khi, klo, ka, va, deep);
+                                  p.type, hd);
+        k3_keys<<<nb, 256, 0, s>>>(x, y, z, n, hd, khi, klo, ka, va, deep, p.type);
         const bool alt = radix_sort_pairs(ka, va, kb, vb, n, 0, 3 * kHalf, tmp, s);
         uint32_t *perm = alt ? vb : va;
         if (deep) {
@@ -797,22 +942,30 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
         c.perm = perm;
         k3_gather<<<nb, 256, 0, s>>>(perm, khi, klo, x, y, z, sz, wx, wy, wz, active, n, P, key,
                                      act8);
-        uint8_t *lcpw = c.lcpw.as<uint8_t>(n);
-        if (n >= p.crit_num)
-            k3_lcpw<<<div_up(n - p.crit_num + 1, 256), 256, 0, s>>>(key, n, D, p.crit_num, lcpw);
-        uint2 *bstat = c.bstat.as<uint2>(nb);
-        k3_settle<<<nb, 256, (size_t)(256 + p.crit_num), s>>>(key, lcpw, n, D, p.crit_num,
-                                                                 settle, mprev, starts, bstat, hd);
+        scan_exclusive_u8(act8, A, n, &hd->n_active, true, tmp, s);
+        uint4 *bstat = c.bstat.as<uint4>(nb);
+        if (t2) {
+            k3_settle_t2<<<nb, 256, 0, s>>>(key, A, n, D, p.max_par, p.max_src, settle, mprev,
+                                            starts, bstat, hd);
+        } else {
+            uint8_t *lcpw = c.lcpw.as<uint8_t>(n);
+            if (n >= p.crit_num)
+                k3_lcpw<<<div_up(n - p.crit_num + 1, 256), 256, 0, s>>>(key, n, D, p.crit_num,
+                                                                        lcpw);
+            k3_settle<<<nb, 256, (size_t)(256 + p.crit_num), s>>>(
+                key, lcpw, n, D, p.crit_num, settle, mprev, starts, bstat, hd);
+        }
         k3_settle_stats<<<1, 1024, 0, s>>>(bstat, nb, hd);
         scan_exclusive_u32(starts, pos, n, &hd->n_cells, true, tmp, s);
-        scan_exclusive_u8(act8, A, n, &hd->n_active, true, tmp, s);
         VPM_LAUNCH_CHECK();
         VPM_HIP(hipMemcpyAsync(c.hdr_h, hd, sizeof(Fmm3Hdr), hipMemcpyDeviceToHost, s));
         VPM_HIP(hipStreamSynchronize(s));
         const Fmm3Hdr &hh = *c.hdr_h;
         if (hh.flags & kF3NonFinite) throw Error(kNonFinite, "non-finite particle coordinate");
+        if (hh.flags & kF3Degenerate)
+            throw Error(kDegenerate, "every particle at one point (the type-2 root has no size)");
         if (hh.flags & kF3Depth)
-            throw Error(kDepth, "octree deeper than 20 levels (at least crit_num coincident particles?)");
+            throw Error(kDepth, "octree deeper than 20 levels (too many coincident particles?)");
         const int want = ((hh.flags & kF3Deeper) || (int)hh.depth > kHalf) ? kFmm3Depth : kHalf;
         if (want != D && attempt == 0) {
             D = want;
@@ -824,6 +977,12 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     }
     const Fmm3Hdr hh = *c.hdr_h;
     if ((int)hh.depth > p.max_depth) throw Error(kDepth, "octree deeper than max_depth");
+    // type 2 asks every level-1 cell for its children (fmm3D.cpp:1150-1152);
+    // get_childID throws for a leaf (treeCell.cpp:384-388), so a level-1 leaf
+    // beside deeper cells has no result in the reference
+    if (t2 && hh.depth >= 2 && hh.min_settle == 1)
+        throw Error(kLevel1Leaf, "type 2: a level-1 cell is a leaf while the tree goes deeper "
+                                 "(the reference's get_childID throws there)");
     const uint32_t ncell = hh.n_cells;
     Fmm3Info &fi = c.info;
     fi.n = n;
@@ -837,9 +996,10 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
 
     Cell3 *cells = c.cells.as<Cell3>(ncell);
     double *mom = c.mom.as<double>((size_t)ncell * kMomStride);
-    Src3 *S = c.src.as<Src3>((size_t)std::max(1u, hh.n_active));
-    k3_emit<<<nb, 256, 0, s>>>(key, settle, mprev, pos, A, n, ncell, hd, p.support, cells);
-    k3_compact<<<nb, 256, 0, s>>>(P, A, n, S);
+    Src3 *S = c.src.as<Src3>((size_t)std::max(1u, t2 ? (uint32_t)n : hh.n_active));
+    k3_emit<<<nb, 256, 0, s>>>(key, settle, mprev, pos, A, n, ncell, hd, p.support, p.type,
+                               cells);
+    k3_compact<<<nb, 256, 0, s>>>(P, A, n, t2 ? 1 : 0, S);
     if (ev) VPM_HIP(hipEventRecord(ev[1], s));
     const int cb = div_up(ncell, 4);
     if (ncell > 0) {
@@ -850,7 +1010,8 @@ void fmm3d_velocity(Ctx3 &c, hipStream_t s, int n, const double *x, const double
     const int waves = div_up(n, kWave);
     unsigned long long *wstat = c.wstat.as<unsigned long long>(4 * (size_t)waves);
     k3_eval<<<div_up(waves, kEvalWaves), kEvalBlock, 0, s>>>(
-        cells, (int)ncell, mom, P, S, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, wstat);
+        cells, (int)ncell, mom, P, S, c.perm, n, p.u_inf, p.v_inf, p.w_inf, u, v, w, t2 ? 1 : 0,
+        wstat);
     if (ev) VPM_HIP(hipEventRecord(ev[3], s));
     k3_eval_stats<<<1, 1024, 0, s>>>(wstat, waves, hd);
     VPM_LAUNCH_CHECK();

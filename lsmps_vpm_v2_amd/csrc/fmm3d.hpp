@@ -28,6 +28,8 @@ struct Fmm3Params {
     double center[3] = {0.0, 0.0, 0.0};  // a point the root box always holds
     double support = 2.0;    // a cell is far when |r|^2 > (support size)^2
     double u_inf = 0.0, v_inf = 0.0, w_inf = 0.0;
+    int type = 1;            // 1: fast3DFMM (FMMCell); 2: fmm3D::calcVelocity (TreeCell)
+    int max_par = 50, max_src = 30;  // type 2 leaf rule: at most these many (active) particles
 };
 
 // Device header of a 3-D solve, read back by the host once per build.
@@ -39,7 +41,8 @@ struct Fmm3Hdr {
     uint32_t n_active;  // total of the active scan
     uint32_t depth;     // deepest leaf level
     uint32_t n_leaves;
-    uint32_t pad;
+    uint32_t min_settle;  // shallowest leaf level
+    double pivot[3];      // type 2: the root box's lower corner
     unsigned long long pairs, m2p;  // evaluation work: direct pairs, far-cell terms
     // wavefront steps: sources staged for near lanes, far cells with a far lane
     unsigned long long wave_pairs, wave_m2p;

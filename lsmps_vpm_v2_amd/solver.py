@@ -306,13 +306,17 @@ class FmmSolver:
     # ------------------------------------------------------------- 3-D path
     @staticmethod
     def fmm3d_opts(crit_num=30, max_depth=20, exp_tree=1.0, center=(0.0, 0.0, 0.0), support=2.0,
-                   u_inf=1.0, v_inf=0.0, w_inf=0.0) -> "_lib.Fmm3dOpts":
+                   u_inf=1.0, v_inf=0.0, w_inf=0.0, type=1, max_par=50,
+                   max_src=30) -> "_lib.Fmm3dOpts":
         """vpmfmm_fmm3d_opts; defaults are the reference's DIM == 3 parameters
         (critNum = src_count_max 30, expTree 1.0, centre 0, free stream
-        (1, 0, 0): global.cpp:609-611, 633-635, 769, 779) and
-        SUPPORT_RADIUS_FACTOR 2.0 (fmm3D.cpp:3)."""
+        (1, 0, 0), par_count_max 50: global.cpp:609-611, 633-635, 769,
+        778-779), SUPPORT_RADIUS_FACTOR 2.0 (fast3DFMM.cpp:4) and
+        VELOCITY_3D_TYPE 1 (velocity_calc.cpp:31; type 2 is fmm3D::calcVelocity
+        over the TreeCell, with the max_par / max_src leaf rule)."""
         o = _lib.Fmm3dOpts()
         o.crit_num, o.max_depth, o.exp_tree, o.support = crit_num, max_depth, exp_tree, support
+        o.type, o.max_par, o.max_src = type, max_par, max_src
         for d in range(3):
             o.center[d] = center[d]
         o.u_inf, o.v_inf, o.w_inf = u_inf, v_inf, w_inf
@@ -565,14 +569,17 @@ class VelocityCalc:
             velocity_type = 2 if dim == 2 else 1
         if dim == 2 and velocity_type not in (1, 2):
             raise ValueError("velocity_type must be 1 (type-1 FMM) or 2 (FMM)")
-        if dim == 3 and velocity_type != 1:
-            raise ValueError("the 3-D velocity type must be 1 (velocity_fmm_3d_fast)")
+        if dim == 3 and velocity_type not in (1, 2):
+            raise ValueError("the 3-D velocity type must be 1 (velocity_fmm_3d_fast) or 2 "
+                             "(velocity_fmm_3d)")
         self.dim = dim
         self.velocity_type = velocity_type
         self.type1 = dict(type1 or {})
         self._solver = FmmSolver(**solver_opts)
         o = self._solver.opts
         self.fmm3d = {"u_inf": o.u_inf, "v_inf": o.v_inf, **(fmm3d or {})}
+        if dim == 3:
+            self.fmm3d.setdefault("type", velocity_type)
 
     @property
     def solver(self) -> FmmSolver:

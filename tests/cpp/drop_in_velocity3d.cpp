@@ -7,6 +7,7 @@
 //
 //   drop_in_velocity3d <n> <out.bin>
 // writes n, then x, y, z, s, vortx, vorty, vortz, active (as double), u, v, w
+// of the default branch (VELOCITY_3D_TYPE 1) and u, v, w of VELOCITY_3D_TYPE 2
 // (n doubles each).
 #define DIM 3
 #include <cmath>
@@ -69,6 +70,18 @@ int main(int argc, char **argv) {
     } catch (const std::range_error &) {
     }
     if ((int)p.w.size() != n) return 1;
+    // VELOCITY_3D_TYPE 2 (velocity_fmm_3d: the TreeCell octree)
+    std::vector<double> u1 = p.u, v1 = p.v, w1 = p.w;
+    vpmfmm_fmm3d_opts o3;
+    vpmfmm_fmm3d_default_opts(&o3);
+    o3.type = 2;
+    vpmfmm::set_default_fmm3d_options(o3);
+    try {
+        velocity_tool.get_velocity(p, grid, 0);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "get_velocity (type 2) failed: %s\n", e.what());
+        return 1;
+    }
     FILE *f = std::fopen(argv[2], "wb");
     if (!f) return 1;
     const double nn = n;
@@ -79,7 +92,7 @@ int main(int argc, char **argv) {
         const double a = p.isActive[i] ? 1.0 : 0.0;
         std::fwrite(&a, sizeof(double), 1, f);
     }
-    for (const std::vector<double> *a : {&p.u, &p.v, &p.w})
+    for (const std::vector<double> *a : {&u1, &v1, &w1, &p.u, &p.v, &p.w})
         std::fwrite(a->data(), sizeof(double), n, f);
     std::fclose(f);
     std::printf("3-D drop-in: %d particles\n", n);

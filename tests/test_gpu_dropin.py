@@ -44,11 +44,18 @@ def test_cpp_drop_in_3d_matches_oracle(gpu, oracle, tmp_path):
     assert r.returncode == 0
     raw = np.fromfile(out, dtype=np.float64)
     assert int(raw[0]) == n
-    x, y, z, s, wx, wy, wz, a, u, v, w = raw[1:].reshape(11, n)
+    x, y, z, s, wx, wy, wz, a, u, v, w, u2, v2, w2 = raw[1:].reshape(14, n)
     # the reference's DIM == 3 defaults: critNum 30, free stream (1, 0, 0)
     uo, vo, wo = oracle.o3_velocity(x, y, z, s, wx, wy, wz, a.astype(np.uint8),
                                     oracle.o3_opts(), 1.0, 0.0, 0.0)
     scale = np.sqrt(np.mean((uo - 1.0) ** 2 + vo ** 2 + wo ** 2))
     err = np.max(np.sqrt((u - uo) ** 2 + (v - vo) ** 2 + (w - wo) ** 2)) / scale
     print(f"3-D drop-in vs oracle: max |d(u,v,w)| / rms = {err:.2e}")
+    assert err <= 1e-10
+    # VELOCITY_3D_TYPE 2 through set_default_fmm3d_options
+    uo, vo, wo = oracle.o3_velocity(x, y, z, s, wx, wy, wz, a.astype(np.uint8),
+                                    oracle.o3_opts(type=2), 1.0, 0.0, 0.0)
+    scale = np.sqrt(np.mean((uo - 1.0) ** 2 + vo ** 2 + wo ** 2))
+    err = np.max(np.sqrt((u2 - uo) ** 2 + (v2 - vo) ** 2 + (w2 - wo) ** 2)) / scale
+    print(f"3-D drop-in type 2 vs oracle: max |d(u,v,w)| / rms = {err:.2e}")
     assert err <= 1e-10

@@ -35,6 +35,12 @@ def _cloud(n, seed, kind="uniform", spread=0.08):
     g = np.random.Generator(np.random.PCG64(seed))
     if kind == "uniform":
         p = g.uniform(-1.0, 1.0, (3, n))
+    elif kind == "mixed":  # clusters over a uniform background (every root octant dense)
+        c = g.uniform(-1.0, 1.0, (5, 3))
+        k = g.integers(0, 5, n)
+        p = (c[k] + spread * g.standard_normal((n, 3))).T
+        bg = g.random(n) < 0.3
+        p[:, bg] = g.uniform(-1.0, 1.0, (3, int(bg.sum())))
     else:
         c = g.uniform(-1.0, 1.0, (5, 3))
         k = g.integers(0, 5, n)
@@ -233,3 +239,109 @@ def test_velocity_calc_dim3(solver):
     vc.get_velocity(p)
     ref = _oracle(x, y, z, s, wx, wy, wz, act, free=(1.0, 0.0, 0.0))
     assert _err((p.u, p.v, p.w), ref, (1.0, 0.0, 0.0)) <= TOL
+
+
+# ---------------------------------------------------------------------------
+# type 2: fmm3D::calcVelocity over the 3-D TreeCell (VELOCITY_3D_TYPE 2)
+# ---------------------------------------------------------------------------
+def _oracle_t2(x, y, z, s, wx, wy, wz, act, free=FREE, **kw):
+    from oracle import oracle as O
+    return O.o3_velocity(x, y, z, s, wx, wy, wz, act, O.o3_opts(type=2, **kw), *free)
+
+
+T2_CASES = [
+    ("uniform_20k", 20000, "uniform", 1.0),
+    ("mixed_30k_inactive", 30000, "mixed", 0.7),
+]
+
+
+@pytest.mark.parametrize("name,n,kind,afrac", T2_CASES, ids=[c[0] for c in T2_CASES])
+def test_type2_velocity_matches_oracle(solver, name, n, kind, afrac):
+    x, y, z, s, wx, wy, wz = _cloud(n, 43, kind)
+    act = (np.random.default_rng(6).random(n) < afrac).astype(np.uint8)
+    print(f"3-D type 2 velocity {name} n={n}", flush=True)
+    got = solver.velocity3d(x, y, z, s, wx, wy, wz, act, type=2)
+    ref = _oracle_t2(x, y, z, s, wx, wy, wz, act)
+    e = _err(got, ref, FREE)
+    st = solver.fmm3d_stats()
+    print(f"  err {e:.2e} cells {st['n_cells']} depth {st['depth']} pairs {st['pairs']}")
+    assert e <= TOL
+    # other leaf limits
+    got = solver.velocity3d(x, y, z, s, wx, wy, wz, act, type=2, max_par=20, max_src=12)
+    ref = _oracle_t2(x, y, z, s, wx, wy, wz, act, max_par=20, max_src=12)
+    assert _err(got, ref, FREE) <= TOL
+
+
+@pytest.mark.parametrize("kind,spread", [("uniform", 0.0), ("mixed", 0.08),
+                                         ("mixed", 2e-5)],
+                         ids=["uniform", "mixed", "deep_mixed"])
+def test_type2_octree_matches_oracle(solver, kind, spread):
+    from oracle import oracle as O
+    n = 12000
+    x, y, z, s, wx, wy, wz = _cloud(n, 47, kind, spread)
+    act = (np.random.default_rng(9).random(n) < 0.8).astype(np.uint8)
+    print(f"3-D type 2 octree {kind} spread={spread}", flush=True)
+    solver.velocity3d(x, y, z, s, wx, wy, wz, act, type=2)
+    st = solver.fmm3d_stats()
+    G = solver.fmm3d_cells()
+    T = O.O3Tree(x, y, z, act, O.o3_opts(type=2))
+    m = T.level > 0
+    ref = _cells_by_key(T.level[m], T.key[m], T.count[m], T.leaf[m], T.active[m],
+                        T.centre[m], T.size[m])
+    got = _cells_by_key(G["level"], G["key"], G["count"], G["leaf"], G["active"], G["centre"],
+                        G["size"])
+    assert st["depth"] == T.depth
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(st["root_centre"], T.centre[0])
+    assert st["root_side"] == T.size[0]
+    perm = solver.fmm3d_perm()
+    idx = {(int(l), int(k)): c for c, (l, k) in enumerate(zip(T.level, T.key))}
+    begin = 0
+    for j in range(len(G["level"])):
+        if G["leaf"][j]:
+            c = int(G["count"][j])
+            np.testing.assert_array_equal(np.sort(perm[begin:begin + c]),
+                                          T.particles(idx[(int(G["level"][j]),
+                                                           int(G["key"][j]))]))
+            begin += c
+    assert begin == n
+
+
+def test_type2_ring_edges_and_mirror(solver):
+    from lsmps_vpm_v2_amd import Particle, VelocityCalc, VpmFmmError, workloads as W
+    f = W.vortex_ring(60000, jitter=0.25, active_frac=0.9)
+    cols = [f[k] for k in ("x", "y", "z", "s", "wx", "wy", "wz")]
+    print(f"3-D type 2 vortex ring n={f['n']}", flush=True)
+    got = solver.velocity3d(*cols, f["active"], type=2)
+    ref = _oracle_t2(*cols, f["active"])
+    assert _err(got, ref, FREE) <= TOL
+    # a level-1 leaf beside deeper cells: the reference throws; here LEVEL1_LEAF
+    x, y, z, s, wx, wy, wz = _cloud(20000, 41)
+    few = np.flatnonzero((x > 0) & (y > 0) & (z > 0))
+    keep = np.setdiff1d(np.arange(20000), few[20:])
+    cols = [t[keep] for t in (x, y, z, s, wx, wy, wz)]
+    with pytest.raises(VpmFmmError) as e:
+        solver.velocity3d(*cols, np.ones(len(keep), np.uint8), type=2)
+    assert e.value.code == 10
+    # one level deep: the walk never starts, the free stream alone
+    u, v, w = solver.velocity3d(x[:40], y[:40], z[:40], s[:40], wx[:40], wy[:40], wz[:40],
+                                np.ones(40, np.uint8), type=2, u_inf=0.5)
+    assert np.all(u == 0.5) and np.all(v == 0.0) and np.all(w == 0.0)
+    # every particle at one point
+    with pytest.raises(VpmFmmError) as e:
+        solver.velocity3d(np.zeros(60), np.zeros(60), np.zeros(60), np.full(60, 0.1),
+                          np.ones(60), np.ones(60), np.ones(60), np.ones(60, np.uint8), type=2)
+    assert e.value.code == 3
+    # the mirror's VELOCITY_3D_TYPE 2
+    x, y, z, s, wx, wy, wz = _cloud(6000, 53, "mixed")
+    act = np.ones(6000, np.uint8)
+    p = Particle(x=x, y=y, s=s, vorticity=np.zeros(6000), isActive=act, z=z, vortx=wx,
+                 vorty=wy, vortz=wz)
+    VelocityCalc(dim=3, velocity_type=2).get_velocity(p)
+    ref = _oracle_t2(x, y, z, s, wx, wy, wz, act)
+    assert _err((p.u, p.v, p.w), ref, FREE) <= TOL
+    # type 1 after type 2 on the same handle is unchanged
+    a = solver.velocity3d(x, y, z, s, wx, wy, wz, act)
+    b = _oracle(x, y, z, s, wx, wy, wz, act)
+    assert _err(a, b, FREE) <= TOL
