@@ -393,6 +393,38 @@ def fmm3d_section(torch, steps, warmup, cpu, accuracy):
                 "seconds_per_solve": tc, "host_physical_cores": phys}
             log(f"[bench] 3-D cpu baseline {tc:.3f} s/solve on {threads} threads; GPU vs "
                 f"oracle max |d| / rms = {out['max_diff_vs_oracle']:.2e}")
+    # the second 3-D path (VELOCITY_3D_TYPE 2, the TreeCell octree) on the same ring
+    s2 = FmmSolver()
+    for _ in range(warmup):
+        s2.velocity3d_device(*dev, act, *outs, type=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s2.velocity3d_device(*dev, act, *outs, type=2)
+    torch.cuda.synchronize()
+    dt2 = (time.perf_counter() - t0) / steps
+    st2 = s2.fmm3d_stats()
+    t2 = {"ms_per_step": dt2 * 1e3, "value": n / dt2, "max_par": 50, "max_src": 30,
+          "n_cells": st2["n_cells"], "n_leaves": st2["n_leaves"], "depth": st2["depth"],
+          "pairs": st2["pairs"], "m2p": st2["m2p"]}
+    log(f"[bench] 3-D type 2: {dt2 * 1e3:.3f} ms/step ({st2['n_cells']} cells)")
+    if accuracy or cpu:
+        from oracle import oracle as O
+        u2, v2, w2 = (t.cpu().numpy() for t in outs)
+        rot2 = np.stack([u2 - 1.0, v2, w2], 1)
+        if accuracy:
+            t2["rel_l2_vs_fp64_direct"] = float(np.sqrt(((rot2[tidx] - d) ** 2).sum() /
+                                                        (d ** 2).sum()))
+        if cpu:
+            t0 = time.perf_counter()
+            ou, ov, ow = O.o3_velocity(f["x"], f["y"], f["z"], f["s"], f["wx"], f["wy"],
+                                       f["wz"], f["active"], O.o3_opts(type=2), 1.0, 0.0, 0.0)
+            tc2 = time.perf_counter() - t0
+            orot = np.stack([ou - 1.0, ov, ow], 1)
+            sc = np.sqrt(np.mean((orot ** 2).sum(1)))
+            t2["max_diff_vs_oracle"] = float(np.max(np.linalg.norm(rot2 - orot, axis=1)) / sc)
+            t2["cpu_baseline_seconds_per_solve"] = tc2
+    out["type2"] = t2
     return out
 
 

@@ -288,7 +288,7 @@ typedef struct vpmfmm_fmm3d_opts {
                            1.0)                                                       */
     double center[3];   /* a point the root box always holds (Pars::x/y/zcenter,
                            global.cpp:609-611: 0)                                     */
-    double support;     /* opening factor (SUPPORT_RADIUS_FACTOR, fmm3D.cpp:3: 2.0)  */
+    double support;     /* opening factor (SUPPORT_RADIUS_FACTOR 2.0: fast3DFMM.cpp:4, fmm3D.cpp:3)*/
     double u_inf, v_inf, w_inf;  /* free stream added to every velocity (get_velocity,
                            velocity_calc.cpp:221-236; global.cpp:633-635: 1, 0, 0)   */
     int32_t type;       /* VELOCITY_3D_TYPE (velocity_calc.cpp:31): 1 (default)

@@ -178,7 +178,7 @@ inline void set_default_velocity_type(int type, const vpmfmm_type1_opts *type1 =
 
 // Options of the 3-D branch: the reference's DIM == 3 parameters (critNum =
 // src_count_max 30, expTree 1 %, centre 0, free stream (1, 0, 0):
-// global.cpp:609-611, 633-635, 769, 779; SUPPORT_RADIUS_FACTOR 2, fmm3D.cpp:3)
+// global.cpp:609-611, 633-635, 769, 779; SUPPORT_RADIUS_FACTOR 2, fast3DFMM.cpp:4)
 // until set_default_fmm3d_options replaces them.  Read at each 3-D call.
 inline vpmfmm_fmm3d_opts &default_fmm3d_slot() {
     static vpmfmm_fmm3d_opts o = [] {

@@ -62,8 +62,8 @@ class Type1Opts(C.Structure):
 
 
 class Fmm3dOpts(C.Structure):
-    """vpmfmm_fmm3d_opts: the 3-D treecode's knobs (fast3DFMM.cpp, fmm3D.cpp:3,
-    global.cpp:140-166, 303, 313)."""
+    """vpmfmm_fmm3d_opts: the 3-D treecodes' knobs (fast3DFMM.cpp:4, fmm3D.cpp:3,
+    velocity_calc.cpp:31, global.cpp:609-611, 633-635, 769, 778-779)."""
     _fields_ = [
         ("crit_num", C.c_int32), ("max_depth", C.c_int32), ("exp_tree", C.c_double),
         ("center", C.c_double * 3), ("support", C.c_double), ("u_inf", C.c_double),

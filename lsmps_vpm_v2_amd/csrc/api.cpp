@@ -1183,7 +1183,7 @@ void vpmfmm_fmm3d_default_opts(vpmfmm_fmm3d_opts *o) {
     o->crit_num = 30;        // FMMCell::critNum = Pars::src_count_max (global.cpp:779)
     o->max_depth = kFmm3Depth;
     o->exp_tree = 1.0;       // Pars::expTree (global.cpp:769)
-    o->support = 2.0;        // SUPPORT_RADIUS_FACTOR (fmm3D.cpp:3)
+    o->support = 2.0;        // SUPPORT_RADIUS_FACTOR (fast3DFMM.cpp:4, fmm3D.cpp:3)
     o->u_inf = 1.0;          // Pars::u_inf, v_inf, w_inf (global.cpp:633-635)
     o->type = 1;             // VELOCITY_3D_TYPE (velocity_calc.cpp:31)
     o->max_par = 50;         // TreeCell: Pars::par_count_max (global.cpp:778)

@@ -39,7 +39,7 @@ typedef struct {
     double exp_tree;    /* root expansion in percent (Pars::expTree = 1.0)                 */
     double center[3];   /* the box always contains this point (Pars::x/y/zcenter = 0)      */
     double support;     /* opening factor: a cell is far when r > support * size
-                           (SUPPORT_RADIUS_FACTOR = 2.0, fmm3D.cpp:3)                     */
+                           (SUPPORT_RADIUS_FACTOR = 2.0, fast3DFMM.cpp:4, fmm3D.cpp:3) */
     int max_depth;      /* refuse deeper trees (the reference has no bound)               */
     int type;           /* 1: fast3DFMM (FMMCell octree, VELOCITY_3D_TYPE 1, the default);
                            2: fmm3D::calcVelocity over the 3-D TreeCell (VELOCITY_3D_TYPE 2) */
