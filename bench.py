@@ -340,6 +340,7 @@ def fmm3d_section(torch, steps, warmup, cpu, accuracy):
     flops = FMM3D_FLOPS_PAIR * st["pairs"] + FMM3D_FLOPS_M2P * st["m2p"]
     achieved = flops / (stage_ms["ms_eval"] * 1e-3) / 1e12
     u, v, w = (t.cpu().numpy() for t in outs)
+    traffic, traffic_src = load_traffic("k3_eval", f["name"])
     log(f"[bench] 3-D: {dt * 1e3:.3f} ms/step; tree {stage_ms['ms_tree']:.3f} moments "
         f"{stage_ms['ms_moments']:.3f} eval {stage_ms['ms_eval']:.3f} ms; eval "
         f"{achieved:.2f} TFLOP/s ({achieved / FP64_PEAK_TFLOPS:.3f} of FP64)")
@@ -355,7 +356,8 @@ def fmm3d_section(torch, steps, warmup, cpu, accuracy):
                  "lane_use_pairs": st["pairs"] / (64.0 * max(1, st["wave_pairs"])),
                  "lane_use_m2p": st["m2p"] / (64.0 * max(1, st["wave_m2p"]))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k3_eval", "flops_per_launch": flops,
                      "ms_per_launch": stage_ms["ms_eval"],
                      "flops_note": f"{FMM3D_FLOPS_PAIR:.0f} per direct pair, "

@@ -345,3 +345,26 @@ def test_type2_ring_edges_and_mirror(solver):
     a = solver.velocity3d(x, y, z, s, wx, wy, wz, act)
     b = _oracle(x, y, z, s, wx, wy, wz, act)
     assert _err(a, b, FREE) <= TOL
+
+
+def test_no_active_particle_and_single_particle(solver):
+    """No active particle: every cell is inactive and skipped, the velocity
+    is the free stream (both types); one particle: the free stream."""
+    x, y, z, s, wx, wy, wz = _cloud(5000, 59, "mixed")
+    act = np.zeros(5000, np.uint8)
+    print("3-D no active particle", flush=True)
+    for t in (1, 2):
+        u, v, w = solver.velocity3d(x, y, z, s, wx, wy, wz, act, type=t, u_inf=0.75,
+                                    w_inf=-0.5)
+        assert np.all(u == 0.75) and np.all(v == 0.0) and np.all(w == -0.5)
+        assert solver.fmm3d_stats()["n_active"] == 0
+    # one particle: type 1's box holds the domain centre too; type 2's box
+    # has no size (the reference divides by a zero cell size there)
+    u, v, w = solver.velocity3d(x[:1], y[:1], z[:1], s[:1], wx[:1], wy[:1], wz[:1],
+                                np.ones(1, np.uint8), u_inf=0.0)
+    assert u[0] == 0.0 and v[0] == 0.0 and w[0] == 0.0
+    from lsmps_vpm_v2_amd import VpmFmmError
+    with pytest.raises(VpmFmmError) as e:
+        solver.velocity3d(x[:1], y[:1], z[:1], s[:1], wx[:1], wy[:1], wz[:1],
+                          np.ones(1, np.uint8), type=2)
+    assert e.value.code == 3
