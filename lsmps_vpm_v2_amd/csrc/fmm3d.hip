@@ -705,13 +705,17 @@ __device__ __forceinline__ void m2p_term(const double *__restrict__ M, double dx
                            i3 - a * zz,       (b * xx - a) * dz, (b * yy - a) * dz,
                            (b * zz - 3.0 * a) * dz, bxyz,        (b * zz - a) * dy,
                            (b * zz - a) * dx};
+    // each cross term accumulated on its own (two FMAs per term)
     double su = 0.0, sv = 0.0, sw = 0.0;
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
         const double Vx = M[k], Vy = M[10 + k], Vz = M[20 + k];
-        su += Vy * mz[k] - Vz * my[k];
-        sv += Vz * mx[k] - Vx * mz[k];
-        sw += Vx * my[k] - Vy * mx[k];
+        su = fma(Vy, mz[k], su);
+        su = fma(-Vz, my[k], su);
+        sv = fma(Vz, mx[k], sv);
+        sv = fma(-Vx, mz[k], sv);
+        sw = fma(Vx, my[k], sw);
+        sw = fma(-Vy, mx[k], sw);
     }
     vu += su;
     vv += sv;
@@ -728,6 +732,28 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     const int per = nb / kXcds;
     if (b >= per * kXcds) return b;
     return (b % kXcds) * per + b / kXcds;
+}
+
+// The direct sum of a near leaf's sources [ab, ae) for the lanes with near
+// set, the sources read as wave-uniform loads (scalar cache), four per
+// batch; SELF drops the pair of a target with itself.
+template <bool SELF>
+__device__ __forceinline__ void direct_pairs(const Src3 *__restrict__ S, uint32_t ab, uint32_t ae,
+                                             double xp, double yp, double zp, double di,
+                                             bool near, double &vu, double &vv, double &vw) {
+#pragma clang fp contract(fast)
+    if (!near) return;
+#pragma unroll 4
+    for (uint32_t j = ab; j < ae; ++j) {
+        const Src3 q = S[j];
+        const double ex = xp - q.x, ey = yp - q.y, ez = zp - q.z;
+        const double r2 = ex * ex + ey * ey + ez * ez;
+        const double ir = rsqrt_pos(r2);
+        const double i3 = (!SELF || q.idx != di) ? ir * ir * ir : 0.0;
+        vu += (q.ay * ez - q.az * ey) * i3;
+        vv += (q.az * ex - q.ax * ez) * i3;
+        vw += (q.ax * ey - q.ay * ex) * i3;
+    }
 }
 
 // A near leaf's active sources are read as wave-uniform loads (scalar
@@ -794,25 +820,15 @@ __global__ void __launch_bounds__(kEvalBlock)
             if (any_near) {
                 const uint32_t ab = C.abegin, ae = C.aend;
                 wpair += ae - ab;
-                if (near) {
-#pragma clang fp contract(fast)
-#pragma unroll 4
-                    for (uint32_t j = ab; j < ae; ++j) {
-                        const Src3 q = S[j];
-                        const double ex = xp - q.x, ey = yp - q.y, ez = zp - q.z;
-                        const double r2 = ex * ex + ey * ey + ez * ez;
-                        const double ir = rsqrt_pos(r2);
-                        const double i3 = q.idx != di ? ir * ir * ir : 0.0;
-                        vu += (q.ay * ez - q.az * ey) * i3;
-                        vv += (q.az * ex - q.ax * ez) * i3;
-                        vw += (q.ax * ey - q.ay * ex) * i3;
-                    }
-                    // the target itself is a source of its own leaf (when
-                    // active, unless every particle is a source)
-                    const bool self = (src_all || my_act) && (uint32_t)i >= C.begin &&
-                                      (uint32_t)i < C.end;
-                    npair += ae - ab - (self ? 1u : 0u);
-                }
+                // the target itself is a source of its own leaf (when active,
+                // unless every particle is a source); only a leaf holding some
+                // lane's own particle needs the self test
+                const bool mine = (uint32_t)i >= C.begin && (uint32_t)i < C.end;
+                if (__ballot(near && mine))
+                    direct_pairs<true>(S, ab, ae, xp, yp, zp, di, near, vu, vv, vw);
+                else
+                    direct_pairs<false>(S, ab, ae, xp, yp, zp, di, near, vu, vv, vw);
+                if (near) npair += ae - ab - ((src_all || my_act) && mine ? 1u : 0u);
             }
             c = (int)C.skip;
         } else {
