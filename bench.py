@@ -106,8 +106,10 @@ def stage_rooflines(st, stage_ms, work, n, fp64):
     sum_s t_s (achieved_s / peak_s) / sum_s t_s.  Bounds: the tree build and
     the list build by HBM (algorithmic bytes: 48 B per particle for the
     binning, read x, y, q and write them sorted; the list bytes of
-    stage_work); M2L on the mixed path by FP32 MFMA; the other stages by the
-    vector ALU (FP32 or FP64 peak)."""
+    stage_work); M2L by the matrix cores (FP32 MFMA on the mixed path, FP64
+    MFMA for list 2 on the FP64 path, whose list 4 runs on the vector ALU at
+    the same FP64 peak); the other stages by the vector ALU (FP32 or FP64
+    peak)."""
     vpeak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
     lists_bytes = 4.0 * (st["n_list1"] + st["n_list2"] + st["n_list3"] + st["n_list4"]) + \
         30.0 * st["n_cells"]
@@ -116,7 +118,7 @@ def stage_rooflines(st, stage_ms, work, n, fp64):
         ("lists", "ms_lists", "hbm", lists_bytes, HBM_PEAK_GBS, "GB/s"),
         ("upward (P2M + M2M)", "ms_upward", "valu", work["p2m_flops"] + work["m2m_flops"],
          vpeak, "TFLOP/s"),
-        ("M2L (list 2" + (" + list 4)" if fp64 else ")"), "ms_m2l", "valu" if fp64 else "mfma",
+        ("M2L (list 2" + (" + list 4)" if fp64 else ")"), "ms_m2l", "mfma",
          work["m2l_flops"], vpeak, "TFLOP/s"),
         ("downward (L2L)", "ms_downward", "valu", work["l2l_flops"], vpeak, "TFLOP/s"),
         ("evaluation (P2P + M2P + L2P)", "ms_eval", "valu", eval_flops(st), vpeak, "TFLOP/s"),

@@ -271,6 +271,16 @@ void ensure_tables(vpmfmm_handle *h) {
     VPM_HIP(hipMemcpyAsync(c.tab_m2l.get(nb), m2l.data(), nb, hipMemcpyHostToDevice, c.stream));
     VPM_HIP(hipMemcpyAsync(c.tab_m2m.get(nb), m2m.data(), nb, hipMemcpyHostToDevice, c.stream));
     VPM_HIP(hipMemcpyAsync(c.tab_l2l.get(nb), l2l.data(), nb, hipMemcpyHostToDevice, c.stream));
+    // the list-2 M2L matrices for the matrix cores (orders up to 16)
+    std::vector<double> r64;
+    c.tab_r64_p = -1;
+    if (P >= 1 && P <= 16) {
+        fmm64_mfma_table(P, m2l.data(), r64);
+        const size_t nr = r64.size() * sizeof(double);
+        VPM_HIP(hipMemcpyAsync(c.tab_r64.get(nr), r64.data(), nr, hipMemcpyHostToDevice,
+                               c.stream));
+        c.tab_r64_p = P;
+    }
     VPM_HIP(hipStreamSynchronize(c.stream));
     h->tab_p = P;
     h->tab_binom = h->o.binom_mode;

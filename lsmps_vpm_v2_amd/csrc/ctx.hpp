@@ -166,6 +166,10 @@ struct Ctx {
 
     // expansions
     DBuf ak, bk, tab_m2l, tab_m2m, tab_l2l;
+    // FP64 list-2 M2L matrices for the matrix cores (fmm64_mfma_table) and
+    // the order they were built for (-1: none)
+    DBuf tab_r64;
+    int tab_r64_p = -1;
     // type-1 tables: M2M (order ndp - 1, as tab_m2m), M2L (ndp + 1) x ndp,
     // L2L coefficients and exponents (ndp + 1)^2, and the all-active flags
     // of a type-1 call
@@ -242,6 +246,11 @@ struct FmmMarks {
 void fmm64_run(Ctx &c, int p, int list4_mode, int stage_mask, double *out_a, double *out_b,
                bool velocity, double u_inf, double v_inf, const FmmMarks &mk,
                bool potential = false);
+
+// The FP64 path's list-2 M2L matrices for the matrix cores, order P <= 16,
+// from its M2L table tab ((P+1) x (P+1)): 40 offsets x 2P x 2(P+1) in
+// v_mfma_f64_16x16x4f64 A-operand lane order.
+void fmm64_mfma_table(int P, const double *tab, std::vector<double> &R);
 
 // Type-1 FMM (fmm64.hip, biotsavart_fmm_2d.cpp:93-446) on a type-1 tree:
 // multipoles of orders 0..ndp-1 (P2M, M2M), locals of orders 0..ndp from

@@ -17,6 +17,7 @@
 // translations (Horner in the translation distance, no per-thread arrays),
 // one wavefront per leaf for the evaluation.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "ctx.hpp"
@@ -140,6 +141,53 @@ __global__ void __launch_bounds__(256)
     st(ak, (size_t)j * P1 + p, acc);
 }
 
+// The same M2M with G parent cells per wavefront, 64 / G lanes each (G = 4
+// for orders P <= 15, 1 up to 16): a cell's four children's coefficients and
+// the table are staged in LDS in one round of loads, then lane p of the cell
+// runs k_m2m64's Horner sums from LDS (the same operations in the same
+// order), so a level costs one dependent load, not one per order.
+constexpr int kTrWaves = 4;    // wavefronts per block of the per-cell translations
+constexpr int kTrP1 = 17;      // orders up to 16
+
+template <int G>
+__global__ void __launch_bounds__(kWave *kTrWaves)
+    k_m2m64w(int j0, int j1, int P, const double *tab, const uint8_t *c_flags, const double *c_cx,
+             const double *c_cy, const int *c_child, double2 *ak) {
+    constexpr int LN = kWave / G;
+    __shared__ double s_tab[kTrP1 * kTrP1];
+    __shared__ double2 sa[kTrWaves][G][4 * kTrP1];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int g = lane / LN, r = lane % LN;
+    const int P1 = P + 1;
+    for (int t = threadIdx.x; t < P1 * P1; t += blockDim.x) s_tab[t] = tab[t];
+    const int j = j0 + ((int)blockIdx.x * kTrWaves + w) * G + g;
+    const bool live = j < j1 && !(c_flags[j] & kLeaf);  // uniform over the cell's lanes
+    int4 c4 = make_int4(-1, -1, -1, -1);
+    if (live) {
+        c4 = reinterpret_cast<const int4 *>(c_child)[j];
+        for (int idx = r; idx < 4 * P1; idx += LN) {
+            const int q = idx / P1, k = idx - q * P1;
+            const int ch = q == 0 ? c4.x : q == 1 ? c4.y : q == 2 ? c4.z : c4.w;
+            sa[w][g][q * kTrP1 + k] = ch >= 0 ? ak[(size_t)ch * P1 + k] : make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    if (!live || r > P) return;
+    const int p = r;
+    const double cx = c_cx[j], cy = c_cy[j];
+    cd acc = {0.0, 0.0};
+    for (int q = 0; q < 4; ++q) {
+        const int ch = q == 0 ? c4.x : q == 1 ? c4.y : q == 2 ? c4.z : c4.w;
+        if (ch < 0) continue;
+        const cd d = {c_cx[ch] - cx, c_cy[ch] - cy};
+        const double2 *a = &sa[w][g][q * kTrP1];
+        cd sacc = ld(a, 0);
+        for (int k = 1; k <= p; ++k) sacc = cadd(cmul(sacc, d), cscale(ld(a, k), s_tab[p * P1 + k]));
+        acc = cadd(acc, sacc);
+    }
+    st(ak, (size_t)j * P1 + p, acc);
+}
+
 // Principal complex logarithm, as std::log: log|z| + i arg z.
 __device__ __forceinline__ cd clog(cd z) { return {log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
 
@@ -223,7 +271,7 @@ constexpr int kM2lwWaves = 4;
 constexpr int kM2lwSrc = 27 + 8;  // list 2 <= 27, list 4 <= 8
 constexpr int kM2lwP1 = 17;       // orders up to 16
 
-template <bool POT>
+template <bool POT, bool L4ONLY>
 __global__ void __launch_bounds__(kWave *kM2lwWaves)
     k_m2l64w(int j0, int nc, int P, int list4_mode, const double *tab, const int *l2,
              const uint8_t *n2, const int *l4, const uint8_t *n4, const double *c_cx,
@@ -239,7 +287,8 @@ __global__ void __launch_bounds__(kWave *kM2lwWaves)
     const int P1 = P + 1, NP = POT ? P1 : P, G = kWave / NP;
     const int g = lane / NP, i = lane % NP, p = (POT ? 0 : 1) + i;
     const double cx = c_cx[j], cy = c_cy[j];
-    const int m2 = n2[j], m4 = n4[j];
+    const int m2 = L4ONLY ? 0 : n2[j], m4 = n4[j];
+    if (L4ONLY && m4 == 0) return;  // uniform over the wavefront
     const int ns = m2 + (list4_mode == 1 ? m4 : 0);
     // staging: w and (potential) log(z_t - z_s) per source, then the a_k rows
     if (lane < ns) {
@@ -302,7 +351,143 @@ __global__ void __launch_bounds__(kWave *kM2lwWaves)
     if (lane < NP) {
         cd sum = {0.0, 0.0};
         for (int gg = 0; gg < G; ++gg) sum = cadd(sum, ld(red[w], gg * NP + lane));
-        st(bk, (size_t)j * P1 + (POT ? 0 : 1) + lane, sum);
+        const size_t o = (size_t)j * P1 + (POT ? 0 : 1) + lane;
+        st(bk, o, L4ONLY ? cadd(ld(bk, o), sum) : sum);
+    }
+}
+
+// List-2 M2L of the FP64 path on the matrix cores (orders P <= 16, field and
+// velocity; the potential keeps k_m2l64w).  A workgroup takes a tile of 16
+// consecutive cells of one level; the tiles of levels 2.. follow each other
+// in the grid.  With len the level's cell side and delta = (zs - zt) / len
+// (|dx|, |dy| <= 3, not both <= 1: 40 offsets), the list-2 sum
+//   b_p = sum_k tab[p][k] a_k (zs - zt)^-(p+k)
+// is, in the scaled coefficients a^_k = a_k len^-k and b^_p = b_p len^p,
+//   b^ = sum over offsets of M_delta a^,  M_delta[p][k] = tab[p][k] delta^-(p+k),
+// a realified 2P x 2(P+1) matrix per offset (fmm64_mfma_table), the A operand
+// of v_mfma_f64_16x16x4f64; the B operand is a source cell's a^ along
+// K = 2(P+1).  Lane l: target column n = l % 16, k-quarter kq = l / 16;
+// wavefront w takes the offsets w, w + kM2l64Waves, ... that a target of the
+// tile uses, and the wavefronts' accumulators are summed through LDS.  The
+// output rows follow the f64 instruction's layout D[4v + l / 16][l % 16].
+// The offset is exact here, where the reference forms zs - zt from rounded
+// centres: the two agree to the rounding of the centres (about 1e-16 of the
+// root box over the cell side).  List 4 is added afterwards by k_m2l64w in
+// its list-4 mode.
+constexpr int kM2l64Waves = 4;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t compact1by1(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWave *kM2l64Waves)
+    k_m2l64_mfma(const DevHdr *hdr, int maxL, const uint32_t *c_key, const int *l2,
+                 const uint8_t *n2, const double *R, const double2 *ak, double2 *bk) {
+    constexpr int P1 = P + 1, K = 2 * P1, KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
+    __shared__ int smap[16][49];
+    __shared__ double red[kM2l64Waves][MT * 4][kWave];
+    __shared__ double s_sc[P1];
+    __shared__ int s_tile[3];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x == 0) {
+        // this block's tile: level L, first cell, cells (up to 16)
+        int b = (int)blockIdx.x, L = 2, j0 = -1, cnt = 0;
+        for (; L <= maxL; ++L) {
+            const int c0 = (int)hdr->lev_off[L], cn = (int)hdr->lev_off[L + 1] - c0;
+            const int nt = (cn + 15) / 16;
+            if (b < nt) {
+                j0 = c0 + 16 * b;
+                cnt = min(16, cn - 16 * b);
+                break;
+            }
+            b -= nt;
+        }
+        s_tile[0] = L;
+        s_tile[1] = j0;
+        s_tile[2] = cnt;
+    }
+    for (int t = threadIdx.x; t < 16 * 49; t += blockDim.x) smap[t / 49][t % 49] = -1;
+    __syncthreads();
+    const int L = s_tile[0], j0 = s_tile[1], cnt = s_tile[2];
+    if (j0 < 0) return;  // (uniform over the block)
+    if (threadIdx.x < P1) {
+        // len^-k, k = 0..P
+        const double inv = ldexp(1.0 / hdr->root_len, L);
+        double sc = 1.0;
+        for (int k = 0; k < (int)threadIdx.x; ++k) sc *= inv;
+        s_sc[threadIdx.x] = sc;
+    }
+    {
+        // source map of the tile: slot (dx+3) + 7 (dy+3) -> source cell or -1
+        const int tn = threadIdx.x % 16, tu = threadIdx.x / 16;
+        const int tj = j0 + tn;
+        if (tn < cnt) {
+            const uint32_t kj = c_key[tj];
+            const int jx = (int)compact1by1(kj), jy = (int)compact1by1(kj >> 1);
+            const int m2 = n2[tj];
+            for (int u = tu; u < m2; u += (int)blockDim.x / 16) {
+                const int sidx = l2[27 * tj + u];
+                const uint32_t ks = c_key[sidx];
+                const int dx = (int)compact1by1(ks) - jx, dy = (int)compact1by1(ks >> 1) - jy;
+                smap[tn][(dx + 3) + 7 * (dy + 3)] = sidx;
+            }
+        }
+    }
+    __syncthreads();
+    const int n = lane % 16, kq = lane / 16;
+    const double *a1 = reinterpret_cast<const double *>(ak);
+    v4d acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
+    int o = 0;
+    for (int slot = 0; slot < 49; ++slot) {
+        const int dx = slot % 7 - 3, dy = slot / 7 - 3;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) continue;
+        const int oc = o++;
+        if (oc % kM2l64Waves != w) continue;
+        const int sidx = smap[n][slot];
+        if (__ballot(sidx >= 0) == 0) continue;  // no target of the tile uses this offset
+        double b[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int col = ks * 4 + kq;  // 2k (Re a_k) or 2k + 1 (Im a_k)
+            b[ks] = (sidx >= 0 && col < K) ? a1[(size_t)sidx * K + col] * s_sc[col >> 1] : 0.0;
+        }
+        const double *Ro = R + (size_t)oc * KS * MT * kWave + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ro[(ks * MT + mt) * kWave], b[ks],
+                                                               acc[mt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[w][mt * 4 + v][lane] = acc[mt][v];
+    __syncthreads();
+    if (w != 0) return;
+    double *bo = reinterpret_cast<double *>(bk);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double sum = 0.0;
+#pragma unroll
+            for (int u = 0; u < kM2l64Waves; ++u) sum += red[u][mt * 4 + v][lane];
+            const int row = mt * 16 + 4 * v + kq;  // Re b^_1, Im b^_1, Re b^_2, ...
+            if (n < cnt && row < 2 * P) {
+                const int p = row / 2 + 1;
+                bo[((size_t)(j0 + n) * P1 + p) * 2 + (row & 1)] = sum * s_sc[p];
+            }
+        }
     }
 }
 
@@ -323,6 +508,40 @@ __global__ void __launch_bounds__(256)
     const double *row = tab + (size_t)p * P1;
     cd s = cscale(ld(bk, b + P), row[P]);
     for (int k = P - 1; k >= p; --k) s = cadd(cmul(s, d), cscale(ld(bk, b + k), row[k]));
+    const size_t o = (size_t)j * P1 + p;
+    st(bk, o, cadd(ld(bk, o), s));
+}
+
+// The same L2L with G child cells per wavefront, 64 / G lanes each (as
+// k_m2m64w): the parent's coefficients and the table are staged in LDS in
+// one round of loads, then lane i of the cell runs k_l2l64's Horner sum for
+// order p from LDS (the same operations in the same order).
+template <bool POT, int G>
+__global__ void __launch_bounds__(kWave *kTrWaves)
+    k_l2l64w(int j0, int j1, int P, const double *tab, const int *c_parent, const double *c_cx,
+             const double *c_cy, double2 *bk) {
+    constexpr int LN = kWave / G;
+    __shared__ double s_tab[kTrP1 * kTrP1];
+    __shared__ double2 sb[kTrWaves][G][kTrP1];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int g = lane / LN, r = lane % LN;
+    const int P1 = P + 1, NP = POT ? P1 : P;
+    for (int t = threadIdx.x; t < P1 * P1; t += blockDim.x) s_tab[t] = tab[t];
+    const int j = j0 + ((int)blockIdx.x * kTrWaves + w) * G + g;
+    const bool live = j < j1;  // uniform over the cell's lanes
+    int par = 0;
+    if (live) {
+        par = c_parent[j];
+        for (int k = r; k < P1; k += LN) sb[w][g][k] = bk[(size_t)par * P1 + k];
+    }
+    __syncthreads();
+    if (!live || r >= NP) return;
+    const int p = (POT ? 0 : 1) + r;
+    const cd d = {c_cx[j] - c_cx[par], c_cy[j] - c_cy[par]};
+    const double *row = s_tab + p * P1;
+    const double2 *b = sb[w][g];
+    cd s = cscale(ld(b, P), row[P]);
+    for (int k = P - 1; k >= p; --k) s = cadd(cmul(s, d), cscale(ld(b, k), row[k]));
     const size_t o = (size_t)j * P1 + p;
     st(bk, o, cadd(ld(bk, o), s));
 }
@@ -702,16 +921,47 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     for (int L = ti.max_level - 1; L >= 1; --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
-        k_m2m64<<<div_up((long long)(j1 - j0) * P1, 256), 256, 0, s>>>(j0, j1, P, t_m2m, c_flags,
-                                                                       c_cx, c_cy, c_child, ak);
+        if (P1 <= kTrP1 - 1)
+            k_m2m64w<4><<<div_up(j1 - j0, kTrWaves * 4), kWave * kTrWaves, 0, s>>>(
+                j0, j1, P, t_m2m, c_flags, c_cx, c_cy, c_child, ak);
+        else if (P1 == kTrP1)
+            k_m2m64w<1><<<div_up(j1 - j0, kTrWaves), kWave * kTrWaves, 0, s>>>(
+                j0, j1, P, t_m2m, c_flags, c_cx, c_cy, c_child, ak);
+        else
+            k_m2m64<<<div_up((long long)(j1 - j0) * P1, 256), 256, 0, s>>>(
+                j0, j1, P, t_m2m, c_flags, c_cx, c_cy, c_child, ak);
     }
     mk.mark(3, s);
     // M2L: every cell of level >= 2
     const int j2 = (int)ti.lev_off[2];
     const int NP = potential ? P1 : P;  // orders formed per cell
-    if (nc > j2 && P < kM2lwP1)
-        (potential ? k_m2l64w<true> : k_m2l64w<false>)<<<div_up(nc - j2, kM2lwWaves),
-                                                         kWave * kM2lwWaves, 0, s>>>(
+    const double *t_r64 = (const double *)c.tab_r64.p;
+    if (nc > j2 && P >= 1 && P < kM2lwP1 && !potential && t_r64 && c.tab_r64_p == P) {
+        // list 2 on the matrix cores, then list 4 added by the wavefront kernel
+        int ntiles = 0;
+        for (int L = 2; L <= ti.max_level; ++L)
+            ntiles += (int)div_up((long long)(ti.lev_off[L + 1] - ti.lev_off[L]), 16LL);
+        const uint32_t *c_key = (const uint32_t *)c.c_key.p;
+        const int *l2 = (const int *)c.l2.p;
+        const uint8_t *n2 = (const uint8_t *)c.n2.p;
+        switch (P) {
+#define M2L64_CASE(PP)                                                                          \
+    case PP:                                                                                    \
+        k_m2l64_mfma<PP><<<ntiles, kWave * kM2l64Waves, 0, s>>>(c.hdr_d, ti.max_level, c_key, l2, \
+                                                              n2, t_r64, ak, bk);               \
+        break;
+            M2L64_CASE(1) M2L64_CASE(2) M2L64_CASE(3) M2L64_CASE(4) M2L64_CASE(5) M2L64_CASE(6)
+            M2L64_CASE(7) M2L64_CASE(8) M2L64_CASE(9) M2L64_CASE(10) M2L64_CASE(11)
+            M2L64_CASE(12) M2L64_CASE(13) M2L64_CASE(14) M2L64_CASE(15) M2L64_CASE(16)
+#undef M2L64_CASE
+            default: break;
+        }
+        k_m2l64w<false, true><<<div_up(nc - j2, kM2lwWaves), kWave * kM2lwWaves, 0, s>>>(
+            j2, nc, P, list4_mode, t_m2l, l2, n2, (const int *)c.l4.p,
+            (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs, as, ak, bk);
+    } else if (nc > j2 && P < kM2lwP1)
+        (potential ? k_m2l64w<true, false> : k_m2l64w<false, false>)<<<
+            div_up(nc - j2, kM2lwWaves), kWave * kM2lwWaves, 0, s>>>(
             j2, nc, P, list4_mode, t_m2l, (const int *)c.l2.p, (const uint8_t *)c.n2.p,
             (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
             as, ak, bk);
@@ -726,9 +976,18 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
     for (int L = 3; L <= ti.max_level; ++L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
-        (potential ? k_l2l64<true> : k_l2l64<false>)<<<div_up((long long)(j1 - j0) * NP, 256), 256,
-                                                          0, s>>>(j0, j1, P, t_l2l, c_parent,
-                                                                  c_cx, c_cy, bk);
+        if (P1 <= kTrP1 - 1)
+            (potential ? k_l2l64w<true, 4> : k_l2l64w<false, 4>)<<<div_up(j1 - j0, kTrWaves * 4),
+                                                                  kWave * kTrWaves, 0, s>>>(
+                j0, j1, P, t_l2l, c_parent, c_cx, c_cy, bk);
+        else if (P1 == kTrP1)
+            (potential ? k_l2l64w<true, 1> : k_l2l64w<false, 1>)<<<div_up(j1 - j0, kTrWaves),
+                                                                  kWave * kTrWaves, 0, s>>>(
+                j0, j1, P, t_l2l, c_parent, c_cx, c_cy, bk);
+        else
+            (potential ? k_l2l64<true> : k_l2l64<false>)<<<div_up((long long)(j1 - j0) * NP, 256),
+                                                            256, 0, s>>>(j0, j1, P, t_l2l, c_parent,
+                                                                         c_cx, c_cy, bk);
     }
     mk.mark(5, s);
     if (potential) {
@@ -884,6 +1143,51 @@ void direct64(int n, const double *x, const double *y, const double *q, const ui
               int nt, const int *tidx, double *ex, double *ey, hipStream_t s) {
     k_direct64<<<div_up(nt, 256), 256, 0, s>>>(n, x, y, q, src_mask, nt, tidx, ex, ey);
     VPM_LAUNCH_CHECK();
+}
+
+// Realified list-2 M2L matrices of order P for k_m2l64_mfma, one per offset
+// delta = (dx, dy) (source minus target centre over the cell side; |dx|,
+// |dy| <= 3, not both <= 1; slot order, dy major): row 2(p-1) (Re b^_p) and
+// 2(p-1)+1 (Im), p = 1..P; column 2k (Re a^_k) and 2k+1 (Im), k = 0..P;
+// entry M[p][k] = tab[p][k] delta^-(p+k), tab being the path's M2L table
+// (tab[p][0] = -1/p, tab[p][k] = -C(p+k-1, k-1) (-1)^k / k with its
+// binomials).  Stored in the A-operand lane order of v_mfma_f64_16x16x4f64:
+// R[((o KS + ks) MT + mt) 64 + l] = M[16 mt + l % 16][4 ks + l / 16].
+void fmm64_mfma_table(int P, const double *tab, std::vector<double> &R) {
+    const int P1 = P + 1, K = 2 * P1, KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
+    R.assign((size_t)40 * KS * MT * kWave, 0.0);
+    std::vector<double> pwr(2 * P + 2), pwi(2 * P + 2);
+    int o = 0;
+    for (int dy = -3; dy <= 3; ++dy) {
+        for (int dx = -3; dx <= 3; ++dx) {
+            if (std::abs(dx) <= 1 && std::abs(dy) <= 1) continue;
+            const double r2 = (double)(dx * dx + dy * dy);
+            const double wr = dx / r2, wi = -dy / r2;  // 1 / delta
+            pwr[0] = 1.0;
+            pwi[0] = 0.0;
+            for (int t = 1; t <= 2 * P + 1; ++t) {
+                pwr[t] = pwr[t - 1] * wr - pwi[t - 1] * wi;
+                pwi[t] = pwr[t - 1] * wi + pwi[t - 1] * wr;
+            }
+            for (int p = 1; p <= P; ++p) {
+                for (int k = 0; k <= P; ++k) {
+                    const double T = tab[(size_t)p * P1 + k];
+                    const double Mr = T * pwr[p + k], Mi = T * pwi[p + k];
+                    const int r0 = 2 * (p - 1), c0 = 2 * k;
+                    const double val[2][2] = {{Mr, -Mi}, {Mi, Mr}};
+                    for (int a = 0; a < 2; ++a) {
+                        for (int bq = 0; bq < 2; ++bq) {
+                            const int row = r0 + a, col = c0 + bq;
+                            const int mt = row / 16, ks = col / 4;
+                            const int lane = (row % 16) + 16 * (col % 4);
+                            R[(((size_t)o * KS + ks) * MT + mt) * kWave + lane] = val[a][bq];
+                        }
+                    }
+                }
+            }
+            ++o;
+        }
+    }
 }
 
 }  // namespace vpm
