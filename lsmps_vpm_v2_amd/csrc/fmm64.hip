@@ -584,24 +584,23 @@ __device__ __forceinline__ double regul64(double rho2) {
 // nothing whatever the kernel; the multipoles hold orders 0..P-1 (stride P)
 // and the local expansions orders 0..P (stride P + 1) in the Cauchy form
 // F = sum_l b_l (z - z0)^l.
-template <int ICUT, bool T1 = false>
-__global__ void __launch_bounds__(kWave *kEvalWaves)
-    k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
-             const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
-             const uint32_t *off3, const int *l3, const double *xs, const double *ys,
-             const double *qs, const double *ss, const uint32_t *perm, const double2 *ak,
-             const double2 *bk, int velocity, double u_inf, double v_inf, int mask,
-             double *out_a, double *out_b, unsigned *flags) {
-    __shared__ double4 stage[kEvalWaves][kStage64];
-    __shared__ double stage_s[ICUT ? kEvalWaves : 1][ICUT ? kStage64 : 1];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int t = blockIdx.x * kEvalWaves + w;
-    if (t >= nl) return;
-    const int j = (int)leaf_idx[t];
+// One leaf's targets in groups of 64 / G lanes, G a compile-time constant so
+// that the source stride and the group reduction are fixed and the pair loop
+// unrolls (four pairs in flight per lane); stage / stage_s are the
+// wavefront's LDS rows.  Returns whether a pair at r = 0 was seen.
+template <int ICUT, bool T1, int G>
+__device__ __forceinline__ bool eval64_leaf(int t, int j, int lane, double4 *stage,
+                                            double *stage_s, int P, const uint32_t *c_begin,
+                                            const uint32_t *c_end, const double *c_cx,
+                                            const double *c_cy, const uint32_t *off1,
+                                            const int *l1, const uint32_t *off3, const int *l3,
+                                            const double *xs, const double *ys, const double *qs,
+                                            const double *ss, const uint32_t *perm,
+                                            const double2 *ak, const double2 *bk, int velocity,
+                                            double u_inf, double v_inf, int mask, double *out_a,
+                                            double *out_b) {
     const uint32_t b = c_begin[j], e = c_end[j];
-    const int nt = (int)(e - b);
-    const int G = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
-    const int LG = kWave / G;
+    constexpr int LG = kWave / G;
     const int g = lane / LG, tl = lane % LG;
     const int P1 = P + 1;
     const double lcx = c_cx[j], lcy = c_cy[j];
@@ -622,8 +621,8 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                 const int take = (int)min<uint32_t>(se - pos, (uint32_t)(kStage64 - fill));
                 for (int r = lane; r < take; r += kWave) {
                     const uint32_t src = pos + r;
-                    stage[w][fill + r] = make_double4(xs[src], ys[src], qs[src], (double)src);
-                    if (ICUT) stage_s[w][fill + r] = ss[src] * ss[src];
+                    stage[fill + r] = make_double4(xs[src], ys[src], qs[src], (double)src);
+                    if (ICUT) stage_s[fill + r] = ss[src] * ss[src];
                 }
                 fill += take;
                 pos += take;
@@ -631,8 +630,9 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
             for (int r = g; r < fill; r += G) {
-                const double4 sv = stage[w][r];
+                const double4 sv = stage[r];
                 const double dx = xi - sv.x, dy = yi - sv.y;
                 const double r2 = dx * dx + dy * dy;
                 if (ICUT == 0 && !T1 && valid && (uint32_t)sv.w != i) {
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
                     ey += qr * dy;
                 }
                 if (ICUT != 0 && valid && r2 > 0.0) {
-                    const double sij2 = 0.25 * (si2 + stage_s[w][r]);
+                    const double sij2 = 0.25 * (si2 + stage_s[r]);
                     const double qr = sv.z * regul64<ICUT>(r2 / sij2) * rcp64(r2);
                     ex += qr * dx;
                     ey += qr * dy;
@@ -701,6 +701,35 @@ __global__ void __launch_bounds__(kWave *kEvalWaves)
             }
         }
     }
+    return dup;
+}
+
+template <int ICUT, bool T1 = false>
+__global__ void __launch_bounds__(kWave *kEvalWaves)
+    k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
+             const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
+             const uint32_t *off3, const int *l3, const double *xs, const double *ys,
+             const double *qs, const double *ss, const uint32_t *perm, const double2 *ak,
+             const double2 *bk, int velocity, double u_inf, double v_inf, int mask,
+             double *out_a, double *out_b, unsigned *flags) {
+    __shared__ double4 stage[kEvalWaves][kStage64];
+    __shared__ double stage_s[ICUT ? kEvalWaves : 1][ICUT ? kStage64 : 1];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int t = blockIdx.x * kEvalWaves + w;
+    if (t >= nl) return;
+    const int j = (int)leaf_idx[t];
+    const int nt = (int)(c_end[j] - c_begin[j]);
+    double *sst = stage_s[ICUT ? w : 0];
+    bool dup;
+#define EVAL64_G(GG)                                                                          \
+    eval64_leaf<ICUT, T1, GG>(t, j, lane, stage[w], sst, P, c_begin, c_end, c_cx, c_cy, off1, \
+                              l1, off3, l3, xs, ys, qs, ss, perm, ak, bk, velocity, u_inf,   \
+                              v_inf, mask, out_a, out_b)
+    if (nt <= 8) dup = EVAL64_G(8);
+    else if (nt <= 16) dup = EVAL64_G(4);
+    else if (nt <= 32) dup = EVAL64_G(2);
+    else dup = EVAL64_G(1);
+#undef EVAL64_G
     if (dup) atomicOr(flags, kFlagDuplicate);
 }
 
