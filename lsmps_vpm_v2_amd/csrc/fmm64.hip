@@ -267,6 +267,9 @@ __global__ void __launch_bounds__(256)
 // added in group order.  Every lane works, and a source's coefficients are
 // read from HBM / L2 once per target instead of once per (target, order).
 // P2L (list4_mode 2) splits the list-4 particles over the groups the same way.
+// With L4ONLY the wave takes list 4 alone (M2L or P2L by list4_mode), skips a
+// cell without one and adds its sum to b_p, which the matrix-core list-2
+// kernel has written.
 constexpr int kM2lwWaves = 4;
 constexpr int kM2lwSrc = 27 + 8;  // list 2 <= 27, list 4 <= 8
 constexpr int kM2lwP1 = 17;       // orders up to 16
