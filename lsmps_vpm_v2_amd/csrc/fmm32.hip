@@ -80,7 +80,9 @@ __device__ __forceinline__ void wave_sync() {
 // powers accumulate a^_k = sum q ((z - zc)/s)^k (k <= p, active particles of
 // active leaves; inactive leaves get zero).  The lanes' sums are reduced
 // through LDS: lane l writes column l of a (order x lane) table, lane k sums
-// row k (stride 65 keeps the rows on distinct banks).
+// row k (stride 65 keeps the rows on distinct banks), in two halves of 9
+// orders, so a wavefront holds 4.7 KB of LDS and 7 wavefronts per SIMD fit
+// (4 with the whole table; the upward pass 3-4 us shorter at 780k).
 constexpr int kP2mWaves = 4;
 
 __global__ void __launch_bounds__(kWave *kP2mWaves)
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(kWave *kP2mWaves)
             const double *c_cy, const double *xs, const double *ys, const double *qs,
             const uint8_t *as, const LevelTab *lt, int kst, float2 *oxy, float *qf,
             float *ahat) {
-    __shared__ float2 red[kP2mWaves][17][kWave + 1];
+    __shared__ float2 red[kP2mWaves][9][kWave + 1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int t = blockIdx.x * kP2mWaves + w;
@@ -122,22 +124,30 @@ __global__ void __launch_bounds__(kWave *kP2mWaves)
         }
     }
 #pragma unroll
-    for (int k = 0; k < 17; ++k)
-        if (k <= p) red[w][k][lane] = make_float2(ar[k], ai[k]);
-    wave_sync();
-    if (lane <= p) {
-        float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
-#pragma unroll 4
-        for (int l = 0; l < kWave; l += 4) {
-            const float2 a0 = red[w][lane][l], a1 = red[w][lane][l + 1];
-            const float2 a2 = red[w][lane][l + 2], a3 = red[w][lane][l + 3];
-            s0.x += a0.x; s0.y += a0.y;
-            s1.x += a1.x; s1.y += a1.y;
-            s2.x += a2.x; s2.y += a2.y;
-            s3.x += a3.x; s3.y += a3.y;
+    for (int h = 0; h < 2; ++h) {
+        const int k0 = 9 * h;
+#pragma unroll
+        for (int kk = 0; kk < 9; ++kk) {
+            const int k = k0 + kk;
+            if (k < 17 && k <= p) red[w][kk][lane] = make_float2(ar[k], ai[k]);
         }
-        stc(ahat, (size_t)j * PM1 + lane,
-            cf{(s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y)});
+        wave_sync();
+        if (lane >= k0 && lane < k0 + 9 && lane <= p) {
+            const int row = lane - k0;
+            float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+#pragma unroll 4
+            for (int l = 0; l < kWave; l += 4) {
+                const float2 a0 = red[w][row][l], a1 = red[w][row][l + 1];
+                const float2 a2 = red[w][row][l + 2], a3 = red[w][row][l + 3];
+                s0.x += a0.x; s0.y += a0.y;
+                s1.x += a1.x; s1.y += a1.y;
+                s2.x += a2.x; s2.y += a2.y;
+                s3.x += a3.x; s3.y += a3.y;
+            }
+            stc(ahat, (size_t)j * PM1 + lane,
+                cf{(s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y)});
+        }
+        wave_sync();
     }
 }
 
