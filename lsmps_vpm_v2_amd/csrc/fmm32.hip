@@ -595,7 +595,9 @@ __global__ void __launch_bounds__(256)
 // One wavefront per target cell; lanes stride over the source particles and
 // keep the running sums of q w^m for every order (<= 16) in registers; the
 // -1/m factor is applied once per order after the lanes' sums are reduced
-// through LDS (lane l writes column l, lane m sums row m; stride 65).
+// through LDS (lane l writes column l, lane m sums row m; stride 65), in two
+// halves of 8 orders so that 8 wavefronts per SIMD fit (4 with the whole
+// table).
 constexpr int kP2lWaves = 4;
 
 __global__ void __launch_bounds__(kWave *kP2lWaves)
@@ -603,7 +605,7 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
              const double *c_cx, const double *c_cy, const uint32_t *c_begin,
              const uint32_t *c_end, const float2 *oxy, const float *qf, const uint8_t *as,
              const LevelTab *lt, int kst, float *b4) {
-    __shared__ float2 red[kP2lWaves][16][kWave + 1];
+    __shared__ float2 red[kP2lWaves][8][kWave + 1];
     const int PM1 = kst / 2;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int j = j0 + blockIdx.x * kP2lWaves + w;
@@ -636,24 +638,31 @@ __global__ void __launch_bounds__(kWave *kP2lWaves)
         }
     }
 #pragma unroll
-    for (int m = 1; m <= 16; ++m)
-        if (m <= p) red[w][m - 1][lane] = make_float2(ar[m - 1], ai[m - 1]);
-    wave_sync();
-    const int m = lane + 1;
-    if (m <= p) {
-        float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
-#pragma unroll 4
-        for (int l = 0; l < kWave; l += 4) {
-            const float2 a0 = red[w][m - 1][l], a1 = red[w][m - 1][l + 1];
-            const float2 a2 = red[w][m - 1][l + 2], a3 = red[w][m - 1][l + 3];
-            s0.x += a0.x; s0.y += a0.y;
-            s1.x += a1.x; s1.y += a1.y;
-            s2.x += a2.x; s2.y += a2.y;
-            s3.x += a3.x; s3.y += a3.y;
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int mm = 1; mm <= 8; ++mm) {
+            const int mo = 8 * h + mm;
+            if (mo <= p) red[w][mm - 1][lane] = make_float2(ar[mo - 1], ai[mo - 1]);
         }
-        const float c = -1.f / (float)m;
-        const cf sum = {((s0.x + s1.x) + (s2.x + s3.x)) * c, ((s0.y + s1.y) + (s2.y + s3.y)) * c};
-        stc(b4, (size_t)j * PM1 + m, sum);
+        wave_sync();
+        const int m = lane + 1;
+        if (m > 8 * h && m <= 8 * h + 8 && m <= p) {
+            const int row = m - 1 - 8 * h;
+            float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+#pragma unroll 4
+            for (int l = 0; l < kWave; l += 4) {
+                const float2 a0 = red[w][row][l], a1 = red[w][row][l + 1];
+                const float2 a2 = red[w][row][l + 2], a3 = red[w][row][l + 3];
+                s0.x += a0.x; s0.y += a0.y;
+                s1.x += a1.x; s1.y += a1.y;
+                s2.x += a2.x; s2.y += a2.y;
+                s3.x += a3.x; s3.y += a3.y;
+            }
+            const float c = -1.f / (float)m;
+            const cf sum = {((s0.x + s1.x) + (s2.x + s3.x)) * c, ((s0.y + s1.y) + (s2.y + s3.y)) * c};
+            stc(b4, (size_t)j * PM1 + m, sum);
+        }
+        wave_sync();
     }
 }
 
