@@ -281,8 +281,11 @@ __global__ void __launch_bounds__(kWave *kM2lwWaves)
              const double *c_cy, const uint32_t *c_begin, const uint32_t *c_end,
              const double *xs, const double *ys, const double *qs, const uint8_t *as,
              const double2 *ak, double2 *bk) {
-    __shared__ double2 sak[kM2lwWaves][kM2lwSrc * kM2lwP1];
-    __shared__ double2 sw[kM2lwWaves][kM2lwSrc], slg[kM2lwWaves][kM2lwSrc];
+    // sources staged per wavefront: lists 2 and 4, or list 4 alone (a
+    // smaller LDS footprint, so more wavefronts reside)
+    constexpr int NS = L4ONLY ? 8 : kM2lwSrc;
+    __shared__ double2 sak[kM2lwWaves][NS * kM2lwP1];
+    __shared__ double2 sw[kM2lwWaves][NS], slg[kM2lwWaves][NS];
     __shared__ double2 red[kM2lwWaves][kWave];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int j = j0 + blockIdx.x * kM2lwWaves + w;
