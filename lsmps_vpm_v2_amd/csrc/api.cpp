@@ -662,6 +662,7 @@ public:
         a_.touched.reserve(96);
         tab_p_ = h->tab_p;
         tab_binom_ = h->tab_binom;
+        tab_r64_p_ = h->ctx.tab_r64_p;
         mx_pmax_ = h->mx_pmax;
         std::memcpy(mx_rptr_, h->mx_rptr_host, sizeof(mx_rptr_));
         fix_valid_ = h->fix_valid;
@@ -683,6 +684,7 @@ public:
         }
         h_->tab_p = tab_p_;
         h_->tab_binom = tab_binom_;
+        h_->ctx.tab_r64_p = tab_r64_p_;  // the order of the table in the handle's buffer
         h_->mx_pmax = mx_pmax_;
         std::memcpy(h_->mx_rptr_host, mx_rptr_, sizeof(mx_rptr_));
         h_->fix_valid = fix_valid_;
@@ -699,7 +701,7 @@ public:
 private:
     vpmfmm_handle *h_;
     Arena a_;
-    int tab_p_, tab_binom_, mx_pmax_;
+    int tab_p_, tab_binom_, tab_r64_p_, mx_pmax_;
     const float *mx_rptr_[17];
     bool fix_valid_;
     uint8_t fix_orders_[kMaxDepth + 2];
