@@ -452,7 +452,13 @@ __global__ void __launch_bounds__(kWave *kM2l64Waves) __attribute__((amdgpu_wave
     __syncthreads();
     const int n = lane % 16, kq = lane / 16;
     const double *a1 = reinterpret_cast<const double *>(ak);
+    // orders 9 and 10: rows 16.. (at most 4) on v_mfma_f64_4x4x4f64, whose
+    // A operand holds row 16 + l % 4, column l / 16 of a 4-step, whose B
+    // operand is laid out as the 16x16x4 one (target l % 16, step row l / 16)
+    // and whose result is row 16 + l / 16 of target l % 16
+    constexpr bool LEFT = 2 * P > 16 && 2 * P <= 20;
     v4d acc[MT];
+    double acc4 = 0.0;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
     int o = 0;
@@ -471,21 +477,36 @@ __global__ void __launch_bounds__(kWave *kM2l64Waves) __attribute__((amdgpu_wave
         }
         const double *Ro = R + (size_t)oc * KS * MT * kWave + lane;
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
+        for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
+            for (int mt = 0; mt < (LEFT ? 1 : MT); ++mt)
                 acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ro[(ks * MT + mt) * kWave], b[ks],
                                                                acc[mt], 0, 0, 0);
+            if (LEFT)
+                acc4 = __builtin_amdgcn_mfma_f64_4x4x4f64(Ro[(ks * MT + 1) * kWave], b[ks], acc4,
+                                                          0, 0, 0);
+        }
     }
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < (LEFT ? 1 : MT); ++mt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) red[w][mt * 4 + v][lane] = acc[mt][v];
+    if (LEFT) red[w][4][lane] = acc4;
     __syncthreads();
     if (w != 0) return;
     double *bo = reinterpret_cast<double *>(bk);
+    if (LEFT) {
+        double sum = 0.0;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
+        for (int u = 0; u < kM2l64Waves; ++u) sum += red[u][4][lane];
+        const int row = 16 + kq;
+        if (n < cnt && row < 2 * P) {
+            const int p = row / 2 + 1;
+            bo[((size_t)(j0 + n) * P1 + p) * 2 + (row & 1)] = sum * s_sc[p];
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < (LEFT ? 1 : MT); ++mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             double sum = 0.0;
@@ -1197,6 +1218,7 @@ void direct64(int n, const double *x, const double *y, const double *q, const ui
 // R[((o KS + ks) MT + mt) 64 + l] = M[16 mt + l % 16][4 ks + l / 16].
 void fmm64_mfma_table(int P, const double *tab, std::vector<double> &R) {
     const int P1 = P + 1, K = 2 * P1, KS = (K + 3) / 4, MT = (2 * P + 15) / 16;
+    const bool left = 2 * P > 16 && 2 * P <= 20;  // rows 16.. for the 4x4x4 instruction
     R.assign((size_t)40 * KS * MT * kWave, 0.0);
     std::vector<double> pwr(2 * P + 2), pwi(2 * P + 2);
     int o = 0;
@@ -1221,8 +1243,16 @@ void fmm64_mfma_table(int P, const double *tab, std::vector<double> &R) {
                         for (int bq = 0; bq < 2; ++bq) {
                             const int row = r0 + a, col = c0 + bq;
                             const int mt = row / 16, ks = col / 4;
-                            const int lane = (row % 16) + 16 * (col % 4);
-                            R[(((size_t)o * KS + ks) * MT + mt) * kWave + lane] = val[a][bq];
+                            const size_t base = (((size_t)o * KS + ks) * MT + mt) * kWave;
+                            if (left && row >= 16) {
+                                // v_mfma_f64_4x4x4f64 A operand: row 16 + l % 4,
+                                // step column l / 16, the same for the four blocks
+                                for (int q = 0; q < 4; ++q)
+                                    R[base + (row - 16) + 4 * q + 16 * (col % 4)] = val[a][bq];
+                            } else {
+                                const int lane = (row % 16) + 16 * (col % 4);
+                                R[base + lane] = val[a][bq];
+                            }
                         }
                     }
                 }
