@@ -738,8 +738,12 @@ __device__ __forceinline__ bool eval64_leaf(int t, int j, int lane, double4 *sta
     return dup;
 }
 
+// The singular kernel is held to 64 registers per lane so that 8 wavefronts
+// per SIMD reside (7 otherwise; 27 us of 188 at 780k); the regularized ones
+// would spill under that limit and keep the compiler's choice.
 template <int ICUT, bool T1 = false>
 __global__ void __launch_bounds__(kWave *kEvalWaves)
+    __attribute__((amdgpu_waves_per_eu(ICUT == 0 ? 8 : 1, 8)))
     k_eval64(const uint32_t *leaf_idx, int nl, int P, const uint32_t *c_begin, const uint32_t *c_end,
              const double *c_cx, const double *c_cy, const uint32_t *off1, const int *l1,
              const uint32_t *off3, const int *l3, const double *xs, const double *ys,
