@@ -188,6 +188,63 @@ __global__ void __launch_bounds__(kWave *kTrWaves)
     st(ak, (size_t)j * P1 + p, acc);
 }
 
+// M2M for the parents of levels Lhi .. 1 (each holding at most kTopCells64
+// cells, orders P <= 15) in one workgroup: k_m2m64w's four-cells-per-wavefront
+// body, a level at a time with a barrier between levels, in place of a launch
+// per level.
+constexpr int kTopWaves64 = 8;
+constexpr int kTopCells64 = 64;
+
+__global__ void __launch_bounds__(kWave *kTopWaves64)
+    k_m2m64_top(int Lhi, const DevHdr *hdr, int P, const double *tab, const uint8_t *c_flags,
+                const double *c_cx, const double *c_cy, const int *c_child, double2 *ak) {
+    constexpr int G = 4, LN = kWave / G, S = kTrP1 - 1;
+    __shared__ double s_tab[kTrP1 * kTrP1];
+    __shared__ double2 sa[kTopWaves64][G][4 * S];
+    __shared__ int s_off[kMaxDepth + 2];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int g = lane / LN, r = lane % LN;
+    const int P1 = P + 1;
+    for (int t = threadIdx.x; t < P1 * P1; t += blockDim.x) s_tab[t] = tab[t];
+    if (threadIdx.x < kMaxDepth + 2) s_off[threadIdx.x] = (int)hdr->lev_off[threadIdx.x];
+    __syncthreads();
+    for (int L = Lhi; L >= 1; --L) {
+        const int j0 = s_off[L], nj = s_off[L + 1] - j0;
+        for (int u0 = 0; u0 < nj; u0 += kTopWaves64 * G) {
+            const int u = u0 + w * G + g, j = j0 + u;
+            const bool live = u < nj && !(c_flags[j] & kLeaf);
+            int4 c4 = make_int4(-1, -1, -1, -1);
+            if (live) {
+                c4 = reinterpret_cast<const int4 *>(c_child)[j];
+                for (int idx = r; idx < 4 * P1; idx += LN) {
+                    const int q = idx / P1, k = idx - q * P1;
+                    const int ch = q == 0 ? c4.x : q == 1 ? c4.y : q == 2 ? c4.z : c4.w;
+                    sa[w][g][q * S + k] = ch >= 0 ? ak[(size_t)ch * P1 + k] : make_double2(0.0, 0.0);
+                }
+            }
+            wave_sync();
+            if (live && r <= P) {
+                const int p = r;
+                const double cx = c_cx[j], cy = c_cy[j];
+                cd acc = {0.0, 0.0};
+                for (int q = 0; q < 4; ++q) {
+                    const int ch = q == 0 ? c4.x : q == 1 ? c4.y : q == 2 ? c4.z : c4.w;
+                    if (ch < 0) continue;
+                    const cd d = {c_cx[ch] - cx, c_cy[ch] - cy};
+                    const double2 *a = &sa[w][g][q * S];
+                    cd sacc = ld(a, 0);
+                    for (int k = 1; k <= p; ++k)
+                        sacc = cadd(cmul(sacc, d), cscale(ld(a, k), s_tab[p * P1 + k]));
+                    acc = cadd(acc, sacc);
+                }
+                st(ak, (size_t)j * P1 + p, acc);
+            }
+            wave_sync();
+        }
+        __syncthreads();  // the next level up reads these multipoles
+    }
+}
+
 // Principal complex logarithm, as std::log: log|z| + i arg z.
 __device__ __forceinline__ cd clog(cd z) { return {log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
 
@@ -574,6 +631,50 @@ __global__ void __launch_bounds__(kWave *kTrWaves)
     for (int k = P - 1; k >= p; --k) s = cadd(cmul(s, d), cscale(ld(b, k), row[k]));
     const size_t o = (size_t)j * P1 + p;
     st(bk, o, cadd(ld(bk, o), s));
+}
+
+// L2L into the cells of levels 3 .. Lhi (each holding at most kTopCells64
+// cells, orders P <= 15) in one workgroup: k_l2l64w's body, a level at a
+// time with a barrier between levels, in place of a launch per level.
+template <bool POT>
+__global__ void __launch_bounds__(kWave *kTopWaves64)
+    k_l2l64_top(int Lhi, const DevHdr *hdr, int P, const double *tab, const int *c_parent,
+                const double *c_cx, const double *c_cy, double2 *bk) {
+    constexpr int G = 4, LN = kWave / G;
+    __shared__ double s_tab[kTrP1 * kTrP1];
+    __shared__ double2 sb[kTopWaves64][G][kTrP1];
+    __shared__ int s_off[kMaxDepth + 2];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int g = lane / LN, r = lane % LN;
+    const int P1 = P + 1, NP = POT ? P1 : P;
+    for (int t = threadIdx.x; t < P1 * P1; t += blockDim.x) s_tab[t] = tab[t];
+    if (threadIdx.x < kMaxDepth + 2) s_off[threadIdx.x] = (int)hdr->lev_off[threadIdx.x];
+    __syncthreads();
+    for (int L = 3; L <= Lhi; ++L) {
+        const int j0 = s_off[L], nj = s_off[L + 1] - j0;
+        for (int u0 = 0; u0 < nj; u0 += kTopWaves64 * G) {
+            const int u = u0 + w * G + g, j = j0 + u;
+            const bool live = u < nj;
+            int par = 0;
+            if (live) {
+                par = c_parent[j];
+                for (int k = r; k < P1; k += LN) sb[w][g][k] = bk[(size_t)par * P1 + k];
+            }
+            wave_sync();
+            if (live && r < NP) {
+                const int p = (POT ? 0 : 1) + r;
+                const cd d = {c_cx[j] - c_cx[par], c_cy[j] - c_cy[par]};
+                const double *row = s_tab + p * P1;
+                const double2 *b = sb[w][g];
+                cd sv = cscale(ld(b, P), row[P]);
+                for (int k = P - 1; k >= p; --k) sv = cadd(cmul(sv, d), cscale(ld(b, k), row[k]));
+                const size_t o = (size_t)j * P1 + p;
+                st(bk, o, cadd(ld(bk, o), sv));
+            }
+            wave_sync();
+        }
+        __syncthreads();  // the next level down reads these coefficients
+    }
 }
 
 // Evaluation: one wavefront per leaf.  Lanes are split into G groups of
@@ -985,7 +1086,13 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
 
     k_p2m64<<<div_up(nl, kP2m64Waves * kP2mLeaves), kWave * kP2m64Waves, 0, s>>>(
         leaf_idx, nl, P, c_flags, c_begin, c_end, c_cx, c_cy, xs, ys, qs, as, ak);
-    for (int L = ti.max_level - 1; L >= 1; --L) {
+    // the top levels (parents 1 .. Lt, each of at most kTopCells64 cells) go to
+    // one workgroup after the per-level launches below them
+    auto lev_cnt = [&](int L) { return (int)(ti.lev_off[L + 1] - ti.lev_off[L]); };
+    int Lt = 0;
+    if (P1 <= kTrP1 - 1)
+        while (Lt + 1 <= ti.max_level - 1 && lev_cnt(Lt + 1) <= kTopCells64) ++Lt;
+    for (int L = ti.max_level - 1; L > Lt; --L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
         if (P1 <= kTrP1 - 1)
@@ -998,6 +1105,9 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
             k_m2m64<<<div_up((long long)(j1 - j0) * P1, 256), 256, 0, s>>>(
                 j0, j1, P, t_m2m, c_flags, c_cx, c_cy, c_child, ak);
     }
+    if (Lt >= 1)
+        k_m2m64_top<<<1, kWave * kTopWaves64, 0, s>>>(Lt, c.hdr_d, P, t_m2m, c_flags, c_cx, c_cy,
+                                                     c_child, ak);
     mk.mark(3, s);
     // M2L: every cell of level >= 2
     const int j2 = (int)ti.lev_off[2];
@@ -1039,8 +1149,16 @@ void fmm64_run(Ctx &c, int P, int list4_mode, int stage_mask, double *out_a, dou
             (const int *)c.l4.p, (const uint8_t *)c.n4.p, c_cx, c_cy, c_begin, c_end, xs, ys, qs,
             as, ak, bk);
     mk.mark(4, s);
-    // L2L: children at levels 3 .. maxL (parents 2 .. maxL-1, fmm2D.cpp:587)
-    for (int L = 3; L <= ti.max_level; ++L) {
+    // L2L: children at levels 3 .. maxL (parents 2 .. maxL-1, fmm2D.cpp:587);
+    // the top children levels (3 .. Lc, each of at most kTopCells64 cells) in
+    // one workgroup first
+    int Lc = 2;
+    if (P1 <= kTrP1 - 1)
+        while (Lc + 1 <= ti.max_level && lev_cnt(Lc + 1) <= kTopCells64) ++Lc;
+    if (Lc >= 3)
+        (potential ? k_l2l64_top<true> : k_l2l64_top<false>)<<<1, kWave * kTopWaves64, 0, s>>>(
+            Lc, c.hdr_d, P, t_l2l, c_parent, c_cx, c_cy, bk);
+    for (int L = std::max(3, Lc + 1); L <= ti.max_level; ++L) {
         const int j0 = (int)ti.lev_off[L], j1 = (int)ti.lev_off[L + 1];
         if (j1 <= j0) continue;
         if (P1 <= kTrP1 - 1)
