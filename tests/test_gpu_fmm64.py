@@ -60,6 +60,27 @@ def test_every_order_matches_oracle(gpu, oracle, p):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("p", [17, 24, 40])
+def test_high_orders_match_oracle(gpu, oracle, p):
+    """Orders 17 .. 40 run the thread-per-(cell, order) M2M, M2L and L2L
+    kernels (no matrix-core M2L, no LDS staging): field and potential against
+    the oracle up to the largest accepted order."""
+    print(f"case clusters n=3000 seed={70 + p} p={p}", flush=True)
+    f = _case("clusters", 3000, 70 + p)
+    o = oracle.opts(p=p)
+    ex0, ey0 = oracle.fmm_field(f["x"], f["y"], f["q"], f["active"], o)
+    s = FmmSolver(p=p)
+    ex, ey = s.field(f["x"], f["y"], f["q"], f["active"])
+    err = rel_l2(ex, ey, ex0, ey0)
+    print(f"field rel-L2 vs oracle {err:.3e}", flush=True)
+    assert err <= TOL
+    phi0 = np.asarray(oracle.fmm_potential(f["x"], f["y"], f["q"], f["active"], o))
+    phi = np.asarray(s.potential(f["x"], f["y"], f["q"], f["active"]))
+    perr = float(np.linalg.norm(phi - phi0) / np.linalg.norm(phi0))
+    print(f"potential rel-L2 vs oracle {perr:.3e}", flush=True)
+    assert perr <= TOL
+
+
 @pytest.mark.parametrize("binom_mode,list4_mode", [(0, 1), (0, 2), (1, 2)])
 def test_field_modes_match_oracle(gpu, oracle, binom_mode, list4_mode):
     f = _case("clusters", 5000, 11)
